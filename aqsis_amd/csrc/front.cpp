@@ -1,0 +1,808 @@
+// Host front end: frame shape, sample plan, split/dice schedule.
+// Compile with -O2 -ffp-contract=off (x86-64 SSE, no FMA contraction).
+#include "front.h"
+
+#include <algorithm>
+#include <atomic>
+#include <random>
+#include <thread>
+
+namespace rh {
+
+// ---------------------------------------------------------------------------
+// Frame shape
+// ---------------------------------------------------------------------------
+static float filter_value(int f, float x, float y, float xw, float yw);
+
+bool make_shape(const rh_options& o, Shape& s, std::string& err) {
+    if (o.xres <= 0 || o.yres <= 0 || o.xres > 32768 || o.yres > 32768) { err = "resolution out of range"; return false; }
+    if (o.xsamples < 1 || o.ysamples < 1 || o.xsamples * o.ysamples > 64) {
+        err = "PixelSamples must be >= 1 each with at most 64 samples per pixel";
+        return false;
+    }
+    if (!(o.filter_xwidth > 0) || !(o.filter_ywidth > 0) || o.filter_xwidth > 8 || o.filter_ywidth > 8) {
+        err = "filter widths must be in (0, 8]";
+        return false;
+    }
+    if (o.filter < RH_FILTER_BOX || o.filter > RH_FILTER_MITCHELL) { err = "unknown pixel filter"; return false; }
+    if (o.bucket_x < 4 || o.bucket_y < 4 || o.bucket_x > 64 || o.bucket_y > 64) { err = "bucket size must be in [4, 64]"; return false; }
+    if (!(o.fov > 0 && o.fov < 180)) { err = "fov must be in (0, 180)"; return false; }
+    if (o.num_displays < 0 || o.num_displays > 16) { err = "num_displays out of range"; return false; }
+    if (!(o.pixel_aspect > 0)) { err = "pixel aspect must be positive"; return false; }
+    s.o = o;
+    // CqRenderer::initialiseCropWindow (renderer.cpp:1619-1626)
+    s.cropX0 = (int)clampv<long>(ceil_l((float)o.xres * o.crop[0]), 0, o.xres);
+    s.cropX1 = (int)clampv<long>(ceil_l((float)o.xres * o.crop[1]), 0, o.xres);
+    s.cropY0 = (int)clampv<long>(ceil_l((float)o.yres * o.crop[2]), 0, o.yres);
+    s.cropY1 = (int)clampv<long>(ceil_l((float)o.yres * o.crop[3]), 0, o.yres);
+    if (s.cropX1 <= s.cropX0 || s.cropY1 <= s.cropY0) { err = "empty crop window"; return false; }
+    s.shiftX = (int)floor_l(o.filter_xwidth / 2.0f);
+    s.shiftY = (int)floor_l(o.filter_ywidth / 2.0f);
+    s.xfwo2 = std::ceil(o.filter_xwidth) * 0.5f;
+    s.yfwo2 = std::ceil(o.filter_ywidth) * 0.5f;
+    if (s.shiftX > 3 || s.shiftY > 3) { err = "filter footprint too large"; return false; }
+    s.nx = o.xsamples; s.ny = o.ysamples; s.ns = s.nx * s.ny;
+    // CqImageBuffer::SetImage (imagebuffer.cpp:174-228)
+    s.nbx = (o.xres - 1) / o.bucket_x + 1;
+    s.nby = (o.yres - 1) / o.bucket_y + 1;
+    s.bx0 = s.cropX0 / o.bucket_x; s.by0 = s.cropY0 / o.bucket_y;
+    s.bx1 = (s.cropX1 - 1) / o.bucket_x + 1; s.by1 = (s.cropY1 - 1) / o.bucket_y + 1;
+    s.nbr = s.bx1 - s.bx0; s.nbrY = s.by1 - s.by0;
+    s.extX0 = s.cropX0 - s.shiftX; s.extY0 = s.cropY0 - s.shiftY;
+    s.extW = s.width() + 2 * s.shiftX; s.extH = s.height() + 2 * s.shiftY;
+
+    // Camera (ri.cpp:586-617; options.cpp:40-160), perspective projection.
+    float par = o.pixel_aspect;
+    float far_ = ((float)o.xres * par) / (float)o.yres;
+    float l, r, t, b;
+    if (far_ >= 1.0) { l = -far_; r = far_; t = 1; b = -1; }
+    else { l = -1; r = 1; t = (float)(1.0 / far_); b = (float)(-1.0 / far_); }
+    float n = o.clip_near, f = o.clip_far;
+    float sfov = (float)std::tan((double)(deg2rad(o.fov) / 2.0f));
+    float a = f / (f - n);
+    m44 c2i = m_identity();
+    c2i.id = false;
+    c2i.e[0][0] = 1.0f / sfov; c2i.e[1][1] = 1.0f / sfov; c2i.e[2][2] = a;
+    c2i.e[3][2] = -n * a; c2i.e[2][3] = 1.0f; c2i.e[3][3] = 0.0f;
+    m44 i2s = m_identity();
+    i2s.id = false;
+    i2s.e[0][0] = 2.0f / (r - l); i2s.e[1][1] = 2.0f / (t - b);
+    i2s.e[3][0] = (-(r + l)) / (r - l); i2s.e[3][1] = (-(t + b)) / (t - b);
+    m44 c2s = m_mul(m_mul(i2s, m_identity()), c2i);
+    float frameX = (far_ >= 1.0) ? (float)o.xres : ((float)o.yres * far_) / par;
+    float frameY = (far_ < 1.0) ? (float)o.yres : ((float)o.xres * par) / far_;
+    m44 s2ndc = m_mul(m_mul(m_diag(0.5f, 0.5f, 1.0f), m_trans(1.0f, 1.0f, 0.0f)), m_diag(1.0f, -1.0f, 1.0f));
+    m44 raster = m_mul(m_mul(m_mul(m_diag(frameX, frameY, 1.0f), s2ndc), c2s), m_identity());
+    s.camToRaster = m_mul(raster, m_identity());
+    s.diceM = s.camToRaster;
+    s.diceM.e[0][2] = s.diceM.e[1][2] = s.diceM.e[2][2] = s.diceM.e[3][2] = 0;
+
+    // Filter weights at subpixel centres (bucketprocessor.cpp:811-856)
+    const int xm = s.shiftX, ym = s.shiftY;
+    s.filterTab.assign((size_t)(2 * xm + 1) * (2 * ym + 1) * s.ns, 0.0f);
+    float cw = std::ceil(o.filter_xwidth), ch = std::ceil(o.filter_ywidth);
+    for (int py = -ym; py <= ym; ++py)
+        for (int px = -xm; px <= xm; ++px) {
+            size_t k = ((size_t)(py + ym) * (2 * xm + 1) + (px + xm)) * s.ns;
+            for (int sy = 0; sy < s.ny; ++sy)
+                for (int sx = 0; sx < s.nx; ++sx, ++k) {
+                    float fx = (sx + 0.5f) / s.nx + px - 0.5f;
+                    float fy = (sy + 0.5f) / s.ny + py - 0.5f;
+                    float w = 0;
+                    if (fx >= -s.xfwo2 && fy >= -s.yfwo2 && fx <= s.xfwo2 && fy <= s.yfwo2)
+                        w = filter_value(o.filter, fx, fy, cw, ch);
+                    s.filterTab[k] = w;
+                }
+        }
+
+    // Buckets and the sample region each one owns (bucketprocessor.cpp:95-137):
+    // display rect +- shift, clipped to the crop window +- shift, minus the
+    // 2*shift bands handed over from the left and upper neighbours.
+    s.buckets.clear();
+    for (int by = s.by0; by < s.by1; ++by)
+        for (int bx = s.bx0; bx < s.bx1; ++bx) {
+            Shape::BucketInfo bi;
+            bi.x = bx * o.bucket_x; bi.y = by * o.bucket_y;
+            bi.w = std::min(o.xres - bi.x, o.bucket_x);
+            bi.h = std::min(o.yres - bi.y, o.bucket_y);
+            int sx0 = bi.x - s.shiftX, sy0 = bi.y - s.shiftY;
+            int sx1 = bi.x + bi.w + s.shiftX, sy1 = bi.y + bi.h + s.shiftY;
+            sx0 = std::max(sx0, s.cropX0 - s.shiftX);
+            sy0 = std::max(sy0, s.cropY0 - s.shiftY);
+            sx1 = std::min(sx1, s.cropX1 + s.shiftX);
+            sy1 = std::min(sy1, s.cropY1 + s.shiftY);
+            if (bx > s.bx0) sx0 += 2 * s.shiftX;
+            if (by > s.by0) sy0 += 2 * s.shiftY;
+            bi.sx0 = sx0; bi.sy0 = sy0; bi.sx1 = sx1; bi.sy1 = sy1;
+            s.buckets.push_back(bi);
+        }
+    return true;
+}
+
+// RiXxxFilter functions (libs/core/filters.cpp:72-342)
+static float filter_value(int f, float x, float y, float xw, float yw) {
+    const double PI = 3.14159265358979323846;
+    switch (f) {
+        case RH_FILTER_BOX: {
+            double a = (std::fabs(x) <= xw / 2.0) ? 1.0 : 0.0, b = (std::fabs(y) <= yw / 2.0) ? 1.0 : 0.0;
+            return (float)(a < b ? a : b);
+        }
+        case RH_FILTER_TRIANGLE: {
+            float hx = (float)(xw / 2.0), hy = (float)(yw / 2.0), ax = std::fabs(x), ay = std::fabs(y);
+            double a = ax <= hx ? (double)((hx - ax) / hx) : 0.0, b = ay <= hy ? (double)((hy - ay) / hy) : 0.0;
+            return (float)(a < b ? a : b);
+        }
+        case RH_FILTER_CATMULLROM: {
+            float r2 = (x * x + y * y);
+            float r = std::sqrt(r2);
+            return (float)((r >= 2.0) ? 0.0 : (r < 1.0) ? (3.0 * r * r2 - 5.0 * r2 + 2.0) : (-r * r2 + 5.0 * r2 - 8.0 * r + 4.0));
+        }
+        case RH_FILTER_GAUSSIAN: {
+            x /= xw; y /= yw;
+            return (float)std::exp(-8.0 * (double)(x * x + y * y));
+        }
+        case RH_FILTER_SINC: {
+            if (x != 0.0) { x = (float)(x * PI); x = (float)(std::cos(0.5 * x / xw) * std::sin((double)x) / x); }
+            else x = 1.0f;
+            if (y != 0.0) { y = (float)(y * PI); y = (float)(std::cos(0.5 * y / yw) * std::sin((double)y) / y); }
+            else y = 1.0f;
+            return x * y;
+        }
+        case RH_FILTER_DISK: {
+            double xx = x * x, yy = y * y;
+            xw *= 0.5; yw *= 0.5;
+            double d = xx / (xw * xw) + yy / (yw * yw);
+            return d < 1.0 ? 1.0f : 0.0f;
+        }
+        case RH_FILTER_BESSEL: {
+            double xx = x * x, yy = y * y;
+            xw *= 0.5; yw *= 0.5;
+            double w = xx / (xw * xw) + yy / (yw * yw);
+            if (w < 1.0) {
+                double d = std::sqrt(xx + yy);
+                if (d != 0.0) { w = std::cos(0.5 * PI * std::sqrt(w)); return (float)(w * 2 * j1(PI * d) / d); }
+                return (float)PI;
+            }
+            return 0.0f;
+        }
+        case RH_FILTER_MITCHELL: {
+            const float B = 1 / 3.0f, C = 1 / 3.0f;
+            float ix = 1.0f / xw, iy = 1.0f / yw;
+            auto ev = [&](float t) -> float {
+                t = std::fabs(2.f * t);
+                if (t > 1.f)
+                    return ((-B - 6 * C) * t * t * t + (6 * B + 30 * C) * t * t + (-12 * B - 48 * C) * t + (8 * B + 24 * C)) * (1.f / 6.f);
+                return ((12 - 9 * B - 6 * C) * t * t * t + (-18 + 12 * B + 6 * C) * t * t + (6 - 2 * B)) * (1.f / 6.f);
+            };
+            return ev(x * ix) * ev(y * iy);
+        }
+    }
+    return 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Sample plan
+// ---------------------------------------------------------------------------
+// aqsis's CqRandom is one MT19937 stream (libs/math/random.cpp); std::mt19937
+// has the same seeding and output.  RandomFloat = u32 * (1/4294967424) in
+// double, narrowed to float; RandomInt(n) = lfloor((double)(n * RandomFloat())).
+namespace {
+struct Stream {
+    std::mt19937 g;
+    float f() { return (float)((double)g() * (1.0 / 4294967424.0)); }
+    float f(float r) { return r * f(); }
+    int i(unsigned n) { double v = f((float)n); return (int)floor_l(v); }
+};
+}  // namespace
+
+void make_plan(const Shape& s, Plan& p) {
+    const int ns = s.ns, nx = s.nx, ny = s.ny;
+    Stream st;
+    st.g.seed(545);  // 'a'+'q'+'s'+'i'+'s' at WorldBegin (ri.cpp:660)
+    if (s.o.jitter) {
+        // CqMultiJitteredSampler (multijitter.h:90-101, multijitter.cpp:83-202)
+        p.cache = 250;
+        p.pos2d.assign((size_t)250 * ns * 2, 0.0f);
+        p.pos1d.assign((size_t)250 * ns, 0.0f);
+        std::vector<int> shuffled((size_t)250 * ns);
+        std::vector<int> cell(2 * ns);
+        for (int pat = 0; pat < 250; ++pat) {
+            const size_t off = (size_t)pat * ns;
+            if (nx == 1 && ny == 1) {
+                float ry = st.f(), rx = st.f();  // CqVector2D(f(), f()): g++ evaluates y first
+                p.pos2d[2 * off] = rx; p.pos2d[2 * off + 1] = ry;
+                p.pos1d[off] = st.f();
+            } else {
+                for (int iy = 0; iy < ny; ++iy)
+                    for (int ix = 0; ix < nx; ++ix) {
+                        cell[2 * (iy * nx + ix)] = iy;
+                        cell[2 * (iy * nx + ix) + 1] = ix;
+                    }
+                for (int iy = 0; iy < ny; ++iy)
+                    for (int ix = nx; ix > 1;) {
+                        int k = st.i(ix);
+                        --ix;
+                        std::swap(cell[2 * (iy * nx + ix) + 1], cell[2 * (iy * nx + k) + 1]);
+                    }
+                for (int ix = 0; ix < nx; ++ix)
+                    for (int iy = ny; iy > 1;) {
+                        int k = st.i(iy);
+                        --iy;
+                        std::swap(cell[2 * (iy * nx + ix)], cell[2 * (k * nx + ix)]);
+                    }
+                const float h = 1.0f / ny, w = 1.0f / nx, c = 1.0f / ns;
+                for (int iy = 0, k = 0; iy < ny; ++iy)
+                    for (int ix = 0; ix < nx; ++ix, ++k) {
+                        float ry = st.f(), rx = st.f();
+                        p.pos2d[2 * (off + k)] = (cell[2 * k] + rx) * c + ix * w;
+                        p.pos2d[2 * (off + k) + 1] = (cell[2 * k + 1] + ry) * c + iy * h;
+                    }
+            }
+            float acc = 0, d = 1.0f / ns, r1 = st.f(d);
+            for (int k = 0; k < ns; ++k) { p.pos1d[off + k] = acc + r1; acc += d; }
+            for (int k = 0; k < ns; ++k) shuffled[off + k] = k;
+            for (int j = ns; j > 1;) {
+                int k = st.i(j);
+                --j;
+                std::swap(shuffled[off + j], shuffled[off + k]);
+            }
+        }
+        st.g.seed(19);
+    } else {
+        // CqGridSampler (grid.cpp:37-63): no random draws at all.
+        p.cache = 1;
+        p.pos2d.assign((size_t)ns * 2, 0.0f);
+        p.pos1d.assign(ns, 0.0f);
+        float xs = (float)(1.0 / nx), ys = (float)(1.0 / ny);
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                p.pos2d[2 * (j * nx + i)] = (float)(xs * (i + 0.5));
+                p.pos2d[2 * (j * nx + i) + 1] = (float)(ys * (j + 0.5));
+            }
+        float dt = (float)(1 / ns), t = (float)(dt * 0.5);
+        for (int i = 0; i < ns; ++i) { p.pos1d[i] = t; t += dt; }
+    }
+    // Bucket loop order: per bucket, 5 draws per sample-region pixel
+    // (imagepixel.cpp:338-347), then one draw per display pixel per display
+    // request (ddmanager.cpp:1050).
+    p.pixPattern.assign((size_t)s.extW * s.extH * 4, 0);
+    p.dither.assign((size_t)s.width() * s.height(), 0.0f);
+    for (const auto& b : s.buckets) {
+        if (s.o.jitter) {
+            for (int y = b.sy0; y < b.sy1; ++y)
+                for (int x = b.sx0; x < b.sx1; ++x) {
+                    uint8_t* q = &p.pixPattern[((size_t)(y - s.extY0) * s.extW + (x - s.extX0)) * 4];
+                    int shuf = st.i(250);
+                    int pos = st.i(250);
+                    (void)st.i(250);  // depth-of-field offsets
+                    int tim = st.i(250);
+                    int lod = st.i(250);
+                    q[0] = (uint8_t)pos; q[1] = (uint8_t)tim; q[2] = (uint8_t)lod; q[3] = (uint8_t)shuf;
+                }
+        }
+        bool outside = (b.x + b.w) <= s.cropX0 || (b.y + b.h) <= s.cropY0 || b.x > s.cropX1 || b.y > s.cropY1;
+        if (outside) continue;
+        for (int d = 0; d < s.o.num_displays; ++d)
+            for (int y = b.y; y < b.y + b.h; ++y)
+                for (int x = b.x; x < b.x + b.w; ++x) {
+                    float v = st.f();
+                    if (d == 0 && x >= s.cropX0 && x < s.cropX1 && y >= s.cropY0 && y < s.cropY1)
+                        p.dither[(size_t)(y - s.cropY0) * s.width() + (x - s.cropX0)] = v;
+                }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Split / dice schedule
+// ---------------------------------------------------------------------------
+namespace {
+
+enum { DIR_U = 0, DIR_V = 1 };
+
+struct Node {
+    int type = 0, prim = 0;
+    int splitCount = 0, splitDir = DIR_U;
+    bool diceableFlag = true;   // m_fDiceable
+    bool discard = false;       // m_fDiscard
+    bool culled = false;        // rejected by CullSurface
+    bool hasBound = false;      // cached raster bound present
+    bool leaf = false;          // Diceable() returned true
+    int uDice = 0, vDice = 0;
+    box3 rb;                    // cached raster bound
+    int xminb = 0, yminb = 0, xmaxb = 0, ymaxb = 0;
+    int kid0 = -1, nkids = 0;   // children in the same tree
+    // geometry
+    float radius = 1, phiMin = 0, phiMax = 0, thetaMin = 0, thetaMax = 360;
+    f3 cp[16];
+};
+
+// sinCosGrid (quadrics.cpp:1493-1518)
+void sincos_grid(float t0, float t1, int n, float* s, float* c) {
+    double pc = std::cos((double)t0), ps = std::sin((double)t0);
+    double dt = (double)((t1 - t0) / (n - 1));
+    double cd = std::cos(dt), sd = std::sin(dt);
+    c[0] = (float)pc; s[0] = (float)ps;
+    for (int i = 1; i < n; ++i) {
+        double cc = cd * pc - sd * ps, ss = sd * pc + cd * ps;
+        c[i] = (float)cc; s[i] = (float)ss;
+        pc = cc; ps = ss;
+    }
+}
+
+// IntersectLine (quadrics.cpp:1527-1541)
+bool line_meet(f3 P1, f3 T1, f3 P2, f3 T2, f3& P) {
+    f3 v = crs(crs(T1, sub(P1, T2)), T1);
+    float vw = dot3(v, T2);
+    if ((double)(vw * vw) < 1.0e-07) return false;
+    P = add(P2, scale(dot3(sub(P1, P2), v) / vw, T2));
+    return true;
+}
+
+box3 sphere_camera_bound(const Node& n, const m44& objToCam) {
+    const double PI = 3.14159265358979323846;
+    // Circle(O=(0,0,0), X=(1,0,0), Y=(0,0,1), r, min phi, max phi) (quadrics.cpp:1360-1400)
+    float as = mn(n.phiMin, n.phiMax), ae = mx(n.phiMin, n.phiMax), r = n.radius;
+    while (ae < as) ae = (float)(ae + 2 * PI);
+    float dth = (ae - as) / 4.0f;
+    f3 X = mk(1, 0, 0), Y = mk(0, 0, 1);
+    f3 prof[9];
+    f3 P0 = add(add(mk(0, 0, 0), scale((float)(r * std::cos((double)as)), X)), scale((float)(r * std::sin((double)as)), Y));
+    f3 T0 = add(scale((float)(-std::sin((double)as)), X), scale((float)std::cos((double)as), Y));
+    f3 P1 = mk(0, 0, 0);
+    prof[0] = P0;
+    float ang = as;
+    for (int i = 1; i <= 4; ++i) {
+        ang += dth;
+        f3 P2 = add(add(mk(0, 0, 0), scale((float)(r * std::cos((double)ang)), X)), scale((float)(r * std::sin((double)ang)), Y));
+        prof[2 * i] = P2;
+        f3 T2 = add(scale((float)(-std::sin((double)ang)), X), scale((float)std::cos((double)ang), Y));
+        line_meet(P0, T0, P2, T2, P1);
+        prof[2 * i - 1] = P1;
+        if (i < 4) { P0 = P2; T0 = T2; }
+    }
+    m44 rot = m_rot(deg2rad(n.thetaMin), Y);
+    for (auto& q : prof) q = m_point(rot, q);
+    // RevolveForBound(profile, S=(0,0,0), Tvec=(0,0,1), theta) (quadrics.cpp:1404-1474)
+    float theta = deg2rad(n.thetaMax - n.thetaMin);
+    if (std::fabs(theta) > 2.0 * PI) theta = theta < 0 ? (float)(-(2.0 * PI)) : (float)(2.0 * PI);
+    float dt = theta / 4.0f;
+    float cs[5], sn[5];
+    for (int i = 1; i <= 4; ++i) {
+        float a = dt * (float)i;
+        cs[i] = (float)std::cos((double)a);
+        sn[i] = (float)std::sin((double)a);
+    }
+    box3 b = empty_box();
+    f3 Tv = mk(0, 0, 1), S = mk(0, 0, 0);
+    P1 = mk(0, 0, 0);
+    for (int j = 0; j < 9; ++j) {
+        f3 pj = prof[j];
+        f3 a = sub(pj, S);
+        float den = len2(Tv);
+        float frac = (den == 0.0) ? 0.0f : dot3(Tv, a) / den;
+        f3 O = add(scale(frac, Tv), S);
+        f3 Xa = sub(pj, O);
+        float rr = std::sqrt(len2(Xa));
+        if ((double)rr < 1e-7) { grow(b, O); continue; }
+        Xa = normalised(Xa);
+        f3 Ya = normalised(crs(Tv, Xa));
+        f3 Q0 = pj;
+        grow(b, Q0);
+        f3 U0 = Ya;
+        for (int i = 1; i <= 4; ++i) {
+            f3 Q2 = add(add(O, scale(rr * cs[i], Xa)), scale(rr * sn[i], Ya));
+            grow(b, Q2);
+            f3 U2 = add(scale(-sn[i], Xa), scale(cs[i], Ya));
+            line_meet(Q0, U0, Q2, U2, P1);
+            grow(b, P1);
+            if (i < 4) { Q0 = Q2; U0 = U2; }
+        }
+    }
+    return box_xform(b, objToCam);
+}
+
+box3 patch_camera_bound(const Node& n, int k) {
+    box3 b;
+    b.lo = mk(FLT_MAX, FLT_MAX, FLT_MAX);
+    b.hi = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+    for (int i = 0; i < k; ++i) {
+        f3 v = n.cp[i];
+        if (v.x < b.lo.x) b.lo.x = v.x;
+        if (v.y < b.lo.y) b.lo.y = v.y;
+        if (v.x > b.hi.x) b.hi.x = v.x;
+        if (v.y > b.hi.y) b.hi.y = v.y;
+        if (v.z < b.lo.z) b.lo.z = v.z;
+        if (v.z > b.hi.z) b.hi.z = v.z;
+    }
+    return b;
+}
+
+struct Ctx {
+    const Shape* s;
+    const rh_prim* prims;
+    std::vector<m44> objToCam;   // sphere transforms
+};
+
+// CqQuadric::EstimateGridSize + Diceable (quadrics.cpp:259-346)
+bool sphere_diceable(Node& n, const Ctx& c) {
+    if (!n.diceableFlag) return false;
+    const int E = 8;
+    float su = 0, sv = 0;
+    m44 M = m_mul(c.s->diceM, c.objToCam[n.prim]);
+    float sT[E + 1], cT[E + 1], sP[E + 1], cP[E + 1];
+    sincos_grid(deg2rad(n.thetaMin), deg2rad(n.thetaMax), E + 1, sT, cT);
+    sincos_grid(n.phiMin, n.phiMax, E + 1, sP, cP);
+    f3 prev = mk(0, 0, 0), up[E];
+    for (int i = 0; i < E; ++i) up[i] = mk(0, 0, 0);
+    for (int v = 0; v <= E; ++v)
+        for (int u = 0; u <= E; ++u) {
+            f3 p = m_point(M, scale(n.radius, mk(cT[u] * cP[v], sT[u] * cP[v], sP[v])));
+            if (v >= 1 && u >= 1) {
+                su = mx(su, len2(sub(p, prev)));
+                sv = mx(sv, len2(sub(up[u - 1], prev)));
+            }
+            if (u >= 1) up[u - 1] = prev;
+            prev = p;
+        }
+    su = std::sqrt(su); sv = std::sqrt(sv);
+    float srt = std::sqrt(c.prims[n.prim].shading_rate);
+    n.uDice = (int)ceil_l(E * su / srt);
+    n.vDice = (int)ceil_l(E * sv / srt);
+    unsigned long too = (unsigned long)n.uDice * n.vDice;
+    n.splitDir = n.uDice > n.vDice ? DIR_U : DIR_V;
+    if (too > 10000) return false;
+    if (n.uDice > 16.0f || n.vDice > 16.0f) return false;
+    return true;
+}
+
+// CqSurfacePatchBicubic::Diceable (patch.cpp:300-379)
+bool bicubic_diceable(Node& n, const Ctx& c) {
+    if (!n.diceableFlag) return false;
+    f3 h[16];
+    for (int i = 0; i < 16; ++i) h[i] = m_point(c.s->diceM, n.cp[i]);
+    float ul = 0, vl = 0;
+    for (int u = 0; u < 16; u += 4)
+        for (int k = 0; k < 3; ++k) {
+            float d = len2(sub(h[u + k + 1], h[u + k]));
+            if (d > ul) ul = d;
+        }
+    for (int v = 0; v < 4; ++v)
+        for (int k = 0; k < 3; ++k) {
+            float d = len2(sub(h[v + 4 * k + 4], h[v + 4 * k]));
+            if (d > vl) vl = d;
+        }
+    float sr = c.prims[n.prim].shading_rate;
+    ul = std::sqrt(ul / sr); vl = std::sqrt(vl / sr);
+    n.splitDir = ul > vl ? DIR_U : DIR_V;
+    ul *= 3; vl *= 3;
+    n.uDice = (int)mx<long>(round_l(ul), 1);
+    n.vDice = (int)mx<long>(round_l(vl), 1);
+    if (ul < FLT_EPSILON || vl < FLT_EPSILON) { n.discard = true; return false; }
+    const float g2 = 256.0f;
+    return !(n.uDice > g2 || n.vDice > g2 || (n.uDice * n.vDice) > g2);
+}
+
+// CqSurfacePatchBilinear::Diceable (patch.cpp:671-736)
+bool bilinear_diceable(Node& n, const Ctx& c) {
+    if (!n.diceableFlag) return false;
+    f3 h[4];
+    for (int i = 0; i < 4; ++i) h[i] = m_point(c.s->diceM, n.cp[i]);
+    float a = len2(sub(h[1], h[0])), b = len2(sub(h[3], h[2]));
+    float ul = a > b ? a : b;
+    a = len2(sub(h[2], h[0])); b = len2(sub(h[3], h[1]));
+    float vl = a > b ? a : b;
+    float sr = c.prims[n.prim].shading_rate;
+    ul = std::sqrt(ul / sr); vl = std::sqrt(vl / sr);
+    n.splitDir = ul > vl ? DIR_U : DIR_V;
+    ul = (float)mx<long>(round_l(ul), 1);
+    vl = (float)mx<long>(round_l(vl), 1);
+    n.uDice = (int)ul; n.vDice = (int)vl;
+    if (ul < FLT_EPSILON || vl < FLT_EPSILON) { n.discard = true; return false; }
+    const float g2 = 256.0f;
+    return !(n.uDice > g2 || n.vDice > g2 || (n.uDice * n.vDice) > g2);
+}
+
+inline f3 mid(f3 a, f3 b) { return scale(0.5f, add(a, b)); }
+inline f3 half(f3 a) { return scale(0.5f, a); }
+inline f3 quarter_sum(f3 a, f3 b) { return scale(0.25f, add(a, b)); }
+
+// bicubicPatchNatSubdiv (patch.cpp:87-129) on dehomogenised points
+void split_bicubic(const f3* P, f3* A, f3* B, bool u) {
+    for (int k = 0; k < 4; ++k) {
+        int i0, s;
+        if (u) { i0 = 4 * k; s = 1; } else { i0 = k; s = 4; }
+        const f3 p0 = P[i0], p1 = P[i0 + s], p2 = P[i0 + 2 * s], p3 = P[i0 + 3 * s];
+        A[i0] = p0;
+        A[i0 + s] = mid(p0, p1);
+        A[i0 + 2 * s] = add(half(A[i0 + s]), quarter_sum(p1, p2));
+        B[i0 + 3 * s] = p3;
+        B[i0 + 2 * s] = mid(p2, p3);
+        B[i0 + s] = add(half(B[i0 + 2 * s]), quarter_sum(p1, p2));
+        A[i0 + 3 * s] = mid(A[i0 + 2 * s], B[i0 + s]);
+        B[i0] = A[i0 + 3 * s];
+    }
+}
+
+// surfaceNaturalSubdivide (surface.cpp:301-327)
+void split_bilinear(const f3* P, f3* A, f3* B, bool u) {
+    for (int i = 0; i < 4; ++i) { A[i] = P[i]; B[i] = P[i]; }
+    if (u) { A[1] = B[0] = mid(P[0], P[1]); A[3] = B[2] = mid(P[2], P[3]); }
+    else { A[2] = B[0] = mid(P[0], P[2]); A[3] = B[1] = mid(P[1], P[3]); }
+}
+
+// CullSurface + bucket range (imagebuffer.cpp:268-345, 353-446)
+void post_geometry(Node& n, const Ctx& c) {
+    const Shape& s = *c.s;
+    const rh_options& o = s.o;
+    box3 B = (n.type == RH_PRIM_SPHERE) ? sphere_camera_bound(n, c.objToCam[n.prim])
+                                        : patch_camera_bound(n, n.type == RH_PRIM_BICUBIC ? 16 : 4);
+    if (B.lo.z >= o.clip_far || B.hi.z <= o.clip_near) { n.culled = true; return; }
+    if (B.lo.z <= FLT_EPSILON) {
+        n.diceableFlag = false;
+        if (n.splitCount > o.eyesplits) n.culled = true;
+        return;
+    }
+    float z0 = B.lo.z, z1 = B.hi.z;
+    B = box_xform(B, s.camToRaster);
+    B.lo.x = B.lo.x - o.filter_xwidth / 2.0f;
+    B.lo.y = B.lo.y - o.filter_ywidth / 2.0f;
+    B.hi.x = B.hi.x + o.filter_xwidth / 2.0f;
+    B.hi.y = B.hi.y + o.filter_ywidth / 2.0f;
+    if (B.lo.x > s.cropX1 || B.lo.y > s.cropY1 || B.hi.x < s.cropX0 || B.hi.y < s.cropY0) { n.culled = true; return; }
+    B.lo.z = z0; B.hi.z = z1;
+    n.rb = B;
+    n.hasBound = true;
+    n.xminb = static_cast<int>(B.lo.x) / o.bucket_x;
+    n.yminb = static_cast<int>(B.lo.y) / o.bucket_y;
+    n.xmaxb = static_cast<int>(B.hi.x) / o.bucket_x;
+    n.ymaxb = static_cast<int>(B.hi.y) / o.bucket_y;
+}
+
+// Expand the whole split tree of one primitive (everything that does not
+// depend on the bucket being processed).
+void build_tree(const Ctx& c, int prim, std::vector<Node>& t) {
+    const rh_prim& p = c.prims[prim];
+    Node root;
+    root.type = p.type;
+    root.prim = prim;
+    if (p.type == RH_PRIM_SPHERE) {
+        root.radius = p.sphere[0];
+        root.thetaMin = 0.0f;
+        root.thetaMax = p.sphere[3];
+        root.phiMin = (float)std::asin((double)(p.sphere[1] / root.radius));
+        root.phiMax = (float)std::asin((double)(p.sphere[2] / root.radius));
+    } else {
+        int k = p.type == RH_PRIM_BICUBIC ? 16 : 4;
+        for (int i = 0; i < k; ++i) root.cp[i] = mk(p.P[i][0], p.P[i][1], p.P[i][2]);
+    }
+    t.clear();
+    t.push_back(root);
+    for (size_t i = 0; i < t.size(); ++i) {
+        post_geometry(t[i], c);
+        if (t[i].culled) continue;
+        Node& n = t[i];
+        bool d;
+        if (n.type == RH_PRIM_SPHERE) d = sphere_diceable(n, c);
+        else if (n.type == RH_PRIM_BICUBIC) d = bicubic_diceable(n, c);
+        else d = bilinear_diceable(n, c);
+        if (d) { n.leaf = true; continue; }
+        if (n.discard) continue;
+        // CqSurface::Split (surface.cpp:529-577): two halves; an undiceable
+        // (eye-split) surface splits each half once more.
+        auto halves = [&](const Node& src, Node& a, Node& b) {
+            a = src; b = src;
+            bool u = src.splitDir == DIR_U;
+            if (src.type == RH_PRIM_SPHERE) {
+                float pc = (float)((src.phiMin + src.phiMax) * 0.5);
+                float tc = (float)((src.thetaMin + src.thetaMax) * 0.5);
+                if (u) { a.thetaMax = tc; b.thetaMin = tc; } else { a.phiMax = pc; b.phiMin = pc; }
+            } else if (src.type == RH_PRIM_BICUBIC) {
+                split_bicubic(src.cp, a.cp, b.cp, u);
+            } else {
+                split_bilinear(src.cp, a.cp, b.cp, u);
+            }
+            for (Node* k : {&a, &b}) {
+                k->splitDir = u ? DIR_V : DIR_U;
+                k->splitCount = src.splitCount + 1;
+                k->diceableFlag = true; k->discard = false; k->culled = false;
+                k->hasBound = false; k->leaf = false; k->kid0 = -1; k->nkids = 0;
+            }
+        };
+        Node a, b;
+        halves(t[i], a, b);
+        std::vector<Node> kids;
+        if (!t[i].diceableFlag && t[i].type != RH_PRIM_BILINEAR) {
+            Node a0, a1, b0, b1;
+            halves(a, a0, a1);
+            halves(b, b0, b1);
+            kids = {a0, a1, b0, b1};
+        } else {
+            kids = {a, b};
+        }
+        t[i].kid0 = (int)t.size();
+        t[i].nkids = (int)kids.size();
+        for (auto& k : kids) t.push_back(k);
+    }
+}
+
+struct HeapCmp {  // closest_surface (bucket.h:164-176)
+    bool operator()(const Node* a, const Node* b) const {
+        if (a->hasBound && b->hasBound) return a->rb.lo.z > b->rb.lo.z;
+        return true;
+    }
+};
+
+}  // namespace
+
+bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, std::string& err, int threads) {
+    Ctx c;
+    c.s = &s;
+    c.prims = prims;
+    c.objToCam.assign(n, m_identity());
+    for (int i = 0; i < n; ++i) {
+        const rh_prim& p = prims[i];
+        if (p.type != RH_PRIM_SPHERE && p.type != RH_PRIM_BICUBIC && p.type != RH_PRIM_BILINEAR) {
+            err = "unknown surface type at index " + std::to_string(i);
+            return false;
+        }
+        if (p.nkeys != 1) { err = "motion keys are not supported by this build"; return false; }
+        if (!(p.shading_rate > 0)) { err = "shading rate must be positive"; return false; }
+        for (int k = 0; k < 3; ++k)
+            if (!(p.Os[k] >= 1.0f)) { err = "partially transparent surfaces are not supported by this build"; return false; }
+        if (p.type == RH_PRIM_SPHERE) {
+            if (!(p.sphere[0] != 0)) { err = "sphere radius must be non-zero"; return false; }
+            if (p.sphere[3] != 360.0f || p.sphere[1] != -p.sphere[0] || p.sphere[2] != p.sphere[0]) {
+                // Partial spheres follow the same code; the CqSphere constructor's
+                // zmin/zmax clamp is not restated, so keep them inside the radius.
+                if (std::fabs(p.sphere[1]) > std::fabs(p.sphere[0]) || std::fabs(p.sphere[2]) > std::fabs(p.sphere[0])) {
+                    err = "sphere zmin/zmax outside the radius";
+                    return false;
+                }
+            }
+            m44 M;
+            bool ident = true;
+            for (int r = 0; r < 4; ++r)
+                for (int cc = 0; cc < 4; ++cc) {
+                    M.e[r][cc] = p.xform[0][4 * r + cc];
+                    if (M.e[r][cc] != (r == cc ? 1.0f : 0.0f)) ident = false;
+                }
+            M.id = ident;
+            c.objToCam[i] = M;
+        }
+    }
+    // 1) Split trees, one per primitive, in parallel.
+    std::vector<std::vector<Node>> trees(n);
+    int nt = std::max(1, std::min(threads, n));
+    std::atomic<int> next(0);
+    auto work = [&]() {
+        for (;;) {
+            int i = next.fetch_add(1);
+            if (i >= n) break;
+            build_tree(c, i, trees[i]);
+        }
+    };
+    if (nt == 1) work();
+    else {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < nt; ++k) pool.emplace_back(work);
+        for (auto& th : pool) th.join();
+    }
+
+    // 2) Bucket loop: post roots in order, pop nearest first, post children.
+    const int nb = (int)s.buckets.size();
+    std::vector<std::vector<Node*>> heaps(nb);
+    std::vector<uint8_t> processed(nb, 0);
+    out = Schedule();
+    auto post = [&](Node* nd) {
+        out.surfaces++;
+        if (nd->culled) return;
+        int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+        if (nd->diceableFlag) { x0 = nd->xminb; y0 = nd->yminb; x1 = nd->xmaxb; y1 = nd->ymaxb; }
+        x0 = clampv(x0, s.bx0, s.bx1 - 1); y0 = clampv(y0, s.by0, s.by1 - 1);
+        x1 = clampv(x1, s.bx0, s.bx1 - 1); y1 = clampv(y1, s.by0, s.by1 - 1);
+        int k = s.bucketIndex(x0, y0);
+        if (!processed[k]) {
+            heaps[k].push_back(nd);
+            std::push_heap(heaps[k].begin(), heaps[k].end(), HeapCmp());
+            return;
+        }
+        int yb = y0, xb = x0 + 1;
+        while (yb <= y1) {
+            while (xb <= x1) {
+                int kk = s.bucketIndex(xb, yb);
+                if (!processed[kk]) {
+                    heaps[kk].push_back(nd);
+                    std::push_heap(heaps[kk].begin(), heaps[kk].end(), HeapCmp());
+                    return;
+                }
+                ++xb;
+            }
+            xb = x0;
+            ++yb;
+        }
+    };
+    for (int i = 0; i < n; ++i) post(&trees[i][0]);
+    uint32_t mpCount = 0, vtxCount = 0;
+    std::vector<float>& P = out.params;
+    for (int k = 0; k < nb; ++k) {
+        auto& h = heaps[k];
+        while (!h.empty()) {
+            Node* nd = h.front();
+            std::pop_heap(h.begin(), h.end(), HeapCmp());
+            h.pop_back();
+            if (nd->leaf) {
+                GridRec g;
+                g.type = nd->type; g.cu = nd->uDice; g.cv = nd->vDice; g.prim = nd->prim;
+                g.mpBase = mpCount; g.vtxBase = vtxCount; g.paramOff = (uint32_t)P.size();
+                g.diceBucket = k;
+                // Candidate buckets: the cached raster bound (already grown by
+                // half the filter width) plus a one-pixel guard band.
+                const rh_options& o = s.o;
+                auto bidx = [](float v, int bs) { return (int)std::floor((double)v / bs); };
+                g.cbx0 = (int16_t)clampv(bidx(nd->rb.lo.x - 1.0f, o.bucket_x), s.bx0, s.bx1 - 1);
+                g.cby0 = (int16_t)clampv(bidx(nd->rb.lo.y - 1.0f, o.bucket_y), s.by0, s.by1 - 1);
+                g.cbx1 = (int16_t)clampv(bidx(nd->rb.hi.x + 1.0f, o.bucket_x), s.bx0, s.bx1 - 1);
+                g.cby1 = (int16_t)clampv(bidx(nd->rb.hi.y + 1.0f, o.bucket_y), s.by0, s.by1 - 1);
+                if ((g.cu + 1) * (g.cv + 1) > 514 || g.cu < 1 || g.cv < 1) {
+                    err = "internal: grid size outside the dicing limits";
+                    return false;
+                }
+                mpCount += (uint32_t)(g.cu * g.cv);
+                vtxCount += (uint32_t)((g.cu + 1) * (g.cv + 1));
+                if (nd->type == RH_PRIM_SPHERE) {
+                    const m44& M = c.objToCam[nd->prim];
+                    P.push_back(nd->radius);
+                    for (int r = 0; r < 4; ++r)
+                        for (int cc = 0; cc < 4; ++cc) P.push_back(M.e[r][cc]);
+                    P.push_back(M.id ? 1.0f : 0.0f);
+                    size_t o0 = P.size();
+                    P.resize(o0 + 2 * (g.cu + 1) + 2 * (g.cv + 1));
+                    sincos_grid(deg2rad(nd->thetaMin), deg2rad(nd->thetaMax), g.cu + 1, &P[o0], &P[o0 + g.cu + 1]);
+                    sincos_grid(nd->phiMin, nd->phiMax, g.cv + 1, &P[o0 + 2 * (g.cu + 1)],
+                                &P[o0 + 2 * (g.cu + 1) + g.cv + 1]);
+                } else {
+                    int kk = nd->type == RH_PRIM_BICUBIC ? 16 : 4;
+                    for (int i = 0; i < kk; ++i) { P.push_back(nd->cp[i].x); P.push_back(nd->cp[i].y); P.push_back(nd->cp[i].z); }
+                }
+                out.grids.push_back(g);
+            } else if (nd->nkids > 0) {
+                std::vector<Node>* tree = nullptr;
+                // children live in the same tree as their parent
+                tree = &trees[nd->prim];
+                for (int q = 0; q < nd->nkids; ++q) post(&(*tree)[nd->kid0 + q]);
+            }
+        }
+        processed[k] = 1;
+    }
+    out.micropolygons = mpCount;
+    out.vertices = vtxCount;
+
+    // 3) Candidate lists per bucket (grids in dicing order), only buckets not
+    // yet processed when the grid was diced.
+    out.candStart.assign(nb + 1, 0);
+    for (const auto& g : out.grids)
+        for (int by = g.cby0; by <= g.cby1; ++by)
+            for (int bx = g.cbx0; bx <= g.cbx1; ++bx) {
+                int k = s.bucketIndex(bx, by);
+                if (k >= g.diceBucket) out.candStart[k + 1]++;
+            }
+    for (int k = 0; k < nb; ++k) out.candStart[k + 1] += out.candStart[k];
+    out.candGrid.assign(out.candStart[nb], 0);
+    out.capStart.assign(nb + 1, 0);
+    std::vector<uint32_t> fill(out.candStart.begin(), out.candStart.end() - 1);
+    for (uint32_t gi = 0; gi < out.grids.size(); ++gi) {
+        const auto& g = out.grids[gi];
+        for (int by = g.cby0; by <= g.cby1; ++by)
+            for (int bx = g.cbx0; bx <= g.cbx1; ++bx) {
+                int k = s.bucketIndex(bx, by);
+                if (k >= g.diceBucket) {
+                    out.candGrid[fill[k]++] = gi;
+                    out.capStart[k + 1] += (uint64_t)(g.cu * g.cv);
+                }
+            }
+    }
+    for (int k = 0; k < nb; ++k) out.capStart[k + 1] += out.capStart[k];
+    return true;
+}
+
+}  // namespace rh

@@ -1,0 +1,627 @@
+// HIP kernels of the hider for gfx950 (CDNA4).
+//
+//   rh_dice_bust   one workgroup per grid: dice (sphere / bicubic / bilinear),
+//                  project to raster, and write one hit-test record per
+//                  micropolygon (its edge equations, inverse-bilinear set-up,
+//                  depths and bound).
+//   rh_bin         one workgroup per bucket: walks the bucket's candidate
+//                  grids in dicing order, keeps the micropolygons the bucket
+//                  would have received (AddMPG) and that touch its sample
+//                  region, and appends them in order with a stable
+//                  block-wide compaction.
+//   rh_sample      one wave per block of 64 samples: walks its bucket's
+//                  ordered list, ballot-filters the micropolygons whose pixel
+//                  window meets the block, and runs the reference's per-sample
+//                  test (bound, occlusion, LOD, point-in-quad, inverse
+//                  bilinear, depth) in list order, so each sample sees the
+//                  micropolygons in exactly the reference's order.
+//   rh_filter      one thread per output pixel: 2-D filter gather with the
+//                  per-sample box test, exposure, quantise + dither.
+//
+// All single-precision code is built with -ffp-contract=off and correctly
+// rounded division; every expression keeps the operand order of the aqsis
+// code it cites.
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cstdint>
+
+#include "devtypes.h"
+
+namespace {
+
+__device__ __forceinline__ int floor_i(float x) {  // static_cast<int>(std::floor(x))
+    return (int)floorf(x);
+}
+__device__ __forceinline__ long lceil_f(float x) {  // math.h:57-61
+    long t = (long)x;
+    return t + (x > 0 && x != (float)t);
+}
+__device__ __forceinline__ long lfloor_d(double x) {  // math.h:47-51
+    long t = (long)x;
+    return t - (x < 0 && x != (double)t);
+}
+__device__ __forceinline__ float fmin_ref(float a, float b) { return (a < b) ? a : b; }
+__device__ __forceinline__ float fmax_ref(float a, float b) { return (a < b) ? b : a; }
+
+struct f3d {
+    float x, y, z;
+};
+__device__ __forceinline__ f3d mk3(float x, float y, float z) { f3d r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ f3d add3(f3d a, f3d b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3d mul3(f3d a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+
+// CqMatrix * CqVector3D (matrix.h:717-750); m[16] is [row][col] row-major.
+__device__ __forceinline__ f3d xform_point(const float* m, bool ident, f3d v) {
+    if (ident) return v;
+    float h = m[3] * v.x + m[7] * v.y + m[11] * v.z + m[15];
+    f3d r;
+    r.x = m[0] * v.x + m[4] * v.y + m[8] * v.z + m[12];
+    r.y = m[1] * v.x + m[5] * v.y + m[9] * v.z + m[13];
+    r.z = m[2] * v.x + m[6] * v.y + m[10] * v.z + m[14];
+    if (h != 1) {
+        float inv = 1 / h;
+        r.x = r.x * inv; r.y = r.y * inv; r.z = r.z * inv;
+    }
+    return r;
+}
+
+// CqForwardDiffBezier (forwarddiff.h:35-103)
+struct FD {
+    float M00, M01, M02, M03, M10, M11, M12, M13, M20, M21, M22, M23;
+    f3d f, df, ddf, dddf;
+    __device__ void init(float dt) {
+        float dt2 = dt * dt, dt3 = dt2 * dt, dt3_3 = 3 * dt3, dt3_6 = 6 * dt3, dt3_18 = 18 * dt3, dt2_3 = 3 * dt2,
+              dt2_6 = 6 * dt2, dt_3 = 3 * dt;
+        M00 = -dt3_6; M01 = dt3_18; M02 = -dt3_18; M03 = dt3_6;
+        M10 = dt2_6 - dt3_6; M11 = dt3_18 - 2 * dt2_6; M12 = dt2_6 - dt3_18; M13 = dt3_6;
+        M20 = dt2_3 - dt_3 - dt3; M21 = dt3_3 - dt2_6 + dt_3; M22 = dt2_3 - dt3_3; M23 = dt3;
+    }
+    __device__ void calc(f3d A, f3d B, f3d C, f3d D) {
+        f = A;
+        df = add3(add3(add3(mul3(A, M20), mul3(B, M21)), mul3(C, M22)), mul3(D, M23));
+        ddf = add3(add3(add3(mul3(A, M10), mul3(B, M11)), mul3(C, M12)), mul3(D, M13));
+        dddf = add3(add3(add3(mul3(A, M00), mul3(B, M01)), mul3(C, M02)), mul3(D, M03));
+    }
+    __device__ f3d get() {
+        f3d r = f;
+        f = add3(f, df);
+        df = add3(df, ddf);
+        ddf = add3(ddf, dddf);
+        return r;
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Dice + bust
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__ fr, const GridDev* __restrict__ grids,
+                                                    const float* __restrict__ params, float4* __restrict__ vtx,
+                                                    float* __restrict__ mprec, float4* __restrict__ mpbound) {
+    __shared__ float4 V[RH_MAX_GRID_VERTS];
+    __shared__ float rows[4 * RH_MAX_GRID_ROWS * 3];
+    const int tid = threadIdx.x;
+    const GridDev G = grids[blockIdx.x];
+    const int cu = G.cu, cv = G.cv, nu = cu + 1, nv = (cu + 1) * (cv + 1);
+    const float* prm = params + G.paramOff;
+
+    if (G.type == RH_PRIM_SPHERE) {
+        // CqSphere::DicePoints + DiceAll (quadrics.cpp:457-492, 82-254)
+        const float radius = prm[SPH_RADIUS_D];
+        const bool ident = prm[SPH_IDENT_D] != 0.0f;
+        const float* sinT = prm + SPH_TABLES_D;
+        const float* cosT = sinT + nu;
+        const float* sinP = cosT + nu;
+        const float* cosP = sinP + (cv + 1);
+        for (int i = tid; i < nv; i += blockDim.x) {
+            int u = i % nu, v = i / nu;
+            f3d unitP = mk3(cosT[u] * cosP[v], sinT[u] * cosP[v], sinP[v]);
+            f3d p = mk3(radius * unitP.x, radius * unitP.y, radius * unitP.z);
+            p = xform_point(prm + SPH_MAT_D, ident, p);
+            V[i] = make_float4(p.x, p.y, p.z, 0.0f);
+        }
+    } else if (G.type == RH_PRIM_BICUBIC) {
+        // bicubicPatchNatDice (patch.cpp:205-243): four v-direction forward
+        // differences give each row's control points, then one u-direction
+        // forward difference per row.
+        if (tid < 4) {
+            FD fd;
+            fd.init(1.0f / (float)cv);
+            const float* c = prm;
+            fd.calc(mk3(c[3 * tid], c[3 * tid + 1], c[3 * tid + 2]),
+                    mk3(c[3 * (4 + tid)], c[3 * (4 + tid) + 1], c[3 * (4 + tid) + 2]),
+                    mk3(c[3 * (8 + tid)], c[3 * (8 + tid) + 1], c[3 * (8 + tid) + 2]),
+                    mk3(c[3 * (12 + tid)], c[3 * (12 + tid) + 1], c[3 * (12 + tid) + 2]));
+            for (int iv = 0; iv <= cv; ++iv) {
+                f3d q = fd.get();
+                float* r = rows + (iv * 4 + tid) * 3;
+                r[0] = q.x; r[1] = q.y; r[2] = q.z;
+            }
+        }
+        __syncthreads();
+        for (int iv = tid; iv <= cv; iv += blockDim.x) {
+            FD fd;
+            fd.init(1.0f / (float)cu);
+            const float* r = rows + iv * 12;
+            fd.calc(mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), mk3(r[6], r[7], r[8]), mk3(r[9], r[10], r[11]));
+            for (int iu = 0; iu <= cu; ++iu) {
+                f3d q = fd.get();
+                V[iv * nu + iu] = make_float4(q.x, q.y, q.z, 0.0f);
+            }
+        }
+    } else {
+        // surfaceNaturalDice + BilinearEvaluate (surface.cpp:365-388, bilinear.h:189-225)
+        const float* c = prm;
+        for (int i = tid; i < nv; i += blockDim.x) {
+            int iu = i % nu, iv = i / nu;
+            float v = (1.0f / (float)cv) * iv;
+            float u = (1.0f / (float)cu) * iu;
+            f3d A = mk3(c[0], c[1], c[2]), B = mk3(c[3], c[4], c[5]), C = mk3(c[6], c[7], c[8]),
+                D = mk3(c[9], c[10], c[11]);
+            f3d AB, CD, R;
+            if (u <= 0.0f) { AB = A; CD = C; }
+            else if (u >= 1.0f) { AB = B; CD = D; }
+            else {
+                AB = add3(mul3(mk3(B.x - A.x, B.y - A.y, B.z - A.z), u), A);
+                CD = add3(mul3(mk3(D.x - C.x, D.y - C.y, D.z - C.z), u), C);
+            }
+            if (v <= 0.0f) R = AB;
+            else if (v >= 1.0f) R = CD;
+            else R = add3(mul3(mk3(CD.x - AB.x, CD.y - AB.y, CD.z - AB.z), v), AB);
+            V[i] = make_float4(R.x, R.y, R.z, 0.0f);
+        }
+    }
+    __syncthreads();
+
+    // Projection to hybrid raster space keeping camera z (micropolygon.cpp:723-731)
+    for (int i = tid; i < nv; i += blockDim.x) {
+        float4 p = V[i];
+        f3d r = xform_point(fr->camToRaster, false, mk3(p.x, p.y, p.z));
+        float4 q = make_float4(r.x, r.y, p.z, 0.0f);
+        V[i] = q;
+        vtx[G.vtxBase + i] = q;
+    }
+    __syncthreads();
+
+    // One hit-test record per micropolygon.
+    const int nmp = cu * cv;
+    for (int m = tid; m < nmp; m += blockDim.x) {
+        const int iu = m % cu, iv = m / cu;
+        const int idx = iv * nu + iu;
+        float4 q0 = V[idx], q1 = V[idx + 1], q2 = V[idx + nu], q3 = V[idx + nu + 1];
+        f3d P[4] = {mk3(q0.x, q0.y, q0.z), mk3(q1.x, q1.y, q1.z), mk3(q2.x, q2.y, q2.z), mk3(q3.x, q3.y, q3.z)};
+        // ComputeVertexOrder (micropolygon.cpp:1207-1284): A,B,C,D = P0,P1,P3,P2
+        f3d pA = P[0], pB = P[1], pC = P[3], pD = P[2];
+        int codeA = 0, codeB = 1, codeC = 3, codeD = 2;
+        auto m2 = [](f3d a, f3d b) {
+            float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+            return dx * dx + dy * dy + dz * dz;
+        };
+        f3d vA = pA, vB = pB, vC = pC;
+        if ((double)m2(pA, pB) < 1e-8) {
+            vB = pC; codeB = codeC; vC = pD; codeC = codeD; codeD = -1;
+        } else if ((double)m2(pB, pC) < 1e-8) {
+            vB = pC; codeB = codeC; vC = pD; codeC = codeD; codeD = -1;
+        } else if ((double)m2(pC, pD) < 1e-8) {
+            vC = pD; codeC = codeD; codeD = -1;
+        } else if ((double)m2(pD, pA) < 1e-8) {
+            codeD = -1;
+        }
+        bool flip = ((vA.x - vB.x) * (vB.y - vC.y)) >= ((vA.y - vB.y) * (vB.x - vC.x));
+        int code;
+        bool degen = codeD == -1;
+        if (!flip)
+            code = degen ? ((codeA & 3) | ((codeC & 3) << 2) | ((codeB & 3) << 4))
+                         : ((codeA & 3) | ((codeD & 3) << 2) | ((codeC & 3) << 4) | ((codeB & 3) << 6));
+        else
+            code = degen ? ((codeA & 3) | ((codeB & 3) << 2) | ((codeC & 3) << 4))
+                         : ((codeA & 3) | ((codeB & 3) << 2) | ((codeC & 3) << 4) | ((codeD & 3) << 6));
+        // cachePointInPolyTest (micropolygon.cpp:1346-1392)
+        f3d pts[4] = {P[(code >> 2) & 3], P[(code >> 4) & 3], P[(code >> 6) & 3], P[code & 3]};
+        float YM[4], XM[4], X[4], Y[4];
+        int j = 3;
+        for (int i = 0; i < 4; ++i) {
+            YM[i] = pts[i].x - pts[j].x;
+            XM[i] = pts[i].y - pts[j].y;
+            X[i] = pts[j].x;
+            Y[i] = pts[j].y;
+            j = i;
+        }
+        if (degen) {
+            for (int i = 2; i < 4; ++i) {
+                YM[i] = pts[3].x - pts[1].x;
+                XM[i] = pts[3].y - pts[1].y;
+                X[i] = pts[1].x;
+                Y[i] = pts[1].y;
+            }
+        }
+        // CqInvBilinear::setVertices (bilinear.h:260-275)
+        float Ex = P[1].x - P[0].x, Ey = P[1].y - P[0].y;
+        float Fx = P[2].x - P[0].x, Fy = P[2].y - P[0].y;
+        float Gx = ((-Ex) - P[2].x) + P[3].x, Gy = ((-Ey) - P[2].y) + P[3].y;
+        float patchSize = fmax_ref(fmax_ref(fabsf(Fx), fabsf(Fy)), fmax_ref(fabsf(Ex), fabsf(Ey)));
+        float irregularity = fmax_ref(fabsf(Gx), fabsf(Gy));
+        bool linear = (double)irregularity < 1e-2 * (double)patchSize;
+        // CalculateBound (micropolygon.cpp:1663-1673): min(min(min(A,B),C),D)
+        float bminx = fmin_ref(fmin_ref(fmin_ref(pA.x, pB.x), pC.x), pD.x);
+        float bminy = fmin_ref(fmin_ref(fmin_ref(pA.y, pB.y), pC.y), pD.y);
+        float bminz = fmin_ref(fmin_ref(fmin_ref(pA.z, pB.z), pC.z), pD.z);
+        float bmaxx = fmax_ref(fmax_ref(fmax_ref(pA.x, pB.x), pC.x), pD.x);
+        float bmaxy = fmax_ref(fmax_ref(fmax_ref(pA.y, pB.y), pC.y), pD.y);
+        float4* out = reinterpret_cast<float4*>(mprec + (size_t)(G.mpBase + m) * RH_MPREC_FLOATS);
+        out[0] = make_float4(YM[0], YM[1], YM[2], YM[3]);
+        out[1] = make_float4(XM[0], XM[1], XM[2], XM[3]);
+        out[2] = make_float4(X[0], X[1], X[2], X[3]);
+        out[3] = make_float4(Y[0], Y[1], Y[2], Y[3]);
+        out[4] = make_float4(P[0].z, P[1].z, P[2].z, P[3].z);
+        out[5] = make_float4(P[0].x, P[0].y, Ex, Ey);
+        out[6] = make_float4(Fx, Fy, Gx, Gy);
+        out[7] = make_float4(bminx, bmaxx, bminy, bmaxy);
+        out[8] = make_float4(bminz, __int_as_float(linear ? 1 : 0), __int_as_float((int)blockIdx.x), 0.0f);
+        mpbound[G.mpBase + m] = make_float4(bminx, bmaxx, bminy, bmaxy);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Binning: the ordered micropolygon list of each bucket
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) rh_bin(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets,
+                                              const GridDev* __restrict__ grids, const uint32_t* __restrict__ candStart,
+                                              const uint32_t* __restrict__ candGrid,
+                                              const unsigned long long* __restrict__ capStart,
+                                              const float4* __restrict__ mpbound, uint2* __restrict__ lists,
+                                              uint32_t* __restrict__ listCount) {
+    __shared__ uint32_t waveTotals[4];
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const BucketDev B = buckets[b];
+    const int bx = B.bx, by = B.by;
+    uint2* out = lists + capStart[b];
+    uint32_t count = 0;
+    const float hfx = fr->filterX * 0.5f, hfy = fr->filterY * 0.5f;   // xFiltSize / 2.0f
+    const float dsx = (float)fr->shiftX, dsy = (float)fr->shiftY;      // lfloor(xFiltSize / 2.0f)
+    for (uint32_t c = candStart[b]; c < candStart[b + 1]; ++c) {
+        const GridDev G = grids[candGrid[c]];
+        const int nmp = G.cu * G.cv;
+        for (int base = 0; base < nmp; base += 256) {
+            const int m = base + tid;
+            bool keep = false;
+            uint32_t win = 0;
+            if (m < nmp) {
+                const float4 bd = mpbound[G.mpBase + m];  // bminx, bmaxx, bminy, bmaxy
+                // CqImageBuffer::AddMPG (imagebuffer.cpp:514-591)
+                bool discard = bd.y < fr->cropX0 - hfx || bd.w < fr->cropY0 - hfy || bd.x > fr->cropX1 + hfx ||
+                               bd.z > fr->cropY1 + hfy;
+                if (!discard) {
+                    float ex0 = bd.x - dsx, ey0 = bd.z - dsy, ex1 = bd.y + dsx, ey1 = bd.w + dsy;
+                    int iXBa = (int)(ex0 / (float)fr->bucketX), iYBa = (int)(ey0 / (float)fr->bucketY);
+                    int iXBb = (int)(ex1 / (float)fr->bucketX), iYBb = (int)(ey1 / (float)fr->bucketY);
+                    bool outside = iXBb < fr->bx0 || iYBb < fr->by0 || iXBa >= fr->bx1 || iYBa >= fr->by1;
+                    if (!outside) {
+                        if (iXBa < fr->bx0) iXBa = fr->bx0;
+                        if (iYBa < fr->by0) iYBa = fr->by0;
+                        if (iXBb >= fr->bx1) iXBb = fr->bx1 - 1;
+                        if (iYBb >= fr->by1) iYBb = fr->by1 - 1;
+                        if (bx >= iXBa && bx <= iXBb && by >= iYBa && by <= iYBb) {
+                            // RenderMPG_Static pixel range clipped to the sample region
+                            // (bucketprocessor.cpp:1120-1140); empty ranges do no work.
+                            int eX = (int)lceil_f(bd.y), eY = (int)lceil_f(bd.w);
+                            if (eX > B.sx1) eX = B.sx1;
+                            if (eY > B.sy1) eY = B.sy1;
+                            int sX = floor_i(bd.x), sY = floor_i(bd.z);
+                            if (sY < B.sy0) sY = B.sy0;
+                            if (sX < B.sx0) sX = B.sx0;
+                            if (sX < eX && sY < eY) {
+                                keep = true;
+                                win = (uint32_t)(sX - B.sx0) | ((uint32_t)(eX - B.sx0) << 8) |
+                                      ((uint32_t)(sY - B.sy0) << 16) | ((uint32_t)(eY - B.sy0) << 24);
+                            }
+                        }
+                    }
+                }
+            }
+            // Stable block-wide compaction.
+            unsigned long long mask = __ballot(keep);
+            uint32_t before = __popcll(mask & ((1ull << lane) - 1ull));
+            if (lane == 0) waveTotals[wave] = (uint32_t)__popcll(mask);
+            __syncthreads();
+            uint32_t waveBase = 0, total = 0;
+            for (int w = 0; w < 4; ++w) {
+                if (w < wave) waveBase += waveTotals[w];
+                total += waveTotals[w];
+            }
+            if (keep) out[count + waveBase + before] = make_uint2(G.mpBase + (uint32_t)m, win);
+            count += total;
+            __syncthreads();
+        }
+    }
+    if (tid == 0) listCount[b] = count;
+}
+
+// ---------------------------------------------------------------------------
+// Sampling: per-sample visibility in the reference's micropolygon order
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets,
+                                                 const int4* __restrict__ blocks, int nblocks,
+                                                 const unsigned long long* __restrict__ capStart,
+                                                 const uint32_t* __restrict__ listCount, const uint2* __restrict__ lists,
+                                                 const float* __restrict__ mprec, const GridDev* __restrict__ grids,
+                                                 const PrimDev* __restrict__ prims, const float* __restrict__ pos2d,
+                                                 const float* __restrict__ pos1d, const uchar4* __restrict__ pixPattern,
+                                                 float4* __restrict__ samples, unsigned long long* __restrict__ counters) {
+    const int lane = threadIdx.x & 63;
+    const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (blk >= nblocks) return;
+    const int4 bl = blocks[blk];  // bucket, x0, y0, -
+    const BucketDev B = buckets[bl.x];
+    const int ns = fr->ns, nx = fr->nx, ny = fr->ny, bw = fr->blockW, bh = fr->blockH;
+    const int pix = lane / ns, si = lane - pix * ns;
+    const int px = bl.y + pix % bw, py = bl.z + pix / bw;
+    const bool live = pix < bw * bh && px < B.sx1 && py < B.sy1;
+    const int sm = si % nx, sn = si / nx;
+
+    float sx = 0.f, sy = 0.f, lod = 0.f;
+    size_t sidx = 0;
+    if (live) {
+        const int ex = px - fr->extX0, ey = py - fr->extY0;
+        const uchar4 pat = pixPattern[(size_t)ey * fr->extW + ex];
+        // position = (x, y) + pattern position (imagepixel.cpp:353)
+        sx = (float)px + pos2d[2 * ((size_t)pat.x * ns + si)];
+        sy = (float)py + pos2d[2 * ((size_t)pat.x * ns + si) + 1];
+        lod = pos1d[(size_t)pat.z * ns + si];
+        sidx = ((size_t)ey * fr->extW + ex) * ns + si;
+    }
+    float occlZ = FLT_MAX, wu = 0.f, wv = 0.f;
+    uint32_t win = 0xffffffffu;
+    uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
+
+    const int bx0 = bl.y - B.sx0, by0 = bl.z - B.sy0;   // block rect relative to the sample region
+    const int bx1 = bx0 + bw, by1 = by0 + bh;
+    const uint2* L = lists + capStart[bl.x];
+    const uint32_t n = listCount[bl.x];
+    for (uint32_t c = 0; c < n; c += 64) {
+        uint2 e = make_uint2(0u, 0u);
+        bool meets = false;
+        if (c + lane < n) {
+            e = L[c + lane];
+            int wx0 = e.y & 255, wx1 = (e.y >> 8) & 255, wy0 = (e.y >> 16) & 255, wy1 = e.y >> 24;
+            meets = wx0 < bx1 && wx1 > bx0 && wy0 < by1 && wy1 > by0;
+        }
+        unsigned long long mask = __ballot(meets);
+        while (mask) {
+            const int j = __ffsll((long long)mask) - 1;
+            mask &= mask - 1ull;
+            const uint32_t id = __builtin_amdgcn_readlane(e.x, j);
+            const float4* R = reinterpret_cast<const float4*>(mprec + (size_t)id * RH_MPREC_FLOATS);
+            const float4 bnd = R[7], r8 = R[8];
+            const float bminx = bnd.x, bmaxx = bnd.y, bminy = bnd.z, bmaxy = bnd.w, bminz = r8.x;
+            // Pixel range and subpixel clip of RenderMPG_Static (bucketprocessor.cpp:1120-1140)
+            int eX = (int)lceil_f(bmaxx), eY = (int)lceil_f(bmaxy);
+            if (eX > B.sx1) eX = B.sx1;
+            if (eY > B.sy1) eY = B.sy1;
+            int sX = floor_i(bminx), sY = floor_i(bminy);
+            if (sY < B.sy0) sY = B.sy0;
+            if (sX < B.sx0) sX = B.sx0;
+            const int im = (bminx < sX) ? 0 : floor_i((bminx - sX) * nx);
+            const int in = (bminy < sY) ? 0 : floor_i((bminy - sY) * ny);
+            const int em = (bmaxx > eX) ? nx : (int)lceil_f((bmaxx - (eX - 1)) * nx);
+            const int en = (bmaxy > eY) ? ny : (int)lceil_f((bmaxy - (eY - 1)) * ny);
+            if (!live || px < sX || px >= eX || py < sY || py >= eY) continue;
+            const int n0 = (py == sY) ? in : 0, n1 = (py == eY - 1) ? en : ny;
+            const int m0 = (px == sX) ? im : 0, m1 = (px == eX - 1) ? em : nx;
+            if (sn < n0 || sn >= n1 || sm < m0 || sm >= m1) continue;
+            ++nTest;
+            // CqBound::Contains2D (bound.h:144-151)
+            if (sx < bminx || sx > bmaxx || sy < bminy || sy > bmaxy) continue;
+            // occlusion skip against the current opaque hit (bucketprocessor.cpp:1179)
+            if (bminz > occlZ) continue;
+            // LevelOfDetailBounds [0, 1] (bucketprocessor.cpp:1183-1189)
+            if (0.0f > lod || lod >= 1.0f) continue;
+            ++nBound;
+            // fContains (micropolygon.cpp:1295-1342): edges 0,1 reject on <= 0, edges 2,3 on < 0
+            const float4 YM = R[0], XM = R[1], X = R[2], Y = R[3];
+            float t0 = ((sy - Y.x) * YM.x) - ((sx - X.x) * XM.x);
+            float t1 = ((sy - Y.y) * YM.y) - ((sx - X.y) * XM.y);
+            float t2 = ((sy - Y.z) * YM.z) - ((sx - X.z) * XM.z);
+            float t3 = ((sy - Y.w) * YM.w) - ((sx - X.w) * XM.w);
+            if (t0 <= 0 || t1 <= 0 || t2 < 0 || t3 < 0) continue;
+            ++nIn;
+            // CqInvBilinear::operator() (bilinear.h:277-305)
+            const float4 r5 = R[5], r6 = R[6], Z = R[4];
+            const float Ax = r5.x, Ay = r5.y, Ex = r5.z, Ey = r5.w, Fx = r6.x, Fy = r6.y, Gx = r6.z, Gy = r6.w;
+            float u = 0.5f, v = 0.5f;
+            {
+                float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
+                float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
+                float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - sx;
+                float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - sy;
+                float det = M1x * M2y - M1y * M2x;
+                det = 1 / det;
+                float dx = det * (bx * M2y - by * M2x);
+                float dy = det * (-(bx * M1y - by * M1x));
+                u = u - dx;
+                v = v - dy;
+            }
+            if (__float_as_int(r8.y) == 0) {
+                ++nIn2;
+                float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
+                float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
+                float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - sx;
+                float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - sy;
+                float det = M1x * M2y - M1y * M2x;
+                if (det != 0) det = 1 / det;
+                float dx = det * (bx * M2y - by * M2x);
+                float dy = det * (-(bx * M1y - by * M1x));
+                u = u - dx;
+                v = v - dy;
+            }
+            // bilerp (bilinear.h:228-236)
+            float w0 = (1 - v) * (1 - u), w1 = (1 - v) * u, w2 = v * (1 - u), w3 = v * u;
+            float D = ((w0 * Z.x + w1 * Z.y) + w2 * Z.z) + w3 * Z.w;
+            // StoreSample (bucketprocessor.cpp:1471-1569): reject when occlZ <= D
+            if (occlZ <= D) continue;
+            ++nHit;
+            occlZ = D;
+            win = id;
+            wu = u;
+            wv = v;
+        }
+    }
+    if (live) {
+        float4 a = make_float4(0.f, 0.f, 0.f, FLT_MAX), b = make_float4(0.f, 0.f, 0.f, __uint_as_float(win));
+        if (win != 0xffffffffu) {
+            const float* R = mprec + (size_t)win * RH_MPREC_FLOATS;
+            const GridDev G = grids[__float_as_int(R[34])];
+            const PrimDev P = prims[G.prim];
+            // InterpolateOutputs, smooth shading with four equal vertex colours
+            // (micropolygon.cpp:1443-1462)
+            float w0 = (1 - wu) * (1 - wv), w1 = wu * (1 - wv), w2 = (1 - wu) * wv, w3 = wu * wv;
+            a.x = ((w0 * P.Ci[0] + w1 * P.Ci[0]) + w2 * P.Ci[0]) + w3 * P.Ci[0];
+            a.y = ((w0 * P.Ci[1] + w1 * P.Ci[1]) + w2 * P.Ci[1]) + w3 * P.Ci[1];
+            a.z = ((w0 * P.Ci[2] + w1 * P.Ci[2]) + w2 * P.Ci[2]) + w3 * P.Ci[2];
+            a.w = occlZ;
+            b.x = ((w0 * P.Oi[0] + w1 * P.Oi[0]) + w2 * P.Oi[0]) + w3 * P.Oi[0];
+            b.y = ((w0 * P.Oi[1] + w1 * P.Oi[1]) + w2 * P.Oi[1]) + w3 * P.Oi[1];
+            b.z = ((w0 * P.Oi[2] + w1 * P.Oi[2]) + w2 * P.Oi[2]) + w3 * P.Oi[2];
+        }
+        samples[2 * sidx] = a;
+        samples[2 * sidx + 1] = b;
+    }
+    // Wave-level reduction of the hider counters.
+    for (int off = 32; off > 0; off >>= 1) {
+        nTest += __shfl_xor(nTest, off);
+        nBound += __shfl_xor(nBound, off);
+        nHit += __shfl_xor(nHit, off);
+        nIn += __shfl_xor(nIn, off);
+        nIn2 += __shfl_xor(nIn2, off);
+    }
+    if (lane == 0) {
+        atomicAdd(&counters[0], (unsigned long long)nTest);
+        atomicAdd(&counters[1], (unsigned long long)nBound);
+        atomicAdd(&counters[2], (unsigned long long)nHit);
+        atomicAdd(&counters[3], (unsigned long long)nIn);
+        atomicAdd(&counters[4], (unsigned long long)nIn2);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Filter + expose + quantise
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) rh_filter(const FrameDev* __restrict__ fr, const float4* __restrict__ samples,
+                                                 const uchar4* __restrict__ pixPattern, const float* __restrict__ pos2d,
+                                                 const float* __restrict__ filterTab, const float* __restrict__ dither,
+                                                 float* __restrict__ channels, uchar4* __restrict__ rgba8) {
+    const int W = fr->cropX1 - fr->cropX0, H = fr->cropY1 - fr->cropY0;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= W * H) return;
+    const int x = fr->cropX0 + t % W, y = fr->cropY0 + t / W;
+    const int ns = fr->ns, xm = fr->shiftX, ym = fr->shiftY;
+    const float xf = fr->xfwo2, yf = fr->yfwo2;
+    const float xcent = x + 0.5f, ycent = y + 0.5f;
+    float gTot = 0.0f;
+    int count = 0;
+    float acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    // FilterBucket, non-separable branch (bucketprocessor.cpp:561-663)
+    for (int fy = -ym; fy <= ym; ++fy)
+        for (int fx = -xm; fx <= xm; ++fx) {
+            const int qx = x + fx, qy = y + fy;
+            const size_t q = (size_t)(qy - fr->extY0) * fr->extW + (qx - fr->extX0);
+            const int pat = pixPattern[q].x;
+            const float* wt = filterTab + ((fy + ym) * (2 * xm + 1) + fx + xm) * ns;
+            for (int si = 0; si < ns; ++si) {
+                float vx = ((float)qx + pos2d[2 * (pat * ns + si)]) - xcent;
+                float vy = ((float)qy + pos2d[2 * (pat * ns + si) + 1]) - ycent;
+                if (vx >= -xf && vy >= -yf && vx <= xf && vy <= yf) {
+                    float g = wt[si];
+                    gTot += g;
+                    const float4 a = samples[2 * (q * ns + si)], b = samples[2 * (q * ns + si) + 1];
+                    if (__float_as_uint(b.w) != 0xffffffffu) {
+                        acc[0] += a.x * g; acc[1] += a.y * g; acc[2] += a.z * g;
+                        acc[3] += b.x * g; acc[4] += b.y * g; acc[5] += b.z * g;
+                        acc[6] += a.w * g;
+                        ++count;
+                    }
+                }
+            }
+        }
+    float ch[9];
+    float coverage;
+    if (count == 0) {
+        for (int k = 0; k < 9; ++k) ch[k] = 0.0f;
+        ch[7] = FLT_MAX;
+        coverage = 0.0f;
+    } else {
+        float inv = (float)(1.0 / (double)gTot);
+        for (int k = 0; k < 6; ++k) ch[k] = acc[k] * inv;
+        ch[7] = acc[6] * inv;
+        coverage = (count >= ns) ? 1.0f : (float)count / (float)ns;
+    }
+    // alpha and coverage (bucketprocessor.cpp:690-707)
+    float a = (ch[3] + ch[4] + ch[5]) / 3.0f;
+    ch[6] = a * coverage;
+    ch[8] = coverage;
+    // ExposeBucket (bucketprocessor.cpp:766-806); skipped with empty buckets and trivial exposure
+    const float gain = fr->exposureGain, gamma = fr->exposureGamma;
+    if (count > 0 && !(gain == 1.0f && gamma == 1.0f)) {
+        float oog = 1.0f / gamma;
+        for (int k = 0; k < 3; ++k) {
+            if (gain != 1.0f) ch[k] *= gain;
+            if (gamma != 1.0f) ch[k] = (float)pow((double)ch[k], (double)oog);
+        }
+    }
+    float* out = channels + (size_t)t * 9;
+    for (int k = 0; k < 9; ++k) out[k] = ch[k];
+    // FormatBucketForDisplay (ddmanager.cpp:1045-1118): rgba, 8-bit
+    const double s = (double)dither[t];
+    const float vals[4] = {ch[0], ch[1], ch[2], ch[6]};
+    unsigned char q[4];
+    for (int k = 0; k < 4; ++k) {
+        double value = vals[k];
+        if (fr->qOne != 0.0f) {
+            double xq = 0.0f + value * (double)(fr->qOne - 0.0f) + ((double)fr->qDither * s);
+            value = (double)(lfloor_d(xq - 0.5) + 1);
+            value = value < (double)fr->qMin ? (double)fr->qMin : (value > (double)fr->qMax ? (double)fr->qMax : value);
+        }
+        q[k] = (unsigned char)value;
+    }
+    rgba8[t] = make_uchar4(q[0], q[1], q[2], q[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+extern "C" hipError_t rh_launch_dice(hipStream_t st, const FrameDev* fr, const GridDev* grids, int ngrids,
+                                     const float* params, float4* vtx, float* mprec, float4* mpbound) {
+    if (ngrids <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rh_dice_bust, dim3(ngrids), dim3(256), 0, st, fr, grids, params, vtx, mprec, mpbound);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rh_launch_bin(hipStream_t st, const FrameDev* fr, const BucketDev* buckets, int nbuckets,
+                                    const GridDev* grids, const uint32_t* candStart, const uint32_t* candGrid,
+                                    const unsigned long long* capStart, const float4* mpbound, uint2* lists,
+                                    uint32_t* listCount) {
+    hipLaunchKernelGGL(rh_bin, dim3(nbuckets), dim3(256), 0, st, fr, buckets, grids, candStart, candGrid, capStart,
+                       mpbound, lists, listCount);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rh_launch_sample(hipStream_t st, const FrameDev* fr, const BucketDev* buckets, const int4* blocks,
+                                       int nblocks, const unsigned long long* capStart, const uint32_t* listCount,
+                                       const uint2* lists, const float* mprec, const GridDev* grids,
+                                       const PrimDev* prims, const float* pos2d, const float* pos1d,
+                                       const uchar4* pixPattern, float4* samples, unsigned long long* counters) {
+    int wg = (nblocks + 3) / 4;
+    hipLaunchKernelGGL(rh_sample, dim3(wg), dim3(256), 0, st, fr, buckets, blocks, nblocks, capStart, listCount, lists,
+                       mprec, grids, prims, pos2d, pos1d, pixPattern, samples, counters);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rh_launch_filter(hipStream_t st, const FrameDev* fr, int npix, const float4* samples,
+                                       const uchar4* pixPattern, const float* pos2d, const float* filterTab,
+                                       const float* dither, float* channels, uchar4* rgba8) {
+    hipLaunchKernelGGL(rh_filter, dim3((npix + 255) / 256), dim3(256), 0, st, fr, samples, pixPattern, pos2d,
+                       filterTab, dither, channels, rgba8);
+    return hipGetLastError();
+}

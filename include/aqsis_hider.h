@@ -1,0 +1,219 @@
+/*
+ * aqsis_hider.h -- C ABI of the MI355X-native REYES hider.
+ *
+ * The hider takes camera-space surfaces and the hot Ri options, and renders
+ * one frame: dice -> bust -> sample -> combine -> filter -> expose ->
+ * quantise.  It sits where aqsis's bucket loop sits, so a host renderer that
+ * today calls
+ *
+ *     CqImageBuffer::SetImage()            (libs/core/imagebuffer.cpp:174-228)
+ *     CqImageBuffer::PostSurface(surface)  (libs/core/imagebuffer.cpp:353-446)
+ *     CqImageBuffer::RenderImage()         (libs/core/imagebuffer.cpp:605-772)
+ *     CqDDManager::DisplayBucket(...)      (libs/core/ddmanager/ddmanager.cpp:146-171)
+ *
+ * calls rh_create / rh_post_surfaces / rh_render / rh_read_* instead.
+ *
+ * Conventions: every entry point returns an rh_status (0 = success); the
+ * message of the last failure on the calling thread is rh_last_error().
+ * No exception crosses the ABI.  Host buffers are owned by the caller;
+ * device memory and HIP streams are owned by the context.  A context is
+ * used by one host thread at a time.
+ */
+#ifndef AQSIS_HIDER_H
+#define AQSIS_HIDER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t rh_status;
+enum {
+    RH_OK = 0,
+    RH_ERR_INVALID = 1,     /* bad argument or option value             */
+    RH_ERR_DEVICE = 2,      /* HIP runtime error                        */
+    RH_ERR_NOMEM = 3,       /* host or device allocation failed         */
+    RH_ERR_STATE = 4,       /* call out of order (e.g. read before render) */
+    RH_ERR_CAPACITY = 5     /* request exceeds a fixed capacity         */
+};
+
+/* Pixel filters, the RiPixelFilter functions of libs/core/filters.cpp:72-342. */
+enum {
+    RH_FILTER_BOX = 0,
+    RH_FILTER_TRIANGLE = 1,
+    RH_FILTER_CATMULLROM = 2,
+    RH_FILTER_GAUSSIAN = 3,
+    RH_FILTER_SINC = 4,
+    RH_FILTER_DISK = 5,
+    RH_FILTER_BESSEL = 6,
+    RH_FILTER_MITCHELL = 7
+};
+
+/* Surface types accepted by rh_post_surfaces. */
+enum {
+    RH_PRIM_SPHERE = 1,     /* RiSphere, libs/core/geometry/quadrics.cpp         */
+    RH_PRIM_BICUBIC = 2,    /* RiPatch "bicubic" (Bezier basis), patch.cpp        */
+    RH_PRIM_BILINEAR = 3    /* RiPatch "bilinear", patch.cpp                       */
+};
+
+/*
+ * The hot options.  Field meanings and defaults follow RiFormat,
+ * RiCropWindow, RiProjection, RiClipping, RiPixelSamples, RiPixelFilter,
+ * RiExposure, RiQuantize "rgba", RiShutter and Option "limits"
+ * (libs/core/api/ri.cpp:730-930, libs/core/options.cpp:273-310,
+ * libs/core/optioncache.cpp:48-117).  rh_options_default() fills them in.
+ */
+typedef struct rh_options {
+    int32_t xres, yres;          /* Format                                   */
+    float pixel_aspect;          /* Format pixelaspectratio (1)              */
+    float crop[4];               /* CropWindow xmin xmax ymin ymax (0 1 0 1) */
+    float fov;                   /* Projection "perspective" "fov" (90)      */
+    float clip_near, clip_far;   /* Clipping (FLT_EPSILON, FLT_MAX)          */
+    int32_t xsamples, ysamples;  /* PixelSamples (2 2)                       */
+    int32_t filter;              /* PixelFilter function (gaussian)          */
+    float filter_xwidth, filter_ywidth; /* PixelFilter widths (2 2)          */
+    float exposure_gain, exposure_gamma;  /* Exposure (1 1)                  */
+    float quantize_one, quantize_min, quantize_max, quantize_dither; /* 255 0 255 0.5 */
+    float shutter_open, shutter_close;    /* Shutter (0 0)                   */
+    int32_t bucket_x, bucket_y;  /* Option "limits" "bucketsize" (16 16)     */
+    int32_t eyesplits;           /* Option "limits" "eyesplits" (10)         */
+    int32_t jitter;              /* Hider "hidden" "jitter" (1)              */
+    int32_t num_displays;        /* display requests that consume dither draws (1) */
+    float zthreshold[3];         /* Option "limits" "zthreshold" (1 1 1)     */
+} rh_options;
+
+/*
+ * One surface, already in camera space (the state in which
+ * CqImageBuffer::PostSurface receives it).  Attributes are the constant
+ * shader's inputs (Color, Opacity; shaders/surface/constant.sl) and
+ * ShadingRate.
+ *
+ * Sphere:   sphere[] = radius, zmin, zmax, thetamax (degrees); xform[0] is
+ *           the object->camera matrix, element [row][col] at 4*row+col with
+ *           RenderMan's row-vector convention (translation in row 3).
+ * Bicubic:  P[0..15] are the Bezier control points, row-major in (v,u).
+ * Bilinear: P[0..3] are the corners (u,v) = (0,0) (1,0) (0,1) (1,1).
+ * nkeys = 2 adds a second motion key: patches carry P in object space and
+ * xform[k] maps object->camera at key_time[k]; spheres use xform[k].
+ */
+typedef struct rh_prim {
+    int32_t type;
+    int32_t nkeys;
+    float shading_rate;
+    float Cs[3];
+    float Os[3];
+    float key_time[2];
+    float sphere[4];
+    float xform[2][16];
+    float P[16][3];
+} rh_prim;
+
+/* Counters of one frame.  spl_* are the reference's own hider counters
+ * (libs/core/stats.h:380-382, incremented at bucketprocessor.cpp:1172,
+ * 1192, 1482). */
+typedef struct rh_stats {
+    uint64_t spl_count;        /* (micropolygon, candidate sample) pairs tested */
+    uint64_t spl_bound_hits;   /* pairs passing the bound + occlusion tests     */
+    uint64_t spl_hits;         /* stored hits                                   */
+    uint64_t inside;           /* pairs passing the point-in-micropolygon test  */
+    uint64_t inside_nonlinear; /* of those, hits needing a second Newton step   */
+    uint64_t grids;            /* diced grids                                   */
+    uint64_t micropolygons;    /* busted micropolygons                          */
+    uint64_t mp_bucket_refs;   /* (micropolygon, bucket) list entries           */
+    uint64_t surfaces;         /* surfaces posted, including split children     */
+    uint64_t occlusion_culled; /* surfaces rejected by the occlusion test       */
+    double ms_schedule;        /* host: split/dice scheduling                   */
+    double ms_upload;          /* host->device copies of the frame inputs       */
+    double ms_dice;            /* device: dice + bust                           */
+    double ms_bin;             /* device: bucket micropolygon lists             */
+    double ms_sample;          /* device: hit test + depth + composite          */
+    double ms_filter;          /* device: filter + expose + quantise            */
+    double ms_device;          /* device: whole frame, event to event           */
+} rh_stats;
+
+/* Work and device memory one frame of a scene needs, from the host
+ * split/dice schedule alone. */
+typedef struct rh_footprint {
+    uint64_t surfaces;         /* surfaces posted, including split children     */
+    uint64_t grids;            /* grids the schedule dices                      */
+    uint64_t micropolygons;
+    uint64_t vertices;
+    uint64_t list_capacity;    /* bucket micropolygon-list entries reserved     */
+    uint64_t device_bytes;     /* device memory of the frame (shape + schedule) */
+} rh_footprint;
+
+typedef struct rh_ctx rh_ctx;
+
+/* Fill *opt with the reference defaults listed above. */
+void rh_options_default(rh_options* opt);
+
+/* Create a context for one frame shape.  ~ SqOptionCache::cacheOptions +
+ * CqImageBuffer::SetImage + CqOptions::InitialiseCamera. */
+rh_status rh_create(const rh_options* opt, rh_ctx** out);
+
+/* Append n camera-space surfaces, in posting order.
+ * ~ CqImageBuffer::PostSurface, called once per surface. */
+rh_status rh_post_surfaces(rh_ctx* ctx, const rh_prim* prims, int32_t n);
+
+/* Drop all posted surfaces (the frame shape and its sample plan stay). */
+rh_status rh_clear_surfaces(rh_ctx* ctx);
+
+/* Render one frame of the posted surfaces.  ~ CqImageBuffer::RenderImage.
+ * The host split/dice schedule is recomputed only when surfaces changed. */
+rh_status rh_render(rh_ctx* ctx);
+
+/* Run only the device part of the last frame again, on the resident
+ * grids and sample plan (dice -> ... -> quantise), for timing. */
+rh_status rh_render_device(rh_ctx* ctx);
+
+/* Width/height of the display region and of the sampled extended region
+ * (display region grown by the filter's discrete shift on each side). */
+rh_status rh_frame_size(const rh_ctx* ctx, int32_t* width, int32_t* height,
+                        int32_t* ext_width, int32_t* ext_height,
+                        int32_t* samples_per_pixel);
+
+/* Filtered channels of the display region, 9 floats per pixel in the
+ * order Ci.rgb, Oi.rgb, a, z, coverage (the channel buffer built by
+ * CqBucketProcessor::FilterBucket, bucketprocessor.cpp:381-744). */
+rh_status rh_read_channels(rh_ctx* ctx, float* out);
+
+/* Quantised, dithered 8-bit rgba of the display region (what
+ * CqDisplayRequest::FormatBucketForDisplay hands to DspyImageData,
+ * ddmanager.cpp:1022-1118). */
+rh_status rh_read_rgba8(rh_ctx* ctx, uint8_t* out);
+
+/* Per-sample visibility of the extended region, sample s of pixel
+ * (x, y) at index (y*ext_width + x)*samples_per_pixel + s:
+ *   mp_id[]  id of the front-most micropolygon hit (0xffffffff: none);
+ *            ids count busted micropolygons in dicing order;
+ *   depth[]  camera-space depth of that hit (FLT_MAX: none);
+ *   rgbaz[]  7 floats per sample after CqImagePixel::Combine
+ *            (imagepixel.cpp:144-332): Ci.rgb, Oi.rgb, z.
+ * Any pointer may be NULL. */
+rh_status rh_read_samples(rh_ctx* ctx, uint32_t* mp_id, float* depth,
+                          float* rgbaz);
+
+rh_status rh_get_stats(const rh_ctx* ctx, rh_stats* out);
+
+/* Host only, no device needed: the split/dice schedule of a scene and the
+ * memory one frame of it occupies (used for admission by footprint). */
+rh_status rh_estimate(const rh_options* opt, const rh_prim* prims, int32_t n,
+                      rh_footprint* out);
+
+/* Host only: the frame's jittered sample positions, 2 floats per sample in
+ * the rh_read_samples layout, and display 0's dither value per display
+ * pixel (either pointer may be NULL).  This is the sample pattern that
+ * CqImagePixel::setSamples builds (imagepixel.cpp:334-359) and that
+ * aqsis's mpdump writes with dumpPixelSamples (mpdump.cpp). */
+rh_status rh_sample_positions(const rh_options* opt, float* xy, float* dither);
+
+void rh_destroy(rh_ctx* ctx);
+
+const char* rh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AQSIS_HIDER_H */

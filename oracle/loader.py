@@ -1,0 +1,85 @@
+"""ORACLE loader -- test infrastructure only.
+
+Loads oracle/_build/liboracle.so (built by `make -C oracle`) and wraps its C
+API.  Only tests/, the smoke check and bench.py's CPU-baseline leg use this
+module; the product never does.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from aqsis_amd.abi import RhOptions, RhPrim, RhStats
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.oracle_frame_size.argtypes = [C.POINTER(RhOptions)] + [C.POINTER(C.c_int32)] * 5
+        L.oracle_render.argtypes = [C.POINTER(RhOptions), C.POINTER(RhPrim), C.c_int32, C.c_int32,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.POINTER(RhStats)]
+        L.oracle_mt_outputs.argtypes = [C.c_uint32, C.c_int32, C.c_void_p]
+        L.oracle_random_floats.argtypes = [C.c_uint32, C.c_int32, C.c_void_p]
+        L.oracle_round_f.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_inv_bilinear.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+        L.oracle_bilerp2.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+        L.oracle_tree_index.argtypes = [C.c_int32] * 4
+        L.oracle_tree_index.restype = C.c_int32
+        L.oracle_filter.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.oracle_filter.restype = C.c_float
+        L.oracle_jitter_tables.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_mp_sample.argtypes = [C.c_void_p, C.c_float, C.c_float, C.POINTER(C.c_float)]
+        L.oracle_mp_sample.restype = C.c_int32
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def frame_size(opts):
+    v = [C.c_int32() for _ in range(5)]
+    lib().oracle_frame_size(C.byref(opts), *[C.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
+def render(opts, prims, occlusion_cull=False, samples=True):
+    """Render on the CPU; returns a dict of numpy arrays and the stats."""
+    from aqsis_amd.scenes import prim_array
+    w, h, ew, eh, spp = frame_size(opts)
+    arr = prim_array(prims)
+    ch = np.zeros((h, w, 9), np.float32)
+    rgba = np.zeros((h, w, 4), np.uint8)
+    ns = ew * eh * spp
+    ids = np.zeros(ns, np.uint32) if samples else None
+    depth = np.zeros(ns, np.float32) if samples else None
+    rgbaz = np.zeros((ns, 7), np.float32) if samples else None
+    st = RhStats()
+    lib().oracle_render(C.byref(opts), arr, len(prims), 1 if occlusion_cull else 0,
+                        _ptr(ch), _ptr(rgba), _ptr(ids), _ptr(depth), _ptr(rgbaz), C.byref(st))
+    return {"channels": ch, "rgba8": rgba, "mp_id": ids, "depth": depth, "rgbaz": rgbaz,
+            "stats": st.as_dict(), "ext": (ew, eh, spp)}
+
+
+def sample_positions(opts):
+    w, h, ew, eh, spp = frame_size(opts)
+    L = lib()
+    L.oracle_sample_positions.argtypes = [C.POINTER(RhOptions), C.c_void_p, C.c_void_p]
+    xy = np.zeros((eh, ew, spp, 2), np.float32)
+    d = np.zeros((h, w), np.float32)
+    L.oracle_sample_positions(C.byref(opts), _ptr(xy), _ptr(d))
+    return xy, d
