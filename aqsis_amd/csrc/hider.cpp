@@ -18,7 +18,7 @@ extern "C" hipError_t rh_launch_dice(hipStream_t, const FrameDev*, const GridDev
 extern "C" hipError_t rh_launch_bin(hipStream_t, const FrameDev*, const BucketDev*, int, const GridDev*,
                                     const uint32_t*, const uint32_t*, const unsigned long long*, const float4*, uint2*,
                                     uint32_t*);
-extern "C" hipError_t rh_launch_sample(hipStream_t, const FrameDev*, const BucketDev*, const int4*, int,
+extern "C" hipError_t rh_launch_sample(hipStream_t, const FrameDev*, const BucketDev*, int,
                                        const unsigned long long*, const uint32_t*, const uint2*, const float*,
                                        const GridDev*, const PrimDev*, const float*, const float*, const uchar4*,
                                        float4*, unsigned long long*);
@@ -281,7 +281,7 @@ rh_status rh_render_device(rh_ctx* c) {
                           c->dCapStart.as<unsigned long long>(), c->dMpbound.as<float4>(), c->dLists.as<uint2>(),
                           c->dListCount.as<uint32_t>()));
     HIP_TRY(hipEventRecord(c->ev[2], c->st));
-    HIP_TRY(rh_launch_sample(c->st, f, c->dBuckets.as<BucketDev>(), c->dBlocks.as<int4>(), c->nblocks,
+    HIP_TRY(rh_launch_sample(c->st, f, c->dBuckets.as<BucketDev>(), nb,
                              c->dCapStart.as<unsigned long long>(), c->dListCount.as<uint32_t>(),
                              c->dLists.as<uint2>(), c->dMprec.as<float>(), c->dGrids.as<GridDev>(),
                              c->dPrims.as<PrimDev>(), c->dPos2d.as<float>(), c->dPos1d.as<float>(),

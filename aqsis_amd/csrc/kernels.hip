@@ -342,151 +342,228 @@ __global__ void __launch_bounds__(256) rh_bin(const FrameDev* __restrict__ fr, c
 // ---------------------------------------------------------------------------
 // Sampling: per-sample visibility in the reference's micropolygon order
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets,
-                                                 const int4* __restrict__ blocks, int nblocks,
-                                                 const unsigned long long* __restrict__ capStart,
-                                                 const uint32_t* __restrict__ listCount, const uint2* __restrict__ lists,
-                                                 const float* __restrict__ mprec, const GridDev* __restrict__ grids,
-                                                 const PrimDev* __restrict__ prims, const float* __restrict__ pos2d,
-                                                 const float* __restrict__ pos1d, const uchar4* __restrict__ pixPattern,
-                                                 float4* __restrict__ samples, unsigned long long* __restrict__ counters) {
-    const int lane = threadIdx.x & 63;
-    const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (blk >= nblocks) return;
-    const int4 bl = blocks[blk];  // bucket, x0, y0, -
-    const BucketDev B = buckets[bl.x];
-    const int ns = fr->ns, nx = fr->nx, ny = fr->ny, bw = fr->blockW, bh = fr->blockH;
-    const int pix = lane / ns, si = lane - pix * ns;
-    const int px = bl.y + pix % bw, py = bl.z + pix / bw;
-    const bool live = pix < bw * bh && px < B.sx1 && py < B.sy1;
-    const int sm = si % nx, sn = si / nx;
+// One 1024-thread workgroup per bucket.  The bucket's ordered micropolygon
+// list is staged through LDS in chunks of RH_SB_CHUNK hit-test records.
+// Each wave owns up to RH_SB_SLOTS blocks of 64 samples (lane = one sample).
+// For every chunk and block, the wave compacts the micropolygons whose
+// candidate rectangle meets the block, each lane builds a bit mask of the
+// ones whose candidate rectangle holds its own sample, and lanes then pop
+// their own candidates in list order.  The candidate rectangle is the exact
+// set of samples RenderMPG_Static visits (bucketprocessor.cpp:1120-1216):
+// the pixel range [floor(bminx), lceil(bmaxx)) clipped to the sample region
+// with the first/last pixel sub-sample clip, expressed in sub-sample grid
+// units, so each sample sees exactly the reference's sequence of tests.
+#define RH_SB_CHUNK 256
+#define RH_SB_WAVES 16
+#define RH_SB_SLOTS 4
 
-    float sx = 0.f, sy = 0.f, lod = 0.f;
-    size_t sidx = 0;
-    if (live) {
-        const int ex = px - fr->extX0, ey = py - fr->extY0;
-        const uchar4 pat = pixPattern[(size_t)ey * fr->extW + ex];
-        // position = (x, y) + pattern position (imagepixel.cpp:353)
-        sx = (float)px + pos2d[2 * ((size_t)pat.x * ns + si)];
-        sy = (float)py + pos2d[2 * ((size_t)pat.x * ns + si) + 1];
-        lod = pos1d[(size_t)pat.z * ns + si];
-        sidx = ((size_t)ey * fr->extW + ex) * ns + si;
-    }
-    float occlZ = FLT_MAX, wu = 0.f, wv = 0.f;
-    uint32_t win = 0xffffffffu;
+__global__ void __launch_bounds__(1024) rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets,
+                                                  const unsigned long long* __restrict__ capStart,
+                                                  const uint32_t* __restrict__ listCount, const uint2* __restrict__ lists,
+                                                  const float4* __restrict__ mprec4, const GridDev* __restrict__ grids,
+                                                  const PrimDev* __restrict__ prims, const float* __restrict__ pos2d,
+                                                  const float* __restrict__ pos1d, const uchar4* __restrict__ pixPattern,
+                                                  float4* __restrict__ samples, unsigned long long* __restrict__ counters) {
+    __shared__ float4 recs[RH_SB_CHUNK * 9];
+    __shared__ uint2 meta[RH_SB_CHUNK];
+    __shared__ uint32_t ids[RH_SB_CHUNK];
+    __shared__ uint16_t matched[RH_SB_WAVES][RH_SB_CHUNK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const BucketDev B = buckets[blockIdx.x];
+    const int ns = fr->ns, nx = fr->nx, ny = fr->ny, bw = fr->blockW, bh = fr->blockH;
+    const int ow = B.sx1 - B.sx0, oh = B.sy1 - B.sy0;
+    const int nbx = (ow + bw - 1) / bw, nby = (oh + bh - 1) / bh, nblk = nbx * nby;
+    const int pix = lane / ns, si = lane - pix * ns;
+    const int lx = pix % bw, ly = pix / bw;
+    const int sm = si % nx, sn = si / nx;
+    const uint2* L = lists + capStart[blockIdx.x];
+    const uint32_t n = listCount[blockIdx.x];
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
 
-    const int bx0 = bl.y - B.sx0, by0 = bl.z - B.sy0;   // block rect relative to the sample region
-    const int bx1 = bx0 + bw, by1 = by0 + bh;
-    const uint2* L = lists + capStart[bl.x];
-    const uint32_t n = listCount[bl.x];
-    for (uint32_t c = 0; c < n; c += 64) {
-        uint2 e = make_uint2(0u, 0u);
-        bool meets = false;
-        if (c + lane < n) {
-            e = L[c + lane];
-            int wx0 = e.y & 255, wx1 = (e.y >> 8) & 255, wy0 = (e.y >> 16) & 255, wy1 = e.y >> 24;
-            meets = wx0 < bx1 && wx1 > bx0 && wy0 < by1 && wy1 > by0;
-        }
-        unsigned long long mask = __ballot(meets);
-        while (mask) {
-            const int j = __ffsll((long long)mask) - 1;
-            mask &= mask - 1ull;
-            const uint32_t id = __builtin_amdgcn_readlane(e.x, j);
-            const float4* R = reinterpret_cast<const float4*>(mprec + (size_t)id * RH_MPREC_FLOATS);
-            const float4 bnd = R[7], r8 = R[8];
-            const float bminx = bnd.x, bmaxx = bnd.y, bminy = bnd.z, bmaxy = bnd.w, bminz = r8.x;
-            // Pixel range and subpixel clip of RenderMPG_Static (bucketprocessor.cpp:1120-1140)
-            int eX = (int)lceil_f(bmaxx), eY = (int)lceil_f(bmaxy);
-            if (eX > B.sx1) eX = B.sx1;
-            if (eY > B.sy1) eY = B.sy1;
-            int sX = floor_i(bminx), sY = floor_i(bminy);
-            if (sY < B.sy0) sY = B.sy0;
-            if (sX < B.sx0) sX = B.sx0;
-            const int im = (bminx < sX) ? 0 : floor_i((bminx - sX) * nx);
-            const int in = (bminy < sY) ? 0 : floor_i((bminy - sY) * ny);
-            const int em = (bmaxx > eX) ? nx : (int)lceil_f((bmaxx - (eX - 1)) * nx);
-            const int en = (bmaxy > eY) ? ny : (int)lceil_f((bmaxy - (eY - 1)) * ny);
-            if (!live || px < sX || px >= eX || py < sY || py >= eY) continue;
-            const int n0 = (py == sY) ? in : 0, n1 = (py == eY - 1) ? en : ny;
-            const int m0 = (px == sX) ? im : 0, m1 = (px == eX - 1) ? em : nx;
-            if (sn < n0 || sn >= n1 || sm < m0 || sm >= m1) continue;
-            ++nTest;
-            // CqBound::Contains2D (bound.h:144-151)
-            if (sx < bminx || sx > bmaxx || sy < bminy || sy > bmaxy) continue;
-            // occlusion skip against the current opaque hit (bucketprocessor.cpp:1179)
-            if (bminz > occlZ) continue;
-            // LevelOfDetailBounds [0, 1] (bucketprocessor.cpp:1183-1189)
-            if (0.0f > lod || lod >= 1.0f) continue;
-            ++nBound;
-            // fContains (micropolygon.cpp:1295-1342): edges 0,1 reject on <= 0, edges 2,3 on < 0
-            const float4 YM = R[0], XM = R[1], X = R[2], Y = R[3];
-            float t0 = ((sy - Y.x) * YM.x) - ((sx - X.x) * XM.x);
-            float t1 = ((sy - Y.y) * YM.y) - ((sx - X.y) * XM.y);
-            float t2 = ((sy - Y.z) * YM.z) - ((sx - X.z) * XM.z);
-            float t3 = ((sy - Y.w) * YM.w) - ((sx - X.w) * XM.w);
-            if (t0 <= 0 || t1 <= 0 || t2 < 0 || t3 < 0) continue;
-            ++nIn;
-            // CqInvBilinear::operator() (bilinear.h:277-305)
-            const float4 r5 = R[5], r6 = R[6], Z = R[4];
-            const float Ax = r5.x, Ay = r5.y, Ex = r5.z, Ey = r5.w, Fx = r6.x, Fy = r6.y, Gx = r6.z, Gy = r6.w;
-            float u = 0.5f, v = 0.5f;
-            {
-                float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
-                float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
-                float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - sx;
-                float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - sy;
-                float det = M1x * M2y - M1y * M2x;
-                det = 1 / det;
-                float dx = det * (bx * M2y - by * M2x);
-                float dy = det * (-(bx * M1y - by * M1x));
-                u = u - dx;
-                v = v - dy;
+    for (int p0 = 0; p0 < nblk; p0 += RH_SB_WAVES * RH_SB_SLOTS) {
+        bool has[RH_SB_SLOTS], live[RH_SB_SLOTS];
+        int gx[RH_SB_SLOTS], gy[RH_SB_SLOTS], bgx[RH_SB_SLOTS], bgy[RH_SB_SLOTS];
+        float sx[RH_SB_SLOTS], sy[RH_SB_SLOTS], lod[RH_SB_SLOTS];
+        float occlZ[RH_SB_SLOTS], wu[RH_SB_SLOTS], wv[RH_SB_SLOTS];
+        uint32_t win[RH_SB_SLOTS];
+#pragma unroll
+        for (int k = 0; k < RH_SB_SLOTS; ++k) {
+            const int blk = p0 + wave + RH_SB_WAVES * k;
+            has[k] = blk < nblk;
+            const int bx = B.sx0 + (blk % nbx) * bw, by = B.sy0 + (blk / nbx) * bh;
+            const int px = bx + lx, py = by + ly;
+            live[k] = has[k] && pix < bw * bh && px < B.sx1 && py < B.sy1;
+            gx[k] = (px - B.sx0) * nx + sm;
+            gy[k] = (py - B.sy0) * ny + sn;
+            bgx[k] = (bx - B.sx0) * nx;
+            bgy[k] = (by - B.sy0) * ny;
+            occlZ[k] = FLT_MAX;
+            wu[k] = 0.f;
+            wv[k] = 0.f;
+            win[k] = 0xffffffffu;
+            sx[k] = 0.f;
+            sy[k] = 0.f;
+            lod[k] = 0.f;
+            if (live[k]) {
+                const uchar4 pat = pixPattern[(size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)];
+                // position = (x, y) + pattern position (imagepixel.cpp:353)
+                sx[k] = (float)px + pos2d[2 * ((size_t)pat.x * ns + si)];
+                sy[k] = (float)py + pos2d[2 * ((size_t)pat.x * ns + si) + 1];
+                lod[k] = pos1d[(size_t)pat.z * ns + si];
             }
-            if (__float_as_int(r8.y) == 0) {
-                ++nIn2;
-                float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
-                float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
-                float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - sx;
-                float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - sy;
-                float det = M1x * M2y - M1y * M2x;
-                if (det != 0) det = 1 / det;
-                float dx = det * (bx * M2y - by * M2x);
-                float dy = det * (-(bx * M1y - by * M1x));
-                u = u - dx;
-                v = v - dy;
+        }
+        for (uint32_t c0 = 0; c0 < n; c0 += RH_SB_CHUNK) {
+            const int cnt = (int)((n - c0) < (uint32_t)RH_SB_CHUNK ? (n - c0) : (uint32_t)RH_SB_CHUNK);
+            __syncthreads();
+            for (int t = tid; t < cnt * 9; t += blockDim.x) {
+                const int r = t / 9, q = t - r * 9;
+                const uint32_t id = L[c0 + r].x;
+                recs[t] = mprec4[(size_t)id * 9 + q];
+                if (q == 0) ids[r] = id;
             }
-            // bilerp (bilinear.h:228-236)
-            float w0 = (1 - v) * (1 - u), w1 = (1 - v) * u, w2 = v * (1 - u), w3 = v * u;
-            float D = ((w0 * Z.x + w1 * Z.y) + w2 * Z.z) + w3 * Z.w;
-            // StoreSample (bucketprocessor.cpp:1471-1569): reject when occlZ <= D
-            if (occlZ <= D) continue;
-            ++nHit;
-            occlZ = D;
-            win = id;
-            wu = u;
-            wv = v;
+            __syncthreads();
+            for (int t = tid; t < cnt; t += blockDim.x) {
+                const float4 bd = recs[t * 9 + 7];
+                const float bminx = bd.x, bmaxx = bd.y, bminy = bd.z, bmaxy = bd.w;
+                int eX = (int)lceil_f(bmaxx), eY = (int)lceil_f(bmaxy);
+                if (eX > B.sx1) eX = B.sx1;
+                if (eY > B.sy1) eY = B.sy1;
+                int sX = floor_i(bminx), sY = floor_i(bminy);
+                if (sY < B.sy0) sY = B.sy0;
+                if (sX < B.sx0) sX = B.sx0;
+                uint32_t gx0 = 0, gx1 = 0, gy0 = 0, gy1 = 0;
+                if (sX < eX && sY < eY) {
+                    const int im = (bminx < sX) ? 0 : floor_i((bminx - sX) * nx);
+                    const int in = (bminy < sY) ? 0 : floor_i((bminy - sY) * ny);
+                    const int em = (bmaxx > eX) ? nx : (int)lceil_f((bmaxx - (eX - 1)) * nx);
+                    const int en = (bmaxy > eY) ? ny : (int)lceil_f((bmaxy - (eY - 1)) * ny);
+                    gx0 = (uint32_t)((sX - B.sx0) * nx + im);
+                    gx1 = (uint32_t)((eX - 1 - B.sx0) * nx + em);
+                    gy0 = (uint32_t)((sY - B.sy0) * ny + in);
+                    gy1 = (uint32_t)((eY - 1 - B.sy0) * ny + en);
+                }
+                meta[t] = make_uint2(gx0 | (gx1 << 16), gy0 | (gy1 << 16));
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < RH_SB_SLOTS; ++k) {
+                if (!has[k]) continue;  // uniform across the wave
+                const int BX0 = bgx[k], BX1 = bgx[k] + bw * nx, BY0 = bgy[k], BY1 = bgy[k] + bh * ny;
+                // Ordered list of the chunk's micropolygons that meet this block.
+                int nm = 0;
+                for (int j0 = 0; j0 < cnt; j0 += 64) {
+                    const int j = j0 + lane;
+                    bool ok = false;
+                    if (j < cnt) {
+                        const uint2 mt = meta[j];
+                        ok = (int)(mt.x & 0xffffu) < BX1 && (int)(mt.x >> 16) > BX0 && (int)(mt.y & 0xffffu) < BY1 &&
+                             (int)(mt.y >> 16) > BY0;
+                    }
+                    const unsigned long long msk = __ballot(ok);
+                    if (ok) matched[wave][nm + __popcll(msk & ((1ull << lane) - 1ull))] = (uint16_t)j;
+                    nm += __popcll(msk);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (int i0 = 0; i0 < nm; i0 += 64) {
+                    const int ni = (nm - i0) < 64 ? (nm - i0) : 64;
+                    unsigned long long cm = 0ull;
+                    for (int i = 0; i < ni; ++i) {
+                        const uint2 mt = meta[matched[wave][i0 + i]];
+                        const bool cand = (uint32_t)gx[k] >= (mt.x & 0xffffu) && (uint32_t)gx[k] < (mt.x >> 16) &&
+                                          (uint32_t)gy[k] >= (mt.y & 0xffffu) && (uint32_t)gy[k] < (mt.y >> 16);
+                        cm |= (unsigned long long)(cand && live[k]) << i;
+                    }
+                    nTest += (uint32_t)__popcll(cm);
+                    while (__ballot(cm != 0ull)) {
+                        if (cm == 0ull) continue;
+                        const int i = __ffsll((long long)cm) - 1;
+                        cm &= cm - 1ull;
+                        const int j = matched[wave][i0 + i];
+                        const float4* R = &recs[j * 9];
+                        const float4 bnd = R[7], r8 = R[8];
+                        // CqBound::Contains2D (bound.h:144-151)
+                        if (sx[k] < bnd.x || sx[k] > bnd.y || sy[k] < bnd.z || sy[k] > bnd.w) continue;
+                        // occlusion skip against the current opaque hit (bucketprocessor.cpp:1179)
+                        if (r8.x > occlZ[k]) continue;
+                        // LevelOfDetailBounds [0, 1] (bucketprocessor.cpp:1183-1189)
+                        if (0.0f > lod[k] || lod[k] >= 1.0f) continue;
+                        ++nBound;
+                        // fContains (micropolygon.cpp:1295-1342): edges 0,1 reject on <= 0, edges 2,3 on < 0
+                        const float4 YM = R[0], XM = R[1], X = R[2], Y = R[3];
+                        const float px_ = sx[k], py_ = sy[k];
+                        const float t0 = ((py_ - Y.x) * YM.x) - ((px_ - X.x) * XM.x);
+                        const float t1 = ((py_ - Y.y) * YM.y) - ((px_ - X.y) * XM.y);
+                        const float t2 = ((py_ - Y.z) * YM.z) - ((px_ - X.z) * XM.z);
+                        const float t3 = ((py_ - Y.w) * YM.w) - ((px_ - X.w) * XM.w);
+                        if (t0 <= 0 || t1 <= 0 || t2 < 0 || t3 < 0) continue;
+                        ++nIn;
+                        // CqInvBilinear::operator() (bilinear.h:277-305)
+                        const float4 r5 = R[5], r6 = R[6], Z = R[4];
+                        const float Ax = r5.x, Ay = r5.y, Ex = r5.z, Ey = r5.w, Fx = r6.x, Fy = r6.y, Gx = r6.z,
+                                    Gy = r6.w;
+                        float u = 0.5f, v = 0.5f;
+                        {
+                            const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
+                            const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
+                            const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px_;
+                            const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py_;
+                            float det = M1x * M2y - M1y * M2x;
+                            det = 1 / det;
+                            u = u - det * (bx * M2y - by * M2x);
+                            v = v - det * (-(bx * M1y - by * M1x));
+                        }
+                        if (__float_as_int(r8.y) == 0) {
+                            ++nIn2;
+                            const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
+                            const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
+                            const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px_;
+                            const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py_;
+                            float det = M1x * M2y - M1y * M2x;
+                            if (det != 0) det = 1 / det;
+                            u = u - det * (bx * M2y - by * M2x);
+                            v = v - det * (-(bx * M1y - by * M1x));
+                        }
+                        // bilerp (bilinear.h:228-236)
+                        const float w0 = (1 - v) * (1 - u), w1 = (1 - v) * u, w2 = v * (1 - u), w3 = v * u;
+                        const float D = ((w0 * Z.x + w1 * Z.y) + w2 * Z.z) + w3 * Z.w;
+                        // StoreSample (bucketprocessor.cpp:1471-1569): reject when occlZ <= D
+                        if (occlZ[k] <= D) continue;
+                        ++nHit;
+                        occlZ[k] = D;
+                        win[k] = ids[j];
+                        wu[k] = u;
+                        wv[k] = v;
+                    }
+                }
+            }
         }
-    }
-    if (live) {
-        float4 a = make_float4(0.f, 0.f, 0.f, FLT_MAX), b = make_float4(0.f, 0.f, 0.f, __uint_as_float(win));
-        if (win != 0xffffffffu) {
-            const float* R = mprec + (size_t)win * RH_MPREC_FLOATS;
-            const GridDev G = grids[__float_as_int(R[34])];
-            const PrimDev P = prims[G.prim];
-            // InterpolateOutputs, smooth shading with four equal vertex colours
-            // (micropolygon.cpp:1443-1462)
-            float w0 = (1 - wu) * (1 - wv), w1 = wu * (1 - wv), w2 = (1 - wu) * wv, w3 = wu * wv;
-            a.x = ((w0 * P.Ci[0] + w1 * P.Ci[0]) + w2 * P.Ci[0]) + w3 * P.Ci[0];
-            a.y = ((w0 * P.Ci[1] + w1 * P.Ci[1]) + w2 * P.Ci[1]) + w3 * P.Ci[1];
-            a.z = ((w0 * P.Ci[2] + w1 * P.Ci[2]) + w2 * P.Ci[2]) + w3 * P.Ci[2];
-            a.w = occlZ;
-            b.x = ((w0 * P.Oi[0] + w1 * P.Oi[0]) + w2 * P.Oi[0]) + w3 * P.Oi[0];
-            b.y = ((w0 * P.Oi[1] + w1 * P.Oi[1]) + w2 * P.Oi[1]) + w3 * P.Oi[1];
-            b.z = ((w0 * P.Oi[2] + w1 * P.Oi[2]) + w2 * P.Oi[2]) + w3 * P.Oi[2];
+#pragma unroll
+        for (int k = 0; k < RH_SB_SLOTS; ++k) {
+            if (!live[k]) continue;
+            const int px = B.sx0 + gx[k] / nx, py = B.sy0 + gy[k] / ny;
+            const size_t sidx = ((size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)) * ns + si;
+            float4 a = make_float4(0.f, 0.f, 0.f, FLT_MAX), b = make_float4(0.f, 0.f, 0.f, __uint_as_float(win[k]));
+            if (win[k] != 0xffffffffu) {
+                const float4 r8 = mprec4[(size_t)win[k] * 9 + 8];
+                const GridDev G = grids[__float_as_int(r8.z)];
+                const PrimDev P = prims[G.prim];
+                const float u = wu[k], v = wv[k];
+                // InterpolateOutputs, smooth shading with four equal vertex colours
+                // (micropolygon.cpp:1443-1462)
+                const float w0 = (1 - u) * (1 - v), w1 = u * (1 - v), w2 = (1 - u) * v, w3 = u * v;
+                a.x = ((w0 * P.Ci[0] + w1 * P.Ci[0]) + w2 * P.Ci[0]) + w3 * P.Ci[0];
+                a.y = ((w0 * P.Ci[1] + w1 * P.Ci[1]) + w2 * P.Ci[1]) + w3 * P.Ci[1];
+                a.z = ((w0 * P.Ci[2] + w1 * P.Ci[2]) + w2 * P.Ci[2]) + w3 * P.Ci[2];
+                a.w = occlZ[k];
+                b.x = ((w0 * P.Oi[0] + w1 * P.Oi[0]) + w2 * P.Oi[0]) + w3 * P.Oi[0];
+                b.y = ((w0 * P.Oi[1] + w1 * P.Oi[1]) + w2 * P.Oi[1]) + w3 * P.Oi[1];
+                b.z = ((w0 * P.Oi[2] + w1 * P.Oi[2]) + w2 * P.Oi[2]) + w3 * P.Oi[2];
+            }
+            samples[2 * sidx] = a;
+            samples[2 * sidx + 1] = b;
         }
-        samples[2 * sidx] = a;
-        samples[2 * sidx + 1] = b;
     }
     // Wave-level reduction of the hider counters.
     for (int off = 32; off > 0; off >>= 1) {
@@ -607,14 +684,14 @@ extern "C" hipError_t rh_launch_bin(hipStream_t st, const FrameDev* fr, const Bu
     return hipGetLastError();
 }
 
-extern "C" hipError_t rh_launch_sample(hipStream_t st, const FrameDev* fr, const BucketDev* buckets, const int4* blocks,
-                                       int nblocks, const unsigned long long* capStart, const uint32_t* listCount,
+extern "C" hipError_t rh_launch_sample(hipStream_t st, const FrameDev* fr, const BucketDev* buckets, int nbuckets,
+                                       const unsigned long long* capStart, const uint32_t* listCount,
                                        const uint2* lists, const float* mprec, const GridDev* grids,
                                        const PrimDev* prims, const float* pos2d, const float* pos1d,
                                        const uchar4* pixPattern, float4* samples, unsigned long long* counters) {
-    int wg = (nblocks + 3) / 4;
-    hipLaunchKernelGGL(rh_sample, dim3(wg), dim3(256), 0, st, fr, buckets, blocks, nblocks, capStart, listCount, lists,
-                       mprec, grids, prims, pos2d, pos1d, pixPattern, samples, counters);
+    hipLaunchKernelGGL(rh_sample, dim3(nbuckets), dim3(1024), 0, st, fr, buckets, capStart, listCount, lists,
+                       reinterpret_cast<const float4*>(mprec), grids, prims, pos2d, pos1d, pixPattern, samples,
+                       counters);
     return hipGetLastError();
 }
 
