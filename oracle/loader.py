@@ -83,3 +83,11 @@ def sample_positions(opts):
     d = np.zeros((h, w), np.float32)
     L.oracle_sample_positions(C.byref(opts), _ptr(xy), _ptr(d))
     return xy, d
+
+
+def dice(prim, u, v):
+    L = lib()
+    L.oracle_dice.argtypes = [C.POINTER(RhPrim), C.c_int32, C.c_int32, C.c_void_p]
+    out = np.zeros(((v + 1) * (u + 1), 3), np.float32)
+    L.oracle_dice(C.byref(prim), u, v, _ptr(out))
+    return out

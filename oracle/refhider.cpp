@@ -1473,6 +1473,34 @@ int oracle_render(const rh_options* opt, const rh_prim* prims, int32_t nprims, i
     return 0;
 }
 
+int oracle_dice(const rh_prim* p, int32_t u, int32_t v, float* out) {
+    Surf s;
+    s.type = p->type;
+    if (p->type == RH_PRIM_SPHERE) {
+        s.radius = p->sphere[0];
+        s.thetaMin = 0.0f;
+        s.thetaMax = p->sphere[3];
+        s.phiMin = (float)std::asin((double)(p->sphere[1] / s.radius));
+        s.phiMax = (float)std::asin((double)(p->sphere[2] / s.radius));
+        bool ident = true;
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) {
+                s.matTx.m[r][c] = p->xform[0][4 * r + c];
+                if (s.matTx.m[r][c] != (r == c ? 1.0f : 0.0f)) ident = false;
+            }
+        s.matTx.ident = ident;
+    } else {
+        int n = p->type == RH_PRIM_BICUBIC ? 16 : 4;
+        for (int k = 0; k < n; ++k) s.cp[k] = v3(p->P[k][0], p->P[k][1], p->P[k][2]);
+    }
+    s.uDice = u;
+    s.vDice = v;
+    Grid g;
+    surf_dice(s, g);
+    for (size_t i = 0; i < g.P.size(); ++i) { out[3 * i] = g.P[i].x; out[3 * i + 1] = g.P[i].y; out[3 * i + 2] = g.P[i].z; }
+    return 0;
+}
+
 int oracle_sample_positions(const rh_options* opt, float* xy, float* dither) {
     std::unique_ptr<Hider> H(new Hider());
     memset(&H->st, 0, sizeof(H->st));

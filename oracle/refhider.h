@@ -33,6 +33,10 @@ int oracle_render(const rh_options* opt, const rh_prim* prims, int32_t nprims,
  * dither draw per display pixel. */
 int oracle_sample_positions(const rh_options* opt, float* xy, float* dither);
 
+/* Camera-space grid points of one surface diced at u x v
+ * (CqSurface::Dice): out[3*((v+1)*(u+1))]. */
+int oracle_dice(const rh_prim* prim, int32_t u, int32_t v, float* out);
+
 /* --- Known-answer hooks for the unit tests ----------------------------- */
 /* n outputs of the global stream after Reseed(seed) (random.cpp:102-157). */
 void oracle_mt_outputs(uint32_t seed, int32_t n, uint32_t* out);

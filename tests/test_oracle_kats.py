@@ -1,0 +1,321 @@
+"""Pins the oracle (CPU restatement of the reference hider) before it is used
+to judge the HIP path: the reference's own unit-test known answers
+(libs/math/math_test.cpp, libs/core/bilinear_test.cpp,
+libs/core/occlusion_test.cpp), the C++ standard's MT19937 known answers,
+closed forms and invariants chosen so that a dropped term, a swapped
+operand or a wrong comparison fails."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from aqsis_amd import scenes
+from aqsis_amd.abi import default_options, FILTERS
+
+
+# --------------------------------------------------------------------------
+# Random stream: libs/math/random.cpp is MT19937 with one global state.
+def test_mt19937_known_answers(oracle):
+    out = np.zeros(10000, np.uint32)
+    oracle.lib().oracle_mt_outputs(5489, 10000, out.ctypes.data_as(C.c_void_p))
+    assert out[0] == 3499211612          # first output for the default seed
+    assert out[9999] == 4123659995       # C++ standard [rand.predef]: 10000th output
+
+
+@pytest.mark.parametrize("seed", [545, 19, 1])
+def test_mt19937_matches_independent_generator(oracle, seed):
+    out = np.zeros(2000, np.uint32)
+    oracle.lib().oracle_mt_outputs(seed, 2000, out.ctypes.data_as(C.c_void_p))
+    ref = np.random.RandomState(seed).randint(0, 2 ** 32, size=2000, dtype=np.uint32)
+    assert np.array_equal(out, ref)
+
+
+def test_random_float_closed_form(oracle):
+    raw = np.zeros(5000, np.uint32)
+    f = np.zeros(5000, np.float32)
+    oracle.lib().oracle_mt_outputs(545, 5000, raw.ctypes.data_as(C.c_void_p))
+    oracle.lib().oracle_random_floats(545, 5000, f.ctypes.data_as(C.c_void_p))
+    expect = (raw.astype(np.float64) * (1.0 / 4294967424.0)).astype(np.float32)
+    assert np.array_equal(f, expect)
+    assert f.max() < 1.0 and f.min() >= 0.0
+
+
+# --------------------------------------------------------------------------
+# Rounding helpers: libs/math/math_test.cpp:36-63
+def test_rounding_known_answers(oracle):
+    x = np.array([1.0, -1.0, 1.1, -1.1, 1.2, 1.5, 1.7, -1.2, -1.5, -1.7], np.float32)
+    fl, ce, ro = (np.zeros(len(x), np.int64) for _ in range(3))
+    oracle.lib().oracle_round_f(x.ctypes.data_as(C.c_void_p), len(x), *(a.ctypes.data_as(C.c_void_p) for a in (fl, ce, ro)))
+    assert list(fl[:4]) == [1, -1, 1, -2]
+    assert list(ce[:4]) == [1, -1, 2, -1]
+    # lround: 1.0->1, 1.2->1, 1.5->2, 1.7->2, -1.0->-1, -1.2->-1, -1.5->-1, -1.7->-2
+    assert [ro[0], ro[4], ro[5], ro[6]] == [1, 1, 2, 2]
+    assert [ro[1], ro[7], ro[8], ro[9]] == [-1, -1, -1, -2]
+
+
+# --------------------------------------------------------------------------
+# Inverse bilinear: the 11 cases of libs/core/bilinear_test.cpp:156-261 with
+# the fixture of :104-146 and the tolerances each case uses.
+def _uv_fixture(u_ex=-1.0, v_ex=-1.0):
+    e = 1e-6
+    pts = [(0.5, 0.5), (0.12345, 0.67891), (0.42042, 0.42042), (0.3141592, 0.2718281),
+           (0, 0), (1, 0), (0, 1), (1, 1), (e, e), (1 - e, e), (e, 1 - e), (1 - e, 1 - e),
+           (0.5, 0), (0.5, 1), (0, 0.5), (1, 0.5), (0.5, e), (0.5, 1 - e), (e, 0.5), (1 - e, 0.5)]
+    out = [(u, v) for u, v in pts if (u_ex == -1 or abs(u_ex - u) > 1e-2) and (v_ex == -1 or abs(v_ex - v) > 1e-2)]
+    return np.array(out, np.float32)
+
+
+_OFF = (1000, 2000)
+INV_BILINEAR_CASES = [
+    ("convex_irregular", [(0.1, 0.1), (1.1, 0), (-0.1, 1.5), (1, 1)], (-1, -1), 1e-3, 1e-3),
+    ("convex_irregular_rot90", [(-0.1, 1.5), (0.1, 0.1), (1, 1), (1.1, 0)], (-1, -1), 1e-3, 1e-3),
+    ("convex_irregular_rot180", [(1, 1), (-0.1, 1.5), (1.1, 0), (0.1, 0.1)], (-1, -1), 1e-3, 1e-3),
+    ("convex_irregular_rot270", [(1.1, 0), (1, 1), (0.1, 0.1), (-0.1, 1.5)], (-1, -1), 1e-3, 1e-3),
+    ("large_offset", [(0 + _OFF[0], 0 + _OFF[1]), (2 + _OFF[0], 0 + _OFF[1]), (0 + _OFF[0], 1 + _OFF[1]),
+                      (2 + _OFF[0], 1 + _OFF[1])], (-1, -1), 1e-3, 1e-3),
+    ("exactly_rectangular", [(0, 0), (2, 0), (0, 1), (2, 1)], (-1, -1), 1e-3, 1e-3),
+    ("almost_rectangular", [(0.0001, 0.000005), (2, 0), (0, 1), (2, 1)], (-1, -1), 2e-4, 1e-4),
+    ("degenerate_u_verts", [(0, 0), (0, 0), (0, 1), (1, 1.5)], (-1, 0), 1e-3, 1e-3),
+    ("degenerate_u_verts2", [(0, 0), (1.1, 0), (0, 1), (0, 1)], (-1, 1), 1e-3, 1e-3),
+    ("degenerate_v_verts", [(0, 0), (1, 0), (0, 0), (1, 1.5)], (0, -1), 1e-3, 1e-3),
+    ("parallel_adjacent_edges_a", [(0, 0), (1, 0), (1, 1.5), (2, 0.01)], (1, 0), 0.03, 0.03),
+    ("parallel_adjacent_edges_b", [(0, 0), (1, 0), (1, 1.5), (2, 0.01)], (-1, -1), 0.2, 0.2),
+]
+
+
+@pytest.mark.parametrize("name,abcd,excl,rel,abs_", INV_BILINEAR_CASES, ids=[c[0] for c in INV_BILINEAR_CASES])
+def test_inverse_bilinear_reference_cases(oracle, name, abcd, excl, rel, abs_):
+    quad = np.array(abcd, np.float32).reshape(8)
+    uv_in = _uv_fixture(*excl)
+    P = np.zeros_like(uv_in)
+    uv_out = np.zeros_like(uv_in)
+    L = oracle.lib()
+    L.oracle_bilerp2(quad.ctypes.data_as(C.c_void_p), uv_in.ctypes.data_as(C.c_void_p), len(uv_in),
+                     P.ctypes.data_as(C.c_void_p))
+    L.oracle_inv_bilinear(quad.ctypes.data_as(C.c_void_p), P.ctypes.data_as(C.c_void_p), len(P),
+                          uv_out.ctypes.data_as(C.c_void_p))
+    for a, b in zip(uv_in.astype(np.float64), uv_out.astype(np.float64)):
+        for x, y in zip(a, b):
+            d = abs(x - y)
+            assert d < abs_ or d < rel * abs(x) or d < rel * abs(y), (name, a, b)
+        assert b[0] >= -abs_ and b[0] <= 1 + abs_
+
+
+def test_inverse_bilinear_one_newton_step_is_exact_for_parallelograms(oracle):
+    # A parallelogram makes the map linear (G = 0): one step lands exactly
+    # on the answer for dyadic parameters.
+    quad = np.array([1, 1, 5, 2, 2, 4, 6, 5], np.float32)
+    uv = np.array([[0.25, 0.5], [0.75, 0.125], [0.5, 0.5]], np.float32)
+    P = np.zeros_like(uv)
+    out = np.zeros_like(uv)
+    L = oracle.lib()
+    L.oracle_bilerp2(quad.ctypes.data_as(C.c_void_p), uv.ctypes.data_as(C.c_void_p), 3, P.ctypes.data_as(C.c_void_p))
+    L.oracle_inv_bilinear(quad.ctypes.data_as(C.c_void_p), P.ctypes.data_as(C.c_void_p), 3, out.ctypes.data_as(C.c_void_p))
+    assert np.array_equal(out, uv)
+
+
+# --------------------------------------------------------------------------
+# Occlusion tree leaf indices: libs/core/occlusion_test.cpp:69-128
+TREE_KATS = [
+    (1, True, 0, 0, 0), (2, True, 0, 0, 1), (2, True, 1, 0, 2),
+    (3, True, 0, 0, 3), (3, True, 0, 1, 4), (3, True, 1, 0, 5), (3, True, 1, 1, 6),
+    (4, True, 0, 0, 7), (4, True, 1, 0, 8), (4, True, 2, 0, 11), (4, True, 3, 0, 12),
+    (4, True, 0, 1, 9), (4, True, 1, 1, 10), (4, True, 2, 1, 13), (4, True, 3, 1, 14),
+    (4, False, 0, 0, 7), (4, False, 1, 0, 9), (4, False, 0, 1, 8), (4, False, 1, 1, 10),
+    (4, False, 0, 2, 11), (4, False, 1, 2, 13), (4, False, 0, 3, 12), (4, False, 1, 3, 14),
+]
+
+
+def test_occlusion_tree_index_known_answers(oracle):
+    assert len(TREE_KATS) == 23
+    for levels, xfirst, x, y, want in TREE_KATS:
+        assert oracle.lib().oracle_tree_index(levels, 1 if xfirst else 0, x, y) == want
+
+
+# --------------------------------------------------------------------------
+# Filters: closed forms of libs/core/filters.cpp
+def test_filter_closed_forms(oracle):
+    f = oracle.lib().oracle_filter
+    g = FILTERS
+    assert f(g["gaussian"], 0.0, 0.0, 2.0, 2.0) == 1.0
+    for x, y in [(0.3, -0.2), (0.9, 0.7), (-0.5, 0.5)]:
+        xf, yf = np.float32(x) / np.float32(2.0), np.float32(y) / np.float32(2.0)
+        want = np.float32(math.exp(-8.0 * float(np.float32(xf * xf + yf * yf))))
+        assert f(g["gaussian"], x, y, 2.0, 2.0) == want
+    # Catmull-Rom (RI spec 3.2 form): 2 at the centre, 0 at r = 1 and r >= 2
+    assert f(g["catmull-rom"], 0.0, 0.0, 3.0, 3.0) == 2.0
+    assert f(g["catmull-rom"], 1.0, 0.0, 3.0, 3.0) == 0.0
+    assert f(g["catmull-rom"], 0.6, 0.8, 3.0, 3.0) == 0.0
+    assert f(g["catmull-rom"], 2.0, 0.0, 3.0, 3.0) == 0.0
+    r = np.float32(0.5)
+    assert f(g["catmull-rom"], 0.5, 0.0, 3.0, 3.0) == np.float32(3.0 * 0.5 * 0.25 - 5.0 * 0.25 + 2.0)
+    r = 1.5
+    assert f(g["catmull-rom"], 1.5, 0.0, 3.0, 3.0) == np.float32(-1.5 * 2.25 + 5.0 * 2.25 - 8.0 * 1.5 + 4.0)
+    assert f(g["box"], 0.4, -0.4, 1.0, 1.0) == 1.0 and f(g["box"], 0.6, 0.0, 1.0, 1.0) == 0.0
+    assert f(g["triangle"], 0.0, 0.0, 2.0, 2.0) == 1.0 and f(g["triangle"], 0.5, 0.0, 2.0, 2.0) == 0.5
+    assert f(g["disk"], 0.3, 0.3, 2.0, 2.0) == 1.0 and f(g["disk"], 0.8, 0.8, 2.0, 2.0) == 0.0
+    assert f(g["sinc"], 0.0, 0.0, 3.0, 3.0) == 1.0
+
+
+# --------------------------------------------------------------------------
+# Multijitter patterns: stratification invariants (multijitter.cpp:39-82)
+@pytest.mark.parametrize("nx,ny", [(4, 4), (2, 2), (3, 3), (8, 8), (2, 3)])
+def test_multijitter_stratification(oracle, nx, ny):
+    n = nx * ny
+    p2 = np.zeros((250, n, 2), np.float32)
+    p1 = np.zeros((250, n), np.float32)
+    sh = np.zeros((250, n), np.int32)
+    oracle.lib().oracle_jitter_tables(nx, ny, p2.ctypes.data_as(C.c_void_p), p1.ctypes.data_as(C.c_void_p),
+                                      sh.ctypes.data_as(C.c_void_p))
+    assert p2.min() >= 0.0 and p2.max() < 1.0
+    for pat in range(250):
+        pos = p2[pat].astype(np.float64)
+        # one sample in each of the nx x ny sub-pixels, sample k in sub-pixel k
+        ix = np.floor(pos[:, 0] * nx).astype(int)
+        iy = np.floor(pos[:, 1] * ny).astype(int)
+        assert np.array_equal(iy * nx + ix, np.arange(n))
+        # Latin hypercube: one sample in each of the n columns and n rows
+        assert sorted(np.floor(pos[:, 0] * n).astype(int)) == list(range(n))
+        assert sorted(np.floor(pos[:, 1] * n).astype(int)) == list(range(n))
+        # 1-D samples: k/n plus one shared offset below 1/n
+        off = p1[pat] - (np.arange(n, dtype=np.float64) / n)
+        assert np.all(np.abs(off - off[0]) < 1e-6) and 0.0 <= off[0] < 1.0 / n + 1e-7
+        assert sorted(sh[pat]) == list(range(n))
+
+
+# --------------------------------------------------------------------------
+# Point-in-micropolygon: on a tiled grid every interior sample is claimed by
+# exactly one micropolygon, including samples exactly on shared edges and
+# vertices (micropolygon.cpp:1309-1312).
+def test_point_in_quad_tiles_exactly_once(oracle):
+    rng = np.random.default_rng(4)
+    N = 5
+    gx, gy = np.meshgrid(np.arange(N + 1, dtype=np.float32), np.arange(N + 1, dtype=np.float32))
+    verts = np.stack([gx * 4, gy * 4], -1)
+    verts[1:N, 1:N] += rng.uniform(-1.2, 1.2, (N - 1, N - 1, 2)).astype(np.float32)
+    z = rng.uniform(5, 6, (N + 1, N + 1)).astype(np.float32)
+    samples = [(x, y) for x in np.arange(0.5, 4 * N - 0.5, 0.37) for y in np.arange(0.5, 4 * N - 0.5, 0.41)]
+    samples += [(float(verts[i, j, 0]), float(verts[i, j, 1])) for i in range(1, N) for j in range(1, N)]
+    samples += [(float((verts[i, j, 0] + verts[i, j + 1, 0]) / 2), float((verts[i, j, 1] + verts[i, j + 1, 1]) / 2))
+                for i in range(1, N) for j in range(0, N)]
+    L = oracle.lib()
+    d = C.c_float()
+    for (x, y) in samples:
+        hits = 0
+        for i in range(N):
+            for j in range(N):
+                P = np.array([[*verts[i, j], z[i, j]], [*verts[i, j + 1], z[i, j + 1]],
+                              [*verts[i + 1, j], z[i + 1, j]], [*verts[i + 1, j + 1], z[i + 1, j + 1]]],
+                             np.float32).reshape(12)
+                hits += L.oracle_mp_sample(P.ctypes.data_as(C.c_void_p), x, y, C.byref(d))
+        assert hits == 1, (x, y, hits)
+
+
+def test_depth_closed_form_on_planar_rectangle(oracle):
+    # z linear in x and y on an axis-aligned rectangle: bilinear depth is the plane.
+    P = np.array([[10, 20, 3.0], [14, 20, 3.5], [10, 22, 4.0], [14, 22, 4.5]], np.float32).reshape(12)
+    d = C.c_float()
+    for x, y in [(11.0, 20.5), (13.0, 21.5), (12.0, 21.0)]:
+        assert oracle.lib().oracle_mp_sample(P.ctypes.data_as(C.c_void_p), x, y, C.byref(d)) == 1
+        want = 3.0 + 0.5 * (x - 10) / 4 + 1.0 * (y - 20) / 2
+        assert abs(d.value - want) <= 2e-7 * want
+
+
+# --------------------------------------------------------------------------
+# Bicubic dicing by forward differencing vs direct Bernstein evaluation
+def test_forward_differencing_matches_bezier(oracle):
+    rng = np.random.default_rng(9)
+    P = rng.uniform(-3, 3, (16, 3)).astype(np.float32)
+    P[:, 2] += 10
+    prim = scenes.bicubic(P, (1, 1, 1))
+    for u, v in [(16, 16), (5, 11), (1, 3)]:
+        got = oracle.dice(prim, u, v).astype(np.float64)
+        B = lambda t: np.array([(1 - t) ** 3, 3 * t * (1 - t) ** 2, 3 * t * t * (1 - t), t ** 3])
+        want = np.zeros_like(got)
+        for iv in range(v + 1):
+            for iu in range(u + 1):
+                bu, bv = B(iu / u), B(iv / v)
+                want[iv * (u + 1) + iu] = np.einsum("i,j,ijk->k", bv, bu, P.astype(np.float64).reshape(4, 4, 3))
+        assert np.max(np.abs(got - want)) < 2e-5 * np.max(np.abs(want))
+
+
+def test_sphere_dicing_lies_on_sphere(oracle):
+    prim = scenes.sphere(1.5, (0.5, -1.0, 12.0), (1, 1, 1))
+    pts = oracle.dice(prim, 12, 9).astype(np.float64)
+    r = np.linalg.norm(pts - np.array([0.5, -1.0, 12.0]), axis=1)
+    assert np.max(np.abs(r - 1.5)) < 1e-5
+
+
+# --------------------------------------------------------------------------
+# Whole-frame closed forms
+def _quad_scene(depths_colors, res=(48, 40), samples=(4, 4), filt=("gaussian", 2.0, 2.0), opac=None):
+    opts = default_options(xres=res[0], yres=res[1], fov=60.0, xsamples=samples[0], ysamples=samples[1],
+                           filter=filt[0], filter_xwidth=filt[1], filter_ywidth=filt[2])
+    prims = []
+    for k, (z, col) in enumerate(depths_colors):
+        s = 2 * z  # wide enough to cover the frame at this depth
+        P = [(-s, -s, z), (s, -s, z), (-s, s, z), (s, s, z)]
+        os_ = (1, 1, 1) if opac is None else opac[k]
+        prims.append(scenes.bilinear(P, col, os_))
+    return opts, prims
+
+
+def test_constant_frame_filters_to_its_colour(oracle):
+    col = (0.25, 0.5, 0.75)
+    opts, prims = _quad_scene([(5.0, col)])
+    r = oracle.render(opts, prims)
+    ch = r["channels"]
+    assert np.max(np.abs(ch[..., 0:3] - np.array(col, np.float32))) < 2e-6
+    assert np.max(np.abs(ch[..., 3:6] - 1.0)) < 2e-6
+    assert np.all(ch[..., 8] == 1.0)                                             # coverage
+    assert np.max(np.abs(ch[..., 6] - 1.0)) < 2e-6                               # alpha = mean(Oi) * coverage
+    assert np.max(np.abs(ch[..., 7] - 5.0)) < 5e-6                               # planar depth
+
+
+def test_two_layer_over_compositing(oracle):
+    # Back-to-front "over" of premultiplied constant-shader colours
+    # (imagepixel.cpp:197-241): C = Ci_f + Ci_b (1 - O_f), A = O_f + O_b (1 - O_f).
+    cf, cb = np.array([0.9, 0.2, 0.1]), np.array([0.1, 0.3, 0.8])
+    of, ob = 0.4, 0.7
+    opts, prims = _quad_scene([(4.0, cf), (6.0, cb)], opac=[(of,) * 3, (ob,) * 3])
+    r = oracle.render(opts, prims)
+    rgbaz = r["rgbaz"][r["mp_id"] != 0xFFFFFFFF].astype(np.float64)
+    want_c = of * cf + ob * cb * (1 - of)
+    want_a = of + ob * (1 - of)
+    assert np.max(np.abs(rgbaz[:, 0:3] - want_c)) < 1e-6
+    assert np.max(np.abs(rgbaz[:, 3:6] - want_a)) < 1e-6
+    # no opaque layer: the depth after combining is the empty occlusion depth
+    assert np.all(rgbaz[:, 6] == np.float32(3.4028234663852886e38))
+    # front-most hit is the nearer surface
+    assert np.all(np.abs(r["depth"][r["mp_id"] != 0xFFFFFFFF] - 4.0) < 1e-5)
+
+
+def test_exposure_and_quantise_closed_forms(oracle):
+    opts, prims = _quad_scene([(5.0, (0.2, 0.6, 0.9))])
+    opts.exposure_gain, opts.exposure_gamma = 1.5, 2.0
+    r = oracle.render(opts, prims)
+    ch = r["channels"]
+    base = np.array([0.2, 0.6, 0.9])
+    assert np.max(np.abs(ch[..., 0:3] - (base * 1.5) ** 0.5)) < 1e-5
+    xy, dither = oracle.sample_positions(opts)
+    v = ch[..., [0, 1, 2, 6]].astype(np.float64)
+    q = np.floor(0.0 + v * 255.0 + 0.5 * dither[..., None].astype(np.float64) - 0.5) + 1
+    q = np.clip(q, 0, 255).astype(np.uint8)
+    assert np.array_equal(r["rgba8"], q)
+
+
+# --------------------------------------------------------------------------
+# Occlusion culling only skips work: identical samples and pixels.
+@pytest.mark.parametrize("make", [lambda: scenes.spheres_scene(seed=1),
+                                  lambda: scenes.patches_scene(seed=5, n=400, res=(320, 180))])
+def test_occlusion_culling_changes_no_result(oracle, make):
+    opts, prims = make()
+    a = oracle.render(opts, prims, occlusion_cull=True)
+    b = oracle.render(opts, prims, occlusion_cull=False)
+    assert np.array_equal(a["depth"].view(np.uint32), b["depth"].view(np.uint32))
+    assert np.array_equal(a["rgbaz"].view(np.uint32), b["rgbaz"].view(np.uint32))
+    assert np.array_equal(a["channels"].view(np.uint32), b["channels"].view(np.uint32))
+    assert np.array_equal(a["rgba8"], b["rgba8"])
+    assert a["stats"]["occlusion_culled"] > 0
+    assert a["stats"]["spl_count"] < b["stats"]["spl_count"]
