@@ -22,8 +22,12 @@ extern "C" hipError_t rh_launch_sample(hipStream_t, const FrameDev*, const Bucke
                                        const unsigned long long*, const uint32_t*, const uint2*, const float*,
                                        const GridDev*, const PrimDev*, const float*, const float*, const uchar4*,
                                        float4*, unsigned long long*);
-extern "C" hipError_t rh_launch_filter(hipStream_t, const FrameDev*, int, const float4*, const uchar4*, const float*,
-                                       const float*, const float*, float*, uchar4*);
+extern "C" size_t rh_filter_tile_lds(int, int, int);
+extern "C" hipError_t rh_launch_box_masks(hipStream_t, const FrameDev*, size_t, const uchar4*, const float*,
+                                          uint16_t*);
+extern "C" hipError_t rh_launch_filter(hipStream_t, const FrameDev*, int, int, int, int, int, const float4*,
+                                       const uint16_t*, const uchar4*, const float*, const float*, const float*,
+                                       float*, uchar4*);
 
 namespace {
 
@@ -86,14 +90,16 @@ struct rh_ctx {
     hipStream_t st = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // frame shape
-    DBuf dFrame, dBuckets, dBlocks, dPos2d, dPos1d, dPix, dDither, dFilter;
+    DBuf dFrame, dBuckets, dBlocks, dPos2d, dPos1d, dPix, dDither, dFilter, dBoxMask;
+    bool tiledFilter = false;
     // schedule
     DBuf dGrids, dParams, dPrims, dCandStart, dCandGrid, dCapStart;
     // per frame
     DBuf dVtx, dMprec, dMpbound, dLists, dListCount, dSamples, dChannels, dRgba, dCounters;
     rh_stats stats;
     ~rh_ctx() {
-        DBuf* all[] = {&dFrame, &dBuckets, &dBlocks, &dPos2d, &dPos1d, &dPix, &dDither, &dFilter, &dGrids, &dParams,
+        DBuf* all[] = {&dFrame, &dBuckets, &dBlocks, &dPos2d, &dPos1d, &dPix, &dDither, &dFilter, &dBoxMask,
+                       &dGrids, &dParams,
                        &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dVtx, &dMprec, &dMpbound, &dLists,
                        &dListCount, &dSamples, &dChannels, &dRgba, &dCounters};
         for (DBuf* b : all) b->release();
@@ -199,9 +205,26 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
         (e = c->dChannels.ensure((size_t)s.width() * s.height() * 9 * sizeof(float))) != hipSuccess ||
         (e = c->dRgba.ensure((size_t)s.width() * s.height() * 4)) != hipSuccess ||
         (e = c->dListCount.ensure(s.buckets.size() * sizeof(uint32_t))) != hipSuccess ||
-        (e = upload(c->dCounters, zero, c->st)) != hipSuccess || (e = hipStreamSynchronize(c->st)) != hipSuccess) {
+        (e = upload(c->dCounters, zero, c->st)) != hipSuccess) {
         delete c;
         return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_DEVICE, std::string("device set-up: ") + hipGetErrorString(e));
+    }
+    // The tiled filter reads each sample's box-test mask, fixed by the frame
+    // shape's sample positions.
+    c->tiledFilter = rh_filter_tile_lds(s.ns, s.shiftX, s.shiftY) != 0;
+    if (c->tiledFilter) {
+        const size_t nsamp = (size_t)s.extW * s.extH * s.ns;
+        if ((e = c->dBoxMask.ensure(nsamp * sizeof(uint16_t))) != hipSuccess ||
+            (e = rh_launch_box_masks(c->st, c->dFrame.as<FrameDev>(), nsamp, c->dPix.as<uchar4>(),
+                                     c->dPos2d.as<float>(), c->dBoxMask.as<uint16_t>())) != hipSuccess) {
+            delete c;
+            return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_DEVICE,
+                        std::string("device set-up: ") + hipGetErrorString(e));
+        }
+    }
+    if ((e = hipStreamSynchronize(c->st)) != hipSuccess) {
+        delete c;
+        return fail(RH_ERR_DEVICE, std::string("device set-up: ") + hipGetErrorString(e));
     }
     *out = c;
     return RH_OK;
@@ -288,7 +311,8 @@ rh_status rh_render_device(rh_ctx* c) {
                              c->dPix.as<uchar4>(), c->dSamples.as<float4>(),
                              c->dCounters.as<unsigned long long>()));
     HIP_TRY(hipEventRecord(c->ev[3], c->st));
-    HIP_TRY(rh_launch_filter(c->st, f, s.width() * s.height(), c->dSamples.as<float4>(), c->dPix.as<uchar4>(),
+    HIP_TRY(rh_launch_filter(c->st, f, s.width(), s.height(), s.ns, s.shiftX, s.shiftY, c->dSamples.as<float4>(),
+                             c->tiledFilter ? c->dBoxMask.as<uint16_t>() : nullptr, c->dPix.as<uchar4>(),
                              c->dPos2d.as<float>(), c->dFilter.as<float>(), c->dDither.as<float>(),
                              c->dChannels.as<float>(), c->dRgba.as<uchar4>()));
     HIP_TRY(hipEventRecord(c->ev[4], c->st));

@@ -585,6 +585,56 @@ __global__ void __launch_bounds__(1024) rh_sample(const FrameDev* __restrict__ f
 // ---------------------------------------------------------------------------
 // Filter + expose + quantise
 // ---------------------------------------------------------------------------
+// Normalise one pixel's weighted sums and write its channels and 8-bit rgba:
+// FilterBucket's tail (bucketprocessor.cpp:640-707), ExposeBucket
+// (bucketprocessor.cpp:766-806) and FormatBucketForDisplay
+// (ddmanager.cpp:1045-1118).  acc = weighted Ci.rgb, Oi.rgb, z.
+__device__ __forceinline__ void finish_pixel(const FrameDev* __restrict__ fr, const float* acc, float gTot, int count,
+                                             int ns, size_t t, const float* __restrict__ dither,
+                                             float* __restrict__ channels, uchar4* __restrict__ rgba8) {
+    float ch[9];
+    float coverage;
+    if (count == 0) {
+        for (int k = 0; k < 9; ++k) ch[k] = 0.0f;
+        ch[7] = FLT_MAX;
+        coverage = 0.0f;
+    } else {
+        float inv = (float)(1.0 / (double)gTot);
+        for (int k = 0; k < 6; ++k) ch[k] = acc[k] * inv;
+        ch[7] = acc[6] * inv;
+        coverage = (count >= ns) ? 1.0f : (float)count / (float)ns;
+    }
+    float a = (ch[3] + ch[4] + ch[5]) / 3.0f;
+    ch[6] = a * coverage;
+    ch[8] = coverage;
+    const float gain = fr->exposureGain, gamma = fr->exposureGamma;
+    if (count > 0 && !(gain == 1.0f && gamma == 1.0f)) {
+        float oog = 1.0f / gamma;
+        for (int k = 0; k < 3; ++k) {
+            if (gain != 1.0f) ch[k] *= gain;
+            if (gamma != 1.0f) ch[k] = (float)pow((double)ch[k], (double)oog);
+        }
+    }
+    float* out = channels + t * 9;
+    for (int k = 0; k < 9; ++k) out[k] = ch[k];
+    const double s = (double)dither[t];
+    const float vals[4] = {ch[0], ch[1], ch[2], ch[6]};
+    unsigned char q[4];
+    for (int k = 0; k < 4; ++k) {
+        double value = vals[k];
+        if (fr->qOne != 0.0f) {
+            double xq = 0.0f + value * (double)(fr->qOne - 0.0f) + ((double)fr->qDither * s);
+            value = (double)(lfloor_d(xq - 0.5) + 1);
+            value = value < (double)fr->qMin ? (double)fr->qMin : (value > (double)fr->qMax ? (double)fr->qMax : value);
+        }
+        q[k] = (unsigned char)value;
+    }
+    rgba8[t] = make_uchar4(q[0], q[1], q[2], q[3]);
+}
+
+// General filter: one thread per display pixel, samples read straight from
+// HBM.  Used when a tile's halo does not fit the tiled kernel's LDS.
+// FilterBucket, non-separable branch (bucketprocessor.cpp:561-663).
 __global__ void __launch_bounds__(256) rh_filter(const FrameDev* __restrict__ fr, const float4* __restrict__ samples,
                                                  const uchar4* __restrict__ pixPattern, const float* __restrict__ pos2d,
                                                  const float* __restrict__ filterTab, const float* __restrict__ dither,
@@ -599,7 +649,6 @@ __global__ void __launch_bounds__(256) rh_filter(const FrameDev* __restrict__ fr
     float gTot = 0.0f;
     int count = 0;
     float acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    // FilterBucket, non-separable branch (bucketprocessor.cpp:561-663)
     for (int fy = -ym; fy <= ym; ++fy)
         for (int fx = -xm; fx <= xm; ++fx) {
             const int qx = x + fx, qy = y + fy;
@@ -622,47 +671,169 @@ __global__ void __launch_bounds__(256) rh_filter(const FrameDev* __restrict__ fr
                 }
             }
         }
-    float ch[9];
-    float coverage;
-    if (count == 0) {
-        for (int k = 0; k < 9; ++k) ch[k] = 0.0f;
-        ch[7] = FLT_MAX;
-        coverage = 0.0f;
-    } else {
-        float inv = (float)(1.0 / (double)gTot);
-        for (int k = 0; k < 6; ++k) ch[k] = acc[k] * inv;
-        ch[7] = acc[6] * inv;
-        coverage = (count >= ns) ? 1.0f : (float)count / (float)ns;
+    finish_pixel(fr, acc, gTot, count, ns, (size_t)t, dither, channels, rgba8);
+}
+
+// Box-test masks of every sample of the extended region, once per frame
+// shape: bit (fx + xm) says that the sample of pixel qx lies inside the
+// filter box of pixel qx - fx (|position - pixel centre| <= ceil(width)/2,
+// bucketprocessor.cpp:600-612, evaluated exactly as the filter loop does),
+// bit (8 + fy + ym) the same for rows.
+__global__ void __launch_bounds__(256) rh_box_masks(const FrameDev* __restrict__ fr,
+                                                    const uchar4* __restrict__ pixPattern,
+                                                    const float* __restrict__ pos2d, uint16_t* __restrict__ masks) {
+    const int ns = fr->ns, xm = fr->shiftX, ym = fr->shiftY, extW = fr->extW;
+    const size_t n = (size_t)extW * fr->extH * ns;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const size_t q = t / ns;
+    const int si = (int)(t - q * ns);
+    const int qx = fr->extX0 + (int)(q % extW), qy = fr->extY0 + (int)(q / extW);
+    const int pat = pixPattern[q].x;
+    const float sx = (float)qx + pos2d[2 * (pat * ns + si)];
+    const float sy = (float)qy + pos2d[2 * (pat * ns + si) + 1];
+    const float xf = fr->xfwo2, yf = fr->yfwo2;
+    uint32_t mask = 0;
+    for (int fx = -xm; fx <= xm; ++fx) {
+        const float xcent = (qx - fx) + 0.5f;
+        const float vx = sx - xcent;
+        if (vx >= -xf && vx <= xf) mask |= 1u << (fx + xm);
     }
-    // alpha and coverage (bucketprocessor.cpp:690-707)
-    float a = (ch[3] + ch[4] + ch[5]) / 3.0f;
-    ch[6] = a * coverage;
-    ch[8] = coverage;
-    // ExposeBucket (bucketprocessor.cpp:766-806); skipped with empty buckets and trivial exposure
-    const float gain = fr->exposureGain, gamma = fr->exposureGamma;
-    if (count > 0 && !(gain == 1.0f && gamma == 1.0f)) {
-        float oog = 1.0f / gamma;
-        for (int k = 0; k < 3; ++k) {
-            if (gain != 1.0f) ch[k] *= gain;
-            if (gamma != 1.0f) ch[k] = (float)pow((double)ch[k], (double)oog);
+    for (int fy = -ym; fy <= ym; ++fy) {
+        const float ycent = (qy - fy) + 0.5f;
+        const float vy = sy - ycent;
+        if (vy >= -yf && vy <= yf) mask |= 1u << (8 + fy + ym);
+    }
+    masks[t] = (uint16_t)mask;
+}
+
+// Tiled filter.  One workgroup per RH_FT x RH_FT tile of display pixels,
+// two waves: wave 0 sums Ci.rgb and z, wave 1 sums Oi.rgb, each lane one
+// pixel, each sum in the reference's order (filter rows, filter columns,
+// samples), so the split changes no rounding.  The tile's halo of samples is
+// staged in LDS once, with RH_FU coalesced loads in flight per lane, as 8
+// dwords per sample: Ci.rgb, z, Oi.rgb and a word holding the sample's
+// box-test mask (bits 0..15) and whether it holds a surface (bit 31).  Halo
+// pixel records are 8*ns+1 dwords apart so that the lanes of a wave read
+// different LDS banks.
+#define RH_FT 8
+#define RH_FU 16
+__global__ void __launch_bounds__(128) rh_filter_tiled(const FrameDev* __restrict__ fr,
+                                                       const float4* __restrict__ samples,
+                                                       const uint16_t* __restrict__ boxMask,
+                                                       const float* __restrict__ filterTab,
+                                                       const float* __restrict__ dither, float* __restrict__ channels,
+                                                       uchar4* __restrict__ rgba8, int tilesX, int ntiles) {
+    // LDS: halo records (RH_FT+2*xm)*(RH_FT+2*ym)*(8*ns+1) floats, then the
+    // filter weights; the tail exchange reuses the first 192 floats.
+    extern __shared__ float lds[];
+    // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+    // contiguous run of tiles so neighbouring tiles share halos in its L2.
+    // The grid is a multiple of 8 workgroups.
+    const int per = gridDim.x >> 3;
+    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;
+    const int ns = fr->ns, xm = fr->shiftX, ym = fr->shiftY;
+    const int extX0 = fr->extX0, extY0 = fr->extY0, extW = fr->extW, extH = fr->extH;
+    const int tx0 = fr->cropX0 + (tile % tilesX) * RH_FT, ty0 = fr->cropY0 + (tile / tilesX) * RH_FT;
+    const int HX = RH_FT + 2 * xm, HY = RH_FT + 2 * ym, NH = HX * HY;
+    const int S = 8 * ns + 1;
+    const int tid = threadIdx.x;
+    // The filter weight table follows the halo in LDS (uniform reads).
+    float* wtab = lds + NH * S;
+    const int nwt = (2 * xm + 1) * (2 * ym + 1) * ns;
+    for (int i = tid; i < nwt; i += 128) wtab[i] = filterTab[i];
+
+    // 1. Stage the halo's sample records (2 float4 per sample) and masks.
+    {
+        const int per_px = 2 * ns, total = NH * per_px;
+        const int dhp = 128 / per_px, dpart = 128 % per_px;
+        int hp = tid / per_px, part = tid % per_px;
+        int hy = hp / HX, hx = hp % HX;
+        for (int base = 0; base < total; base += 128 * RH_FU) {
+            float4 v[RH_FU];
+            uint32_t mk[RH_FU];
+            int dst[RH_FU];
+#pragma unroll
+            for (int u = 0; u < RH_FU; ++u) {
+                const int i = base + u * 128 + tid;
+                v[u] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0xffffffffu));
+                mk[u] = 0;
+                dst[u] = -1;
+                if (i < total) {
+                    const int qx = tx0 - xm + hx, qy = ty0 - ym + hy;
+                    dst[u] = (hp * S + part * 4) * 2 + (part & 1);
+                    if (qx >= extX0 && qx < extX0 + extW && qy >= extY0 && qy < extY0 + extH) {
+                        const size_t q = (size_t)(qy - extY0) * extW + (qx - extX0);
+                        v[u] = samples[q * per_px + part];
+                        if (part & 1) mk[u] = boxMask[q * ns + (part >> 1)];
+                    }
+                }
+                part += dpart;
+                int adv = dhp;
+                if (part >= per_px) { part -= per_px; ++adv; }
+                hp += adv;
+                hx += adv;
+                while (hx >= HX) { hx -= HX; ++hy; }
+            }
+#pragma unroll
+            for (int u = 0; u < RH_FU; ++u) {
+                if (dst[u] < 0) continue;
+                float* d = lds + (dst[u] >> 1);
+                d[0] = v[u].x;
+                d[1] = v[u].y;
+                d[2] = v[u].z;
+                d[3] = (dst[u] & 1) ? __uint_as_float(mk[u] | ((__float_as_uint(v[u].w) != 0xffffffffu) ? 0x80000000u : 0u))
+                                    : v[u].w;
+            }
         }
     }
-    float* out = channels + (size_t)t * 9;
-    for (int k = 0; k < 9; ++k) out[k] = ch[k];
-    // FormatBucketForDisplay (ddmanager.cpp:1045-1118): rgba, 8-bit
-    const double s = (double)dither[t];
-    const float vals[4] = {ch[0], ch[1], ch[2], ch[6]};
-    unsigned char q[4];
-    for (int k = 0; k < 4; ++k) {
-        double value = vals[k];
-        if (fr->qOne != 0.0f) {
-            double xq = 0.0f + value * (double)(fr->qOne - 0.0f) + ((double)fr->qDither * s);
-            value = (double)(lfloor_d(xq - 0.5) + 1);
-            value = value < (double)fr->qMin ? (double)fr->qMin : (value > (double)fr->qMax ? (double)fr->qMax : value);
-        }
-        q[k] = (unsigned char)value;
+    __syncthreads();
+    // 2. Weighted sums, one lane per pixel per half of the channels.
+    const int lane = tid & 63, half = tid >> 6;
+    const int lx = lane & (RH_FT - 1), ly = lane / RH_FT;
+    const int x = tx0 + lx, y = ty0 + ly;
+    const bool valid = x < fr->cropX1 && y < fr->cropY1;
+    float gTot = 0.0f;
+    int count = 0;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (valid) {
+        const int c0 = half ? 4 : 0;
+        for (int fy = -ym; fy <= ym; ++fy)
+            for (int fx = -xm; fx <= xm; ++fx) {
+                const float* rec = lds + ((ly + fy + ym) * HX + (lx + fx + xm)) * S + c0;
+                const float* wt = wtab + ((fy + ym) * (2 * xm + 1) + fx + xm) * ns;
+                const uint32_t need = (1u << (fx + xm)) | (1u << (8 + fy + ym));
+#pragma unroll 4
+                for (int si = 0; si < ns; ++si) {
+                    const uint32_t m = __float_as_uint(rec[si * 8 + 7 - c0]);
+                    if ((m & need) == need) {
+                        const float g = wt[si];
+                        gTot += g;
+                        if (m >> 31) {
+                            acc[0] += rec[si * 8 + 0] * g;
+                            acc[1] += rec[si * 8 + 1] * g;
+                            acc[2] += rec[si * 8 + 2] * g;
+                            if (!half) acc[3] += rec[si * 8 + 3] * g;
+                            ++count;
+                        }
+                    }
+                }
+            }
     }
-    rgba8[t] = make_uchar4(q[0], q[1], q[2], q[3]);
+    __syncthreads();                  // the halo records are no longer read
+    if (half) {
+        lds[lane * 3 + 0] = acc[0];
+        lds[lane * 3 + 1] = acc[1];
+        lds[lane * 3 + 2] = acc[2];
+    }
+    __syncthreads();
+    if (!half && valid) {
+        const float full[7] = {acc[0], acc[1], acc[2], lds[lane * 3 + 0], lds[lane * 3 + 1], lds[lane * 3 + 2],
+                               acc[3]};
+        const size_t t = (size_t)(y - fr->cropY0) * (fr->cropX1 - fr->cropX0) + (x - fr->cropX0);
+        finish_pixel(fr, full, gTot, count, ns, t, dither, channels, rgba8);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -695,10 +866,41 @@ extern "C" hipError_t rh_launch_sample(hipStream_t st, const FrameDev* fr, const
     return hipGetLastError();
 }
 
-extern "C" hipError_t rh_launch_filter(hipStream_t st, const FrameDev* fr, int npix, const float4* samples,
+// LDS bytes of one tiled-filter workgroup, or 0 if the tiled kernel cannot
+// take this frame shape (mask bits or LDS size exceeded).
+extern "C" size_t rh_filter_tile_lds(int ns, int xm, int ym) {
+    if (2 * xm + 1 > 8 || 2 * ym + 1 > 8) return 0;
+    const size_t bytes = ((size_t)(RH_FT + 2 * xm) * (RH_FT + 2 * ym) * (8 * ns + 1) +
+                          (size_t)(2 * xm + 1) * (2 * ym + 1) * ns) * sizeof(float);
+    return bytes <= 64 * 1024 ? bytes : 0;
+}
+
+extern "C" hipError_t rh_launch_box_masks(hipStream_t st, const FrameDev* fr, size_t nsamples,
+                                          const uchar4* pixPattern, const float* pos2d, uint16_t* masks) {
+    if (nsamples == 0) return hipSuccess;
+    hipLaunchKernelGGL(rh_box_masks, dim3((unsigned)((nsamples + 255) / 256)), dim3(256), 0, st, fr, pixPattern,
+                       pos2d, masks);
+    return hipGetLastError();
+}
+
+// boxMask: the frame shape's masks from rh_launch_box_masks, or null when
+// rh_filter_tile_lds is 0 for it (then the general kernel runs).
+extern "C" hipError_t rh_launch_filter(hipStream_t st, const FrameDev* fr, int width, int height, int ns, int xm,
+                                       int ym, const float4* samples, const uint16_t* boxMask,
                                        const uchar4* pixPattern, const float* pos2d, const float* filterTab,
                                        const float* dither, float* channels, uchar4* rgba8) {
-    hipLaunchKernelGGL(rh_filter, dim3((npix + 255) / 256), dim3(256), 0, st, fr, samples, pixPattern, pos2d,
-                       filterTab, dither, channels, rgba8);
+    const int npix = width * height;
+    if (npix <= 0) return hipSuccess;
+    const size_t lds = boxMask ? rh_filter_tile_lds(ns, xm, ym) : 0;
+    if (lds) {
+        const int tilesX = (width + RH_FT - 1) / RH_FT, tilesY = (height + RH_FT - 1) / RH_FT;
+        const int ntiles = tilesX * tilesY;
+        const int per = (ntiles + 7) / 8;
+        hipLaunchKernelGGL(rh_filter_tiled, dim3(8 * per), dim3(128), lds, st, fr, samples, boxMask, filterTab,
+                           dither, channels, rgba8, tilesX, ntiles);
+    } else {
+        hipLaunchKernelGGL(rh_filter, dim3((npix + 255) / 256), dim3(256), 0, st, fr, samples, pixPattern, pos2d,
+                           filterTab, dither, channels, rgba8);
+    }
     return hipGetLastError();
 }

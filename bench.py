@@ -90,7 +90,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="hd")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-kernel", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_kernel:
+        import torch
+        n = args.copy_kernel // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4}))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; the hider's multi-GPU sharding "
               "(screen-space bucket rows per GPU) is a design note, not code.", file=sys.stderr)

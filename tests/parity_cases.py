@@ -45,6 +45,12 @@ def spheres_exposure_mitchell():
     return opts, prims
 
 
+def spheres_wide_gaussian():
+    # A 5x5 filter footprint (DiscreteShift 2) with 3x3 samples and a frame
+    # that is not a whole number of 8x8 pixel tiles.
+    return scenes.spheres_scene(seed=19, n=40, res=(150, 98), samples=(3, 3), filt=("gaussian", 4.0, 4.0))
+
+
 CASES = {
     "spheres_config": spheres_config,
     "patches_small": patches_small,
@@ -53,4 +59,5 @@ CASES = {
     "patches_crop_odd": patches_crop_odd,
     "patches_gridsampler": patches_gridsampler,
     "spheres_exposure_mitchell": spheres_exposure_mitchell,
+    "spheres_wide_gaussian": spheres_wide_gaussian,
 }
