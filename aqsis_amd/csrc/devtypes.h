@@ -20,6 +20,10 @@
 //  [28..31] bound min x, max x, min y, max y
 //  [32] bound min z  [33] linear flag (int bits)  [34] grid index (int bits)  [35] unused
 #define RH_MPREC_FLOATS 36
+// Device counters: 0..4 the hider's SPL counters (sample tests, bound hits,
+// stored hits, point-in-polygon hits, second Newton steps), 5 tasks whose
+// list overflowed, 6 micropolygon-bucket references.
+#define RH_NCOUNTERS 8
 // Sphere parameter layout (see front.h)
 #define SPH_RADIUS_D 0
 #define SPH_MAT_D 1
@@ -51,7 +55,13 @@ struct BucketDev {
     int x, y, w, h;
     int sx0, sy0, sx1, sy1;       // owned sample region
     int bx, by;                   // bucket coordinates
-    int pad0, pad1;
+    int task0, ntask;             // the bucket's sampling tasks
+};
+
+struct TaskDev {
+    int bucket;
+    int x0, y0, x1, y1;           // pixels, inside the bucket's owned region
+    int pad0, pad1, pad2;
 };
 
 struct PrimDev {

@@ -116,6 +116,43 @@ bool make_shape(const rh_options& o, Shape& s, std::string& err) {
             bi.sx0 = sx0; bi.sy0 = sy0; bi.sx1 = sx1; bi.sy1 = sy1;
             s.buckets.push_back(bi);
         }
+
+    // Wave blocks: 64 samples as a near-square power-of-two pixel block, a
+    // row of pixels when 64 / ns is not a power of two, one pixel when
+    // ns >= 64.
+    int ppw = 64 / s.ns;
+    if (ppw < 1) ppw = 1;
+    if ((ppw & (ppw - 1)) == 0) {
+        int lg = 0;
+        while ((1 << lg) < ppw) ++lg;
+        s.blockW = 1 << ((lg + 1) / 2);
+        s.blockH = ppw / s.blockW;
+    } else {
+        s.blockW = ppw;
+        s.blockH = 1;
+    }
+    // Tasks: split each owned region into at most 4 x 4 near-equal parts of
+    // at least 4 blocks per side.
+    const int ntx = std::max(1, std::min(4, o.bucket_x / (4 * s.blockW)));
+    const int nty = std::max(1, std::min(4, o.bucket_y / (4 * s.blockH)));
+    s.tasks.clear();
+    s.taskStart.assign(1, 0);
+    for (size_t k = 0; k < s.buckets.size(); ++k) {
+        const auto& b = s.buckets[k];
+        const int ow = b.sx1 - b.sx0, oh = b.sy1 - b.sy0;
+        if (ow > 0 && oh > 0) {
+            const int tx = std::min(ntx, ow), ty = std::min(nty, oh);
+            for (int j = 0; j < ty; ++j)
+                for (int i = 0; i < tx; ++i) {
+                    Shape::TaskInfo t;
+                    t.bucket = (int)k;
+                    t.x0 = b.sx0 + (i * ow) / tx; t.x1 = b.sx0 + ((i + 1) * ow) / tx;
+                    t.y0 = b.sy0 + (j * oh) / ty; t.y1 = b.sy0 + ((j + 1) * oh) / ty;
+                    s.tasks.push_back(t);
+                }
+        }
+        s.taskStart.push_back((int)s.tasks.size());
+    }
     return true;
 }
 
@@ -743,6 +780,8 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
                 g.cby0 = (int16_t)clampv(bidx(nd->rb.lo.y - 1.0f, o.bucket_y), s.by0, s.by1 - 1);
                 g.cbx1 = (int16_t)clampv(bidx(nd->rb.hi.x + 1.0f, o.bucket_x), s.bx0, s.bx1 - 1);
                 g.cby1 = (int16_t)clampv(bidx(nd->rb.hi.y + 1.0f, o.bucket_y), s.by0, s.by1 - 1);
+                g.px0 = nd->rb.lo.x - 2.0f; g.py0 = nd->rb.lo.y - 2.0f;
+                g.px1 = nd->rb.hi.x + 2.0f; g.py1 = nd->rb.hi.y + 2.0f;
                 if ((g.cu + 1) * (g.cv + 1) > 514 || g.cu < 1 || g.cv < 1) {
                     err = "internal: grid size outside the dicing limits";
                     return false;
@@ -788,7 +827,8 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
             }
     for (int k = 0; k < nb; ++k) out.candStart[k + 1] += out.candStart[k];
     out.candGrid.assign(out.candStart[nb], 0);
-    out.capStart.assign(nb + 1, 0);
+    const int ntask = (int)s.tasks.size();
+    out.capStart.assign(ntask + 1, 0);
     std::vector<uint32_t> fill(out.candStart.begin(), out.candStart.end() - 1);
     for (uint32_t gi = 0; gi < out.grids.size(); ++gi) {
         const auto& g = out.grids[gi];
@@ -797,11 +837,19 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
                 int k = s.bucketIndex(bx, by);
                 if (k >= g.diceBucket) {
                     out.candGrid[fill[k]++] = gi;
-                    out.capStart[k + 1] += (uint64_t)(g.cu * g.cv);
+                    // A micropolygon enters a task's list only if its pixel
+                    // window meets the task, and its window lies inside the
+                    // grid's guarded bound.
+                    for (int t = s.taskStart[k]; t < s.taskStart[k + 1]; ++t) {
+                        const auto& T = s.tasks[t];
+                        if (g.px0 < (float)T.x1 && g.px1 > (float)T.x0 && g.py0 < (float)T.y1 &&
+                            g.py1 > (float)T.y0)
+                            out.capStart[t + 1] += (uint64_t)(g.cu * g.cv);
+                    }
                 }
             }
     }
-    for (int k = 0; k < nb; ++k) out.capStart[k + 1] += out.capStart[k];
+    for (int t = 0; t < ntask; ++t) out.capStart[t + 1] += out.capStart[t];
     return true;
 }
 

@@ -34,6 +34,16 @@ struct Shape {
     };
     std::vector<BucketInfo> buckets;
     int bucketIndex(int bx, int by) const { return (by - by0) * nbr + (bx - bx0); }
+    // Device work split: a wave block is blockW x blockH pixels (64 samples
+    // or one pixel); each bucket's owned region is cut into tasks of about
+    // 4x4 blocks, one sampling workgroup each.  Tasks are in bucket order.
+    int blockW = 1, blockH = 1;
+    struct TaskInfo {
+        int bucket;
+        int x0, y0, x1, y1;               // pixels, inside the bucket's owned region
+    };
+    std::vector<TaskInfo> tasks;
+    std::vector<int> taskStart;           // buckets + 1
 };
 
 bool make_shape(const rh_options& o, Shape& s, std::string& err);
@@ -56,6 +66,7 @@ struct GridRec {
     uint32_t mpBase, vtxBase, paramOff;
     int32_t diceBucket;                   // region bucket index processed when diced
     int16_t cbx0, cby0, cbx1, cby1;       // candidate bucket range (inclusive)
+    float px0, py0, px1, py1;             // raster bound with a 2-pixel guard band
 };
 
 struct Schedule {
@@ -67,7 +78,7 @@ struct Schedule {
     // Candidate grids per region bucket (CSR over the bucket list), in dicing order.
     std::vector<uint32_t> candStart;      // buckets + 1
     std::vector<uint32_t> candGrid;
-    std::vector<uint64_t> capStart;       // buckets + 1, per-bucket list capacity offsets
+    std::vector<uint64_t> capStart;       // tasks + 1, per-task list capacity offsets
 };
 
 // Parameter layouts in Schedule::params (all floats):

@@ -15,13 +15,13 @@
 
 extern "C" hipError_t rh_launch_dice(hipStream_t, const FrameDev*, const GridDev*, int, const float*, float4*, float*,
                                      float4*);
-extern "C" hipError_t rh_launch_bin(hipStream_t, const FrameDev*, const BucketDev*, int, const GridDev*,
-                                    const uint32_t*, const uint32_t*, const unsigned long long*, const float4*, uint2*,
-                                    uint32_t*);
-extern "C" hipError_t rh_launch_sample(hipStream_t, const FrameDev*, const BucketDev*, int,
-                                       const unsigned long long*, const uint32_t*, const uint2*, const float*,
+extern "C" hipError_t rh_launch_bin(hipStream_t, const FrameDev*, const BucketDev*, int, const TaskDev*,
+                                    const GridDev*, const uint32_t*, const uint32_t*, const unsigned long long*,
+                                    const float4*, uint32_t*, uint32_t*, uint32_t*, unsigned long long*);
+extern "C" hipError_t rh_launch_sample(hipStream_t, const FrameDev*, const BucketDev*, const TaskDev*, int,
+                                       const unsigned long long*, const uint32_t*, const uint32_t*, const float*,
                                        const GridDev*, const PrimDev*, const float*, const float*, const uchar4*,
-                                       float4*, unsigned long long*);
+                                       float4*, uint32_t*, const uint32_t*, int, unsigned long long*);
 extern "C" size_t rh_filter_tile_lds(int, int, int);
 extern "C" hipError_t rh_launch_box_masks(hipStream_t, const FrameDev*, size_t, const uchar4*, const float*,
                                           uint16_t*);
@@ -86,42 +86,27 @@ struct rh_ctx {
     rh::Schedule sched;
     bool scheduled = false;
     bool rendered = false;
-    int nblocks = 0;
     hipStream_t st = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // frame shape
-    DBuf dFrame, dBuckets, dBlocks, dPos2d, dPos1d, dPix, dDither, dFilter, dBoxMask;
+    DBuf dFrame, dBuckets, dTasks, dPos2d, dPos1d, dPix, dDither, dFilter, dBoxMask;
     bool tiledFilter = false;
     // schedule
     DBuf dGrids, dParams, dPrims, dCandStart, dCandGrid, dCapStart;
     // per frame
-    DBuf dVtx, dMprec, dMpbound, dLists, dListCount, dSamples, dChannels, dRgba, dCounters;
+    DBuf dVtx, dMprec, dMpbound, dLists, dListCount, dSamples, dChannels, dRgba, dCounters, dTaskCounts, dBucketRefs;
     rh_stats stats;
     ~rh_ctx() {
-        DBuf* all[] = {&dFrame, &dBuckets, &dBlocks, &dPos2d, &dPos1d, &dPix, &dDither, &dFilter, &dBoxMask,
+        DBuf* all[] = {&dFrame, &dBuckets, &dTasks, &dPos2d, &dPos1d, &dPix, &dDither, &dFilter, &dBoxMask,
                        &dGrids, &dParams,
                        &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dVtx, &dMprec, &dMpbound, &dLists,
-                       &dListCount, &dSamples, &dChannels, &dRgba, &dCounters};
+                       &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dTaskCounts, &dBucketRefs};
         for (DBuf* b : all) b->release();
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
 };
-
-static void block_shape(int ns, int& bw, int& bh) {
-    int ppw = 64 / ns;
-    if (ppw < 1) ppw = 1;
-    if ((ppw & (ppw - 1)) == 0) {
-        int lg = 0;
-        while ((1 << lg) < ppw) ++lg;
-        bw = 1 << ((lg + 1) / 2);
-        bh = ppw / bw;
-    } else {
-        bw = ppw;
-        bh = 1;
-    }
-}
 
 extern "C" {
 
@@ -175,36 +160,40 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
     f.bx0 = s.bx0; f.by0 = s.by0; f.bx1 = s.bx1; f.by1 = s.by1;
     f.extX0 = s.extX0; f.extY0 = s.extY0; f.extW = s.extW; f.extH = s.extH;
     f.ns = s.ns; f.nx = s.nx; f.ny = s.ny;
-    block_shape(s.ns, f.blockW, f.blockH);
+    f.blockW = s.blockW; f.blockH = s.blockH;
     f.exposureGain = opt->exposure_gain; f.exposureGamma = opt->exposure_gamma;
     f.qOne = opt->quantize_one; f.qMin = opt->quantize_min; f.qMax = opt->quantize_max; f.qDither = opt->quantize_dither;
 
     std::vector<BucketDev> bk;
-    std::vector<int32_t> blocks;
     for (size_t k = 0; k < s.buckets.size(); ++k) {
         const auto& b = s.buckets[k];
         BucketDev d;
         d.x = b.x; d.y = b.y; d.w = b.w; d.h = b.h;
         d.sx0 = b.sx0; d.sy0 = b.sy0; d.sx1 = b.sx1; d.sy1 = b.sy1;
         d.bx = s.bx0 + (int)(k % s.nbr); d.by = s.by0 + (int)(k / s.nbr);
-        d.pad0 = d.pad1 = 0;
+        d.task0 = s.taskStart[k];
+        d.ntask = s.taskStart[k + 1] - s.taskStart[k];
         bk.push_back(d);
-        for (int y = b.sy0; y < b.sy1; y += f.blockH)
-            for (int x = b.sx0; x < b.sx1; x += f.blockW) {
-                blocks.push_back((int32_t)k); blocks.push_back(x); blocks.push_back(y); blocks.push_back(0);
-            }
     }
-    c->nblocks = (int)(blocks.size() / 4);
+    std::vector<TaskDev> tk(s.tasks.size());
+    for (size_t t = 0; t < s.tasks.size(); ++t) {
+        const auto& T = s.tasks[t];
+        tk[t].bucket = T.bucket;
+        tk[t].x0 = T.x0; tk[t].y0 = T.y0; tk[t].x1 = T.x1; tk[t].y1 = T.y1;
+        tk[t].pad0 = tk[t].pad1 = tk[t].pad2 = 0;
+    }
     std::vector<FrameDev> fv(1, f);
-    std::vector<uint32_t> zero(5 * 2, 0);
+    std::vector<unsigned long long> zero(RH_NCOUNTERS, 0);
     if ((e = upload(c->dFrame, fv, c->st)) != hipSuccess || (e = upload(c->dBuckets, bk, c->st)) != hipSuccess ||
-        (e = upload(c->dBlocks, blocks, c->st)) != hipSuccess || (e = upload(c->dPos2d, c->plan.pos2d, c->st)) != hipSuccess ||
+        (e = upload(c->dTasks, tk, c->st)) != hipSuccess || (e = upload(c->dPos2d, c->plan.pos2d, c->st)) != hipSuccess ||
         (e = upload(c->dPos1d, c->plan.pos1d, c->st)) != hipSuccess || (e = upload(c->dPix, c->plan.pixPattern, c->st)) != hipSuccess ||
         (e = upload(c->dDither, c->plan.dither, c->st)) != hipSuccess || (e = upload(c->dFilter, s.filterTab, c->st)) != hipSuccess ||
         (e = c->dSamples.ensure((size_t)s.extW * s.extH * s.ns * 32)) != hipSuccess ||
         (e = c->dChannels.ensure((size_t)s.width() * s.height() * 9 * sizeof(float))) != hipSuccess ||
         (e = c->dRgba.ensure((size_t)s.width() * s.height() * 4)) != hipSuccess ||
-        (e = c->dListCount.ensure(s.buckets.size() * sizeof(uint32_t))) != hipSuccess ||
+        (e = c->dListCount.ensure(s.tasks.size() * sizeof(uint32_t))) != hipSuccess ||
+        (e = c->dTaskCounts.ensure(s.tasks.size() * 5 * sizeof(uint32_t))) != hipSuccess ||
+        (e = c->dBucketRefs.ensure(s.buckets.size() * sizeof(uint32_t))) != hipSuccess ||
         (e = upload(c->dCounters, zero, c->st)) != hipSuccess) {
         delete c;
         return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_DEVICE, std::string("device set-up: ") + hipGetErrorString(e));
@@ -280,7 +269,7 @@ static rh_status schedule(rh_ctx* c) {
     HIP_TRY(c->dVtx.ensure(S.vertices * sizeof(float4)));
     HIP_TRY(c->dMprec.ensure(S.micropolygons * RH_MPREC_FLOATS * sizeof(float)));
     HIP_TRY(c->dMpbound.ensure(S.micropolygons * sizeof(float4)));
-    HIP_TRY(c->dLists.ensure((size_t)S.capStart.back() * sizeof(uint2)));
+    HIP_TRY(c->dLists.ensure((size_t)S.capStart.back() * sizeof(uint32_t)));
     HIP_TRY(hipStreamSynchronize(c->st));
     c->stats.ms_upload = ms_since(t1);
     c->scheduled = true;
@@ -294,22 +283,23 @@ rh_status rh_render_device(rh_ctx* c) {
     const rh::Shape& s = c->shape;
     const FrameDev* f = c->dFrame.as<FrameDev>();
     const int nb = (int)s.buckets.size();
-    HIP_TRY(hipMemsetAsync(c->dCounters.p, 0, 5 * sizeof(unsigned long long), c->st));
+    HIP_TRY(hipMemsetAsync(c->dCounters.p, 0, RH_NCOUNTERS * sizeof(unsigned long long), c->st));
     HIP_TRY(hipEventRecord(c->ev[0], c->st));
     HIP_TRY(rh_launch_dice(c->st, f, c->dGrids.as<GridDev>(), (int)S.grids.size(), c->dParams.as<float>(),
                            c->dVtx.as<float4>(), c->dMprec.as<float>(), c->dMpbound.as<float4>()));
     HIP_TRY(hipEventRecord(c->ev[1], c->st));
-    HIP_TRY(rh_launch_bin(c->st, f, c->dBuckets.as<BucketDev>(), nb, c->dGrids.as<GridDev>(),
-                          c->dCandStart.as<uint32_t>(), c->dCandGrid.as<uint32_t>(),
-                          c->dCapStart.as<unsigned long long>(), c->dMpbound.as<float4>(), c->dLists.as<uint2>(),
-                          c->dListCount.as<uint32_t>()));
+    HIP_TRY(rh_launch_bin(c->st, f, c->dBuckets.as<BucketDev>(), nb, c->dTasks.as<TaskDev>(),
+                          c->dGrids.as<GridDev>(), c->dCandStart.as<uint32_t>(), c->dCandGrid.as<uint32_t>(),
+                          c->dCapStart.as<unsigned long long>(), c->dMpbound.as<float4>(),
+                          c->dLists.as<uint32_t>(), c->dListCount.as<uint32_t>(), c->dBucketRefs.as<uint32_t>(),
+                          c->dCounters.as<unsigned long long>()));
     HIP_TRY(hipEventRecord(c->ev[2], c->st));
-    HIP_TRY(rh_launch_sample(c->st, f, c->dBuckets.as<BucketDev>(), nb,
+    HIP_TRY(rh_launch_sample(c->st, f, c->dBuckets.as<BucketDev>(), c->dTasks.as<TaskDev>(), (int)s.tasks.size(),
                              c->dCapStart.as<unsigned long long>(), c->dListCount.as<uint32_t>(),
-                             c->dLists.as<uint2>(), c->dMprec.as<float>(), c->dGrids.as<GridDev>(),
+                             c->dLists.as<uint32_t>(), c->dMprec.as<float>(), c->dGrids.as<GridDev>(),
                              c->dPrims.as<PrimDev>(), c->dPos2d.as<float>(), c->dPos1d.as<float>(),
-                             c->dPix.as<uchar4>(), c->dSamples.as<float4>(),
-                             c->dCounters.as<unsigned long long>()));
+                             c->dPix.as<uchar4>(), c->dSamples.as<float4>(), c->dTaskCounts.as<uint32_t>(),
+                             c->dBucketRefs.as<uint32_t>(), nb, c->dCounters.as<unsigned long long>()));
     HIP_TRY(hipEventRecord(c->ev[3], c->st));
     HIP_TRY(rh_launch_filter(c->st, f, s.width(), s.height(), s.ns, s.shiftX, s.shiftY, c->dSamples.as<float4>(),
                              c->tiledFilter ? c->dBoxMask.as<uint16_t>() : nullptr, c->dPix.as<uchar4>(),
@@ -317,7 +307,7 @@ rh_status rh_render_device(rh_ctx* c) {
                              c->dChannels.as<float>(), c->dRgba.as<uchar4>()));
     HIP_TRY(hipEventRecord(c->ev[4], c->st));
     HIP_TRY(hipEventSynchronize(c->ev[4]));
-    unsigned long long cnt[5];
+    unsigned long long cnt[RH_NCOUNTERS];
     HIP_TRY(hipMemcpy(cnt, c->dCounters.p, sizeof(cnt), hipMemcpyDeviceToHost));
     float t[4];
     for (int i = 0; i < 4; ++i) HIP_TRY(hipEventElapsedTime(&t[i], c->ev[i], c->ev[i + 1]));
@@ -337,11 +327,10 @@ rh_status rh_render_device(rh_ctx* c) {
     float total = 0;
     HIP_TRY(hipEventElapsedTime(&total, c->ev[0], c->ev[4]));
     c->stats.ms_device = total;
-    std::vector<uint32_t> lc(nb);
-    HIP_TRY(hipMemcpy(lc.data(), c->dListCount.p, nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    uint64_t refs = 0;
-    for (uint32_t v : lc) refs += v;
-    c->stats.mp_bucket_refs = refs;
+    c->stats.mp_bucket_refs = cnt[6];
+    if (cnt[5] != 0)
+        return fail(RH_ERR_CAPACITY, "internal: a task's micropolygon list exceeded its capacity (" +
+                                         std::to_string(cnt[5]) + " tasks)");
     c->rendered = true;
     return RH_OK;
 }
@@ -430,7 +419,7 @@ rh_status rh_estimate(const rh_options* opt, const rh_prim* prims, int32_t n, rh
     out->list_capacity = S.capStart.back();
     out->device_bytes = frame_bytes(s) + S.grids.size() * sizeof(GridDev) + S.params.size() * 4 +
                         (uint64_t)n * sizeof(PrimDev) + S.candGrid.size() * 4 + S.vertices * 16 +
-                        S.micropolygons * (RH_MPREC_FLOATS * 4 + 16) + S.capStart.back() * 8;
+                        S.micropolygons * (RH_MPREC_FLOATS * 4 + 16) + S.capStart.back() * 4;
     return RH_OK;
 }
 

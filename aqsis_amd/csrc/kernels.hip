@@ -264,21 +264,38 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Binning: the ordered micropolygon list of each bucket
+// Binning: the ordered micropolygon list of each sampling task
 // ---------------------------------------------------------------------------
+// One workgroup per bucket.  Every micropolygon of the bucket's candidate
+// grids (in dicing order) goes through CqImageBuffer::AddMPG's tests for this
+// bucket; a kept one is appended, in order, to the list of each of the
+// bucket's tasks that its clipped pixel window meets.  Lists never exceed the
+// host's capacities; an overflow is counted in counters[5].
+#define RH_MAX_TASKS 16
 __global__ void __launch_bounds__(256) rh_bin(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets,
-                                              const GridDev* __restrict__ grids, const uint32_t* __restrict__ candStart,
+                                              const TaskDev* __restrict__ tasks, const GridDev* __restrict__ grids,
+                                              const uint32_t* __restrict__ candStart,
                                               const uint32_t* __restrict__ candGrid,
                                               const unsigned long long* __restrict__ capStart,
-                                              const float4* __restrict__ mpbound, uint2* __restrict__ lists,
-                                              uint32_t* __restrict__ listCount) {
-    __shared__ uint32_t waveTotals[4];
+                                              const float4* __restrict__ mpbound, uint32_t* __restrict__ lists,
+                                              uint32_t* __restrict__ listCount, uint32_t* __restrict__ bucketRefs,
+                                              unsigned long long* __restrict__ counters) {
+    __shared__ uint32_t wtot[4][RH_MAX_TASKS];
+    __shared__ uint32_t wrefs[4];
+    __shared__ uint32_t tcount[RH_MAX_TASKS];
+    __shared__ int trect[RH_MAX_TASKS][4];
     const int b = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const BucketDev B = buckets[b];
-    const int bx = B.bx, by = B.by;
-    uint2* out = lists + capStart[b];
-    uint32_t count = 0;
+    const int bx = B.bx, by = B.by, nt = B.ntask, t0 = B.task0;
+    if (tid < nt) {
+        const TaskDev T = tasks[t0 + tid];
+        tcount[tid] = 0;
+        trect[tid][0] = T.x0 - B.sx0; trect[tid][1] = T.x1 - B.sx0;
+        trect[tid][2] = T.y0 - B.sy0; trect[tid][3] = T.y1 - B.sy0;
+    }
+    __syncthreads();
+    uint32_t refs = 0;
     const float hfx = fr->filterX * 0.5f, hfy = fr->filterY * 0.5f;   // xFiltSize / 2.0f
     const float dsx = (float)fr->shiftX, dsy = (float)fr->shiftY;      // lfloor(xFiltSize / 2.0f)
     for (uint32_t c = candStart[b]; c < candStart[b + 1]; ++c) {
@@ -287,7 +304,7 @@ __global__ void __launch_bounds__(256) rh_bin(const FrameDev* __restrict__ fr, c
         for (int base = 0; base < nmp; base += 256) {
             const int m = base + tid;
             bool keep = false;
-            uint32_t win = 0;
+            int sX = 0, eX = 0, sY = 0, eY = 0;
             if (m < nmp) {
                 const float4 bd = mpbound[G.mpBase + m];  // bminx, bmaxx, bminy, bmaxy
                 // CqImageBuffer::AddMPG (imagebuffer.cpp:514-591)
@@ -306,93 +323,147 @@ __global__ void __launch_bounds__(256) rh_bin(const FrameDev* __restrict__ fr, c
                         if (bx >= iXBa && bx <= iXBb && by >= iYBa && by <= iYBb) {
                             // RenderMPG_Static pixel range clipped to the sample region
                             // (bucketprocessor.cpp:1120-1140); empty ranges do no work.
-                            int eX = (int)lceil_f(bd.y), eY = (int)lceil_f(bd.w);
+                            eX = (int)lceil_f(bd.y); eY = (int)lceil_f(bd.w);
                             if (eX > B.sx1) eX = B.sx1;
                             if (eY > B.sy1) eY = B.sy1;
-                            int sX = floor_i(bd.x), sY = floor_i(bd.z);
+                            sX = floor_i(bd.x); sY = floor_i(bd.z);
                             if (sY < B.sy0) sY = B.sy0;
                             if (sX < B.sx0) sX = B.sx0;
-                            if (sX < eX && sY < eY) {
-                                keep = true;
-                                win = (uint32_t)(sX - B.sx0) | ((uint32_t)(eX - B.sx0) << 8) |
-                                      ((uint32_t)(sY - B.sy0) << 16) | ((uint32_t)(eY - B.sy0) << 24);
-                            }
+                            keep = sX < eX && sY < eY;
+                            sX -= B.sx0; eX -= B.sx0; sY -= B.sy0; eY -= B.sy0;
                         }
                     }
                 }
             }
-            // Stable block-wide compaction.
-            unsigned long long mask = __ballot(keep);
-            uint32_t before = __popcll(mask & ((1ull << lane) - 1ull));
-            if (lane == 0) waveTotals[wave] = (uint32_t)__popcll(mask);
-            __syncthreads();
-            uint32_t waveBase = 0, total = 0;
-            for (int w = 0; w < 4; ++w) {
-                if (w < wave) waveBase += waveTotals[w];
-                total += waveTotals[w];
+            refs += (uint32_t)__popcll(__ballot(keep));
+            // Stable workgroup-wide compaction into each task's list.
+            for (int t = 0; t < nt; ++t) {
+                const bool ok = keep && sX < trect[t][1] && eX > trect[t][0] && sY < trect[t][3] && eY > trect[t][2];
+                const unsigned long long msk = __ballot(ok);
+                if (lane == 0) wtot[wave][t] = (uint32_t)__popcll(msk);
             }
-            if (keep) out[count + waveBase + before] = make_uint2(G.mpBase + (uint32_t)m, win);
-            count += total;
+            __syncthreads();
+            for (int t = 0; t < nt; ++t) {
+                const bool ok = keep && sX < trect[t][1] && eX > trect[t][0] && sY < trect[t][3] && eY > trect[t][2];
+                const unsigned long long msk = __ballot(ok);
+                if (ok) {
+                    uint32_t pos = tcount[t] + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
+                    for (int w = 0; w < wave; ++w) pos += wtot[w][t];
+                    const unsigned long long c0 = capStart[t0 + t];
+                    if ((unsigned long long)pos < capStart[t0 + t + 1] - c0) lists[c0 + pos] = G.mpBase + (uint32_t)m;
+                }
+            }
+            __syncthreads();
+            if (tid < nt) tcount[tid] += wtot[0][tid] + wtot[1][tid] + wtot[2][tid] + wtot[3][tid];
             __syncthreads();
         }
     }
-    if (tid == 0) listCount[b] = count;
+    if (tid < nt) {
+        const unsigned long long cap = capStart[t0 + tid + 1] - capStart[t0 + tid];
+        const uint32_t cnt = tcount[tid];
+        listCount[t0 + tid] = (unsigned long long)cnt < cap ? cnt : (uint32_t)cap;
+        if ((unsigned long long)cnt > cap) atomicAdd(&counters[5], 1ull);
+    }
+    if (lane == 0) wrefs[wave] = refs;
+    __syncthreads();
+    if (tid == 0) bucketRefs[b] = wrefs[0] + wrefs[1] + wrefs[2] + wrefs[3];
+}
+
+// Frame counters from the per-task and per-bucket rows: counters[0..4] the
+// hider's SPL counts, counters[6] the micropolygon-bucket references.
+// One workgroup.
+__global__ void __launch_bounds__(1024) rh_reduce_counters(const uint32_t* __restrict__ taskCounts, int ntasks,
+                                                           const uint32_t* __restrict__ bucketRefs, int nbuckets,
+                                                           unsigned long long* __restrict__ counters) {
+    __shared__ unsigned long long part[6][32];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    unsigned long long acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int t = tid; t < ntasks; t += 1024)
+        for (int c = 0; c < 5; ++c) acc[c] += taskCounts[(size_t)t * 5 + c];
+    for (int b = tid; b < nbuckets; b += 1024) acc[5] += bucketRefs[b];
+    for (int c = 0; c < 6; ++c) {
+        unsigned long long v = acc[c];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) part[c][wave] = v;
+    }
+    __syncthreads();
+    if (tid < 6) {
+        unsigned long long v = 0;
+        for (int w = 0; w < 16; ++w) v += part[tid][w];
+        counters[tid < 5 ? tid : 6] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Sampling: per-sample visibility in the reference's micropolygon order
 // ---------------------------------------------------------------------------
-// One 1024-thread workgroup per bucket.  The bucket's ordered micropolygon
-// list is staged through LDS in chunks of RH_SB_CHUNK hit-test records.
-// Each wave owns up to RH_SB_SLOTS blocks of 64 samples (lane = one sample).
-// For every chunk and block, the wave compacts the micropolygons whose
-// candidate rectangle meets the block, each lane builds a bit mask of the
-// ones whose candidate rectangle holds its own sample, and lanes then pop
-// their own candidates in list order.  The candidate rectangle is the exact
-// set of samples RenderMPG_Static visits (bucketprocessor.cpp:1120-1216):
-// the pixel range [floor(bminx), lceil(bmaxx)) clipped to the sample region
-// with the first/last pixel sub-sample clip, expressed in sub-sample grid
-// units, so each sample sees exactly the reference's sequence of tests.
-#define RH_SB_CHUNK 256
-#define RH_SB_WAVES 16
-#define RH_SB_SLOTS 4
+// One 256-thread workgroup per task (a part of a bucket's owned sample
+// region).  The task's ordered micropolygon list is staged through LDS in
+// chunks of RH_S_CH hit-test records.  Each wave owns up to RH_S_SLOTS blocks
+// of 64 samples (lane = one sample).  For every chunk and block the wave
+// compacts the micropolygons whose candidate rectangle meets the block; then
+//  (a) for each of them in turn, every lane tests whether its sample is a
+//      candidate and lies in the micropolygon's bound (order-free tests,
+//      uniform across the wave), keeping a bit mask of the bound hits;
+//  (b) each lane pops its bound hits in list order and runs the
+//      order-dependent part: the occlusion skip against its current depth,
+//      point-in-polygon, inverse bilinear, depth and the store test.
+// The candidate rectangle is the exact set of samples RenderMPG_Static visits
+// (bucketprocessor.cpp:1120-1216): the pixel range [floor(bminx),
+// lceil(bmaxx)) clipped to the bucket's sample region with the first/last
+// pixel sub-sample clip, in sub-sample grid units, so every sample sees the
+// reference's sequence of tests.
+#define RH_S_CH 224
+#define RH_S_WAVES 4
+#define RH_S_SLOTS 4
+#define RH_S_GROUP 16
 
-__global__ void __launch_bounds__(1024) rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets,
-                                                  const unsigned long long* __restrict__ capStart,
-                                                  const uint32_t* __restrict__ listCount, const uint2* __restrict__ lists,
-                                                  const float4* __restrict__ mprec4, const GridDev* __restrict__ grids,
-                                                  const PrimDev* __restrict__ prims, const float* __restrict__ pos2d,
-                                                  const float* __restrict__ pos1d, const uchar4* __restrict__ pixPattern,
-                                                  float4* __restrict__ samples, unsigned long long* __restrict__ counters) {
-    __shared__ float4 recs[RH_SB_CHUNK * 9];
-    __shared__ uint2 meta[RH_SB_CHUNK];
-    __shared__ uint32_t ids[RH_SB_CHUNK];
-    __shared__ uint16_t matched[RH_SB_WAVES][RH_SB_CHUNK];
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets,
+                                                 const TaskDev* __restrict__ tasks, int ntasks,
+                                                 const unsigned long long* __restrict__ capStart,
+                                                 const uint32_t* __restrict__ listCount,
+                                                 const uint32_t* __restrict__ lists,
+                                                 const float4* __restrict__ mprec4, const GridDev* __restrict__ grids,
+                                                 const PrimDev* __restrict__ prims, const float* __restrict__ pos2d,
+                                                 const float* __restrict__ pos1d, const uchar4* __restrict__ pixPattern,
+                                                 float4* __restrict__ samples, uint32_t* __restrict__ taskCounts) {
+    __shared__ float4 recs[RH_S_CH * 9];
+    __shared__ uint32_t wsum[RH_S_WAVES][5];
+    __shared__ uint2 meta[RH_S_CH];
+    __shared__ uint32_t ids[RH_S_CH];
+    __shared__ uint8_t matched[RH_S_WAVES][RH_S_CH];
+    // Workgroups are dealt round-robin to the 8 XCDs; runs of RH_S_GROUP
+    // consecutive tasks (neighbours sharing micropolygons) go to one XCD.
+    // The grid is a multiple of 8 * RH_S_GROUP workgroups.
+    const int kx = blockIdx.x >> 3;
+    const int task = ((kx / RH_S_GROUP) * 8 + (blockIdx.x & 7)) * RH_S_GROUP + kx % RH_S_GROUP;
+    if (task >= ntasks) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const BucketDev B = buckets[blockIdx.x];
+    const TaskDev T = tasks[task];
+    const BucketDev B = buckets[T.bucket];
     const int ns = fr->ns, nx = fr->nx, ny = fr->ny, bw = fr->blockW, bh = fr->blockH;
-    const int ow = B.sx1 - B.sx0, oh = B.sy1 - B.sy0;
-    const int nbx = (ow + bw - 1) / bw, nby = (oh + bh - 1) / bh, nblk = nbx * nby;
+    const int tw = T.x1 - T.x0, th = T.y1 - T.y0;
+    const int nbx = (tw + bw - 1) / bw, nby = (th + bh - 1) / bh, nblk = nbx * nby;
     const int pix = lane / ns, si = lane - pix * ns;
     const int lx = pix % bw, ly = pix / bw;
     const int sm = si % nx, sn = si / nx;
-    const uint2* L = lists + capStart[blockIdx.x];
-    const uint32_t n = listCount[blockIdx.x];
+    const uint32_t* L = lists + capStart[task];
+    const uint32_t n = listCount[task];
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
 
-    for (int p0 = 0; p0 < nblk; p0 += RH_SB_WAVES * RH_SB_SLOTS) {
-        bool has[RH_SB_SLOTS], live[RH_SB_SLOTS];
-        int gx[RH_SB_SLOTS], gy[RH_SB_SLOTS], bgx[RH_SB_SLOTS], bgy[RH_SB_SLOTS];
-        float sx[RH_SB_SLOTS], sy[RH_SB_SLOTS], lod[RH_SB_SLOTS];
-        float occlZ[RH_SB_SLOTS], wu[RH_SB_SLOTS], wv[RH_SB_SLOTS];
-        uint32_t win[RH_SB_SLOTS];
+    for (int p0 = 0; p0 < nblk; p0 += RH_S_WAVES * RH_S_SLOTS) {
+        bool has[RH_S_SLOTS], live[RH_S_SLOTS], lodOk[RH_S_SLOTS];
+        int gx[RH_S_SLOTS], gy[RH_S_SLOTS], bgx[RH_S_SLOTS], bgy[RH_S_SLOTS];
+        float sx[RH_S_SLOTS], sy[RH_S_SLOTS];
+        float occlZ[RH_S_SLOTS], wu[RH_S_SLOTS], wv[RH_S_SLOTS];
+        uint32_t win[RH_S_SLOTS];
 #pragma unroll
-        for (int k = 0; k < RH_SB_SLOTS; ++k) {
-            const int blk = p0 + wave + RH_SB_WAVES * k;
+        for (int k = 0; k < RH_S_SLOTS; ++k) {
+            const int blk = p0 + wave + RH_S_WAVES * k;
             has[k] = blk < nblk;
-            const int bx = B.sx0 + (blk % nbx) * bw, by = B.sy0 + (blk / nbx) * bh;
+            const int bx = T.x0 + (blk % nbx) * bw, by = T.y0 + (blk / nbx) * bh;
             const int px = bx + lx, py = by + ly;
-            live[k] = has[k] && pix < bw * bh && px < B.sx1 && py < B.sy1;
+            live[k] = has[k] && pix < bw * bh && px < T.x1 && py < T.y1;
             gx[k] = (px - B.sx0) * nx + sm;
             gy[k] = (py - B.sy0) * ny + sn;
             bgx[k] = (bx - B.sx0) * nx;
@@ -403,28 +474,29 @@ __global__ void __launch_bounds__(1024) rh_sample(const FrameDev* __restrict__ f
             win[k] = 0xffffffffu;
             sx[k] = 0.f;
             sy[k] = 0.f;
-            lod[k] = 0.f;
+            lodOk[k] = false;
             if (live[k]) {
                 const uchar4 pat = pixPattern[(size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)];
                 // position = (x, y) + pattern position (imagepixel.cpp:353)
                 sx[k] = (float)px + pos2d[2 * ((size_t)pat.x * ns + si)];
                 sy[k] = (float)py + pos2d[2 * ((size_t)pat.x * ns + si) + 1];
-                lod[k] = pos1d[(size_t)pat.z * ns + si];
+                const float lod = pos1d[(size_t)pat.z * ns + si];
+                // LevelOfDetailBounds [0, 1] (bucketprocessor.cpp:1183-1189)
+                lodOk[k] = !(0.0f > lod || lod >= 1.0f);
             }
         }
-        for (uint32_t c0 = 0; c0 < n; c0 += RH_SB_CHUNK) {
-            const int cnt = (int)((n - c0) < (uint32_t)RH_SB_CHUNK ? (n - c0) : (uint32_t)RH_SB_CHUNK);
+        for (uint32_t c0 = 0; c0 < n; c0 += RH_S_CH) {
+            const int cnt = (int)((n - c0) < (uint32_t)RH_S_CH ? (n - c0) : (uint32_t)RH_S_CH);
             __syncthreads();
-            for (int t = tid; t < cnt * 9; t += blockDim.x) {
-                const int r = t / 9, q = t - r * 9;
-                const uint32_t id = L[c0 + r].x;
-                recs[t] = mprec4[(size_t)id * 9 + q];
-                if (q == 0) ids[r] = id;
-            }
-            __syncthreads();
-            for (int t = tid; t < cnt; t += blockDim.x) {
-                const float4 bd = recs[t * 9 + 7];
-                const float bminx = bd.x, bmaxx = bd.y, bminy = bd.z, bmaxy = bd.w;
+            if (tid < cnt) {
+                const uint32_t id = L[c0 + tid];
+                float4 r[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) r[q] = mprec4[(size_t)id * 9 + q];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) recs[tid * 9 + q] = r[q];
+                ids[tid] = id;
+                const float bminx = r[7].x, bmaxx = r[7].y, bminy = r[7].z, bmaxy = r[7].w;
                 int eX = (int)lceil_f(bmaxx), eY = (int)lceil_f(bmaxy);
                 if (eX > B.sx1) eX = B.sx1;
                 if (eY > B.sy1) eY = B.sy1;
@@ -442,11 +514,11 @@ __global__ void __launch_bounds__(1024) rh_sample(const FrameDev* __restrict__ f
                     gy0 = (uint32_t)((sY - B.sy0) * ny + in);
                     gy1 = (uint32_t)((eY - 1 - B.sy0) * ny + en);
                 }
-                meta[t] = make_uint2(gx0 | (gx1 << 16), gy0 | (gy1 << 16));
+                meta[tid] = make_uint2(gx0 | (gx1 << 16), gy0 | (gy1 << 16));
             }
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < RH_SB_SLOTS; ++k) {
+            for (int k = 0; k < RH_S_SLOTS; ++k) {
                 if (!has[k]) continue;  // uniform across the wave
                 const int BX0 = bgx[k], BX1 = bgx[k] + bw * nx, BY0 = bgy[k], BY1 = bgy[k] + bh * ny;
                 // Ordered list of the chunk's micropolygons that meet this block.
@@ -460,7 +532,7 @@ __global__ void __launch_bounds__(1024) rh_sample(const FrameDev* __restrict__ f
                              (int)(mt.y >> 16) > BY0;
                     }
                     const unsigned long long msk = __ballot(ok);
-                    if (ok) matched[wave][nm + __popcll(msk & ((1ull << lane) - 1ull))] = (uint16_t)j;
+                    if (ok) matched[wave][nm + __popcll(msk & ((1ull << lane) - 1ull))] = (uint8_t)j;
                     nm += __popcll(msk);
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -468,27 +540,31 @@ __global__ void __launch_bounds__(1024) rh_sample(const FrameDev* __restrict__ f
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 for (int i0 = 0; i0 < nm; i0 += 64) {
                     const int ni = (nm - i0) < 64 ? (nm - i0) : 64;
-                    unsigned long long cm = 0ull;
+                    // (a) candidate and bound tests, one micropolygon at a time.
+                    unsigned long long pb = 0ull;
                     for (int i = 0; i < ni; ++i) {
-                        const uint2 mt = meta[matched[wave][i0 + i]];
-                        const bool cand = (uint32_t)gx[k] >= (mt.x & 0xffffu) && (uint32_t)gx[k] < (mt.x >> 16) &&
-                                          (uint32_t)gy[k] >= (mt.y & 0xffffu) && (uint32_t)gy[k] < (mt.y >> 16);
-                        cm |= (unsigned long long)(cand && live[k]) << i;
+                        const int j = __builtin_amdgcn_readfirstlane((int)matched[wave][i0 + i]);
+                        const uint2 mt = meta[j];
+                        const float4 bnd = recs[j * 9 + 7];
+                        const bool cand = live[k] && (uint32_t)gx[k] >= (mt.x & 0xffffu) &&
+                                          (uint32_t)gx[k] < (mt.x >> 16) && (uint32_t)gy[k] >= (mt.y & 0xffffu) &&
+                                          (uint32_t)gy[k] < (mt.y >> 16);
+                        nTest += cand ? 1u : 0u;
+                        // CqBound::Contains2D (bound.h:144-151)
+                        const bool in = cand && lodOk[k] &&
+                                        !(sx[k] < bnd.x || sx[k] > bnd.y || sy[k] < bnd.z || sy[k] > bnd.w);
+                        pb |= (unsigned long long)in << i;
                     }
-                    nTest += (uint32_t)__popcll(cm);
-                    while (__ballot(cm != 0ull)) {
-                        if (cm == 0ull) continue;
-                        const int i = __ffsll((long long)cm) - 1;
-                        cm &= cm - 1ull;
+                    // (b) the order-dependent tests, each lane its own hits in list order.
+                    while (__ballot(pb != 0ull)) {
+                        if (pb == 0ull) continue;
+                        const int i = __ffsll((long long)pb) - 1;
+                        pb &= pb - 1ull;
                         const int j = matched[wave][i0 + i];
                         const float4* R = &recs[j * 9];
-                        const float4 bnd = R[7], r8 = R[8];
-                        // CqBound::Contains2D (bound.h:144-151)
-                        if (sx[k] < bnd.x || sx[k] > bnd.y || sy[k] < bnd.z || sy[k] > bnd.w) continue;
+                        const float4 r8 = R[8];
                         // occlusion skip against the current opaque hit (bucketprocessor.cpp:1179)
                         if (r8.x > occlZ[k]) continue;
-                        // LevelOfDetailBounds [0, 1] (bucketprocessor.cpp:1183-1189)
-                        if (0.0f > lod[k] || lod[k] >= 1.0f) continue;
                         ++nBound;
                         // fContains (micropolygon.cpp:1295-1342): edges 0,1 reject on <= 0, edges 2,3 on < 0
                         const float4 YM = R[0], XM = R[1], X = R[2], Y = R[3];
@@ -540,7 +616,7 @@ __global__ void __launch_bounds__(1024) rh_sample(const FrameDev* __restrict__ f
             }
         }
 #pragma unroll
-        for (int k = 0; k < RH_SB_SLOTS; ++k) {
+        for (int k = 0; k < RH_S_SLOTS; ++k) {
             if (!live[k]) continue;
             const int px = B.sx0 + gx[k] / nx, py = B.sy0 + gy[k] / ny;
             const size_t sidx = ((size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)) * ns + si;
@@ -565,7 +641,9 @@ __global__ void __launch_bounds__(1024) rh_sample(const FrameDev* __restrict__ f
             samples[2 * sidx + 1] = b;
         }
     }
-    // Wave-level reduction of the hider counters.
+    // The task's hider counters: wave sums, then one row per task (summed
+    // over tasks by rh_reduce_counters; same-address atomics from every wave
+    // would serialise in L2).
     for (int off = 32; off > 0; off >>= 1) {
         nTest += __shfl_xor(nTest, off);
         nBound += __shfl_xor(nBound, off);
@@ -574,11 +652,14 @@ __global__ void __launch_bounds__(1024) rh_sample(const FrameDev* __restrict__ f
         nIn2 += __shfl_xor(nIn2, off);
     }
     if (lane == 0) {
-        atomicAdd(&counters[0], (unsigned long long)nTest);
-        atomicAdd(&counters[1], (unsigned long long)nBound);
-        atomicAdd(&counters[2], (unsigned long long)nHit);
-        atomicAdd(&counters[3], (unsigned long long)nIn);
-        atomicAdd(&counters[4], (unsigned long long)nIn2);
+        wsum[wave][0] = nTest; wsum[wave][1] = nBound; wsum[wave][2] = nHit; wsum[wave][3] = nIn;
+        wsum[wave][4] = nIn2;
+    }
+    __syncthreads();
+    if (tid < 5) {
+        uint32_t v = 0;
+        for (int w = 0; w < RH_S_WAVES; ++w) v += wsum[w][tid];
+        taskCounts[(size_t)task * 5 + tid] = v;
     }
 }
 
@@ -847,25 +928,34 @@ extern "C" hipError_t rh_launch_dice(hipStream_t st, const FrameDev* fr, const G
 }
 
 extern "C" hipError_t rh_launch_bin(hipStream_t st, const FrameDev* fr, const BucketDev* buckets, int nbuckets,
-                                    const GridDev* grids, const uint32_t* candStart, const uint32_t* candGrid,
-                                    const unsigned long long* capStart, const float4* mpbound, uint2* lists,
-                                    uint32_t* listCount) {
-    hipLaunchKernelGGL(rh_bin, dim3(nbuckets), dim3(256), 0, st, fr, buckets, grids, candStart, candGrid, capStart,
-                       mpbound, lists, listCount);
+                                    const TaskDev* tasks, const GridDev* grids, const uint32_t* candStart,
+                                    const uint32_t* candGrid, const unsigned long long* capStart,
+                                    const float4* mpbound, uint32_t* lists, uint32_t* listCount,
+                                    uint32_t* bucketRefs, unsigned long long* counters) {
+    if (nbuckets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rh_bin, dim3(nbuckets), dim3(256), 0, st, fr, buckets, tasks, grids, candStart, candGrid,
+                       capStart, mpbound, lists, listCount, bucketRefs, counters);
     return hipGetLastError();
 }
 
-extern "C" hipError_t rh_launch_sample(hipStream_t st, const FrameDev* fr, const BucketDev* buckets, int nbuckets,
-                                       const unsigned long long* capStart, const uint32_t* listCount,
-                                       const uint2* lists, const float* mprec, const GridDev* grids,
-                                       const PrimDev* prims, const float* pos2d, const float* pos1d,
-                                       const uchar4* pixPattern, float4* samples, unsigned long long* counters) {
-    hipLaunchKernelGGL(rh_sample, dim3(nbuckets), dim3(1024), 0, st, fr, buckets, capStart, listCount, lists,
-                       reinterpret_cast<const float4*>(mprec), grids, prims, pos2d, pos1d, pixPattern, samples,
+extern "C" hipError_t rh_launch_sample(hipStream_t st, const FrameDev* fr, const BucketDev* buckets,
+                                       const TaskDev* tasks, int ntasks, const unsigned long long* capStart,
+                                       const uint32_t* listCount, const uint32_t* lists, const float* mprec,
+                                       const GridDev* grids, const PrimDev* prims, const float* pos2d,
+                                       const float* pos1d, const uchar4* pixPattern, float4* samples,
+                                       uint32_t* taskCounts, const uint32_t* bucketRefs, int nbuckets,
+                                       unsigned long long* counters) {
+    if (ntasks > 0) {
+        const int unit = 8 * RH_S_GROUP;
+        const int nwg = (ntasks + unit - 1) / unit * unit;
+        hipLaunchKernelGGL(rh_sample, dim3(nwg), dim3(256), 0, st, fr, buckets, tasks, ntasks, capStart, listCount,
+                           lists, reinterpret_cast<const float4*>(mprec), grids, prims, pos2d, pos1d, pixPattern,
+                           samples, taskCounts);
+    }
+    hipLaunchKernelGGL(rh_reduce_counters, dim3(1), dim3(1024), 0, st, taskCounts, ntasks, bucketRefs, nbuckets,
                        counters);
     return hipGetLastError();
 }
-
 // LDS bytes of one tiled-filter workgroup, or 0 if the tiled kernel cannot
 // take this frame shape (mask bits or LDS size exceeded).
 extern "C" size_t rh_filter_tile_lds(int ns, int xm, int ym) {
