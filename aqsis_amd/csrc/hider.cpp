@@ -21,7 +21,9 @@ extern "C" hipError_t rh_launch_bin(hipStream_t, const FrameDev*, const BucketDe
 extern "C" hipError_t rh_launch_sample(hipStream_t, const FrameDev*, const BucketDev*, const TaskDev*, int,
                                        const unsigned long long*, const uint32_t*, const uint32_t*, const float*,
                                        const GridDev*, const PrimDev*, const float*, const float*, const uchar4*,
-                                       float4*, uint32_t*, const uint32_t*, int, unsigned long long*);
+                                       float4*, uint32_t*);
+extern "C" hipError_t rh_launch_reduce_counters(hipStream_t, const uint32_t*, int, const uint32_t*, int,
+                                                unsigned long long*);
 extern "C" size_t rh_filter_tile_lds(int, int, int);
 extern "C" hipError_t rh_launch_box_masks(hipStream_t, const FrameDev*, size_t, const uchar4*, const float*,
                                           uint16_t*);
@@ -298,9 +300,10 @@ rh_status rh_render_device(rh_ctx* c) {
                              c->dCapStart.as<unsigned long long>(), c->dListCount.as<uint32_t>(),
                              c->dLists.as<uint32_t>(), c->dMprec.as<float>(), c->dGrids.as<GridDev>(),
                              c->dPrims.as<PrimDev>(), c->dPos2d.as<float>(), c->dPos1d.as<float>(),
-                             c->dPix.as<uchar4>(), c->dSamples.as<float4>(), c->dTaskCounts.as<uint32_t>(),
-                             c->dBucketRefs.as<uint32_t>(), nb, c->dCounters.as<unsigned long long>()));
+                             c->dPix.as<uchar4>(), c->dSamples.as<float4>(), c->dTaskCounts.as<uint32_t>()));
     HIP_TRY(hipEventRecord(c->ev[3], c->st));
+    HIP_TRY(rh_launch_reduce_counters(c->st, c->dTaskCounts.as<uint32_t>(), (int)s.tasks.size(),
+                                      c->dBucketRefs.as<uint32_t>(), nb, c->dCounters.as<unsigned long long>()));
     HIP_TRY(rh_launch_filter(c->st, f, s.width(), s.height(), s.ns, s.shiftX, s.shiftY, c->dSamples.as<float4>(),
                              c->tiledFilter ? c->dBoxMask.as<uint16_t>() : nullptr, c->dPix.as<uchar4>(),
                              c->dPos2d.as<float>(), c->dFilter.as<float>(), c->dDither.as<float>(),

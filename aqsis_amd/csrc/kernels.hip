@@ -943,15 +943,19 @@ extern "C" hipError_t rh_launch_sample(hipStream_t st, const FrameDev* fr, const
                                        const uint32_t* listCount, const uint32_t* lists, const float* mprec,
                                        const GridDev* grids, const PrimDev* prims, const float* pos2d,
                                        const float* pos1d, const uchar4* pixPattern, float4* samples,
-                                       uint32_t* taskCounts, const uint32_t* bucketRefs, int nbuckets,
-                                       unsigned long long* counters) {
-    if (ntasks > 0) {
-        const int unit = 8 * RH_S_GROUP;
-        const int nwg = (ntasks + unit - 1) / unit * unit;
-        hipLaunchKernelGGL(rh_sample, dim3(nwg), dim3(256), 0, st, fr, buckets, tasks, ntasks, capStart, listCount,
-                           lists, reinterpret_cast<const float4*>(mprec), grids, prims, pos2d, pos1d, pixPattern,
-                           samples, taskCounts);
-    }
+                                       uint32_t* taskCounts) {
+    if (ntasks <= 0) return hipSuccess;
+    const int unit = 8 * RH_S_GROUP;
+    const int nwg = (ntasks + unit - 1) / unit * unit;
+    hipLaunchKernelGGL(rh_sample, dim3(nwg), dim3(256), 0, st, fr, buckets, tasks, ntasks, capStart, listCount, lists,
+                       reinterpret_cast<const float4*>(mprec), grids, prims, pos2d, pos1d, pixPattern, samples,
+                       taskCounts);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rh_launch_reduce_counters(hipStream_t st, const uint32_t* taskCounts, int ntasks,
+                                                const uint32_t* bucketRefs, int nbuckets,
+                                                unsigned long long* counters) {
     hipLaunchKernelGGL(rh_reduce_counters, dim3(1), dim3(1024), 0, st, taskCounts, ntasks, bucketRefs, nbuckets,
                        counters);
     return hipGetLastError();

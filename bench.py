@@ -58,6 +58,20 @@ def algorithmic_ops(st):
             + 39 * st["inside_nonlinear"])
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` at the HD workload, as measured by
+    rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled by the copy-kernel calibration)
+    and recorded in profiles/hd_patches_frame_kernels.md; None if absent."""
+    path = os.path.join(ROOT, "profiles", "hd_patches_frame_kernels.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(opts, prims):
     from oracle import loader
     import copy
@@ -168,7 +182,10 @@ def main():
             "peak_plain_valu": PEAK_VALU_TLANEOPS,
             "frac_plain_valu": achieved_tops / PEAK_VALU_TLANEOPS,
             "algorithmic_ops_per_launch": ops,
-            "traffic": None,
+            "launch_ms": sample_avg_ms,
+            "traffic": profiled_traffic("rh_sample") if args.workload == "hd" else None,
+            "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (copy-kernel calibrated), "
+                              "profiles/hd_patches_frame_kernels.md",
         },
     }
     if not args.no_cpu:

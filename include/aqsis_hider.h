@@ -126,7 +126,7 @@ typedef struct rh_stats {
     double ms_schedule;        /* host: split/dice scheduling                   */
     double ms_upload;          /* host->device copies of the frame inputs       */
     double ms_dice;            /* device: dice + bust                           */
-    double ms_bin;             /* device: bucket micropolygon lists             */
+    double ms_bin;             /* device: per-task micropolygon lists           */
     double ms_sample;          /* device: hit test + depth + composite          */
     double ms_filter;          /* device: filter + expose + quantise            */
     double ms_device;          /* device: whole frame, event to event           */
@@ -139,7 +139,7 @@ typedef struct rh_footprint {
     uint64_t grids;            /* grids the schedule dices                      */
     uint64_t micropolygons;
     uint64_t vertices;
-    uint64_t list_capacity;    /* bucket micropolygon-list entries reserved     */
+    uint64_t list_capacity;    /* task micropolygon-list entries reserved       */
     uint64_t device_bytes;     /* device memory of the frame (shape + schedule) */
 } rh_footprint;
 

@@ -21,9 +21,7 @@ int oracle_frame_size(const rh_options* opt, int32_t* width, int32_t* height,
  * (bucketprocessor.cpp:947-960); 0 turns that culling off, which changes
  * only which occluded surfaces get diced and in what order.  Output
  * buffers follow the layouts of rh_read_channels / rh_read_rgba8 /
- * rh_read_samples in aqsis_hider.h; any output pointer may be NULL.
- * threads > 1 is accepted only for timing runs that need no per-sample
- * outputs; results are identical to threads == 1. */
+ * rh_read_samples in aqsis_hider.h; any output pointer may be NULL. */
 int oracle_render(const rh_options* opt, const rh_prim* prims, int32_t nprims,
                   int32_t occlusion_cull, float* channels, uint8_t* rgba8,
                   uint32_t* mp_id, float* depth, float* rgbaz, rh_stats* stats);
