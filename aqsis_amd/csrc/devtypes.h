@@ -18,12 +18,17 @@
 //  [16..19] vertex depths z0..z3 (grid order)
 //  [20..21] A  [22..23] E  [24..25] F  [26..27] G   (inverse bilinear)
 //  [28..31] bound min x, max x, min y, max y
-//  [32] bound min z  [33] linear flag (int bits)  [34] grid index (int bits)  [35] unused
+//  [32] bound min z  [33] linear flag (int bits)  [34] grid index (int bits)
+//  [35] opaque flag (int bits): Oi >= (1,1,1) component-wise (color.h:432-435)
 #define RH_MPREC_FLOATS 36
 // Device counters: 0..4 the hider's SPL counters (sample tests, bound hits,
 // stored hits, point-in-polygon hits, second Newton steps), 5 tasks whose
-// list overflowed, 6 micropolygon-bucket references.
-#define RH_NCOUNTERS 8
+// list overflowed, 6 micropolygon-bucket references, 7 transparent-hit
+// arena entries taken (may exceed the capacity: then the frame is redone
+// with a larger arena), 8 samples whose hit list exceeded RH_MAX_HITS.
+#define RH_NCOUNTERS 9
+// Transparent hits one sample can hold (its occluding hit included).
+#define RH_MAX_HITS 64
 // Sphere parameter layout (see front.h)
 #define SPH_RADIUS_D 0
 #define SPH_MAT_D 1
@@ -43,6 +48,8 @@ struct FrameDev {
     int blockW, blockH;           // pixels per 64-sample wave block
     float exposureGain, exposureGamma;
     float qOne, qMin, qMax, qDither;
+    float zThreshold[3];          // Option "limits" "zthreshold"
+    uint32_t hitCapacity;         // transparent-hit arena entries
 };
 
 struct GridDev {
@@ -69,3 +76,37 @@ struct PrimDev {
     float Oi[3];
     float pad[2];
 };
+
+// Arguments of one sampling launch (see rh_sample in kernels.hip).
+struct SampleLaunch {
+    const FrameDev* fr;
+    const BucketDev* buckets;
+    const TaskDev* tasks;
+    const uint2* items;                   // (task, first block | block count << 16)
+    int itemBase, nStatic;                // static grids: items [itemBase, itemBase + nStatic)
+    const uint32_t* nDevice;              // persistent: item count on the device
+    int persistent;
+    uint32_t* queueHead;
+    const unsigned long long* capStart;
+    const uint32_t* listCount;
+    const uint32_t* lists;
+    const float* mprec;
+    const GridDev* grids;
+    const PrimDev* prims;
+    const float* pos2d;
+    const float* pos1d;
+    const uchar4* pixPattern;
+    float4* samples;
+    float4* hitData;
+    uint32_t* hitNext;
+    uint32_t* hitCounter;
+    uint4* combineList;
+    float* frontDepth;
+    uint32_t* rowCounts;
+    int rowBase;
+};
+
+// Sampling kernel variants: LDS chunk of CH records, SLOTS blocks per wave.
+#define RH_SAMPLE_VARIANTS 4
+#define RH_VARIANT_CH(v) ((v) < 2 ? 224 : 96)
+#define RH_VARIANT_SLOTS(v) (((v) & 1) ? 2 : 4)

@@ -684,8 +684,6 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
         }
         if (p.nkeys != 1) { err = "motion keys are not supported by this build"; return false; }
         if (!(p.shading_rate > 0)) { err = "shading rate must be positive"; return false; }
-        for (int k = 0; k < 3; ++k)
-            if (!(p.Os[k] >= 1.0f)) { err = "partially transparent surfaces are not supported by this build"; return false; }
         if (p.type == RH_PRIM_SPHERE) {
             if (!(p.sphere[0] != 0)) { err = "sphere radius must be non-zero"; return false; }
             if (p.sphere[3] != 360.0f || p.sphere[1] != -p.sphere[0] || p.sphere[2] != p.sphere[0]) {

@@ -2,6 +2,7 @@
 // per-frame launch sequence and read-back.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <new>
@@ -13,15 +14,18 @@
 #include "devtypes.h"
 #include "front.h"
 
-extern "C" hipError_t rh_launch_dice(hipStream_t, const FrameDev*, const GridDev*, int, const float*, float4*, float*,
-                                     float4*);
+extern "C" hipError_t rh_launch_dice(hipStream_t, const FrameDev*, const GridDev*, int, const PrimDev*,
+                                     const float*, float4*, float*, float4*);
 extern "C" hipError_t rh_launch_bin(hipStream_t, const FrameDev*, const BucketDev*, int, const TaskDev*,
                                     const GridDev*, const uint32_t*, const uint32_t*, const unsigned long long*,
                                     const float4*, uint32_t*, uint32_t*, uint32_t*, unsigned long long*);
-extern "C" hipError_t rh_launch_sample(hipStream_t, const FrameDev*, const BucketDev*, const TaskDev*, int,
-                                       const unsigned long long*, const uint32_t*, const uint32_t*, const float*,
-                                       const GridDev*, const PrimDev*, const float*, const float*, const uchar4*,
-                                       float4*, uint32_t*);
+extern "C" hipError_t rh_launch_sample(hipStream_t, int, int, const SampleLaunch*);
+extern "C" hipError_t rh_sample_occupancy(int, int*);
+extern "C" hipError_t rh_launch_plan_items(hipStream_t, const FrameDev*, const TaskDev*, int, const uint32_t*,
+                                           uint32_t, int, uint2*, uint32_t*, uint32_t*);
+extern "C" hipError_t rh_launch_frame_begin(hipStream_t, unsigned long long*, uint32_t*);
+extern "C" hipError_t rh_launch_combine(hipStream_t, const FrameDev*, size_t, const uint4*, const uint32_t*,
+                                        const float4*, const uint32_t*, float4*, float*);
 extern "C" hipError_t rh_launch_reduce_counters(hipStream_t, const uint32_t*, int, const uint32_t*, int,
                                                 unsigned long long*);
 extern "C" size_t rh_filter_tile_lds(int, int, int);
@@ -97,12 +101,33 @@ struct rh_ctx {
     DBuf dGrids, dParams, dPrims, dCandStart, dCandGrid, dCapStart;
     // per frame
     DBuf dVtx, dMprec, dMpbound, dLists, dListCount, dSamples, dChannels, dRgba, dCounters, dTaskCounts, dBucketRefs;
+    // transparent hits: arena of (Ci, depth), (Oi, id) pairs and list links
+    DBuf dHitData, dHitNext, dHitCtl, dCombine, dFrontDepth;
+    bool transparent = false;             // some posted surface has Os < 1
+    // work items: one per task (grid and per-bucket forms), cost-split (batched)
+    DBuf dItemsStatic, dItems, dItemCount, dQueue, dRows;
+    int maxItems = 0, persistentGrid = 0, maxRows = 0;
+    rh_tuning tuning;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    bool graphTimed = false;              // the graph holds the stage events
+    int scheduleThreads = 0;              // host threads of the split/dice schedule (0: up to 16)
+    void dropGraph() {
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graphExec = nullptr;
+        graph = nullptr;
+    }
+    FrameDev frame;                       // host copy of dFrame
     rh_stats stats;
     ~rh_ctx() {
         DBuf* all[] = {&dFrame, &dBuckets, &dTasks, &dPos2d, &dPos1d, &dPix, &dDither, &dFilter, &dBoxMask,
                        &dGrids, &dParams,
                        &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dVtx, &dMprec, &dMpbound, &dLists,
-                       &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dTaskCounts, &dBucketRefs};
+                       &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dTaskCounts, &dBucketRefs,
+                       &dHitData, &dHitNext, &dHitCtl, &dCombine, &dFrontDepth, &dItemsStatic, &dItems,
+                       &dItemCount, &dQueue, &dRows};
+        dropGraph();
         for (DBuf* b : all) b->release();
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -165,6 +190,9 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
     f.blockW = s.blockW; f.blockH = s.blockH;
     f.exposureGain = opt->exposure_gain; f.exposureGamma = opt->exposure_gamma;
     f.qOne = opt->quantize_one; f.qMin = opt->quantize_min; f.qMax = opt->quantize_max; f.qDither = opt->quantize_dither;
+    for (int k = 0; k < 3; ++k) f.zThreshold[k] = opt->zthreshold[k];
+    f.hitCapacity = 0;
+    c->frame = f;
 
     std::vector<BucketDev> bk;
     for (size_t k = 0; k < s.buckets.size(); ++k) {
@@ -196,6 +224,7 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
         (e = c->dListCount.ensure(s.tasks.size() * sizeof(uint32_t))) != hipSuccess ||
         (e = c->dTaskCounts.ensure(s.tasks.size() * 5 * sizeof(uint32_t))) != hipSuccess ||
         (e = c->dBucketRefs.ensure(s.buckets.size() * sizeof(uint32_t))) != hipSuccess ||
+        (e = c->dHitCtl.ensure(4 * sizeof(uint32_t))) != hipSuccess ||
         (e = upload(c->dCounters, zero, c->st)) != hipSuccess) {
         delete c;
         return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_DEVICE, std::string("device set-up: ") + hipGetErrorString(e));
@@ -213,7 +242,42 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
                         std::string("device set-up: ") + hipGetErrorString(e));
         }
     }
-    if ((e = hipStreamSynchronize(c->st)) != hipSuccess) {
+    // Work items: one per task for the static forms; up to one per block
+    // for the batched form.
+    {
+        std::vector<uint2> it(s.tasks.size());
+        int total = 0;
+        for (size_t t = 0; t < s.tasks.size(); ++t) {
+            const auto& T = s.tasks[t];
+            const int nblk = ((T.x1 - T.x0 + s.blockW - 1) / s.blockW) * ((T.y1 - T.y0 + s.blockH - 1) / s.blockH);
+            it[t] = make_uint2((uint32_t)t, (uint32_t)nblk << 16);
+            total += nblk;
+        }
+        c->maxItems = std::max(total, 1);
+        int cus = 0, dev = 0, occ = 0;
+        if ((e = hipGetDevice(&dev)) == hipSuccess)
+            e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        int best = 1;
+        for (int v = 0; v < RH_SAMPLE_VARIANTS && e == hipSuccess; ++v) {
+            e = rh_sample_occupancy(v, &occ);
+            best = std::max(best, occ);
+        }
+        c->persistentGrid = std::max(1, cus * best);
+        const int unit = 8 * 16;
+        const int staticGrid = ((int)s.tasks.size() + unit - 1) / unit * unit;
+        c->maxRows = std::max(staticGrid, c->persistentGrid);
+        if (e == hipSuccess) e = upload(c->dItemsStatic, it, c->st);
+        if (e == hipSuccess) e = c->dItems.ensure((size_t)c->maxItems * sizeof(uint2));
+        if (e == hipSuccess) e = c->dItemCount.ensure(16);
+        if (e == hipSuccess) e = c->dQueue.ensure(16);
+        if (e == hipSuccess) e = c->dRows.ensure((size_t)c->maxRows * 5 * sizeof(uint32_t));
+    }
+    c->tuning.mode = RH_MODE_GRID;
+    c->tuning.lds_records = 224;
+    c->tuning.blocks_per_wave = 4;
+    c->tuning.split_cost = 1u << 16;
+    c->tuning.frame_ms = 0.0;
+    if (e != hipSuccess || (e = hipStreamSynchronize(c->st)) != hipSuccess) {
         delete c;
         return fail(RH_ERR_DEVICE, std::string("device set-up: ") + hipGetErrorString(e));
     }
@@ -236,11 +300,26 @@ rh_status rh_clear_surfaces(rh_ctx* c) {
     return RH_OK;
 }
 
+// Size the transparent-hit arena and publish its capacity to the kernels.
+static rh_status set_hit_capacity(rh_ctx* c, uint64_t n) {
+    if (n < 64) n = 64;
+    if (n > 0x7fffffffull) return fail(RH_ERR_CAPACITY, "transparent hits exceed 2^31 entries");
+    HIP_TRY(c->dHitData.ensure(n * 2 * sizeof(float4)));
+    HIP_TRY(c->dHitNext.ensure(n * sizeof(uint32_t)));
+    c->frame.hitCapacity = (uint32_t)n;
+    c->dropGraph();
+    std::vector<FrameDev> fv(1, c->frame);
+    HIP_TRY(upload(c->dFrame, fv, c->st));
+    return RH_OK;
+}
+
 static rh_status schedule(rh_ctx* c) {
+    c->dropGraph();
     auto t0 = std::chrono::steady_clock::now();
     std::string err;
     int threads = (int)std::thread::hardware_concurrency();
     if (threads > 16) threads = 16;
+    if (c->scheduleThreads > 0) threads = c->scheduleThreads;
     if (!rh::make_schedule(c->shape, c->prims.data(), (int)c->prims.size(), c->sched, err, threads))
         return fail(RH_ERR_INVALID, err);
     c->stats.ms_schedule = ms_since(t0);
@@ -272,48 +351,160 @@ static rh_status schedule(rh_ctx* c) {
     HIP_TRY(c->dMprec.ensure(S.micropolygons * RH_MPREC_FLOATS * sizeof(float)));
     HIP_TRY(c->dMpbound.ensure(S.micropolygons * sizeof(float4)));
     HIP_TRY(c->dLists.ensure((size_t)S.capStart.back() * sizeof(uint32_t)));
+    // Transparent hits need list storage; start at two per sample and grow
+    // when a frame asks for more.
+    bool transparent = false;
+    for (const auto& q : c->prims)
+        if (!(q.Os[0] >= 1.0f && q.Os[1] >= 1.0f && q.Os[2] >= 1.0f)) transparent = true;
+    c->transparent = transparent;
+    const rh::Shape& s = c->shape;
+    const uint64_t nsamp = (uint64_t)s.extW * s.extH * s.ns;
+    // one entry per sample to start (at least 1M, at most 64M); a frame that
+    // needs more is redone once with the exact size
+    const uint64_t want = transparent ? std::min<uint64_t>(std::max<uint64_t>(1u << 20, nsamp), 1u << 26) : 64;
+    HIP_TRY(c->dCombine.ensure((transparent ? nsamp : 1) * sizeof(uint4)));
+    HIP_TRY(c->dFrontDepth.ensure((transparent ? nsamp : 1) * sizeof(float)));
+    if (want > c->frame.hitCapacity) {
+        rh_status r = set_hit_capacity(c, want);
+        if (r != RH_OK) return r;
+    }
     HIP_TRY(hipStreamSynchronize(c->st));
     c->stats.ms_upload = ms_since(t1);
     c->scheduled = true;
     return RH_OK;
 }
 
+static int variant_of(const rh_tuning& t) {
+    return (t.lds_records == 224 ? 0 : 2) + (t.blocks_per_wave == 4 ? 0 : 1);
+}
+
+// Enqueue one device frame on the context's stream.  With `timed`, HIP
+// events bracket the stages (not inside a graph capture).
+static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
+    const rh::Schedule& S = c->sched;
+    const rh::Shape& s = c->shape;
+    const FrameDev* f = c->dFrame.as<FrameDev>();
+    const int nb = (int)s.buckets.size(), nt = (int)s.tasks.size();
+    const rh_tuning& tu = c->tuning;
+    hipError_t e;
+#define ENQ(x) do { if ((e = (x)) != hipSuccess) return e; } while (0)
+    if (timed) ENQ(hipEventRecord(c->ev[0], c->st));
+    ENQ(rh_launch_frame_begin(c->st, c->dCounters.as<unsigned long long>(), c->dHitCtl.as<uint32_t>()));
+    ENQ(rh_launch_dice(c->st, f, c->dGrids.as<GridDev>(), (int)S.grids.size(), c->dPrims.as<PrimDev>(),
+                       c->dParams.as<float>(), c->dVtx.as<float4>(), c->dMprec.as<float>(), c->dMpbound.as<float4>()));
+    if (timed) ENQ(hipEventRecord(c->ev[1], c->st));
+    ENQ(rh_launch_bin(c->st, f, c->dBuckets.as<BucketDev>(), nb, c->dTasks.as<TaskDev>(), c->dGrids.as<GridDev>(),
+                      c->dCandStart.as<uint32_t>(), c->dCandGrid.as<uint32_t>(), c->dCapStart.as<unsigned long long>(),
+                      c->dMpbound.as<float4>(), c->dLists.as<uint32_t>(), c->dListCount.as<uint32_t>(),
+                      c->dBucketRefs.as<uint32_t>(), c->dCounters.as<unsigned long long>()));
+    if (timed) ENQ(hipEventRecord(c->ev[2], c->st));
+    SampleLaunch a;
+    a.fr = f; a.buckets = c->dBuckets.as<BucketDev>(); a.tasks = c->dTasks.as<TaskDev>();
+    a.items = c->dItemsStatic.as<uint2>(); a.itemBase = 0; a.nStatic = nt; a.nDevice = c->dItemCount.as<uint32_t>();
+    a.persistent = 0; a.queueHead = c->dQueue.as<uint32_t>();
+    a.capStart = c->dCapStart.as<unsigned long long>(); a.listCount = c->dListCount.as<uint32_t>();
+    a.lists = c->dLists.as<uint32_t>(); a.mprec = c->dMprec.as<float>(); a.grids = c->dGrids.as<GridDev>();
+    a.prims = c->dPrims.as<PrimDev>(); a.pos2d = c->dPos2d.as<float>(); a.pos1d = c->dPos1d.as<float>();
+    a.pixPattern = c->dPix.as<uchar4>(); a.samples = c->dSamples.as<float4>(); a.hitData = c->dHitData.as<float4>();
+    a.hitNext = c->dHitNext.as<uint32_t>(); a.hitCounter = c->dHitCtl.as<uint32_t>();
+    a.combineList = c->dCombine.as<uint4>(); a.frontDepth = c->transparent ? c->dFrontDepth.as<float>() : nullptr;
+    a.rowCounts = c->dRows.as<uint32_t>(); a.rowBase = 0;
+    const int variant = variant_of(tu);
+    int rows = 0;
+    if (tu.mode == RH_MODE_PLAIN) {
+        // One launch per bucket, in bucket order.
+        for (int b = 0; b < nb; ++b) {
+            a.itemBase = s.taskStart[b];
+            a.nStatic = s.taskStart[b + 1] - s.taskStart[b];
+            a.rowBase = s.taskStart[b];
+            ENQ(rh_launch_sample(c->st, variant, a.nStatic, &a));
+        }
+        rows = nt;
+    } else if (tu.mode == RH_MODE_BATCHED) {
+        ENQ(rh_launch_plan_items(c->st, f, c->dTasks.as<TaskDev>(), nt, c->dListCount.as<uint32_t>(), tu.split_cost,
+                                 4 * tu.blocks_per_wave, c->dItems.as<uint2>(), c->dItemCount.as<uint32_t>(),
+                                 c->dQueue.as<uint32_t>()));
+        a.items = c->dItems.as<uint2>();
+        a.persistent = 1;
+        ENQ(rh_launch_sample(c->st, variant, c->persistentGrid, &a));
+        rows = c->persistentGrid;
+    } else {
+        const int unit = 8 * 16;
+        rows = (nt + unit - 1) / unit * unit;
+        ENQ(rh_launch_sample(c->st, variant, rows, &a));
+    }
+    if (c->transparent)
+        ENQ(rh_launch_combine(c->st, f, (size_t)s.extW * s.extH * s.ns, c->dCombine.as<uint4>(),
+                              c->dHitCtl.as<uint32_t>(), c->dHitData.as<float4>(), c->dHitNext.as<uint32_t>(),
+                              c->dSamples.as<float4>(), c->dFrontDepth.as<float>()));
+    if (timed) ENQ(hipEventRecord(c->ev[3], c->st));
+    ENQ(rh_launch_reduce_counters(c->st, c->dRows.as<uint32_t>(), rows, c->dBucketRefs.as<uint32_t>(), nb,
+                                  c->dCounters.as<unsigned long long>()));
+    ENQ(rh_launch_filter(c->st, f, s.width(), s.height(), s.ns, s.shiftX, s.shiftY, c->dSamples.as<float4>(),
+                         c->tiledFilter ? c->dBoxMask.as<uint16_t>() : nullptr, c->dPix.as<uchar4>(),
+                         c->dPos2d.as<float>(), c->dFilter.as<float>(), c->dDither.as<float>(),
+                         c->dChannels.as<float>(), c->dRgba.as<uchar4>()));
+    if (timed) ENQ(hipEventRecord(c->ev[4], c->st));
+#undef ENQ
+    return hipSuccess;
+}
+
 rh_status rh_render_device(rh_ctx* c) {
     if (!c) return fail(RH_ERR_INVALID, "null context");
     if (!c->scheduled) return fail(RH_ERR_STATE, "rh_render must run before rh_render_device");
     const rh::Schedule& S = c->sched;
-    const rh::Shape& s = c->shape;
-    const FrameDev* f = c->dFrame.as<FrameDev>();
-    const int nb = (int)s.buckets.size();
-    HIP_TRY(hipMemsetAsync(c->dCounters.p, 0, RH_NCOUNTERS * sizeof(unsigned long long), c->st));
-    HIP_TRY(hipEventRecord(c->ev[0], c->st));
-    HIP_TRY(rh_launch_dice(c->st, f, c->dGrids.as<GridDev>(), (int)S.grids.size(), c->dParams.as<float>(),
-                           c->dVtx.as<float4>(), c->dMprec.as<float>(), c->dMpbound.as<float4>()));
-    HIP_TRY(hipEventRecord(c->ev[1], c->st));
-    HIP_TRY(rh_launch_bin(c->st, f, c->dBuckets.as<BucketDev>(), nb, c->dTasks.as<TaskDev>(),
-                          c->dGrids.as<GridDev>(), c->dCandStart.as<uint32_t>(), c->dCandGrid.as<uint32_t>(),
-                          c->dCapStart.as<unsigned long long>(), c->dMpbound.as<float4>(),
-                          c->dLists.as<uint32_t>(), c->dListCount.as<uint32_t>(), c->dBucketRefs.as<uint32_t>(),
-                          c->dCounters.as<unsigned long long>()));
-    HIP_TRY(hipEventRecord(c->ev[2], c->st));
-    HIP_TRY(rh_launch_sample(c->st, f, c->dBuckets.as<BucketDev>(), c->dTasks.as<TaskDev>(), (int)s.tasks.size(),
-                             c->dCapStart.as<unsigned long long>(), c->dListCount.as<uint32_t>(),
-                             c->dLists.as<uint32_t>(), c->dMprec.as<float>(), c->dGrids.as<GridDev>(),
-                             c->dPrims.as<PrimDev>(), c->dPos2d.as<float>(), c->dPos1d.as<float>(),
-                             c->dPix.as<uchar4>(), c->dSamples.as<float4>(), c->dTaskCounts.as<uint32_t>()));
-    HIP_TRY(hipEventRecord(c->ev[3], c->st));
-    HIP_TRY(rh_launch_reduce_counters(c->st, c->dTaskCounts.as<uint32_t>(), (int)s.tasks.size(),
-                                      c->dBucketRefs.as<uint32_t>(), nb, c->dCounters.as<unsigned long long>()));
-    HIP_TRY(rh_launch_filter(c->st, f, s.width(), s.height(), s.ns, s.shiftX, s.shiftY, c->dSamples.as<float4>(),
-                             c->tiledFilter ? c->dBoxMask.as<uint16_t>() : nullptr, c->dPix.as<uchar4>(),
-                             c->dPos2d.as<float>(), c->dFilter.as<float>(), c->dDither.as<float>(),
-                             c->dChannels.as<float>(), c->dRgba.as<uchar4>()));
-    HIP_TRY(hipEventRecord(c->ev[4], c->st));
-    HIP_TRY(hipEventSynchronize(c->ev[4]));
+    const bool graphed = c->tuning.mode == RH_MODE_BATCHED;
     unsigned long long cnt[RH_NCOUNTERS];
-    HIP_TRY(hipMemcpy(cnt, c->dCounters.p, sizeof(cnt), hipMemcpyDeviceToHost));
-    float t[4];
-    for (int i = 0; i < 4; ++i) HIP_TRY(hipEventElapsedTime(&t[i], c->ev[i], c->ev[i + 1]));
+    for (int attempt = 0;; ++attempt) {
+        if (graphed) {
+            // The batched form replays one captured graph per frame.
+            if (!c->graphExec) {
+                // Stage events are captured as event nodes where the runtime
+                // allows it; otherwise only the whole frame is timed.
+                for (int timedCapture = 1; timedCapture >= 0 && !c->graphExec; --timedCapture) {
+                    HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+                    hipError_t e = enqueue_frame(c, timedCapture != 0);
+                    hipGraph_t g = nullptr;
+                    hipError_t e2 = hipStreamEndCapture(c->st, &g);
+                    if (e == hipSuccess && e2 == hipSuccess) {
+                        c->graph = g;
+                        HIP_TRY(hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0));
+                        c->graphTimed = timedCapture != 0;
+                    } else {
+                        if (g) (void)hipGraphDestroy(g);
+                        (void)hipGetLastError();
+                        if (!timedCapture)
+                            return fail(RH_ERR_DEVICE, std::string("frame graph capture: ") +
+                                                           hipGetErrorString(e != hipSuccess ? e : e2));
+                    }
+                }
+            }
+            if (!c->graphTimed) HIP_TRY(hipEventRecord(c->ev[0], c->st));
+            HIP_TRY(hipGraphLaunch(c->graphExec, c->st));
+            if (!c->graphTimed) HIP_TRY(hipEventRecord(c->ev[4], c->st));
+        } else {
+            HIP_TRY(enqueue_frame(c, true));
+        }
+        if (graphed) HIP_TRY(hipStreamSynchronize(c->st));
+        else HIP_TRY(hipEventSynchronize(c->ev[4]));
+        HIP_TRY(hipMemcpy(cnt, c->dCounters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+        uint32_t hc[2];
+        HIP_TRY(hipMemcpy(hc, c->dHitCtl.p, sizeof(hc), hipMemcpyDeviceToHost));
+        if (hc[0] > c->frame.hitCapacity) {
+            // The arena ran out: redo the frame with room for every hit.
+            if (attempt >= 2) return fail(RH_ERR_CAPACITY, "transparent-hit arena kept overflowing");
+            rh_status r = set_hit_capacity(c, (uint64_t)hc[0] + hc[0] / 4 + 64);
+            if (r != RH_OK) return r;
+            continue;
+        }
+        if (hc[1] != 0)
+            return fail(RH_ERR_CAPACITY, "a sample collected more than " + std::to_string(RH_MAX_HITS - 1) +
+                                             " transparent hits");
+        break;
+    }
+    float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (!graphed || c->graphTimed)
+        for (int i = 0; i < 4; ++i) HIP_TRY(hipEventElapsedTime(&t[i], c->ev[i], c->ev[i + 1]));
     c->stats.spl_count = cnt[0];
     c->stats.spl_bound_hits = cnt[1];
     c->stats.spl_hits = cnt[2];
@@ -335,6 +526,63 @@ rh_status rh_render_device(rh_ctx* c) {
         return fail(RH_ERR_CAPACITY, "internal: a task's micropolygon list exceeded its capacity (" +
                                          std::to_string(cnt[5]) + " tasks)");
     c->rendered = true;
+    return RH_OK;
+}
+
+rh_status rh_set_tuning(rh_ctx* c, const rh_tuning* t) {
+    if (!c || !t) return fail(RH_ERR_INVALID, "null argument");
+    if (t->mode != RH_MODE_GRID && t->mode != RH_MODE_PLAIN && t->mode != RH_MODE_BATCHED)
+        return fail(RH_ERR_INVALID, "unknown mode");
+    if (t->lds_records != 224 && t->lds_records != 96) return fail(RH_ERR_INVALID, "lds_records must be 224 or 96");
+    if (t->blocks_per_wave != 4 && t->blocks_per_wave != 2)
+        return fail(RH_ERR_INVALID, "blocks_per_wave must be 4 or 2");
+    if (t->split_cost == 0) return fail(RH_ERR_INVALID, "split_cost must be positive");
+    const double ms = c->tuning.frame_ms;
+    c->tuning = *t;
+    c->tuning.frame_ms = ms;
+    c->dropGraph();
+    return RH_OK;
+}
+
+rh_status rh_get_tuning(const rh_ctx* c, rh_tuning* t) {
+    if (!c || !t) return fail(RH_ERR_INVALID, "null argument");
+    *t = c->tuning;
+    return RH_OK;
+}
+
+rh_status rh_autotune(rh_ctx* c, rh_tuning* chosen) {
+    if (!c) return fail(RH_ERR_INVALID, "null context");
+    if (!c->scheduled) {
+        rh_status r = schedule(c);
+        if (r != RH_OK) return r;
+    }
+    rh_tuning best;
+    double bestMs = 1e300;
+    const int lds[2] = {224, 96}, bpw[2] = {4, 2};
+    const uint32_t split[3] = {1u << 14, 1u << 16, 1u << 18};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int k = 0; k < 3; ++k) {
+                rh_tuning t;
+                t.mode = RH_MODE_BATCHED;
+                t.lds_records = lds[i];
+                t.blocks_per_wave = bpw[j];
+                t.split_cost = split[k];
+                t.frame_ms = 0.0;
+                rh_status r = rh_set_tuning(c, &t);
+                // first frame captures the graph; the second is timed
+                for (int rep = 0; rep < 2 && r == RH_OK; ++rep) r = rh_render_device(c);
+                if (r != RH_OK) return r;
+                if (c->stats.ms_device < bestMs) {
+                    bestMs = c->stats.ms_device;
+                    best = t;
+                }
+            }
+    best.frame_ms = bestMs;
+    rh_status r = rh_set_tuning(c, &best);
+    if (r != RH_OK) return r;
+    c->tuning.frame_ms = bestMs;
+    if (chosen) *chosen = c->tuning;
     return RH_OK;
 }
 
@@ -376,14 +624,19 @@ rh_status rh_read_samples(rh_ctx* c, uint32_t* mp_id, float* depth, float* rgbaz
     if (!c) return fail(RH_ERR_INVALID, "null context");
     if (!c->rendered) return fail(RH_ERR_STATE, "no rendered frame");
     const size_t n = (size_t)c->shape.extW * c->shape.extH * c->shape.ns;
-    std::vector<float> buf(n * 8);
+    std::vector<float> buf(n * 8), front;
     HIP_TRY(hipMemcpy(buf.data(), c->dSamples.p, n * 32, hipMemcpyDeviceToHost));
+    if (c->transparent && depth) {
+        // depth of the front-most hit, kept apart for combined samples
+        front.resize(n);
+        HIP_TRY(hipMemcpy(front.data(), c->dFrontDepth.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    }
     for (size_t i = 0; i < n; ++i) {
         const float* r = &buf[i * 8];
         uint32_t id;
         std::memcpy(&id, &r[7], 4);
         if (mp_id) mp_id[i] = id;
-        if (depth) depth[i] = r[3];
+        if (depth) depth[i] = front.empty() ? r[3] : front[i];
         if (rgbaz) {
             float* o = rgbaz + i * 7;
             o[0] = r[0]; o[1] = r[1]; o[2] = r[2]; o[3] = r[4]; o[4] = r[5]; o[5] = r[6]; o[6] = r[3];
@@ -399,6 +652,25 @@ rh_status rh_get_stats(const rh_ctx* c, rh_stats* out) {
 }
 
 void rh_destroy(rh_ctx* c) { delete c; }
+
+}  // extern "C"
+
+// Used by the serving loop (server.cpp).
+uint64_t rh_internal_device_bytes(const rh_ctx* c) {
+    const DBuf* all[] = {&c->dFrame, &c->dBuckets, &c->dTasks, &c->dPos2d, &c->dPos1d, &c->dPix, &c->dDither,
+                         &c->dFilter, &c->dBoxMask, &c->dGrids, &c->dParams, &c->dPrims, &c->dCandStart,
+                         &c->dCandGrid, &c->dCapStart, &c->dVtx, &c->dMprec, &c->dMpbound, &c->dLists,
+                         &c->dListCount, &c->dSamples, &c->dChannels, &c->dRgba, &c->dCounters, &c->dTaskCounts,
+                         &c->dBucketRefs, &c->dHitData, &c->dHitNext, &c->dHitCtl, &c->dCombine, &c->dFrontDepth,
+                         &c->dItemsStatic, &c->dItems, &c->dItemCount, &c->dQueue, &c->dRows};
+    uint64_t total = 0;
+    for (const DBuf* b : all) total += b->cap;
+    return total;
+}
+
+void rh_internal_set_schedule_threads(rh_ctx* c, int threads) { c->scheduleThreads = threads; }
+
+extern "C" {
 
 static uint64_t frame_bytes(const rh::Shape& s) {
     uint64_t ext = (uint64_t)s.extW * s.extH;
@@ -423,6 +695,16 @@ rh_status rh_estimate(const rh_options* opt, const rh_prim* prims, int32_t n, rh
     out->device_bytes = frame_bytes(s) + S.grids.size() * sizeof(GridDev) + S.params.size() * 4 +
                         (uint64_t)n * sizeof(PrimDev) + S.candGrid.size() * 4 + S.vertices * 16 +
                         S.micropolygons * (RH_MPREC_FLOATS * 4 + 16) + S.capStart.back() * 4;
+    bool transparent = false;
+    for (int32_t i = 0; i < n; ++i)
+        if (!(prims[i].Os[0] >= 1.0f && prims[i].Os[1] >= 1.0f && prims[i].Os[2] >= 1.0f)) transparent = true;
+    if (transparent) {
+        // hit arena (36 B an entry, one entry per sample to start), combine
+        // list and front depths
+        const uint64_t nsamp = (uint64_t)s.extW * s.extH * s.ns;
+        const uint64_t arena = std::min<uint64_t>(std::max<uint64_t>(1u << 20, nsamp), 1u << 26);
+        out->device_bytes += arena * 36 + nsamp * 20;
+    }
     return RH_OK;
 }
 

@@ -26,6 +26,7 @@
 #include <cstdint>
 
 #include "devtypes.h"
+#include "hitsort.h"
 
 namespace {
 
@@ -97,6 +98,7 @@ struct FD {
 // Dice + bust
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__ fr, const GridDev* __restrict__ grids,
+                                                    const PrimDev* __restrict__ prims,
                                                     const float* __restrict__ params, float4* __restrict__ vtx,
                                                     float* __restrict__ mprec, float4* __restrict__ mpbound) {
     __shared__ float4 V[RH_MAX_GRID_VERTS];
@@ -249,6 +251,9 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
         float bminz = fmin_ref(fmin_ref(fmin_ref(pA.z, pB.z), pC.z), pD.z);
         float bmaxx = fmax_ref(fmax_ref(fmax_ref(pA.x, pB.x), pC.x), pD.x);
         float bmaxy = fmax_ref(fmax_ref(fmax_ref(pA.y, pB.y), pC.y), pD.y);
+        // isOpaque = Oi >= (1,1,1) component-wise (micropolygon.cpp:1482, color.h:432-435)
+        const PrimDev& Pr = prims[G.prim];
+        const bool opaque = Pr.Oi[0] >= 1.0f && Pr.Oi[1] >= 1.0f && Pr.Oi[2] >= 1.0f;
         float4* out = reinterpret_cast<float4*>(mprec + (size_t)(G.mpBase + m) * RH_MPREC_FLOATS);
         out[0] = make_float4(YM[0], YM[1], YM[2], YM[3]);
         out[1] = make_float4(XM[0], XM[1], XM[2], XM[3]);
@@ -258,7 +263,8 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
         out[5] = make_float4(P[0].x, P[0].y, Ex, Ey);
         out[6] = make_float4(Fx, Fy, Gx, Gy);
         out[7] = make_float4(bminx, bmaxx, bminy, bmaxy);
-        out[8] = make_float4(bminz, __int_as_float(linear ? 1 : 0), __int_as_float((int)blockIdx.x), 0.0f);
+        out[8] = make_float4(bminz, __int_as_float(linear ? 1 : 0), __int_as_float((int)blockIdx.x),
+                             __int_as_float(opaque ? 1 : 0));
         mpbound[G.mpBase + m] = make_float4(bminx, bmaxx, bminy, bmaxy);
     }
 }
@@ -369,17 +375,17 @@ __global__ void __launch_bounds__(256) rh_bin(const FrameDev* __restrict__ fr, c
     if (tid == 0) bucketRefs[b] = wrefs[0] + wrefs[1] + wrefs[2] + wrefs[3];
 }
 
-// Frame counters from the per-task and per-bucket rows: counters[0..4] the
+// Frame counters from the sampling rows and per-bucket rows: counters[0..4] the
 // hider's SPL counts, counters[6] the micropolygon-bucket references.
 // One workgroup.
-__global__ void __launch_bounds__(1024) rh_reduce_counters(const uint32_t* __restrict__ taskCounts, int ntasks,
+__global__ void __launch_bounds__(1024) rh_reduce_counters(const uint32_t* __restrict__ rowCounts, int nrows,
                                                            const uint32_t* __restrict__ bucketRefs, int nbuckets,
                                                            unsigned long long* __restrict__ counters) {
     __shared__ unsigned long long part[6][32];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     unsigned long long acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int t = tid; t < ntasks; t += 1024)
-        for (int c = 0; c < 5; ++c) acc[c] += taskCounts[(size_t)t * 5 + c];
+    for (int t = tid; t < nrows; t += 1024)
+        for (int c = 0; c < 5; ++c) acc[c] += rowCounts[(size_t)t * 5 + c];
     for (int b = tid; b < nbuckets; b += 1024) acc[5] += bucketRefs[b];
     for (int c = 0; c < 6; ++c) {
         unsigned long long v = acc[c];
@@ -413,54 +419,103 @@ __global__ void __launch_bounds__(1024) rh_reduce_counters(const uint32_t* __res
 // lceil(bmaxx)) clipped to the bucket's sample region with the first/last
 // pixel sub-sample clip, in sub-sample grid units, so every sample sees the
 // reference's sequence of tests.
-#define RH_S_CH 224
+// Colour and opacity of a hit: InterpolateOutputs with smooth shading of
+// four equal vertex values (micropolygon.cpp:1443-1462).  A = (Ci.rgb, depth),
+// B = (Oi.rgb, micropolygon id bits).
+__device__ __forceinline__ void shade_hit(const PrimDev& P, float u, float v, float D, uint32_t id, float4& A,
+                                          float4& B) {
+    const float w0 = (1 - u) * (1 - v), w1 = u * (1 - v), w2 = (1 - u) * v, w3 = u * v;
+    A.x = ((w0 * P.Ci[0] + w1 * P.Ci[0]) + w2 * P.Ci[0]) + w3 * P.Ci[0];
+    A.y = ((w0 * P.Ci[1] + w1 * P.Ci[1]) + w2 * P.Ci[1]) + w3 * P.Ci[1];
+    A.z = ((w0 * P.Ci[2] + w1 * P.Ci[2]) + w2 * P.Ci[2]) + w3 * P.Ci[2];
+    A.w = D;
+    B.x = ((w0 * P.Oi[0] + w1 * P.Oi[0]) + w2 * P.Oi[0]) + w3 * P.Oi[0];
+    B.y = ((w0 * P.Oi[1] + w1 * P.Oi[1]) + w2 * P.Oi[1]) + w3 * P.Oi[1];
+    B.z = ((w0 * P.Oi[2] + w1 * P.Oi[2]) + w2 * P.Oi[2]) + w3 * P.Oi[2];
+    B.w = __uint_as_float(id);
+}
+
 #define RH_S_WAVES 4
-#define RH_S_SLOTS 4
 #define RH_S_GROUP 16
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets,
-                                                 const TaskDev* __restrict__ tasks, int ntasks,
-                                                 const unsigned long long* __restrict__ capStart,
-                                                 const uint32_t* __restrict__ listCount,
-                                                 const uint32_t* __restrict__ lists,
-                                                 const float4* __restrict__ mprec4, const GridDev* __restrict__ grids,
-                                                 const PrimDev* __restrict__ prims, const float* __restrict__ pos2d,
-                                                 const float* __restrict__ pos1d, const uchar4* __restrict__ pixPattern,
-                                                 float4* __restrict__ samples, uint32_t* __restrict__ taskCounts) {
-    __shared__ float4 recs[RH_S_CH * 9];
+// Work items: (task, first block | block count << 16).  CH records per LDS
+// chunk, SLOTS blocks of 64 samples per wave per pass.  Three launch forms:
+// a static grid over the items (one workgroup each, runs of RH_S_GROUP
+// items per XCD when the grid is a multiple of 8 * RH_S_GROUP), the same per
+// bucket (itemBase/nStatic select the bucket's items), and persistent
+// workgroups pulling items from a queue (persistent != 0; the count is in
+// *nDevice).  Each workgroup writes one row of counters at rowBase +
+// blockIdx.x.
+template <int CH, int SLOTS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets, const TaskDev* __restrict__ tasks,
+          const uint2* __restrict__ items, int itemBase, int nStatic, const uint32_t* __restrict__ nDevice,
+          int persistent, uint32_t* __restrict__ queueHead, const unsigned long long* __restrict__ capStart,
+          const uint32_t* __restrict__ listCount, const uint32_t* __restrict__ lists,
+          const float4* __restrict__ mprec4, const GridDev* __restrict__ grids, const PrimDev* __restrict__ prims,
+          const float* __restrict__ pos2d, const float* __restrict__ pos1d, const uchar4* __restrict__ pixPattern,
+          float4* __restrict__ samples, float4* __restrict__ hitData, uint32_t* __restrict__ hitNext,
+          uint32_t* __restrict__ hitCounter, uint4* __restrict__ combineList, float* __restrict__ frontDepth,
+          uint32_t* __restrict__ rowCounts, int rowBase) {
+    __shared__ float4 recs[CH * 9];
     __shared__ uint32_t wsum[RH_S_WAVES][5];
-    __shared__ uint2 meta[RH_S_CH];
-    __shared__ uint32_t ids[RH_S_CH];
-    __shared__ uint8_t matched[RH_S_WAVES][RH_S_CH];
-    // Workgroups are dealt round-robin to the 8 XCDs; runs of RH_S_GROUP
-    // consecutive tasks (neighbours sharing micropolygons) go to one XCD.
-    // The grid is a multiple of 8 * RH_S_GROUP workgroups.
-    const int kx = blockIdx.x >> 3;
-    const int task = ((kx / RH_S_GROUP) * 8 + (blockIdx.x & 7)) * RH_S_GROUP + kx % RH_S_GROUP;
-    if (task >= ntasks) return;
+    __shared__ uint2 meta[CH];
+    __shared__ uint32_t ids[CH];
+    __shared__ uint8_t matched[RH_S_WAVES][CH];
+    __shared__ int sItem;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const TaskDev T = tasks[task];
-    const BucketDev B = buckets[T.bucket];
     const int ns = fr->ns, nx = fr->nx, ny = fr->ny, bw = fr->blockW, bh = fr->blockH;
-    const int tw = T.x1 - T.x0, th = T.y1 - T.y0;
-    const int nbx = (tw + bw - 1) / bw, nby = (th + bh - 1) / bh, nblk = nbx * nby;
     const int pix = lane / ns, si = lane - pix * ns;
     const int lx = pix % bw, ly = pix / bw;
     const int sm = si % nx, sn = si / nx;
+    uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
+    uint32_t wbBase = 0;                  // the wave's current arena block
+    int wbLeft = 0;
+    bool lost = false;
+    const int itemEnd = persistent ? (int)nDevice[0] : itemBase + nStatic;
+
+    for (int round = 0;; ++round) {
+    int it;
+    if (persistent) {
+        if (tid == 0) sItem = (int)atomicAdd(queueHead, 1u);
+        __syncthreads();
+        it = sItem;
+        __syncthreads();
+    } else {
+        if (round > 0) break;
+        if (gridDim.x % (8 * RH_S_GROUP) == 0) {
+            // Workgroups are dealt round-robin to the 8 XCDs; runs of
+            // RH_S_GROUP consecutive items (neighbours sharing micropolygons)
+            // go to one XCD.
+            const int kx = blockIdx.x >> 3;
+            it = itemBase + ((kx / RH_S_GROUP) * 8 + (blockIdx.x & 7)) * RH_S_GROUP + kx % RH_S_GROUP;
+        } else {
+            it = itemBase + blockIdx.x;
+        }
+    }
+    if (it >= itemEnd) break;
+    const uint2 item = items[it];
+    const int task = (int)item.x, blk0 = (int)(item.y & 0xffffu), blkN = (int)(item.y >> 16);
+    const TaskDev T = tasks[task];
+    const BucketDev B = buckets[T.bucket];
+    const int tw = T.x1 - T.x0;
+    const int nbx = (tw + bw - 1) / bw;
     const uint32_t* L = lists + capStart[task];
     const uint32_t n = listCount[task];
-    uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
 
-    for (int p0 = 0; p0 < nblk; p0 += RH_S_WAVES * RH_S_SLOTS) {
-        bool has[RH_S_SLOTS], live[RH_S_SLOTS], lodOk[RH_S_SLOTS];
-        int gx[RH_S_SLOTS], gy[RH_S_SLOTS], bgx[RH_S_SLOTS], bgy[RH_S_SLOTS];
-        float sx[RH_S_SLOTS], sy[RH_S_SLOTS];
-        float occlZ[RH_S_SLOTS], wu[RH_S_SLOTS], wv[RH_S_SLOTS];
-        uint32_t win[RH_S_SLOTS];
+    for (int p0 = 0; p0 < blkN; p0 += RH_S_WAVES * SLOTS) {
+        bool has[SLOTS], live[SLOTS], lodOk[SLOTS];
+        int gx[SLOTS], gy[SLOTS], bgx[SLOTS], bgy[SLOTS];
+        float sx[SLOTS], sy[SLOTS];
+        float occlZ[SLOTS], wu[SLOTS], wv[SLOTS];
+        uint32_t win[SLOTS], head[SLOTS];
+        int nh[SLOTS];
 #pragma unroll
-        for (int k = 0; k < RH_S_SLOTS; ++k) {
-            const int blk = p0 + wave + RH_S_WAVES * k;
-            has[k] = blk < nblk;
+        for (int k = 0; k < SLOTS; ++k) {
+            head[k] = 0xffffffffu;
+            nh[k] = 0;
+            const int blk = blk0 + p0 + wave + RH_S_WAVES * k;
+            has[k] = p0 + wave + RH_S_WAVES * k < blkN;
             const int bx = T.x0 + (blk % nbx) * bw, by = T.y0 + (blk / nbx) * bh;
             const int px = bx + lx, py = by + ly;
             live[k] = has[k] && pix < bw * bh && px < T.x1 && py < T.y1;
@@ -485,8 +540,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
                 lodOk[k] = !(0.0f > lod || lod >= 1.0f);
             }
         }
-        for (uint32_t c0 = 0; c0 < n; c0 += RH_S_CH) {
-            const int cnt = (int)((n - c0) < (uint32_t)RH_S_CH ? (n - c0) : (uint32_t)RH_S_CH);
+        for (uint32_t c0 = 0; c0 < n; c0 += CH) {
+            const int cnt = (int)((n - c0) < (uint32_t)CH ? (n - c0) : (uint32_t)CH);
             __syncthreads();
             if (tid < cnt) {
                 const uint32_t id = L[c0 + tid];
@@ -518,7 +573,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
             }
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < RH_S_SLOTS; ++k) {
+            for (int k = 0; k < SLOTS; ++k) {
                 if (!has[k]) continue;  // uniform across the wave
                 const int BX0 = bgx[k], BX1 = bgx[k] + bw * nx, BY0 = bgy[k], BY1 = bgy[k] + bh * ny;
                 // Ordered list of the chunk's micropolygons that meet this block.
@@ -557,93 +612,135 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
                     }
                     // (b) the order-dependent tests, each lane its own hits in list order.
                     while (__ballot(pb != 0ull)) {
-                        if (pb == 0ull) continue;
-                        const int i = __ffsll((long long)pb) - 1;
-                        pb &= pb - 1ull;
-                        const int j = matched[wave][i0 + i];
-                        const float4* R = &recs[j * 9];
-                        const float4 r8 = R[8];
-                        // occlusion skip against the current opaque hit (bucketprocessor.cpp:1179)
-                        if (r8.x > occlZ[k]) continue;
-                        ++nBound;
-                        // fContains (micropolygon.cpp:1295-1342): edges 0,1 reject on <= 0, edges 2,3 on < 0
-                        const float4 YM = R[0], XM = R[1], X = R[2], Y = R[3];
-                        const float px_ = sx[k], py_ = sy[k];
-                        const float t0 = ((py_ - Y.x) * YM.x) - ((px_ - X.x) * XM.x);
-                        const float t1 = ((py_ - Y.y) * YM.y) - ((px_ - X.y) * XM.y);
-                        const float t2 = ((py_ - Y.z) * YM.z) - ((px_ - X.z) * XM.z);
-                        const float t3 = ((py_ - Y.w) * YM.w) - ((px_ - X.w) * XM.w);
-                        if (t0 <= 0 || t1 <= 0 || t2 < 0 || t3 < 0) continue;
-                        ++nIn;
-                        // CqInvBilinear::operator() (bilinear.h:277-305)
-                        const float4 r5 = R[5], r6 = R[6], Z = R[4];
-                        const float Ax = r5.x, Ay = r5.y, Ex = r5.z, Ey = r5.w, Fx = r6.x, Fy = r6.y, Gx = r6.z,
-                                    Gy = r6.w;
-                        float u = 0.5f, v = 0.5f;
-                        {
-                            const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
-                            const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
-                            const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px_;
-                            const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py_;
-                            float det = M1x * M2y - M1y * M2x;
-                            det = 1 / det;
-                            u = u - det * (bx * M2y - by * M2x);
-                            v = v - det * (-(bx * M1y - by * M1x));
+                        bool app = false;          // a transparent hit to append
+                        float4 hA, hB;
+                        do {
+                            if (pb == 0ull) break;
+                            const int i = __ffsll((long long)pb) - 1;
+                            pb &= pb - 1ull;
+                            const int j = matched[wave][i0 + i];
+                            const float4* R = &recs[j * 9];
+                            const float4 r8 = R[8];
+                            // occlusion skip against the current opaque hit (bucketprocessor.cpp:1179)
+                            if (r8.x > occlZ[k]) break;
+                            ++nBound;
+                            // fContains (micropolygon.cpp:1295-1342): edges 0,1 reject on <= 0, edges 2,3 on < 0
+                            const float4 YM = R[0], XM = R[1], X = R[2], Y = R[3];
+                            const float px_ = sx[k], py_ = sy[k];
+                            const float t0 = ((py_ - Y.x) * YM.x) - ((px_ - X.x) * XM.x);
+                            const float t1 = ((py_ - Y.y) * YM.y) - ((px_ - X.y) * XM.y);
+                            const float t2 = ((py_ - Y.z) * YM.z) - ((px_ - X.z) * XM.z);
+                            const float t3 = ((py_ - Y.w) * YM.w) - ((px_ - X.w) * XM.w);
+                            if (t0 <= 0 || t1 <= 0 || t2 < 0 || t3 < 0) break;
+                            ++nIn;
+                            // CqInvBilinear::operator() (bilinear.h:277-305)
+                            const float4 r5 = R[5], r6 = R[6], Z = R[4];
+                            const float Ax = r5.x, Ay = r5.y, Ex = r5.z, Ey = r5.w, Fx = r6.x, Fy = r6.y,
+                                        Gx = r6.z, Gy = r6.w;
+                            float u = 0.5f, v = 0.5f;
+                            {
+                                const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
+                                const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
+                                const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px_;
+                                const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py_;
+                                float det = M1x * M2y - M1y * M2x;
+                                det = 1 / det;
+                                u = u - det * (bx * M2y - by * M2x);
+                                v = v - det * (-(bx * M1y - by * M1x));
+                            }
+                            if (__float_as_int(r8.y) == 0) {
+                                ++nIn2;
+                                const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
+                                const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
+                                const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px_;
+                                const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py_;
+                                float det = M1x * M2y - M1y * M2x;
+                                if (det != 0) det = 1 / det;
+                                u = u - det * (bx * M2y - by * M2x);
+                                v = v - det * (-(bx * M1y - by * M1x));
+                            }
+                            // bilerp (bilinear.h:228-236)
+                            const float w0 = (1 - v) * (1 - u), w1 = (1 - v) * u, w2 = v * (1 - u), w3 = v * u;
+                            const float D = ((w0 * Z.x + w1 * Z.y) + w2 * Z.z) + w3 * Z.w;
+                            // StoreSample (bucketprocessor.cpp:1471-1569): reject when occlZ <= D
+                            if (occlZ[k] <= D) break;
+                            ++nHit;
+                            if (__float_as_int(r8.w) != 0) {
+                                // opaque: the occluding hit
+                                occlZ[k] = D;
+                                win[k] = ids[j];
+                                wu[k] = u;
+                                wv[k] = v;
+                            } else {
+                                // transparent: a new entry of the sample's hit list
+                                const PrimDev& P = prims[grids[__float_as_int(r8.z)].prim];
+                                shade_hit(P, u, v, D, ids[j], hA, hB);
+                                app = true;
+                            }
+                        } while (false);
+                        // Append the wave's transparent hits (converged): entries come
+                        // from 64-entry blocks of the frame's arena, one atomic per block.
+                        const unsigned long long am = __ballot(app);
+                        if (am) {
+                            const int na = __popcll(am);
+                            if (na > wbLeft) {
+                                uint32_t base = 0;
+                                if (lane == 0) base = atomicAdd(hitCounter, 64u);
+                                wbBase = __shfl(base, 0);
+                                wbLeft = 64;
+                            }
+                            if (app) {
+                                const uint32_t slot =
+                                    wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & ((1ull << lane) - 1ull));
+                                if (slot < fr->hitCapacity && nh[k] < RH_MAX_HITS - 1) {
+                                    hitData[2 * (size_t)slot] = hA;
+                                    hitData[2 * (size_t)slot + 1] = hB;
+                                    hitNext[slot] = head[k];
+                                    head[k] = slot;
+                                    ++nh[k];
+                                } else if (slot < fr->hitCapacity) {
+                                    lost = true;   // more hits than one sample can combine
+                                }
+                            }
+                            wbLeft -= na;
                         }
-                        if (__float_as_int(r8.y) == 0) {
-                            ++nIn2;
-                            const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
-                            const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
-                            const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px_;
-                            const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py_;
-                            float det = M1x * M2y - M1y * M2x;
-                            if (det != 0) det = 1 / det;
-                            u = u - det * (bx * M2y - by * M2x);
-                            v = v - det * (-(bx * M1y - by * M1x));
-                        }
-                        // bilerp (bilinear.h:228-236)
-                        const float w0 = (1 - v) * (1 - u), w1 = (1 - v) * u, w2 = v * (1 - u), w3 = v * u;
-                        const float D = ((w0 * Z.x + w1 * Z.y) + w2 * Z.z) + w3 * Z.w;
-                        // StoreSample (bucketprocessor.cpp:1471-1569): reject when occlZ <= D
-                        if (occlZ[k] <= D) continue;
-                        ++nHit;
-                        occlZ[k] = D;
-                        win[k] = ids[j];
-                        wu[k] = u;
-                        wv[k] = v;
                     }
                 }
             }
         }
 #pragma unroll
-        for (int k = 0; k < RH_S_SLOTS; ++k) {
+        for (int k = 0; k < SLOTS; ++k) {
             if (!live[k]) continue;
             const int px = B.sx0 + gx[k] / nx, py = B.sy0 + gy[k] / ny;
             const size_t sidx = ((size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)) * ns + si;
             float4 a = make_float4(0.f, 0.f, 0.f, FLT_MAX), b = make_float4(0.f, 0.f, 0.f, __uint_as_float(win[k]));
             if (win[k] != 0xffffffffu) {
                 const float4 r8 = mprec4[(size_t)win[k] * 9 + 8];
-                const GridDev G = grids[__float_as_int(r8.z)];
-                const PrimDev P = prims[G.prim];
-                const float u = wu[k], v = wv[k];
-                // InterpolateOutputs, smooth shading with four equal vertex colours
-                // (micropolygon.cpp:1443-1462)
-                const float w0 = (1 - u) * (1 - v), w1 = u * (1 - v), w2 = (1 - u) * v, w3 = u * v;
-                a.x = ((w0 * P.Ci[0] + w1 * P.Ci[0]) + w2 * P.Ci[0]) + w3 * P.Ci[0];
-                a.y = ((w0 * P.Ci[1] + w1 * P.Ci[1]) + w2 * P.Ci[1]) + w3 * P.Ci[1];
-                a.z = ((w0 * P.Ci[2] + w1 * P.Ci[2]) + w2 * P.Ci[2]) + w3 * P.Ci[2];
-                a.w = occlZ[k];
-                b.x = ((w0 * P.Oi[0] + w1 * P.Oi[0]) + w2 * P.Oi[0]) + w3 * P.Oi[0];
-                b.y = ((w0 * P.Oi[1] + w1 * P.Oi[1]) + w2 * P.Oi[1]) + w3 * P.Oi[1];
-                b.z = ((w0 * P.Oi[2] + w1 * P.Oi[2]) + w2 * P.Oi[2]) + w3 * P.Oi[2];
+                const PrimDev& P = prims[grids[__float_as_int(r8.z)].prim];
+                shade_hit(P, wu[k], wv[k], occlZ[k], win[k], a, b);
             }
             samples[2 * sidx] = a;
             samples[2 * sidx + 1] = b;
+            if (frontDepth) frontDepth[sidx] = occlZ[k];
+            // Samples with transparent hits are combined by rh_combine.
+            const bool multi = nh[k] > 0;
+            const unsigned long long cm = __ballot(multi);
+            if (cm) {
+                uint32_t base = 0;
+                if (lane == __ffsll((long long)cm) - 1) base = atomicAdd(&hitCounter[2], (uint32_t)__popcll(cm));
+                base = __shfl(base, __ffsll((long long)cm) - 1);
+                if (multi) {
+                    const uint32_t e = base + (uint32_t)__popcll(cm & ((1ull << lane) - 1ull));
+                    combineList[e] = make_uint4((uint32_t)sidx, (uint32_t)(sidx >> 32), head[k], (uint32_t)nh[k]);
+                }
+            }
         }
     }
-    // The task's hider counters: wave sums, then one row per task (summed
-    // over tasks by rh_reduce_counters; same-address atomics from every wave
-    // would serialise in L2).
+    }   // items
+    if (__ballot(lost) && lane == 0) atomicAdd(&hitCounter[1], 1u);
+    // The workgroup's hider counters: wave sums, then one row (summed by
+    // rh_reduce_counters; same-address atomics from every wave would
+    // serialise in L2).
     for (int off = 32; off > 0; off >>= 1) {
         nTest += __shfl_xor(nTest, off);
         nBound += __shfl_xor(nBound, off);
@@ -659,8 +756,131 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
     if (tid < 5) {
         uint32_t v = 0;
         for (int w = 0; w < RH_S_WAVES; ++w) v += wsum[w][tid];
-        taskCounts[(size_t)task * 5 + tid] = v;
+        rowCounts[(size_t)(rowBase + blockIdx.x) * 5 + tid] = v;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: work items ordered by cost
+// ---------------------------------------------------------------------------
+// One workgroup.  A task's cost is (list length + 16) x blocks; a task is cut
+// into k items of near-equal block ranges, at least enough that no item
+// needs more than one pass (itemBlocks blocks), and more while an item would
+// cost over splitCost (down to one block per item).  Items are written in
+// descending order of log2(cost), so the persistent sampler takes the
+// largest first; the order inside a class is unspecified (results do not
+// depend on it).  Also resets the work queue.
+__global__ void __launch_bounds__(1024) rh_plan_items(const FrameDev* __restrict__ fr,
+                                                      const TaskDev* __restrict__ tasks, int ntasks,
+                                                      const uint32_t* __restrict__ listCount, uint32_t splitCost,
+                                                      int itemBlocks, uint2* __restrict__ items,
+                                                      uint32_t* __restrict__ itemCount,
+                                                      uint32_t* __restrict__ queueHead) {
+    __shared__ uint32_t hist[33], offs[33];
+    const int tid = threadIdx.x;
+    const int bw = fr->blockW, bh = fr->blockH;
+    if (tid < 33) hist[tid] = 0;
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int t = tid; t < ntasks; t += 1024) {
+            const TaskDev T = tasks[t];
+            const int nblk = ((T.x1 - T.x0 + bw - 1) / bw) * ((T.y1 - T.y0 + bh - 1) / bh);
+            const unsigned long long cost = (unsigned long long)(listCount[t] + 16u) * (unsigned)nblk;
+            int k = (nblk + itemBlocks - 1) / itemBlocks;
+            while (k < nblk && cost / (unsigned long long)k > splitCost) k = (2 * k < nblk) ? 2 * k : nblk;
+            // items of `per` blocks; the last may be shorter, so count them
+            const int per = (nblk + k - 1) / k;
+            const int nitems = (nblk + per - 1) / per;
+            unsigned long long c = cost / (unsigned long long)nitems;
+            if (c > 0xffffffffull) c = 0xffffffffull;
+            const int bin = 32 - __clz((uint32_t)c | 1u);   // 1..32
+            if (pass == 0) {
+                atomicAdd(&hist[bin], (uint32_t)nitems);
+            } else {
+                for (int b0 = 0; b0 < nblk; b0 += per) {
+                    const int nb = (nblk - b0) < per ? (nblk - b0) : per;
+                    const uint32_t slot = atomicAdd(&offs[bin], 1u);
+                    items[slot] = make_uint2((uint32_t)t, (uint32_t)b0 | ((uint32_t)nb << 16));
+                }
+            }
+        }
+        __syncthreads();
+        if (pass == 0 && tid == 0) {
+            uint32_t run = 0;
+            for (int b = 32; b >= 0; --b) {
+                offs[b] = run;
+                run += hist[b];
+            }
+            itemCount[0] = run;
+            queueHead[0] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// Clears the frame's counters (graph-safe: a kernel, not a memset).
+__global__ void rh_frame_begin(unsigned long long* __restrict__ counters, uint32_t* __restrict__ hitCtl) {
+    const int t = threadIdx.x;
+    if (t < RH_NCOUNTERS) counters[t] = 0ull;
+    if (t < 4) hitCtl[t] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Combine: samples with transparent hits
+// ---------------------------------------------------------------------------
+// CqImagePixel::Combine (imagepixel.cpp:144-332, no CSG, no matte) for each
+// sample the sampling kernel listed: its occluding hit (already in the
+// sample buffer, if any) joins its transparent hits, the list is sorted by
+// depth as std::sort does, then composited back to front; the depth is that
+// of the nearest hit whose opacity reaches zthreshold, else the occluding
+// depth; the front-most hit gives the id, and its depth goes to frontDepth.
+// One thread per listed sample.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
+                                                  const uint32_t* __restrict__ hitCounter,
+                                                  const float4* __restrict__ hitData,
+                                                  const uint32_t* __restrict__ hitNext, float4* __restrict__ samples,
+                                                  float* __restrict__ frontDepth) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= hitCounter[2]) return;
+    const uint4 e = list[t];
+    const size_t sidx = (size_t)e.x | ((size_t)e.y << 32);
+    uint32_t head = e.z;
+    int n = (int)e.w;
+    const float4 oA = samples[2 * sidx], oB = samples[2 * sidx + 1];
+    const float occlZ = oA.w;
+    float d[RH_MAX_HITS];
+    uint32_t x[RH_MAX_HITS];
+    // the list is linked newest first; rebuild insertion order
+    for (int q = n - 1; q >= 0; --q) {
+        x[q] = head;
+        d[q] = hitData[2 * (size_t)head].w;
+        head = hitNext[head];
+    }
+    if (__float_as_uint(oB.w) != 0xffffffffu) {
+        x[n] = 0xffffffffu;
+        d[n] = oA.w;
+        ++n;
+    }
+    rhsort::sort(d, x, n);
+    float col[3] = {0.0f, 0.0f, 0.0f}, opa[3] = {0.0f, 0.0f, 0.0f};
+    float opaqueDepth = occlZ;
+    const float zt0 = fr->zThreshold[0], zt1 = fr->zThreshold[1], zt2 = fr->zThreshold[2];
+    for (int q = n - 1; q >= 0; --q) {
+        const float4 hA = x[q] == 0xffffffffu ? oA : hitData[2 * (size_t)x[q]];
+        const float4 hB = x[q] == 0xffffffffu ? oB : hitData[2 * (size_t)x[q] + 1];
+        const float Ci[3] = {hA.x, hA.y, hA.z}, Oi[3] = {hB.x, hB.y, hB.z};
+        for (int c = 0; c < 3; ++c) {
+            const float oc = Oi[c] < 0.0f ? 0.0f : (Oi[c] > 1.0f ? 1.0f : Oi[c]);
+            col[c] = col[c] * (1.0f - oc) + Ci[c];
+            opa[c] = (1.0f - opa[c]) * Oi[c] + opa[c];
+        }
+        if (Oi[0] >= zt0 && Oi[1] >= zt1 && Oi[2] >= zt2) opaqueDepth = hA.w;
+    }
+    const uint32_t front = x[0] == 0xffffffffu ? __float_as_uint(oB.w) : __float_as_uint(hitData[2 * (size_t)x[0] + 1].w);
+    samples[2 * sidx] = make_float4(col[0], col[1], col[2], opaqueDepth);
+    frontDepth[sidx] = d[0];
+    samples[2 * sidx + 1] = make_float4(opa[0], opa[1], opa[2], __uint_as_float(front));
 }
 
 // ---------------------------------------------------------------------------
@@ -921,9 +1141,11 @@ __global__ void __launch_bounds__(128) rh_filter_tiled(const FrameDev* __restric
 // Host-side launchers
 // ---------------------------------------------------------------------------
 extern "C" hipError_t rh_launch_dice(hipStream_t st, const FrameDev* fr, const GridDev* grids, int ngrids,
-                                     const float* params, float4* vtx, float* mprec, float4* mpbound) {
+                                     const PrimDev* prims, const float* params, float4* vtx, float* mprec,
+                                     float4* mpbound) {
     if (ngrids <= 0) return hipSuccess;
-    hipLaunchKernelGGL(rh_dice_bust, dim3(ngrids), dim3(256), 0, st, fr, grids, params, vtx, mprec, mpbound);
+    hipLaunchKernelGGL(rh_dice_bust, dim3(ngrids), dim3(256), 0, st, fr, grids, prims, params, vtx, mprec,
+                       mpbound);
     return hipGetLastError();
 }
 
@@ -938,18 +1160,61 @@ extern "C" hipError_t rh_launch_bin(hipStream_t st, const FrameDev* fr, const Bu
     return hipGetLastError();
 }
 
-extern "C" hipError_t rh_launch_sample(hipStream_t st, const FrameDev* fr, const BucketDev* buckets,
-                                       const TaskDev* tasks, int ntasks, const unsigned long long* capStart,
-                                       const uint32_t* listCount, const uint32_t* lists, const float* mprec,
-                                       const GridDev* grids, const PrimDev* prims, const float* pos2d,
-                                       const float* pos1d, const uchar4* pixPattern, float4* samples,
-                                       uint32_t* taskCounts) {
-    if (ntasks <= 0) return hipSuccess;
-    const int unit = 8 * RH_S_GROUP;
-    const int nwg = (ntasks + unit - 1) / unit * unit;
-    hipLaunchKernelGGL(rh_sample, dim3(nwg), dim3(256), 0, st, fr, buckets, tasks, ntasks, capStart, listCount, lists,
-                       reinterpret_cast<const float4*>(mprec), grids, prims, pos2d, pos1d, pixPattern, samples,
-                       taskCounts);
+template <int CH, int SLOTS>
+static void launch_sample_t(hipStream_t st, int grid, const SampleLaunch& a) {
+    hipLaunchKernelGGL((rh_sample<CH, SLOTS>), dim3(grid), dim3(256), 0, st, a.fr, a.buckets, a.tasks, a.items,
+                       a.itemBase, a.nStatic, a.nDevice, a.persistent, a.queueHead, a.capStart, a.listCount, a.lists,
+                       reinterpret_cast<const float4*>(a.mprec), a.grids, a.prims, a.pos2d, a.pos1d, a.pixPattern,
+                       a.samples, a.hitData, a.hitNext, a.hitCounter, a.combineList, a.frontDepth, a.rowCounts,
+                       a.rowBase);
+}
+
+// One sampling launch of `grid` workgroups with kernel variant `variant`
+// (RH_VARIANT_CH / RH_VARIANT_SLOTS).
+extern "C" hipError_t rh_launch_sample(hipStream_t st, int variant, int grid, const SampleLaunch* a) {
+    if (grid <= 0) return hipSuccess;
+    switch (variant) {
+    case 0: launch_sample_t<224, 4>(st, grid, *a); break;
+    case 1: launch_sample_t<224, 2>(st, grid, *a); break;
+    case 2: launch_sample_t<96, 4>(st, grid, *a); break;
+    case 3: launch_sample_t<96, 2>(st, grid, *a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Workgroups of one sampling variant that fit the device at once.
+extern "C" hipError_t rh_sample_occupancy(int variant, int* blocksPerCU) {
+    switch (variant) {
+    case 0: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<224, 4>, 256, 0);
+    case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<224, 2>, 256, 0);
+    case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<96, 4>, 256, 0);
+    case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<96, 2>, 256, 0);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" hipError_t rh_launch_plan_items(hipStream_t st, const FrameDev* fr, const TaskDev* tasks, int ntasks,
+                                           const uint32_t* listCount, uint32_t splitCost, int itemBlocks,
+                                           uint2* items, uint32_t* itemCount, uint32_t* queueHead) {
+    hipLaunchKernelGGL(rh_plan_items, dim3(1), dim3(1024), 0, st, fr, tasks, ntasks, listCount, splitCost,
+                       itemBlocks, items, itemCount, queueHead);
+    return hipGetLastError();
+}
+
+// maxEntries: the combine list's capacity (the extended region's samples);
+// the kernel reads the actual count from hitCounter[2].
+extern "C" hipError_t rh_launch_combine(hipStream_t st, const FrameDev* fr, size_t maxEntries, const uint4* list,
+                                        const uint32_t* hitCounter, const float4* hitData, const uint32_t* hitNext,
+                                        float4* samples, float* frontDepth) {
+    if (maxEntries == 0) return hipSuccess;
+    hipLaunchKernelGGL(rh_combine, dim3((unsigned)((maxEntries + 255) / 256)), dim3(256), 0, st, fr, list,
+                       hitCounter, hitData, hitNext, samples, frontDepth);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rh_launch_frame_begin(hipStream_t st, unsigned long long* counters, uint32_t* hitCtl) {
+    hipLaunchKernelGGL(rh_frame_begin, dim3(1), dim3(64), 0, st, counters, hitCtl);
     return hipGetLastError();
 }
 

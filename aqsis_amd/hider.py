@@ -30,9 +30,20 @@ class RhFootprint(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class RhTuning(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("lds_records", C.c_int32), ("blocks_per_wave", C.c_int32),
+                ("split_cost", C.c_uint32), ("frame_ms", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+MODES = {"grid": 0, "plain": 1, "batched": 2}
+
 EXPORTS = ["rh_options_default", "rh_create", "rh_post_surfaces", "rh_clear_surfaces", "rh_render",
            "rh_render_device", "rh_frame_size", "rh_read_channels", "rh_read_rgba8", "rh_read_samples",
-           "rh_get_stats", "rh_destroy", "rh_last_error", "rh_estimate", "rh_sample_positions"]
+           "rh_get_stats", "rh_destroy", "rh_last_error", "rh_estimate", "rh_sample_positions",
+           "rh_set_tuning", "rh_get_tuning", "rh_autotune"]
 
 
 def lib():
@@ -61,6 +72,9 @@ def lib():
         L.rh_last_error.restype = C.c_char_p
         L.rh_estimate.argtypes = [P(RhOptions), P(RhPrim), C.c_int32, P(RhFootprint)]
         L.rh_sample_positions.argtypes = [P(RhOptions), C.c_void_p, C.c_void_p]
+        L.rh_set_tuning.argtypes = [C.c_void_p, P(RhTuning)]
+        L.rh_get_tuning.argtypes = [C.c_void_p, P(RhTuning)]
+        L.rh_autotune.argtypes = [C.c_void_p, P(RhTuning)]
         for name in EXPORTS:
             if name not in ("rh_options_default", "rh_destroy", "rh_last_error"):
                 getattr(L, name).restype = C.c_int32
@@ -131,6 +145,32 @@ class Hider:
         _check(lib().rh_get_stats(self._ctx, C.byref(st)))
         return st.as_dict()
 
+    def tuning(self):
+        t = RhTuning()
+        _check(lib().rh_get_tuning(self._ctx, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, mode=None, lds_records=None, blocks_per_wave=None, split_cost=None):
+        """Device layout of the following frames (see rh_tuning); unspecified
+        fields keep their current values.  mode: "grid", "plain", "batched"."""
+        t = RhTuning()
+        _check(lib().rh_get_tuning(self._ctx, C.byref(t)))
+        if mode is not None:
+            t.mode = MODES[mode] if isinstance(mode, str) else int(mode)
+        if lds_records is not None:
+            t.lds_records = lds_records
+        if blocks_per_wave is not None:
+            t.blocks_per_wave = blocks_per_wave
+        if split_cost is not None:
+            t.split_cost = split_cost
+        _check(lib().rh_set_tuning(self._ctx, C.byref(t)))
+
+    def autotune(self):
+        """Pick the fastest batched setting for the posted scene (renders frames)."""
+        t = RhTuning()
+        _check(lib().rh_autotune(self._ctx, C.byref(t)))
+        return t.as_dict()
+
     def close(self):
         if self._ctx:
             lib().rh_destroy(self._ctx)
@@ -166,3 +206,99 @@ def sample_positions(opts, ext_shape, display_shape):
     d = np.zeros((h, w), np.float32)
     _check(lib().rh_sample_positions(C.byref(opts), _ptr(xy), _ptr(d)))
     return xy, d
+
+
+class RhServerOptions(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("schedule_threads", C.c_int32), ("memory_budget", C.c_uint64)]
+
+
+class RhJobInfo(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int32), ("state", C.c_int32), ("width", C.c_int32),
+                ("height", C.c_int32), ("device_bytes", C.c_uint64), ("spl_count", C.c_uint64),
+                ("ms_queue", C.c_double), ("ms_run", C.c_double), ("ms_total", C.c_double),
+                ("ms_device", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+SERVER_EXPORTS = ["rh_server_create", "rh_server_submit", "rh_server_wait", "rh_server_poll", "rh_server_read",
+                  "rh_server_release", "rh_server_memory", "rh_server_destroy", "rh_server_last_error"]
+
+
+def _server_lib():
+    L = lib()
+    if not getattr(L, "_server_bound", False):
+        P = C.POINTER
+        L.rh_server_create.argtypes = [P(RhServerOptions), P(C.c_void_p)]
+        L.rh_server_submit.argtypes = [C.c_void_p, P(RhOptions), P(RhPrim), C.c_int32, P(C.c_uint64)]
+        L.rh_server_wait.argtypes = [C.c_void_p, C.c_uint64, P(RhJobInfo)]
+        L.rh_server_poll.argtypes = [C.c_void_p, P(RhJobInfo), C.c_int32, P(C.c_int32)]
+        L.rh_server_read.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
+        L.rh_server_release.argtypes = [C.c_void_p, C.c_uint64]
+        L.rh_server_memory.argtypes = [C.c_void_p, P(C.c_uint64), P(C.c_uint64)]
+        L.rh_server_destroy.argtypes = [C.c_void_p]
+        L.rh_server_destroy.restype = None
+        L.rh_server_last_error.argtypes = []
+        L.rh_server_last_error.restype = C.c_char_p
+        for name in SERVER_EXPORTS[:-2]:
+            getattr(L, name).restype = C.c_int32
+        L._server_bound = True
+    return L
+
+
+def _scheck(status):
+    if status != 0:
+        raise HiderError(f"status {status}: {_server_lib().rh_server_last_error().decode()}")
+
+
+class Server:
+    """Serving loop (rh_server_*): jobs of any frame shape run concurrently on
+    one GPU, admitted by device-memory footprint, contexts resident by shape."""
+
+    def __init__(self, slots=4, memory_budget=64 << 30, schedule_threads=0):
+        self._srv = C.c_void_p()
+        o = RhServerOptions(slots, schedule_threads, memory_budget)
+        _scheck(_server_lib().rh_server_create(C.byref(o), C.byref(self._srv)))
+
+    def submit(self, opts, prims):
+        arr, n = _prims(prims)
+        jid = C.c_uint64()
+        _scheck(_server_lib().rh_server_submit(self._srv, C.byref(opts), arr, n, C.byref(jid)))
+        return jid.value
+
+    def wait(self, jid):
+        info = RhJobInfo()
+        _scheck(_server_lib().rh_server_wait(self._srv, jid, C.byref(info)))
+        return info.as_dict()
+
+    def poll(self, max_jobs=256):
+        infos = (RhJobInfo * max_jobs)()
+        n = C.c_int32()
+        _scheck(_server_lib().rh_server_poll(self._srv, infos, max_jobs, C.byref(n)))
+        return [infos[i].as_dict() for i in range(n.value)]
+
+    def result(self, jid, width, height):
+        ch = np.zeros((height, width, 9), np.float32)
+        rgba = np.zeros((height, width, 4), np.uint8)
+        _scheck(_server_lib().rh_server_read(self._srv, jid, _ptr(ch), _ptr(rgba)))
+        return ch, rgba
+
+    def release(self, jid):
+        _scheck(_server_lib().rh_server_release(self._srv, jid))
+
+    def memory(self):
+        r, hw = C.c_uint64(), C.c_uint64()
+        _scheck(_server_lib().rh_server_memory(self._srv, C.byref(r), C.byref(hw)))
+        return r.value, hw.value
+
+    def close(self):
+        if self._srv:
+            _server_lib().rh_server_destroy(self._srv)
+            self._srv = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -151,6 +151,93 @@ def patches_scene(seed=1, n=10000, res=(1920, 1080), fov=45.0, znear=8.0,
     return opts, prims
 
 
+def _prim_records(n):
+    """A zeroed ctypes array of n RhPrim and a numpy view of its bytes."""
+    arr = (RhPrim * n)()
+    raw = np.frombuffer(arr, dtype=np.uint8).reshape(n, C.sizeof(RhPrim))
+    return arr, raw
+
+
+def _field(raw, name, shape, dtype=np.float32):
+    off = getattr(RhPrim, name).offset
+    size = int(np.prod(shape)) * np.dtype(dtype).itemsize
+    return raw[:, off:off + size].view(dtype).reshape((raw.shape[0],) + tuple(shape))
+
+
+def bicubic_records(P, Cs, Os, shading_rate=1.0):
+    """Bicubic patches as a ctypes RhPrim array, built without a Python loop
+    per patch: P (n, 16, 3), Cs (n, 3), Os (n, 3)."""
+    P = np.asarray(P, np.float32)
+    n = P.shape[0]
+    arr, raw = _prim_records(n)
+    _field(raw, "type", (), np.int32)[:] = RH_PRIM_BICUBIC
+    _field(raw, "nkeys", (), np.int32)[:] = 1
+    _field(raw, "shading_rate", ())[:] = shading_rate
+    _field(raw, "Cs", (3,))[:] = np.asarray(Cs, np.float32)
+    _field(raw, "Os", (3,))[:] = np.asarray(Os, np.float32)
+    eye = np.asarray(_identity16(), np.float32)
+    _field(raw, "xform", (2, 16))[:] = eye
+    _field(raw, "P", (16, 3))[:] = P
+    return arr
+
+
+def clustered_scene(seed=1, res=(2048, 2048), bucket=16, median=2000.0, sigma=1.5, cap=200000,
+                    patch_px=16.0, fov=45.0, znear=8.0, zfar=40.0, jitter=0.1, samples=(4, 4),
+                    filt=("gaussian", 2.0, 2.0), opacity=None):
+    """The batched configuration: bicubic patches clustered so that the
+    micropolygon count of each bucket follows LogNormal(ln median, sigma),
+    clipped at `cap`.  A patch about `patch_px` pixels across dices to about
+    patch_px^2 micropolygons at ShadingRate 1, so bucket b receives
+    round(N_b / patch_px^2) patches (N_b its drawn count; a fractional part
+    is kept with that probability), centred in the bucket with N(0, bucket/4)
+    pixel offsets, at uniform depths in [znear, zfar].  Returns (opts, prims)
+    with prims a ctypes RhPrim array."""
+    rng = np.random.default_rng(seed)
+    opts = default_options(xres=res[0], yres=res[1], fov=fov, xsamples=samples[0], ysamples=samples[1],
+                           filter=filt[0], filter_xwidth=filt[1], filter_ywidth=filt[2],
+                           bucket_x=bucket, bucket_y=bucket)
+    nbx, nby = (res[0] + bucket - 1) // bucket, (res[1] + bucket - 1) // bucket
+    counts = np.minimum(rng.lognormal(np.log(median), sigma, nbx * nby), cap)
+    per = counts / (patch_px * patch_px)
+    npatch = np.floor(per).astype(np.int64)
+    npatch += rng.uniform(size=per.shape) < (per - npatch)
+    bidx = np.repeat(np.arange(nbx * nby), npatch)
+    n = bidx.size
+    # raster centre of each patch, then back to camera space at its depth
+    cx = (bidx % nbx) * bucket + bucket / 2.0 + rng.normal(0.0, bucket / 4.0, n)
+    cy = (bidx // nbx) * bucket + bucket / 2.0 + rng.normal(0.0, bucket / 4.0, n)
+    aspect = res[0] / res[1]
+    tanh = np.tan(np.radians(fov) / 2.0)
+    sx = tanh * max(aspect, 1.0)
+    sy = tanh * max(1.0 / aspect, 1.0)
+    z = rng.uniform(znear, zfar, n)
+    x = (cx / res[0] * 2.0 - 1.0) * sx * z
+    y = (1.0 - cy / res[1] * 2.0) * sy * z
+    px_per_unit_at_1 = (res[0] / 2.0) / sx
+    size = patch_px * z / px_per_unit_at_1
+    # random planar lattice, normal biased toward the camera, plus noise
+    nrm = rng.normal(size=(n, 3)) + np.array([0.0, 0.0, -2.0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    r = rng.normal(size=(n, 3))
+    ua = np.cross(nrm, r)
+    ua /= np.linalg.norm(ua, axis=1, keepdims=True)
+    va = np.cross(nrm, ua)
+    g = np.arange(4) / 3.0 - 0.5
+    su = np.tile(g, 4)             # iu varies fastest
+    sv = np.repeat(g, 4)
+    centre = np.stack([x, y, z], axis=1)
+    P = (centre[:, None, :] + (su[None, :, None] * size[:, None, None]) * ua[:, None, :]
+         + (sv[None, :, None] * size[:, None, None]) * va[:, None, :])
+    P += rng.normal(0.0, jitter, P.shape) * size[:, None, None]
+    cs = rng.uniform(0.0, 1.0, (n, 3))
+    if opacity is None:
+        os_ = np.ones((n, 3))
+    else:
+        o = rng.uniform(opacity[0], opacity[1], n)
+        os_ = np.repeat(o[:, None], 3, axis=1)
+    return opts, bicubic_records(P, cs, os_)
+
+
 def to_rib(opts, prims):
     """RIB text of a scene (camera space; inspection only)."""
     names = {v: k for k, v in __import__("aqsis_amd.abi", fromlist=["FILTERS"]).FILTERS.items()}

@@ -143,6 +143,26 @@ typedef struct rh_footprint {
     uint64_t device_bytes;     /* device memory of the frame (shape + schedule) */
 } rh_footprint;
 
+/*
+ * How a frame is laid out on the device.  None changes a result.
+ *   RH_MODE_GRID     one launch per stage over all sampling tasks (default);
+ *   RH_MODE_PLAIN    the sampling stage launched once per bucket, in bucket
+ *                    order (the reference's one-bucket-at-a-time loop);
+ *   RH_MODE_BATCHED  one captured hipGraph per frame; tasks are cut into
+ *                    work items by cost (list length x blocks), ordered
+ *                    largest first, and pulled from a device queue by
+ *                    persistent workgroups.
+ */
+enum { RH_MODE_GRID = 0, RH_MODE_PLAIN = 1, RH_MODE_BATCHED = 2 };
+
+typedef struct rh_tuning {
+    int32_t mode;              /* RH_MODE_*                                     */
+    int32_t lds_records;       /* micropolygon records per LDS chunk: 224 or 96 */
+    int32_t blocks_per_wave;   /* 64-sample blocks per wave per pass: 4 or 2    */
+    uint32_t split_cost;       /* batched: largest work item, in list entries x blocks */
+    double frame_ms;           /* rh_autotune: device time of the chosen setting */
+} rh_tuning;
+
 typedef struct rh_ctx rh_ctx;
 
 /* Fill *opt with the reference defaults listed above. */
@@ -166,6 +186,16 @@ rh_status rh_render(rh_ctx* ctx);
 /* Run only the device part of the last frame again, on the resident
  * grids and sample plan (dice -> ... -> quantise), for timing. */
 rh_status rh_render_device(rh_ctx* ctx);
+
+/* Device layout of the following frames (no counterpart in the reference:
+ * it has one CPU bucket loop).  RH_ERR_INVALID for values outside those
+ * listed in rh_tuning. */
+rh_status rh_set_tuning(rh_ctx* ctx, const rh_tuning* t);
+rh_status rh_get_tuning(const rh_ctx* ctx, rh_tuning* t);
+
+/* Time every batched setting (LDS chunk, blocks per wave, split cost) on
+ * the posted scene, keep the fastest, and return it.  Renders frames. */
+rh_status rh_autotune(rh_ctx* ctx, rh_tuning* chosen);
 
 /* Width/height of the display region and of the sampled extended region
  * (display region grown by the filter's discrete shift on each side). */
@@ -211,6 +241,58 @@ rh_status rh_sample_positions(const rh_options* opt, float* xy, float* dither);
 void rh_destroy(rh_ctx* ctx);
 
 const char* rh_last_error(void);
+
+/*
+ * Serving (no counterpart in the reference, which renders one frame per
+ * process): render jobs of any frame shape are submitted as they arrive and
+ * run concurrently on one GPU, each on its own context and HIP stream.
+ * Contexts stay resident after a job, keyed by frame shape, so the next job
+ * of that shape reuses its sample plan and device buffers.  Jobs are
+ * admitted in arrival order when one of `slots` workers is free and the
+ * device memory of the resident contexts plus the job's rh_estimate
+ * footprint fits `memory_budget` (idle contexts are evicted oldest first).
+ * A job's result does not depend on what else runs.
+ */
+typedef struct rh_server_options {
+    int32_t slots;             /* jobs in flight at once (1..64)                */
+    int32_t schedule_threads;  /* host threads per job schedule (0: 16/slots)   */
+    uint64_t memory_budget;    /* device bytes for resident contexts            */
+} rh_server_options;
+
+typedef struct rh_job_info {
+    uint64_t id;
+    int32_t status;            /* rh_status of the job                          */
+    int32_t state;             /* 0 queued, 1 running, 2 done                   */
+    int32_t width, height;     /* display size                                  */
+    uint64_t device_bytes;     /* footprint estimate used for admission         */
+    uint64_t spl_count;        /* sample tests of the frame                     */
+    double ms_queue;           /* submit -> admitted                            */
+    double ms_run;             /* admitted -> done (host schedule + device)     */
+    double ms_total;           /* submit -> done                                */
+    double ms_device;          /* device frame time                             */
+} rh_job_info;
+
+typedef struct rh_server rh_server;
+
+rh_status rh_server_create(const rh_server_options* opt, rh_server** out);
+/* Queue a job; inputs are copied.  RH_ERR_CAPACITY if it alone exceeds the
+ * budget. */
+rh_status rh_server_submit(rh_server* srv, const rh_options* opt, const rh_prim* prims, int32_t n,
+                           uint64_t* id);
+/* Block until job `id` is done; the job's own status is returned. */
+rh_status rh_server_wait(rh_server* srv, uint64_t id, rh_job_info* info);
+/* Finished jobs not yet reported by wait or poll (up to max). */
+rh_status rh_server_poll(rh_server* srv, rh_job_info* infos, int32_t max, int32_t* count);
+/* Copy a finished job's channels (9 floats a pixel, as rh_read_channels)
+ * and rgba8; either pointer may be NULL. */
+rh_status rh_server_read(rh_server* srv, uint64_t id, float* channels, uint8_t* rgba8);
+/* Forget a finished job and its host results. */
+rh_status rh_server_release(rh_server* srv, uint64_t id);
+/* Device bytes held by resident contexts now, and the most ever held. */
+rh_status rh_server_memory(rh_server* srv, uint64_t* resident, uint64_t* high_water);
+/* Finish running jobs, fail queued ones, free everything. */
+void rh_server_destroy(rh_server* srv);
+const char* rh_server_last_error(void);
 
 #ifdef __cplusplus
 }

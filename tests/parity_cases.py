@@ -51,6 +51,36 @@ def spheres_wide_gaussian():
     return scenes.spheres_scene(seed=19, n=40, res=(150, 98), samples=(3, 3), filt=("gaussian", 4.0, 4.0))
 
 
+def patches_transparent():
+    # Os uniform in [0.2, 0.8] per patch: per-sample hit lists, depth sort and
+    # back-to-front compositing (imagepixel.cpp:144-332).
+    return scenes.patches_scene(seed=23, n=300, res=(192, 128), opacity=(0.2, 0.8),
+                                filt=("gaussian", 2.0, 2.0))
+
+
+def patches_transparent_8x8_zthreshold():
+    # The transparent configuration's sampling (8x8, gaussian 2x2) on a small
+    # frame, with zthreshold 0.5 so transparent hits also set the depth.
+    opts, prims = scenes.patches_scene(seed=29, n=120, res=(96, 64), samples=(8, 8), opacity=(0.2, 0.8),
+                                       filt=("gaussian", 2.0, 2.0))
+    opts.zthreshold[:] = [0.5, 0.5, 0.5]
+    return opts, prims
+
+
+def spheres_mixed_opacity():
+    opts, prims = scenes.spheres_scene(seed=31, n=48, res=(160, 120))
+    for i, p in enumerate(prims):
+        if i % 2:
+            p.Os[:] = [0.5, 0.25, 0.75]
+    return opts, prims
+
+
+def patches_deep_transparent():
+    # About six transparent layers per sample: more hits than the first
+    # arena holds, so the frame is redone with a larger one.
+    return scenes.patches_scene(seed=37, n=100, res=(128, 96), opacity=(0.2, 0.8), filt=("box", 1.0, 1.0))
+
+
 CASES = {
     "spheres_config": spheres_config,
     "patches_small": patches_small,
@@ -60,4 +90,8 @@ CASES = {
     "patches_gridsampler": patches_gridsampler,
     "spheres_exposure_mitchell": spheres_exposure_mitchell,
     "spheres_wide_gaussian": spheres_wide_gaussian,
+    "patches_transparent": patches_transparent,
+    "patches_transparent_8x8_zthreshold": patches_transparent_8x8_zthreshold,
+    "spheres_mixed_opacity": spheres_mixed_opacity,
+    "patches_deep_transparent": patches_deep_transparent,
 }

@@ -33,10 +33,13 @@ def test_library_exports_every_declared_entry_point():
 def test_struct_layouts_match_the_header(tmp_path):
     """The ctypes mirrors have the C compiler's sizes and field offsets."""
     import subprocess
-    from aqsis_amd.hider import RhFootprint
+    from aqsis_amd.hider import RhFootprint, RhTuning, RhServerOptions, RhJobInfo
     src = tmp_path / "layout.c"
     fields = {"rh_options": [f for f, _ in RhOptions._fields_], "rh_prim": [f for f, _ in RhPrim._fields_],
-              "rh_stats": [f for f, _ in RhStats._fields_], "rh_footprint": [f for f, _ in RhFootprint._fields_]}
+              "rh_stats": [f for f, _ in RhStats._fields_], "rh_footprint": [f for f, _ in RhFootprint._fields_],
+              "rh_tuning": [f for f, _ in RhTuning._fields_],
+              "rh_server_options": [f for f, _ in RhServerOptions._fields_],
+              "rh_job_info": [f for f, _ in RhJobInfo._fields_]}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT}/include/aqsis_hider.h"',
              'int main(void) {']
     for st, fs in fields.items():
@@ -49,7 +52,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     want = []
-    for cls in (RhOptions, RhPrim, RhStats, RhFootprint):
+    for cls in (RhOptions, RhPrim, RhStats, RhFootprint, RhTuning, RhServerOptions, RhJobInfo):
         want.append(C.sizeof(cls))
         want += [getattr(cls, f).offset for f, _ in cls._fields_]
     assert got == want
@@ -67,6 +70,8 @@ def test_create_without_gpu_fails_loudly_or_works():
         pytest.skip("a GPU is present")
     with pytest.raises(hider.HiderError, match="no HIP device"):
         hider.Hider(opts)
+    with pytest.raises(hider.HiderError, match="no HIP device"):
+        hider.Server(slots=2)
 
 
 @pytest.mark.parametrize("field,value,msg", [
