@@ -313,6 +313,8 @@ static rh_status set_hit_capacity(rh_ctx* c, uint64_t n) {
     return RH_OK;
 }
 
+static rh_status upload_schedule(rh_ctx* c);
+
 static rh_status schedule(rh_ctx* c) {
     c->dropGraph();
     auto t0 = std::chrono::steady_clock::now();
@@ -323,6 +325,12 @@ static rh_status schedule(rh_ctx* c) {
     if (!rh::make_schedule(c->shape, c->prims.data(), (int)c->prims.size(), c->sched, err, threads))
         return fail(RH_ERR_INVALID, err);
     c->stats.ms_schedule = ms_since(t0);
+    return upload_schedule(c);
+}
+
+// Device copies of a computed schedule.
+static rh_status upload_schedule(rh_ctx* c) {
+    c->dropGraph();
     auto t1 = std::chrono::steady_clock::now();
     const rh::Schedule& S = c->sched;
     std::vector<GridDev> g(S.grids.size());
@@ -559,15 +567,15 @@ rh_status rh_autotune(rh_ctx* c, rh_tuning* chosen) {
     rh_tuning best;
     double bestMs = 1e300;
     const int lds[2] = {224, 96}, bpw[2] = {4, 2};
-    const uint32_t split[3] = {1u << 14, 1u << 16, 1u << 18};
+    const uint32_t split[4] = {0, 1u << 14, 1u << 16, 1u << 18};   // 0: the static grid
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
-            for (int k = 0; k < 3; ++k) {
+            for (int k = 0; k < 4; ++k) {
                 rh_tuning t;
-                t.mode = RH_MODE_BATCHED;
+                t.mode = split[k] ? RH_MODE_BATCHED : RH_MODE_GRID;
                 t.lds_records = lds[i];
                 t.blocks_per_wave = bpw[j];
-                t.split_cost = split[k];
+                t.split_cost = split[k] ? split[k] : (1u << 16);
                 t.frame_ms = 0.0;
                 rh_status r = rh_set_tuning(c, &t);
                 // first frame captures the graph; the second is timed
@@ -668,8 +676,6 @@ uint64_t rh_internal_device_bytes(const rh_ctx* c) {
     return total;
 }
 
-void rh_internal_set_schedule_threads(rh_ctx* c, int threads) { c->scheduleThreads = threads; }
-
 extern "C" {
 
 static uint64_t frame_bytes(const rh::Shape& s) {
@@ -677,6 +683,50 @@ static uint64_t frame_bytes(const rh::Shape& s) {
     return ext * s.ns * 32 + ext * 4 + (uint64_t)s.width() * s.height() * (9 * 4 + 4 + 4) +
            s.buckets.size() * (sizeof(BucketDev) + 4 + 8 + 4 + 64);
 }
+
+}  // extern "C"
+
+// Device bytes of one frame of a scheduled scene.
+static uint64_t footprint_bytes(const rh::Shape& s, const rh::Schedule& S, const rh_prim* prims, int32_t n) {
+    uint64_t bytes = frame_bytes(s) + S.grids.size() * sizeof(GridDev) + S.params.size() * 4 +
+                     (uint64_t)n * sizeof(PrimDev) + S.candGrid.size() * 4 + S.vertices * 16 +
+                     S.micropolygons * (RH_MPREC_FLOATS * 4 + 16) + S.capStart.back() * 4;
+    bool transparent = false;
+    for (int32_t i = 0; i < n; ++i)
+        if (!(prims[i].Os[0] >= 1.0f && prims[i].Os[1] >= 1.0f && prims[i].Os[2] >= 1.0f)) transparent = true;
+    if (transparent) {
+        // hit arena (36 B an entry, one entry per sample to start), combine
+        // list and front depths
+        const uint64_t nsamp = (uint64_t)s.extW * s.extH * s.ns;
+        const uint64_t arena = std::min<uint64_t>(std::max<uint64_t>(1u << 20, nsamp), 1u << 26);
+        bytes += arena * 36 + nsamp * 20;
+    }
+    return bytes;
+}
+
+// Used by the serving loop: the split/dice schedule of a job, computed once
+// when it is submitted, with its footprint; and installing it in a context.
+bool rh_internal_schedule(const rh_options& opt, const rh_prim* prims, int32_t n, int threads, rh::Schedule& S,
+                          uint64_t& bytes, double& ms, std::string& err) {
+    auto t0 = std::chrono::steady_clock::now();
+    rh::Shape s;
+    if (!rh::make_shape(opt, s, err)) return false;
+    if (!rh::make_schedule(s, prims, n, S, err, threads)) return false;
+    bytes = footprint_bytes(s, S, prims, n);
+    ms = ms_since(t0);
+    return true;
+}
+
+rh_status rh_internal_post_scheduled(rh_ctx* c, const rh_prim* prims, int32_t n, rh::Schedule&& S, double ms) {
+    c->prims.assign(prims, prims + n);
+    c->sched = std::move(S);
+    c->stats.ms_schedule = ms;
+    rh_status r = upload_schedule(c);
+    if (r == RH_OK) c->rendered = false;
+    return r;
+}
+
+extern "C" {
 
 rh_status rh_estimate(const rh_options* opt, const rh_prim* prims, int32_t n, rh_footprint* out) {
     if (!opt || !out || n < 0 || (n > 0 && !prims)) return fail(RH_ERR_INVALID, "bad arguments");
@@ -692,19 +742,7 @@ rh_status rh_estimate(const rh_options* opt, const rh_prim* prims, int32_t n, rh
     out->micropolygons = S.micropolygons;
     out->vertices = S.vertices;
     out->list_capacity = S.capStart.back();
-    out->device_bytes = frame_bytes(s) + S.grids.size() * sizeof(GridDev) + S.params.size() * 4 +
-                        (uint64_t)n * sizeof(PrimDev) + S.candGrid.size() * 4 + S.vertices * 16 +
-                        S.micropolygons * (RH_MPREC_FLOATS * 4 + 16) + S.capStart.back() * 4;
-    bool transparent = false;
-    for (int32_t i = 0; i < n; ++i)
-        if (!(prims[i].Os[0] >= 1.0f && prims[i].Os[1] >= 1.0f && prims[i].Os[2] >= 1.0f)) transparent = true;
-    if (transparent) {
-        // hit arena (36 B an entry, one entry per sample to start), combine
-        // list and front depths
-        const uint64_t nsamp = (uint64_t)s.extW * s.extH * s.ns;
-        const uint64_t arena = std::min<uint64_t>(std::max<uint64_t>(1u << 20, nsamp), 1u << 26);
-        out->device_bytes += arena * 36 + nsamp * 20;
-    }
+    out->device_bytes = footprint_bytes(s, S, prims, n);
     return RH_OK;
 }
 

@@ -22,9 +22,12 @@
 #include <vector>
 
 #include "../../include/aqsis_hider.h"
+#include "front.h"
 
 uint64_t rh_internal_device_bytes(const rh_ctx* c);
-void rh_internal_set_schedule_threads(rh_ctx* c, int threads);
+bool rh_internal_schedule(const rh_options& opt, const rh_prim* prims, int32_t n, int threads, rh::Schedule& S,
+                          uint64_t& bytes, double& ms, std::string& err);
+rh_status rh_internal_post_scheduled(rh_ctx* c, const rh_prim* prims, int32_t n, rh::Schedule&& S, double ms);
 
 namespace {
 
@@ -56,7 +59,9 @@ struct Job {
     uint64_t id = 0;
     rh_options opt;
     std::vector<rh_prim> prims;
-    uint64_t estimate = 0;                // rh_estimate's device bytes
+    uint64_t estimate = 0;                // device bytes of the scheduled frame
+    rh::Schedule sched;                   // computed at submission, used once
+    double msSchedule = 0.0;
     int32_t status = RH_OK;
     int state = 0;                        // 0 queued, 1 running, 2 done
     bool collected = false;
@@ -127,13 +132,10 @@ struct rh_server {
 
     void run(Job* j, Resident& r) {
         rh_status st = RH_OK;
-        if (!r.ctx) {
-            st = rh_create(&j->opt, &r.ctx);
-            if (st == RH_OK) rh_internal_set_schedule_threads(r.ctx, opt.schedule_threads);
-        } else {
-            st = rh_clear_surfaces(r.ctx);
-        }
-        if (st == RH_OK) st = rh_post_surfaces(r.ctx, j->prims.data(), (int32_t)j->prims.size());
+        if (!r.ctx) st = rh_create(&j->opt, &r.ctx);
+        if (st == RH_OK)
+            st = rh_internal_post_scheduled(r.ctx, j->prims.data(), (int32_t)j->prims.size(), std::move(j->sched),
+                                            j->msSchedule);
         if (st == RH_OK) st = rh_render(r.ctx);
         if (st == RH_OK) st = rh_frame_size(r.ctx, &j->width, &j->height, nullptr, nullptr, nullptr);
         if (st == RH_OK) {
@@ -237,12 +239,10 @@ rh_status rh_server_submit(rh_server* s, const rh_options* opt, const rh_prim* p
     std::unique_ptr<Job> j(new Job());
     j->opt = *opt;
     j->prims.assign(prims, prims + n);
-    rh_footprint fp;
-    rh_status st = rh_estimate(opt, prims, n, &fp);
-    if (st != RH_OK) return sfail(st, rh_last_error());
-    // the estimate covers shape and schedule; add the transparent-hit arena
-    // the first frame reserves
-    j->estimate = fp.device_bytes;
+    // the split/dice schedule, on the submitting thread, fixes the footprint
+    std::string err;
+    if (!rh_internal_schedule(*opt, prims, n, s->opt.schedule_threads, j->sched, j->estimate, j->msSchedule, err))
+        return sfail(RH_ERR_INVALID, err);
     if (j->estimate > s->opt.memory_budget)
         return sfail(RH_ERR_CAPACITY, "job needs " + std::to_string(j->estimate) +
                                           " device bytes, more than the server's budget");

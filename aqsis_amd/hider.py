@@ -166,7 +166,8 @@ class Hider:
         _check(lib().rh_set_tuning(self._ctx, C.byref(t)))
 
     def autotune(self):
-        """Pick the fastest batched setting for the posted scene (renders frames)."""
+        """Pick the fastest layout (static grid or batched setting) for the
+        posted scene (renders frames)."""
         t = RhTuning()
         _check(lib().rh_autotune(self._ctx, C.byref(t)))
         return t.as_dict()

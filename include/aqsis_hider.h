@@ -193,8 +193,9 @@ rh_status rh_render_device(rh_ctx* ctx);
 rh_status rh_set_tuning(rh_ctx* ctx, const rh_tuning* t);
 rh_status rh_get_tuning(const rh_ctx* ctx, rh_tuning* t);
 
-/* Time every batched setting (LDS chunk, blocks per wave, split cost) on
- * the posted scene, keep the fastest, and return it.  Renders frames. */
+/* Time the static grid and every batched setting (LDS chunk, blocks per
+ * wave, split cost) on the posted scene, keep the fastest, and return it.
+ * Renders frames. */
 rh_status rh_autotune(rh_ctx* ctx, rh_tuning* chosen);
 
 /* Width/height of the display region and of the sampled extended region
