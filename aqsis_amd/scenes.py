@@ -203,9 +203,17 @@ def clustered_scene(seed=1, res=(2048, 2048), bucket=16, median=2000.0, sigma=1.
     npatch += rng.uniform(size=per.shape) < (per - npatch)
     bidx = np.repeat(np.arange(nbx * nby), npatch)
     n = bidx.size
-    # raster centre of each patch, then back to camera space at its depth
     cx = (bidx % nbx) * bucket + bucket / 2.0 + rng.normal(0.0, bucket / 4.0, n)
     cy = (bidx // nbx) * bucket + bucket / 2.0 + rng.normal(0.0, bucket / 4.0, n)
+    return opts, _patch_records(rng, cx, cy, res, fov, znear, zfar, patch_px, jitter, opacity)
+
+
+def _patch_records(rng, cx, cy, res, fov, znear, zfar, patch_px, jitter, opacity):
+    """Bicubic patches about `patch_px` pixels across centred on raster
+    points (cx, cy) at uniform depths in [znear, zfar]: a random planar 4x4
+    lattice with its normal biased toward the camera plus N(0, jitter) noise
+    relative to its size, Cs uniform, Os = 1 or uniform in `opacity`."""
+    n = cx.size
     aspect = res[0] / res[1]
     tanh = np.tan(np.radians(fov) / 2.0)
     sx = tanh * max(aspect, 1.0)
@@ -215,7 +223,6 @@ def clustered_scene(seed=1, res=(2048, 2048), bucket=16, median=2000.0, sigma=1.
     y = (1.0 - cy / res[1] * 2.0) * sy * z
     px_per_unit_at_1 = (res[0] / 2.0) / sx
     size = patch_px * z / px_per_unit_at_1
-    # random planar lattice, normal biased toward the camera, plus noise
     nrm = rng.normal(size=(n, 3)) + np.array([0.0, 0.0, -2.0])
     nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
     r = rng.normal(size=(n, 3))
@@ -235,7 +242,31 @@ def clustered_scene(seed=1, res=(2048, 2048), bucket=16, median=2000.0, sigma=1.
     else:
         o = rng.uniform(opacity[0], opacity[1], n)
         os_ = np.repeat(o[:, None], 3, axis=1)
-    return opts, bicubic_records(P, cs, os_)
+    return bicubic_records(P, cs, os_)
+
+
+def serving_requests(seed=1, count=2000, sides=(256, 512, 1024, 2048), spp=(2, 4, 8), patches=(100, 20000),
+                     opacity=(0.2, 1.0), rate_hz=None):
+    """The serving stream: `count` render requests, each a square frame of a
+    side drawn from `sides`, PixelSamples n x n with n from `spp`, a number
+    of bicubic patches uniform in `patches` (about 28 pixels across at
+    1920 pixels of width, scaled with the side), Os uniform in `opacity`,
+    gaussian 2x2.  With rate_hz, arrival times of a Poisson process of that
+    rate are returned as well.  Yields (opts, prims, arrival_s)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    for _ in range(count):
+        side = int(rng.choice(sides))
+        n_s = int(rng.choice(spp))
+        npatch = int(rng.integers(patches[0], patches[1] + 1))
+        opts = default_options(xres=side, yres=side, fov=45.0, xsamples=n_s, ysamples=n_s, filter="gaussian",
+                               filter_xwidth=2.0, filter_ywidth=2.0)
+        cx = rng.uniform(0.0, side, npatch)
+        cy = rng.uniform(0.0, side, npatch)
+        prims = _patch_records(rng, cx, cy, (side, side), 45.0, 8.0, 40.0, 28.0 * side / 1920.0, 0.1, opacity)
+        if rate_hz:
+            t += rng.exponential(1.0 / rate_hz)
+        yield opts, prims, t
 
 
 def to_rib(opts, prims):

@@ -134,6 +134,74 @@ def cpu_baseline(opts, prims):
     }
 
 
+def serving(args):
+    """The serving form: a Poisson stream of render requests of mixed side
+    (256..2048), PixelSamples (2x2, 4x4, 8x8), patch count (100..20,000) and
+    opacity (0.2..1) submitted to one rh_server as they arrive (a pool of
+    client threads computes each job's split/dice schedule at submission);
+    jobs run concurrently on the GPU, admitted by device-memory footprint.
+    Latency is client-side, arrival to result."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    from aqsis_amd import scenes
+    from aqsis_amd.hider import Server
+    reqs = list(scenes.serving_requests(seed=1, count=args.requests, rate_hz=args.rate))
+    lat, infos, errs = [None] * len(reqs), [None] * len(reqs), []
+    with Server(slots=args.slots, memory_budget=args.budget_gb << 30) as srv:
+        t_start = time.perf_counter()
+
+        def client(k):
+            opts, prims, arrival = reqs[k]
+            delay = t_start + arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            t_arr = t_start + arrival          # latency counts from the scheduled arrival
+            try:
+                jid = srv.submit(opts, prims)
+                info = srv.wait(jid)
+                srv.release(jid)
+            except Exception as e:      # a refused or failed job is reported, not fatal
+                errs.append(str(e))
+                return
+            lat[k] = (time.perf_counter() - t_arr) * 1e3
+            infos[k] = info
+
+        with ThreadPoolExecutor(max_workers=args.clients) as pool:
+            list(pool.map(client, range(len(reqs))))
+        wall = time.perf_counter() - t_start
+        resident, high = srv.memory()
+    done = [i for i in range(len(reqs)) if infos[i] is not None]
+    ls = sorted(lat[i] for i in done)
+    tests = sum(infos[i]["spl_count"] for i in done)
+    dev = sum(infos[i]["ms_device"] for i in done)
+    pct = lambda q: ls[min(len(ls) - 1, int(q * len(ls)))] if ls else None
+    return {
+        "metric": "micropolygon-sample tests/s",
+        "value": tests / wall,
+        "unit": "sample-tests/s",
+        "n_gpus": 1,
+        "steps": len(done),
+        "warmup": 0,
+        "ms_per_step": wall * 1e3 / max(1, len(done)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "serving: Poisson stream of render requests, side in {256,512,1024,2048}, "
+                               "PixelSamples in {2x2,4x4,8x8}, 100-20,000 bicubic patches, Os in [0.2,1], "
+                               "gaussian 2x2", "requests": len(reqs), "rate_hz": args.rate, "slots": args.slots,
+                   "memory_budget_gb": args.budget_gb, "client_threads": args.clients},
+        "frames_per_s": len(done) / wall,
+        "latency_ms_p50": pct(0.5),
+        "latency_ms_p99": pct(0.99),
+        "device_busy_fraction": dev / (wall * 1e3),
+        "device_memory_high_water_gb": high / 2 ** 30,
+        "failed": len(errs),
+        "errors": errs[:5],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +213,11 @@ def main():
                     help="device layout: auto = autotuned batched form")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="frames of the one-launch-per-bucket form timed beside (0: skip)")
+    ap.add_argument("--requests", type=int, default=200, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=40.0, help="serving: Poisson arrival rate (1/s)")
+    ap.add_argument("--slots", type=int, default=4, help="serving: jobs in flight")
+    ap.add_argument("--budget-gb", type=int, default=160, help="serving: device memory budget")
+    ap.add_argument("--clients", type=int, default=32, help="serving: client threads submitting jobs")
     ap.add_argument("--copy-kernel", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -163,6 +236,9 @@ def main():
               "(screen-space bucket rows per GPU) is a design note, not code.", file=sys.stderr)
         raise SystemExit(2)
 
+    if args.workload == "serving":
+        print(json.dumps(serving(args)))
+        return
     from aqsis_amd.hider import Hider
     opts, prims, desc = workload(args.workload)
     h = Hider(opts)
