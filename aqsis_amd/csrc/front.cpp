@@ -848,6 +848,9 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
             }
     }
     for (int t = 0; t < ntask; ++t) out.capStart[t + 1] += out.capStart[t];
+    out.segBucket.resize(out.candGrid.size());
+    for (int k = 0; k < nb; ++k)
+        for (uint32_t c = out.candStart[k]; c < out.candStart[k + 1]; ++c) out.segBucket[c] = (uint32_t)k;
     return true;
 }
 

@@ -78,6 +78,7 @@ struct Schedule {
     // Candidate grids per region bucket (CSR over the bucket list), in dicing order.
     std::vector<uint32_t> candStart;      // buckets + 1
     std::vector<uint32_t> candGrid;
+    std::vector<uint32_t> segBucket;      // bucket of each candidate entry (a binning segment)
     std::vector<uint64_t> capStart;       // tasks + 1, per-task list capacity offsets
 };
 

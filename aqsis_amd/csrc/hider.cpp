@@ -17,8 +17,9 @@
 extern "C" hipError_t rh_launch_dice(hipStream_t, const FrameDev*, const GridDev*, int, const PrimDev*,
                                      const float*, float4*, float*, float4*);
 extern "C" hipError_t rh_launch_bin(hipStream_t, const FrameDev*, const BucketDev*, int, const TaskDev*,
-                                    const GridDev*, const uint32_t*, const uint32_t*, const unsigned long long*,
-                                    const float4*, uint32_t*, uint32_t*, uint32_t*, unsigned long long*);
+                                    const GridDev*, const uint32_t*, const uint32_t*, const uint32_t*, size_t,
+                                    const unsigned long long*, const float4*, uint32_t*, uint32_t*, uint32_t*,
+                                    uint32_t*, unsigned long long*);
 extern "C" hipError_t rh_launch_sample(hipStream_t, int, int, const SampleLaunch*);
 extern "C" hipError_t rh_sample_occupancy(int, int*);
 extern "C" hipError_t rh_launch_plan_items(hipStream_t, const FrameDev*, const TaskDev*, int, const uint32_t*,
@@ -98,15 +99,16 @@ struct rh_ctx {
     DBuf dFrame, dBuckets, dTasks, dPos2d, dPos1d, dPix, dDither, dFilter, dBoxMask;
     bool tiledFilter = false;
     // schedule
-    DBuf dGrids, dParams, dPrims, dCandStart, dCandGrid, dCapStart;
+    DBuf dGrids, dParams, dPrims, dCandStart, dCandGrid, dCapStart, dSegBucket, dSegCounts;
     // per frame
     DBuf dVtx, dMprec, dMpbound, dLists, dListCount, dSamples, dChannels, dRgba, dCounters, dTaskCounts, dBucketRefs;
     // transparent hits: arena of (Ci, depth), (Oi, id) pairs and list links
     DBuf dHitData, dHitNext, dHitCtl, dCombine, dFrontDepth;
     bool transparent = false;             // some posted surface has Os < 1
     // work items: one per task (grid and per-bucket forms), cost-split (batched)
-    DBuf dItemsStatic, dItems, dItemCount, dQueue, dRows;
+    DBuf dItemsStatic, dItems, dItemCount, dQueue, dRows, dLaneTabs;
     int maxItems = 0, persistentGrid = 0, maxRows = 0;
+    size_t rowTabOffset = 0;              // rowTab's start in dLaneTabs
     rh_tuning tuning;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -126,7 +128,7 @@ struct rh_ctx {
                        &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dVtx, &dMprec, &dMpbound, &dLists,
                        &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dTaskCounts, &dBucketRefs,
                        &dHitData, &dHitNext, &dHitCtl, &dCombine, &dFrontDepth, &dItemsStatic, &dItems,
-                       &dItemCount, &dQueue, &dRows};
+                       &dItemCount, &dQueue, &dRows, &dSegBucket, &dSegCounts, &dLaneTabs};
         dropGraph();
         for (DBuf* b : all) b->release();
         for (auto& e : ev)
@@ -271,6 +273,22 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
         if (e == hipSuccess) e = c->dItemCount.ensure(16);
         if (e == hipSuccess) e = c->dQueue.ensure(16);
         if (e == hipSuccess) e = c->dRows.ensure((size_t)c->maxRows * 5 * sizeof(uint32_t));
+        // Lane masks of a wave block by sub-sample column and row range.
+        const int W = s.blockW * s.nx, H = s.blockH * s.ny;
+        std::vector<unsigned long long> tabs((size_t)(W + 1) * (W + 1) + (size_t)(H + 1) * (H + 1), 0ull);
+        for (int l = 0; l < 64; ++l) {
+            const int pix = l / s.ns, si = l % s.ns;
+            if (pix >= s.blockW * s.blockH) continue;
+            const int cx = (pix % s.blockW) * s.nx + si % s.nx, cy = (pix / s.blockW) * s.ny + si / s.nx;
+            for (int a = 0; a <= W; ++a)
+                for (int b = a + 1; b <= W; ++b)
+                    if (a <= cx && cx < b) tabs[(size_t)a * (W + 1) + b] |= 1ull << l;
+            for (int a = 0; a <= H; ++a)
+                for (int b = a + 1; b <= H; ++b)
+                    if (a <= cy && cy < b) tabs[(size_t)(W + 1) * (W + 1) + (size_t)a * (H + 1) + b] |= 1ull << l;
+        }
+        if (e == hipSuccess) e = upload(c->dLaneTabs, tabs, c->st);
+        c->rowTabOffset = (size_t)(W + 1) * (W + 1);
     }
     c->tuning.mode = RH_MODE_GRID;
     c->tuning.lds_records = 224;
@@ -298,6 +316,20 @@ rh_status rh_clear_surfaces(rh_ctx* c) {
     c->scheduled = false;
     c->rendered = false;
     return RH_OK;
+}
+
+// Transparent-hit arena entries a frame is likely to need: each transparent
+// micropolygon covers about one pixel at ShadingRate 1, so about ns samples
+// (at least 1M entries).  A frame that needs more is redone once with the
+// exact size.
+static uint64_t arena_guess(const rh::Shape& s, const rh::Schedule& S, const rh_prim* prims) {
+    uint64_t tmp = 0;
+    for (const auto& g : S.grids) {
+        const rh_prim& q = prims[g.prim];
+        if (!(q.Os[0] >= 1.0f && q.Os[1] >= 1.0f && q.Os[2] >= 1.0f)) tmp += (uint64_t)g.cu * g.cv;
+    }
+    if (tmp == 0) return 0;
+    return std::min<uint64_t>(std::max<uint64_t>(1u << 20, tmp * (uint64_t)s.ns), 0x7fffffffull);
 }
 
 // Size the transparent-hit arena and publish its capacity to the kernels.
@@ -354,6 +386,8 @@ static rh_status upload_schedule(rh_ctx* c) {
     HIP_TRY(upload(c->dPrims, p, c->st));
     HIP_TRY(upload(c->dCandStart, S.candStart, c->st));
     HIP_TRY(upload(c->dCandGrid, S.candGrid, c->st));
+    HIP_TRY(upload(c->dSegBucket, S.segBucket, c->st));
+    HIP_TRY(c->dSegCounts.ensure(S.candGrid.size() * 17 * sizeof(uint32_t)));
     HIP_TRY(upload(c->dCapStart, cap, c->st));
     HIP_TRY(c->dVtx.ensure(S.vertices * sizeof(float4)));
     HIP_TRY(c->dMprec.ensure(S.micropolygons * RH_MPREC_FLOATS * sizeof(float)));
@@ -367,9 +401,7 @@ static rh_status upload_schedule(rh_ctx* c) {
     c->transparent = transparent;
     const rh::Shape& s = c->shape;
     const uint64_t nsamp = (uint64_t)s.extW * s.extH * s.ns;
-    // one entry per sample to start (at least 1M, at most 64M); a frame that
-    // needs more is redone once with the exact size
-    const uint64_t want = transparent ? std::min<uint64_t>(std::max<uint64_t>(1u << 20, nsamp), 1u << 26) : 64;
+    const uint64_t want = transparent ? arena_guess(s, S, c->prims.data()) : 64;
     HIP_TRY(c->dCombine.ensure((transparent ? nsamp : 1) * sizeof(uint4)));
     HIP_TRY(c->dFrontDepth.ensure((transparent ? nsamp : 1) * sizeof(float)));
     if (want > c->frame.hitCapacity) {
@@ -402,8 +434,9 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
                        c->dParams.as<float>(), c->dVtx.as<float4>(), c->dMprec.as<float>(), c->dMpbound.as<float4>()));
     if (timed) ENQ(hipEventRecord(c->ev[1], c->st));
     ENQ(rh_launch_bin(c->st, f, c->dBuckets.as<BucketDev>(), nb, c->dTasks.as<TaskDev>(), c->dGrids.as<GridDev>(),
-                      c->dCandStart.as<uint32_t>(), c->dCandGrid.as<uint32_t>(), c->dCapStart.as<unsigned long long>(),
-                      c->dMpbound.as<float4>(), c->dLists.as<uint32_t>(), c->dListCount.as<uint32_t>(),
+                      c->dCandStart.as<uint32_t>(), c->dCandGrid.as<uint32_t>(), c->dSegBucket.as<uint32_t>(),
+                      S.candGrid.size(), c->dCapStart.as<unsigned long long>(), c->dMpbound.as<float4>(),
+                      c->dSegCounts.as<uint32_t>(), c->dLists.as<uint32_t>(), c->dListCount.as<uint32_t>(),
                       c->dBucketRefs.as<uint32_t>(), c->dCounters.as<unsigned long long>()));
     if (timed) ENQ(hipEventRecord(c->ev[2], c->st));
     SampleLaunch a;
@@ -417,6 +450,8 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
     a.hitNext = c->dHitNext.as<uint32_t>(); a.hitCounter = c->dHitCtl.as<uint32_t>();
     a.combineList = c->dCombine.as<uint4>(); a.frontDepth = c->transparent ? c->dFrontDepth.as<float>() : nullptr;
     a.rowCounts = c->dRows.as<uint32_t>(); a.rowBase = 0;
+    a.colTab = c->dLaneTabs.as<unsigned long long>();
+    a.rowTab = c->dLaneTabs.as<unsigned long long>() + c->rowTabOffset;
     const int variant = variant_of(tu);
     int rows = 0;
     if (tu.mode == RH_MODE_PLAIN) {
@@ -667,10 +702,10 @@ void rh_destroy(rh_ctx* c) { delete c; }
 uint64_t rh_internal_device_bytes(const rh_ctx* c) {
     const DBuf* all[] = {&c->dFrame, &c->dBuckets, &c->dTasks, &c->dPos2d, &c->dPos1d, &c->dPix, &c->dDither,
                          &c->dFilter, &c->dBoxMask, &c->dGrids, &c->dParams, &c->dPrims, &c->dCandStart,
-                         &c->dCandGrid, &c->dCapStart, &c->dVtx, &c->dMprec, &c->dMpbound, &c->dLists,
+                         &c->dCandGrid, &c->dCapStart, &c->dSegBucket, &c->dSegCounts, &c->dVtx, &c->dMprec, &c->dMpbound, &c->dLists,
                          &c->dListCount, &c->dSamples, &c->dChannels, &c->dRgba, &c->dCounters, &c->dTaskCounts,
                          &c->dBucketRefs, &c->dHitData, &c->dHitNext, &c->dHitCtl, &c->dCombine, &c->dFrontDepth,
-                         &c->dItemsStatic, &c->dItems, &c->dItemCount, &c->dQueue, &c->dRows};
+                         &c->dItemsStatic, &c->dItems, &c->dItemCount, &c->dQueue, &c->dRows, &c->dLaneTabs};
     uint64_t total = 0;
     for (const DBuf* b : all) total += b->cap;
     return total;
@@ -689,17 +724,15 @@ static uint64_t frame_bytes(const rh::Shape& s) {
 // Device bytes of one frame of a scheduled scene.
 static uint64_t footprint_bytes(const rh::Shape& s, const rh::Schedule& S, const rh_prim* prims, int32_t n) {
     uint64_t bytes = frame_bytes(s) + S.grids.size() * sizeof(GridDev) + S.params.size() * 4 +
-                     (uint64_t)n * sizeof(PrimDev) + S.candGrid.size() * 4 + S.vertices * 16 +
+                     (uint64_t)n * sizeof(PrimDev) + S.candGrid.size() * (4 + 4 + 17 * 4) + S.vertices * 16 +
                      S.micropolygons * (RH_MPREC_FLOATS * 4 + 16) + S.capStart.back() * 4;
     bool transparent = false;
     for (int32_t i = 0; i < n; ++i)
         if (!(prims[i].Os[0] >= 1.0f && prims[i].Os[1] >= 1.0f && prims[i].Os[2] >= 1.0f)) transparent = true;
     if (transparent) {
-        // hit arena (36 B an entry, one entry per sample to start), combine
-        // list and front depths
+        // hit arena (36 B an entry), combine list and front depths
         const uint64_t nsamp = (uint64_t)s.extW * s.extH * s.ns;
-        const uint64_t arena = std::min<uint64_t>(std::max<uint64_t>(1u << 20, nsamp), 1u << 26);
-        bytes += arena * 36 + nsamp * 20;
+        bytes += arena_guess(s, S, prims) * 36 + nsamp * 20;
     }
     return bytes;
 }

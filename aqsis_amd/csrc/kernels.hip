@@ -1,22 +1,27 @@
 // HIP kernels of the hider for gfx950 (CDNA4).
 //
-//   rh_dice_bust   one workgroup per grid: dice (sphere / bicubic / bilinear),
-//                  project to raster, and write one hit-test record per
-//                  micropolygon (its edge equations, inverse-bilinear set-up,
-//                  depths and bound).
-//   rh_bin         one workgroup per bucket: walks the bucket's candidate
-//                  grids in dicing order, keeps the micropolygons the bucket
-//                  would have received (AddMPG) and that touch its sample
-//                  region, and appends them in order with a stable
-//                  block-wide compaction.
-//   rh_sample      one wave per block of 64 samples: walks its bucket's
-//                  ordered list, ballot-filters the micropolygons whose pixel
-//                  window meets the block, and runs the reference's per-sample
-//                  test (bound, occlusion, LOD, point-in-quad, inverse
-//                  bilinear, depth) in list order, so each sample sees the
-//                  micropolygons in exactly the reference's order.
-//   rh_filter      one thread per output pixel: 2-D filter gather with the
-//                  per-sample box test, exposure, quantise + dither.
+//   rh_dice_bust        one workgroup per grid: dice (sphere / bicubic /
+//                       bilinear), project to raster, and write one hit-test
+//                       record per micropolygon (edge equations, inverse-
+//                       bilinear set-up, depths, bound, opacity).
+//   rh_bin_count/scan/write
+//                       per (bucket, candidate grid) segment: keep the
+//                       micropolygons the bucket would have received (AddMPG)
+//                       and append them, in order, to the lists of the tasks
+//                       (parts of the bucket's sample region) they touch.
+//   rh_plan_items       batched form: cut tasks into work items by cost,
+//                       largest first.
+//   rh_sample           one wave per block of 64 samples, a workgroup per
+//                       work item: the reference's per-sample tests (bound,
+//                       occlusion, LOD, point-in-quad, inverse bilinear,
+//                       depth, store) in list order; transparent hits go to
+//                       per-sample lists.
+//   rh_combine          samples with transparent hits: depth sort as
+//                       std::sort, back-to-front compositing.
+//   rh_box_masks        per frame shape: each sample's filter-box tests.
+//   rh_filter_tiled     8x8-pixel tiles, halo in LDS: the filter gather,
+//                       exposure, quantise + dither (rh_filter: the general
+//                       thread-per-pixel form).
 //
 // All single-precision code is built with -ffp-contract=off and correctly
 // rounded division; every expression keeps the operand order of the aqsis
@@ -272,107 +277,167 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
 // ---------------------------------------------------------------------------
 // Binning: the ordered micropolygon list of each sampling task
 // ---------------------------------------------------------------------------
-// One workgroup per bucket.  Every micropolygon of the bucket's candidate
-// grids (in dicing order) goes through CqImageBuffer::AddMPG's tests for this
-// bucket; a kept one is appended, in order, to the list of each of the
-// bucket's tasks that its clipped pixel window meets.  Lists never exceed the
-// host's capacities; an overflow is counted in counters[5].
+// A segment is one (bucket, candidate grid) pair; a bucket's segments are in
+// dicing order and a grid has at most 256 micropolygons, so one 256-thread
+// workgroup per segment sees all of them.  Every micropolygon goes through
+// CqImageBuffer::AddMPG's tests for the bucket (imagebuffer.cpp:514-591),
+// and a kept one belongs to the list of each of the bucket's tasks that its
+// clipped pixel window meets.  Three passes keep the reference's order with
+// the work spread evenly however uneven the buckets are: count per segment
+// and task, an exclusive scan over each bucket's segments, then an ordered
+// write.  Lists never exceed the host's capacities; an overflow is counted
+// in counters[5].
 #define RH_MAX_TASKS 16
-__global__ void __launch_bounds__(256) rh_bin(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets,
-                                              const TaskDev* __restrict__ tasks, const GridDev* __restrict__ grids,
-                                              const uint32_t* __restrict__ candStart,
-                                              const uint32_t* __restrict__ candGrid,
-                                              const unsigned long long* __restrict__ capStart,
-                                              const float4* __restrict__ mpbound, uint32_t* __restrict__ lists,
-                                              uint32_t* __restrict__ listCount, uint32_t* __restrict__ bucketRefs,
-                                              unsigned long long* __restrict__ counters) {
-    __shared__ uint32_t wtot[4][RH_MAX_TASKS];
-    __shared__ uint32_t wrefs[4];
-    __shared__ uint32_t tcount[RH_MAX_TASKS];
-    __shared__ int trect[RH_MAX_TASKS][4];
+#define RH_SEG_STRIDE (RH_MAX_TASKS + 1)   // per segment: a count per task, then kept
+
+// AddMPG for bucket B, then RenderMPG_Static's pixel range clipped to the
+// bucket's sample region (bucketprocessor.cpp:1120-1140), relative to the
+// region; empty ranges do no work and are not kept.
+__device__ __forceinline__ bool bin_window(const FrameDev* __restrict__ fr, const BucketDev& B, const float4 bd,
+                                           int& sX, int& eX, int& sY, int& eY) {
+    const float hfx = fr->filterX * 0.5f, hfy = fr->filterY * 0.5f;   // xFiltSize / 2.0f
+    const float dsx = (float)fr->shiftX, dsy = (float)fr->shiftY;      // lfloor(xFiltSize / 2.0f)
+    const bool discard = bd.y < fr->cropX0 - hfx || bd.w < fr->cropY0 - hfy || bd.x > fr->cropX1 + hfx ||
+                         bd.z > fr->cropY1 + hfy;
+    if (discard) return false;
+    const float ex0 = bd.x - dsx, ey0 = bd.z - dsy, ex1 = bd.y + dsx, ey1 = bd.w + dsy;
+    int iXBa = (int)(ex0 / (float)fr->bucketX), iYBa = (int)(ey0 / (float)fr->bucketY);
+    int iXBb = (int)(ex1 / (float)fr->bucketX), iYBb = (int)(ey1 / (float)fr->bucketY);
+    if (iXBb < fr->bx0 || iYBb < fr->by0 || iXBa >= fr->bx1 || iYBa >= fr->by1) return false;
+    if (iXBa < fr->bx0) iXBa = fr->bx0;
+    if (iYBa < fr->by0) iYBa = fr->by0;
+    if (iXBb >= fr->bx1) iXBb = fr->bx1 - 1;
+    if (iYBb >= fr->by1) iYBb = fr->by1 - 1;
+    if (!(B.bx >= iXBa && B.bx <= iXBb && B.by >= iYBa && B.by <= iYBb)) return false;
+    eX = (int)lceil_f(bd.y); eY = (int)lceil_f(bd.w);
+    if (eX > B.sx1) eX = B.sx1;
+    if (eY > B.sy1) eY = B.sy1;
+    sX = floor_i(bd.x); sY = floor_i(bd.z);
+    if (sY < B.sy0) sY = B.sy0;
+    if (sX < B.sx0) sX = B.sx0;
+    if (!(sX < eX && sY < eY)) return false;
+    sX -= B.sx0; eX -= B.sx0; sY -= B.sy0; eY -= B.sy0;
+    return true;
+}
+
+// Pass 1: per segment, the kept micropolygons per task and in all.
+__global__ void __launch_bounds__(256) rh_bin_count(const FrameDev* __restrict__ fr,
+                                                    const BucketDev* __restrict__ buckets,
+                                                    const TaskDev* __restrict__ tasks, const GridDev* __restrict__ grids,
+                                                    const uint32_t* __restrict__ candGrid,
+                                                    const uint32_t* __restrict__ segBucket,
+                                                    const float4* __restrict__ mpbound,
+                                                    uint32_t* __restrict__ segCounts) {
+    __shared__ uint32_t wtot[4][RH_SEG_STRIDE];
+    const size_t seg = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const BucketDev B = buckets[segBucket[seg]];
+    const GridDev G = grids[candGrid[seg]];
+    const int nt = B.ntask;
+    int sX = 0, eX = 0, sY = 0, eY = 0;
+    const bool keep = tid < G.cu * G.cv && bin_window(fr, B, mpbound[G.mpBase + tid], sX, eX, sY, eY);
+    for (int t = 0; t <= nt; ++t) {
+        bool ok = keep;
+        if (t < nt) {
+            const TaskDev T = tasks[B.task0 + t];
+            ok = keep && sX < T.x1 - B.sx0 && eX > T.x0 - B.sx0 && sY < T.y1 - B.sy0 && eY > T.y0 - B.sy0;
+        }
+        const unsigned long long m = __ballot(ok);
+        if (lane == 0) wtot[wave][t < nt ? t : RH_MAX_TASKS] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (tid <= nt) {
+        const int k = tid < nt ? tid : RH_MAX_TASKS;
+        segCounts[seg * RH_SEG_STRIDE + k] = wtot[0][k] + wtot[1][k] + wtot[2][k] + wtot[3][k];
+    }
+}
+
+// Pass 2: per bucket, the exclusive scan of each task's counts over the
+// bucket's segments (in place), list lengths and the bucket's kept count.
+__global__ void __launch_bounds__(256) rh_bin_scan(const BucketDev* __restrict__ buckets,
+                                                   const uint32_t* __restrict__ candStart,
+                                                   const unsigned long long* __restrict__ capStart,
+                                                   uint32_t* __restrict__ segCounts, uint32_t* __restrict__ listCount,
+                                                   uint32_t* __restrict__ bucketRefs,
+                                                   unsigned long long* __restrict__ counters) {
+    __shared__ uint32_t wsum[4];
     const int b = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const BucketDev B = buckets[b];
-    const int bx = B.bx, by = B.by, nt = B.ntask, t0 = B.task0;
-    if (tid < nt) {
-        const TaskDev T = tasks[t0 + tid];
-        tcount[tid] = 0;
-        trect[tid][0] = T.x0 - B.sx0; trect[tid][1] = T.x1 - B.sx0;
-        trect[tid][2] = T.y0 - B.sy0; trect[tid][3] = T.y1 - B.sy0;
-    }
-    __syncthreads();
-    uint32_t refs = 0;
-    const float hfx = fr->filterX * 0.5f, hfy = fr->filterY * 0.5f;   // xFiltSize / 2.0f
-    const float dsx = (float)fr->shiftX, dsy = (float)fr->shiftY;      // lfloor(xFiltSize / 2.0f)
-    for (uint32_t c = candStart[b]; c < candStart[b + 1]; ++c) {
-        const GridDev G = grids[candGrid[c]];
-        const int nmp = G.cu * G.cv;
-        for (int base = 0; base < nmp; base += 256) {
-            const int m = base + tid;
-            bool keep = false;
-            int sX = 0, eX = 0, sY = 0, eY = 0;
-            if (m < nmp) {
-                const float4 bd = mpbound[G.mpBase + m];  // bminx, bmaxx, bminy, bmaxy
-                // CqImageBuffer::AddMPG (imagebuffer.cpp:514-591)
-                bool discard = bd.y < fr->cropX0 - hfx || bd.w < fr->cropY0 - hfy || bd.x > fr->cropX1 + hfx ||
-                               bd.z > fr->cropY1 + hfy;
-                if (!discard) {
-                    float ex0 = bd.x - dsx, ey0 = bd.z - dsy, ex1 = bd.y + dsx, ey1 = bd.w + dsy;
-                    int iXBa = (int)(ex0 / (float)fr->bucketX), iYBa = (int)(ey0 / (float)fr->bucketY);
-                    int iXBb = (int)(ex1 / (float)fr->bucketX), iYBb = (int)(ey1 / (float)fr->bucketY);
-                    bool outside = iXBb < fr->bx0 || iYBb < fr->by0 || iXBa >= fr->bx1 || iYBa >= fr->by1;
-                    if (!outside) {
-                        if (iXBa < fr->bx0) iXBa = fr->bx0;
-                        if (iYBa < fr->by0) iYBa = fr->by0;
-                        if (iXBb >= fr->bx1) iXBb = fr->bx1 - 1;
-                        if (iYBb >= fr->by1) iYBb = fr->by1 - 1;
-                        if (bx >= iXBa && bx <= iXBb && by >= iYBa && by <= iYBb) {
-                            // RenderMPG_Static pixel range clipped to the sample region
-                            // (bucketprocessor.cpp:1120-1140); empty ranges do no work.
-                            eX = (int)lceil_f(bd.y); eY = (int)lceil_f(bd.w);
-                            if (eX > B.sx1) eX = B.sx1;
-                            if (eY > B.sy1) eY = B.sy1;
-                            sX = floor_i(bd.x); sY = floor_i(bd.z);
-                            if (sY < B.sy0) sY = B.sy0;
-                            if (sX < B.sx0) sX = B.sx0;
-                            keep = sX < eX && sY < eY;
-                            sX -= B.sx0; eX -= B.sx0; sY -= B.sy0; eY -= B.sy0;
-                        }
-                    }
-                }
+    const uint32_t s0 = candStart[b], s1 = candStart[b + 1];
+    for (int t = 0; t <= B.ntask; ++t) {
+        const int k = t < B.ntask ? t : RH_MAX_TASKS;
+        uint32_t run = 0;
+        for (uint32_t c0 = s0; c0 < s1; c0 += 256) {
+            const uint32_t s = c0 + tid;
+            const uint32_t v = s < s1 ? segCounts[(size_t)s * RH_SEG_STRIDE + k] : 0u;
+            // inclusive scan in the wave
+            uint32_t x = v;
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t y = __shfl_up(x, off);
+                if (lane >= off) x += y;
             }
-            refs += (uint32_t)__popcll(__ballot(keep));
-            // Stable workgroup-wide compaction into each task's list.
-            for (int t = 0; t < nt; ++t) {
-                const bool ok = keep && sX < trect[t][1] && eX > trect[t][0] && sY < trect[t][3] && eY > trect[t][2];
-                const unsigned long long msk = __ballot(ok);
-                if (lane == 0) wtot[wave][t] = (uint32_t)__popcll(msk);
-            }
+            if (lane == 63) wsum[wave] = x;
             __syncthreads();
-            for (int t = 0; t < nt; ++t) {
-                const bool ok = keep && sX < trect[t][1] && eX > trect[t][0] && sY < trect[t][3] && eY > trect[t][2];
-                const unsigned long long msk = __ballot(ok);
-                if (ok) {
-                    uint32_t pos = tcount[t] + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
-                    for (int w = 0; w < wave; ++w) pos += wtot[w][t];
-                    const unsigned long long c0 = capStart[t0 + t];
-                    if ((unsigned long long)pos < capStart[t0 + t + 1] - c0) lists[c0 + pos] = G.mpBase + (uint32_t)m;
-                }
+            uint32_t base = run, total = 0;
+            for (int w = 0; w < 4; ++w) {
+                if (w < wave) base += wsum[w];
+                total += wsum[w];
             }
-            __syncthreads();
-            if (tid < nt) tcount[tid] += wtot[0][tid] + wtot[1][tid] + wtot[2][tid] + wtot[3][tid];
+            if (s < s1 && k < RH_MAX_TASKS) segCounts[(size_t)s * RH_SEG_STRIDE + k] = base + x - v;
+            run += total;
             __syncthreads();
         }
+        if (tid == 0) {
+            if (k == RH_MAX_TASKS) {
+                bucketRefs[b] = run;
+            } else {
+                const unsigned long long cap = capStart[B.task0 + t + 1] - capStart[B.task0 + t];
+                listCount[B.task0 + t] = (unsigned long long)run < cap ? run : (uint32_t)cap;
+                if ((unsigned long long)run > cap) atomicAdd(&counters[5], 1ull);
+            }
+        }
     }
-    if (tid < nt) {
-        const unsigned long long cap = capStart[t0 + tid + 1] - capStart[t0 + tid];
-        const uint32_t cnt = tcount[tid];
-        listCount[t0 + tid] = (unsigned long long)cnt < cap ? cnt : (uint32_t)cap;
-        if ((unsigned long long)cnt > cap) atomicAdd(&counters[5], 1ull);
+}
+
+// Pass 3: per segment, each task's kept micropolygons in order at the
+// segment's offset.
+__global__ void __launch_bounds__(256) rh_bin_write(const FrameDev* __restrict__ fr,
+                                                    const BucketDev* __restrict__ buckets,
+                                                    const TaskDev* __restrict__ tasks, const GridDev* __restrict__ grids,
+                                                    const uint32_t* __restrict__ candGrid,
+                                                    const uint32_t* __restrict__ segBucket,
+                                                    const float4* __restrict__ mpbound,
+                                                    const uint32_t* __restrict__ segCounts,
+                                                    const unsigned long long* __restrict__ capStart,
+                                                    uint32_t* __restrict__ lists) {
+    __shared__ uint32_t wtot[4][RH_MAX_TASKS];
+    const size_t seg = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const BucketDev B = buckets[segBucket[seg]];
+    const GridDev G = grids[candGrid[seg]];
+    const int nt = B.ntask;
+    int sX = 0, eX = 0, sY = 0, eY = 0;
+    const bool keep = tid < G.cu * G.cv && bin_window(fr, B, mpbound[G.mpBase + tid], sX, eX, sY, eY);
+    uint32_t okMask = 0;
+    for (int t = 0; t < nt; ++t) {
+        const TaskDev T = tasks[B.task0 + t];
+        const bool ok = keep && sX < T.x1 - B.sx0 && eX > T.x0 - B.sx0 && sY < T.y1 - B.sy0 && eY > T.y0 - B.sy0;
+        okMask |= (ok ? 1u : 0u) << t;
+        const unsigned long long m = __ballot(ok);
+        if (lane == 0) wtot[wave][t] = (uint32_t)__popcll(m);
     }
-    if (lane == 0) wrefs[wave] = refs;
     __syncthreads();
-    if (tid == 0) bucketRefs[b] = wrefs[0] + wrefs[1] + wrefs[2] + wrefs[3];
+    for (int t = 0; t < nt; ++t) {
+        const bool ok = (okMask >> t) & 1u;
+        const unsigned long long m = __ballot(ok);
+        if (ok) {
+            uint32_t pos = segCounts[seg * RH_SEG_STRIDE + t] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            for (int w = 0; w < wave; ++w) pos += wtot[w][t];
+            const unsigned long long c0 = capStart[B.task0 + t];
+            if ((unsigned long long)pos < capStart[B.task0 + t + 1] - c0) lists[c0 + pos] = G.mpBase + (uint32_t)tid;
+        }
+    }
 }
 
 // Frame counters from the sampling rows and per-bucket rows: counters[0..4] the
@@ -456,7 +521,8 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
           const float* __restrict__ pos2d, const float* __restrict__ pos1d, const uchar4* __restrict__ pixPattern,
           float4* __restrict__ samples, float4* __restrict__ hitData, uint32_t* __restrict__ hitNext,
           uint32_t* __restrict__ hitCounter, uint4* __restrict__ combineList, float* __restrict__ frontDepth,
-          uint32_t* __restrict__ rowCounts, int rowBase) {
+          uint32_t* __restrict__ rowCounts, int rowBase, const unsigned long long* __restrict__ colTab,
+          const unsigned long long* __restrict__ rowTab) {
     __shared__ float4 recs[CH * 9];
     __shared__ uint32_t wsum[RH_S_WAVES][5];
     __shared__ uint2 meta[CH];
@@ -469,6 +535,8 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
     const int lx = pix % bw, ly = pix / bw;
     const int sm = si % nx, sn = si / nx;
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
+    uint32_t nTestW = 0;                  // sample tests of the wave (uniform)
+    const int CW = bw * nx, CH_ = bh * ny;  // block size in sub-samples
     uint32_t wbBase = 0;                  // the wave's current arena block
     int wbLeft = 0;
     bool lost = false;
@@ -576,6 +644,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
             for (int k = 0; k < SLOTS; ++k) {
                 if (!has[k]) continue;  // uniform across the wave
                 const int BX0 = bgx[k], BX1 = bgx[k] + bw * nx, BY0 = bgy[k], BY1 = bgy[k] + bh * ny;
+                const unsigned long long liveM = __ballot(live[k]), lodM = __ballot(lodOk[k]);
                 // Ordered list of the chunk's micropolygons that meet this block.
                 int nm = 0;
                 for (int j0 = 0; j0 < cnt; j0 += 64) {
@@ -595,18 +664,26 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 for (int i0 = 0; i0 < nm; i0 += 64) {
                     const int ni = (nm - i0) < 64 ? (nm - i0) : 64;
-                    // (a) candidate and bound tests, one micropolygon at a time.
+                    // (a) candidate and bound tests, one micropolygon at a time.  The
+                    // lanes whose sample lies in the micropolygon's candidate
+                    // rectangle form a mask computed on the scalar unit from
+                    // per-frame column and row lane masks.
                     unsigned long long pb = 0ull;
                     for (int i = 0; i < ni; ++i) {
                         const int j = __builtin_amdgcn_readfirstlane((int)matched[wave][i0 + i]);
                         const uint2 mt = meta[j];
                         const float4 bnd = recs[j * 9 + 7];
-                        const bool cand = live[k] && (uint32_t)gx[k] >= (mt.x & 0xffffu) &&
-                                          (uint32_t)gx[k] < (mt.x >> 16) && (uint32_t)gy[k] >= (mt.y & 0xffffu) &&
-                                          (uint32_t)gy[k] < (mt.y >> 16);
-                        nTest += cand ? 1u : 0u;
+                        const int mx = __builtin_amdgcn_readfirstlane((int)mt.x);
+                        const int my = __builtin_amdgcn_readfirstlane((int)mt.y);
+                        int c0 = (mx & 0xffff) - BX0, c1 = ((int)((uint32_t)mx >> 16)) - BX0;
+                        int r0 = (my & 0xffff) - BY0, r1 = ((int)((uint32_t)my >> 16)) - BY0;
+                        c0 = c0 < 0 ? 0 : c0; c1 = c1 > CW ? CW : c1;
+                        r0 = r0 < 0 ? 0 : r0; r1 = r1 > CH_ ? CH_ : r1;
+                        unsigned long long cm = 0ull;
+                        if (c0 < c1 && r0 < r1) cm = colTab[c0 * (CW + 1) + c1] & rowTab[r0 * (CH_ + 1) + r1] & liveM;
+                        nTestW += (uint32_t)__popcll(cm);
                         // CqBound::Contains2D (bound.h:144-151)
-                        const bool in = cand && lodOk[k] &&
+                        const bool in = __builtin_amdgcn_inverse_ballot_w64(cm & lodM) &&
                                         !(sx[k] < bnd.x || sx[k] > bnd.y || sy[k] < bnd.z || sy[k] > bnd.w);
                         pb |= (unsigned long long)in << i;
                     }
@@ -737,6 +814,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
         }
     }
     }   // items
+    if (lane == 0) nTest += nTestW;
     if (__ballot(lost) && lane == 0) atomicAdd(&hitCounter[1], 1u);
     // The workgroup's hider counters: wave sums, then one row (summed by
     // rh_reduce_counters; same-address atomics from every wave would
@@ -1151,12 +1229,19 @@ extern "C" hipError_t rh_launch_dice(hipStream_t st, const FrameDev* fr, const G
 
 extern "C" hipError_t rh_launch_bin(hipStream_t st, const FrameDev* fr, const BucketDev* buckets, int nbuckets,
                                     const TaskDev* tasks, const GridDev* grids, const uint32_t* candStart,
-                                    const uint32_t* candGrid, const unsigned long long* capStart,
-                                    const float4* mpbound, uint32_t* lists, uint32_t* listCount,
-                                    uint32_t* bucketRefs, unsigned long long* counters) {
+                                    const uint32_t* candGrid, const uint32_t* segBucket, size_t nsegments,
+                                    const unsigned long long* capStart, const float4* mpbound, uint32_t* segCounts,
+                                    uint32_t* lists, uint32_t* listCount, uint32_t* bucketRefs,
+                                    unsigned long long* counters) {
     if (nbuckets <= 0) return hipSuccess;
-    hipLaunchKernelGGL(rh_bin, dim3(nbuckets), dim3(256), 0, st, fr, buckets, tasks, grids, candStart, candGrid,
-                       capStart, mpbound, lists, listCount, bucketRefs, counters);
+    if (nsegments > 0)
+        hipLaunchKernelGGL(rh_bin_count, dim3((unsigned)nsegments), dim3(256), 0, st, fr, buckets, tasks, grids,
+                           candGrid, segBucket, mpbound, segCounts);
+    hipLaunchKernelGGL(rh_bin_scan, dim3(nbuckets), dim3(256), 0, st, buckets, candStart, capStart, segCounts,
+                       listCount, bucketRefs, counters);
+    if (nsegments > 0)
+        hipLaunchKernelGGL(rh_bin_write, dim3((unsigned)nsegments), dim3(256), 0, st, fr, buckets, tasks, grids,
+                           candGrid, segBucket, mpbound, segCounts, capStart, lists);
     return hipGetLastError();
 }
 
@@ -1166,7 +1251,7 @@ static void launch_sample_t(hipStream_t st, int grid, const SampleLaunch& a) {
                        a.itemBase, a.nStatic, a.nDevice, a.persistent, a.queueHead, a.capStart, a.listCount, a.lists,
                        reinterpret_cast<const float4*>(a.mprec), a.grids, a.prims, a.pos2d, a.pos1d, a.pixPattern,
                        a.samples, a.hitData, a.hitNext, a.hitCounter, a.combineList, a.frontDepth, a.rowCounts,
-                       a.rowBase);
+                       a.rowBase, a.colTab, a.rowTab);
 }
 
 // One sampling launch of `grid` workgroups with kernel variant `variant`
