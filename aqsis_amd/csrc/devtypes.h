@@ -12,15 +12,18 @@
 // Largest grid: u*v <= 256 micropolygons gives at most 2 x 257 vertices.
 #define RH_MAX_GRID_VERTS 514
 #define RH_MAX_GRID_ROWS 257
-// Hit-test record of one micropolygon, in floats:
-//  [0..3] edge y-multipliers  [4..7] edge x-multipliers
-//  [8..11] edge x origins     [12..15] edge y origins
-//  [16..19] vertex depths z0..z3 (grid order)
-//  [20..21] A  [22..23] E  [24..25] F  [26..27] G   (inverse bilinear)
-//  [28..31] bound min x, max x, min y, max y
-//  [32] bound min z  [33] linear flag (int bits)  [34] grid index (int bits)
-//  [35] opaque flag (int bits): Oi >= (1,1,1) component-wise (color.h:432-435)
-#define RH_MPREC_FLOATS 36
+// Micropolygon record, in floats (the sampler rebuilds edge equations and
+// the inverse-bilinear set-up from it when it stages a record):
+//  [0..7]   raster x, y of the four vertices in grid order
+//           (iu,iv), (iu+1,iv), (iu,iv+1), (iu+1,iv+1)
+//  [8..11]  vertex depths z0..z3 (camera z, grid order)
+//  [12..15] bound min x, max x, min y, max y
+//  [16]     bound min z
+//  [17]     flags (int bits): vertex-order code (bits 0..7), degenerate (8),
+//           linear, i.e. one Newton step (9), opaque: Oi >= (1,1,1)
+//           component-wise, color.h:432-435 (10)
+//  [18]     grid index (int bits)  [19] unused
+#define RH_MPREC_FLOATS 20
 // Device counters: 0..4 the hider's SPL counters (sample tests, bound hits,
 // stored hits, point-in-polygon hits, second Newton steps), 5 tasks whose
 // list overflowed, 6 micropolygon-bucket references, 7 transparent-hit

@@ -224,25 +224,6 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
         else
             code = degen ? ((codeA & 3) | ((codeB & 3) << 2) | ((codeC & 3) << 4))
                          : ((codeA & 3) | ((codeB & 3) << 2) | ((codeC & 3) << 4) | ((codeD & 3) << 6));
-        // cachePointInPolyTest (micropolygon.cpp:1346-1392)
-        f3d pts[4] = {P[(code >> 2) & 3], P[(code >> 4) & 3], P[(code >> 6) & 3], P[code & 3]};
-        float YM[4], XM[4], X[4], Y[4];
-        int j = 3;
-        for (int i = 0; i < 4; ++i) {
-            YM[i] = pts[i].x - pts[j].x;
-            XM[i] = pts[i].y - pts[j].y;
-            X[i] = pts[j].x;
-            Y[i] = pts[j].y;
-            j = i;
-        }
-        if (degen) {
-            for (int i = 2; i < 4; ++i) {
-                YM[i] = pts[3].x - pts[1].x;
-                XM[i] = pts[3].y - pts[1].y;
-                X[i] = pts[1].x;
-                Y[i] = pts[1].y;
-            }
-        }
         // CqInvBilinear::setVertices (bilinear.h:260-275)
         float Ex = P[1].x - P[0].x, Ey = P[1].y - P[0].y;
         float Fx = P[2].x - P[0].x, Fy = P[2].y - P[0].y;
@@ -260,16 +241,13 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
         const PrimDev& Pr = prims[G.prim];
         const bool opaque = Pr.Oi[0] >= 1.0f && Pr.Oi[1] >= 1.0f && Pr.Oi[2] >= 1.0f;
         float4* out = reinterpret_cast<float4*>(mprec + (size_t)(G.mpBase + m) * RH_MPREC_FLOATS);
-        out[0] = make_float4(YM[0], YM[1], YM[2], YM[3]);
-        out[1] = make_float4(XM[0], XM[1], XM[2], XM[3]);
-        out[2] = make_float4(X[0], X[1], X[2], X[3]);
-        out[3] = make_float4(Y[0], Y[1], Y[2], Y[3]);
-        out[4] = make_float4(P[0].z, P[1].z, P[2].z, P[3].z);
-        out[5] = make_float4(P[0].x, P[0].y, Ex, Ey);
-        out[6] = make_float4(Fx, Fy, Gx, Gy);
-        out[7] = make_float4(bminx, bmaxx, bminy, bmaxy);
-        out[8] = make_float4(bminz, __int_as_float(linear ? 1 : 0), __int_as_float((int)blockIdx.x),
-                             __int_as_float(opaque ? 1 : 0));
+        const uint32_t flags = (uint32_t)(code & 0xff) | (degen ? 1u << 8 : 0u) | (linear ? 1u << 9 : 0u) |
+                               (opaque ? 1u << 10 : 0u);
+        out[0] = make_float4(P[0].x, P[0].y, P[1].x, P[1].y);
+        out[1] = make_float4(P[2].x, P[2].y, P[3].x, P[3].y);
+        out[2] = make_float4(P[0].z, P[1].z, P[2].z, P[3].z);
+        out[3] = make_float4(bminx, bmaxx, bminy, bmaxy);
+        out[4] = make_float4(bminz, __uint_as_float(flags), __int_as_float((int)blockIdx.x), 0.0f);
         mpbound[G.mpBase + m] = make_float4(bminx, bmaxx, bminy, bmaxy);
     }
 }
@@ -484,6 +462,62 @@ __global__ void __launch_bounds__(1024) rh_reduce_counters(const uint32_t* __res
 // lceil(bmaxx)) clipped to the bucket's sample region with the first/last
 // pixel sub-sample clip, in sub-sample grid units, so every sample sees the
 // reference's sequence of tests.
+// The sampling form of a micropolygon record, from its compact form (the
+// four raster vertices in grid order, their depths, bound, minimum depth,
+// flags and grid): cachePointInPolyTest's edge equations on the vertices in
+// the order the vertex-order code gives (micropolygon.cpp:1346-1392, a
+// degenerate micropolygon repeats its closing edge) and CqInvBilinear's
+// set-up on the grid-order vertices (bilinear.h:260-275).  Out: YM, XM, X, Y,
+// z, (A, E), (F, G), bound, (minz, linear, grid, opaque).
+__device__ __forceinline__ float pick4(int i, float a0, float a1, float a2, float a3) {
+    return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
+}
+
+__device__ __forceinline__ void expand_record(const float4* __restrict__ src, float4* out) {
+    const float4 a = src[0], b = src[1], zz = src[2], bd = src[3], r4 = src[4];
+    const uint32_t fl = __float_as_uint(r4.y);
+    const int code = (int)(fl & 0xffu);
+    const bool degen = (fl >> 8) & 1u;
+    // the vertices in the order the code gives
+    float qx[4], qy[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int o = i < 3 ? (code >> (2 * (i + 1))) & 3 : code & 3;
+        qx[i] = pick4(o, a.x, a.z, b.x, b.z);
+        qy[i] = pick4(o, a.y, a.w, b.y, b.w);
+    }
+    float YM[4], XM[4], X[4], Y[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int j = (i + 3) & 3;
+        YM[i] = qx[i] - qx[j];
+        XM[i] = qy[i] - qy[j];
+        X[i] = qx[j];
+        Y[i] = qy[j];
+    }
+    if (degen) {
+#pragma unroll
+        for (int i = 2; i < 4; ++i) {
+            YM[i] = qx[3] - qx[1];
+            XM[i] = qy[3] - qy[1];
+            X[i] = qx[1];
+            Y[i] = qy[1];
+        }
+    }
+    const float Ex = a.z - a.x, Ey = a.w - a.y;
+    const float Fx = b.x - a.x, Fy = b.y - a.y;
+    const float Gx = ((-Ex) - b.x) + b.z, Gy = ((-Ey) - b.y) + b.w;
+    out[0] = make_float4(YM[0], YM[1], YM[2], YM[3]);
+    out[1] = make_float4(XM[0], XM[1], XM[2], XM[3]);
+    out[2] = make_float4(X[0], X[1], X[2], X[3]);
+    out[3] = make_float4(Y[0], Y[1], Y[2], Y[3]);
+    out[4] = zz;
+    out[5] = make_float4(a.x, a.y, Ex, Ey);
+    out[6] = make_float4(Fx, Fy, Gx, Gy);
+    out[7] = bd;
+    out[8] = make_float4(r4.x, __int_as_float((fl >> 9) & 1u), r4.z, __int_as_float((fl >> 10) & 1u));
+}
+
 // Colour and opacity of a hit: InterpolateOutputs with smooth shading of
 // four equal vertex values (micropolygon.cpp:1443-1462).  A = (Ci.rgb, depth),
 // B = (Oi.rgb, micropolygon id bits).
@@ -614,8 +648,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
             if (tid < cnt) {
                 const uint32_t id = L[c0 + tid];
                 float4 r[9];
-#pragma unroll
-                for (int q = 0; q < 9; ++q) r[q] = mprec4[(size_t)id * 9 + q];
+                expand_record(mprec4 + (size_t)id * (RH_MPREC_FLOATS / 4), r);
 #pragma unroll
                 for (int q = 0; q < 9; ++q) recs[tid * 9 + q] = r[q];
                 ids[tid] = id;
@@ -792,7 +825,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
             const size_t sidx = ((size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)) * ns + si;
             float4 a = make_float4(0.f, 0.f, 0.f, FLT_MAX), b = make_float4(0.f, 0.f, 0.f, __uint_as_float(win[k]));
             if (win[k] != 0xffffffffu) {
-                const float4 r8 = mprec4[(size_t)win[k] * 9 + 8];
+                const float4 r8 = mprec4[(size_t)win[k] * (RH_MPREC_FLOATS / 4) + 4];
                 const PrimDev& P = prims[grids[__float_as_int(r8.z)].prim];
                 shade_hit(P, wu[k], wv[k], occlZ[k], win[k], a, b);
             }
@@ -1183,20 +1216,46 @@ __global__ void __launch_bounds__(128) rh_filter_tiled(const FrameDev* __restric
                 const float* rec = lds + ((ly + fy + ym) * HX + (lx + fx + xm)) * S + c0;
                 const float* wt = wtab + ((fy + ym) * (2 * xm + 1) + fx + xm) * ns;
                 const uint32_t need = (1u << (fx + xm)) | (1u << (8 + fy + ym));
-#pragma unroll 4
-                for (int si = 0; si < ns; ++si) {
-                    const uint32_t m = __float_as_uint(rec[si * 8 + 7 - c0]);
-                    if ((m & need) == need) {
-                        const float g = wt[si];
-                        gTot += g;
-                        if (m >> 31) {
-                            acc[0] += rec[si * 8 + 0] * g;
-                            acc[1] += rec[si * 8 + 1] * g;
-                            acc[2] += rec[si * 8 + 2] * g;
-                            if (!half) acc[3] += rec[si * 8 + 3] * g;
-                            ++count;
-                        }
+                // Four samples at a time: every LDS read is issued first, then
+                // the sums take each sample in order through selects, which
+                // perform exactly the additions the branches would.  (In the
+                // second wave the fourth value read is the mask word; its sum
+                // is never used.)
+                int si = 0;
+                for (; si + 4 <= ns; si += 4) {
+                    uint32_t m[4];
+                    float g[4], d[4][4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float* r = rec + (si + q) * 8;
+                        m[q] = __float_as_uint(r[7 - c0]);
+                        g[q] = wt[si + q];
+                        d[q][0] = r[0]; d[q][1] = r[1]; d[q][2] = r[2]; d[q][3] = r[3];
                     }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const bool in = (m[q] & need) == need;
+                        const bool hit = in && (m[q] >> 31);
+                        gTot = in ? gTot + g[q] : gTot;
+                        acc[0] = hit ? acc[0] + d[q][0] * g[q] : acc[0];
+                        acc[1] = hit ? acc[1] + d[q][1] * g[q] : acc[1];
+                        acc[2] = hit ? acc[2] + d[q][2] * g[q] : acc[2];
+                        acc[3] = hit ? acc[3] + d[q][3] * g[q] : acc[3];
+                        count += hit ? 1 : 0;
+                    }
+                }
+                for (; si < ns; ++si) {
+                    const float* r = rec + si * 8;
+                    const uint32_t m = __float_as_uint(r[7 - c0]);
+                    const float g = wt[si];
+                    const bool in = (m & need) == need;
+                    const bool hit = in && (m >> 31);
+                    gTot = in ? gTot + g : gTot;
+                    acc[0] = hit ? acc[0] + r[0] * g : acc[0];
+                    acc[1] = hit ? acc[1] + r[1] * g : acc[1];
+                    acc[2] = hit ? acc[2] + r[2] * g : acc[2];
+                    acc[3] = hit ? acc[3] + r[3] * g : acc[3];
+                    count += hit ? 1 : 0;
                 }
             }
     }

@@ -270,12 +270,19 @@ def serving_requests(seed=1, count=2000, sides=(256, 512, 1024, 2048), spp=(2, 4
 
 
 def to_rib(opts, prims):
-    """RIB text of a scene (camera space; inspection only)."""
+    """RIB text of a scene in camera space (spheres as a translation), which
+    aqsis_amd.rib.parse reads back to the same options and records."""
     names = {v: k for k, v in __import__("aqsis_amd.abi", fromlist=["FILTERS"]).FILTERS.items()}
     out = [f'Format {opts.xres} {opts.yres} {opts.pixel_aspect}',
            f'PixelSamples {opts.xsamples} {opts.ysamples}',
            f'PixelFilter "{names[opts.filter]}" {opts.filter_xwidth} {opts.filter_ywidth}',
            f'Exposure {opts.exposure_gain} {opts.exposure_gamma}',
+           f'Quantize "rgba" {opts.quantize_one} {opts.quantize_min} {opts.quantize_max} {opts.quantize_dither}',
+           f'CropWindow {opts.crop[0]} {opts.crop[1]} {opts.crop[2]} {opts.crop[3]}',
+           f'Clipping {opts.clip_near} {opts.clip_far}',
+           f'Option "limits" "bucketsize" [{opts.bucket_x} {opts.bucket_y}] "eyesplits" [{opts.eyesplits}]',
+           f'Option "limits" "zthreshold" [{opts.zthreshold[0]} {opts.zthreshold[1]} {opts.zthreshold[2]}]',
+           f'Hider "hidden" "jitter" [{opts.jitter}]',
            f'Projection "perspective" "fov" [{opts.fov}]',
            'WorldBegin', 'Surface "constant"']
     for p in prims:
