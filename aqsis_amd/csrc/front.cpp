@@ -662,9 +662,17 @@ void build_tree(const Ctx& c, int prim, std::vector<Node>& t) {
     }
 }
 
+// A bucket heap entry carries the comparator's inputs so that heap moves
+// and comparisons do not touch the nodes.
+struct HeapEnt {
+    float z;                    // cached raster bound's minimum z
+    bool hasBound;
+    Node* nd;
+};
+
 struct HeapCmp {  // closest_surface (bucket.h:164-176)
-    bool operator()(const Node* a, const Node* b) const {
-        if (a->hasBound && b->hasBound) return a->rb.lo.z > b->rb.lo.z;
+    bool operator()(const HeapEnt& a, const HeapEnt& b) const {
+        if (a.hasBound && b.hasBound) return a.z > b.z;
         return true;
     }
 };
@@ -725,7 +733,7 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
 
     // 2) Bucket loop: post roots in order, pop nearest first, post children.
     const int nb = (int)s.buckets.size();
-    std::vector<std::vector<Node*>> heaps(nb);
+    std::vector<std::vector<HeapEnt>> heaps(nb);
     std::vector<uint8_t> processed(nb, 0);
     out = Schedule();
     auto post = [&](Node* nd) {
@@ -736,8 +744,9 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
         x0 = clampv(x0, s.bx0, s.bx1 - 1); y0 = clampv(y0, s.by0, s.by1 - 1);
         x1 = clampv(x1, s.bx0, s.bx1 - 1); y1 = clampv(y1, s.by0, s.by1 - 1);
         int k = s.bucketIndex(x0, y0);
+        const HeapEnt ent{nd->rb.lo.z, nd->hasBound, nd};
         if (!processed[k]) {
-            heaps[k].push_back(nd);
+            heaps[k].push_back(ent);
             std::push_heap(heaps[k].begin(), heaps[k].end(), HeapCmp());
             return;
         }
@@ -746,7 +755,7 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
             while (xb <= x1) {
                 int kk = s.bucketIndex(xb, yb);
                 if (!processed[kk]) {
-                    heaps[kk].push_back(nd);
+                    heaps[kk].push_back(ent);
                     std::push_heap(heaps[kk].begin(), heaps[kk].end(), HeapCmp());
                     return;
                 }
@@ -762,7 +771,7 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
     for (int k = 0; k < nb; ++k) {
         auto& h = heaps[k];
         while (!h.empty()) {
-            Node* nd = h.front();
+            Node* nd = h.front().nd;
             std::pop_heap(h.begin(), h.end(), HeapCmp());
             h.pop_back();
             if (nd->leaf) {

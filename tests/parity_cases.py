@@ -1,5 +1,7 @@
 """Scenes used by the parity tests: small enough for the oracle to finish in
 seconds, covering every option the HIP path implements."""
+import numpy as np
+
 from aqsis_amd import scenes
 from aqsis_amd.abi import default_options
 
@@ -81,6 +83,17 @@ def patches_deep_transparent():
     return scenes.patches_scene(seed=37, n=100, res=(128, 96), opacity=(0.2, 0.8), filt=("box", 1.0, 1.0))
 
 
+def bilinear_patches():
+    # Bilinear patches (CqSurfacePatchBilinear, BilinearEvaluate dicing):
+    # corners of each of the HD generator's patches, mixed opacity.
+    opts, prims = scenes.patches_scene(seed=43, n=250, res=(160, 120), filt=("catmull-rom", 3.0, 3.0))
+    out = []
+    for i, p in enumerate(prims):
+        P = np.array([list(p.P[k]) for k in (0, 3, 12, 15)], np.float32)
+        out.append(scenes.bilinear(P, list(p.Cs), (1.0, 1.0, 1.0) if i % 3 else (0.6, 0.6, 0.6), 1.0))
+    return opts, out
+
+
 CASES = {
     "spheres_config": spheres_config,
     "patches_small": patches_small,
@@ -94,4 +107,5 @@ CASES = {
     "patches_transparent_8x8_zthreshold": patches_transparent_8x8_zthreshold,
     "spheres_mixed_opacity": spheres_mixed_opacity,
     "patches_deep_transparent": patches_deep_transparent,
+    "bilinear_patches": bilinear_patches,
 }
