@@ -101,7 +101,7 @@ struct rh_ctx {
     // schedule
     DBuf dGrids, dParams, dPrims, dCandStart, dCandGrid, dCapStart, dSegBucket, dSegCounts;
     // per frame
-    DBuf dVtx, dMprec, dMpbound, dLists, dListCount, dSamples, dChannels, dRgba, dCounters, dTaskCounts, dBucketRefs;
+    DBuf dVtx, dMprec, dMpbound, dLists, dListCount, dSamples, dChannels, dRgba, dCounters, dBucketRefs;
     // transparent hits: arena of (Ci, depth), (Oi, id) pairs and list links
     DBuf dHitData, dHitNext, dHitCtl, dCombine, dFrontDepth;
     bool transparent = false;             // some posted surface has Os < 1
@@ -126,7 +126,7 @@ struct rh_ctx {
         DBuf* all[] = {&dFrame, &dBuckets, &dTasks, &dPos2d, &dPos1d, &dPix, &dDither, &dFilter, &dBoxMask,
                        &dGrids, &dParams,
                        &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dVtx, &dMprec, &dMpbound, &dLists,
-                       &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dTaskCounts, &dBucketRefs,
+                       &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dBucketRefs,
                        &dHitData, &dHitNext, &dHitCtl, &dCombine, &dFrontDepth, &dItemsStatic, &dItems,
                        &dItemCount, &dQueue, &dRows, &dSegBucket, &dSegCounts, &dLaneTabs};
         dropGraph();
@@ -224,7 +224,6 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
         (e = c->dChannels.ensure((size_t)s.width() * s.height() * 9 * sizeof(float))) != hipSuccess ||
         (e = c->dRgba.ensure((size_t)s.width() * s.height() * 4)) != hipSuccess ||
         (e = c->dListCount.ensure(s.tasks.size() * sizeof(uint32_t))) != hipSuccess ||
-        (e = c->dTaskCounts.ensure(s.tasks.size() * 5 * sizeof(uint32_t))) != hipSuccess ||
         (e = c->dBucketRefs.ensure(s.buckets.size() * sizeof(uint32_t))) != hipSuccess ||
         (e = c->dHitCtl.ensure(4 * sizeof(uint32_t))) != hipSuccess ||
         (e = upload(c->dCounters, zero, c->st)) != hipSuccess) {
@@ -703,7 +702,7 @@ uint64_t rh_internal_device_bytes(const rh_ctx* c) {
     const DBuf* all[] = {&c->dFrame, &c->dBuckets, &c->dTasks, &c->dPos2d, &c->dPos1d, &c->dPix, &c->dDither,
                          &c->dFilter, &c->dBoxMask, &c->dGrids, &c->dParams, &c->dPrims, &c->dCandStart,
                          &c->dCandGrid, &c->dCapStart, &c->dSegBucket, &c->dSegCounts, &c->dVtx, &c->dMprec, &c->dMpbound, &c->dLists,
-                         &c->dListCount, &c->dSamples, &c->dChannels, &c->dRgba, &c->dCounters, &c->dTaskCounts,
+                         &c->dListCount, &c->dSamples, &c->dChannels, &c->dRgba, &c->dCounters,
                          &c->dBucketRefs, &c->dHitData, &c->dHitNext, &c->dHitCtl, &c->dCombine, &c->dFrontDepth,
                          &c->dItemsStatic, &c->dItems, &c->dItemCount, &c->dQueue, &c->dRows, &c->dLaneTabs};
     uint64_t total = 0;
