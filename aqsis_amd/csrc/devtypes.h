@@ -107,11 +107,6 @@ struct SampleLaunch {
     float* frontDepth;
     uint32_t* rowCounts;
     int rowBase;
-    // lane masks of a wave block: colTab[a * (W + 1) + b] holds the lanes
-    // whose sample column (in sub-samples, W = blockW * nx across) is in
-    // [a, b); rowTab the same for rows (H = blockH * ny)
-    const unsigned long long* colTab;
-    const unsigned long long* rowTab;
 };
 
 // Sampling kernel variants: LDS chunk of CH records, SLOTS blocks per wave.

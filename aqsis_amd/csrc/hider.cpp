@@ -106,13 +106,11 @@ struct rh_ctx {
     DBuf dHitData, dHitNext, dHitCtl, dCombine, dFrontDepth;
     bool transparent = false;             // some posted surface has Os < 1
     // work items: one per task (grid and per-bucket forms), cost-split (batched)
-    DBuf dItemsStatic, dItems, dItemCount, dQueue, dRows, dLaneTabs;
+    DBuf dItemsStatic, dItems, dItemCount, dQueue, dRows;
     int maxItems = 0, persistentGrid = 0, maxRows = 0;
-    size_t rowTabOffset = 0;              // rowTab's start in dLaneTabs
     rh_tuning tuning;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
-    bool graphTimed = false;              // the graph holds the stage events
     int scheduleThreads = 0;              // host threads of the split/dice schedule (0: up to 16)
     void dropGraph() {
         if (graphExec) (void)hipGraphExecDestroy(graphExec);
@@ -128,7 +126,7 @@ struct rh_ctx {
                        &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dVtx, &dMprec, &dMpbound, &dLists,
                        &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dBucketRefs,
                        &dHitData, &dHitNext, &dHitCtl, &dCombine, &dFrontDepth, &dItemsStatic, &dItems,
-                       &dItemCount, &dQueue, &dRows, &dSegBucket, &dSegCounts, &dLaneTabs};
+                       &dItemCount, &dQueue, &dRows, &dSegBucket, &dSegCounts};
         dropGraph();
         for (DBuf* b : all) b->release();
         for (auto& e : ev)
@@ -272,27 +270,11 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
         if (e == hipSuccess) e = c->dItemCount.ensure(16);
         if (e == hipSuccess) e = c->dQueue.ensure(16);
         if (e == hipSuccess) e = c->dRows.ensure((size_t)c->maxRows * 5 * sizeof(uint32_t));
-        // Lane masks of a wave block by sub-sample column and row range.
-        const int W = s.blockW * s.nx, H = s.blockH * s.ny;
-        std::vector<unsigned long long> tabs((size_t)(W + 1) * (W + 1) + (size_t)(H + 1) * (H + 1), 0ull);
-        for (int l = 0; l < 64; ++l) {
-            const int pix = l / s.ns, si = l % s.ns;
-            if (pix >= s.blockW * s.blockH) continue;
-            const int cx = (pix % s.blockW) * s.nx + si % s.nx, cy = (pix / s.blockW) * s.ny + si / s.nx;
-            for (int a = 0; a <= W; ++a)
-                for (int b = a + 1; b <= W; ++b)
-                    if (a <= cx && cx < b) tabs[(size_t)a * (W + 1) + b] |= 1ull << l;
-            for (int a = 0; a <= H; ++a)
-                for (int b = a + 1; b <= H; ++b)
-                    if (a <= cy && cy < b) tabs[(size_t)(W + 1) * (W + 1) + (size_t)a * (H + 1) + b] |= 1ull << l;
-        }
-        if (e == hipSuccess) e = upload(c->dLaneTabs, tabs, c->st);
-        c->rowTabOffset = (size_t)(W + 1) * (W + 1);
     }
     c->tuning.mode = RH_MODE_GRID;
     c->tuning.lds_records = 224;
     c->tuning.blocks_per_wave = 4;
-    c->tuning.split_cost = 1u << 16;
+    c->tuning.split_cost = 1u << 20;
     c->tuning.frame_ms = 0.0;
     if (e != hipSuccess || (e = hipStreamSynchronize(c->st)) != hipSuccess) {
         delete c;
@@ -449,8 +431,6 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
     a.hitNext = c->dHitNext.as<uint32_t>(); a.hitCounter = c->dHitCtl.as<uint32_t>();
     a.combineList = c->dCombine.as<uint4>(); a.frontDepth = c->transparent ? c->dFrontDepth.as<float>() : nullptr;
     a.rowCounts = c->dRows.as<uint32_t>(); a.rowBase = 0;
-    a.colTab = c->dLaneTabs.as<unsigned long long>();
-    a.rowTab = c->dLaneTabs.as<unsigned long long>() + c->rowTabOffset;
     const int variant = variant_of(tu);
     int rows = 0;
     if (tu.mode == RH_MODE_PLAIN) {
@@ -501,34 +481,27 @@ rh_status rh_render_device(rh_ctx* c) {
         if (graphed) {
             // The batched form replays one captured graph per frame.
             if (!c->graphExec) {
-                // Stage events are captured as event nodes where the runtime
-                // allows it; otherwise only the whole frame is timed.
-                for (int timedCapture = 1; timedCapture >= 0 && !c->graphExec; --timedCapture) {
-                    HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-                    hipError_t e = enqueue_frame(c, timedCapture != 0);
-                    hipGraph_t g = nullptr;
-                    hipError_t e2 = hipStreamEndCapture(c->st, &g);
-                    if (e == hipSuccess && e2 == hipSuccess) {
-                        c->graph = g;
-                        HIP_TRY(hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0));
-                        c->graphTimed = timedCapture != 0;
-                    } else {
-                        if (g) (void)hipGraphDestroy(g);
-                        (void)hipGetLastError();
-                        if (!timedCapture)
-                            return fail(RH_ERR_DEVICE, std::string("frame graph capture: ") +
-                                                           hipGetErrorString(e != hipSuccess ? e : e2));
-                    }
+                // Only the whole frame is timed in this form: events recorded
+                // inside a captured graph do not report elapsed times here.
+                HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+                hipError_t e = enqueue_frame(c, false);
+                hipGraph_t g = nullptr;
+                hipError_t e2 = hipStreamEndCapture(c->st, &g);
+                if (e != hipSuccess || e2 != hipSuccess) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return fail(RH_ERR_DEVICE, std::string("frame graph capture: ") +
+                                                   hipGetErrorString(e != hipSuccess ? e : e2));
                 }
+                c->graph = g;
+                HIP_TRY(hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0));
             }
-            if (!c->graphTimed) HIP_TRY(hipEventRecord(c->ev[0], c->st));
+            HIP_TRY(hipEventRecord(c->ev[0], c->st));
             HIP_TRY(hipGraphLaunch(c->graphExec, c->st));
-            if (!c->graphTimed) HIP_TRY(hipEventRecord(c->ev[4], c->st));
+            HIP_TRY(hipEventRecord(c->ev[4], c->st));
         } else {
             HIP_TRY(enqueue_frame(c, true));
         }
-        if (graphed) HIP_TRY(hipStreamSynchronize(c->st));
-        else HIP_TRY(hipEventSynchronize(c->ev[4]));
+        HIP_TRY(hipEventSynchronize(c->ev[4]));
         HIP_TRY(hipMemcpy(cnt, c->dCounters.p, sizeof(cnt), hipMemcpyDeviceToHost));
         uint32_t hc[2];
         HIP_TRY(hipMemcpy(hc, c->dHitCtl.p, sizeof(hc), hipMemcpyDeviceToHost));
@@ -545,7 +518,7 @@ rh_status rh_render_device(rh_ctx* c) {
         break;
     }
     float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (!graphed || c->graphTimed)
+    if (!graphed)
         for (int i = 0; i < 4; ++i) HIP_TRY(hipEventElapsedTime(&t[i], c->ev[i], c->ev[i + 1]));
     c->stats.spl_count = cnt[0];
     c->stats.spl_bound_hits = cnt[1];
@@ -601,10 +574,13 @@ rh_status rh_autotune(rh_ctx* c, rh_tuning* chosen) {
     rh_tuning best;
     double bestMs = 1e300;
     const int lds[2] = {224, 96}, bpw[2] = {4, 2};
-    const uint32_t split[4] = {0, 1u << 14, 1u << 16, 1u << 18};   // 0: the static grid
+    // 0: the static grid.  Small split costs make many short items whose list
+    // re-reads outweigh the balance they buy on dense frames; large ones
+    // leave a few heavy buckets as the tail.
+    const uint32_t split[5] = {0, 1u << 16, 1u << 18, 1u << 20, 1u << 22};
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < 5; ++k) {
                 rh_tuning t;
                 t.mode = split[k] ? RH_MODE_BATCHED : RH_MODE_GRID;
                 t.lds_records = lds[i];
@@ -704,7 +680,7 @@ uint64_t rh_internal_device_bytes(const rh_ctx* c) {
                          &c->dCandGrid, &c->dCapStart, &c->dSegBucket, &c->dSegCounts, &c->dVtx, &c->dMprec, &c->dMpbound, &c->dLists,
                          &c->dListCount, &c->dSamples, &c->dChannels, &c->dRgba, &c->dCounters,
                          &c->dBucketRefs, &c->dHitData, &c->dHitNext, &c->dHitCtl, &c->dCombine, &c->dFrontDepth,
-                         &c->dItemsStatic, &c->dItems, &c->dItemCount, &c->dQueue, &c->dRows, &c->dLaneTabs};
+                         &c->dItemsStatic, &c->dItems, &c->dItemCount, &c->dQueue, &c->dRows};
     uint64_t total = 0;
     for (const DBuf* b : all) total += b->cap;
     return total;

@@ -555,8 +555,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
           const float* __restrict__ pos2d, const float* __restrict__ pos1d, const uchar4* __restrict__ pixPattern,
           float4* __restrict__ samples, float4* __restrict__ hitData, uint32_t* __restrict__ hitNext,
           uint32_t* __restrict__ hitCounter, uint4* __restrict__ combineList, float* __restrict__ frontDepth,
-          uint32_t* __restrict__ rowCounts, int rowBase, const unsigned long long* __restrict__ colTab,
-          const unsigned long long* __restrict__ rowTab) {
+          uint32_t* __restrict__ rowCounts, int rowBase) {
     __shared__ float4 recs[CH * 9];
     __shared__ uint32_t wsum[RH_S_WAVES][5];
     __shared__ uint2 meta[CH];
@@ -569,8 +568,6 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
     const int lx = pix % bw, ly = pix / bw;
     const int sm = si % nx, sn = si / nx;
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
-    uint32_t nTestW = 0;                  // sample tests of the wave (uniform)
-    const int CW = bw * nx, CH_ = bh * ny;  // block size in sub-samples
     uint32_t wbBase = 0;                  // the wave's current arena block
     int wbLeft = 0;
     bool lost = false;
@@ -677,7 +674,6 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
             for (int k = 0; k < SLOTS; ++k) {
                 if (!has[k]) continue;  // uniform across the wave
                 const int BX0 = bgx[k], BX1 = bgx[k] + bw * nx, BY0 = bgy[k], BY1 = bgy[k] + bh * ny;
-                const unsigned long long liveM = __ballot(live[k]), lodM = __ballot(lodOk[k]);
                 // Ordered list of the chunk's micropolygons that meet this block.
                 int nm = 0;
                 for (int j0 = 0; j0 < cnt; j0 += 64) {
@@ -697,26 +693,18 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 for (int i0 = 0; i0 < nm; i0 += 64) {
                     const int ni = (nm - i0) < 64 ? (nm - i0) : 64;
-                    // (a) candidate and bound tests, one micropolygon at a time.  The
-                    // lanes whose sample lies in the micropolygon's candidate
-                    // rectangle form a mask computed on the scalar unit from
-                    // per-frame column and row lane masks.
+                    // (a) candidate and bound tests, one micropolygon at a time.
                     unsigned long long pb = 0ull;
                     for (int i = 0; i < ni; ++i) {
                         const int j = __builtin_amdgcn_readfirstlane((int)matched[wave][i0 + i]);
                         const uint2 mt = meta[j];
                         const float4 bnd = recs[j * 9 + 7];
-                        const int mx = __builtin_amdgcn_readfirstlane((int)mt.x);
-                        const int my = __builtin_amdgcn_readfirstlane((int)mt.y);
-                        int c0 = (mx & 0xffff) - BX0, c1 = ((int)((uint32_t)mx >> 16)) - BX0;
-                        int r0 = (my & 0xffff) - BY0, r1 = ((int)((uint32_t)my >> 16)) - BY0;
-                        c0 = c0 < 0 ? 0 : c0; c1 = c1 > CW ? CW : c1;
-                        r0 = r0 < 0 ? 0 : r0; r1 = r1 > CH_ ? CH_ : r1;
-                        unsigned long long cm = 0ull;
-                        if (c0 < c1 && r0 < r1) cm = colTab[c0 * (CW + 1) + c1] & rowTab[r0 * (CH_ + 1) + r1] & liveM;
-                        nTestW += (uint32_t)__popcll(cm);
+                        const bool cand = live[k] && (uint32_t)gx[k] >= (mt.x & 0xffffu) &&
+                                          (uint32_t)gx[k] < (mt.x >> 16) && (uint32_t)gy[k] >= (mt.y & 0xffffu) &&
+                                          (uint32_t)gy[k] < (mt.y >> 16);
+                        nTest += cand ? 1u : 0u;
                         // CqBound::Contains2D (bound.h:144-151)
-                        const bool in = __builtin_amdgcn_inverse_ballot_w64(cm & lodM) &&
+                        const bool in = cand && lodOk[k] &&
                                         !(sx[k] < bnd.x || sx[k] > bnd.y || sy[k] < bnd.z || sy[k] > bnd.w);
                         pb |= (unsigned long long)in << i;
                     }
@@ -847,7 +835,6 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
         }
     }
     }   // items
-    if (lane == 0) nTest += nTestW;
     if (__ballot(lost) && lane == 0) atomicAdd(&hitCounter[1], 1u);
     // The workgroup's hider counters: wave sums, then one row (summed by
     // rh_reduce_counters; same-address atomics from every wave would
@@ -1310,7 +1297,7 @@ static void launch_sample_t(hipStream_t st, int grid, const SampleLaunch& a) {
                        a.itemBase, a.nStatic, a.nDevice, a.persistent, a.queueHead, a.capStart, a.listCount, a.lists,
                        reinterpret_cast<const float4*>(a.mprec), a.grids, a.prims, a.pos2d, a.pos1d, a.pixPattern,
                        a.samples, a.hitData, a.hitNext, a.hitCounter, a.combineList, a.frontDepth, a.rowCounts,
-                       a.rowBase, a.colTab, a.rowTab);
+                       a.rowBase);
 }
 
 // One sampling launch of `grid` workgroups with kernel variant `variant`
