@@ -27,6 +27,7 @@
 // rounded division; every expression keeps the operand order of the aqsis
 // code it cites.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cfloat>
 #include <cstdint>
 
@@ -256,8 +257,7 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
 // Binning: the ordered micropolygon list of each sampling task
 // ---------------------------------------------------------------------------
 // A segment is one (bucket, candidate grid) pair; a bucket's segments are in
-// dicing order and a grid has at most 256 micropolygons, so one 256-thread
-// workgroup per segment sees all of them.  Every micropolygon goes through
+// dicing order and a grid has at most 256 micropolygons.  Every micropolygon goes through
 // CqImageBuffer::AddMPG's tests for the bucket (imagebuffer.cpp:514-591),
 // and a kept one belongs to the list of each of the bucket's tasks that its
 // clipped pixel window meets.  Three passes keep the reference's order with
@@ -298,35 +298,42 @@ __device__ __forceinline__ bool bin_window(const FrameDev* __restrict__ fr, cons
     return true;
 }
 
-// Pass 1: per segment, the kept micropolygons per task and in all.
+// Pass 1: per segment, the kept micropolygons per task and in all.  One
+// wave per segment at a time (grid-stride over segments), 64 micropolygons
+// per step; no barriers.
 __global__ void __launch_bounds__(256) rh_bin_count(const FrameDev* __restrict__ fr,
                                                     const BucketDev* __restrict__ buckets,
                                                     const TaskDev* __restrict__ tasks, const GridDev* __restrict__ grids,
                                                     const uint32_t* __restrict__ candGrid,
-                                                    const uint32_t* __restrict__ segBucket,
+                                                    const uint32_t* __restrict__ segBucket, size_t nseg,
                                                     const float4* __restrict__ mpbound,
                                                     uint32_t* __restrict__ segCounts) {
-    __shared__ uint32_t wtot[4][RH_SEG_STRIDE];
-    const size_t seg = blockIdx.x;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const BucketDev B = buckets[segBucket[seg]];
-    const GridDev G = grids[candGrid[seg]];
-    const int nt = B.ntask;
-    int sX = 0, eX = 0, sY = 0, eY = 0;
-    const bool keep = tid < G.cu * G.cv && bin_window(fr, B, mpbound[G.mpBase + tid], sX, eX, sY, eY);
-    for (int t = 0; t <= nt; ++t) {
-        bool ok = keep;
-        if (t < nt) {
-            const TaskDev T = tasks[B.task0 + t];
-            ok = keep && sX < T.x1 - B.sx0 && eX > T.x0 - B.sx0 && sY < T.y1 - B.sy0 && eY > T.y0 - B.sy0;
+    const int lane = threadIdx.x & 63;
+    const size_t waves = (size_t)gridDim.x * (blockDim.x >> 6);
+    for (size_t seg = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); seg < nseg; seg += waves) {
+        const BucketDev B = buckets[segBucket[seg]];
+        const GridDev G = grids[candGrid[seg]];
+        const int nt = B.ntask, nmp = G.cu * G.cv;
+        uint32_t cnt[RH_SEG_STRIDE];
+        for (int t = 0; t < RH_SEG_STRIDE; ++t) cnt[t] = 0;
+        for (int base = 0; base < nmp; base += 64) {
+            const int m = base + lane;
+            int sX = 0, eX = 0, sY = 0, eY = 0;
+            const bool keep = m < nmp && bin_window(fr, B, mpbound[G.mpBase + m], sX, eX, sY, eY);
+            cnt[RH_MAX_TASKS] += (uint32_t)__popcll(__ballot(keep));
+            for (int t = 0; t < nt; ++t) {
+                const TaskDev T = tasks[B.task0 + t];
+                const bool ok = keep && sX < T.x1 - B.sx0 && eX > T.x0 - B.sx0 && sY < T.y1 - B.sy0 &&
+                                eY > T.y0 - B.sy0;
+                cnt[t] += (uint32_t)__popcll(__ballot(ok));
+            }
         }
-        const unsigned long long m = __ballot(ok);
-        if (lane == 0) wtot[wave][t < nt ? t : RH_MAX_TASKS] = (uint32_t)__popcll(m);
-    }
-    __syncthreads();
-    if (tid <= nt) {
-        const int k = tid < nt ? tid : RH_MAX_TASKS;
-        segCounts[seg * RH_SEG_STRIDE + k] = wtot[0][k] + wtot[1][k] + wtot[2][k] + wtot[3][k];
+        if (lane < nt) {
+            uint32_t v = 0;
+            for (int t = 0; t < RH_MAX_TASKS; ++t) v = t == lane ? cnt[t] : v;
+            segCounts[seg * RH_SEG_STRIDE + lane] = v;
+        }
+        if (lane == 0) segCounts[seg * RH_SEG_STRIDE + RH_MAX_TASKS] = cnt[RH_MAX_TASKS];
     }
 }
 
@@ -379,41 +386,41 @@ __global__ void __launch_bounds__(256) rh_bin_scan(const BucketDev* __restrict__
 }
 
 // Pass 3: per segment, each task's kept micropolygons in order at the
-// segment's offset.
+// segment's offset.  One wave per segment at a time, as pass 1.
 __global__ void __launch_bounds__(256) rh_bin_write(const FrameDev* __restrict__ fr,
                                                     const BucketDev* __restrict__ buckets,
                                                     const TaskDev* __restrict__ tasks, const GridDev* __restrict__ grids,
                                                     const uint32_t* __restrict__ candGrid,
-                                                    const uint32_t* __restrict__ segBucket,
+                                                    const uint32_t* __restrict__ segBucket, size_t nseg,
                                                     const float4* __restrict__ mpbound,
                                                     const uint32_t* __restrict__ segCounts,
                                                     const unsigned long long* __restrict__ capStart,
                                                     uint32_t* __restrict__ lists) {
-    __shared__ uint32_t wtot[4][RH_MAX_TASKS];
-    const size_t seg = blockIdx.x;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const BucketDev B = buckets[segBucket[seg]];
-    const GridDev G = grids[candGrid[seg]];
-    const int nt = B.ntask;
-    int sX = 0, eX = 0, sY = 0, eY = 0;
-    const bool keep = tid < G.cu * G.cv && bin_window(fr, B, mpbound[G.mpBase + tid], sX, eX, sY, eY);
-    uint32_t okMask = 0;
-    for (int t = 0; t < nt; ++t) {
-        const TaskDev T = tasks[B.task0 + t];
-        const bool ok = keep && sX < T.x1 - B.sx0 && eX > T.x0 - B.sx0 && sY < T.y1 - B.sy0 && eY > T.y0 - B.sy0;
-        okMask |= (ok ? 1u : 0u) << t;
-        const unsigned long long m = __ballot(ok);
-        if (lane == 0) wtot[wave][t] = (uint32_t)__popcll(m);
-    }
-    __syncthreads();
-    for (int t = 0; t < nt; ++t) {
-        const bool ok = (okMask >> t) & 1u;
-        const unsigned long long m = __ballot(ok);
-        if (ok) {
-            uint32_t pos = segCounts[seg * RH_SEG_STRIDE + t] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            for (int w = 0; w < wave; ++w) pos += wtot[w][t];
-            const unsigned long long c0 = capStart[B.task0 + t];
-            if ((unsigned long long)pos < capStart[B.task0 + t + 1] - c0) lists[c0 + pos] = G.mpBase + (uint32_t)tid;
+    const int lane = threadIdx.x & 63;
+    const size_t waves = (size_t)gridDim.x * (blockDim.x >> 6);
+    for (size_t seg = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); seg < nseg; seg += waves) {
+        const BucketDev B = buckets[segBucket[seg]];
+        const GridDev G = grids[candGrid[seg]];
+        const int nt = B.ntask, nmp = G.cu * G.cv;
+        uint32_t pos[RH_MAX_TASKS];
+        for (int t = 0; t < nt; ++t) pos[t] = segCounts[seg * RH_SEG_STRIDE + t];
+        for (int base = 0; base < nmp; base += 64) {
+            const int m = base + lane;
+            int sX = 0, eX = 0, sY = 0, eY = 0;
+            const bool keep = m < nmp && bin_window(fr, B, mpbound[G.mpBase + m], sX, eX, sY, eY);
+            if (!__ballot(keep)) continue;
+            for (int t = 0; t < nt; ++t) {
+                const TaskDev T = tasks[B.task0 + t];
+                const bool ok = keep && sX < T.x1 - B.sx0 && eX > T.x0 - B.sx0 && sY < T.y1 - B.sy0 &&
+                                eY > T.y0 - B.sy0;
+                const unsigned long long msk = __ballot(ok);
+                if (ok) {
+                    const uint32_t p = pos[t] + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
+                    const unsigned long long c0 = capStart[B.task0 + t];
+                    if ((unsigned long long)p < capStart[B.task0 + t + 1] - c0) lists[c0 + p] = G.mpBase + (uint32_t)m;
+                }
+                pos[t] += (uint32_t)__popcll(msk);
+            }
         }
     }
 }
@@ -1280,14 +1287,16 @@ extern "C" hipError_t rh_launch_bin(hipStream_t st, const FrameDev* fr, const Bu
                                     uint32_t* lists, uint32_t* listCount, uint32_t* bucketRefs,
                                     unsigned long long* counters) {
     if (nbuckets <= 0) return hipSuccess;
+    // enough waves to fill the device; each walks segments grid-stride
+    const size_t wgs = std::min<size_t>((nsegments + 3) / 4, 8192);
     if (nsegments > 0)
-        hipLaunchKernelGGL(rh_bin_count, dim3((unsigned)nsegments), dim3(256), 0, st, fr, buckets, tasks, grids,
-                           candGrid, segBucket, mpbound, segCounts);
+        hipLaunchKernelGGL(rh_bin_count, dim3((unsigned)wgs), dim3(256), 0, st, fr, buckets, tasks, grids, candGrid,
+                           segBucket, nsegments, mpbound, segCounts);
     hipLaunchKernelGGL(rh_bin_scan, dim3(nbuckets), dim3(256), 0, st, buckets, candStart, capStart, segCounts,
                        listCount, bucketRefs, counters);
     if (nsegments > 0)
-        hipLaunchKernelGGL(rh_bin_write, dim3((unsigned)nsegments), dim3(256), 0, st, fr, buckets, tasks, grids,
-                           candGrid, segBucket, mpbound, segCounts, capStart, lists);
+        hipLaunchKernelGGL(rh_bin_write, dim3((unsigned)wgs), dim3(256), 0, st, fr, buckets, tasks, grids, candGrid,
+                           segBucket, nsegments, mpbound, segCounts, capStart, lists);
     return hipGetLastError();
 }
 
