@@ -111,5 +111,5 @@ struct SampleLaunch {
 
 // Sampling kernel variants: LDS chunk of CH records, SLOTS blocks per wave.
 #define RH_SAMPLE_VARIANTS 4
-#define RH_VARIANT_CH(v) ((v) < 2 ? 224 : 96)
+#define RH_VARIANT_CH(v) ((v) < 2 ? 216 : 96)
 #define RH_VARIANT_SLOTS(v) (((v) & 1) ? 2 : 4)

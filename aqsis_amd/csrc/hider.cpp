@@ -272,7 +272,7 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
         if (e == hipSuccess) e = c->dRows.ensure((size_t)c->maxRows * 5 * sizeof(uint32_t));
     }
     c->tuning.mode = RH_MODE_GRID;
-    c->tuning.lds_records = 224;
+    c->tuning.lds_records = 216;
     c->tuning.blocks_per_wave = 4;
     c->tuning.split_cost = 1u << 20;
     c->tuning.frame_ms = 0.0;
@@ -396,7 +396,7 @@ static rh_status upload_schedule(rh_ctx* c) {
 }
 
 static int variant_of(const rh_tuning& t) {
-    return (t.lds_records == 224 ? 0 : 2) + (t.blocks_per_wave == 4 ? 0 : 1);
+    return (t.lds_records == 216 ? 0 : 2) + (t.blocks_per_wave == 4 ? 0 : 1);
 }
 
 // Enqueue one device frame on the context's stream.  With `timed`, HIP
@@ -548,7 +548,7 @@ rh_status rh_set_tuning(rh_ctx* c, const rh_tuning* t) {
     if (!c || !t) return fail(RH_ERR_INVALID, "null argument");
     if (t->mode != RH_MODE_GRID && t->mode != RH_MODE_PLAIN && t->mode != RH_MODE_BATCHED)
         return fail(RH_ERR_INVALID, "unknown mode");
-    if (t->lds_records != 224 && t->lds_records != 96) return fail(RH_ERR_INVALID, "lds_records must be 224 or 96");
+    if (t->lds_records != 216 && t->lds_records != 96) return fail(RH_ERR_INVALID, "lds_records must be 216 or 96");
     if (t->blocks_per_wave != 4 && t->blocks_per_wave != 2)
         return fail(RH_ERR_INVALID, "blocks_per_wave must be 4 or 2");
     if (t->split_cost == 0) return fail(RH_ERR_INVALID, "split_cost must be positive");
@@ -573,7 +573,7 @@ rh_status rh_autotune(rh_ctx* c, rh_tuning* chosen) {
     }
     rh_tuning best;
     double bestMs = 1e300;
-    const int lds[2] = {224, 96}, bpw[2] = {4, 2};
+    const int lds[2] = {216, 96}, bpw[2] = {4, 2};
     // 0: the static grid.  Small split costs make many short items whose list
     // re-reads outweigh the balance they buy on dense frames; large ones
     // leave a few heavy buckets as the tail.

@@ -453,14 +453,18 @@ __global__ void __launch_bounds__(1024) rh_reduce_counters(const uint32_t* __res
 // ---------------------------------------------------------------------------
 // Sampling: per-sample visibility in the reference's micropolygon order
 // ---------------------------------------------------------------------------
-// One 256-thread workgroup per task (a part of a bucket's owned sample
-// region).  The task's ordered micropolygon list is staged through LDS in
-// chunks of RH_S_CH hit-test records.  Each wave owns up to RH_S_SLOTS blocks
-// of 64 samples (lane = one sample).  For every chunk and block the wave
-// compacts the micropolygons whose candidate rectangle meets the block; then
-//  (a) for each of them in turn, every lane tests whether its sample is a
-//      candidate and lies in the micropolygon's bound (order-free tests,
-//      uniform across the wave), keeping a bit mask of the bound hits;
+// One 256-thread workgroup per work item (blocks of a task, a part of a
+// bucket's owned sample region).  The task's ordered micropolygon list is
+// staged through LDS in chunks of CH hit-test records.  Each wave owns up to
+// SLOTS blocks of 64 samples; lane = one sample, in row-major order of the
+// block's sub-sample grid, so a candidate rectangle is a 64-bit lane mask.
+// For every chunk and block the wave compacts the micropolygons whose
+// candidate rectangle meets the block, leaving out (but counting as tested)
+// those whose minimum depth lies beyond every sample's current opaque depth,
+// which the occlusion test would skip anyway; then, 64 at a time,
+//  (a) for each of them in turn, the lanes of its rectangle test their sample
+//      against its bound (order-free, uniform across the wave), keeping a
+//      bit mask of the bound hits;
 //  (b) each lane pops its bound hits in list order and runs the
 //      order-dependent part: the occlusion skip against its current depth,
 //      point-in-polygon, inverse bilinear, depth and the store test.
@@ -476,11 +480,30 @@ __global__ void __launch_bounds__(1024) rh_reduce_counters(const uint32_t* __res
 // degenerate micropolygon repeats its closing edge) and CqInvBilinear's
 // set-up on the grid-order vertices (bilinear.h:260-275).  Out: YM, XM, X, Y,
 // z, (A, E), (F, G), bound, (minz, linear, grid, opaque).
+// Lanes of a wave block are samples in row-major order of the block's
+// sub-sample grid (W = blockW * nx columns, H = blockH * ny rows, W * H <= 64).
+// below(n): the lanes 0..n-1.
+__device__ __forceinline__ unsigned long long lanes_below(int n) {
+    return n >= 64 ? ~0ull : (1ull << n) - 1ull;
+}
+
+// The lanes whose samples lie in a candidate rectangle (absolute sub-sample
+// grid ranges [lo, hi) packed lo | hi << 16), for the block whose first
+// column and row are BX0, BY0.  rep has bit r * W set for every row r.
+__device__ __forceinline__ unsigned long long block_rect_lanes(uint2 mt, int BX0, int BY0, int W, int H,
+                                                               unsigned long long rep) {
+    const int x0 = max((int)(mt.x & 0xffffu) - BX0, 0), x1 = min((int)(mt.x >> 16) - BX0, W);
+    const int y0 = max((int)(mt.y & 0xffffu) - BY0, 0), y1 = min((int)(mt.y >> 16) - BY0, H);
+    if (x1 <= x0 || y1 <= y0) return 0ull;
+    const unsigned long long cols = (lanes_below(x1) & ~lanes_below(x0)) * rep;
+    return cols & lanes_below(y1 * W) & ~lanes_below(y0 * W);
+}
+
 __device__ __forceinline__ float pick4(int i, float a0, float a1, float a2, float a3) {
     return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
 }
 
-__device__ __forceinline__ void expand_record(const float4* __restrict__ src, float4* out) {
+__device__ __forceinline__ void expand_record(const float4* __restrict__ src, uint32_t id, float4* out) {
     const float4 a = src[0], b = src[1], zz = src[2], bd = src[3], r4 = src[4];
     const uint32_t fl = __float_as_uint(r4.y);
     const int code = (int)(fl & 0xffu);
@@ -522,7 +545,7 @@ __device__ __forceinline__ void expand_record(const float4* __restrict__ src, fl
     out[5] = make_float4(a.x, a.y, Ex, Ey);
     out[6] = make_float4(Fx, Fy, Gx, Gy);
     out[7] = bd;
-    out[8] = make_float4(r4.x, __int_as_float((fl >> 9) & 1u), r4.z, __int_as_float((fl >> 10) & 1u));
+    out[8] = make_float4(r4.x, __uint_as_float((fl >> 9) & 3u), r4.z, __uint_as_float(id));
 }
 
 // Colour and opacity of a hit: InterpolateOutputs with smooth shading of
@@ -566,15 +589,22 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
     __shared__ float4 recs[CH * 9];
     __shared__ uint32_t wsum[RH_S_WAVES][5];
     __shared__ uint2 meta[CH];
-    __shared__ uint32_t ids[CH];
     __shared__ uint8_t matched[RH_S_WAVES][CH];
+    __shared__ unsigned long long batM[RH_S_WAVES][64];
+    __shared__ float4 batB[RH_S_WAVES][64];
     __shared__ int sItem;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ns = fr->ns, nx = fr->nx, ny = fr->ny, bw = fr->blockW, bh = fr->blockH;
-    const int pix = lane / ns, si = lane - pix * ns;
-    const int lx = pix % bw, ly = pix / bw;
-    const int sm = si % nx, sn = si / nx;
+    // lane -> (column, row) of the block's sub-sample grid -> pixel, sub-sample
+    const int BW = bw * nx, BH = bh * ny;
+    const int rx = lane % BW, ry = lane / BW;
+    const bool laneOk = lane < BW * BH;
+    const int lx = rx / nx, sm = rx - lx * nx, ly = ry / ny, sn = ry - ly * ny;
+    const int pix = ly * bw + lx, si = sn * nx + sm;
+    unsigned long long rowRep = 0ull;
+    for (int r = 0; r < BH; ++r) rowRep |= 1ull << (r * BW);
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
+    uint32_t nTestW = 0;                  // wave-uniform part of nTest
     uint32_t wbBase = 0;                  // the wave's current arena block
     int wbLeft = 0;
     bool lost = false;
@@ -624,7 +654,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
             has[k] = p0 + wave + RH_S_WAVES * k < blkN;
             const int bx = T.x0 + (blk % nbx) * bw, by = T.y0 + (blk / nbx) * bh;
             const int px = bx + lx, py = by + ly;
-            live[k] = has[k] && pix < bw * bh && px < T.x1 && py < T.y1;
+            live[k] = has[k] && laneOk && px < T.x1 && py < T.y1;
             gx[k] = (px - B.sx0) * nx + sm;
             gy[k] = (py - B.sy0) * ny + sn;
             bgx[k] = (bx - B.sx0) * nx;
@@ -652,10 +682,9 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
             if (tid < cnt) {
                 const uint32_t id = L[c0 + tid];
                 float4 r[9];
-                expand_record(mprec4 + (size_t)id * (RH_MPREC_FLOATS / 4), r);
+                expand_record(mprec4 + (size_t)id * (RH_MPREC_FLOATS / 4), id, r);
 #pragma unroll
                 for (int q = 0; q < 9; ++q) recs[tid * 9 + q] = r[q];
-                ids[tid] = id;
                 const float bminx = r[7].x, bmaxx = r[7].y, bminy = r[7].z, bmaxy = r[7].w;
                 int eX = (int)lceil_f(bmaxx), eY = (int)lceil_f(bmaxy);
                 if (eX > B.sx1) eX = B.sx1;
@@ -680,17 +709,30 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
 #pragma unroll
             for (int k = 0; k < SLOTS; ++k) {
                 if (!has[k]) continue;  // uniform across the wave
-                const int BX0 = bgx[k], BX1 = bgx[k] + bw * nx, BY0 = bgy[k], BY1 = bgy[k] + bh * ny;
+                const int BX0 = bgx[k], BX1 = bgx[k] + BW, BY0 = bgy[k], BY1 = bgy[k] + BH;
+                const unsigned long long liveM = __ballot(live[k]), inM = __ballot(live[k] && lodOk[k]);
+                // The block's farthest opaque depth so far: a micropolygon whose
+                // minimum depth lies beyond it is skipped by every sample's
+                // occlusion test (depths only decrease), so it is counted as
+                // tested and not visited.
+                float zmax = live[k] ? occlZ[k] : -FLT_MAX;
+                for (int off = 32; off > 0; off >>= 1) zmax = fmaxf(zmax, __shfl_xor(zmax, off));
                 // Ordered list of the chunk's micropolygons that meet this block.
                 int nm = 0;
                 for (int j0 = 0; j0 < cnt; j0 += 64) {
                     const int j = j0 + lane;
-                    bool ok = false;
+                    bool ok = false, hidden = false;
+                    uint2 mt = make_uint2(0u, 0u);
                     if (j < cnt) {
-                        const uint2 mt = meta[j];
+                        mt = meta[j];
                         ok = (int)(mt.x & 0xffffu) < BX1 && (int)(mt.x >> 16) > BX0 && (int)(mt.y & 0xffffu) < BY1 &&
                              (int)(mt.y >> 16) > BY0;
+                        if (ok && recs[j * 9 + 8].x > zmax) {
+                            hidden = true;
+                            ok = false;
+                        }
                     }
+                    if (hidden) nTest += (uint32_t)__popcll(block_rect_lanes(mt, BX0, BY0, BW, BH, rowRep) & liveM);
                     const unsigned long long msk = __ballot(ok);
                     if (ok) matched[wave][nm + __popcll(msk & ((1ull << lane) - 1ull))] = (uint8_t)j;
                     nm += __popcll(msk);
@@ -698,23 +740,58 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const float psx = sx[k], psy = sy[k];
                 for (int i0 = 0; i0 < nm; i0 += 64) {
                     const int ni = (nm - i0) < 64 ? (nm - i0) : 64;
-                    // (a) candidate and bound tests, one micropolygon at a time.
-                    unsigned long long pb = 0ull;
-                    for (int i = 0; i < ni; ++i) {
-                        const int j = __builtin_amdgcn_readfirstlane((int)matched[wave][i0 + i]);
-                        const uint2 mt = meta[j];
-                        const float4 bnd = recs[j * 9 + 7];
-                        const bool cand = live[k] && (uint32_t)gx[k] >= (mt.x & 0xffffu) &&
-                                          (uint32_t)gx[k] < (mt.x >> 16) && (uint32_t)gy[k] >= (mt.y & 0xffffu) &&
-                                          (uint32_t)gy[k] < (mt.y >> 16);
-                        nTest += cand ? 1u : 0u;
-                        // CqBound::Contains2D (bound.h:144-151)
-                        const bool in = cand && lodOk[k] &&
-                                        !(sx[k] < bnd.x || sx[k] > bnd.y || sy[k] < bnd.z || sy[k] > bnd.w);
-                        pb |= (unsigned long long)in << i;
+                    const int ni4 = (ni + 3) & ~3;
+                    // The batch: lane i holds matched micropolygon i0 + i as the
+                    // lanes of its candidate rectangle that go on to the bound
+                    // test, and its bound; it counts the rectangle's live lanes
+                    // as tested.  Padding entries have no lanes.
+                    if (lane < ni4) {
+                        unsigned long long m = 0ull;
+                        float4 bd = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (lane < ni) {
+                            const int jj = matched[wave][i0 + lane];
+                            m = block_rect_lanes(meta[jj], BX0, BY0, BW, BH, rowRep);
+                            bd = recs[jj * 9 + 7];
+                            nTest += (uint32_t)__popcll(m & liveM);
+                        }
+                        batM[wave][lane] = m & inM;
+                        batB[wave][lane] = bd;
                     }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    // (a) candidate and bound tests, one micropolygon at a time;
+                    // the first micropolygon ends in the high bit.
+                    uint32_t ph = 0u, pl = 0u;
+                    // four entries: broadcast reads first, then branch-free tests
+                    auto test4 = [&](int i, uint32_t acc) -> uint32_t {
+                        unsigned long long mm[4];
+                        float4 bd[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            mm[q] = batM[wave][i + q];
+                            bd[q] = batB[wave][i + q];
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            // CqBound::Contains2D (bound.h:144-151)
+                            const bool inB = !(psx < bd[q].x) & !(psx > bd[q].y) & !(psy < bd[q].z) & !(psy > bd[q].w);
+                            const bool in = __builtin_amdgcn_inverse_ballot_w64(mm[q]) & inB;
+                            acc = acc + acc + (uint32_t)in;
+                        }
+                        return acc;
+                    };
+                    const int n0 = ni4 < 32 ? ni4 : 32, n1 = ni4 - n0;
+                    for (int i = 0; i < n0; i += 4) ph = test4(i, ph);
+                    for (int i = 32; i < ni4; i += 4) pl = test4(i, pl);
+                    // bit i of pb: micropolygon i0 + i
+                    ph = ph << (32 - n0);
+                    pl = n1 > 0 ? pl << (32 - n1) : 0u;
+                    unsigned long long pb = ((unsigned long long)__builtin_bitreverse32(pl) << 32) |
+                                            (unsigned long long)__builtin_bitreverse32(ph);
                     // (b) the order-dependent tests, each lane its own hits in list order.
                     while (__ballot(pb != 0ull)) {
                         bool app = false;          // a transparent hit to append
@@ -753,7 +830,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                                 u = u - det * (bx * M2y - by * M2x);
                                 v = v - det * (-(bx * M1y - by * M1x));
                             }
-                            if (__float_as_int(r8.y) == 0) {
+                            if ((__float_as_uint(r8.y) & 1u) == 0) {
                                 ++nIn2;
                                 const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
                                 const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
@@ -770,16 +847,16 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                             // StoreSample (bucketprocessor.cpp:1471-1569): reject when occlZ <= D
                             if (occlZ[k] <= D) break;
                             ++nHit;
-                            if (__float_as_int(r8.w) != 0) {
+                            if ((__float_as_uint(r8.y) & 2u) != 0) {
                                 // opaque: the occluding hit
                                 occlZ[k] = D;
-                                win[k] = ids[j];
+                                win[k] = __float_as_uint(r8.w);
                                 wu[k] = u;
                                 wv[k] = v;
                             } else {
                                 // transparent: a new entry of the sample's hit list
                                 const PrimDev& P = prims[grids[__float_as_int(r8.z)].prim];
-                                shade_hit(P, u, v, D, ids[j], hA, hB);
+                                shade_hit(P, u, v, D, __float_as_uint(r8.w), hA, hB);
                                 app = true;
                             }
                         } while (false);
@@ -843,6 +920,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
     }
     }   // items
     if (__ballot(lost) && lane == 0) atomicAdd(&hitCounter[1], 1u);
+    if (lane == 0) nTest += nTestW;
     // The workgroup's hider counters: wave sums, then one row (summed by
     // rh_reduce_counters; same-address atomics from every wave would
     // serialise in L2).
@@ -1314,8 +1392,8 @@ static void launch_sample_t(hipStream_t st, int grid, const SampleLaunch& a) {
 extern "C" hipError_t rh_launch_sample(hipStream_t st, int variant, int grid, const SampleLaunch* a) {
     if (grid <= 0) return hipSuccess;
     switch (variant) {
-    case 0: launch_sample_t<224, 4>(st, grid, *a); break;
-    case 1: launch_sample_t<224, 2>(st, grid, *a); break;
+    case 0: launch_sample_t<216, 4>(st, grid, *a); break;
+    case 1: launch_sample_t<216, 2>(st, grid, *a); break;
     case 2: launch_sample_t<96, 4>(st, grid, *a); break;
     case 3: launch_sample_t<96, 2>(st, grid, *a); break;
     default: return hipErrorInvalidValue;
@@ -1326,8 +1404,8 @@ extern "C" hipError_t rh_launch_sample(hipStream_t st, int variant, int grid, co
 // Workgroups of one sampling variant that fit the device at once.
 extern "C" hipError_t rh_sample_occupancy(int variant, int* blocksPerCU) {
     switch (variant) {
-    case 0: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<224, 4>, 256, 0);
-    case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<224, 2>, 256, 0);
+    case 0: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<216, 4>, 256, 0);
+    case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<216, 2>, 256, 0);
     case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<96, 4>, 256, 0);
     case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<96, 2>, 256, 0);
     default: return hipErrorInvalidValue;

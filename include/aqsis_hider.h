@@ -157,7 +157,7 @@ enum { RH_MODE_GRID = 0, RH_MODE_PLAIN = 1, RH_MODE_BATCHED = 2 };
 
 typedef struct rh_tuning {
     int32_t mode;              /* RH_MODE_*                                     */
-    int32_t lds_records;       /* micropolygon records per LDS chunk: 224 or 96 */
+    int32_t lds_records;       /* micropolygon records per LDS chunk: 216 or 96 */
     int32_t blocks_per_wave;   /* 64-sample blocks per wave per pass: 4 or 2    */
     uint32_t split_cost;       /* batched: largest work item, in list entries x blocks */
     double frame_ms;           /* rh_autotune: device time of the chosen setting */

@@ -39,7 +39,7 @@ def test_layouts_give_identical_frames(case):
         h.set_tuning(mode="plain")
         h.render_device()
         _same(ref, _frame(h))
-        for lds in (224, 96):
+        for lds in (216, 96):
             for bpw in (4, 2):
                 for split in (1 << 10, 1 << 16):
                     h.set_tuning(mode="batched", lds_records=lds, blocks_per_wave=bpw, split_cost=split)
