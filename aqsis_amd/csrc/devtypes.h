@@ -66,6 +66,7 @@ struct BucketDev {
     int sx0, sy0, sx1, sy1;       // owned sample region
     int bx, by;                   // bucket coordinates
     int task0, ntask;             // the bucket's sampling tasks
+    int ntx, nty;                 // ... a row-major grid of ntx x nty
 };
 
 struct TaskDev {

@@ -19,7 +19,7 @@ extern "C" hipError_t rh_launch_dice(hipStream_t, const FrameDev*, const GridDev
 extern "C" hipError_t rh_launch_bin(hipStream_t, const FrameDev*, const BucketDev*, int, const TaskDev*,
                                     const GridDev*, const uint32_t*, const uint32_t*, const uint32_t*, size_t,
                                     const unsigned long long*, const float4*, uint32_t*, uint32_t*, uint32_t*,
-                                    uint32_t*, unsigned long long*);
+                                    uint32_t*, unsigned long long*, int, int);
 extern "C" hipError_t rh_launch_sample(hipStream_t, int, int, const SampleLaunch*);
 extern "C" hipError_t rh_sample_occupancy(int, int*);
 extern "C" hipError_t rh_launch_plan_items(hipStream_t, const FrameDev*, const TaskDev*, int, const uint32_t*,
@@ -92,6 +92,7 @@ struct rh_ctx {
     std::vector<rh_prim> prims;
     rh::Schedule sched;
     bool scheduled = false;
+    int binTaskSide = 1;                  // largest task grid side of a bucket
     bool rendered = false;
     hipStream_t st = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -195,6 +196,7 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
     c->frame = f;
 
     std::vector<BucketDev> bk;
+    c->binTaskSide = 1;
     for (size_t k = 0; k < s.buckets.size(); ++k) {
         const auto& b = s.buckets[k];
         BucketDev d;
@@ -203,6 +205,11 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
         d.bx = s.bx0 + (int)(k % s.nbr); d.by = s.by0 + (int)(k / s.nbr);
         d.task0 = s.taskStart[k];
         d.ntask = s.taskStart[k + 1] - s.taskStart[k];
+        // tasks are in row-major order: the first row's length is the width
+        d.ntx = 0;
+        while (d.ntx < d.ntask && s.tasks[d.task0 + d.ntx].y0 == s.tasks[d.task0].y0) ++d.ntx;
+        d.nty = d.ntx > 0 ? d.ntask / d.ntx : 0;
+        c->binTaskSide = std::max(c->binTaskSide, std::max(d.ntx, d.nty));
         bk.push_back(d);
     }
     std::vector<TaskDev> tk(s.tasks.size());
@@ -418,7 +425,8 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
                       c->dCandStart.as<uint32_t>(), c->dCandGrid.as<uint32_t>(), c->dSegBucket.as<uint32_t>(),
                       S.candGrid.size(), c->dCapStart.as<unsigned long long>(), c->dMpbound.as<float4>(),
                       c->dSegCounts.as<uint32_t>(), c->dLists.as<uint32_t>(), c->dListCount.as<uint32_t>(),
-                      c->dBucketRefs.as<uint32_t>(), c->dCounters.as<unsigned long long>()));
+                      c->dBucketRefs.as<uint32_t>(), c->dCounters.as<unsigned long long>(), c->binTaskSide,
+                      ((c->frame.bucketX & (c->frame.bucketX - 1)) == 0 && (c->frame.bucketY & (c->frame.bucketY - 1)) == 0) ? 1 : 0));
     if (timed) ENQ(hipEventRecord(c->ev[2], c->st));
     SampleLaunch a;
     a.fr = f; a.buckets = c->dBuckets.as<BucketDev>(); a.tasks = c->dTasks.as<TaskDev>();

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cstdint>
 
 #include "devtypes.h"
@@ -42,6 +43,12 @@ __device__ __forceinline__ int floor_i(float x) {  // static_cast<int>(std::floo
 __device__ __forceinline__ long lceil_f(float x) {  // math.h:57-61
     long t = (long)x;
     return t + (x > 0 && x != (float)t);
+}
+// (int)lceil(x), the low 32 bits of lceil's long: ceil(x) inside
+// (-2^31, 2^31), the long form elsewhere and for NaN.
+__device__ __forceinline__ int lceil_i(float x) {
+    if (fabsf(x) < 2147483648.0f) return (int)ceilf(x);
+    return (int)lceil_f(x);
 }
 __device__ __forceinline__ long lfloor_d(double x) {  // math.h:47-51
     long t = (long)x;
@@ -268,39 +275,81 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
 #define RH_MAX_TASKS 16
 #define RH_SEG_STRIDE (RH_MAX_TASKS + 1)   // per segment: a count per task, then kept
 
-// AddMPG for bucket B, then RenderMPG_Static's pixel range clipped to the
-// bucket's sample region (bucketprocessor.cpp:1120-1140), relative to the
-// region; empty ranges do no work and are not kept.
-__device__ __forceinline__ bool bin_window(const FrameDev* __restrict__ fr, const BucketDev& B, const float4 bd,
-                                           int& sX, int& eX, int& sY, int& eY) {
+// The frame's constants of AddMPG's bucket test, read once per wave.
+struct BinConst {
+    float dx0, dy0, dx1, dy1;   // crop window grown by half the filter width
+    float dsx, dsy;             // bound growth: lfloor(filter width / 2)
+    float bxs, bys;             // bucket size
+    float ibx, iby;             // 1 / bucket size (used when a power of two)
+};
+
+__device__ __forceinline__ BinConst bin_const(const FrameDev* __restrict__ fr) {
+    BinConst k;
     const float hfx = fr->filterX * 0.5f, hfy = fr->filterY * 0.5f;   // xFiltSize / 2.0f
-    const float dsx = (float)fr->shiftX, dsy = (float)fr->shiftY;      // lfloor(xFiltSize / 2.0f)
-    const bool discard = bd.y < fr->cropX0 - hfx || bd.w < fr->cropY0 - hfy || bd.x > fr->cropX1 + hfx ||
-                         bd.z > fr->cropY1 + hfy;
-    if (discard) return false;
-    const float ex0 = bd.x - dsx, ey0 = bd.z - dsy, ex1 = bd.y + dsx, ey1 = bd.w + dsy;
-    int iXBa = (int)(ex0 / (float)fr->bucketX), iYBa = (int)(ey0 / (float)fr->bucketY);
-    int iXBb = (int)(ex1 / (float)fr->bucketX), iYBb = (int)(ey1 / (float)fr->bucketY);
-    if (iXBb < fr->bx0 || iYBb < fr->by0 || iXBa >= fr->bx1 || iYBa >= fr->by1) return false;
-    if (iXBa < fr->bx0) iXBa = fr->bx0;
-    if (iYBa < fr->by0) iYBa = fr->by0;
-    if (iXBb >= fr->bx1) iXBb = fr->bx1 - 1;
-    if (iYBb >= fr->by1) iYBb = fr->by1 - 1;
-    if (!(B.bx >= iXBa && B.bx <= iXBb && B.by >= iYBa && B.by <= iYBb)) return false;
-    eX = (int)lceil_f(bd.y); eY = (int)lceil_f(bd.w);
-    if (eX > B.sx1) eX = B.sx1;
-    if (eY > B.sy1) eY = B.sy1;
-    sX = floor_i(bd.x); sY = floor_i(bd.z);
-    if (sY < B.sy0) sY = B.sy0;
-    if (sX < B.sx0) sX = B.sx0;
-    if (!(sX < eX && sY < eY)) return false;
+    k.dx0 = fr->cropX0 - hfx; k.dy0 = fr->cropY0 - hfy;
+    k.dx1 = fr->cropX1 + hfx; k.dy1 = fr->cropY1 + hfy;
+    k.dsx = (float)fr->shiftX; k.dsy = (float)fr->shiftY;
+    k.bxs = (float)fr->bucketX; k.bys = (float)fr->bucketY;
+    k.ibx = 1.0f / k.bxs; k.iby = 1.0f / k.bys;
+    return k;
+}
+
+// AddMPG for bucket B (imagebuffer.cpp:514-591), then RenderMPG_Static's
+// pixel range clipped to the bucket's sample region
+// (bucketprocessor.cpp:1120-1140), relative to the region; empty ranges are
+// not kept.  Branch-free.  The bucket index clamps of AddMPG are left out:
+// B lies inside the bucket region, so they never change the outcome.  A
+// division by a power-of-two bucket size equals the multiplication by its
+// reciprocal (both round the same exact quotient).
+template <bool POW2>
+__device__ __forceinline__ bool bin_window(const BinConst& k, const BucketDev& B, const float4 bd, int& sX, int& eX,
+                                           int& sY, int& eY) {
+    const bool discard = (bd.y < k.dx0) | (bd.w < k.dy0) | (bd.x > k.dx1) | (bd.z > k.dy1);
+    const float ex0 = bd.x - k.dsx, ey0 = bd.z - k.dsy, ex1 = bd.y + k.dsx, ey1 = bd.w + k.dsy;
+    const int iXBa = (int)(POW2 ? ex0 * k.ibx : ex0 / k.bxs), iYBa = (int)(POW2 ? ey0 * k.iby : ey0 / k.bys);
+    const int iXBb = (int)(POW2 ? ex1 * k.ibx : ex1 / k.bxs), iYBb = (int)(POW2 ? ey1 * k.iby : ey1 / k.bys);
+    const bool inB = (iXBa <= B.bx) & (B.bx <= iXBb) & (iYBa <= B.by) & (B.by <= iYBb);
+    eX = min(lceil_i(bd.y), B.sx1);
+    eY = min(lceil_i(bd.w), B.sy1);
+    sX = max(floor_i(bd.x), B.sx0);
+    sY = max(floor_i(bd.z), B.sy0);
+    const bool keep = !discard & inB & (sX < eX) & (sY < eY);
     sX -= B.sx0; eX -= B.sx0; sY -= B.sy0; eY -= B.sy0;
-    return true;
+    return keep;
+}
+
+// A bucket's tasks form a row-major grid of B.ntx x B.nty rectangles; the
+// column and row ranges relative to the bucket's sample region (columns or
+// rows beyond the grid are empty ranges that nothing meets).
+template <int MT>
+__device__ __forceinline__ void task_ranges(const TaskDev* __restrict__ tasks, const BucketDev& B, int* cLo,
+                                            int* cHi, int* rLo, int* rHi) {
+#pragma unroll
+    for (int c = 0; c < MT; ++c) {
+        cLo[c] = INT_MAX;
+        cHi[c] = INT_MIN;
+        if (c < B.ntx) {
+            const TaskDev T = tasks[B.task0 + c];
+            cLo[c] = T.x0 - B.sx0;
+            cHi[c] = T.x1 - B.sx0;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < MT; ++r) {
+        rLo[r] = INT_MAX;
+        rHi[r] = INT_MIN;
+        if (r < B.nty) {
+            const TaskDev T = tasks[B.task0 + r * B.ntx];
+            rLo[r] = T.y0 - B.sy0;
+            rHi[r] = T.y1 - B.sy0;
+        }
+    }
 }
 
 // Pass 1: per segment, the kept micropolygons per task and in all.  One
 // wave per segment at a time (grid-stride over segments), 64 micropolygons
-// per step; no barriers.
+// per step; no barriers.  MT: the largest task grid side (2 or 4).
+template <int MT, bool POW2>
 __global__ void __launch_bounds__(256) rh_bin_count(const FrameDev* __restrict__ fr,
                                                     const BucketDev* __restrict__ buckets,
                                                     const TaskDev* __restrict__ tasks, const GridDev* __restrict__ grids,
@@ -309,31 +358,41 @@ __global__ void __launch_bounds__(256) rh_bin_count(const FrameDev* __restrict__
                                                     const float4* __restrict__ mpbound,
                                                     uint32_t* __restrict__ segCounts) {
     const int lane = threadIdx.x & 63;
+    const BinConst K = bin_const(fr);
     const size_t waves = (size_t)gridDim.x * (blockDim.x >> 6);
     for (size_t seg = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); seg < nseg; seg += waves) {
         const BucketDev B = buckets[segBucket[seg]];
         const GridDev G = grids[candGrid[seg]];
-        const int nt = B.ntask, nmp = G.cu * G.cv;
-        uint32_t cnt[RH_SEG_STRIDE];
-        for (int t = 0; t < RH_SEG_STRIDE; ++t) cnt[t] = 0;
+        const int nmp = G.cu * G.cv;
+        int cLo[MT], cHi[MT], rLo[MT], rHi[MT];
+        task_ranges<MT>(tasks, B, cLo, cHi, rLo, rHi);
+        uint32_t cnt[MT * MT], all = 0;
+#pragma unroll
+        for (int t = 0; t < MT * MT; ++t) cnt[t] = 0;
         for (int base = 0; base < nmp; base += 64) {
             const int m = base + lane;
-            int sX = 0, eX = 0, sY = 0, eY = 0;
-            const bool keep = m < nmp && bin_window(fr, B, mpbound[G.mpBase + m], sX, eX, sY, eY);
-            cnt[RH_MAX_TASKS] += (uint32_t)__popcll(__ballot(keep));
-            for (int t = 0; t < nt; ++t) {
-                const TaskDev T = tasks[B.task0 + t];
-                const bool ok = keep && sX < T.x1 - B.sx0 && eX > T.x0 - B.sx0 && sY < T.y1 - B.sy0 &&
-                                eY > T.y0 - B.sy0;
-                cnt[t] += (uint32_t)__popcll(__ballot(ok));
-            }
+            const float4 bd = mpbound[G.mpBase + min(m, nmp - 1)];
+            int sX, eX, sY, eY;
+            const bool keep = (m < nmp) & bin_window<POW2>(K, B, bd, sX, eX, sY, eY);
+            all += (uint32_t)__popcll(__ballot(keep));
+            bool co[MT], ro[MT];
+#pragma unroll
+            for (int c = 0; c < MT; ++c) co[c] = keep & (sX < cHi[c]) & (eX > cLo[c]);
+#pragma unroll
+            for (int r = 0; r < MT; ++r) ro[r] = (sY < rHi[r]) & (eY > rLo[r]);
+#pragma unroll
+            for (int r = 0; r < MT; ++r)
+#pragma unroll
+                for (int c = 0; c < MT; ++c) cnt[r * MT + c] += (uint32_t)__popcll(__ballot(co[c] & ro[r]));
         }
-        if (lane < nt) {
+        if (lane < B.ntask) {
+            const int r = lane / B.ntx, c = lane - r * B.ntx;
             uint32_t v = 0;
-            for (int t = 0; t < RH_MAX_TASKS; ++t) v = t == lane ? cnt[t] : v;
+#pragma unroll
+            for (int t = 0; t < MT * MT; ++t) v = t == r * MT + c ? cnt[t] : v;
             segCounts[seg * RH_SEG_STRIDE + lane] = v;
         }
-        if (lane == 0) segCounts[seg * RH_SEG_STRIDE + RH_MAX_TASKS] = cnt[RH_MAX_TASKS];
+        if (lane == 0) segCounts[seg * RH_SEG_STRIDE + RH_MAX_TASKS] = all;
     }
 }
 
@@ -387,6 +446,7 @@ __global__ void __launch_bounds__(256) rh_bin_scan(const BucketDev* __restrict__
 
 // Pass 3: per segment, each task's kept micropolygons in order at the
 // segment's offset.  One wave per segment at a time, as pass 1.
+template <int MT, bool POW2>
 __global__ void __launch_bounds__(256) rh_bin_write(const FrameDev* __restrict__ fr,
                                                     const BucketDev* __restrict__ buckets,
                                                     const TaskDev* __restrict__ tasks, const GridDev* __restrict__ grids,
@@ -397,30 +457,55 @@ __global__ void __launch_bounds__(256) rh_bin_write(const FrameDev* __restrict__
                                                     const unsigned long long* __restrict__ capStart,
                                                     uint32_t* __restrict__ lists) {
     const int lane = threadIdx.x & 63;
+    const BinConst K = bin_const(fr);
     const size_t waves = (size_t)gridDim.x * (blockDim.x >> 6);
     for (size_t seg = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); seg < nseg; seg += waves) {
         const BucketDev B = buckets[segBucket[seg]];
         const GridDev G = grids[candGrid[seg]];
-        const int nt = B.ntask, nmp = G.cu * G.cv;
-        uint32_t pos[RH_MAX_TASKS];
-        for (int t = 0; t < nt; ++t) pos[t] = segCounts[seg * RH_SEG_STRIDE + t];
+        const int nmp = G.cu * G.cv;
+        int cLo[MT], cHi[MT], rLo[MT], rHi[MT];
+        task_ranges<MT>(tasks, B, cLo, cHi, rLo, rHi);
+        // per task of the grid: next position, list start and capacity
+        uint32_t pos[MT * MT], lim[MT * MT];
+        unsigned long long start[MT * MT];
+#pragma unroll
+        for (int r = 0; r < MT; ++r)
+#pragma unroll
+            for (int c = 0; c < MT; ++c) {
+                pos[r * MT + c] = 0;
+                lim[r * MT + c] = 0;
+                start[r * MT + c] = 0;
+                if (r < B.nty && c < B.ntx) {
+                    const int t = B.task0 + r * B.ntx + c;
+                    pos[r * MT + c] = segCounts[seg * RH_SEG_STRIDE + (t - B.task0)];
+                    start[r * MT + c] = capStart[t];
+                    lim[r * MT + c] = (uint32_t)(capStart[t + 1] - capStart[t]);
+                }
+            }
         for (int base = 0; base < nmp; base += 64) {
             const int m = base + lane;
-            int sX = 0, eX = 0, sY = 0, eY = 0;
-            const bool keep = m < nmp && bin_window(fr, B, mpbound[G.mpBase + m], sX, eX, sY, eY);
+            const float4 bd = mpbound[G.mpBase + min(m, nmp - 1)];
+            int sX, eX, sY, eY;
+            const bool keep = (m < nmp) & bin_window<POW2>(K, B, bd, sX, eX, sY, eY);
             if (!__ballot(keep)) continue;
-            for (int t = 0; t < nt; ++t) {
-                const TaskDev T = tasks[B.task0 + t];
-                const bool ok = keep && sX < T.x1 - B.sx0 && eX > T.x0 - B.sx0 && sY < T.y1 - B.sy0 &&
-                                eY > T.y0 - B.sy0;
-                const unsigned long long msk = __ballot(ok);
-                if (ok) {
-                    const uint32_t p = pos[t] + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
-                    const unsigned long long c0 = capStart[B.task0 + t];
-                    if ((unsigned long long)p < capStart[B.task0 + t + 1] - c0) lists[c0 + p] = G.mpBase + (uint32_t)m;
+            bool co[MT], ro[MT];
+#pragma unroll
+            for (int c = 0; c < MT; ++c) co[c] = keep & (sX < cHi[c]) & (eX > cLo[c]);
+#pragma unroll
+            for (int r = 0; r < MT; ++r) ro[r] = (sY < rHi[r]) & (eY > rLo[r]);
+#pragma unroll
+            for (int r = 0; r < MT; ++r)
+#pragma unroll
+                for (int c = 0; c < MT; ++c) {
+                    const int t = r * MT + c;
+                    const bool ok = co[c] & ro[r];
+                    const unsigned long long msk = __ballot(ok);
+                    if (ok) {
+                        const uint32_t p = pos[t] + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
+                        if (p < lim[t]) lists[start[t] + p] = G.mpBase + (uint32_t)m;
+                    }
+                    pos[t] += (uint32_t)__popcll(msk);
                 }
-                pos[t] += (uint32_t)__popcll(msk);
-            }
         }
     }
 }
@@ -686,7 +771,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
 #pragma unroll
                 for (int q = 0; q < 9; ++q) recs[tid * 9 + q] = r[q];
                 const float bminx = r[7].x, bmaxx = r[7].y, bminy = r[7].z, bmaxy = r[7].w;
-                int eX = (int)lceil_f(bmaxx), eY = (int)lceil_f(bmaxy);
+                int eX = lceil_i(bmaxx), eY = lceil_i(bmaxy);
                 if (eX > B.sx1) eX = B.sx1;
                 if (eY > B.sy1) eY = B.sy1;
                 int sX = floor_i(bminx), sY = floor_i(bminy);
@@ -696,8 +781,8 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                 if (sX < eX && sY < eY) {
                     const int im = (bminx < sX) ? 0 : floor_i((bminx - sX) * nx);
                     const int in = (bminy < sY) ? 0 : floor_i((bminy - sY) * ny);
-                    const int em = (bmaxx > eX) ? nx : (int)lceil_f((bmaxx - (eX - 1)) * nx);
-                    const int en = (bmaxy > eY) ? ny : (int)lceil_f((bmaxy - (eY - 1)) * ny);
+                    const int em = (bmaxx > eX) ? nx : lceil_i((bmaxx - (eX - 1)) * nx);
+                    const int en = (bmaxy > eY) ? ny : lceil_i((bmaxy - (eY - 1)) * ny);
                     gx0 = (uint32_t)((sX - B.sx0) * nx + im);
                     gx1 = (uint32_t)((eX - 1 - B.sx0) * nx + em);
                     gy0 = (uint32_t)((sY - B.sy0) * ny + in);
@@ -1358,23 +1443,45 @@ extern "C" hipError_t rh_launch_dice(hipStream_t st, const FrameDev* fr, const G
     return hipGetLastError();
 }
 
+template <int MT, bool POW2>
+static void launch_bin_t(hipStream_t st, size_t wgs, const FrameDev* fr, const BucketDev* buckets, int nbuckets,
+                         const TaskDev* tasks, const GridDev* grids, const uint32_t* candStart,
+                         const uint32_t* candGrid, const uint32_t* segBucket, size_t nsegments,
+                         const unsigned long long* capStart, const float4* mpbound, uint32_t* segCounts,
+                         uint32_t* lists, uint32_t* listCount, uint32_t* bucketRefs, unsigned long long* counters) {
+    if (nsegments > 0)
+        hipLaunchKernelGGL((rh_bin_count<MT, POW2>), dim3((unsigned)wgs), dim3(256), 0, st, fr, buckets, tasks, grids,
+                           candGrid, segBucket, nsegments, mpbound, segCounts);
+    hipLaunchKernelGGL(rh_bin_scan, dim3(nbuckets), dim3(256), 0, st, buckets, candStart, capStart, segCounts,
+                       listCount, bucketRefs, counters);
+    if (nsegments > 0)
+        hipLaunchKernelGGL((rh_bin_write<MT, POW2>), dim3((unsigned)wgs), dim3(256), 0, st, fr, buckets, tasks, grids,
+                           candGrid, segBucket, nsegments, mpbound, segCounts, capStart, lists);
+}
+
+// taskSide: the largest task grid side of any bucket (at most 4); pow2: both
+// bucket sizes are powers of two.
 extern "C" hipError_t rh_launch_bin(hipStream_t st, const FrameDev* fr, const BucketDev* buckets, int nbuckets,
                                     const TaskDev* tasks, const GridDev* grids, const uint32_t* candStart,
                                     const uint32_t* candGrid, const uint32_t* segBucket, size_t nsegments,
                                     const unsigned long long* capStart, const float4* mpbound, uint32_t* segCounts,
                                     uint32_t* lists, uint32_t* listCount, uint32_t* bucketRefs,
-                                    unsigned long long* counters) {
+                                    unsigned long long* counters, int taskSide, int pow2) {
     if (nbuckets <= 0) return hipSuccess;
+    if (taskSide < 1 || taskSide > 4) return hipErrorInvalidValue;
     // enough waves to fill the device; each walks segments grid-stride
     const size_t wgs = std::min<size_t>((nsegments + 3) / 4, 8192);
-    if (nsegments > 0)
-        hipLaunchKernelGGL(rh_bin_count, dim3((unsigned)wgs), dim3(256), 0, st, fr, buckets, tasks, grids, candGrid,
-                           segBucket, nsegments, mpbound, segCounts);
-    hipLaunchKernelGGL(rh_bin_scan, dim3(nbuckets), dim3(256), 0, st, buckets, candStart, capStart, segCounts,
-                       listCount, bucketRefs, counters);
-    if (nsegments > 0)
-        hipLaunchKernelGGL(rh_bin_write, dim3((unsigned)wgs), dim3(256), 0, st, fr, buckets, tasks, grids, candGrid,
-                           segBucket, nsegments, mpbound, segCounts, capStart, lists);
+    if (taskSide <= 2) {
+        if (pow2) launch_bin_t<2, true>(st, wgs, fr, buckets, nbuckets, tasks, grids, candStart, candGrid, segBucket,
+                                        nsegments, capStart, mpbound, segCounts, lists, listCount, bucketRefs, counters);
+        else launch_bin_t<2, false>(st, wgs, fr, buckets, nbuckets, tasks, grids, candStart, candGrid, segBucket,
+                                    nsegments, capStart, mpbound, segCounts, lists, listCount, bucketRefs, counters);
+    } else {
+        if (pow2) launch_bin_t<4, true>(st, wgs, fr, buckets, nbuckets, tasks, grids, candStart, candGrid, segBucket,
+                                        nsegments, capStart, mpbound, segCounts, lists, listCount, bucketRefs, counters);
+        else launch_bin_t<4, false>(st, wgs, fr, buckets, nbuckets, tasks, grids, candStart, candGrid, segBucket,
+                                    nsegments, capStart, mpbound, segCounts, lists, listCount, bucketRefs, counters);
+    }
     return hipGetLastError();
 }
 
