@@ -879,72 +879,75 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                                             (unsigned long long)__builtin_bitreverse32(ph);
                     // (b) the order-dependent tests, each lane its own hits in list order.
                     while (__ballot(pb != 0ull)) {
-                        bool app = false;          // a transparent hit to append
-                        float4 hA, hB;
-                        do {
-                            if (pb == 0ull) break;
+                        // One hit test per lane; the outcome is applied below in
+                        // one place (opaque store or transparent append).
+                        bool hit = false;
+                        float D = 0.f, u = 0.5f, v = 0.5f;
+                        uint32_t hflags = 0u, hid = 0u, hgrid = 0u;
+                        if (pb != 0ull) {
                             const int i = __ffsll((long long)pb) - 1;
                             pb &= pb - 1ull;
                             const int j = matched[wave][i0 + i];
                             const float4* R = &recs[j * 9];
                             const float4 r8 = R[8];
                             // occlusion skip against the current opaque hit (bucketprocessor.cpp:1179)
-                            if (r8.x > occlZ[k]) break;
-                            ++nBound;
-                            // fContains (micropolygon.cpp:1295-1342): edges 0,1 reject on <= 0, edges 2,3 on < 0
-                            const float4 YM = R[0], XM = R[1], X = R[2], Y = R[3];
-                            const float px_ = sx[k], py_ = sy[k];
-                            const float t0 = ((py_ - Y.x) * YM.x) - ((px_ - X.x) * XM.x);
-                            const float t1 = ((py_ - Y.y) * YM.y) - ((px_ - X.y) * XM.y);
-                            const float t2 = ((py_ - Y.z) * YM.z) - ((px_ - X.z) * XM.z);
-                            const float t3 = ((py_ - Y.w) * YM.w) - ((px_ - X.w) * XM.w);
-                            if (t0 <= 0 || t1 <= 0 || t2 < 0 || t3 < 0) break;
-                            ++nIn;
-                            // CqInvBilinear::operator() (bilinear.h:277-305)
-                            const float4 r5 = R[5], r6 = R[6], Z = R[4];
-                            const float Ax = r5.x, Ay = r5.y, Ex = r5.z, Ey = r5.w, Fx = r6.x, Fy = r6.y,
-                                        Gx = r6.z, Gy = r6.w;
-                            float u = 0.5f, v = 0.5f;
-                            {
-                                const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
-                                const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
-                                const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px_;
-                                const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py_;
-                                float det = M1x * M2y - M1y * M2x;
-                                det = 1 / det;
-                                u = u - det * (bx * M2y - by * M2x);
-                                v = v - det * (-(bx * M1y - by * M1x));
+                            if (!(r8.x > occlZ[k])) {
+                                ++nBound;
+                                // fContains (micropolygon.cpp:1295-1342): edges 0,1 reject on <= 0, edges 2,3 on < 0
+                                const float4 YM = R[0], XM = R[1], X = R[2], Y = R[3];
+                                const float px_ = sx[k], py_ = sy[k];
+                                const float t0 = ((py_ - Y.x) * YM.x) - ((px_ - X.x) * XM.x);
+                                const float t1 = ((py_ - Y.y) * YM.y) - ((px_ - X.y) * XM.y);
+                                const float t2 = ((py_ - Y.z) * YM.z) - ((px_ - X.z) * XM.z);
+                                const float t3 = ((py_ - Y.w) * YM.w) - ((px_ - X.w) * XM.w);
+                                if (!(t0 <= 0 || t1 <= 0 || t2 < 0 || t3 < 0)) {
+                                    ++nIn;
+                                    // CqInvBilinear::operator() (bilinear.h:277-305)
+                                    const float4 r5 = R[5], r6 = R[6], Z = R[4];
+                                    const float Ax = r5.x, Ay = r5.y, Ex = r5.z, Ey = r5.w, Fx = r6.x, Fy = r6.y,
+                                                Gx = r6.z, Gy = r6.w;
+                                    {
+                                        const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
+                                        const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
+                                        const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px_;
+                                        const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py_;
+                                        float det = M1x * M2y - M1y * M2x;
+                                        det = 1 / det;
+                                        u = u - det * (bx * M2y - by * M2x);
+                                        v = v - det * (-(bx * M1y - by * M1x));
+                                    }
+                                    hflags = __float_as_uint(r8.y);
+                                    if ((hflags & 1u) == 0) {
+                                        ++nIn2;
+                                        const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
+                                        const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
+                                        const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px_;
+                                        const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py_;
+                                        float det = M1x * M2y - M1y * M2x;
+                                        if (det != 0) det = 1 / det;
+                                        u = u - det * (bx * M2y - by * M2x);
+                                        v = v - det * (-(bx * M1y - by * M1x));
+                                    }
+                                    // bilerp (bilinear.h:228-236)
+                                    const float w0 = (1 - v) * (1 - u), w1 = (1 - v) * u, w2 = v * (1 - u),
+                                                w3 = v * u;
+                                    D = ((w0 * Z.x + w1 * Z.y) + w2 * Z.z) + w3 * Z.w;
+                                    // StoreSample (bucketprocessor.cpp:1471-1569): reject when occlZ <= D
+                                    hit = !(occlZ[k] <= D);
+                                    hid = __float_as_uint(r8.w);
+                                    hgrid = __float_as_uint(r8.z);
+                                }
                             }
-                            if ((__float_as_uint(r8.y) & 1u) == 0) {
-                                ++nIn2;
-                                const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
-                                const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
-                                const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px_;
-                                const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py_;
-                                float det = M1x * M2y - M1y * M2x;
-                                if (det != 0) det = 1 / det;
-                                u = u - det * (bx * M2y - by * M2x);
-                                v = v - det * (-(bx * M1y - by * M1x));
-                            }
-                            // bilerp (bilinear.h:228-236)
-                            const float w0 = (1 - v) * (1 - u), w1 = (1 - v) * u, w2 = v * (1 - u), w3 = v * u;
-                            const float D = ((w0 * Z.x + w1 * Z.y) + w2 * Z.z) + w3 * Z.w;
-                            // StoreSample (bucketprocessor.cpp:1471-1569): reject when occlZ <= D
-                            if (occlZ[k] <= D) break;
-                            ++nHit;
-                            if ((__float_as_uint(r8.y) & 2u) != 0) {
-                                // opaque: the occluding hit
-                                occlZ[k] = D;
-                                win[k] = __float_as_uint(r8.w);
-                                wu[k] = u;
-                                wv[k] = v;
-                            } else {
-                                // transparent: a new entry of the sample's hit list
-                                const PrimDev& P = prims[grids[__float_as_int(r8.z)].prim];
-                                shade_hit(P, u, v, D, __float_as_uint(r8.w), hA, hB);
-                                app = true;
-                            }
-                        } while (false);
+                        }
+                        nHit += hit ? 1u : 0u;
+                        // opaque: the occluding hit
+                        const bool opq = hit && (hflags & 2u) != 0;
+                        occlZ[k] = opq ? D : occlZ[k];
+                        win[k] = opq ? hid : win[k];
+                        wu[k] = opq ? u : wu[k];
+                        wv[k] = opq ? v : wv[k];
+                        // transparent: a new entry of the sample's hit list
+                        const bool app = hit && (hflags & 2u) == 0;
                         // Append the wave's transparent hits (converged): entries come
                         // from 64-entry blocks of the frame's arena, one atomic per block.
                         const unsigned long long am = __ballot(app);
@@ -960,6 +963,8 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                                 const uint32_t slot =
                                     wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & ((1ull << lane) - 1ull));
                                 if (slot < fr->hitCapacity && nh[k] < RH_MAX_HITS - 1) {
+                                    float4 hA, hB;
+                                    shade_hit(prims[grids[hgrid].prim], u, v, D, hid, hA, hB);
                                     hitData[2 * (size_t)slot] = hA;
                                     hitData[2 * (size_t)slot + 1] = hB;
                                     hitNext[slot] = head[k];
