@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdint>
+#include <mutex>
 #include <cstring>
 #include <new>
 #include <string>
@@ -52,22 +54,27 @@ rh_status fail(rh_status code, const std::string& msg) {
             return fail(RH_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));   \
     } while (0)
 
+// A device buffer of a context, allocated in its stream's order from the
+// device's memory pool: growing or freeing one does not wait for other
+// contexts' work, and freed memory is reused without returning it to the
+// driver (keep_pool_memory).
 struct DBuf {
     void* p = nullptr;
     size_t cap = 0;
+    hipStream_t st = nullptr;             // the owning context's stream
     hipError_t ensure(size_t bytes) {
         if (bytes == 0) bytes = 16;
         if (bytes <= cap) return hipSuccess;
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, st);
         p = nullptr;
         cap = 0;
         size_t n = bytes + bytes / 8 + 256;
-        hipError_t e = hipMalloc(&p, n);
+        hipError_t e = hipMallocAsync(&p, n, st);
         if (e == hipSuccess) cap = n;
         return e;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, st);
         p = nullptr;
         cap = 0;
     }
@@ -121,15 +128,19 @@ struct rh_ctx {
     }
     FrameDev frame;                       // host copy of dFrame
     rh_stats stats;
-    ~rh_ctx() {
+    template <typename F> void each_buffer(F f) {
         DBuf* all[] = {&dFrame, &dBuckets, &dTasks, &dPos2d, &dPos1d, &dPix, &dDither, &dFilter, &dBoxMask,
                        &dGrids, &dParams,
                        &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dVtx, &dMprec, &dMpbound, &dLists,
                        &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dBucketRefs,
                        &dHitData, &dHitNext, &dHitCtl, &dCombine, &dFrontDepth, &dItemsStatic, &dItems,
                        &dItemCount, &dQueue, &dRows, &dSegBucket, &dSegCounts};
+        for (DBuf* b : all) f(*b);
+    }
+    ~rh_ctx() {
         dropGraph();
-        for (DBuf* b : all) b->release();
+        each_buffer([](DBuf& b) { b.release(); });
+        if (st) (void)hipStreamSynchronize(st);   // the frees are ordered and done
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
@@ -161,6 +172,25 @@ void rh_options_default(rh_options* o) {
 
 const char* rh_last_error(void) { return g_err.c_str(); }
 
+}  // extern "C"
+
+// The device's memory pool keeps freed memory for reuse instead of handing
+// it back to the driver at each synchronisation: contexts of different
+// frame shapes come and go in the serving loop.
+static void keep_pool_memory() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        hipMemPool_t pool;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+    });
+}
+
+extern "C" {
+
 rh_status rh_create(const rh_options* opt, rh_ctx** out) {
     if (!opt || !out) return fail(RH_ERR_INVALID, "null argument");
     *out = nullptr;
@@ -173,7 +203,9 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
     if (!rh::make_shape(*opt, c->shape, err)) { delete c; return fail(RH_ERR_INVALID, err); }
     rh::make_plan(c->shape, c->plan);
     const rh::Shape& s = c->shape;
+    keep_pool_memory();
     hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    c->each_buffer([c](DBuf& b) { b.st = c->st; });
     for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
     if (e != hipSuccess) { delete c; return fail(RH_ERR_DEVICE, std::string("stream/event creation: ") + hipGetErrorString(e)); }
 
@@ -479,11 +511,14 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
     return hipSuccess;
 }
 
-rh_status rh_render_device(rh_ctx* c) {
+// One frame on the resident inputs.  The batched form replays its frame
+// graph unless `eager` asks for the same launches issued one by one with a
+// timestamp between the stages.
+static rh_status render_frame(rh_ctx* c, bool eager) {
     if (!c) return fail(RH_ERR_INVALID, "null context");
     if (!c->scheduled) return fail(RH_ERR_STATE, "rh_render must run before rh_render_device");
     const rh::Schedule& S = c->sched;
-    const bool graphed = c->tuning.mode == RH_MODE_BATCHED;
+    const bool graphed = c->tuning.mode == RH_MODE_BATCHED && !eager;
     unsigned long long cnt[RH_NCOUNTERS];
     for (int attempt = 0;; ++attempt) {
         if (graphed) {
@@ -551,6 +586,10 @@ rh_status rh_render_device(rh_ctx* c) {
     c->rendered = true;
     return RH_OK;
 }
+
+rh_status rh_render_device(rh_ctx* c) { return render_frame(c, false); }
+
+rh_status rh_render_stages(rh_ctx* c) { return render_frame(c, true); }
 
 rh_status rh_set_tuning(rh_ctx* c, const rh_tuning* t) {
     if (!c || !t) return fail(RH_ERR_INVALID, "null argument");
@@ -717,7 +756,8 @@ static uint64_t footprint_bytes(const rh::Shape& s, const rh::Schedule& S, const
         const uint64_t nsamp = (uint64_t)s.extW * s.extH * s.ns;
         bytes += arena_guess(s, S, prims) * 36 + nsamp * 20;
     }
-    return bytes;
+    // device buffers are allocated with 1/8 slack (DBuf::ensure)
+    return bytes + bytes / 8 + 64 * 256;
 }
 
 // Used by the serving loop: the split/dice schedule of a job, computed once

@@ -2,10 +2,12 @@
 // concurrently on one GPU.  Each running job owns a hider context (its own
 // HIP stream), so jobs overlap on the device; contexts stay resident after a
 // job, keyed by frame shape, so a later job of the same shape reuses the
-// sample plan and device buffers.  A job is admitted, in arrival order, when
-// a slot is free and the device memory of all resident contexts plus its
-// estimated footprint fits the budget (idle contexts are evicted oldest
-// first to make room).
+// sample plan and device buffers.  Submission only copies the inputs: a pool
+// of scheduler threads computes each job's split/dice schedule (which fixes
+// its device footprint), and a job is admitted, in the order its schedule
+// completed, when a slot is free and the device memory of all resident
+// contexts plus its footprint fits the budget (idle contexts are evicted
+// oldest first to make room).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,6 +21,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/aqsis_hider.h"
@@ -78,19 +81,89 @@ struct Job {
 struct rh_server {
     rh_server_options opt;
     std::mutex m;
-    std::condition_variable cvWork, cvDone;
-    std::deque<Job*> queue;
+    std::condition_variable cvWork, cvDone, cvSched;
+    std::deque<Job*> schedQueue;          // submitted, schedule not yet computed
+    std::deque<Job*> queue;               // scheduled, waiting for admission
     std::map<uint64_t, std::unique_ptr<Job>> jobs;
     std::list<Resident> resident;
     uint64_t nextId = 1;
     uint64_t maxBytes = 0;                // high-water mark of resident device memory
     bool stop = false;
-    std::vector<std::thread> workers;
+    int threadsPerSchedule = 1;
+    std::vector<std::thread> workers, schedulers;
 
-    uint64_t residentBytes() const {
+    // Under the lock: a job ends without running.
+    void finish_unrun(Job* j, int32_t status, const std::string& msg) {
+        j->status = status;
+        j->error = msg;
+        j->state = 2;
+        j->tStart = j->tEnd = Clock::now();
+        j->prims.clear();
+        j->prims.shrink_to_fit();
+    }
+
+    void scheduler() {
+        for (;;) {
+            Job* j = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cvSched.wait(lk, [&] { return stop || !schedQueue.empty(); });
+                if (stop) {
+                    while (!schedQueue.empty()) {
+                        finish_unrun(schedQueue.front(), RH_ERR_STATE, "server stopped");
+                        schedQueue.pop_front();
+                    }
+                    cvDone.notify_all();
+                    return;
+                }
+                j = schedQueue.front();
+                schedQueue.pop_front();
+            }
+            std::string err;
+            const bool ok = rh_internal_schedule(j->opt, j->prims.data(), (int32_t)j->prims.size(),
+                                                 threadsPerSchedule, j->sched, j->estimate, j->msSchedule, err);
+            {
+                std::lock_guard<std::mutex> lk(m);
+                if (!ok) {
+                    finish_unrun(j, RH_ERR_INVALID, err);
+                } else if (j->estimate > opt.memory_budget) {
+                    finish_unrun(j, RH_ERR_CAPACITY, "job needs " + std::to_string(j->estimate) +
+                                                         " device bytes, more than the server's budget");
+                } else {
+                    queue.push_back(j);
+                }
+            }
+            cvDone.notify_all();
+            cvWork.notify_all();
+        }
+    }
+
+    // Evicted contexts wait here to be destroyed outside the lock (freeing
+    // device memory can take a while); their bytes count until then.
+    std::vector<std::pair<rh_ctx*, uint64_t>> graveyard;
+    uint64_t pendingBytes = 0;
+
+    uint64_t liveBytes() const {
         uint64_t t = 0;
         for (const auto& r : resident) t += r.bytes;
         return t;
+    }
+    uint64_t residentBytes() const { return liveBytes() + pendingBytes; }
+
+    // Not under the lock: destroy what was evicted.
+    void bury() {
+        std::vector<std::pair<rh_ctx*, uint64_t>> dead;
+        {
+            std::lock_guard<std::mutex> lk(m);
+            dead.swap(graveyard);
+        }
+        if (dead.empty()) return;
+        for (auto& d : dead) rh_destroy(d.first);
+        {
+            std::lock_guard<std::mutex> lk(m);
+            for (auto& d : dead) pendingBytes -= d.second;
+        }
+        cvWork.notify_all();
     }
 
     // Under the lock: pick a context for the head job if it can be admitted.
@@ -103,10 +176,12 @@ struct rh_server {
                 if (residentBytes() + extra > opt.memory_budget && !evict(extra, &*it)) return false;
                 it->busy = true;
                 it->bytes = std::max(it->bytes, j->estimate);
+                maxBytes = std::max(maxBytes, residentBytes());
                 slot = it;
                 return true;
             }
         if (residentBytes() + j->estimate > opt.memory_budget && !evict(j->estimate, nullptr)) return false;
+        maxBytes = std::max(maxBytes, residentBytes() + j->estimate);
         Resident r;
         r.key = key;
         r.bytes = j->estimate;
@@ -116,18 +191,22 @@ struct rh_server {
         return true;
     }
 
-    // Evict idle contexts, oldest first, until `need` more bytes fit.
+    // Evict idle contexts, oldest first, until `need` more bytes fit once
+    // the evicted ones are destroyed; true if they fit now.
     bool evict(uint64_t need, const Resident* keep) {
-        while (residentBytes() + need > opt.memory_budget) {
+        while (liveBytes() + need > opt.memory_budget) {
             auto victim = resident.end();
             for (auto it = resident.begin(); it != resident.end(); ++it)
                 if (!it->busy && &*it != keep && (victim == resident.end() || it->lastUse < victim->lastUse))
                     victim = it;
             if (victim == resident.end()) return false;
-            if (victim->ctx) rh_destroy(victim->ctx);
+            if (victim->ctx) {
+                graveyard.emplace_back(victim->ctx, victim->bytes);
+                pendingBytes += victim->bytes;
+            }
             resident.erase(victim);
         }
-        return true;
+        return residentBytes() + need <= opt.memory_budget;
     }
 
     void run(Job* j, Resident& r) {
@@ -157,17 +236,23 @@ struct rh_server {
             std::list<Resident>::iterator slot;
             {
                 std::unique_lock<std::mutex> lk(m);
-                cvWork.wait(lk, [&] { return stop || (!queue.empty() && admit(queue.front(), slot)); });
+                for (;;) {
+                    if (stop || (!queue.empty() && admit(queue.front(), slot))) break;
+                    if (!graveyard.empty()) {
+                        // room is being made: destroy the evicted contexts, retry
+                        lk.unlock();
+                        bury();
+                        lk.lock();
+                        continue;
+                    }
+                    cvWork.wait(lk);
+                }
                 if (stop && queue.empty()) return;
                 if (stop) {
                     // drain: fail what was never admitted
                     while (!queue.empty()) {
-                        Job* q = queue.front();
+                        finish_unrun(queue.front(), RH_ERR_STATE, "server stopped");
                         queue.pop_front();
-                        q->status = RH_ERR_STATE;
-                        q->error = "server stopped";
-                        q->state = 2;
-                        q->tStart = q->tEnd = Clock::now();
                     }
                     cvDone.notify_all();
                     return;
@@ -178,23 +263,32 @@ struct rh_server {
                 j->tStart = Clock::now();
             }
             cvWork.notify_all();          // the next job may fit as well
+            bury();
             run(j, *slot);
             {
                 std::lock_guard<std::mutex> lk(m);
                 if (j->status != RH_OK || !slot->ctx) {
-                    if (slot->ctx) rh_destroy(slot->ctx);
+                    if (slot->ctx) {
+                        graveyard.emplace_back(slot->ctx, slot->bytes);
+                        pendingBytes += slot->bytes;
+                    }
                     resident.erase(slot);
                 } else {
-                    slot->bytes = std::max(slot->bytes, rh_internal_device_bytes(slot->ctx));
+                    slot->bytes = rh_internal_device_bytes(slot->ctx);
                     slot->busy = false;
                     slot->lastUse = Clock::now();
                 }
                 maxBytes = std::max(maxBytes, residentBytes());
+                // a frame may have grown past its estimate (a transparent-hit
+                // arena redone at its exact size): back within the budget,
+                // dropping idle contexts oldest first
+                evict(0, nullptr);
                 j->tEnd = Clock::now();
                 j->state = 2;
             }
             cvDone.notify_all();
             cvWork.notify_all();
+            bury();
         }
     }
 
@@ -228,7 +322,11 @@ rh_status rh_server_create(const rh_server_options* opt, rh_server** out) {
     rh_server* s = new (std::nothrow) rh_server();
     if (!s) return sfail(RH_ERR_NOMEM, "out of host memory");
     s->opt = *opt;
-    if (s->opt.schedule_threads <= 0) s->opt.schedule_threads = std::max(1, 16 / s->opt.slots);
+    if (s->opt.schedule_threads <= 0) s->opt.schedule_threads = 16;
+    // schedulers of about four threads each
+    const int nsched = std::max(1, s->opt.schedule_threads / 4);
+    s->threadsPerSchedule = std::max(1, s->opt.schedule_threads / nsched);
+    for (int i = 0; i < nsched; ++i) s->schedulers.emplace_back([s] { s->scheduler(); });
     for (int i = 0; i < s->opt.slots; ++i) s->workers.emplace_back([s] { s->worker(); });
     *out = s;
     return RH_OK;
@@ -239,23 +337,16 @@ rh_status rh_server_submit(rh_server* s, const rh_options* opt, const rh_prim* p
     std::unique_ptr<Job> j(new Job());
     j->opt = *opt;
     j->prims.assign(prims, prims + n);
-    // the split/dice schedule, on the submitting thread, fixes the footprint
-    std::string err;
-    if (!rh_internal_schedule(*opt, prims, n, s->opt.schedule_threads, j->sched, j->estimate, j->msSchedule, err))
-        return sfail(RH_ERR_INVALID, err);
-    if (j->estimate > s->opt.memory_budget)
-        return sfail(RH_ERR_CAPACITY, "job needs " + std::to_string(j->estimate) +
-                                          " device bytes, more than the server's budget");
     j->tSubmit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(s->m);
         if (s->stop) return sfail(RH_ERR_STATE, "server stopped");
         j->id = s->nextId++;
         *id = j->id;
-        s->queue.push_back(j.get());
+        s->schedQueue.push_back(j.get());
         s->jobs[j->id] = std::move(j);
     }
-    s->cvWork.notify_all();
+    s->cvSched.notify_one();
     return RH_OK;
 }
 
@@ -326,7 +417,10 @@ void rh_server_destroy(rh_server* s) {
         s->stop = true;
     }
     s->cvWork.notify_all();
+    s->cvSched.notify_all();
+    for (auto& t : s->schedulers) t.join();
     for (auto& t : s->workers) t.join();
+    s->bury();
     for (auto& r : s->resident)
         if (r.ctx) rh_destroy(r.ctx);
     delete s;

@@ -137,37 +137,50 @@ def cpu_baseline(opts, prims):
 def serving(args):
     """The serving form: a Poisson stream of render requests of mixed side
     (256..2048), PixelSamples (2x2, 4x4, 8x8), patch count (100..20,000) and
-    opacity (0.2..1) submitted to one rh_server as they arrive (a pool of
-    client threads computes each job's split/dice schedule at submission);
-    jobs run concurrently on the GPU, admitted by device-memory footprint.
-    Latency is client-side, arrival to result."""
+    opacity (0.2..1) submitted to one rh_server as they arrive; the server's
+    scheduler threads compute each job's split/dice schedule, and jobs run
+    concurrently on the GPU, admitted by device-memory footprint.  Latency is
+    client-side, arrival to result."""
     import threading
-    from concurrent.futures import ThreadPoolExecutor
     from aqsis_amd import scenes
     from aqsis_amd.hider import Server
     reqs = list(scenes.serving_requests(seed=1, count=args.requests, rate_hz=args.rate))
     lat, infos, errs = [None] * len(reqs), [None] * len(reqs), []
     with Server(slots=args.slots, memory_budget=args.budget_gb << 30) as srv:
+        # One thread submits each request at its arrival time (submission
+        # returns at once); this thread collects finished jobs by polling.
+        ids = {}
+        lock = threading.Lock()
         t_start = time.perf_counter()
 
-        def client(k):
-            opts, prims, arrival = reqs[k]
-            delay = t_start + arrival - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            t_arr = t_start + arrival          # latency counts from the scheduled arrival
-            try:
+        def arrivals():
+            for k, (opts, prims, arrival) in enumerate(reqs):
+                delay = t_start + arrival - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
                 jid = srv.submit(opts, prims)
-                info = srv.wait(jid)
-                srv.release(jid)
-            except Exception as e:      # a refused or failed job is reported, not fatal
-                errs.append(str(e))
-                return
-            lat[k] = (time.perf_counter() - t_arr) * 1e3
-            infos[k] = info
+                with lock:
+                    ids[jid] = k
 
-        with ThreadPoolExecutor(max_workers=args.clients) as pool:
-            list(pool.map(client, range(len(reqs))))
+        sub = threading.Thread(target=arrivals)
+        sub.start()
+        finished = 0
+        while finished < len(reqs):
+            got = srv.poll()
+            now = time.perf_counter()
+            for info in got:
+                with lock:
+                    k = ids[info["id"]]
+                finished += 1
+                if info["status"] != 0:
+                    errs.append(f"job {k}: status {info['status']}")
+                else:
+                    lat[k] = (now - (t_start + reqs[k][2])) * 1e3   # from the scheduled arrival
+                    infos[k] = info
+                srv.release(info["id"])
+            if not got:
+                time.sleep(0.001)
+        sub.join()
         wall = time.perf_counter() - t_start
         resident, high = srv.memory()
     done = [i for i in range(len(reqs)) if infos[i] is not None]
@@ -175,6 +188,7 @@ def serving(args):
     tests = sum(infos[i]["spl_count"] for i in done)
     dev = sum(infos[i]["ms_device"] for i in done)
     pct = lambda q: ls[min(len(ls) - 1, int(q * len(ls)))] if ls else None
+    mean = lambda v: sum(v) / len(v) if v else None
     return {
         "metric": "micropolygon-sample tests/s",
         "value": tests / wall,
@@ -191,11 +205,17 @@ def serving(args):
         "config": {"workload": "serving: Poisson stream of render requests, side in {256,512,1024,2048}, "
                                "PixelSamples in {2x2,4x4,8x8}, 100-20,000 bicubic patches, Os in [0.2,1], "
                                "gaussian 2x2", "requests": len(reqs), "rate_hz": args.rate, "slots": args.slots,
-                   "memory_budget_gb": args.budget_gb, "client_threads": args.clients},
+                   "memory_budget_gb": args.budget_gb},
         "frames_per_s": len(done) / wall,
         "latency_ms_p50": pct(0.5),
         "latency_ms_p99": pct(0.99),
         "device_busy_fraction": dev / (wall * 1e3),
+        # where a request's time goes (means over the finished ones): the
+        # split/dice schedule and the wait for admission, the run (context
+        # set-up, upload, frame, read-back) and the device frame
+        "mean_ms": {"queue_incl_schedule": mean([infos[i]["ms_queue"] for i in done]),
+                    "run": mean([infos[i]["ms_run"] for i in done]),
+                    "device": mean([infos[i]["ms_device"] for i in done])},
         "device_memory_high_water_gb": high / 2 ** 30,
         "failed": len(errs),
         "errors": errs[:5],
@@ -217,7 +237,6 @@ def main():
     ap.add_argument("--rate", type=float, default=40.0, help="serving: Poisson arrival rate (1/s)")
     ap.add_argument("--slots", type=int, default=4, help="serving: jobs in flight")
     ap.add_argument("--budget-gb", type=int, default=160, help="serving: device memory budget")
-    ap.add_argument("--clients", type=int, default=32, help="serving: client threads submitting jobs")
     ap.add_argument("--copy-kernel", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -270,11 +289,18 @@ def main():
     tests_per_s = st["spl_count"] * args.steps / wall
     dev_sorted = sorted(dev_ms)
     ops = algorithmic_ops(st)
+    if tuning["mode"] == 2:
+        # The batched form replays a frame graph, which carries no stage
+        # timestamps: the same launches issued one by one give the stage times.
+        dice_ms, bin_ms, sample_ms, filt_ms = [], [], [], []
+        for _ in range(3):
+            h.render_stages()
+            sst = h.stats()
+            dice_ms.append(sst["ms_dice"]); bin_ms.append(sst["ms_bin"])
+            sample_ms.append(sst["ms_sample"]); filt_ms.append(sst["ms_filter"])
     sample_avg_ms = sum(sample_ms) / len(sample_ms)
-    # rh_sample's launch time in the timed frames (HIP events on its stream;
-    # in the batched form they are nodes of the frame graph), else the static
-    # grid's first frame
-    roof_ms = sample_avg_ms if sample_avg_ms > 0 else grid_stages["sample"]
+    # rh_sample's launch time: HIP events on its stream around it
+    roof_ms = sample_avg_ms
     achieved_tops = ops / (roof_ms * 1e-3) / 1e12
     out = {
         "metric": "micropolygon-sample tests/s",

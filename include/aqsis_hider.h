@@ -187,6 +187,12 @@ rh_status rh_render(rh_ctx* ctx);
  * grids and sample plan (dice -> ... -> quantise), for timing. */
 rh_status rh_render_device(rh_ctx* ctx);
 
+/* rh_render_device with the kernels issued one by one (also in the batched
+ * form, which otherwise replays a frame graph) and a device timestamp
+ * between the stages, filling the per-stage times of rh_stats.  Same result
+ * and counters.  No counterpart in the reference. */
+rh_status rh_render_stages(rh_ctx* ctx);
+
 /* Device layout of the following frames (no counterpart in the reference:
  * it has one CPU bucket loop).  RH_ERR_INVALID for values outside those
  * listed in rh_tuning. */
@@ -256,7 +262,7 @@ const char* rh_last_error(void);
  */
 typedef struct rh_server_options {
     int32_t slots;             /* jobs in flight at once (1..64)                */
-    int32_t schedule_threads;  /* host threads per job schedule (0: 16/slots)   */
+    int32_t schedule_threads;  /* host threads for job schedules in all (0: 16) */
     uint64_t memory_budget;    /* device bytes for resident contexts            */
 } rh_server_options;
 
@@ -276,8 +282,9 @@ typedef struct rh_job_info {
 typedef struct rh_server rh_server;
 
 rh_status rh_server_create(const rh_server_options* opt, rh_server** out);
-/* Queue a job; inputs are copied.  RH_ERR_CAPACITY if it alone exceeds the
- * budget. */
+/* Queue a job and return at once; inputs are copied.  Its split/dice
+ * schedule is computed by the server's scheduler threads; a job that alone
+ * exceeds the budget ends with RH_ERR_CAPACITY (reported by wait / poll). */
 rh_status rh_server_submit(rh_server* srv, const rh_options* opt, const rh_prim* prims, int32_t n,
                            uint64_t* id);
 /* Block until job `id` is done; the job's own status is returned. */

@@ -50,6 +50,10 @@ def test_layouts_give_identical_frames(case):
                         h.render_device()
                         _same(ref, _frame(h))
                         h.set_tuning(mode="batched", lds_records=lds, blocks_per_wave=bpw, split_cost=split)
+                    h.render_stages()              # the same launches, one by one
+                    _same(ref, _frame(h))
+                    st = h.stats()
+                    assert st["ms_sample"] > 0 and st["ms_dice"] > 0 and st["ms_filter"] > 0
 
 
 def test_autotune_keeps_results():

@@ -31,7 +31,7 @@ def _alone(opts, prims):
         return h.channels(), h.rgba8()
 
 
-@pytest.mark.parametrize("slots,budget", [(4, 64 << 30), (3, 600 << 20)])
+@pytest.mark.parametrize("slots,budget", [(4, 64 << 30), (3, 800 << 20)])
 def test_results_same_alone_and_in_the_mix(slots, budget):
     from aqsis_amd.hider import Server
     jobs = _jobs()
@@ -47,12 +47,13 @@ def test_results_same_alone_and_in_the_mix(slots, budget):
             assert np.array_equal(rgba, ref[k][1]), k
             srv.release(jid)
         resident, high = srv.memory()
-        assert resident <= high
+        assert resident <= high and resident <= budget
 
 
 def test_job_over_budget_is_refused():
     from aqsis_amd.hider import HiderError, Server
     opts, prims = CASES["patches_small"]()
     with Server(slots=1, memory_budget=1 << 20) as srv:
+        jid = srv.submit(opts, prims)          # returns at once
         with pytest.raises(HiderError, match="budget"):
-            srv.submit(opts, prims)
+            srv.wait(jid)
