@@ -65,6 +65,7 @@ class RhPrim(C.Structure):
         ("sphere", C.c_float * 4),
         ("xform", (C.c_float * 16) * 2),
         ("P", (C.c_float * 3) * 16),
+        ("P1", (C.c_float * 3) * 16),
     ]
 
 

@@ -151,6 +151,34 @@ def patches_scene(seed=1, n=10000, res=(1920, 1080), fov=45.0, znear=8.0,
     return opts, prims
 
 
+def motion_scene(seed=1, n=600, res=(256, 256), fov=45.0, znear=8.0, zfar=40.0, size_px=28.0,
+                 jitter=0.1, samples=(8, 8), filt=("gaussian", 2.0, 2.0), opacity=(0.2, 0.8),
+                 shift_sigma=0.05, shutter=(0.0, 1.0), key_times=(0.0, 1.0), shading_rate=1.0):
+    """Transparency and motion blur: `n` bicubic patches as patches_scene,
+    each a deforming surface whose second key (RiMotionBegin [t0 t1] with the
+    patch twice) is the first moved in the image plane by N(0, shift_sigma *
+    width) raster pixels per axis, shutter open over the keys.  Os uniform in
+    `opacity`."""
+    rng = np.random.default_rng(seed)
+    opts, prims = patches_scene(seed=seed, n=n, res=res, fov=fov, znear=znear, zfar=zfar, size_px=size_px,
+                                jitter=jitter, samples=samples, filt=filt, opacity=opacity,
+                                shading_rate=shading_rate)
+    opts.shutter_open, opts.shutter_close = float(shutter[0]), float(shutter[1])
+    aspect = res[0] / res[1]
+    sx = math.tan(math.radians(fov) / 2.0) * max(aspect, 1.0)
+    px_per_unit_at_1 = (res[0] / 2.0) / sx
+    for p in prims:
+        P = np.array([list(p.P[i]) for i in range(16)], dtype=np.float64)
+        z = float(P[:, 2].mean())
+        d = rng.normal(0.0, shift_sigma * res[0], 2) * z / px_per_unit_at_1   # raster px -> camera units
+        P1 = P + np.array([d[0], -d[1], 0.0])
+        p.nkeys = 2
+        p.key_time[:] = [float(key_times[0]), float(key_times[1])]
+        for i in range(16):
+            p.P1[i][:] = [float(v) for v in np.float32(P1[i])]
+    return opts, prims
+
+
 def _prim_records(n):
     """A zeroed ctypes array of n RhPrim and a numpy view of its bytes."""
     arr = (RhPrim * n)()

@@ -94,8 +94,10 @@ typedef struct rh_options {
  *           RenderMan's row-vector convention (translation in row 3).
  * Bicubic:  P[0..15] are the Bezier control points, row-major in (v,u).
  * Bilinear: P[0..3] are the corners (u,v) = (0,0) (1,0) (0,1) (1,1).
- * nkeys = 2 adds a second motion key: patches carry P in object space and
- * xform[k] maps object->camera at key_time[k]; spheres use xform[k].
+ * nkeys = 2 makes it a deforming surface (geometry given inside RiMotionBegin
+ * / RiMotionEnd: CqDeformingSurface, geometry/surface.h:617-720) with keys at
+ * key_time[0] < key_time[1]: patches have key 1's camera-space points in P1
+ * (same layout as P), spheres key 1's object->camera matrix in xform[1].
  */
 typedef struct rh_prim {
     int32_t type;
@@ -107,6 +109,7 @@ typedef struct rh_prim {
     float sphere[4];
     float xform[2][16];
     float P[16][3];
+    float P1[16][3];
 } rh_prim;
 
 /* Counters of one frame.  spl_* are the reference's own hider counters

@@ -85,6 +85,15 @@ def sample_positions(opts):
     return xy, d
 
 
+def sample_times(opts):
+    w, h, ew, eh, spp = frame_size(opts)
+    L = lib()
+    L.oracle_sample_times.argtypes = [C.POINTER(RhOptions), C.c_void_p]
+    t = np.zeros((eh, ew, spp), np.float32)
+    L.oracle_sample_times(C.byref(opts), _ptr(t))
+    return t
+
+
 def dice(prim, u, v):
     L = lib()
     L.oracle_dice.argtypes = [C.POINTER(RhPrim), C.c_int32, C.c_int32, C.c_void_p]

@@ -12,6 +12,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <limits>
 #include <cstdint>
 #include <vector>
 #include <algorithm>
@@ -75,6 +76,17 @@ inline V3 vmax(V3 a, V3 b) { return v3(tmax(a.x, b.x), tmax(a.y, b.y), tmax(a.z,
 // 2D helpers: vector2d.h:372-385
 inline float cross2(V2 a, V2 b) { return a.x * b.y - a.y * b.x; }
 inline float maxNorm(V2 v) { return tmax(std::fabs(v.x), std::fabs(v.y)); }
+// CqVector2D::Magnitude2 (vector2d.h:132-138)
+inline float mag2_2d(float x, float y) {
+    if (y == 0.0) return x * x;
+    else if (x == 0.0) return y * y;
+    return (x * x) + (y * y);
+}
+// isClose (math.h:174-182)
+inline bool isClose(float x1, float x2, float tol = 10 * std::numeric_limits<float>::epsilon()) {
+    float d = std::fabs(x1 - x2);
+    return d <= tol * std::fabs(x1) || d <= tol * std::fabs(x2);
+}
 
 // ---------------------------------------------------------------------------
 // CqMatrix: include/aqsis/math/matrix.h, libs/math/matrix.cpp.  Element

@@ -15,6 +15,7 @@
 #include "refcore.h"
 
 #include <chrono>
+#include <climits>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -182,7 +183,26 @@ struct Surf {
     Mat matTx;
     // patches: camera-space control points
     V3 cp[16];
+    // Deformation blur (CqDeformingSurface, geometry/surface.h:617-720): the
+    // fields above are the deforming surface's own state plus its key-0
+    // surface; key 1's geometry follows.  The key surfaces keep their own
+    // m_fDiceable / m_fDiscard, which nothing clears.
+    bool motion = false;
+    float keyTime[2] = {0.0f, 0.0f};
+    V3 cp1[16];
+    Mat matTx1;
 };
+
+// The key-1 surface of a deforming surface.
+static Surf key1_of(const Surf& s) {
+    Surf k = s;
+    k.motion = false;
+    for (int i = 0; i < 16; ++i) k.cp[i] = s.cp1[i];
+    k.matTx = s.matTx1;
+    k.fDiceable = true;
+    k.fDiscard = false;
+    return k;
+}
 
 struct Grid {
     int cu = 0, cv = 0;
@@ -199,6 +219,15 @@ struct MP {
     int indexCode = 0;
     Bound bound;
     int refs = 0;
+    // CqMicroPolygonMotion (micropolygon.h:872-965): two keys of the four
+    // raster vertices (grid order 0, 1, cu+1, cu+2), their times and bounds,
+    // and the sub-bound list built on first use.
+    bool motion = false;
+    V3 K[2][4];
+    float T[2] = {0.0f, 0.0f};
+    Bound keyBound[2];
+    std::vector<Bound> subBound;
+    std::vector<float> subTime;
 };
 static const int Degeneracy_Mask = 0x8000000;  // micropolygon.h:791
 
@@ -325,9 +354,23 @@ static Bound patch_bound(const Surf& s, int n) {
     return r;
 }
 
-static Bound surf_bound(const Surf& s) {
+static Bound surf_bound_static(const Surf& s) {
     if (s.type == RH_PRIM_SPHERE) return sphere_bound(s);
     return patch_bound(s, s.type == RH_PRIM_BICUBIC ? 16 : 4);
+}
+
+// CqDeformingSurface::Bound (surface.h:629-640): the keys' bounds encapsulated.
+static Bound surf_bound(const Surf& s) {
+    if (!s.motion) return surf_bound_static(s);
+    Bound b;
+    b.mn = v3(FLT_MAX, FLT_MAX, FLT_MAX);
+    b.mx = v3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+    Bound k0 = surf_bound_static(s), k1 = surf_bound_static(key1_of(s));
+    for (const Bound* B : {&k0, &k1}) {
+        b.mx = v3(tmax(b.mx.x, B->mx.x), tmax(b.mx.y, B->mx.y), tmax(b.mx.z, B->mx.z));
+        b.mn = v3(tmin(b.mn.x, B->mn.x), tmin(b.mn.y, B->mn.y), tmin(b.mn.z, B->mn.z));
+    }
+    return b;
 }
 
 // sinCosGrid (quadrics.cpp:1493-1518): float endpoints, float step, double
@@ -460,12 +503,26 @@ static bool bilinear_diceable(Surf& s, const Mat& matCtoR, const rh_prim& prim) 
     return true;
 }
 
-static bool surf_diceable(Surf& s, const Mat& dc, const rh_prim& prim) {
+static bool surf_diceable_key(Surf& s, const Mat& dc, const rh_prim& prim) {
     switch (s.type) {
         case RH_PRIM_SPHERE: return sphere_diceable(s, dc, prim);
         case RH_PRIM_BICUBIC: return bicubic_diceable(s, dc, prim);
         default: return bilinear_diceable(s, dc, prim);
     }
+}
+
+// CqDeformingSurface::Diceable (surface.h:698-707): key 0 decides (with its
+// own m_fDiceable) and its dice sizes and split direction are copied to the
+// other keys (CopySplitInfo, surface.h:163-168); a discard decided by key 0
+// stays with key 0.
+static bool surf_diceable(Surf& s, const Mat& dc, const rh_prim& prim) {
+    if (!s.motion) return surf_diceable_key(s, dc, prim);
+    const bool outerDiceable = s.fDiceable, outerDiscard = s.fDiscard;
+    s.fDiceable = true;
+    bool f = surf_diceable_key(s, dc, prim);
+    s.fDiceable = outerDiceable;
+    s.fDiscard = outerDiscard;
+    return f;
 }
 
 // bicubicPatchNatSubdiv (patch.cpp:87-129) on true (dehomogenised) values;
@@ -514,7 +571,13 @@ static void bilinear_subdiv(const V3* P, V3* R1, V3* R2, bool u) {
 
 // CqSurface::Split (surface.cpp:529-577) / CqSurfacePatchBilinear::Split
 // (patch.cpp:748-...) without phantom vertices.
+static void surf_split_motion(const Surf& s, std::vector<Surf*>& out);
+
 static void surf_split(const Surf& s, std::vector<Surf*>& out) {
+    if (s.motion) {
+        surf_split_motion(s, out);
+        return;
+    }
     Surf* a = new Surf(s);
     Surf* b = new Surf(s);
     bool u = s.splitDir == SplitDir_U;
@@ -551,6 +614,31 @@ static void surf_split(const Surf& s, std::vector<Surf*>& out) {
     }
     out.push_back(a);
     out.push_back(b);
+}
+
+// CqDeformingSurface::Split (surface.h:657-692): every key splits the same way
+// (their m_fDiceable is never cleared, so never twice) and the halves are
+// paired into new deforming surfaces (m_fDiceable true, m_SplitCount + 1).
+static void surf_split_motion(const Surf& s, std::vector<Surf*>& out) {
+    Surf k0 = s;
+    k0.motion = false;
+    k0.fDiceable = true;
+    Surf k1 = key1_of(s);
+    std::vector<Surf*> h0, h1;
+    surf_split(k0, h0);
+    surf_split(k1, h1);
+    for (size_t i = 0; i < h0.size(); ++i) {
+        Surf* c = h0[i];
+        c->motion = true;
+        c->keyTime[0] = s.keyTime[0];
+        c->keyTime[1] = s.keyTime[1];
+        for (int k = 0; k < 16; ++k) c->cp1[k] = h1[i]->cp[k];
+        c->matTx1 = h1[i]->matTx;
+        c->fDiceable = true;
+        c->splitCount = s.splitCount + 1;
+        delete h1[i];
+        out.push_back(c);
+    }
 }
 
 // CqForwardDiffBezier (forwarddiff.h:35-103) over V3.
@@ -891,6 +979,10 @@ struct Hider {
     // RenderSurface: dice -> shade (constant) -> bust (bucketprocessor.cpp:942-1059,
     // micropolygon.cpp:345-527, 641-892).
     void dice_shade_bust(Surf* s) {
+        if (s->motion) {
+            dice_shade_bust_motion(s);
+            return;
+        }
         const rh_prim& p = prims[s->prim];
         Grid* g = new Grid();
         surf_dice(*s, *g);
@@ -933,6 +1025,75 @@ struct Hider {
             }
         release_grid(g);
     }
+    // CqDeformingSurface::Dice (surface.h:643-655) -> CqMotionMicroPolyGrid:
+    // every key diced with key 0's sizes; Shade(false) shades key 0 and never
+    // culls the grid (micropolygon.cpp:920-924, 493-520), and its Split
+    // busts every quad (no transparency cull) into a CqMicroPolygonMotion
+    // with one key per grid key (micropolygon.cpp:946-1156; two-sided
+    // surfaces, so no backface cull).
+    void dice_shade_bust_motion(Surf* s) {
+        const rh_prim& p = prims[s->prim];
+        Grid* g = new Grid();
+        surf_dice(*s, *g);
+        Grid g1;
+        surf_dice(key1_of(*s), g1);
+        st.grids++;
+        for (int k = 0; k < 3; ++k) { g->Oi[k] = p.Os[k]; g->Ci[k] = p.Os[k] * p.Cs[k]; }
+        const int cu = g->cu, cv = g->cv;
+        for (Grid* gk : {g, &g1})
+            for (int i = (int)gk->P.size() - 1; i >= 0; --i) {
+                float zdepth = gk->P[i].z;
+                V3 r = mat_vec3(F.camToRaster, gk->P[i]);
+                r.z = zdepth;
+                gk->P[i] = r;
+            }
+        g->mpBase = mpCounter;
+        g->refs = 1;
+        for (int iv = 0; iv < cv; ++iv)
+            for (int iu = 0; iu < cu; ++iu) {
+                int idx = iv * (cu + 1) + iu;
+                MP* mp = new MP();
+                mp->g = g;
+                g->refs++;
+                mp->index = idx;
+                mp->id = mpCounter++;
+                st.micropolygons++;
+                mp->motion = true;
+                const Grid* keys[2] = {g, &g1};
+                for (int k = 0; k < 2; ++k) {
+                    const std::vector<V3>& P = keys[k]->P;
+                    mp->K[k][0] = P[idx];
+                    mp->K[k][1] = P[idx + 1];
+                    mp->K[k][2] = P[idx + cu + 1];
+                    mp->K[k][3] = P[idx + cu + 2];
+                    mp->T[k] = s->keyTime[k];
+                    // CqMovingMicroPolygonKey::GetBound (micropolygon.cpp:1974-1989)
+                    const V3* q = mp->K[k];
+                    Bound& B = mp->keyBound[k];
+                    B.mn = v3(tmin(q[0].x, tmin(q[1].x, tmin(q[2].x, q[3].x))),
+                              tmin(q[0].y, tmin(q[1].y, tmin(q[2].y, q[3].y))),
+                              tmin(q[0].z, tmin(q[1].z, tmin(q[2].z, q[3].z))));
+                    B.mx = v3(tmax(q[0].x, tmax(q[1].x, tmax(q[2].x, q[3].x))),
+                              tmax(q[0].y, tmax(q[1].y, tmax(q[2].y, q[3].y))),
+                              tmax(q[0].z, tmax(q[1].z, tmax(q[2].z, q[3].z))));
+                }
+                // AppendKey (micropolygon.cpp:1952-1967): m_Bound is key 0's
+                // bound encapsulating key 1's
+                mp->bound = mp->keyBound[0];
+                const Bound& B1 = mp->keyBound[1];
+                mp->bound.mx = v3(tmax(mp->bound.mx.x, B1.mx.x), tmax(mp->bound.mx.y, B1.mx.y),
+                                  tmax(mp->bound.mx.z, B1.mx.z));
+                mp->bound.mn = v3(tmin(mp->bound.mn.x, B1.mn.x), tmin(mp->bound.mn.y, B1.mn.y),
+                                  tmin(mp->bound.mn.z, B1.mn.z));
+                // Initialise: the vertex order of the primary key (key 0)
+                mp->indexCode = vertex_order(*g, idx);
+                mp->refs = 1;
+                add_mpg(mp);
+                release_mp(mp);
+            }
+        release_grid(g);
+    }
+
     void release_grid(Grid* g) { if (--g->refs == 0) delete g; }
     void release_mp(MP* mp) {
         if (--mp->refs == 0) { release_grid(mp->g); delete mp; }
@@ -1083,6 +1244,148 @@ struct Hider {
             }
     }
 
+    // CqMicroPolygonMotion::BuildBoundList (micropolygon.cpp:1689-1756) for
+    // two keys: the number of sub-bounds is the micropolygon's raster move in
+    // its own widths, at most timeRanges; each sub-bound encapsulates the
+    // bounds interpolated at its two end times (extrapolated past the last
+    // key), and carries its start time.
+    void build_bound_list(MP* mp, int timeRanges) {
+        const float opentime = F.o.shutter_open, closetime = F.o.shutter_close;
+        const Bound& kb0 = mp->keyBound[0];
+        const Bound& kb1 = mp->keyBound[1];
+        const float polyLen2 = mag2_2d(kb0.mx.x - kb0.mn.x, kb0.mx.y - kb0.mn.y);
+        const float moveDist2 = mag2_2d(mp->K[0][0].x - mp->K[1][0].x, mp->K[0][0].y - mp->K[1][0].y);
+        const float lengths = std::sqrt(moveDist2 / polyLen2);
+        // lfloor of an infinite or NaN ratio is LONG_MIN on x86-64 (the
+        // conversion's "integer indefinite"), so max(1, .) gives 1
+        long moved = (lengths < 9.2e18f) ? lfloor(lengths) : LONG_MIN;
+        if (moved < 1) moved = 1;
+        const int divisions = (int)tmin<long>(moved, timeRanges);
+        const float dt = (closetime - opentime) / divisions;
+        float time = opentime + dt;
+        Bound bound = kb0;
+        mp->subBound.resize(divisions);
+        mp->subTime.resize(divisions);
+        for (int i = 0; i < divisions; ++i) {
+            const float mix = (time - mp->T[0]) / (mp->T[1] - mp->T[0]);
+            Bound mid = kb0;
+            mid.mn = mid.mn + mix * (kb1.mn - kb0.mn);
+            mid.mx = mid.mx + mix * (kb1.mx - kb0.mx);
+            bound.mx = v3(tmax(bound.mx.x, mid.mx.x), tmax(bound.mx.y, mid.mx.y), tmax(bound.mx.z, mid.mx.z));
+            bound.mn = v3(tmin(bound.mn.x, mid.mn.x), tmin(bound.mn.y, mid.mn.y), tmin(bound.mn.z, mid.mn.z));
+            mp->subBound[i] = bound;
+            mp->subTime[i] = time - dt;
+            bound = mid;
+            time += dt;
+        }
+    }
+
+    // CqMicroPolygonMotion::Sample (micropolygon.cpp:1768-1914) without DoF
+    // or triangle split lines: the tight bound and the vertices at the
+    // sample's time, then the static point-in-polygon test on them.
+    bool sample_motion(const MP* mp, V2 vecSample, float time, float& D, V2& uv, bool& linear) {
+        int iIndex = 0;
+        float Fraction = 0.0f;
+        bool Exact = true;
+        if (time > mp->T[0]) {
+            if (time >= mp->T[1]) {
+                iIndex = 1;
+            } else {
+                Fraction = (time - mp->T[0]) / (mp->T[1] - mp->T[0]);
+                Exact = (mp->T[0] == time);
+            }
+        }
+        Bound tight;
+        if (Exact) {
+            tight = mp->keyBound[iIndex];
+        } else {
+            const Bound& b1 = mp->keyBound[0];
+            const Bound& b2 = mp->keyBound[1];
+            tight.mn = (1 - Fraction) * b1.mn + Fraction * b2.mn;
+            tight.mx = (1 - Fraction) * b1.mx + Fraction * b2.mx;
+        }
+        if (vecSample.x < tight.mn.x || vecSample.x > tight.mx.x || vecSample.y < tight.mn.y ||
+            vecSample.y > tight.mx.y)
+            return false;
+        V3 pts[4];
+        if (Exact) {
+            for (int i = 0; i < 4; ++i) pts[i] = mp->K[iIndex][i];
+        } else {
+            const float F1 = 1.0f - Fraction;
+            for (int i = 0; i < 4; ++i) pts[i] = (F1 * mp->K[0][i]) + (Fraction * mp->K[1][i]);
+        }
+        HitCache hc;
+        cache_point_in_poly(hc, pts, mp->indexCode);
+        linear = hc.linear;
+        return contains(hc, vecSample, D, uv);
+    }
+
+    // RenderMPG_MBOrDof for a moving micropolygon without DoF
+    // (bucketprocessor.cpp:1221-1469).  A sub-bound visits the samples of its
+    // pixels whose index lies in the window its times give (sample i's time
+    // lies in the i-th of n equal parts of the shutter); every visit counts
+    // into SPL_count, and samples outside the sub-bound's times are skipped.
+    // An index window reaching past a pixel's samples (the reference reads
+    // beyond them) stops at the last sample.
+    void render_mp_motion(MP* mp) {
+        const Grid& g = *mp->g;
+        const bool isOpaque = g.Oi[0] >= 1.0f && g.Oi[1] >= 1.0f && g.Oi[2] >= 1.0f;
+        const float opentime = F.o.shutter_open, closetime = F.o.shutter_close;
+        const int numSamples = F.nsamp;
+        const bool fastShutter = isClose(closetime, opentime);
+        const float timePerSample = fastShutter ? 0.0f : numSamples / (closetime - opentime);
+        const int timeRanges = tmax(4, F.nx * F.ny);
+        if (mp->subBound.empty()) build_bound_list(mp, timeRanges);
+        const int nb = (int)mp->subBound.size();
+        for (int b = 0; b < nb; ++b) {
+            const Bound& Bd = mp->subBound[b];
+            const float time0 = mp->subTime[b];
+            const float time1 = (b != nb - 1) ? mp->subTime[b + 1] : closetime;
+            if (time1 < opentime || time0 > closetime) continue;
+            int indexT0, indexT1;
+            if (fastShutter) {
+                indexT0 = 0;
+                indexT1 = numSamples;
+            } else {
+                indexT0 = (int)tmax<long>(0, lfloor((time0 - opentime) * timePerSample));
+                indexT1 = (int)lceil((time1 - opentime) * timePerSample);
+            }
+            if (Bd.mn.z > F.o.clip_far || Bd.mx.z < F.o.clip_near) continue;
+            int eX = (int)lceil(Bd.mx.x), eY = (int)lceil(Bd.mx.y);
+            if (eX > sMaxX) eX = sMaxX;
+            if (eY > sMaxY) eY = sMaxY;
+            int sX = static_cast<int>(std::floor(Bd.mn.x)), sY = static_cast<int>(std::floor(Bd.mn.y));
+            if (sY < sMinY) sY = sMinY;
+            if (sX < sMinX) sX = sMinX;
+            if (sX >= eX || sY >= eY) continue;
+            for (int iY = sY; iY < eY; ++iY)
+                for (int iX = sX; iX < eX; ++iX) {
+                    int index = indexT0;
+                    do {
+                        if (index >= numSamples) break;
+                        Sample& sd = sample_at(iX, iY, index);
+                        const V2 vecP = sd.pos;
+                        const float time = sd.time;
+                        index++;
+                        st.spl_count++;
+                        if (time < time0 || time > time1) continue;
+                        if (vecP.x < Bd.mn.x || vecP.x > Bd.mx.x || vecP.y < Bd.mn.y || vecP.y > Bd.mx.y) continue;
+                        if (Bd.mn.z > sd.occlZ) continue;
+                        if (0.0f > sd.lod || sd.lod >= 1.0f) continue;
+                        st.spl_bound_hits++;
+                        float D;
+                        V2 uv;
+                        bool linear = false;
+                        if (sample_motion(mp, vecP, time, D, uv, linear)) {
+                            st.inside++;
+                            if (!linear) st.inside_nonlinear++;
+                            store_sample(mp, iX, iY, index - 1, D, uv, isOpaque);
+                        }
+                    } while (index < indexT1);
+                }
+        }
+    }
+
     // StoreSample (bucketprocessor.cpp:1471-1569) with min depth filter.
     void store_sample(MP* mp, int px, int py, int index, float D, V2 uv, bool isOpaque) {
         Sample& sd = sample_at(px, py, index);
@@ -1195,7 +1498,11 @@ struct Hider {
     }
 
     void render_waiting(Bucket& b) {
-        for (size_t i = 0; i < b.mps.size(); ++i) render_mp_static(b.mps[i]);
+        // RenderMicroPoly (bucketprocessor.cpp:1067-1091): no DoF here
+        for (size_t i = 0; i < b.mps.size(); ++i) {
+            if (b.mps[i]->motion) render_mp_motion(b.mps[i]);
+            else render_mp_static(b.mps[i]);
+        }
         for (MP* mp : b.mps) release_mp(mp);
         b.mps.clear();
         if (cull) tree.update();
@@ -1431,18 +1738,26 @@ int oracle_render(const rh_options* opt, const rh_prim* prims, int32_t nprims, i
             s->thetaMax = p.sphere[3];
             s->phiMin = (float)std::asin((double)(p.sphere[1] / s->radius));
             s->phiMax = (float)std::asin((double)(p.sphere[2] / s->radius));
-            Mat M;
-            bool ident = true;
-            for (int r = 0; r < 4; ++r)
-                for (int c = 0; c < 4; ++c) {
-                    M.m[r][c] = p.xform[0][4 * r + c];
-                    if (M.m[r][c] != (r == c ? 1.0f : 0.0f)) ident = false;
-                }
-            M.ident = ident;
-            s->matTx = M;
+            for (int key = 0; key < p.nkeys; ++key) {
+                Mat M;
+                bool ident = true;
+                for (int r = 0; r < 4; ++r)
+                    for (int c = 0; c < 4; ++c) {
+                        M.m[r][c] = p.xform[key][4 * r + c];
+                        if (M.m[r][c] != (r == c ? 1.0f : 0.0f)) ident = false;
+                    }
+                M.ident = ident;
+                (key == 0 ? s->matTx : s->matTx1) = M;
+            }
         } else {
             int n = p.type == RH_PRIM_BICUBIC ? 16 : 4;
             for (int k = 0; k < n; ++k) s->cp[k] = v3(p.P[k][0], p.P[k][1], p.P[k][2]);
+            for (int k = 0; k < n; ++k) s->cp1[k] = v3(p.P1[k][0], p.P1[k][1], p.P1[k][2]);
+        }
+        if (p.nkeys == 2) {
+            s->motion = true;
+            s->keyTime[0] = p.key_time[0];
+            s->keyTime[1] = p.key_time[1];
         }
         H->post_surface(s);
     }
@@ -1519,6 +1834,27 @@ int oracle_sample_positions(const rh_options* opt, float* xy, float* dither) {
     if (xy)
         for (size_t i = 0; i < H->samples.size(); ++i) { xy[2 * i] = H->samples[i].pos.x; xy[2 * i + 1] = H->samples[i].pos.y; }
     if (dither) memcpy(dither, H->ditherRec.data(), H->ditherRec.size() * sizeof(float));
+    return 0;
+}
+
+// Per-sample shutter times of the extended image (setSamples,
+// imagepixel.cpp:334-359), for tests.
+int oracle_sample_times(const rh_options* opt, float* times) {
+    std::unique_ptr<Hider> H(new Hider());
+    memset(&H->st, 0, sizeof(H->st));
+    setup_frame(*opt, H->F);
+    Frame& F = H->F;
+    H->prims = nullptr;
+    H->cull = false;
+    H->samples.resize((size_t)F.extW * F.extH * F.nsamp);
+    H->pixelHasValid.assign((size_t)F.extW * F.extH, 0);
+    const int W = F.cropXMax - F.cropXMin, Hh = F.cropYMax - F.cropYMin;
+    H->channels.assign((size_t)W * Hh * 9, 0.0f);
+    H->rgba8.assign((size_t)W * Hh * 4, 0);
+    H->ditherRec.assign((size_t)W * Hh, 0.0f);
+    H->set_image();
+    H->render();
+    for (size_t i = 0; i < H->samples.size(); ++i) times[i] = H->samples[i].time;
     return 0;
 }
 

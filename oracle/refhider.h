@@ -30,6 +30,8 @@ int oracle_render(const rh_options* opt, const rh_prim* prims, int32_t nprims,
  * passes (imagepixel.cpp:334-359), 2 floats per sample, and display 0's
  * dither draw per display pixel. */
 int oracle_sample_positions(const rh_options* opt, float* xy, float* dither);
+/* Shutter time of every sample of the extended image (ext pixels x spp). */
+int oracle_sample_times(const rh_options* opt, float* times);
 
 /* Camera-space grid points of one surface diced at u x v
  * (CqSurface::Dice): out[3*((v+1)*(u+1))]. */
