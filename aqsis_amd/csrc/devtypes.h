@@ -9,6 +9,10 @@
 #define RH_PRIM_BILINEAR 3
 #endif
 
+// GridDev::type flag of a deforming surface's grid (two motion keys): its
+// parameters hold key 0's then key 1's, then the key times.
+#define RH_GRID_MOTION 0x100
+
 // Largest grid: u*v <= 256 micropolygons gives at most 2 x 257 vertices.
 #define RH_MAX_GRID_VERTS 514
 #define RH_MAX_GRID_ROWS 257
@@ -24,6 +28,13 @@
 //           component-wise, color.h:432-435 (10)
 //  [18]     grid index (int bits)  [19] unused
 #define RH_MPREC_FLOATS 20
+// Motion-frame micropolygon record, in floats:
+//  [0..11]  key 0: x, y, z of the four vertices in grid order
+//  [12..23] key 1 (a static micropolygon repeats key 0)
+//  [24]     flags as above, plus RH_MB_MOVING  [25] grid index (int bits)
+//  [26..27] key times  [28..31] static bound min x, max x, min y, max y
+#define RH_MPREC_MB_FLOATS 32
+#define RH_MB_MOVING (1u << 11)
 // Device counters: 0..4 the hider's SPL counters (sample tests, bound hits,
 // stored hits, point-in-polygon hits, second Newton steps), 5 tasks whose
 // list overflowed, 6 micropolygon-bucket references, 7 transparent-hit
@@ -53,6 +64,8 @@ struct FrameDev {
     float qOne, qMin, qMax, qDither;
     float zThreshold[3];          // Option "limits" "zthreshold"
     uint32_t hitCapacity;         // transparent-hit arena entries
+    float shutterOpen, shutterClose;
+    float clipNear, clipFar;
 };
 
 struct GridDev {

@@ -348,9 +348,13 @@ struct Node {
     box3 rb;                    // cached raster bound
     int xminb = 0, yminb = 0, xmaxb = 0, ymaxb = 0;
     int kid0 = -1, nkids = 0;   // children in the same tree
-    // geometry
+    // geometry (key 0 of a deforming surface)
     float radius = 1, phiMin = 0, phiMax = 0, thetaMin = 0, thetaMax = 360;
     f3 cp[16];
+    // A deforming surface (CqDeformingSurface, geometry/surface.h:617-720):
+    // key 1's control points (spheres: key 1's matrix, Ctx::objToCam1).
+    bool motion = false;
+    f3 cp1[16];
 };
 
 // sinCosGrid (quadrics.cpp:1493-1518)
@@ -458,6 +462,7 @@ struct Ctx {
     const Shape* s;
     const rh_prim* prims;
     std::vector<m44> objToCam;   // sphere transforms
+    std::vector<m44> objToCam1;  // key 1 of moving spheres
 };
 
 // CqQuadric::EstimateGridSize + Diceable (quadrics.cpp:259-346)
@@ -573,6 +578,22 @@ void post_geometry(Node& n, const Ctx& c) {
     const rh_options& o = s.o;
     box3 B = (n.type == RH_PRIM_SPHERE) ? sphere_camera_bound(n, c.objToCam[n.prim])
                                         : patch_camera_bound(n, n.type == RH_PRIM_BICUBIC ? 16 : 4);
+    if (n.motion) {
+        // CqDeformingSurface::Bound (surface.h:629-640): the keys' bounds
+        // encapsulated into an empty box
+        Node k1 = n;
+        for (int i = 0; i < 16; ++i) k1.cp[i] = n.cp1[i];
+        box3 B1 = (n.type == RH_PRIM_SPHERE) ? sphere_camera_bound(n, c.objToCam1[n.prim])
+                                             : patch_camera_bound(k1, n.type == RH_PRIM_BICUBIC ? 16 : 4);
+        box3 u;
+        u.lo = mk(FLT_MAX, FLT_MAX, FLT_MAX);
+        u.hi = mk(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+        for (const box3* b : {&B, &B1}) {
+            u.hi = mk(mx(u.hi.x, b->hi.x), mx(u.hi.y, b->hi.y), mx(u.hi.z, b->hi.z));
+            u.lo = mk(mn(u.lo.x, b->lo.x), mn(u.lo.y, b->lo.y), mn(u.lo.z, b->lo.z));
+        }
+        B = u;
+    }
     if (B.lo.z >= o.clip_far || B.hi.z <= o.clip_near) { n.culled = true; return; }
     if (B.lo.z <= FLT_EPSILON) {
         n.diceableFlag = false;
@@ -611,7 +632,10 @@ void build_tree(const Ctx& c, int prim, std::vector<Node>& t) {
     } else {
         int k = p.type == RH_PRIM_BICUBIC ? 16 : 4;
         for (int i = 0; i < k; ++i) root.cp[i] = mk(p.P[i][0], p.P[i][1], p.P[i][2]);
+        if (p.nkeys == 2)
+            for (int i = 0; i < k; ++i) root.cp1[i] = mk(p.P1[i][0], p.P1[i][1], p.P1[i][2]);
     }
+    root.motion = p.nkeys == 2;
     t.clear();
     t.push_back(root);
     for (size_t i = 0; i < t.size(); ++i) {
@@ -619,9 +643,16 @@ void build_tree(const Ctx& c, int prim, std::vector<Node>& t) {
         if (t[i].culled) continue;
         Node& n = t[i];
         bool d;
+        // A deforming surface asks key 0 (CqDeformingSurface::Diceable,
+        // surface.h:698-707), whose own m_fDiceable is never cleared; a
+        // discard key 0 decides stays with the key, so the deforming surface
+        // splits instead.
+        const bool outerFlag = n.diceableFlag, outerDiscard = n.discard;
+        if (n.motion) n.diceableFlag = true;
         if (n.type == RH_PRIM_SPHERE) d = sphere_diceable(n, c);
         else if (n.type == RH_PRIM_BICUBIC) d = bicubic_diceable(n, c);
         else d = bilinear_diceable(n, c);
+        if (n.motion) { n.diceableFlag = outerFlag; n.discard = outerDiscard; }
         if (d) { n.leaf = true; continue; }
         if (n.discard) continue;
         // CqSurface::Split (surface.cpp:529-577): two halves; an undiceable
@@ -635,8 +666,10 @@ void build_tree(const Ctx& c, int prim, std::vector<Node>& t) {
                 if (u) { a.thetaMax = tc; b.thetaMin = tc; } else { a.phiMax = pc; b.phiMin = pc; }
             } else if (src.type == RH_PRIM_BICUBIC) {
                 split_bicubic(src.cp, a.cp, b.cp, u);
+                if (src.motion) split_bicubic(src.cp1, a.cp1, b.cp1, u);
             } else {
                 split_bilinear(src.cp, a.cp, b.cp, u);
+                if (src.motion) split_bilinear(src.cp1, a.cp1, b.cp1, u);
             }
             for (Node* k : {&a, &b}) {
                 k->splitDir = u ? DIR_V : DIR_U;
@@ -648,7 +681,9 @@ void build_tree(const Ctx& c, int prim, std::vector<Node>& t) {
         Node a, b;
         halves(t[i], a, b);
         std::vector<Node> kids;
-        if (!t[i].diceableFlag && t[i].type != RH_PRIM_BILINEAR) {
+        // (a deforming surface's keys split once: CqDeformingSurface::Split,
+        // surface.h:657-692)
+        if (!t[i].diceableFlag && t[i].type != RH_PRIM_BILINEAR && !t[i].motion) {
             Node a0, a1, b0, b1;
             halves(a, a0, a1);
             halves(b, b0, b1);
@@ -684,13 +719,18 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
     c.s = &s;
     c.prims = prims;
     c.objToCam.assign(n, m_identity());
+    c.objToCam1.assign(n, m_identity());
     for (int i = 0; i < n; ++i) {
         const rh_prim& p = prims[i];
         if (p.type != RH_PRIM_SPHERE && p.type != RH_PRIM_BICUBIC && p.type != RH_PRIM_BILINEAR) {
             err = "unknown surface type at index " + std::to_string(i);
             return false;
         }
-        if (p.nkeys != 1) { err = "motion keys are not supported by this build"; return false; }
+        if (p.nkeys != 1 && p.nkeys != 2) { err = "nkeys must be 1 or 2"; return false; }
+        if (p.nkeys == 2 && !(p.key_time[0] < p.key_time[1])) {
+            err = "motion key times must increase";
+            return false;
+        }
         if (!(p.shading_rate > 0)) { err = "shading rate must be positive"; return false; }
         if (p.type == RH_PRIM_SPHERE) {
             if (!(p.sphere[0] != 0)) { err = "sphere radius must be non-zero"; return false; }
@@ -711,6 +751,17 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
                 }
             M.id = ident;
             c.objToCam[i] = M;
+            if (p.nkeys == 2) {
+                m44 M1;
+                bool id1 = true;
+                for (int r = 0; r < 4; ++r)
+                    for (int cc = 0; cc < 4; ++cc) {
+                        M1.e[r][cc] = p.xform[1][4 * r + cc];
+                        if (M1.e[r][cc] != (r == cc ? 1.0f : 0.0f)) id1 = false;
+                    }
+                M1.id = id1;
+                c.objToCam1[i] = M1;
+            }
         }
     }
     // 1) Split trees, one per primitive, in parallel.
@@ -794,21 +845,33 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
                     return false;
                 }
                 mpCount += (uint32_t)(g.cu * g.cv);
-                vtxCount += (uint32_t)((g.cu + 1) * (g.cv + 1));
-                if (nd->type == RH_PRIM_SPHERE) {
-                    const m44& M = c.objToCam[nd->prim];
-                    P.push_back(nd->radius);
-                    for (int r = 0; r < 4; ++r)
-                        for (int cc = 0; cc < 4; ++cc) P.push_back(M.e[r][cc]);
-                    P.push_back(M.id ? 1.0f : 0.0f);
-                    size_t o0 = P.size();
-                    P.resize(o0 + 2 * (g.cu + 1) + 2 * (g.cv + 1));
-                    sincos_grid(deg2rad(nd->thetaMin), deg2rad(nd->thetaMax), g.cu + 1, &P[o0], &P[o0 + g.cu + 1]);
-                    sincos_grid(nd->phiMin, nd->phiMax, g.cv + 1, &P[o0 + 2 * (g.cu + 1)],
-                                &P[o0 + 2 * (g.cu + 1) + g.cv + 1]);
-                } else {
-                    int kk = nd->type == RH_PRIM_BICUBIC ? 16 : 4;
-                    for (int i = 0; i < kk; ++i) { P.push_back(nd->cp[i].x); P.push_back(nd->cp[i].y); P.push_back(nd->cp[i].z); }
+                vtxCount += (uint32_t)((g.cu + 1) * (g.cv + 1)) * (nd->motion ? 2u : 1u);
+                // parameters of each key in turn; a deforming surface's keys
+                // share the dice sizes (CopySplitInfo) and its key times follow
+                for (int key = 0; key < (nd->motion ? 2 : 1); ++key) {
+                    if (nd->type == RH_PRIM_SPHERE) {
+                        const m44& M = key ? c.objToCam1[nd->prim] : c.objToCam[nd->prim];
+                        P.push_back(nd->radius);
+                        for (int r = 0; r < 4; ++r)
+                            for (int cc = 0; cc < 4; ++cc) P.push_back(M.e[r][cc]);
+                        P.push_back(M.id ? 1.0f : 0.0f);
+                        size_t o0 = P.size();
+                        P.resize(o0 + 2 * (g.cu + 1) + 2 * (g.cv + 1));
+                        sincos_grid(deg2rad(nd->thetaMin), deg2rad(nd->thetaMax), g.cu + 1, &P[o0],
+                                    &P[o0 + g.cu + 1]);
+                        sincos_grid(nd->phiMin, nd->phiMax, g.cv + 1, &P[o0 + 2 * (g.cu + 1)],
+                                    &P[o0 + 2 * (g.cu + 1) + g.cv + 1]);
+                    } else {
+                        int kk = nd->type == RH_PRIM_BICUBIC ? 16 : 4;
+                        const f3* cp = key ? nd->cp1 : nd->cp;
+                        for (int i = 0; i < kk; ++i) { P.push_back(cp[i].x); P.push_back(cp[i].y); P.push_back(cp[i].z); }
+                    }
+                }
+                if (nd->motion) {
+                    g.type |= GRID_MOTION;
+                    P.push_back(c.prims[nd->prim].key_time[0]);
+                    P.push_back(c.prims[nd->prim].key_time[1]);
+                    out.motion = true;
                 }
                 out.grids.push_back(g);
             } else if (nd->nkids > 0) {

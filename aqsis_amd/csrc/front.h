@@ -60,6 +60,10 @@ struct Plan {
 
 void make_plan(const Shape& s, Plan& p);
 
+// GridRec::type flag of a deforming surface's grid: its parameters hold key
+// 0's then key 1's (same layout), then the two key times.
+constexpr int32_t GRID_MOTION = 0x100;
+
 // One diced grid in dicing order.
 struct GridRec {
     int32_t type, cu, cv, prim;
@@ -75,6 +79,7 @@ struct Schedule {
     uint64_t micropolygons = 0;
     uint64_t vertices = 0;
     uint64_t surfaces = 0;
+    bool motion = false;                  // some grid has two motion keys
     // Candidate grids per region bucket (CSR over the bucket list), in dicing order.
     std::vector<uint32_t> candStart;      // buckets + 1
     std::vector<uint32_t> candGrid;

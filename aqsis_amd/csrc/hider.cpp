@@ -17,7 +17,8 @@
 #include "front.h"
 
 extern "C" hipError_t rh_launch_dice(hipStream_t, const FrameDev*, const GridDev*, int, const PrimDev*,
-                                     const float*, float4*, float*, float4*);
+                                     const float*, float4*, float*, float4*, float4*);
+extern "C" hipError_t rh_launch_sample_mb(hipStream_t, const SampleLaunch*, int, const float4*);
 extern "C" hipError_t rh_launch_bin(hipStream_t, const FrameDev*, const BucketDev*, int, const TaskDev*,
                                     const GridDev*, const uint32_t*, const uint32_t*, const uint32_t*, size_t,
                                     const unsigned long long*, const float4*, uint32_t*, uint32_t*, uint32_t*,
@@ -37,6 +38,8 @@ extern "C" hipError_t rh_launch_box_masks(hipStream_t, const FrameDev*, size_t, 
 extern "C" hipError_t rh_launch_filter(hipStream_t, const FrameDev*, int, int, int, int, int, const float4*,
                                        const uint16_t*, const uchar4*, const float*, const float*, const float*,
                                        float*, uchar4*);
+
+static_assert(rh::GRID_MOTION == RH_GRID_MOTION, "motion grid flag differs between host and device");
 
 namespace {
 
@@ -110,6 +113,7 @@ struct rh_ctx {
     DBuf dGrids, dParams, dPrims, dCandStart, dCandGrid, dCapStart, dSegBucket, dSegCounts;
     // per frame
     DBuf dVtx, dMprec, dMpbound, dLists, dListCount, dSamples, dChannels, dRgba, dCounters, dBucketRefs;
+    DBuf dMprecMB;                        // motion-frame micropolygon records
     // transparent hits: arena of (Ci, depth), (Oi, id) pairs and list links
     DBuf dHitData, dHitNext, dHitCtl, dCombine, dFrontDepth;
     bool transparent = false;             // some posted surface has Os < 1
@@ -134,7 +138,7 @@ struct rh_ctx {
                        &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dVtx, &dMprec, &dMpbound, &dLists,
                        &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dBucketRefs,
                        &dHitData, &dHitNext, &dHitCtl, &dCombine, &dFrontDepth, &dItemsStatic, &dItems,
-                       &dItemCount, &dQueue, &dRows, &dSegBucket, &dSegCounts};
+                       &dItemCount, &dQueue, &dRows, &dSegBucket, &dSegCounts, &dMprecMB};
         for (DBuf* b : all) f(*b);
     }
     ~rh_ctx() {
@@ -225,6 +229,8 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
     f.qOne = opt->quantize_one; f.qMin = opt->quantize_min; f.qMax = opt->quantize_max; f.qDither = opt->quantize_dither;
     for (int k = 0; k < 3; ++k) f.zThreshold[k] = opt->zthreshold[k];
     f.hitCapacity = 0;
+    f.shutterOpen = opt->shutter_open; f.shutterClose = opt->shutter_close;
+    f.clipNear = opt->clip_near; f.clipFar = opt->clip_far;
     c->frame = f;
 
     std::vector<BucketDev> bk;
@@ -413,6 +419,7 @@ static rh_status upload_schedule(rh_ctx* c) {
     HIP_TRY(c->dMprec.ensure(S.micropolygons * RH_MPREC_FLOATS * sizeof(float)));
     HIP_TRY(c->dMpbound.ensure(S.micropolygons * sizeof(float4)));
     HIP_TRY(c->dLists.ensure((size_t)S.capStart.back() * sizeof(uint32_t)));
+    if (S.motion) HIP_TRY(c->dMprecMB.ensure(S.micropolygons * RH_MPREC_MB_FLOATS * sizeof(float)));
     // Transparent hits need list storage; start at two per sample and grow
     // when a frame asks for more.
     bool transparent = false;
@@ -451,7 +458,8 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
     if (timed) ENQ(hipEventRecord(c->ev[0], c->st));
     ENQ(rh_launch_frame_begin(c->st, c->dCounters.as<unsigned long long>(), c->dHitCtl.as<uint32_t>()));
     ENQ(rh_launch_dice(c->st, f, c->dGrids.as<GridDev>(), (int)S.grids.size(), c->dPrims.as<PrimDev>(),
-                       c->dParams.as<float>(), c->dVtx.as<float4>(), c->dMprec.as<float>(), c->dMpbound.as<float4>()));
+                       c->dParams.as<float>(), c->dVtx.as<float4>(), c->dMprec.as<float>(), c->dMpbound.as<float4>(),
+                       S.motion ? c->dMprecMB.as<float4>() : nullptr));
     if (timed) ENQ(hipEventRecord(c->ev[1], c->st));
     ENQ(rh_launch_bin(c->st, f, c->dBuckets.as<BucketDev>(), nb, c->dTasks.as<TaskDev>(), c->dGrids.as<GridDev>(),
                       c->dCandStart.as<uint32_t>(), c->dCandGrid.as<uint32_t>(), c->dSegBucket.as<uint32_t>(),
@@ -473,7 +481,12 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
     a.rowCounts = c->dRows.as<uint32_t>(); a.rowBase = 0;
     const int variant = variant_of(tu);
     int rows = 0;
-    if (tu.mode == RH_MODE_PLAIN) {
+    if (S.motion) {
+        // Frames with moving micropolygons take the motion form of the
+        // sampler, whatever the layout.
+        ENQ(rh_launch_sample_mb(c->st, &a, nt, c->dMprecMB.as<float4>()));
+        rows = nt;
+    } else if (tu.mode == RH_MODE_PLAIN) {
         // One launch per bucket, in bucket order.
         for (int b = 0; b < nb; ++b) {
             a.itemBase = s.taskStart[b];
@@ -727,7 +740,7 @@ uint64_t rh_internal_device_bytes(const rh_ctx* c) {
                          &c->dCandGrid, &c->dCapStart, &c->dSegBucket, &c->dSegCounts, &c->dVtx, &c->dMprec, &c->dMpbound, &c->dLists,
                          &c->dListCount, &c->dSamples, &c->dChannels, &c->dRgba, &c->dCounters,
                          &c->dBucketRefs, &c->dHitData, &c->dHitNext, &c->dHitCtl, &c->dCombine, &c->dFrontDepth,
-                         &c->dItemsStatic, &c->dItems, &c->dItemCount, &c->dQueue, &c->dRows};
+                         &c->dItemsStatic, &c->dItems, &c->dItemCount, &c->dQueue, &c->dRows, &c->dMprecMB};
     uint64_t total = 0;
     for (const DBuf* b : all) total += b->cap;
     return total;
@@ -747,7 +760,8 @@ static uint64_t frame_bytes(const rh::Shape& s) {
 static uint64_t footprint_bytes(const rh::Shape& s, const rh::Schedule& S, const rh_prim* prims, int32_t n) {
     uint64_t bytes = frame_bytes(s) + S.grids.size() * sizeof(GridDev) + S.params.size() * 4 +
                      (uint64_t)n * sizeof(PrimDev) + S.candGrid.size() * (4 + 4 + 17 * 4) + S.vertices * 16 +
-                     S.micropolygons * (RH_MPREC_FLOATS * 4 + 16) + S.capStart.back() * 4;
+                     S.micropolygons * (RH_MPREC_FLOATS * 4 + 16) + S.capStart.back() * 4 +
+                     (S.motion ? S.micropolygons * RH_MPREC_MB_FLOATS * 4 : 0);
     bool transparent = false;
     for (int32_t i = 0; i < n; ++i)
         if (!(prims[i].Os[0] >= 1.0f && prims[i].Os[1] >= 1.0f && prims[i].Os[2] >= 1.0f)) transparent = true;

@@ -110,18 +110,14 @@ struct FD {
 // ---------------------------------------------------------------------------
 // Dice + bust
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__ fr, const GridDev* __restrict__ grids,
-                                                    const PrimDev* __restrict__ prims,
-                                                    const float* __restrict__ params, float4* __restrict__ vtx,
-                                                    float* __restrict__ mprec, float4* __restrict__ mpbound) {
-    __shared__ float4 V[RH_MAX_GRID_VERTS];
-    __shared__ float rows[4 * RH_MAX_GRID_ROWS * 3];
+// One key's grid points in camera space into V (LDS): the dicers of the
+// three surface types.  `type` without the motion flag; prm: the key's
+// parameters.
+__device__ __forceinline__ void dice_points(int type, int cu, int cv, const float* __restrict__ prm, float4* V,
+                                            float* rows) {
     const int tid = threadIdx.x;
-    const GridDev G = grids[blockIdx.x];
-    const int cu = G.cu, cv = G.cv, nu = cu + 1, nv = (cu + 1) * (cv + 1);
-    const float* prm = params + G.paramOff;
-
-    if (G.type == RH_PRIM_SPHERE) {
+    const int nu = cu + 1, nv = (cu + 1) * (cv + 1);
+    if (type == RH_PRIM_SPHERE) {
         // CqSphere::DicePoints + DiceAll (quadrics.cpp:457-492, 82-254)
         const float radius = prm[SPH_RADIUS_D];
         const bool ident = prm[SPH_IDENT_D] != 0.0f;
@@ -136,7 +132,7 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
             p = xform_point(prm + SPH_MAT_D, ident, p);
             V[i] = make_float4(p.x, p.y, p.z, 0.0f);
         }
-    } else if (G.type == RH_PRIM_BICUBIC) {
+    } else if (type == RH_PRIM_BICUBIC) {
         // bicubicPatchNatDice (patch.cpp:205-243): four v-direction forward
         // differences give each row's control points, then one u-direction
         // forward difference per row.
@@ -189,6 +185,31 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
     }
     __syncthreads();
 
+}
+
+// Floats of one key's parameters (the layouts the host schedule writes).
+__device__ __forceinline__ int key_param_floats(int type, int cu, int cv) {
+    return type == RH_PRIM_SPHERE ? SPH_TABLES_D + 2 * (cu + 1) + 2 * (cv + 1)
+                                  : (type == RH_PRIM_BICUBIC ? 48 : 12);
+}
+
+__global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__ fr, const GridDev* __restrict__ grids,
+                                                    const PrimDev* __restrict__ prims,
+                                                    const float* __restrict__ params, float4* __restrict__ vtx,
+                                                    float* __restrict__ mprec, float4* __restrict__ mpbound,
+                                                    float4* __restrict__ mprecMB) {
+    __shared__ float4 V[RH_MAX_GRID_VERTS];
+    __shared__ float4 V1[RH_MAX_GRID_VERTS];
+    __shared__ float rows[4 * RH_MAX_GRID_ROWS * 3];
+    const int tid = threadIdx.x;
+    const GridDev G = grids[blockIdx.x];
+    const int cu = G.cu, cv = G.cv, nu = cu + 1, nv = (cu + 1) * (cv + 1);
+    const float* prm = params + G.paramOff;
+
+    const int type = G.type & ~RH_GRID_MOTION;
+    const bool motion = (G.type & RH_GRID_MOTION) != 0;
+    dice_points(type, cu, cv, prm, V, rows);
+    __syncthreads();
     // Projection to hybrid raster space keeping camera z (micropolygon.cpp:723-731)
     for (int i = tid; i < nv; i += blockDim.x) {
         float4 p = V[i];
@@ -196,6 +217,22 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
         float4 q = make_float4(r.x, r.y, p.z, 0.0f);
         V[i] = q;
         vtx[G.vtxBase + i] = q;
+    }
+    // A deforming surface's grid: key 1 diced with key 0's sizes and
+    // projected the same way (CqDeformingSurface::Dice, surface.h:643-655;
+    // CqMotionMicroPolyGrid::Split, micropolygon.cpp:970-1024).
+    const float* prm1 = prm + key_param_floats(type, cu, cv);
+    if (motion) {
+        __syncthreads();
+        dice_points(type, cu, cv, prm1, V1, rows);
+        __syncthreads();
+        for (int i = tid; i < nv; i += blockDim.x) {
+            float4 p = V1[i];
+            f3d r = xform_point(fr->camToRaster, false, mk3(p.x, p.y, p.z));
+            float4 q = make_float4(r.x, r.y, p.z, 0.0f);
+            V1[i] = q;
+            vtx[G.vtxBase + nv + i] = q;
+        }
     }
     __syncthreads();
 
@@ -256,7 +293,48 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
         out[2] = make_float4(P[0].z, P[1].z, P[2].z, P[3].z);
         out[3] = make_float4(bminx, bmaxx, bminy, bmaxy);
         out[4] = make_float4(bminz, __uint_as_float(flags), __int_as_float((int)blockIdx.x), 0.0f);
-        mpbound[G.mpBase + m] = make_float4(bminx, bmaxx, bminy, bmaxy);
+        if (!mprecMB) {
+            mpbound[G.mpBase + m] = make_float4(bminx, bmaxx, bminy, bmaxy);
+            continue;
+        }
+        // Motion frames: every micropolygon also gets the record the motion
+        // sampler reads: both keys' vertices in grid order (a static
+        // micropolygon repeats key 0), flags, grid, key times, and its static
+        // bound.  A moving one is binned by m_Bound, key 0's bound
+        // encapsulating key 1's (AppendKey, micropolygon.cpp:1952-1967; key
+        // bounds as CqMovingMicroPolygonKey::GetBound, :1974-1989).
+        f3d K1[4] = {P[0], P[1], P[2], P[3]};
+        if (motion) {
+            float4 r0 = V1[idx], r1 = V1[idx + 1], r2 = V1[idx + nu], r3 = V1[idx + nu + 1];
+            K1[0] = mk3(r0.x, r0.y, r0.z); K1[1] = mk3(r1.x, r1.y, r1.z);
+            K1[2] = mk3(r2.x, r2.y, r2.z); K1[3] = mk3(r3.x, r3.y, r3.z);
+        }
+        float4* mb = mprecMB + (size_t)(G.mpBase + m) * (RH_MPREC_MB_FLOATS / 4);
+        mb[0] = make_float4(P[0].x, P[0].y, P[0].z, P[1].x);
+        mb[1] = make_float4(P[1].y, P[1].z, P[2].x, P[2].y);
+        mb[2] = make_float4(P[2].z, P[3].x, P[3].y, P[3].z);
+        mb[3] = make_float4(K1[0].x, K1[0].y, K1[0].z, K1[1].x);
+        mb[4] = make_float4(K1[1].y, K1[1].z, K1[2].x, K1[2].y);
+        mb[5] = make_float4(K1[2].z, K1[3].x, K1[3].y, K1[3].z);
+        const float t0 = motion ? prm1[key_param_floats(type, cu, cv)] : 0.0f;
+        const float t1 = motion ? prm1[key_param_floats(type, cu, cv) + 1] : 0.0f;
+        mb[6] = make_float4(__uint_as_float(flags | (motion ? RH_MB_MOVING : 0u)), __int_as_float((int)blockIdx.x), t0, t1);
+        mb[7] = make_float4(bminx, bmaxx, bminy, bmaxy);
+        if (motion) {
+            float kb[2][6];
+            const f3d* keys[2] = {P, K1};
+            for (int k = 0; k < 2; ++k) {
+                const f3d* q = keys[k];
+                kb[k][0] = fmin_ref(q[0].x, fmin_ref(q[1].x, fmin_ref(q[2].x, q[3].x)));
+                kb[k][1] = fmax_ref(q[0].x, fmax_ref(q[1].x, fmax_ref(q[2].x, q[3].x)));
+                kb[k][2] = fmin_ref(q[0].y, fmin_ref(q[1].y, fmin_ref(q[2].y, q[3].y)));
+                kb[k][3] = fmax_ref(q[0].y, fmax_ref(q[1].y, fmax_ref(q[2].y, q[3].y)));
+            }
+            mpbound[G.mpBase + m] = make_float4(fmin_ref(kb[0][0], kb[1][0]), fmax_ref(kb[0][1], kb[1][1]),
+                                                fmin_ref(kb[0][2], kb[1][2]), fmax_ref(kb[0][3], kb[1][3]));
+        } else {
+            mpbound[G.mpBase + m] = make_float4(bminx, bmaxx, bminy, bmaxy);
+        }
     }
 }
 
@@ -1034,6 +1112,475 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
 }
 
 // ---------------------------------------------------------------------------
+// Sampling of frames with moving micropolygons
+// ---------------------------------------------------------------------------
+// The point-in-polygon test of one sample against four raster vertices in
+// grid order with a given vertex-order code: cachePointInPolyTest's edge
+// equations (micropolygon.cpp:1346-1392), fContains's strictness
+// (:1295-1342), CqInvBilinear with one or two Newton steps
+// (bilinear.h:260-310) and the depth bilerp (:228-236).
+struct MBHit {
+    bool in;
+    bool linear;
+    float D, u, v;
+};
+
+__device__ __forceinline__ MBHit quad_hit(const f3d* pin, uint32_t code, bool degen, float px, float py) {
+    MBHit h;
+    h.in = false;
+    h.D = 0.f; h.u = 0.5f; h.v = 0.5f;
+    const float Ex = pin[1].x - pin[0].x, Ey = pin[1].y - pin[0].y;
+    const float Fx = pin[2].x - pin[0].x, Fy = pin[2].y - pin[0].y;
+    const float Gx = ((-Ex) - pin[2].x) + pin[3].x, Gy = ((-Ey) - pin[2].y) + pin[3].y;
+    const float patchSize = fmax_ref(fmax_ref(fabsf(Fx), fabsf(Fy)), fmax_ref(fabsf(Ex), fabsf(Ey)));
+    const float irregularity = fmax_ref(fabsf(Gx), fabsf(Gy));
+    h.linear = (double)irregularity < 1e-2 * (double)patchSize;
+    float qx[4], qy[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int o = i < 3 ? (int)(code >> (2 * (i + 1))) & 3 : (int)code & 3;
+        const f3d q = o == 0 ? pin[0] : (o == 1 ? pin[1] : (o == 2 ? pin[2] : pin[3]));
+        qx[i] = q.x;
+        qy[i] = q.y;
+    }
+    float YM[4], XM[4], X[4], Y[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int j = (i + 3) & 3;
+        YM[i] = qx[i] - qx[j];
+        XM[i] = qy[i] - qy[j];
+        X[i] = qx[j];
+        Y[i] = qy[j];
+    }
+    if (degen) {
+#pragma unroll
+        for (int i = 2; i < 4; ++i) {
+            YM[i] = qx[3] - qx[1];
+            XM[i] = qy[3] - qy[1];
+            X[i] = qx[1];
+            Y[i] = qy[1];
+        }
+    }
+    const float t0 = ((py - Y[0]) * YM[0]) - ((px - X[0]) * XM[0]);
+    const float t1 = ((py - Y[1]) * YM[1]) - ((px - X[1]) * XM[1]);
+    const float t2 = ((py - Y[2]) * YM[2]) - ((px - X[2]) * XM[2]);
+    const float t3 = ((py - Y[3]) * YM[3]) - ((px - X[3]) * XM[3]);
+    if (t0 <= 0 || t1 <= 0 || t2 < 0 || t3 < 0) return h;
+    h.in = true;
+    const float Ax = pin[0].x, Ay = pin[0].y;
+    float u = 0.5f, v = 0.5f;
+    {
+        const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
+        const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
+        const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px;
+        const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py;
+        float det = M1x * M2y - M1y * M2x;
+        det = 1 / det;
+        u = u - det * (bx * M2y - by * M2x);
+        v = v - det * (-(bx * M1y - by * M1x));
+    }
+    if (!h.linear) {
+        const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
+        const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
+        const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px;
+        const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py;
+        float det = M1x * M2y - M1y * M2x;
+        if (det != 0) det = 1 / det;
+        u = u - det * (bx * M2y - by * M2x);
+        v = v - det * (-(bx * M1y - by * M1x));
+    }
+    const float w0 = (1 - v) * (1 - u), w1 = (1 - v) * u, w2 = v * (1 - u), w3 = v * u;
+    h.D = ((w0 * pin[0].z + w1 * pin[1].z) + w2 * pin[2].z) + w3 * pin[3].z;
+    h.u = u;
+    h.v = v;
+    return h;
+}
+
+// lfloor / lceil of math.h:47-61 to int where the result fits.
+__device__ __forceinline__ int lfloor_i(float x) {
+    if (fabsf(x) < 2147483648.0f) return (int)floorf(x);
+    long t = (long)x;
+    return (int)(t - (x < 0 && x != (float)t));
+}
+
+// The motion form of the sampler: one workgroup per task, one wave per
+// block of 64 samples (lanes in row-major sub-sample order, as rh_sample),
+// the task's list staged through LDS in chunks of CH records.  Each lane
+// walks the list in order and runs, for its own sample, exactly the visits
+// the reference makes:
+//  * a static micropolygon: RenderMPG_Static (bucketprocessor.cpp:1097-1218);
+//  * a moving one: RenderMPG_MBOrDof without DoF (:1221-1469) over the
+//    sub-bounds of CqMicroPolygonMotion::BuildBoundList
+//    (micropolygon.cpp:1689-1756): a sub-bound visits the samples of its
+//    pixels whose index lies in the window its times give, each visit
+//    counted as a sample test, then the time window, bound, occlusion and
+//    LOD tests and CqMicroPolygonMotion::Sample (:1768-1914): the tight
+//    bound and the vertices at the sample's time, and the point-in-polygon
+//    test on them.  A window reaching past a pixel's samples stops at the
+//    last one.
+template <int CH>
+__global__ void __launch_bounds__(256) rh_sample_mb(const FrameDev* __restrict__ fr,
+                                                    const BucketDev* __restrict__ buckets,
+                                                    const TaskDev* __restrict__ tasks, int ntasks,
+                                                    const unsigned long long* __restrict__ capStart,
+                                                    const uint32_t* __restrict__ listCount,
+                                                    const uint32_t* __restrict__ lists,
+                                                    const float4* __restrict__ mprecMB,
+                                                    const GridDev* __restrict__ grids,
+                                                    const PrimDev* __restrict__ prims,
+                                                    const float* __restrict__ pos2d, const float* __restrict__ pos1d,
+                                                    const uchar4* __restrict__ pixPattern,
+                                                    float4* __restrict__ samples, float4* __restrict__ hitData,
+                                                    uint32_t* __restrict__ hitNext, uint32_t* __restrict__ hitCounter,
+                                                    uint4* __restrict__ combineList, float* __restrict__ frontDepth,
+                                                    uint32_t* __restrict__ rowCounts, int rowBase) {
+    __shared__ float4 recs[CH * 8];
+    __shared__ uint32_t ids[CH];
+    __shared__ uint32_t wsum[RH_S_WAVES][5];
+    const int task = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
+    uint32_t wbBase = 0;
+    int wbLeft = 0;
+    bool lost = false;
+    if (task < ntasks) {
+    const int ns = fr->ns, nx = fr->nx, ny = fr->ny, bw = fr->blockW, bh = fr->blockH;
+    const int BW = bw * nx, BH = bh * ny;
+    const int rx = lane % BW, ry = lane / BW;
+    const bool laneOk = lane < BW * BH;
+    const int lx = rx / nx, sm = rx - lx * nx, ly = ry / ny, sn = ry - ly * ny;
+    const int si = sn * nx + sm;
+    const TaskDev T = tasks[task];
+    const BucketDev B = buckets[T.bucket];
+    const int nbx = (T.x1 - T.x0 + bw - 1) / bw, nby = (T.y1 - T.y0 + bh - 1) / bh;
+    const uint32_t* L = lists + capStart[task];
+    const uint32_t n = listCount[task];
+    const float opentime = fr->shutterOpen, closetime = fr->shutterClose;
+    // isClose (math.h:174-182)
+    const float dsh = fabsf(closetime - opentime);
+    const bool fastShutter = dsh <= 1.1920929e-06f * fabsf(closetime) || dsh <= 1.1920929e-06f * fabsf(opentime);
+    const float timePerSample = fastShutter ? 0.0f : ns / (closetime - opentime);
+    const int timeRanges = max(4, nx * ny);
+    const float clipNear = fr->clipNear, clipFar = fr->clipFar;
+
+    for (int p0 = 0; p0 < nbx * nby; p0 += RH_S_WAVES) {
+        const int blk = p0 + wave;
+        const bool has = blk < nbx * nby;       // uniform per wave
+        const int bx = T.x0 + (blk % nbx) * bw, by = T.y0 + (blk / nbx) * bh;
+        const int px = bx + lx, py = by + ly;
+        const bool live = has && laneOk && px < T.x1 && py < T.y1;
+        const int gx = (px - B.sx0) * nx + sm, gy = (py - B.sy0) * ny + sn;
+        float sx = 0.f, sy = 0.f, stime = 0.f;
+        bool lodOk = false;
+        if (live) {
+            const uchar4 pat = pixPattern[(size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)];
+            sx = (float)px + pos2d[2 * ((size_t)pat.x * ns + si)];
+            sy = (float)py + pos2d[2 * ((size_t)pat.x * ns + si) + 1];
+            // time = (close - open) * t + open (imagepixel.cpp:353-355)
+            stime = (closetime - opentime) * pos1d[(size_t)pat.y * ns + si] + opentime;
+            const float lod = pos1d[(size_t)pat.z * ns + si];
+            lodOk = !(0.0f > lod || lod >= 1.0f);
+        }
+        float occlZ = FLT_MAX, wu = 0.f, wv = 0.f;
+        uint32_t win = 0xffffffffu, head = 0xffffffffu;
+        int nh = 0;
+
+        for (uint32_t c0 = 0; c0 < n; c0 += CH) {
+            const int cnt = (int)((n - c0) < (uint32_t)CH ? (n - c0) : (uint32_t)CH);
+            __syncthreads();
+            for (int q = tid; q < cnt * 8; q += blockDim.x) {
+                const uint32_t id = L[c0 + q / 8];
+                recs[q] = mprecMB[(size_t)id * 8 + (q & 7)];
+                if ((q & 7) == 0) ids[q / 8] = id;
+            }
+            __syncthreads();
+            if (!has) continue;
+            for (int j = 0; j < cnt; ++j) {
+                const float4* R = &recs[j * 8];
+                const float4 r0 = R[0], r1 = R[1], r2 = R[2], r6 = R[6], r7 = R[7];
+                const uint32_t flags = __float_as_uint(r6.x);
+                const uint32_t code = flags & 0xffu;
+                const bool degen = (flags >> 8) & 1u;
+                const bool opaque = (flags >> 10) & 1u;
+                const f3d K0[4] = {mk3(r0.x, r0.y, r0.z), mk3(r0.w, r1.x, r1.y), mk3(r1.z, r1.w, r2.x),
+                                   mk3(r2.y, r2.z, r2.w)};
+                if (!(flags & RH_MB_MOVING)) {
+                    // RenderMPG_Static: the candidate samples of the bound's
+                    // pixel range with the first/last pixel sub-sample clip
+                    const float bminx = r7.x, bmaxx = r7.y, bminy = r7.z, bmaxy = r7.w;
+                    // CalculateBound's min z: min(min(min(A,B),C),D), A..D = P0,P1,P3,P2
+                    const float bminz = fmin_ref(fmin_ref(fmin_ref(K0[0].z, K0[1].z), K0[3].z), K0[2].z);
+                    int eX = lceil_i(bmaxx), eY = lceil_i(bmaxy);
+                    if (eX > B.sx1) eX = B.sx1;
+                    if (eY > B.sy1) eY = B.sy1;
+                    int sX = floor_i(bminx), sY = floor_i(bminy);
+                    if (sY < B.sy0) sY = B.sy0;
+                    if (sX < B.sx0) sX = B.sx0;
+                    bool cand = false;
+                    if (sX < eX && sY < eY) {
+                        const int im = (bminx < sX) ? 0 : floor_i((bminx - sX) * nx);
+                        const int in = (bminy < sY) ? 0 : floor_i((bminy - sY) * ny);
+                        const int em = (bmaxx > eX) ? nx : lceil_i((bmaxx - (eX - 1)) * nx);
+                        const int en = (bmaxy > eY) ? ny : lceil_i((bmaxy - (eY - 1)) * ny);
+                        const int gx0 = (sX - B.sx0) * nx + im, gx1 = (eX - 1 - B.sx0) * nx + em;
+                        const int gy0 = (sY - B.sy0) * ny + in, gy1 = (eY - 1 - B.sy0) * ny + en;
+                        cand = live && gx >= gx0 && gx < gx1 && gy >= gy0 && gy < gy1;
+                    }
+                    bool hit = false;
+                    float D = 0.f, u = 0.f, v = 0.f;
+                    if (cand) {
+                        ++nTest;
+                        const bool inb = !(sx < bminx || sx > bmaxx || sy < bminy || sy > bmaxy);
+                        if (inb && !(bminz > occlZ) && lodOk) {
+                            ++nBound;
+                            const MBHit h = quad_hit(K0, code, degen, sx, sy);
+                            if (h.in) {
+                                ++nIn;
+                                if (!h.linear) ++nIn2;
+                                hit = !(occlZ <= h.D);
+                                D = h.D; u = h.u; v = h.v;
+                            }
+                        }
+                    }
+                    // StoreSample (bucketprocessor.cpp:1471-1569)
+                    nHit += hit ? 1u : 0u;
+                    const bool opq = hit && opaque;
+                    occlZ = opq ? D : occlZ;
+                    win = opq ? ids[j] : win;
+                    wu = opq ? u : wu;
+                    wv = opq ? v : wv;
+                    const bool app = hit && !opaque;
+                    const unsigned long long am = __ballot(app);
+                    if (am) {
+                        const int na = __popcll(am);
+                        if (na > wbLeft) {
+                            uint32_t base = 0;
+                            if (lane == 0) base = atomicAdd(hitCounter, 64u);
+                            wbBase = __shfl(base, 0);
+                            wbLeft = 64;
+                        }
+                        if (app) {
+                            const uint32_t slot =
+                                wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & ((1ull << lane) - 1ull));
+                            if (slot < fr->hitCapacity && nh < RH_MAX_HITS - 1) {
+                                float4 hA, hB;
+                                shade_hit(prims[grids[__float_as_int(r6.y)].prim], u, v, D, ids[j], hA, hB);
+                                hitData[2 * (size_t)slot] = hA;
+                                hitData[2 * (size_t)slot + 1] = hB;
+                                hitNext[slot] = head;
+                                head = slot;
+                                ++nh;
+                            } else if (slot < fr->hitCapacity) {
+                                lost = true;
+                            }
+                        }
+                        wbLeft -= na;
+                    }
+                    continue;
+                }
+                // A moving micropolygon.
+                const float4 r3 = R[3], r4 = R[4], r5 = R[5];
+                const f3d K1[4] = {mk3(r3.x, r3.y, r3.z), mk3(r3.w, r4.x, r4.y), mk3(r4.z, r4.w, r5.x),
+                                   mk3(r5.y, r5.z, r5.w)};
+                const float T0 = r6.z, T1 = r6.w;
+                // key bounds (CqMovingMicroPolygonKey::GetBound): x, y, z min and max
+                float kb[2][6];
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const f3d* q = k ? K1 : K0;
+                    kb[k][0] = fmin_ref(q[0].x, fmin_ref(q[1].x, fmin_ref(q[2].x, q[3].x)));
+                    kb[k][1] = fmin_ref(q[0].y, fmin_ref(q[1].y, fmin_ref(q[2].y, q[3].y)));
+                    kb[k][2] = fmin_ref(q[0].z, fmin_ref(q[1].z, fmin_ref(q[2].z, q[3].z)));
+                    kb[k][3] = fmax_ref(q[0].x, fmax_ref(q[1].x, fmax_ref(q[2].x, q[3].x)));
+                    kb[k][4] = fmax_ref(q[0].y, fmax_ref(q[1].y, fmax_ref(q[2].y, q[3].y)));
+                    kb[k][5] = fmax_ref(q[0].z, fmax_ref(q[1].z, fmax_ref(q[2].z, q[3].z)));
+                }
+                // BuildBoundList: sub-bounds from the raster move in widths
+                float polyLen2, moveDist2;
+                {
+                    const float wx = kb[0][3] - kb[0][0], wy = kb[0][4] - kb[0][1];
+                    polyLen2 = (wy == 0.0f) ? wx * wx : ((wx == 0.0f) ? wy * wy : (wx * wx) + (wy * wy));
+                    const float mxd = K0[0].x - K1[0].x, myd = K0[0].y - K1[0].y;
+                    moveDist2 = (myd == 0.0f) ? mxd * mxd : ((mxd == 0.0f) ? myd * myd : (mxd * mxd) + (myd * myd));
+                }
+                const float lengths = sqrtf(moveDist2 / polyLen2);
+                // lfloor of an infinite or NaN ratio is LONG_MIN on x86-64: max(1, .) = 1
+                int moved = (lengths < 9.2e18f) ? (lengths < 2147483648.0f ? lfloor_i(lengths) : 0x7fffffff) : 1;
+                if (moved < 1) moved = 1;
+                const int divisions = moved < timeRanges ? moved : timeRanges;
+                const float dt = (closetime - opentime) / divisions;
+                float time = opentime + dt;
+                float bnd[6] = {kb[0][0], kb[0][1], kb[0][2], kb[0][3], kb[0][4], kb[0][5]};
+                for (int i = 0; i < divisions; ++i) {
+                    const float mix = (time - T0) / (T1 - T0);
+                    float mid[6];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) {
+                        mid[c] = kb[0][c] + mix * (kb[1][c] - kb[0][c]);
+                        bnd[c] = c < 3 ? fmin_ref(bnd[c], mid[c]) : fmax_ref(bnd[c], mid[c]);
+                    }
+                    const float time0 = time - dt;
+                    const float time1 = (i != divisions - 1) ? (time + dt) - dt : closetime;
+                    const float sbminx = bnd[0], sbminy = bnd[1], sbminz = bnd[2], sbmaxx = bnd[3],
+                                sbmaxy = bnd[4], sbmaxz = bnd[5];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) bnd[c] = mid[c];
+                    time += dt;
+                    if (time1 < opentime || time0 > closetime) continue;
+                    int indexT0, indexT1;
+                    if (fastShutter) {
+                        indexT0 = 0;
+                        indexT1 = ns;
+                    } else {
+                        indexT0 = lfloor_i((time0 - opentime) * timePerSample);
+                        if (indexT0 < 0) indexT0 = 0;
+                        indexT1 = lceil_i((time1 - opentime) * timePerSample);
+                    }
+                    if (sbminz > clipFar || sbmaxz < clipNear) continue;
+                    int eX = lceil_i(sbmaxx), eY = lceil_i(sbmaxy);
+                    if (eX > B.sx1) eX = B.sx1;
+                    if (eY > B.sy1) eY = B.sy1;
+                    int sX = floor_i(sbminx), sY = floor_i(sbminy);
+                    if (sY < B.sy0) sY = B.sy0;
+                    if (sX < B.sx0) sX = B.sx0;
+                    if (sX >= eX || sY >= eY) continue;
+                    const int idxEnd = indexT1 > indexT0 + 1 ? indexT1 : indexT0 + 1;
+                    const bool visit = live && px >= sX && px < eX && py >= sY && py < eY && si >= indexT0 &&
+                                       si < idxEnd && si < ns;
+                    bool hit = false;
+                    float D = 0.f, u = 0.f, v = 0.f;
+                    if (visit) {
+                        ++nTest;
+                        const bool inb = !(sx < sbminx || sx > sbmaxx || sy < sbminy || sy > sbmaxy);
+                        if (!(stime < time0 || stime > time1) && inb && !(sbminz > occlZ) && lodOk) {
+                            ++nBound;
+                            // CqMicroPolygonMotion::Sample: the key span of the time
+                            int iIndex = 0;
+                            float Fraction = 0.0f;
+                            bool Exact = true;
+                            if (stime > T0) {
+                                if (stime >= T1) {
+                                    iIndex = 1;
+                                } else {
+                                    Fraction = (stime - T0) / (T1 - T0);
+                                    Exact = (T0 == stime);
+                                }
+                            }
+                            float tb[4];
+                            if (Exact) {
+                                tb[0] = kb[iIndex][0]; tb[1] = kb[iIndex][3]; tb[2] = kb[iIndex][1]; tb[3] = kb[iIndex][4];
+                            } else {
+                                const float F1 = 1 - Fraction;
+                                tb[0] = F1 * kb[0][0] + Fraction * kb[1][0];
+                                tb[1] = F1 * kb[0][3] + Fraction * kb[1][3];
+                                tb[2] = F1 * kb[0][1] + Fraction * kb[1][1];
+                                tb[3] = F1 * kb[0][4] + Fraction * kb[1][4];
+                            }
+                            if (!(sx < tb[0] || sx > tb[1] || sy < tb[2] || sy > tb[3])) {
+                                f3d pts[4];
+                                if (Exact) {
+#pragma unroll
+                                    for (int q = 0; q < 4; ++q) pts[q] = iIndex ? K1[q] : K0[q];
+                                } else {
+                                    const float F1 = 1.0f - Fraction;
+#pragma unroll
+                                    for (int q = 0; q < 4; ++q)
+                                        pts[q] = mk3((F1 * K0[q].x) + (Fraction * K1[q].x),
+                                                     (F1 * K0[q].y) + (Fraction * K1[q].y),
+                                                     (F1 * K0[q].z) + (Fraction * K1[q].z));
+                                }
+                                const MBHit h = quad_hit(pts, code, degen, sx, sy);
+                                if (h.in) {
+                                    ++nIn;
+                                    if (!h.linear) ++nIn2;
+                                    hit = !(occlZ <= h.D);
+                                    D = h.D; u = h.u; v = h.v;
+                                }
+                            }
+                        }
+                    }
+                    nHit += hit ? 1u : 0u;
+                    const bool opq = hit && opaque;
+                    occlZ = opq ? D : occlZ;
+                    win = opq ? ids[j] : win;
+                    wu = opq ? u : wu;
+                    wv = opq ? v : wv;
+                    const bool app = hit && !opaque;
+                    const unsigned long long am = __ballot(app);
+                    if (am) {
+                        const int na = __popcll(am);
+                        if (na > wbLeft) {
+                            uint32_t base = 0;
+                            if (lane == 0) base = atomicAdd(hitCounter, 64u);
+                            wbBase = __shfl(base, 0);
+                            wbLeft = 64;
+                        }
+                        if (app) {
+                            const uint32_t slot =
+                                wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & ((1ull << lane) - 1ull));
+                            if (slot < fr->hitCapacity && nh < RH_MAX_HITS - 1) {
+                                float4 hA, hB;
+                                shade_hit(prims[grids[__float_as_int(r6.y)].prim], u, v, D, ids[j], hA, hB);
+                                hitData[2 * (size_t)slot] = hA;
+                                hitData[2 * (size_t)slot + 1] = hB;
+                                hitNext[slot] = head;
+                                head = slot;
+                                ++nh;
+                            } else if (slot < fr->hitCapacity) {
+                                lost = true;
+                            }
+                        }
+                        wbLeft -= na;
+                    }
+                }
+            }
+        }
+        if (live) {
+            const size_t sidx = ((size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)) * ns + si;
+            float4 a = make_float4(0.f, 0.f, 0.f, FLT_MAX), b = make_float4(0.f, 0.f, 0.f, __uint_as_float(win));
+            if (win != 0xffffffffu) {
+                const float4 r6w = mprecMB[(size_t)win * 8 + 6];
+                shade_hit(prims[grids[__float_as_int(r6w.y)].prim], wu, wv, occlZ, win, a, b);
+            }
+            samples[2 * sidx] = a;
+            samples[2 * sidx + 1] = b;
+            if (frontDepth) frontDepth[sidx] = occlZ;
+        }
+        const bool multi = live && nh > 0;
+        const unsigned long long cm = __ballot(multi);
+        if (cm) {
+            uint32_t base = 0;
+            if (lane == __ffsll((long long)cm) - 1) base = atomicAdd(&hitCounter[2], (uint32_t)__popcll(cm));
+            base = __shfl(base, __ffsll((long long)cm) - 1);
+            if (multi) {
+                const size_t sidx = ((size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)) * ns + si;
+                const uint32_t e = base + (uint32_t)__popcll(cm & ((1ull << lane) - 1ull));
+                combineList[e] = make_uint4((uint32_t)sidx, (uint32_t)(sidx >> 32), head, (uint32_t)nh);
+            }
+        }
+    }
+    }   // task < ntasks
+    if (__ballot(lost) && lane == 0) atomicAdd(&hitCounter[1], 1u);
+    for (int off = 32; off > 0; off >>= 1) {
+        nTest += __shfl_xor(nTest, off);
+        nBound += __shfl_xor(nBound, off);
+        nHit += __shfl_xor(nHit, off);
+        nIn += __shfl_xor(nIn, off);
+        nIn2 += __shfl_xor(nIn2, off);
+    }
+    if (lane == 0) {
+        wsum[wave][0] = nTest; wsum[wave][1] = nBound; wsum[wave][2] = nHit; wsum[wave][3] = nIn;
+        wsum[wave][4] = nIn2;
+    }
+    __syncthreads();
+    if (tid < 5) {
+        uint32_t v = 0;
+        for (int w = 0; w < RH_S_WAVES; ++w) v += wsum[w][tid];
+        rowCounts[(size_t)(rowBase + blockIdx.x) * 5 + tid] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Batched form: work items ordered by cost
 // ---------------------------------------------------------------------------
 // One workgroup.  A task's cost is (list length + 16) x blocks; a task is cut
@@ -1441,10 +1988,10 @@ __global__ void __launch_bounds__(128) rh_filter_tiled(const FrameDev* __restric
 // ---------------------------------------------------------------------------
 extern "C" hipError_t rh_launch_dice(hipStream_t st, const FrameDev* fr, const GridDev* grids, int ngrids,
                                      const PrimDev* prims, const float* params, float4* vtx, float* mprec,
-                                     float4* mpbound) {
+                                     float4* mpbound, float4* mprecMB) {
     if (ngrids <= 0) return hipSuccess;
     hipLaunchKernelGGL(rh_dice_bust, dim3(ngrids), dim3(256), 0, st, fr, grids, prims, params, vtx, mprec,
-                       mpbound);
+                       mpbound, mprecMB);
     return hipGetLastError();
 }
 
@@ -1510,6 +2057,17 @@ extern "C" hipError_t rh_launch_sample(hipStream_t st, int variant, int grid, co
     case 3: launch_sample_t<96, 2>(st, grid, *a); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// The motion form of the sampler: one workgroup per task.
+extern "C" hipError_t rh_launch_sample_mb(hipStream_t st, const SampleLaunch* a, int ntasks,
+                                          const float4* mprecMB) {
+    if (ntasks <= 0) return hipSuccess;
+    hipLaunchKernelGGL((rh_sample_mb<64>), dim3(ntasks), dim3(256), 0, st, a->fr, a->buckets, a->tasks, ntasks,
+                       a->capStart, a->listCount, a->lists, mprecMB, a->grids, a->prims, a->pos2d, a->pos1d,
+                       a->pixPattern, a->samples, a->hitData, a->hitNext, a->hitCounter, a->combineList,
+                       a->frontDepth, a->rowCounts, a->rowBase);
     return hipGetLastError();
 }
 

@@ -94,6 +94,41 @@ def bilinear_patches():
     return opts, out
 
 
+def motion_transparent_8x8():
+    # The transparency + motion blur configuration at half the linear size:
+    # 8x8 samples, gaussian 2x2, Os 0.2-0.8, every patch a deforming surface
+    # moved N(0, 0.05 W) pixels between keys 0 and 1, shutter [0 1].
+    return scenes.motion_scene(seed=21, n=160, res=(128, 96))
+
+
+def motion_mixed_opaque():
+    # Moving and static opaque patches in one frame (4x4, catmull-rom 3x3),
+    # keys at 0.25 and 0.75 inside a [0, 1] shutter, so sample times before
+    # and after the keys use the end keys' vertices exactly.
+    opts, prims = scenes.motion_scene(seed=22, n=220, res=(160, 112), samples=(4, 4), opacity=None,
+                                      filt=("catmull-rom", 3.0, 3.0), shift_sigma=0.08, key_times=(0.25, 0.75))
+    for i, p in enumerate(prims):
+        if i % 3 == 0:
+            p.nkeys = 1
+    return opts, prims
+
+
+def motion_spheres():
+    # Moving spheres (key 1's object->camera matrix translated), 3x3 samples.
+    opts, prims = scenes.spheres_scene(seed=23, n=40, res=(144, 112), samples=(3, 3), filt=("box", 1.0, 1.0))
+    opts.shutter_open, opts.shutter_close = 0.0, 1.0
+    rng = np.random.default_rng(23)
+    for p in prims:
+        d = rng.normal(0.0, 0.4, 3)
+        d[2] = 0.0
+        p.nkeys = 2
+        p.key_time[:] = [0.0, 1.0]
+        m = list(p.xform[0])
+        m[12] += float(d[0]); m[13] += float(d[1])
+        p.xform[1][:] = [float(np.float32(v)) for v in m]
+    return opts, prims
+
+
 CASES = {
     "spheres_config": spheres_config,
     "patches_small": patches_small,
@@ -108,4 +143,7 @@ CASES = {
     "spheres_mixed_opacity": spheres_mixed_opacity,
     "patches_deep_transparent": patches_deep_transparent,
     "bilinear_patches": bilinear_patches,
+    "motion_transparent_8x8": motion_transparent_8x8,
+    "motion_mixed_opaque": motion_mixed_opaque,
+    "motion_spheres": motion_spheres,
 }
