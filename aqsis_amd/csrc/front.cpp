@@ -908,12 +908,15 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
                 if (k >= g.diceBucket) {
                     out.candGrid[fill[k]++] = gi;
                     // A micropolygon enters a task's list only if its pixel
-                    // window meets the task, and its window lies inside the
-                    // grid's guarded bound.
+                    // window meets the task, and a static one's window lies
+                    // inside the grid's guarded bound.  A moving one's
+                    // window (its sub-bounds) may reach past its keys'
+                    // bound to anywhere in the bucket.
+                    const bool anyTask = (g.type & GRID_MOTION) != 0;
                     for (int t = s.taskStart[k]; t < s.taskStart[k + 1]; ++t) {
                         const auto& T = s.tasks[t];
-                        if (g.px0 < (float)T.x1 && g.px1 > (float)T.x0 && g.py0 < (float)T.y1 &&
-                            g.py1 > (float)T.y0)
+                        if (anyTask || (g.px0 < (float)T.x1 && g.px1 > (float)T.x0 && g.py0 < (float)T.y1 &&
+                                        g.py1 > (float)T.y0))
                             out.capStart[t + 1] += (uint64_t)(g.cu * g.cv);
                     }
                 }

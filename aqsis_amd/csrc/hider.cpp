@@ -22,7 +22,7 @@ extern "C" hipError_t rh_launch_sample_mb(hipStream_t, const SampleLaunch*, int,
 extern "C" hipError_t rh_launch_bin(hipStream_t, const FrameDev*, const BucketDev*, int, const TaskDev*,
                                     const GridDev*, const uint32_t*, const uint32_t*, const uint32_t*, size_t,
                                     const unsigned long long*, const float4*, uint32_t*, uint32_t*, uint32_t*,
-                                    uint32_t*, unsigned long long*, int, int);
+                                    uint32_t*, unsigned long long*, int, int, const float4*);
 extern "C" hipError_t rh_launch_sample(hipStream_t, int, int, const SampleLaunch*);
 extern "C" hipError_t rh_sample_occupancy(int, int*);
 extern "C" hipError_t rh_launch_plan_items(hipStream_t, const FrameDev*, const TaskDev*, int, const uint32_t*,
@@ -466,7 +466,8 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
                       S.candGrid.size(), c->dCapStart.as<unsigned long long>(), c->dMpbound.as<float4>(),
                       c->dSegCounts.as<uint32_t>(), c->dLists.as<uint32_t>(), c->dListCount.as<uint32_t>(),
                       c->dBucketRefs.as<uint32_t>(), c->dCounters.as<unsigned long long>(), c->binTaskSide,
-                      ((c->frame.bucketX & (c->frame.bucketX - 1)) == 0 && (c->frame.bucketY & (c->frame.bucketY - 1)) == 0) ? 1 : 0));
+                      ((c->frame.bucketX & (c->frame.bucketX - 1)) == 0 && (c->frame.bucketY & (c->frame.bucketY - 1)) == 0) ? 1 : 0,
+                      S.motion ? c->dMprecMB.as<float4>() : nullptr));
     if (timed) ENQ(hipEventRecord(c->ev[2], c->st));
     SampleLaunch a;
     a.fr = f; a.buckets = c->dBuckets.as<BucketDev>(); a.tasks = c->dTasks.as<TaskDev>();

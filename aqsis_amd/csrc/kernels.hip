@@ -54,6 +54,12 @@ __device__ __forceinline__ long lfloor_d(double x) {  // math.h:47-51
     long t = (long)x;
     return t - (x < 0 && x != (double)t);
 }
+// lfloor / lceil of math.h:47-61 to int where the result fits.
+__device__ __forceinline__ int lfloor_i(float x) {
+    if (fabsf(x) < 2147483648.0f) return (int)floorf(x);
+    long t = (long)x;
+    return (int)(t - (x < 0 && x != (float)t));
+}
 __device__ __forceinline__ float fmin_ref(float a, float b) { return (a < b) ? a : b; }
 __device__ __forceinline__ float fmax_ref(float a, float b) { return (a < b) ? b : a; }
 
@@ -63,6 +69,57 @@ struct f3d {
 __device__ __forceinline__ f3d mk3(float x, float y, float z) { f3d r; r.x = x; r.y = y; r.z = z; return r; }
 __device__ __forceinline__ f3d add3(f3d a, f3d b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
 __device__ __forceinline__ f3d mul3(f3d a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+
+// Moving micropolygons.  Key bounds as CqMovingMicroPolygonKey::GetBound
+// (micropolygon.cpp:1974-1989), kb[k] = {min x, min y, min z, max x, max y,
+// max z} of key k's four vertices.
+__device__ __forceinline__ void mb_key_bound(const f3d* q, float* kb) {
+    kb[0] = fmin_ref(q[0].x, fmin_ref(q[1].x, fmin_ref(q[2].x, q[3].x)));
+    kb[1] = fmin_ref(q[0].y, fmin_ref(q[1].y, fmin_ref(q[2].y, q[3].y)));
+    kb[2] = fmin_ref(q[0].z, fmin_ref(q[1].z, fmin_ref(q[2].z, q[3].z)));
+    kb[3] = fmax_ref(q[0].x, fmax_ref(q[1].x, fmax_ref(q[2].x, q[3].x)));
+    kb[4] = fmax_ref(q[0].y, fmax_ref(q[1].y, fmax_ref(q[2].y, q[3].y)));
+    kb[5] = fmax_ref(q[0].z, fmax_ref(q[1].z, fmax_ref(q[2].z, q[3].z)));
+}
+
+// CqMicroPolygonMotion::BuildBoundList's sub-bound count
+// (micropolygon.cpp:1697-1712): the raster move of the first vertex in key
+// 0's bound widths, at least 1, at most timeRanges.  lfloor of an infinite
+// or NaN ratio is LONG_MIN on x86-64, so the count is then 1.
+__device__ __forceinline__ int mb_divisions(const float* kb0, f3d a0, f3d a1, int timeRanges) {
+    const float wx = kb0[3] - kb0[0], wy = kb0[4] - kb0[1];
+    const float polyLen2 = (wy == 0.0f) ? wx * wx : ((wx == 0.0f) ? wy * wy : (wx * wx) + (wy * wy));
+    const float mxd = a0.x - a1.x, myd = a0.y - a1.y;
+    const float moveDist2 = (myd == 0.0f) ? mxd * mxd : ((mxd == 0.0f) ? myd * myd : (mxd * mxd) + (myd * myd));
+    const float lengths = sqrtf(moveDist2 / polyLen2);
+    int moved = (lengths < 9.2e18f) ? (lengths < 2147483648.0f ? lfloor_i(lengths) : 0x7fffffff) : 1;
+    if (moved < 1) moved = 1;
+    return moved < timeRanges ? moved : timeRanges;
+}
+
+// The key bounds mixed at `mix` (extrapolated outside [0, 1]), component c.
+__device__ __forceinline__ float mb_mix(const float (*kb)[6], int c, float mix) {
+    return kb[0][c] + mix * (kb[1][c] - kb[0][c]);
+}
+
+// The raster hull {min x, max x, min y, max y} of every sub-bound
+// BuildBoundList (micropolygon.cpp:1689-1756) makes: key 0's bound and the
+// mixed bounds at each sub-bound's end time, with the same time steps.
+__device__ __forceinline__ float4 mb_hull(const float (*kb)[6], int divisions, float T0, float T1, float opentime,
+                                          float closetime) {
+    const float dt = (closetime - opentime) / divisions;
+    float time = opentime + dt;
+    float4 h = make_float4(kb[0][0], kb[0][3], kb[0][1], kb[0][4]);
+    for (int i = 0; i < divisions; ++i) {
+        const float mix = (time - T0) / (T1 - T0);
+        h.x = fmin_ref(h.x, mb_mix(kb, 0, mix));
+        h.y = fmax_ref(h.y, mb_mix(kb, 3, mix));
+        h.z = fmin_ref(h.z, mb_mix(kb, 1, mix));
+        h.w = fmax_ref(h.w, mb_mix(kb, 4, mix));
+        time += dt;
+    }
+    return h;
+}
 
 // CqMatrix * CqVector3D (matrix.h:717-750); m[16] is [row][col] row-major.
 __device__ __forceinline__ f3d xform_point(const float* m, bool ident, f3d v) {
@@ -299,10 +356,11 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
         }
         // Motion frames: every micropolygon also gets the record the motion
         // sampler reads: both keys' vertices in grid order (a static
-        // micropolygon repeats key 0), flags, grid, key times, and its static
-        // bound.  A moving one is binned by m_Bound, key 0's bound
-        // encapsulating key 1's (AppendKey, micropolygon.cpp:1952-1967; key
-        // bounds as CqMovingMicroPolygonKey::GetBound, :1974-1989).
+        // micropolygon repeats key 0), flags, grid, key times, and its pixel
+        // window bound: the static bound, or for a moving one the hull of its
+        // sub-bounds, which reach past the keys when the shutter does.  A
+        // moving one meets AddMPG's bucket test with m_Bound, key 0's bound
+        // encapsulating key 1's (AppendKey, micropolygon.cpp:1952-1967).
         f3d K1[4] = {P[0], P[1], P[2], P[3]};
         if (motion) {
             float4 r0 = V1[idx], r1 = V1[idx + 1], r2 = V1[idx + nu], r3 = V1[idx + nu + 1];
@@ -319,21 +377,17 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
         const float t0 = motion ? prm1[key_param_floats(type, cu, cv)] : 0.0f;
         const float t1 = motion ? prm1[key_param_floats(type, cu, cv) + 1] : 0.0f;
         mb[6] = make_float4(__uint_as_float(flags | (motion ? RH_MB_MOVING : 0u)), __int_as_float((int)blockIdx.x), t0, t1);
-        mb[7] = make_float4(bminx, bmaxx, bminy, bmaxy);
         if (motion) {
             float kb[2][6];
-            const f3d* keys[2] = {P, K1};
-            for (int k = 0; k < 2; ++k) {
-                const f3d* q = keys[k];
-                kb[k][0] = fmin_ref(q[0].x, fmin_ref(q[1].x, fmin_ref(q[2].x, q[3].x)));
-                kb[k][1] = fmax_ref(q[0].x, fmax_ref(q[1].x, fmax_ref(q[2].x, q[3].x)));
-                kb[k][2] = fmin_ref(q[0].y, fmin_ref(q[1].y, fmin_ref(q[2].y, q[3].y)));
-                kb[k][3] = fmax_ref(q[0].y, fmax_ref(q[1].y, fmax_ref(q[2].y, q[3].y)));
-            }
-            mpbound[G.mpBase + m] = make_float4(fmin_ref(kb[0][0], kb[1][0]), fmax_ref(kb[0][1], kb[1][1]),
-                                                fmin_ref(kb[0][2], kb[1][2]), fmax_ref(kb[0][3], kb[1][3]));
+            mb_key_bound(P, kb[0]);
+            mb_key_bound(K1, kb[1]);
+            mpbound[G.mpBase + m] = make_float4(fmin_ref(kb[0][0], kb[1][0]), fmax_ref(kb[0][3], kb[1][3]),
+                                                fmin_ref(kb[0][1], kb[1][1]), fmax_ref(kb[0][4], kb[1][4]));
+            const int divisions = mb_divisions(kb[0], P[0], K1[0], max(4, fr->nx * fr->ny));
+            mb[7] = mb_hull(kb, divisions, t0, t1, fr->shutterOpen, fr->shutterClose);
         } else {
             mpbound[G.mpBase + m] = make_float4(bminx, bmaxx, bminy, bmaxy);
+            mb[7] = make_float4(bminx, bmaxx, bminy, bmaxy);
         }
     }
 }
@@ -372,25 +426,27 @@ __device__ __forceinline__ BinConst bin_const(const FrameDev* __restrict__ fr) {
     return k;
 }
 
-// AddMPG for bucket B (imagebuffer.cpp:514-591), then RenderMPG_Static's
-// pixel range clipped to the bucket's sample region
-// (bucketprocessor.cpp:1120-1140), relative to the region; empty ranges are
-// not kept.  Branch-free.  The bucket index clamps of AddMPG are left out:
+// AddMPG for bucket B (imagebuffer.cpp:514-591) with bound bd, then the
+// pixel range of window bound wd clipped to the bucket's sample region
+// (RenderMPG_Static, bucketprocessor.cpp:1120-1140; for a moving
+// micropolygon wd is the hull of the sub-bounds RenderMPG_MBOrDof visits,
+// :1290-1310), relative to the region; empty ranges are not kept.
+// Branch-free.  The bucket index clamps of AddMPG are left out:
 // B lies inside the bucket region, so they never change the outcome.  A
 // division by a power-of-two bucket size equals the multiplication by its
 // reciprocal (both round the same exact quotient).
 template <bool POW2>
-__device__ __forceinline__ bool bin_window(const BinConst& k, const BucketDev& B, const float4 bd, int& sX, int& eX,
-                                           int& sY, int& eY) {
+__device__ __forceinline__ bool bin_window(const BinConst& k, const BucketDev& B, const float4 bd, const float4 wd,
+                                           int& sX, int& eX, int& sY, int& eY) {
     const bool discard = (bd.y < k.dx0) | (bd.w < k.dy0) | (bd.x > k.dx1) | (bd.z > k.dy1);
     const float ex0 = bd.x - k.dsx, ey0 = bd.z - k.dsy, ex1 = bd.y + k.dsx, ey1 = bd.w + k.dsy;
     const int iXBa = (int)(POW2 ? ex0 * k.ibx : ex0 / k.bxs), iYBa = (int)(POW2 ? ey0 * k.iby : ey0 / k.bys);
     const int iXBb = (int)(POW2 ? ex1 * k.ibx : ex1 / k.bxs), iYBb = (int)(POW2 ? ey1 * k.iby : ey1 / k.bys);
     const bool inB = (iXBa <= B.bx) & (B.bx <= iXBb) & (iYBa <= B.by) & (B.by <= iYBb);
-    eX = min(lceil_i(bd.y), B.sx1);
-    eY = min(lceil_i(bd.w), B.sy1);
-    sX = max(floor_i(bd.x), B.sx0);
-    sY = max(floor_i(bd.z), B.sy0);
+    eX = min(lceil_i(wd.y), B.sx1);
+    eY = min(lceil_i(wd.w), B.sy1);
+    sX = max(floor_i(wd.x), B.sx0);
+    sY = max(floor_i(wd.z), B.sy0);
     const bool keep = !discard & inB & (sX < eX) & (sY < eY);
     sX -= B.sx0; eX -= B.sx0; sY -= B.sy0; eY -= B.sy0;
     return keep;
@@ -427,13 +483,14 @@ __device__ __forceinline__ void task_ranges(const TaskDev* __restrict__ tasks, c
 // Pass 1: per segment, the kept micropolygons per task and in all.  One
 // wave per segment at a time (grid-stride over segments), 64 micropolygons
 // per step; no barriers.  MT: the largest task grid side (2 or 4).
-template <int MT, bool POW2>
+template <int MT, bool POW2, bool MB>
 __global__ void __launch_bounds__(256) rh_bin_count(const FrameDev* __restrict__ fr,
                                                     const BucketDev* __restrict__ buckets,
                                                     const TaskDev* __restrict__ tasks, const GridDev* __restrict__ grids,
                                                     const uint32_t* __restrict__ candGrid,
                                                     const uint32_t* __restrict__ segBucket, size_t nseg,
                                                     const float4* __restrict__ mpbound,
+                                                    const float4* __restrict__ mprecMB,
                                                     uint32_t* __restrict__ segCounts) {
     const int lane = threadIdx.x & 63;
     const BinConst K = bin_const(fr);
@@ -449,9 +506,11 @@ __global__ void __launch_bounds__(256) rh_bin_count(const FrameDev* __restrict__
         for (int t = 0; t < MT * MT; ++t) cnt[t] = 0;
         for (int base = 0; base < nmp; base += 64) {
             const int m = base + lane;
-            const float4 bd = mpbound[G.mpBase + min(m, nmp - 1)];
+            const uint32_t id = G.mpBase + min(m, nmp - 1);
+            const float4 bd = mpbound[id];
+            const float4 wd = MB ? mprecMB[(size_t)id * (RH_MPREC_MB_FLOATS / 4) + 7] : bd;
             int sX, eX, sY, eY;
-            const bool keep = (m < nmp) & bin_window<POW2>(K, B, bd, sX, eX, sY, eY);
+            const bool keep = (m < nmp) & bin_window<POW2>(K, B, bd, wd, sX, eX, sY, eY);
             all += (uint32_t)__popcll(__ballot(keep));
             bool co[MT], ro[MT];
 #pragma unroll
@@ -524,13 +583,14 @@ __global__ void __launch_bounds__(256) rh_bin_scan(const BucketDev* __restrict__
 
 // Pass 3: per segment, each task's kept micropolygons in order at the
 // segment's offset.  One wave per segment at a time, as pass 1.
-template <int MT, bool POW2>
+template <int MT, bool POW2, bool MB>
 __global__ void __launch_bounds__(256) rh_bin_write(const FrameDev* __restrict__ fr,
                                                     const BucketDev* __restrict__ buckets,
                                                     const TaskDev* __restrict__ tasks, const GridDev* __restrict__ grids,
                                                     const uint32_t* __restrict__ candGrid,
                                                     const uint32_t* __restrict__ segBucket, size_t nseg,
                                                     const float4* __restrict__ mpbound,
+                                                    const float4* __restrict__ mprecMB,
                                                     const uint32_t* __restrict__ segCounts,
                                                     const unsigned long long* __restrict__ capStart,
                                                     uint32_t* __restrict__ lists) {
@@ -562,9 +622,11 @@ __global__ void __launch_bounds__(256) rh_bin_write(const FrameDev* __restrict__
             }
         for (int base = 0; base < nmp; base += 64) {
             const int m = base + lane;
-            const float4 bd = mpbound[G.mpBase + min(m, nmp - 1)];
+            const uint32_t id = G.mpBase + min(m, nmp - 1);
+            const float4 bd = mpbound[id];
+            const float4 wd = MB ? mprecMB[(size_t)id * (RH_MPREC_MB_FLOATS / 4) + 7] : bd;
             int sX, eX, sY, eY;
-            const bool keep = (m < nmp) & bin_window<POW2>(K, B, bd, sX, eX, sY, eY);
+            const bool keep = (m < nmp) & bin_window<POW2>(K, B, bd, wd, sX, eX, sY, eY);
             if (!__ballot(keep)) continue;
             bool co[MT], ro[MT];
 #pragma unroll
@@ -1196,13 +1258,6 @@ __device__ __forceinline__ MBHit quad_hit(const f3d* pin, uint32_t code, bool de
     return h;
 }
 
-// lfloor / lceil of math.h:47-61 to int where the result fits.
-__device__ __forceinline__ int lfloor_i(float x) {
-    if (fabsf(x) < 2147483648.0f) return (int)floorf(x);
-    long t = (long)x;
-    return (int)(t - (x < 0 && x != (float)t));
-}
-
 // The motion form of the sampler: one workgroup per task, one wave per
 // block of 64 samples (lanes in row-major sub-sample order, as rh_sample),
 // the task's list staged through LDS in chunks of CH records.  Each lane
@@ -1383,31 +1438,12 @@ __global__ void __launch_bounds__(256) rh_sample_mb(const FrameDev* __restrict__
                 const f3d K1[4] = {mk3(r3.x, r3.y, r3.z), mk3(r3.w, r4.x, r4.y), mk3(r4.z, r4.w, r5.x),
                                    mk3(r5.y, r5.z, r5.w)};
                 const float T0 = r6.z, T1 = r6.w;
-                // key bounds (CqMovingMicroPolygonKey::GetBound): x, y, z min and max
                 float kb[2][6];
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const f3d* q = k ? K1 : K0;
-                    kb[k][0] = fmin_ref(q[0].x, fmin_ref(q[1].x, fmin_ref(q[2].x, q[3].x)));
-                    kb[k][1] = fmin_ref(q[0].y, fmin_ref(q[1].y, fmin_ref(q[2].y, q[3].y)));
-                    kb[k][2] = fmin_ref(q[0].z, fmin_ref(q[1].z, fmin_ref(q[2].z, q[3].z)));
-                    kb[k][3] = fmax_ref(q[0].x, fmax_ref(q[1].x, fmax_ref(q[2].x, q[3].x)));
-                    kb[k][4] = fmax_ref(q[0].y, fmax_ref(q[1].y, fmax_ref(q[2].y, q[3].y)));
-                    kb[k][5] = fmax_ref(q[0].z, fmax_ref(q[1].z, fmax_ref(q[2].z, q[3].z)));
-                }
-                // BuildBoundList: sub-bounds from the raster move in widths
-                float polyLen2, moveDist2;
-                {
-                    const float wx = kb[0][3] - kb[0][0], wy = kb[0][4] - kb[0][1];
-                    polyLen2 = (wy == 0.0f) ? wx * wx : ((wx == 0.0f) ? wy * wy : (wx * wx) + (wy * wy));
-                    const float mxd = K0[0].x - K1[0].x, myd = K0[0].y - K1[0].y;
-                    moveDist2 = (myd == 0.0f) ? mxd * mxd : ((mxd == 0.0f) ? myd * myd : (mxd * mxd) + (myd * myd));
-                }
-                const float lengths = sqrtf(moveDist2 / polyLen2);
-                // lfloor of an infinite or NaN ratio is LONG_MIN on x86-64: max(1, .) = 1
-                int moved = (lengths < 9.2e18f) ? (lengths < 2147483648.0f ? lfloor_i(lengths) : 0x7fffffff) : 1;
-                if (moved < 1) moved = 1;
-                const int divisions = moved < timeRanges ? moved : timeRanges;
+                mb_key_bound(K0, kb[0]);
+                mb_key_bound(K1, kb[1]);
+                // BuildBoundList: each sub-bound encapsulates the bounds at
+                // its two end times
+                const int divisions = mb_divisions(kb[0], K0[0], K1[0], timeRanges);
                 const float dt = (closetime - opentime) / divisions;
                 float time = opentime + dt;
                 float bnd[6] = {kb[0][0], kb[0][1], kb[0][2], kb[0][3], kb[0][4], kb[0][5]};
@@ -1416,7 +1452,7 @@ __global__ void __launch_bounds__(256) rh_sample_mb(const FrameDev* __restrict__
                     float mid[6];
 #pragma unroll
                     for (int c = 0; c < 6; ++c) {
-                        mid[c] = kb[0][c] + mix * (kb[1][c] - kb[0][c]);
+                        mid[c] = mb_mix(kb, c, mix);
                         bnd[c] = c < 3 ? fmin_ref(bnd[c], mid[c]) : fmax_ref(bnd[c], mid[c]);
                     }
                     const float time0 = time - dt;
@@ -1995,45 +2031,57 @@ extern "C" hipError_t rh_launch_dice(hipStream_t st, const FrameDev* fr, const G
     return hipGetLastError();
 }
 
-template <int MT, bool POW2>
+template <int MT, bool POW2, bool MB>
 static void launch_bin_t(hipStream_t st, size_t wgs, const FrameDev* fr, const BucketDev* buckets, int nbuckets,
                          const TaskDev* tasks, const GridDev* grids, const uint32_t* candStart,
                          const uint32_t* candGrid, const uint32_t* segBucket, size_t nsegments,
-                         const unsigned long long* capStart, const float4* mpbound, uint32_t* segCounts,
-                         uint32_t* lists, uint32_t* listCount, uint32_t* bucketRefs, unsigned long long* counters) {
+                         const unsigned long long* capStart, const float4* mpbound, const float4* mprecMB,
+                         uint32_t* segCounts, uint32_t* lists, uint32_t* listCount, uint32_t* bucketRefs,
+                         unsigned long long* counters) {
     if (nsegments > 0)
-        hipLaunchKernelGGL((rh_bin_count<MT, POW2>), dim3((unsigned)wgs), dim3(256), 0, st, fr, buckets, tasks, grids,
-                           candGrid, segBucket, nsegments, mpbound, segCounts);
+        hipLaunchKernelGGL((rh_bin_count<MT, POW2, MB>), dim3((unsigned)wgs), dim3(256), 0, st, fr, buckets, tasks,
+                           grids, candGrid, segBucket, nsegments, mpbound, mprecMB, segCounts);
     hipLaunchKernelGGL(rh_bin_scan, dim3(nbuckets), dim3(256), 0, st, buckets, candStart, capStart, segCounts,
                        listCount, bucketRefs, counters);
     if (nsegments > 0)
-        hipLaunchKernelGGL((rh_bin_write<MT, POW2>), dim3((unsigned)wgs), dim3(256), 0, st, fr, buckets, tasks, grids,
-                           candGrid, segBucket, nsegments, mpbound, segCounts, capStart, lists);
+        hipLaunchKernelGGL((rh_bin_write<MT, POW2, MB>), dim3((unsigned)wgs), dim3(256), 0, st, fr, buckets, tasks,
+                           grids, candGrid, segBucket, nsegments, mpbound, mprecMB, segCounts, capStart, lists);
+}
+
+template <bool MB>
+static void launch_bin_mb(hipStream_t st, size_t wgs, int taskSide, int pow2, const FrameDev* fr,
+                          const BucketDev* buckets, int nbuckets, const TaskDev* tasks, const GridDev* grids,
+                          const uint32_t* candStart, const uint32_t* candGrid, const uint32_t* segBucket,
+                          size_t nsegments, const unsigned long long* capStart, const float4* mpbound,
+                          const float4* mprecMB, uint32_t* segCounts, uint32_t* lists, uint32_t* listCount,
+                          uint32_t* bucketRefs, unsigned long long* counters) {
+    auto* f = taskSide <= 2 ? (pow2 ? &launch_bin_t<2, true, MB> : &launch_bin_t<2, false, MB>)
+                            : (pow2 ? &launch_bin_t<4, true, MB> : &launch_bin_t<4, false, MB>);
+    f(st, wgs, fr, buckets, nbuckets, tasks, grids, candStart, candGrid, segBucket, nsegments, capStart, mpbound,
+      mprecMB, segCounts, lists, listCount, bucketRefs, counters);
 }
 
 // taskSide: the largest task grid side of any bucket (at most 4); pow2: both
-// bucket sizes are powers of two.
+// bucket sizes are powers of two; mprecMB: the motion records of a frame
+// with moving micropolygons (their pixel windows), else null.
 extern "C" hipError_t rh_launch_bin(hipStream_t st, const FrameDev* fr, const BucketDev* buckets, int nbuckets,
                                     const TaskDev* tasks, const GridDev* grids, const uint32_t* candStart,
                                     const uint32_t* candGrid, const uint32_t* segBucket, size_t nsegments,
                                     const unsigned long long* capStart, const float4* mpbound, uint32_t* segCounts,
                                     uint32_t* lists, uint32_t* listCount, uint32_t* bucketRefs,
-                                    unsigned long long* counters, int taskSide, int pow2) {
+                                    unsigned long long* counters, int taskSide, int pow2, const float4* mprecMB) {
     if (nbuckets <= 0) return hipSuccess;
     if (taskSide < 1 || taskSide > 4) return hipErrorInvalidValue;
     // enough waves to fill the device; each walks segments grid-stride
     const size_t wgs = std::min<size_t>((nsegments + 3) / 4, 8192);
-    if (taskSide <= 2) {
-        if (pow2) launch_bin_t<2, true>(st, wgs, fr, buckets, nbuckets, tasks, grids, candStart, candGrid, segBucket,
-                                        nsegments, capStart, mpbound, segCounts, lists, listCount, bucketRefs, counters);
-        else launch_bin_t<2, false>(st, wgs, fr, buckets, nbuckets, tasks, grids, candStart, candGrid, segBucket,
-                                    nsegments, capStart, mpbound, segCounts, lists, listCount, bucketRefs, counters);
-    } else {
-        if (pow2) launch_bin_t<4, true>(st, wgs, fr, buckets, nbuckets, tasks, grids, candStart, candGrid, segBucket,
-                                        nsegments, capStart, mpbound, segCounts, lists, listCount, bucketRefs, counters);
-        else launch_bin_t<4, false>(st, wgs, fr, buckets, nbuckets, tasks, grids, candStart, candGrid, segBucket,
-                                    nsegments, capStart, mpbound, segCounts, lists, listCount, bucketRefs, counters);
-    }
+    if (mprecMB)
+        launch_bin_mb<true>(st, wgs, taskSide, pow2, fr, buckets, nbuckets, tasks, grids, candStart, candGrid,
+                            segBucket, nsegments, capStart, mpbound, mprecMB, segCounts, lists, listCount, bucketRefs,
+                            counters);
+    else
+        launch_bin_mb<false>(st, wgs, taskSide, pow2, fr, buckets, nbuckets, tasks, grids, candStart, candGrid,
+                             segBucket, nsegments, capStart, mpbound, nullptr, segCounts, lists, listCount,
+                             bucketRefs, counters);
     return hipGetLastError();
 }
 
