@@ -78,7 +78,7 @@ class RhStats(C.Structure):
         ("surfaces", C.c_uint64), ("occlusion_culled", C.c_uint64),
         ("ms_schedule", C.c_double), ("ms_upload", C.c_double),
         ("ms_dice", C.c_double), ("ms_bin", C.c_double), ("ms_sample", C.c_double),
-        ("ms_filter", C.c_double), ("ms_device", C.c_double),
+        ("ms_filter", C.c_double), ("ms_device", C.c_double), ("ms_composite", C.c_double),
     ]
 
     def as_dict(self):

@@ -105,7 +105,7 @@ struct rh_ctx {
     int binTaskSide = 1;                  // largest task grid side of a bucket
     bool rendered = false;
     hipStream_t st = nullptr;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // frame shape
     DBuf dFrame, dBuckets, dTasks, dPos2d, dPos1d, dPix, dDither, dFilter, dBoxMask;
     bool tiledFilter = false;
@@ -210,7 +210,7 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
     keep_pool_memory();
     hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     c->each_buffer([c](DBuf& b) { b.st = c->st; });
-    for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
     if (e != hipSuccess) { delete c; return fail(RH_ERR_DEVICE, std::string("stream/event creation: ") + hipGetErrorString(e)); }
 
     FrameDev f;
@@ -509,6 +509,7 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
         rows = (nt + unit - 1) / unit * unit;
         ENQ(rh_launch_sample(c->st, variant, rows, &a));
     }
+    if (timed) ENQ(hipEventRecord(c->ev[5], c->st));
     if (c->transparent)
         ENQ(rh_launch_combine(c->st, f, (size_t)s.extW * s.extH * s.ns, c->dCombine.as<uint4>(),
                               c->dHitCtl.as<uint32_t>(), c->dHitData.as<float4>(), c->dHitNext.as<uint32_t>(),
@@ -574,9 +575,11 @@ static rh_status render_frame(rh_ctx* c, bool eager) {
                                              " transparent hits");
         break;
     }
-    float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (!graphed)
+    float t[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tComp = 0.0f;
+    if (!graphed) {
         for (int i = 0; i < 4; ++i) HIP_TRY(hipEventElapsedTime(&t[i], c->ev[i], c->ev[i + 1]));
+        HIP_TRY(hipEventElapsedTime(&tComp, c->ev[5], c->ev[3]));
+    }
     c->stats.spl_count = cnt[0];
     c->stats.spl_bound_hits = cnt[1];
     c->stats.spl_hits = cnt[2];
@@ -590,6 +593,7 @@ static rh_status render_frame(rh_ctx* c, bool eager) {
     c->stats.ms_bin = t[1];
     c->stats.ms_sample = t[2];
     c->stats.ms_filter = t[3];
+    c->stats.ms_composite = tComp;
     float total = 0;
     HIP_TRY(hipEventElapsedTime(&total, c->ev[0], c->ev[4]));
     c->stats.ms_device = total;

@@ -1187,7 +1187,9 @@ struct MBHit {
     float D, u, v;
 };
 
-__device__ __forceinline__ MBHit quad_hit(const f3d* pin, uint32_t code, bool degen, float px, float py) {
+__device__ __forceinline__ MBHit quad_hit(const f3d p0, const f3d p1, const f3d p2, const f3d p3, uint32_t code,
+                                          bool degen, float px, float py) {
+    const f3d pin[4] = {p0, p1, p2, p3};
     MBHit h;
     h.in = false;
     h.D = 0.f; h.u = 0.5f; h.v = 0.5f;
@@ -1201,9 +1203,8 @@ __device__ __forceinline__ MBHit quad_hit(const f3d* pin, uint32_t code, bool de
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int o = i < 3 ? (int)(code >> (2 * (i + 1))) & 3 : (int)code & 3;
-        const f3d q = o == 0 ? pin[0] : (o == 1 ? pin[1] : (o == 2 ? pin[2] : pin[3]));
-        qx[i] = q.x;
-        qy[i] = q.y;
+        qx[i] = o == 0 ? p0.x : (o == 1 ? p1.x : (o == 2 ? p2.x : p3.x));
+        qy[i] = o == 0 ? p0.y : (o == 1 ? p1.y : (o == 2 ? p2.y : p3.y));
     }
     float YM[4], XM[4], X[4], Y[4];
 #pragma unroll
@@ -1258,42 +1259,68 @@ __device__ __forceinline__ MBHit quad_hit(const f3d* pin, uint32_t code, bool de
     return h;
 }
 
-// The motion form of the sampler: one workgroup per task, one wave per
-// block of 64 samples (lanes in row-major sub-sample order, as rh_sample),
-// the task's list staged through LDS in chunks of CH records.  Each lane
-// walks the list in order and runs, for its own sample, exactly the visits
-// the reference makes:
-//  * a static micropolygon: RenderMPG_Static (bucketprocessor.cpp:1097-1218);
-//  * a moving one: RenderMPG_MBOrDof without DoF (:1221-1469) over the
-//    sub-bounds of CqMicroPolygonMotion::BuildBoundList
-//    (micropolygon.cpp:1689-1756): a sub-bound visits the samples of its
-//    pixels whose index lies in the window its times give, each visit
-//    counted as a sample test, then the time window, bound, occlusion and
-//    LOD tests and CqMicroPolygonMotion::Sample (:1768-1914): the tight
-//    bound and the vertices at the sample's time, and the point-in-polygon
-//    test on them.  A window reaching past a pixel's samples stops at the
-//    last one.
-template <int CH>
-__global__ void __launch_bounds__(256) rh_sample_mb(const FrameDev* __restrict__ fr,
-                                                    const BucketDev* __restrict__ buckets,
-                                                    const TaskDev* __restrict__ tasks, int ntasks,
-                                                    const unsigned long long* __restrict__ capStart,
-                                                    const uint32_t* __restrict__ listCount,
-                                                    const uint32_t* __restrict__ lists,
-                                                    const float4* __restrict__ mprecMB,
-                                                    const GridDev* __restrict__ grids,
-                                                    const PrimDev* __restrict__ prims,
-                                                    const float* __restrict__ pos2d, const float* __restrict__ pos1d,
-                                                    const uchar4* __restrict__ pixPattern,
-                                                    float4* __restrict__ samples, float4* __restrict__ hitData,
-                                                    uint32_t* __restrict__ hitNext, uint32_t* __restrict__ hitCounter,
-                                                    uint4* __restrict__ combineList, float* __restrict__ frontDepth,
-                                                    uint32_t* __restrict__ rowCounts, int rowBase) {
-    __shared__ float4 recs[CH * 8];
-    __shared__ uint32_t ids[CH];
-    __shared__ uint32_t wsum[RH_S_WAVES][5];
+// The motion form of the sampler.  A frame with moving micropolygons visits
+// samples per sub-bound (RenderMPG_MBOrDof, bucketprocessor.cpp:1221-1469):
+// a sub-bound of CqMicroPolygonMotion::BuildBoundList
+// (micropolygon.cpp:1689-1756) visits the samples of its pixels whose index
+// lies in the window its times give (sample i's time lies in the i-th of ns
+// equal parts of the shutter), each visit counted as a sample test; then the
+// time window, bound, occlusion and LOD tests and CqMicroPolygonMotion::Sample
+// (:1768-1914): the tight bound and the vertices at the sample's time, and
+// the point-in-polygon test on them.  A static micropolygon in the same frame
+// is RenderMPG_Static (:1097-1218).  A window reaching past a pixel's
+// samples stops at the last one.
+//
+// One workgroup of NW waves per task.  The task's list is taken in chunks of
+// records staged in LDS; for each chunk:
+//  1. every (record, sub-bound) pair is evaluated once for the task, by
+//     all threads together, and the pairs whose visits meet the task become
+//     entries, kept in pair order (the reference's visiting order): the
+//     sub-sample rectangle, the sample index window and the sub-bound's two
+//     end times;
+//  2. each wave holds one block of samples (lanes in row-major sub-sample
+//     order, as rh_sample), gathers the entries meeting its block into a
+//     buffer of up to 64, and turns them into one 64-bit mask per lane of the
+//     entries that visit the lane's sample; the lanes then take their own
+//     entries lowest bit first, so each sample sees its visits in order and a
+//     wave runs as many visit steps as its busiest lane has visits.
+#define RH_MB_RECS 128     // records staged per chunk
+#define RH_MB_PAIRS 2048   // (record, sub-bound) pairs per chunk at most
+
+template <int NW>
+__global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restrict__ fr,
+                                                        const BucketDev* __restrict__ buckets,
+                                                        const TaskDev* __restrict__ tasks, int ntasks,
+                                                        const unsigned long long* __restrict__ capStart,
+                                                        const uint32_t* __restrict__ listCount,
+                                                        const uint32_t* __restrict__ lists,
+                                                        const float4* __restrict__ mprecMB,
+                                                        const GridDev* __restrict__ grids,
+                                                        const PrimDev* __restrict__ prims,
+                                                        const float* __restrict__ pos2d, const float* __restrict__ pos1d,
+                                                        const uchar4* __restrict__ pixPattern,
+                                                        float4* __restrict__ samples, float4* __restrict__ hitData,
+                                                        uint32_t* __restrict__ hitNext, uint32_t* __restrict__ hitCounter,
+                                                        uint4* __restrict__ combineList, float* __restrict__ frontDepth,
+                                                        uint32_t* __restrict__ rowCounts, int rowBase) {
+    __shared__ float4 recs[RH_MB_RECS * 8];
+    __shared__ float kbs[RH_MB_RECS][12];          // key bounds, kb[k][c] at [k * 6 + c]
+    __shared__ float dts[RH_MB_RECS];
+    __shared__ int divs[RH_MB_RECS];
+    __shared__ int offs[RH_MB_RECS + 1];
+    __shared__ uint32_t ids[RH_MB_RECS];
+    // entries: record | sub-bound << 8 (0xff: static) | index window << 16;
+    // sub-sample x and y ranges (lo | hi << 16); the sub-bound's previous and
+    // own times
+    __shared__ uint32_t eA[RH_MB_PAIRS], eX[RH_MB_PAIRS], eY[RH_MB_PAIRS];
+    __shared__ float eTp[RH_MB_PAIRS], eTc[RH_MB_PAIRS];
+    __shared__ uint16_t buf[NW][128];
+    __shared__ uint32_t wtot[NW];
+    __shared__ int nfitShared;
+    __shared__ uint32_t wsum[NW][5];
     const int task = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const unsigned long long lanesBelow = (1ull << lane) - 1ull;
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
     uint32_t wbBase = 0;
     int wbLeft = 0;
@@ -1317,14 +1344,19 @@ __global__ void __launch_bounds__(256) rh_sample_mb(const FrameDev* __restrict__
     const float timePerSample = fastShutter ? 0.0f : ns / (closetime - opentime);
     const int timeRanges = max(4, nx * ny);
     const float clipNear = fr->clipNear, clipFar = fr->clipFar;
+    // the task in sub-sample units of the bucket's sample region
+    const int tgx0 = (T.x0 - B.sx0) * nx, tgx1 = (T.x1 - B.sx0) * nx;
+    const int tgy0 = (T.y0 - B.sy0) * ny, tgy1 = (T.y1 - B.sy0) * ny;
 
-    for (int p0 = 0; p0 < nbx * nby; p0 += RH_S_WAVES) {
+    for (int p0 = 0; p0 < nbx * nby; p0 += NW) {
         const int blk = p0 + wave;
         const bool has = blk < nbx * nby;       // uniform per wave
         const int bx = T.x0 + (blk % nbx) * bw, by = T.y0 + (blk / nbx) * bh;
         const int px = bx + lx, py = by + ly;
         const bool live = has && laneOk && px < T.x1 && py < T.y1;
         const int gx = (px - B.sx0) * nx + sm, gy = (py - B.sy0) * ny + sn;
+        const int bgx0 = (bx - B.sx0) * nx, bgx1 = (min(bx + bw, T.x1) - B.sx0) * nx;
+        const int bgy0 = (by - B.sy0) * ny, bgy1 = (min(by + bh, T.y1) - B.sy0) * ny;
         float sx = 0.f, sy = 0.f, stime = 0.f;
         bool lodOk = false;
         if (live) {
@@ -1340,236 +1372,356 @@ __global__ void __launch_bounds__(256) rh_sample_mb(const FrameDev* __restrict__
         uint32_t win = 0xffffffffu, head = 0xffffffffu;
         int nh = 0;
 
-        for (uint32_t c0 = 0; c0 < n; c0 += CH) {
-            const int cnt = (int)((n - c0) < (uint32_t)CH ? (n - c0) : (uint32_t)CH);
+        for (uint32_t c0 = 0; c0 < n;) {
+            const int cnt = (int)((n - c0) < (uint32_t)RH_MB_RECS ? (n - c0) : (uint32_t)RH_MB_RECS);
             __syncthreads();
-            for (int q = tid; q < cnt * 8; q += blockDim.x) {
+            for (int q = tid; q < cnt * 8; q += NW * 64) {
                 const uint32_t id = L[c0 + q / 8];
                 recs[q] = mprecMB[(size_t)id * 8 + (q & 7)];
                 if ((q & 7) == 0) ids[q / 8] = id;
             }
             __syncthreads();
-            if (!has) continue;
-            for (int j = 0; j < cnt; ++j) {
-                const float4* R = &recs[j * 8];
-                const float4 r0 = R[0], r1 = R[1], r2 = R[2], r6 = R[6], r7 = R[7];
-                const uint32_t flags = __float_as_uint(r6.x);
-                const uint32_t code = flags & 0xffu;
-                const bool degen = (flags >> 8) & 1u;
-                const bool opaque = (flags >> 10) & 1u;
-                const f3d K0[4] = {mk3(r0.x, r0.y, r0.z), mk3(r0.w, r1.x, r1.y), mk3(r1.z, r1.w, r2.x),
-                                   mk3(r2.y, r2.z, r2.w)};
-                if (!(flags & RH_MB_MOVING)) {
-                    // RenderMPG_Static: the candidate samples of the bound's
-                    // pixel range with the first/last pixel sub-sample clip
-                    const float bminx = r7.x, bmaxx = r7.y, bminy = r7.z, bmaxy = r7.w;
-                    // CalculateBound's min z: min(min(min(A,B),C),D), A..D = P0,P1,P3,P2
-                    const float bminz = fmin_ref(fmin_ref(fmin_ref(K0[0].z, K0[1].z), K0[3].z), K0[2].z);
-                    int eX = lceil_i(bmaxx), eY = lceil_i(bmaxy);
-                    if (eX > B.sx1) eX = B.sx1;
-                    if (eY > B.sy1) eY = B.sy1;
-                    int sX = floor_i(bminx), sY = floor_i(bminy);
-                    if (sY < B.sy0) sY = B.sy0;
-                    if (sX < B.sx0) sX = B.sx0;
-                    bool cand = false;
-                    if (sX < eX && sY < eY) {
-                        const int im = (bminx < sX) ? 0 : floor_i((bminx - sX) * nx);
-                        const int in = (bminy < sY) ? 0 : floor_i((bminy - sY) * ny);
-                        const int em = (bmaxx > eX) ? nx : lceil_i((bmaxx - (eX - 1)) * nx);
-                        const int en = (bmaxy > eY) ? ny : lceil_i((bmaxy - (eY - 1)) * ny);
-                        const int gx0 = (sX - B.sx0) * nx + im, gx1 = (eX - 1 - B.sx0) * nx + em;
-                        const int gy0 = (sY - B.sy0) * ny + in, gy1 = (eY - 1 - B.sy0) * ny + en;
-                        cand = live && gx >= gx0 && gx < gx1 && gy >= gy0 && gy < gy1;
-                    }
-                    bool hit = false;
-                    float D = 0.f, u = 0.f, v = 0.f;
-                    if (cand) {
-                        ++nTest;
-                        const bool inb = !(sx < bminx || sx > bmaxx || sy < bminy || sy > bmaxy);
-                        if (inb && !(bminz > occlZ) && lodOk) {
-                            ++nBound;
-                            const MBHit h = quad_hit(K0, code, degen, sx, sy);
-                            if (h.in) {
-                                ++nIn;
-                                if (!h.linear) ++nIn2;
-                                hit = !(occlZ <= h.D);
-                                D = h.D; u = h.u; v = h.v;
-                            }
-                        }
-                    }
-                    // StoreSample (bucketprocessor.cpp:1471-1569)
-                    nHit += hit ? 1u : 0u;
-                    const bool opq = hit && opaque;
-                    occlZ = opq ? D : occlZ;
-                    win = opq ? ids[j] : win;
-                    wu = opq ? u : wu;
-                    wv = opq ? v : wv;
-                    const bool app = hit && !opaque;
-                    const unsigned long long am = __ballot(app);
-                    if (am) {
-                        const int na = __popcll(am);
-                        if (na > wbLeft) {
-                            uint32_t base = 0;
-                            if (lane == 0) base = atomicAdd(hitCounter, 64u);
-                            wbBase = __shfl(base, 0);
-                            wbLeft = 64;
-                        }
-                        if (app) {
-                            const uint32_t slot =
-                                wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & ((1ull << lane) - 1ull));
-                            if (slot < fr->hitCapacity && nh < RH_MAX_HITS - 1) {
-                                float4 hA, hB;
-                                shade_hit(prims[grids[__float_as_int(r6.y)].prim], u, v, D, ids[j], hA, hB);
-                                hitData[2 * (size_t)slot] = hA;
-                                hitData[2 * (size_t)slot + 1] = hB;
-                                hitNext[slot] = head;
-                                head = slot;
-                                ++nh;
-                            } else if (slot < fr->hitCapacity) {
-                                lost = true;
-                            }
-                        }
-                        wbLeft -= na;
-                    }
-                    continue;
-                }
-                // A moving micropolygon.
-                const float4 r3 = R[3], r4 = R[4], r5 = R[5];
-                const f3d K1[4] = {mk3(r3.x, r3.y, r3.z), mk3(r3.w, r4.x, r4.y), mk3(r4.z, r4.w, r5.x),
-                                   mk3(r5.y, r5.z, r5.w)};
-                const float T0 = r6.z, T1 = r6.w;
-                float kb[2][6];
-                mb_key_bound(K0, kb[0]);
-                mb_key_bound(K1, kb[1]);
-                // BuildBoundList: each sub-bound encapsulates the bounds at
-                // its two end times
-                const int divisions = mb_divisions(kb[0], K0[0], K1[0], timeRanges);
-                const float dt = (closetime - opentime) / divisions;
-                float time = opentime + dt;
-                float bnd[6] = {kb[0][0], kb[0][1], kb[0][2], kb[0][3], kb[0][4], kb[0][5]};
-                for (int i = 0; i < divisions; ++i) {
-                    const float mix = (time - T0) / (T1 - T0);
-                    float mid[6];
+            // per record: key bounds, sub-bound count and time step
+            if (tid < cnt) {
+                const float4* R = &recs[tid * 8];
+                const uint32_t flags = __float_as_uint(R[6].x);
+                int d = 1;
+                float dtv = 0.0f;
+                if (flags & RH_MB_MOVING) {
+                    const float4 r0 = R[0], r1 = R[1], r2 = R[2], r3 = R[3], r4 = R[4], r5 = R[5];
+                    const f3d K0[4] = {mk3(r0.x, r0.y, r0.z), mk3(r0.w, r1.x, r1.y), mk3(r1.z, r1.w, r2.x),
+                                       mk3(r2.y, r2.z, r2.w)};
+                    const f3d K1[4] = {mk3(r3.x, r3.y, r3.z), mk3(r3.w, r4.x, r4.y), mk3(r4.z, r4.w, r5.x),
+                                       mk3(r5.y, r5.z, r5.w)};
+                    float kb[2][6];
+                    mb_key_bound(K0, kb[0]);
+                    mb_key_bound(K1, kb[1]);
 #pragma unroll
                     for (int c = 0; c < 6; ++c) {
-                        mid[c] = mb_mix(kb, c, mix);
-                        bnd[c] = c < 3 ? fmin_ref(bnd[c], mid[c]) : fmax_ref(bnd[c], mid[c]);
+                        kbs[tid][c] = kb[0][c];
+                        kbs[tid][6 + c] = kb[1][c];
                     }
-                    const float time0 = time - dt;
-                    const float time1 = (i != divisions - 1) ? (time + dt) - dt : closetime;
-                    const float sbminx = bnd[0], sbminy = bnd[1], sbminz = bnd[2], sbmaxx = bnd[3],
-                                sbmaxy = bnd[4], sbmaxz = bnd[5];
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) bnd[c] = mid[c];
-                    time += dt;
-                    if (time1 < opentime || time0 > closetime) continue;
-                    int indexT0, indexT1;
-                    if (fastShutter) {
-                        indexT0 = 0;
-                        indexT1 = ns;
+                    d = mb_divisions(kb[0], K0[0], K1[0], timeRanges);
+                    dtv = (closetime - opentime) / d;
+                }
+                divs[tid] = d;
+                dts[tid] = dtv;
+            }
+            __syncthreads();
+            // pair offsets, and the records whose pairs fit in one pass
+            if (wave == 0) {
+                const int k0 = 2 * lane, k1 = k0 + 1;
+                const int d0 = k0 < cnt ? divs[k0] : 0, d1 = k1 < cnt ? divs[k1] : 0;
+                int incl = d0 + d1;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int y = __shfl_up(incl, off);
+                    if (lane >= off) incl += y;
+                }
+                const int excl = incl - d0 - d1;
+                offs[k0] = excl;
+                offs[k1] = excl + d0;
+                if (lane == 63) offs[RH_MB_RECS] = incl;
+                int fit = (k0 < cnt && excl + d0 <= RH_MB_PAIRS ? 1 : 0) + (k1 < cnt && incl <= RH_MB_PAIRS ? 1 : 0);
+                for (int off = 32; off > 0; off >>= 1) fit += __shfl_xor(fit, off);
+                if (lane == 0) nfitShared = fit;
+            }
+            __syncthreads();
+            const int nrec = nfitShared;
+            const int npairs = offs[nrec];
+            // 1. entries in pair order
+            int nent = 0;
+            for (int pp = 0; pp < npairs; pp += NW * 64) {
+                const int p = pp + tid;
+                bool keep = false;
+                uint32_t ea = 0, ex = 0, ey = 0;
+                float tp = 0.0f, tc = 0.0f;
+                if (p < npairs) {
+                    int lo = 0, hi = nrec - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (offs[mid] <= p) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const int r = lo, b = p - offs[r];
+                    const float4* R = &recs[r * 8];
+                    const float4 r6 = R[6];
+                    const uint32_t flags = __float_as_uint(r6.x);
+                    if (!(flags & RH_MB_MOVING)) {
+                        // RenderMPG_Static's candidate samples: the bound's
+                        // pixel range with the first/last pixel sub-sample clip
+                        const float4 r7 = R[7];
+                        const float bminx = r7.x, bmaxx = r7.y, bminy = r7.z, bmaxy = r7.w;
+                        int eX = lceil_i(bmaxx), eY = lceil_i(bmaxy);
+                        if (eX > B.sx1) eX = B.sx1;
+                        if (eY > B.sy1) eY = B.sy1;
+                        int sX = floor_i(bminx), sY = floor_i(bminy);
+                        if (sY < B.sy0) sY = B.sy0;
+                        if (sX < B.sx0) sX = B.sx0;
+                        if (sX < eX && sY < eY) {
+                            const int im = (bminx < sX) ? 0 : floor_i((bminx - sX) * nx);
+                            const int in = (bminy < sY) ? 0 : floor_i((bminy - sY) * ny);
+                            const int em = (bmaxx > eX) ? nx : lceil_i((bmaxx - (eX - 1)) * nx);
+                            const int en = (bmaxy > eY) ? ny : lceil_i((bmaxy - (eY - 1)) * ny);
+                            const int gx0 = max((sX - B.sx0) * nx + im, tgx0), gx1 = min((eX - 1 - B.sx0) * nx + em, tgx1);
+                            const int gy0 = max((sY - B.sy0) * ny + in, tgy0), gy1 = min((eY - 1 - B.sy0) * ny + en, tgy1);
+                            keep = gx0 < gx1 && gy0 < gy1;
+                            ea = (uint32_t)r | (0xffu << 8) | ((uint32_t)ns << 24);
+                            ex = (uint32_t)gx0 | ((uint32_t)gx1 << 16);
+                            ey = (uint32_t)gy0 | ((uint32_t)gy1 << 16);
+                        }
                     } else {
-                        indexT0 = lfloor_i((time0 - opentime) * timePerSample);
-                        if (indexT0 < 0) indexT0 = 0;
-                        indexT1 = lceil_i((time1 - opentime) * timePerSample);
-                    }
-                    if (sbminz > clipFar || sbmaxz < clipNear) continue;
-                    int eX = lceil_i(sbmaxx), eY = lceil_i(sbmaxy);
-                    if (eX > B.sx1) eX = B.sx1;
-                    if (eY > B.sy1) eY = B.sy1;
-                    int sX = floor_i(sbminx), sY = floor_i(sbminy);
-                    if (sY < B.sy0) sY = B.sy0;
-                    if (sX < B.sx0) sX = B.sx0;
-                    if (sX >= eX || sY >= eY) continue;
-                    const int idxEnd = indexT1 > indexT0 + 1 ? indexT1 : indexT0 + 1;
-                    const bool visit = live && px >= sX && px < eX && py >= sY && py < eY && si >= indexT0 &&
-                                       si < idxEnd && si < ns;
-                    bool hit = false;
-                    float D = 0.f, u = 0.f, v = 0.f;
-                    if (visit) {
-                        ++nTest;
-                        const bool inb = !(sx < sbminx || sx > sbmaxx || sy < sbminy || sy > sbmaxy);
-                        if (!(stime < time0 || stime > time1) && inb && !(sbminz > occlZ) && lodOk) {
-                            ++nBound;
-                            // CqMicroPolygonMotion::Sample: the key span of the time
-                            int iIndex = 0;
-                            float Fraction = 0.0f;
-                            bool Exact = true;
-                            if (stime > T0) {
-                                if (stime >= T1) {
-                                    iIndex = 1;
-                                } else {
-                                    Fraction = (stime - T0) / (T1 - T0);
-                                    Exact = (T0 == stime);
-                                }
-                            }
-                            float tb[4];
-                            if (Exact) {
-                                tb[0] = kb[iIndex][0]; tb[1] = kb[iIndex][3]; tb[2] = kb[iIndex][1]; tb[3] = kb[iIndex][4];
+                        const float* kb0 = &kbs[r][0];
+                        const float* kb1 = &kbs[r][6];
+                        const float T0 = r6.z, T1 = r6.w;
+                        const int d = divs[r];
+                        const float dt = dts[r];
+                        float time = opentime + dt;
+                        for (int i = 0; i < b; ++i) {
+                            tp = time;
+                            time += dt;
+                        }
+                        tc = time;
+                        const float mixc = (tc - T0) / (T1 - T0);
+                        const float mixp = (tp - T0) / (T1 - T0);
+                        float sb[6];
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) {
+                            const float cur = kb0[c] + mixc * (kb1[c] - kb0[c]);
+                            const float prev = b == 0 ? kb0[c] : kb0[c] + mixp * (kb1[c] - kb0[c]);
+                            sb[c] = c < 3 ? fmin_ref(prev, cur) : fmax_ref(prev, cur);
+                        }
+                        const float time0 = tc - dt;
+                        const float time1 = (b != d - 1) ? (tc + dt) - dt : closetime;
+                        if (!(time1 < opentime || time0 > closetime) && !(sb[2] > clipFar || sb[5] < clipNear)) {
+                            int indexT0, indexT1;
+                            if (fastShutter) {
+                                indexT0 = 0;
+                                indexT1 = ns;
                             } else {
-                                const float F1 = 1 - Fraction;
-                                tb[0] = F1 * kb[0][0] + Fraction * kb[1][0];
-                                tb[1] = F1 * kb[0][3] + Fraction * kb[1][3];
-                                tb[2] = F1 * kb[0][1] + Fraction * kb[1][1];
-                                tb[3] = F1 * kb[0][4] + Fraction * kb[1][4];
+                                indexT0 = lfloor_i((time0 - opentime) * timePerSample);
+                                if (indexT0 < 0) indexT0 = 0;
+                                indexT1 = lceil_i((time1 - opentime) * timePerSample);
                             }
-                            if (!(sx < tb[0] || sx > tb[1] || sy < tb[2] || sy > tb[3])) {
-                                f3d pts[4];
-                                if (Exact) {
-#pragma unroll
-                                    for (int q = 0; q < 4; ++q) pts[q] = iIndex ? K1[q] : K0[q];
-                                } else {
-                                    const float F1 = 1.0f - Fraction;
-#pragma unroll
-                                    for (int q = 0; q < 4; ++q)
-                                        pts[q] = mk3((F1 * K0[q].x) + (Fraction * K1[q].x),
-                                                     (F1 * K0[q].y) + (Fraction * K1[q].y),
-                                                     (F1 * K0[q].z) + (Fraction * K1[q].z));
-                                }
-                                const MBHit h = quad_hit(pts, code, degen, sx, sy);
-                                if (h.in) {
-                                    ++nIn;
-                                    if (!h.linear) ++nIn2;
-                                    hit = !(occlZ <= h.D);
-                                    D = h.D; u = h.u; v = h.v;
-                                }
+                            int eX = lceil_i(sb[3]), eY = lceil_i(sb[4]);
+                            if (eX > B.sx1) eX = B.sx1;
+                            if (eY > B.sy1) eY = B.sy1;
+                            int sX = floor_i(sb[0]), sY = floor_i(sb[1]);
+                            if (sY < B.sy0) sY = B.sy0;
+                            if (sX < B.sx0) sX = B.sx0;
+                            if (sX < eX && sY < eY && indexT0 < ns) {
+                                int idxEnd = indexT1 > indexT0 + 1 ? indexT1 : indexT0 + 1;
+                                if (idxEnd > ns) idxEnd = ns;
+                                const int gx0 = max((sX - B.sx0) * nx, tgx0), gx1 = min((eX - B.sx0) * nx, tgx1);
+                                const int gy0 = max((sY - B.sy0) * ny, tgy0), gy1 = min((eY - B.sy0) * ny, tgy1);
+                                keep = gx0 < gx1 && gy0 < gy1;
+                                ea = (uint32_t)r | ((uint32_t)b << 8) | ((uint32_t)indexT0 << 16) |
+                                     ((uint32_t)idxEnd << 24);
+                                ex = (uint32_t)gx0 | ((uint32_t)gx1 << 16);
+                                ey = (uint32_t)gy0 | ((uint32_t)gy1 << 16);
                             }
                         }
-                    }
-                    nHit += hit ? 1u : 0u;
-                    const bool opq = hit && opaque;
-                    occlZ = opq ? D : occlZ;
-                    win = opq ? ids[j] : win;
-                    wu = opq ? u : wu;
-                    wv = opq ? v : wv;
-                    const bool app = hit && !opaque;
-                    const unsigned long long am = __ballot(app);
-                    if (am) {
-                        const int na = __popcll(am);
-                        if (na > wbLeft) {
-                            uint32_t base = 0;
-                            if (lane == 0) base = atomicAdd(hitCounter, 64u);
-                            wbBase = __shfl(base, 0);
-                            wbLeft = 64;
-                        }
-                        if (app) {
-                            const uint32_t slot =
-                                wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & ((1ull << lane) - 1ull));
-                            if (slot < fr->hitCapacity && nh < RH_MAX_HITS - 1) {
-                                float4 hA, hB;
-                                shade_hit(prims[grids[__float_as_int(r6.y)].prim], u, v, D, ids[j], hA, hB);
-                                hitData[2 * (size_t)slot] = hA;
-                                hitData[2 * (size_t)slot + 1] = hB;
-                                hitNext[slot] = head;
-                                head = slot;
-                                ++nh;
-                            } else if (slot < fr->hitCapacity) {
-                                lost = true;
-                            }
-                        }
-                        wbLeft -= na;
                     }
                 }
+                const unsigned long long m = __ballot(keep);
+                if (lane == 0) wtot[wave] = (uint32_t)__popcll(m);
+                __syncthreads();
+                int before = nent, total = 0;
+                for (int w = 0; w < NW; ++w) {
+                    const int v = (int)wtot[w];
+                    if (w < wave) before += v;
+                    total += v;
+                }
+                if (keep) {
+                    const int pos = before + __popcll(m & lanesBelow);
+                    eA[pos] = ea; eX[pos] = ex; eY[pos] = ey; eTp[pos] = tp; eTc[pos] = tc;
+                }
+                nent += total;
+                __syncthreads();
             }
+            // 2. the block's visits
+            if (has) {
+                int fill = 0, g = 0;
+                for (;;) {
+                    while (fill < 64 && g < nent) {
+                        const int e = g + lane;
+                        bool rel = false;
+                        if (e < nent) {
+                            const uint32_t xr = eX[e], yr = eY[e];
+                            rel = ((int)(xr & 0xffffu) < bgx1) & ((int)(xr >> 16) > bgx0) &
+                                  ((int)(yr & 0xffffu) < bgy1) & ((int)(yr >> 16) > bgy0);
+                        }
+                        const unsigned long long m = __ballot(rel);
+                        if (rel) buf[wave][fill + __popcll(m & lanesBelow)] = (uint16_t)e;
+                        fill += __popcll(m);
+                        g += 64;
+                    }
+                    if (fill == 0) break;
+                    const int nq = fill < 64 ? fill : 64;
+                    // the entries visiting this lane's sample, one bit each
+                    unsigned long long vm = 0;
+                    for (int q = 0; q < nq; ++q) {
+                        const int e = buf[wave][q];
+                        const uint32_t a = eA[e], xr = eX[e], yr = eY[e];
+                        const bool in = live & (gx >= (int)(xr & 0xffffu)) & (gx < (int)(xr >> 16)) &
+                                        (gy >= (int)(yr & 0xffffu)) & (gy < (int)(yr >> 16)) &
+                                        (si >= (int)((a >> 16) & 0xffu)) & (si < (int)(a >> 24));
+                        vm |= (unsigned long long)in << q;
+                    }
+                    while (__ballot(vm != 0)) {
+                        const bool act = vm != 0;
+                        const int q = act ? __ffsll((long long)vm) - 1 : 0;
+                        vm &= vm - 1;
+                        bool hit = false, opaque = false;
+                        float D = 0.f, u = 0.f, v = 0.f;
+                        uint32_t hid = 0;
+                        int hgrid = 0;
+                        if (act) {
+                            const int e = buf[wave][q];
+                            const uint32_t a = eA[e];
+                            const int r = (int)(a & 0xffu), b = (int)((a >> 8) & 0xffu);
+                            const float4* R = &recs[r * 8];
+                            const float4 r6 = R[6];
+                            const uint32_t flags = __float_as_uint(r6.x);
+                            const uint32_t code = flags & 0xffu;
+                            const bool degen = (flags >> 8) & 1u;
+                            opaque = (flags >> 10) & 1u;
+                            hid = ids[r];
+                            hgrid = __float_as_int(r6.y);
+                            ++nTest;
+                            if (b == 0xff) {
+                                const float4 r0 = R[0], r1 = R[1], r2 = R[2], r7 = R[7];
+                                const f3d K0[4] = {mk3(r0.x, r0.y, r0.z), mk3(r0.w, r1.x, r1.y),
+                                                   mk3(r1.z, r1.w, r2.x), mk3(r2.y, r2.z, r2.w)};
+                                // CalculateBound's min z: min(min(min(A,B),C),D), A..D = P0,P1,P3,P2
+                                const float bminz = fmin_ref(fmin_ref(fmin_ref(K0[0].z, K0[1].z), K0[3].z), K0[2].z);
+                                const bool inb = !(sx < r7.x || sx > r7.y || sy < r7.z || sy > r7.w);
+                                if (inb && !(bminz > occlZ) && lodOk) {
+                                    ++nBound;
+                                    const MBHit h = quad_hit(K0[0], K0[1], K0[2], K0[3], code, degen, sx, sy);
+                                    if (h.in) {
+                                        ++nIn;
+                                        if (!h.linear) ++nIn2;
+                                        hit = !(occlZ <= h.D);
+                                        D = h.D; u = h.u; v = h.v;
+                                    }
+                                }
+                            } else {
+                                const float* kb0 = &kbs[r][0];
+                                const float* kb1 = &kbs[r][6];
+                                const float T0 = r6.z, T1 = r6.w;
+                                const int d = divs[r];
+                                const float dt = dts[r];
+                                const float tc = eTc[e], tp = eTp[e];
+                                const float time0 = tc - dt;
+                                const float time1 = (b != d - 1) ? (tc + dt) - dt : closetime;
+                                const float mixc = (tc - T0) / (T1 - T0);
+                                const float mixp = (tp - T0) / (T1 - T0);
+                                float sb[5];
+#pragma unroll
+                                for (int c = 0; c < 5; ++c) {
+                                    const float cur = kb0[c] + mixc * (kb1[c] - kb0[c]);
+                                    const float prev = b == 0 ? kb0[c] : kb0[c] + mixp * (kb1[c] - kb0[c]);
+                                    sb[c] = c < 3 ? fmin_ref(prev, cur) : fmax_ref(prev, cur);
+                                }
+                                const bool inb = !(sx < sb[0] || sx > sb[3] || sy < sb[1] || sy > sb[4]);
+                                if (!(stime < time0 || stime > time1) && inb && !(sb[2] > occlZ) && lodOk) {
+                                    ++nBound;
+                                    // CqMicroPolygonMotion::Sample: the key span of the time
+                                    int iIndex = 0;
+                                    float Fraction = 0.0f;
+                                    bool Exact = true;
+                                    if (stime > T0) {
+                                        if (stime >= T1) {
+                                            iIndex = 1;
+                                        } else {
+                                            Fraction = (stime - T0) / (T1 - T0);
+                                            Exact = (T0 == stime);
+                                        }
+                                    }
+                                    const float* kbi = iIndex ? kb1 : kb0;
+                                    float tb[4];
+                                    if (Exact) {
+                                        tb[0] = kbi[0]; tb[1] = kbi[3]; tb[2] = kbi[1]; tb[3] = kbi[4];
+                                    } else {
+                                        const float F1 = 1 - Fraction;
+                                        tb[0] = F1 * kb0[0] + Fraction * kb1[0];
+                                        tb[1] = F1 * kb0[3] + Fraction * kb1[3];
+                                        tb[2] = F1 * kb0[1] + Fraction * kb1[1];
+                                        tb[3] = F1 * kb0[4] + Fraction * kb1[4];
+                                    }
+                                    if (!(sx < tb[0] || sx > tb[1] || sy < tb[2] || sy > tb[3])) {
+                                        const float4 r0 = R[0], r1 = R[1], r2 = R[2], r3 = R[3], r4 = R[4], r5 = R[5];
+                                        const f3d K0[4] = {mk3(r0.x, r0.y, r0.z), mk3(r0.w, r1.x, r1.y),
+                                                           mk3(r1.z, r1.w, r2.x), mk3(r2.y, r2.z, r2.w)};
+                                        const f3d K1[4] = {mk3(r3.x, r3.y, r3.z), mk3(r3.w, r4.x, r4.y),
+                                                           mk3(r4.z, r4.w, r5.x), mk3(r5.y, r5.z, r5.w)};
+                                        // a key's vertices when the time is exact, else
+                                        // (1 - F) K0 + F K1; selected per component
+                                        const float F1 = 1.0f - Fraction;
+                                        const bool k1 = iIndex != 0;
+                                        f3d pts[4];
+#pragma unroll
+                                        for (int k = 0; k < 4; ++k) {
+                                            const f3d mixed = mk3((F1 * K0[k].x) + (Fraction * K1[k].x),
+                                                                  (F1 * K0[k].y) + (Fraction * K1[k].y),
+                                                                  (F1 * K0[k].z) + (Fraction * K1[k].z));
+                                            pts[k].x = Exact ? (k1 ? K1[k].x : K0[k].x) : mixed.x;
+                                            pts[k].y = Exact ? (k1 ? K1[k].y : K0[k].y) : mixed.y;
+                                            pts[k].z = Exact ? (k1 ? K1[k].z : K0[k].z) : mixed.z;
+                                        }
+                                        const MBHit h = quad_hit(pts[0], pts[1], pts[2], pts[3], code, degen, sx, sy);
+                                        if (h.in) {
+                                            ++nIn;
+                                            if (!h.linear) ++nIn2;
+                                            hit = !(occlZ <= h.D);
+                                            D = h.D; u = h.u; v = h.v;
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                        // StoreSample (bucketprocessor.cpp:1471-1569)
+                        nHit += hit ? 1u : 0u;
+                        const bool opq = hit && opaque;
+                        occlZ = opq ? D : occlZ;
+                        win = opq ? hid : win;
+                        wu = opq ? u : wu;
+                        wv = opq ? v : wv;
+                        const bool app = hit && !opaque;
+                        const unsigned long long am = __ballot(app);
+                        if (am) {
+                            const int na = __popcll(am);
+                            if (na > wbLeft) {
+                                uint32_t base = 0;
+                                if (lane == 0) base = atomicAdd(hitCounter, 64u);
+                                wbBase = __shfl(base, 0);
+                                wbLeft = 64;
+                            }
+                            if (app) {
+                                const uint32_t slot = wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & lanesBelow);
+                                if (slot < fr->hitCapacity && nh < RH_MAX_HITS - 1) {
+                                    float4 hA, hB;
+                                    shade_hit(prims[grids[hgrid].prim], u, v, D, hid, hA, hB);
+                                    hitData[2 * (size_t)slot] = hA;
+                                    hitData[2 * (size_t)slot + 1] = hB;
+                                    hitNext[slot] = head;
+                                    head = slot;
+                                    ++nh;
+                                } else if (slot < fr->hitCapacity) {
+                                    lost = true;
+                                }
+                            }
+                            wbLeft -= na;
+                        }
+                    }
+                    // keep what the buffer holds past the first 64
+                    const int rest = fill - nq;
+                    const uint16_t keepE = lane < rest ? buf[wave][64 + lane] : (uint16_t)0;
+                    if (lane < rest) buf[wave][lane] = keepE;
+                    fill = rest;
+                }
+            }
+            c0 += (uint32_t)nrec;
         }
         if (live) {
             const size_t sidx = ((size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)) * ns + si;
@@ -1590,7 +1742,7 @@ __global__ void __launch_bounds__(256) rh_sample_mb(const FrameDev* __restrict__
             base = __shfl(base, __ffsll((long long)cm) - 1);
             if (multi) {
                 const size_t sidx = ((size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)) * ns + si;
-                const uint32_t e = base + (uint32_t)__popcll(cm & ((1ull << lane) - 1ull));
+                const uint32_t e = base + (uint32_t)__popcll(cm & lanesBelow);
                 combineList[e] = make_uint4((uint32_t)sidx, (uint32_t)(sidx >> 32), head, (uint32_t)nh);
             }
         }
@@ -1611,7 +1763,7 @@ __global__ void __launch_bounds__(256) rh_sample_mb(const FrameDev* __restrict__
     __syncthreads();
     if (tid < 5) {
         uint32_t v = 0;
-        for (int w = 0; w < RH_S_WAVES; ++w) v += wsum[w][tid];
+        for (int w = 0; w < NW; ++w) v += wsum[w][tid];
         rowCounts[(size_t)(rowBase + blockIdx.x) * 5 + tid] = v;
     }
 }
@@ -2108,11 +2260,12 @@ extern "C" hipError_t rh_launch_sample(hipStream_t st, int variant, int grid, co
     return hipGetLastError();
 }
 
-// The motion form of the sampler: one workgroup per task.
+// The motion form of the sampler: one workgroup of RH_MB_WAVES waves per task.
+#define RH_MB_WAVES 16
 extern "C" hipError_t rh_launch_sample_mb(hipStream_t st, const SampleLaunch* a, int ntasks,
                                           const float4* mprecMB) {
     if (ntasks <= 0) return hipSuccess;
-    hipLaunchKernelGGL((rh_sample_mb<64>), dim3(ntasks), dim3(256), 0, st, a->fr, a->buckets, a->tasks, ntasks,
+    hipLaunchKernelGGL((rh_sample_mb<RH_MB_WAVES>), dim3(ntasks), dim3(RH_MB_WAVES * 64), 0, st, a->fr, a->buckets, a->tasks, ntasks,
                        a->capStart, a->listCount, a->lists, mprecMB, a->grids, a->prims, a->pos2d, a->pos1d,
                        a->pixPattern, a->samples, a->hitData, a->hitNext, a->hitCounter, a->combineList,
                        a->frontDepth, a->rowCounts, a->rowBase);

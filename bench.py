@@ -13,7 +13,12 @@ The CPU baseline is the oracle (a C++ restatement of the reference hider,
 occlusion culling on as in the reference) on the host cores: the whole frame
 in one crop band per thread, and one thread on a quarter of the frame.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hd|spheres]
+Other workloads: "clustered" (2048x2048, micropolygons per bucket log-normal),
+"motion" (256x256, PixelSamples 8x8, transparent deforming patches, shutter
+0-1), "spheres" (256x256, 64 spheres) and "serving" (a Poisson request
+stream).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hd|clustered|motion|spheres|serving]
 """
 import argparse
 import json
@@ -45,6 +50,14 @@ def workload(name):
                             "clipped at 200k (seed 1), opaque",
                 "width": 2048, "height": 2048, "pixel_samples": [4, 4], "filter": ["gaussian", 2, 2],
                 "patches": len(prims), "bucket": [16, 16]}
+    elif name == "motion":
+        opts, prims = scenes.motion_scene(seed=1, n=600, res=(256, 256))
+        desc = {"workload": "plain hider with transparency and motion blur: 256x256, 16x16 buckets, PixelSamples 8x8, "
+                            "gaussian 2x2, ShadingRate 1, 600 bicubic Bezier patches (seed 1), Os uniform in "
+                            "[0.2,0.8] per patch, two motion keys at times 0 and 1 (deforming surfaces moved by "
+                            "N(0, 0.05 x width) pixels per axis), shutter 0-1",
+                "width": 256, "height": 256, "pixel_samples": [8, 8], "filter": ["gaussian", 2, 2],
+                "patches": 600, "bucket": [16, 16], "motion_keys": 2, "shutter": [0, 1]}
     elif name == "spheres":
         opts, prims = scenes.spheres_scene(seed=1)
         desc = {"workload": "plain hider CPU-runnable case: 256x256, 16x16 buckets, PixelSamples 4x4, "
@@ -56,11 +69,18 @@ def workload(name):
     return opts, prims, desc
 
 
-def algorithmic_ops(st):
+def algorithmic_ops(st, motion=False):
     """FP32 lane-operations of the sampling kernel, counted from the
-    reference's arithmetic (bound, occlusion and LOD tests: 7 per candidate;
-    four edge functions: 24 per bound hit; one Newton step + bilinear depth +
-    store test: 53 per point-in-quad hit; a second Newton step: 39)."""
+    reference's arithmetic.  Static micropolygons: bound, occlusion and LOD
+    tests, 7 per candidate; four edge functions, 24 per bound hit; one Newton
+    step + bilinear depth + store test, 53 per point-in-quad hit; a second
+    Newton step, 39.  Moving micropolygons (RenderMPG_MBOrDof and the motion
+    Sample): the time window adds 2 per visit; a bound hit adds the key span
+    (5), the tight bound at the sample's time (16), the vertices at that time
+    (36) and the edge and bilinear set-up (22) to the 24 of the edges."""
+    if motion:
+        return (9 * st["spl_count"] + 103 * st["spl_bound_hits"] + 53 * st["inside"]
+                + 39 * st["inside_nonlinear"])
     return (7 * st["spl_count"] + 24 * st["spl_bound_hits"] + 53 * st["inside"]
             + 39 * st["inside_nonlinear"])
 
@@ -276,31 +296,36 @@ def main():
         tuning = h.tuning()
     for _ in range(args.warmup):
         h.render_device()
-    dev_ms, sample_ms, dice_ms, bin_ms, filt_ms = [], [], [], [], []
+    dev_ms, sample_ms, dice_ms, bin_ms, filt_ms, comp_ms = [], [], [], [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         h.render_device()          # returns after the frame's last kernel completed
         st = h.stats()
         dev_ms.append(st["ms_device"]); sample_ms.append(st["ms_sample"])
         dice_ms.append(st["ms_dice"]); bin_ms.append(st["ms_bin"]); filt_ms.append(st["ms_filter"])
+        comp_ms.append(st["ms_composite"])
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     fps = args.steps / wall
     tests_per_s = st["spl_count"] * args.steps / wall
     dev_sorted = sorted(dev_ms)
-    ops = algorithmic_ops(st)
+    motion = args.workload == "motion"
+    ops = algorithmic_ops(st, motion)
     if tuning["mode"] == 2:
         # The batched form replays a frame graph, which carries no stage
         # timestamps: the same launches issued one by one give the stage times.
-        dice_ms, bin_ms, sample_ms, filt_ms = [], [], [], []
+        dice_ms, bin_ms, sample_ms, filt_ms, comp_ms = [], [], [], [], []
         for _ in range(3):
             h.render_stages()
             sst = h.stats()
             dice_ms.append(sst["ms_dice"]); bin_ms.append(sst["ms_bin"])
             sample_ms.append(sst["ms_sample"]); filt_ms.append(sst["ms_filter"])
+            comp_ms.append(sst["ms_composite"])
     sample_avg_ms = sum(sample_ms) / len(sample_ms)
-    # rh_sample's launch time: HIP events on its stream around it
-    roof_ms = sample_avg_ms
+    # the sampler's launch time: HIP events on its stream around it (the
+    # sampling stage less the transparent-hit combine that follows it)
+    comp_avg_ms = sum(comp_ms) / len(comp_ms)
+    roof_ms = sample_avg_ms - comp_avg_ms
     achieved_tops = ops / (roof_ms * 1e-3) / 1e12
     out = {
         "metric": "micropolygon-sample tests/s",
@@ -320,7 +345,8 @@ def main():
         "frame_ms_p50": dev_sorted[len(dev_sorted) // 2],
         "frame_ms_p99": dev_sorted[min(len(dev_sorted) - 1, int(0.99 * len(dev_sorted)))],
         "stage_ms": {"dice_bust": sum(dice_ms) / len(dice_ms), "bin": sum(bin_ms) / len(bin_ms),
-                     "sample": sample_avg_ms, "filter_quantise": sum(filt_ms) / len(filt_ms)},
+                     "sample": sample_avg_ms, "filter_quantise": sum(filt_ms) / len(filt_ms),
+                     "composite_in_sample": comp_avg_ms},
         "stage_ms_grid_first_frame": grid_stages,
         "tuning": {"mode": ["grid", "plain", "batched"][tuning["mode"]], "lds_records": tuning["lds_records"],
                    "blocks_per_wave": tuning["blocks_per_wave"], "split_cost": tuning["split_cost"]},
@@ -332,7 +358,7 @@ def main():
                                       "grids", "micropolygons", "mp_bucket_refs")},
         "roofline": {
             "bound": "valu",
-            "kernel": "rh_sample",
+            "kernel": "rh_sample_mb" if motion else "rh_sample",
             "achieved": achieved_tops,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",

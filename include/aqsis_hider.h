@@ -133,6 +133,8 @@ typedef struct rh_stats {
     double ms_sample;          /* device: hit test + depth + composite          */
     double ms_filter;          /* device: filter + expose + quantise            */
     double ms_device;          /* device: whole frame, event to event           */
+    double ms_composite;       /* device: transparent hit combine, a part of
+                                  ms_sample (0 for an opaque frame)             */
 } rh_stats;
 
 /* Work and device memory one frame of a scene needs, from the host
