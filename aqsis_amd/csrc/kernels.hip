@@ -1284,8 +1284,8 @@ __device__ __forceinline__ MBHit quad_hit(const f3d p0, const f3d p1, const f3d 
 //     entries that visit the lane's sample; the lanes then take their own
 //     entries lowest bit first, so each sample sees its visits in order and a
 //     wave runs as many visit steps as its busiest lane has visits.
-#define RH_MB_RECS 128     // records staged per chunk
-#define RH_MB_PAIRS 2048   // (record, sub-bound) pairs per chunk at most
+#define RH_MB_RECS 256     // records staged per chunk (4 per lane of one wave)
+#define RH_MB_PAIRS 4096   // (record, sub-bound) pairs per chunk at most
 
 template <int NW>
 __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restrict__ fr,
@@ -1410,18 +1410,27 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
             __syncthreads();
             // pair offsets, and the records whose pairs fit in one pass
             if (wave == 0) {
-                const int k0 = 2 * lane, k1 = k0 + 1;
-                const int d0 = k0 < cnt ? divs[k0] : 0, d1 = k1 < cnt ? divs[k1] : 0;
-                int incl = d0 + d1;
+                constexpr int PER = RH_MB_RECS / 64;
+                int dk[PER], incl = 0;
+#pragma unroll
+                for (int j = 0; j < PER; ++j) {
+                    dk[j] = PER * lane + j < cnt ? divs[PER * lane + j] : 0;
+                    incl += dk[j];
+                }
                 for (int off = 1; off < 64; off <<= 1) {
                     const int y = __shfl_up(incl, off);
                     if (lane >= off) incl += y;
                 }
-                const int excl = incl - d0 - d1;
-                offs[k0] = excl;
-                offs[k1] = excl + d0;
+                int run = incl, fit = 0;
+#pragma unroll
+                for (int j = 0; j < PER; ++j) run -= dk[j];
+#pragma unroll
+                for (int j = 0; j < PER; ++j) {
+                    offs[PER * lane + j] = run;
+                    run += dk[j];
+                    fit += (PER * lane + j < cnt && run <= RH_MB_PAIRS) ? 1 : 0;
+                }
                 if (lane == 63) offs[RH_MB_RECS] = incl;
-                int fit = (k0 < cnt && excl + d0 <= RH_MB_PAIRS ? 1 : 0) + (k1 < cnt && incl <= RH_MB_PAIRS ? 1 : 0);
                 for (int off = 32; off > 0; off >>= 1) fit += __shfl_xor(fit, off);
                 if (lane == 0) nfitShared = fit;
             }

@@ -11,12 +11,19 @@ token rules: strings in double quotes, numbers, [ ] arrays, # comments):
   Hider "hidden" "jitter", Display (ignored), FrameBegin/End, WorldBegin/End,
   AttributeBegin/End, TransformBegin/End, Identity, Transform,
   ConcatTransform, Translate, Rotate, Scale, Color, Opacity,
-  Surface "constant", Sphere, Patch "bicubic" / "bilinear" "P".
+  Surface "constant", Sphere, Patch "bicubic" / "bilinear" "P",
+  MotionBegin [t0 t1] / MotionEnd around two requests of one kind: two
+  transformations (the camera's before WorldBegin, or an object's) or two
+  patches of the same type (a deforming surface).
 
 Transformations use RenderMan's row-vector convention: a request's matrix
 M is applied before the current one (CTM <- M * CTM); transformations before
 WorldBegin make the camera transformation, and surfaces are posted in camera
-space.  Anything else raises RibError naming the request and line.
+space.  A surface posted while the camera or its own transformation moves,
+or given as two motion keys, is a two-key deforming surface (nkeys = 2):
+patches carry key 1's camera-space points in P1, spheres key 1's
+object->camera matrix in xform[1].  Anything else raises RibError naming the
+request and line.
 """
 import math
 import re
@@ -124,17 +131,81 @@ def _rotate(angle, x, y, z):
                      [0.0, 0.0, 0.0, 1.0]])
 
 
+_TRANSFORMS = ("Identity", "Transform", "ConcatTransform", "Translate", "Scale", "Rotate")
+
+
+def _transform(req, flat, ctm):
+    """The current transformation after one transformation request."""
+    if req == "Identity":
+        return np.eye(4)
+    if req == "Transform":
+        return np.array(flat[:16], dtype=np.float64).reshape(4, 4)
+    if req == "ConcatTransform":
+        return np.array(flat[:16], dtype=np.float64).reshape(4, 4) @ ctm
+    if req == "Translate":
+        t = np.eye(4)
+        t[3, :3] = flat[:3]
+        return t @ ctm
+    if req == "Scale":
+        return np.diag([flat[0], flat[1], flat[2], 1.0]) @ ctm
+    return _rotate(*flat[:4]) @ ctm
+
+
+def _patch_points(line, args, m):
+    """(kind, camera-space control points) of a Patch request under m."""
+    kind = args[0]
+    pts = _params(args[1:]).get("P")
+    n = {"bicubic": 16, "bilinear": 4}.get(kind)
+    if n is None:
+        raise RibError(f"line {line}: Patch type {kind!r} is not supported")
+    if pts is None or len(pts) != 3 * n:
+        raise RibError(f"line {line}: Patch \"{kind}\" needs {3 * n} values of \"P\"")
+    P = np.array(pts, dtype=np.float64).reshape(n, 3)
+    if not np.array_equal(m, np.eye(4)):
+        P = np.hstack([P, np.ones((n, 1))]) @ m
+        P = P[:, :3] / P[:, 3:4]
+    return kind, P.astype(np.float32)
+
+
+def _set_keys(prim, times, P1=None, xform1=None):
+    prim.nkeys = 2
+    prim.key_time[:] = [float(times[0]), float(times[1])]
+    if P1 is not None:
+        for i in range(len(P1)):
+            prim.P1[i][:] = [float(v) for v in P1[i]]
+    if xform1 is not None:
+        prim.xform[1][:] = [float(v) for v in xform1.reshape(-1)]
+
+
 def parse(text):
     """RIB text -> (rh_options, [rh_prim]) for the hider."""
     opts = default_options()
     prims = []
-    ctm = np.eye(4)
-    camera = np.eye(4)
+    # The current transformation: one matrix, or one per motion key when it
+    # moves (`keys`, with the key times in `ktimes`); likewise the camera.
+    ctm, keys = np.eye(4), None
+    camera, camkeys = np.eye(4), None
+    ktimes = None
     in_world = False
     attr = {"Cs": (1.0, 1.0, 1.0), "Os": (1.0, 1.0, 1.0), "rate": 1.0, "surface": "constant"}
     astack, tstack = [], []
+    motion = None          # the open MotionBegin block: times, line, requests
+
+    def key_matrices():
+        """Object->camera matrices per key, or None when nothing moves."""
+        if keys is None and camkeys is None:
+            return None
+        k = keys or [ctm, ctm]
+        c = camkeys or [camera, camera]
+        return [k[0] @ c[0], k[1] @ c[1]]
+
     for line, req, args in _requests(text):
         flat = [a for a in _flat(args) if not isinstance(a, str)]
+        if motion is not None and req != "MotionEnd":
+            if req == "MotionBegin":
+                raise RibError(f"line {line}: MotionBegin inside a motion block")
+            motion["reqs"].append((line, req, args, flat))
+            continue
         try:
             if req == "Format":
                 opts.xres, opts.yres = int(flat[0]), int(flat[1])
@@ -183,36 +254,62 @@ def parse(text):
                     opts.jitter = int(p["jitter"][0])
             elif req in ("Display", "FrameBegin", "FrameEnd", "Declare", "Sides"):
                 pass
+            elif req == "MotionBegin":
+                if len(flat) != 2 or not flat[0] < flat[1]:
+                    raise RibError(f"line {line}: MotionBegin needs two increasing times (two motion keys)")
+                motion = {"times": (flat[0], flat[1]), "line": line, "reqs": []}
+            elif req == "MotionEnd":
+                if motion is None:
+                    raise RibError(f"line {line}: MotionEnd without MotionBegin")
+                block, motion = motion, None
+                reqs = block["reqs"]
+                names = {r[1] for r in reqs}
+                if len(reqs) != 2 or len(names) != 1:
+                    raise RibError(f"line {block['line']}: a motion block needs one request per key, "
+                                   f"both of one kind")
+                if ktimes is not None and tuple(ktimes) != tuple(block["times"]):
+                    raise RibError(f"line {block['line']}: motion blocks with different key times")
+                kind = reqs[0][1]
+                if kind in _TRANSFORMS:
+                    base = keys or [ctm, ctm]
+                    keys = [_transform(kind, reqs[k][3], base[k]) for k in range(2)]
+                    ctm = keys[0]
+                    ktimes = block["times"]
+                elif kind == "Patch":
+                    if not in_world:
+                        raise RibError(f"line {block['line']}: Patch outside WorldBegin/WorldEnd")
+                    mk = key_matrices() or [ctm @ camera] * 2
+                    k0, P0 = _patch_points(reqs[0][0], reqs[0][2], mk[0])
+                    k1, P1 = _patch_points(reqs[1][0], reqs[1][2], mk[1])
+                    if k0 != k1:
+                        raise RibError(f"line {block['line']}: the motion keys are patches of different types")
+                    make = scenes.bicubic if k0 == "bicubic" else scenes.bilinear
+                    p = make(P0, attr["Cs"], attr["Os"], attr["rate"])
+                    _set_keys(p, block["times"], P1=P1)
+                    prims.append(p)
+                else:
+                    raise RibError(f"line {block['line']}: {kind} is not supported inside a motion block")
             elif req == "WorldBegin":
-                camera = ctm.copy()
-                ctm = np.eye(4)
+                camera, camkeys = ctm.copy(), keys
+                ctm, keys = np.eye(4), None
                 in_world = True
             elif req == "WorldEnd":
                 in_world = False
             elif req == "AttributeBegin":
                 astack.append(dict(attr))
-                tstack.append(ctm.copy())
+                tstack.append((ctm.copy(), keys))
             elif req == "AttributeEnd":
                 attr = astack.pop()
-                ctm = tstack.pop()
+                ctm, keys = tstack.pop()
             elif req == "TransformBegin":
-                tstack.append(ctm.copy())
+                tstack.append((ctm.copy(), keys))
             elif req == "TransformEnd":
-                ctm = tstack.pop()
-            elif req == "Identity":
-                ctm = np.eye(4)
-            elif req == "Transform":
-                ctm = np.array(flat[:16], dtype=np.float64).reshape(4, 4)
-            elif req == "ConcatTransform":
-                ctm = np.array(flat[:16], dtype=np.float64).reshape(4, 4) @ ctm
-            elif req == "Translate":
-                t = np.eye(4)
-                t[3, :3] = flat[:3]
-                ctm = t @ ctm
-            elif req == "Scale":
-                ctm = np.diag([flat[0], flat[1], flat[2], 1.0]) @ ctm
-            elif req == "Rotate":
-                ctm = _rotate(*flat[:4]) @ ctm
+                ctm, keys = tstack.pop()
+            elif req in _TRANSFORMS:
+                # a static request applies to every key of a moving transformation
+                ctm = _transform(req, flat, ctm)
+                if keys is not None:
+                    keys = [_transform(req, flat, k) for k in keys]
             elif req == "Color":
                 attr["Cs"] = tuple(flat[:3])
             elif req == "Opacity":
@@ -224,33 +321,31 @@ def parse(text):
             elif req == "Sphere":
                 if not in_world:
                     raise RibError(f"line {line}: Sphere outside WorldBegin/WorldEnd")
-                m = ctm @ camera
+                mk = key_matrices()
+                m = ctm @ camera if mk is None else mk[0]
                 p = scenes.sphere(flat[0], (0.0, 0.0, 0.0), attr["Cs"], attr["Os"], attr["rate"])
                 p.sphere[:] = [float(v) for v in flat[:4]]
                 p.xform[0][:] = [float(v) for v in m.reshape(-1)]
                 p.xform[1][:] = [float(v) for v in m.reshape(-1)]
+                if mk is not None:
+                    _set_keys(p, ktimes, xform1=mk[1])
                 prims.append(p)
             elif req == "Patch":
                 if not in_world:
                     raise RibError(f"line {line}: Patch outside WorldBegin/WorldEnd")
-                kind = args[0]
-                pts = _params(args[1:]).get("P")
-                n = {"bicubic": 16, "bilinear": 4}.get(kind)
-                if n is None:
-                    raise RibError(f"line {line}: Patch type {kind!r} is not supported")
-                if pts is None or len(pts) != 3 * n:
-                    raise RibError(f"line {line}: Patch \"{kind}\" needs {3 * n} values of \"P\"")
-                P = np.array(pts, dtype=np.float64).reshape(n, 3)
-                m = ctm @ camera
-                if not np.array_equal(m, np.eye(4)):
-                    P = np.hstack([P, np.ones((n, 1))]) @ m
-                    P = P[:, :3] / P[:, 3:4]
+                mk = key_matrices()
+                kind, P = _patch_points(line, args, ctm @ camera if mk is None else mk[0])
                 make = scenes.bicubic if kind == "bicubic" else scenes.bilinear
-                prims.append(make(P.astype(np.float32), attr["Cs"], attr["Os"], attr["rate"]))
+                p = make(P, attr["Cs"], attr["Os"], attr["rate"])
+                if mk is not None:
+                    _set_keys(p, ktimes, P1=_patch_points(line, args, mk[1])[1])
+                prims.append(p)
             else:
                 raise RibError(f"line {line}: request {req} is not supported by the hider front end")
         except (IndexError, TypeError) as e:
             raise RibError(f"line {line}: malformed {req}: {e}") from None
+    if motion is not None:
+        raise RibError(f"line {motion['line']}: MotionBegin without MotionEnd")
     return opts, prims
 
 

@@ -299,7 +299,8 @@ def serving_requests(seed=1, count=2000, sides=(256, 512, 1024, 2048), spp=(2, 4
 
 def to_rib(opts, prims):
     """RIB text of a scene in camera space (spheres as a translation), which
-    aqsis_amd.rib.parse reads back to the same options and records."""
+    aqsis_amd.rib.parse reads back to the same options and records; two-key
+    surfaces as a MotionBegin block of two patches or two translations."""
     names = {v: k for k, v in __import__("aqsis_amd.abi", fromlist=["FILTERS"]).FILTERS.items()}
     out = [f'Format {opts.xres} {opts.yres} {opts.pixel_aspect}',
            f'PixelSamples {opts.xsamples} {opts.ysamples}',
@@ -311,6 +312,7 @@ def to_rib(opts, prims):
            f'Option "limits" "bucketsize" [{opts.bucket_x} {opts.bucket_y}] "eyesplits" [{opts.eyesplits}]',
            f'Option "limits" "zthreshold" [{opts.zthreshold[0]} {opts.zthreshold[1]} {opts.zthreshold[2]}]',
            f'Hider "hidden" "jitter" [{opts.jitter}]',
+           f'Shutter {opts.shutter_open} {opts.shutter_close}',
            f'Projection "perspective" "fov" [{opts.fov}]',
            'WorldBegin', 'Surface "constant"']
     for p in prims:
@@ -318,15 +320,28 @@ def to_rib(opts, prims):
         out.append(f'Color [{p.Cs[0]} {p.Cs[1]} {p.Cs[2]}]')
         out.append(f'Opacity [{p.Os[0]} {p.Os[1]} {p.Os[2]}]')
         out.append(f'ShadingRate {p.shading_rate}')
+        moving = p.nkeys == 2
+        if moving:
+            motion = f'MotionBegin [{p.key_time[0]} {p.key_time[1]}]'
         if p.type == RH_PRIM_SPHERE:
-            m = list(p.xform[0])
-            out.append(f'Translate {m[12]} {m[13]} {m[14]}')
+            keys = [list(p.xform[k]) for k in range(2 if moving else 1)]
+            if moving:
+                out.append(motion)
+            for m in keys:
+                out.append(f'Translate {m[12]} {m[13]} {m[14]}')
+            if moving:
+                out.append('MotionEnd')
             out.append(f'Sphere {p.sphere[0]} {p.sphere[1]} {p.sphere[2]} {p.sphere[3]}')
         else:
             n = 16 if p.type == RH_PRIM_BICUBIC else 4
-            pts = " ".join(f"{p.P[i][0]} {p.P[i][1]} {p.P[i][2]}" for i in range(n))
             kind = "bicubic" if p.type == RH_PRIM_BICUBIC else "bilinear"
-            out.append(f'Patch "{kind}" "P" [{pts}]')
+            if moving:
+                out.append(motion)
+            for P in ([p.P, p.P1] if moving else [p.P]):
+                pts = " ".join(f"{P[i][0]} {P[i][1]} {P[i][2]}" for i in range(n))
+                out.append(f'Patch "{kind}" "P" [{pts}]')
+            if moving:
+                out.append('MotionEnd')
         out.append('AttributeEnd')
     out.append('WorldEnd')
     return "\n".join(out) + "\n"
