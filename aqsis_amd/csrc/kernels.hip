@@ -2032,18 +2032,82 @@ __global__ void __launch_bounds__(256) rh_box_masks(const FrameDev* __restrict__
 // staged in LDS once, with RH_FU coalesced loads in flight per lane, as 8
 // dwords per sample: Ci.rgb, z, Oi.rgb and a word holding the sample's
 // box-test mask (bits 0..15) and whether it holds a surface (bit 31).  Halo
-// pixel records are 8*ns+1 dwords apart so that the lanes of a wave read
+// pixel records are 8*ns+RH_FPAD dwords apart so that the lanes of a wave read
 // different LDS banks.
 #define RH_FT 8
 #define RH_FU 16
+#ifndef RH_FPAD
+#define RH_FPAD 1
+#endif
+
+// The weighted sums of one pixel over its filter footprint, in the
+// reference's order (filter rows, filter columns, samples): NC channels
+// starting at `rec0` of each staged sample record (8 words apart), the
+// sample's mask word at offset `mw`.  A sample outside the pixel's box adds
+// nothing, one inside adds its weight to gTot, and one holding a surface also
+// adds weight x value to each channel sum (gTot and count only with TOT).
+// The adds are unconditional with the weight zeroed where the reference
+// skips: x + 0 * v == x exactly, since the sums start at +0 and never become
+// -0.  Weights are read with uniform (scalar) loads.
+template <int NC, bool TOT>
+__device__ __forceinline__ void filter_taps(const float* __restrict__ rec0, const float* __restrict__ filterTab,
+                                            int ns, int xm, int ym, int HX, int S, int lx, int ly, int mw,
+                                            float* acc, float& gTot, int& count) {
+    for (int fy = -ym; fy <= ym; ++fy)
+        for (int fx = -xm; fx <= xm; ++fx) {
+            const float* rec = rec0 + ((ly + fy + ym) * HX + (lx + fx + xm)) * S;
+            const float* wt = filterTab + ((fy + ym) * (2 * xm + 1) + fx + xm) * ns;
+            const uint32_t need = (1u << (fx + xm)) | (1u << (8 + fy + ym));
+            // four samples at a time, every LDS read issued first
+            int si = 0;
+            for (; si + 4 <= ns; si += 4) {
+                uint32_t m[4];
+                float d[4][NC];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float* r = rec + (si + q) * 8;
+                    m[q] = __float_as_uint(r[mw]);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) d[q][c] = r[c];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const bool in = (m[q] & need) == need;
+                    const bool hit = in && (m[q] >> 31);
+                    const float g = wt[si + q];
+                    const float ghit = hit ? g : 0.0f;
+                    if (TOT) {
+                        gTot = gTot + (in ? g : 0.0f);
+                        count += hit ? 1 : 0;
+                    }
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) acc[c] = acc[c] + d[q][c] * ghit;
+                }
+            }
+            for (; si < ns; ++si) {
+                const float* r = rec + si * 8;
+                const uint32_t mm = __float_as_uint(r[mw]);
+                const bool in = (mm & need) == need;
+                const bool hit = in && (mm >> 31);
+                const float g = wt[si];
+                const float ghit = hit ? g : 0.0f;
+                if (TOT) {
+                    gTot = gTot + (in ? g : 0.0f);
+                    count += hit ? 1 : 0;
+                }
+#pragma unroll
+                for (int c = 0; c < NC; ++c) acc[c] = acc[c] + r[c] * ghit;
+            }
+        }
+}
 __global__ void __launch_bounds__(128) rh_filter_tiled(const FrameDev* __restrict__ fr,
                                                        const float4* __restrict__ samples,
                                                        const uint16_t* __restrict__ boxMask,
                                                        const float* __restrict__ filterTab,
                                                        const float* __restrict__ dither, float* __restrict__ channels,
                                                        uchar4* __restrict__ rgba8, int tilesX, int ntiles) {
-    // LDS: halo records (RH_FT+2*xm)*(RH_FT+2*ym)*(8*ns+1) floats, then the
-    // filter weights; the tail exchange reuses the first 192 floats.
+    // LDS: halo records (RH_FT+2*xm)*(RH_FT+2*ym)*(8*ns+RH_FPAD) floats; the
+    // tail exchange reuses the first 192 floats.
     extern __shared__ float lds[];
     // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
     // contiguous run of tiles so neighbouring tiles share halos in its L2.
@@ -2055,12 +2119,8 @@ __global__ void __launch_bounds__(128) rh_filter_tiled(const FrameDev* __restric
     const int extX0 = fr->extX0, extY0 = fr->extY0, extW = fr->extW, extH = fr->extH;
     const int tx0 = fr->cropX0 + (tile % tilesX) * RH_FT, ty0 = fr->cropY0 + (tile / tilesX) * RH_FT;
     const int HX = RH_FT + 2 * xm, HY = RH_FT + 2 * ym, NH = HX * HY;
-    const int S = 8 * ns + 1;
+    const int S = 8 * ns + RH_FPAD;
     const int tid = threadIdx.x;
-    // The filter weight table follows the halo in LDS (uniform reads).
-    float* wtab = lds + NH * S;
-    const int nwt = (2 * xm + 1) * (2 * ym + 1) * ns;
-    for (int i = tid; i < nwt; i += 128) wtab[i] = filterTab[i];
 
     // 1. Stage the halo's sample records (2 float4 per sample) and masks.
     {
@@ -2116,54 +2176,8 @@ __global__ void __launch_bounds__(128) rh_filter_tiled(const FrameDev* __restric
     int count = 0;
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (valid) {
-        const int c0 = half ? 4 : 0;
-        for (int fy = -ym; fy <= ym; ++fy)
-            for (int fx = -xm; fx <= xm; ++fx) {
-                const float* rec = lds + ((ly + fy + ym) * HX + (lx + fx + xm)) * S + c0;
-                const float* wt = wtab + ((fy + ym) * (2 * xm + 1) + fx + xm) * ns;
-                const uint32_t need = (1u << (fx + xm)) | (1u << (8 + fy + ym));
-                // Four samples at a time: every LDS read is issued first, then
-                // the sums take each sample in order through selects, which
-                // perform exactly the additions the branches would.  (In the
-                // second wave the fourth value read is the mask word; its sum
-                // is never used.)
-                int si = 0;
-                for (; si + 4 <= ns; si += 4) {
-                    uint32_t m[4];
-                    float g[4], d[4][4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const float* r = rec + (si + q) * 8;
-                        m[q] = __float_as_uint(r[7 - c0]);
-                        g[q] = wt[si + q];
-                        d[q][0] = r[0]; d[q][1] = r[1]; d[q][2] = r[2]; d[q][3] = r[3];
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const bool in = (m[q] & need) == need;
-                        const bool hit = in && (m[q] >> 31);
-                        gTot = in ? gTot + g[q] : gTot;
-                        acc[0] = hit ? acc[0] + d[q][0] * g[q] : acc[0];
-                        acc[1] = hit ? acc[1] + d[q][1] * g[q] : acc[1];
-                        acc[2] = hit ? acc[2] + d[q][2] * g[q] : acc[2];
-                        acc[3] = hit ? acc[3] + d[q][3] * g[q] : acc[3];
-                        count += hit ? 1 : 0;
-                    }
-                }
-                for (; si < ns; ++si) {
-                    const float* r = rec + si * 8;
-                    const uint32_t m = __float_as_uint(r[7 - c0]);
-                    const float g = wt[si];
-                    const bool in = (m & need) == need;
-                    const bool hit = in && (m >> 31);
-                    gTot = in ? gTot + g : gTot;
-                    acc[0] = hit ? acc[0] + r[0] * g : acc[0];
-                    acc[1] = hit ? acc[1] + r[1] * g : acc[1];
-                    acc[2] = hit ? acc[2] + r[2] * g : acc[2];
-                    acc[3] = hit ? acc[3] + r[3] * g : acc[3];
-                    count += hit ? 1 : 0;
-                }
-            }
+        if (half) filter_taps<3, false>(lds + 4, filterTab, ns, xm, ym, HX, S, lx, ly, 3, acc, gTot, count);
+        else filter_taps<4, true>(lds, filterTab, ns, xm, ym, HX, S, lx, ly, 7, acc, gTot, count);
     }
     __syncthreads();                  // the halo records are no longer read
     if (half) {
@@ -2327,8 +2341,7 @@ extern "C" hipError_t rh_launch_reduce_counters(hipStream_t st, const uint32_t* 
 // take this frame shape (mask bits or LDS size exceeded).
 extern "C" size_t rh_filter_tile_lds(int ns, int xm, int ym) {
     if (2 * xm + 1 > 8 || 2 * ym + 1 > 8) return 0;
-    const size_t bytes = ((size_t)(RH_FT + 2 * xm) * (RH_FT + 2 * ym) * (8 * ns + 1) +
-                          (size_t)(2 * xm + 1) * (2 * ym + 1) * ns) * sizeof(float);
+    const size_t bytes = (size_t)(RH_FT + 2 * xm) * (RH_FT + 2 * ym) * (8 * ns + RH_FPAD) * sizeof(float);
     return bytes <= 64 * 1024 ? bytes : 0;
 }
 
