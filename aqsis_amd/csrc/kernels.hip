@@ -1284,8 +1284,30 @@ __device__ __forceinline__ MBHit quad_hit(const f3d p0, const f3d p1, const f3d 
 //     entries that visit the lane's sample; the lanes then take their own
 //     entries lowest bit first, so each sample sees its visits in order and a
 //     wave runs as many visit steps as its busiest lane has visits.
-#define RH_MB_RECS 256     // records staged per chunk (4 per lane of one wave)
-#define RH_MB_PAIRS 4096   // (record, sub-bound) pairs per chunk at most
+#ifndef RH_MB_RECS
+#define RH_MB_RECS 128     // records staged per chunk (a multiple of 64)
+#endif
+#ifndef RH_MB_PAIRS
+#define RH_MB_PAIRS 2048   // (record, sub-bound) pairs per chunk at most
+#endif
+#define RH_MB_RING 128     // visit results held per wave
+
+// The position of the k-th (from 0) set bit of m, which has more than k.
+__device__ __forceinline__ int select_bit(unsigned long long m, int k) {
+    int pos = 0;
+    int c = __popc((uint32_t)m);
+    if (k >= c) { k -= c; m >>= 32; pos = 32; }
+    uint32_t w = (uint32_t)m;
+    c = __popc(w & 0xffffu);
+    if (k >= c) { k -= c; w >>= 16; pos += 16; }
+    c = __popc(w & 0xffu);
+    if (k >= c) { k -= c; w >>= 8; pos += 8; }
+    c = __popc(w & 0xfu);
+    if (k >= c) { k -= c; w >>= 4; pos += 4; }
+    c = __popc(w & 0x3u);
+    if (k >= c) { k -= c; w >>= 2; pos += 2; }
+    return pos + ((k >= (int)(w & 1u)) ? 1 : 0);
+}
 
 template <int NW>
 __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restrict__ fr,
@@ -1309,12 +1331,18 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
     __shared__ int divs[RH_MB_RECS];
     __shared__ int offs[RH_MB_RECS + 1];
     __shared__ uint32_t ids[RH_MB_RECS];
-    // entries: record | sub-bound << 8 (0xff: static) | index window << 16;
-    // sub-sample x and y ranges (lo | hi << 16); the sub-bound's previous and
-    // own times
-    __shared__ uint32_t eA[RH_MB_PAIRS], eX[RH_MB_PAIRS], eY[RH_MB_PAIRS];
-    __shared__ float eTp[RH_MB_PAIRS], eTc[RH_MB_PAIRS];
+    // entries: (record | sub-bound << 8 (0xff: static) | index window << 16,
+    // sub-sample x range, y range (lo | hi << 16), min z); the bound
+    // (min x, max x, min y, max y); the times (time0, time1)
+    __shared__ uint4 E0[RH_MB_PAIRS];
+    __shared__ float4 E1[RH_MB_PAIRS];
+    __shared__ float2 E2[RH_MB_PAIRS];
     __shared__ uint16_t buf[NW][128];
+    // per wave: the visit results ring, and per rank its first position and lanes
+    __shared__ uint32_t ringCode[NW][RH_MB_RING];
+    __shared__ float ringD[NW][RH_MB_RING], ringU[NW][RH_MB_RING], ringV[NW][RH_MB_RING];
+    __shared__ int rankStart[NW][65];
+    __shared__ unsigned long long rankLanes[NW][64];
     __shared__ uint32_t wtot[NW];
     __shared__ int nfitShared;
     __shared__ uint32_t wsum[NW][5];
@@ -1443,7 +1471,9 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                 const int p = pp + tid;
                 bool keep = false;
                 uint32_t ea = 0, ex = 0, ey = 0;
-                float tp = 0.0f, tc = 0.0f;
+                float emz = 0.0f;
+                float4 ebd = make_float4(0.f, 0.f, 0.f, 0.f);
+                float2 ett = make_float2(-INFINITY, INFINITY);
                 if (p < npairs) {
                     int lo = 0, hi = nrec - 1;
                     while (lo < hi) {
@@ -1477,6 +1507,10 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                             ea = (uint32_t)r | (0xffu << 8) | ((uint32_t)ns << 24);
                             ex = (uint32_t)gx0 | ((uint32_t)gx1 << 16);
                             ey = (uint32_t)gy0 | ((uint32_t)gy1 << 16);
+                            // CalculateBound's min z: min(min(min(A,B),C),D), A..D = P0,P1,P3,P2
+                            const float z0 = R[0].z, z1 = R[1].y, z2 = R[2].x, z3 = R[2].w;
+                            emz = fmin_ref(fmin_ref(fmin_ref(z0, z1), z3), z2);
+                            ebd = r7;
                         }
                     } else {
                         const float* kb0 = &kbs[r][0];
@@ -1484,12 +1518,12 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                         const float T0 = r6.z, T1 = r6.w;
                         const int d = divs[r];
                         const float dt = dts[r];
-                        float time = opentime + dt;
+                        float time = opentime + dt, tp = 0.0f;
                         for (int i = 0; i < b; ++i) {
                             tp = time;
                             time += dt;
                         }
-                        tc = time;
+                        const float tc = time;
                         const float mixc = (tc - T0) / (T1 - T0);
                         const float mixp = (tp - T0) / (T1 - T0);
                         float sb[6];
@@ -1527,6 +1561,9 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                                      ((uint32_t)idxEnd << 24);
                                 ex = (uint32_t)gx0 | ((uint32_t)gx1 << 16);
                                 ey = (uint32_t)gy0 | ((uint32_t)gy1 << 16);
+                                emz = sb[2];
+                                ebd = make_float4(sb[0], sb[3], sb[1], sb[4]);
+                                ett = make_float2(time0, time1);
                             }
                         }
                     }
@@ -1542,7 +1579,9 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                 }
                 if (keep) {
                     const int pos = before + __popcll(m & lanesBelow);
-                    eA[pos] = ea; eX[pos] = ex; eY[pos] = ey; eTp[pos] = tp; eTc[pos] = tc;
+                    E0[pos] = make_uint4(ea, ex, ey, __float_as_uint(emz));
+                    E1[pos] = ebd;
+                    E2[pos] = ett;
                 }
                 nent += total;
                 __syncthreads();
@@ -1555,7 +1594,8 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                         const int e = g + lane;
                         bool rel = false;
                         if (e < nent) {
-                            const uint32_t xr = eX[e], yr = eY[e];
+                            const uint4 h = E0[e];
+                            const uint32_t xr = h.y, yr = h.z;
                             rel = ((int)(xr & 0xffffu) < bgx1) & ((int)(xr >> 16) > bgx0) &
                                   ((int)(yr & 0xffffu) < bgy1) & ((int)(yr >> 16) > bgy0);
                         }
@@ -1566,85 +1606,104 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                     }
                     if (fill == 0) break;
                     const int nq = fill < 64 ? fill : 64;
-                    // the entries visiting this lane's sample, one bit each
+                    // The entries visiting this lane's sample (every visit
+                    // is a sample test), and of those the ones passing the
+                    // tests that do not depend on earlier visits: the
+                    // sub-bound's time window, its bound (Contains2D) and
+                    // LOD (bucketprocessor.cpp:1393-1420; a static entry's
+                    // times are unbounded).
                     unsigned long long vm = 0;
+                    uint32_t nVisit = 0;
                     for (int q = 0; q < nq; ++q) {
                         const int e = buf[wave][q];
-                        const uint32_t a = eA[e], xr = eX[e], yr = eY[e];
-                        const bool in = live & (gx >= (int)(xr & 0xffffu)) & (gx < (int)(xr >> 16)) &
-                                        (gy >= (int)(yr & 0xffffu)) & (gy < (int)(yr >> 16)) &
-                                        (si >= (int)((a >> 16) & 0xffu)) & (si < (int)(a >> 24));
-                        vm |= (unsigned long long)in << q;
+                        const uint4 h = E0[e];
+                        const float4 bd = E1[e];
+                        const float2 tt = E2[e];
+                        const bool visit = live & (gx >= (int)(h.y & 0xffffu)) & (gx < (int)(h.y >> 16)) &
+                                           (gy >= (int)(h.z & 0xffffu)) & (gy < (int)(h.z >> 16)) &
+                                           (si >= (int)((h.x >> 16) & 0xffu)) & (si < (int)(h.x >> 24));
+                        const bool pass = visit & !(stime < tt.x || stime > tt.y) &
+                                          !(sx < bd.x || sx > bd.y || sy < bd.z || sy > bd.w) & lodOk;
+                        nVisit += visit ? 1u : 0u;
+                        vm |= (unsigned long long)pass << q;
                     }
-                    while (__ballot(vm != 0)) {
-                        const bool act = vm != 0;
-                        const int q = act ? __ffsll((long long)vm) - 1 : 0;
-                        vm &= vm - 1;
-                        bool hit = false, opaque = false;
-                        float D = 0.f, u = 0.f, v = 0.f;
-                        uint32_t hid = 0;
-                        int hgrid = 0;
-                        if (act) {
-                            const int e = buf[wave][q];
-                            const uint32_t a = eA[e];
-                            const int r = (int)(a & 0xffu), b = (int)((a >> 8) & 0xffu);
-                            const float4* R = &recs[r * 8];
-                            const float4 r6 = R[6];
-                            const uint32_t flags = __float_as_uint(r6.x);
-                            const uint32_t code = flags & 0xffu;
-                            const bool degen = (flags >> 8) & 1u;
-                            opaque = (flags >> 10) & 1u;
-                            hid = ids[r];
-                            hgrid = __float_as_int(r6.y);
-                            ++nTest;
-                            if (b == 0xff) {
-                                const float4 r0 = R[0], r1 = R[1], r2 = R[2], r7 = R[7];
+                    nTest += nVisit;
+                    // Two passes over the visits of these entries.  A: every
+                    // visit's tests but the occlusion test, 64 visits a step
+                    // whichever samples they belong to, taken in rank-major
+                    // order (each lane's first visit, then each lane's
+                    // second, ...), results into a ring in LDS.  B: once a
+                    // rank is complete, each lane applies its visit of that
+                    // rank in order: occlusion test, counters, StoreSample.
+                    const int cnt = __popcll(vm);
+                    int maxc = cnt;
+                    for (int off = 32; off > 0; off >>= 1) maxc = max(maxc, __shfl_xor(maxc, off));
+                    int total = 0;
+                    for (int r = 0; r < maxc; ++r) {
+                        const unsigned long long lm = __ballot(cnt > r);
+                        if (lane == 0) {
+                            rankStart[wave][r] = total;
+                            rankLanes[wave][r] = lm;
+                        }
+                        total += __popcll(lm);
+                    }
+                    if (lane == 0) rankStart[wave][maxc] = total;
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    const uint32_t vmLo = (uint32_t)vm, vmHi = (uint32_t)(vm >> 32);
+                    int done = 0, rB = 0;
+                    while (done < total) {
+                        // A: visits [done, done + 64)
+                        {
+                            const int p = done + lane;
+                            const bool pv = p < total;
+                            int rk = 0;
+                            {
+                                int lo = 0, hi = maxc - 1;
+                                while (lo < hi) {
+                                    const int mid = (lo + hi + 1) >> 1;
+                                    if (rankStart[wave][mid] <= p) lo = mid;
+                                    else hi = mid - 1;
+                                }
+                                rk = lo;
+                            }
+                            const int own = pv ? select_bit(rankLanes[wave][rk], p - rankStart[wave][rk]) : lane;
+                            const unsigned long long vmo =
+                                ((unsigned long long)(uint32_t)__shfl((int)vmHi, own) << 32) | (uint32_t)__shfl((int)vmLo, own);
+                            const float osx = __shfl(sx, own), osy = __shfl(sy, own), ost = __shfl(stime, own);
+                            uint32_t fl = 0;
+                            float D = 0.f, u = 0.f, v = 0.f;
+                            int q = 0;
+                            if (pv) {
+                                q = select_bit(vmo, rk);
+                                const int e = buf[wave][q];
+                                const uint32_t a = E0[e].x;
+                                const int r = (int)(a & 0xffu), b = (int)((a >> 8) & 0xffu);
+                                const float4* R = &recs[r * 8];
+                                const float4 r6 = R[6];
+                                const uint32_t flags = __float_as_uint(r6.x);
+                                const uint32_t code = flags & 0xffu;
+                                const bool degen = (flags >> 8) & 1u;
+                                const float4 r0 = R[0], r1 = R[1], r2 = R[2];
                                 const f3d K0[4] = {mk3(r0.x, r0.y, r0.z), mk3(r0.w, r1.x, r1.y),
                                                    mk3(r1.z, r1.w, r2.x), mk3(r2.y, r2.z, r2.w)};
-                                // CalculateBound's min z: min(min(min(A,B),C),D), A..D = P0,P1,P3,P2
-                                const float bminz = fmin_ref(fmin_ref(fmin_ref(K0[0].z, K0[1].z), K0[3].z), K0[2].z);
-                                const bool inb = !(sx < r7.x || sx > r7.y || sy < r7.z || sy > r7.w);
-                                if (inb && !(bminz > occlZ) && lodOk) {
-                                    ++nBound;
-                                    const MBHit h = quad_hit(K0[0], K0[1], K0[2], K0[3], code, degen, sx, sy);
-                                    if (h.in) {
-                                        ++nIn;
-                                        if (!h.linear) ++nIn2;
-                                        hit = !(occlZ <= h.D);
-                                        D = h.D; u = h.u; v = h.v;
-                                    }
-                                }
-                            } else {
-                                const float* kb0 = &kbs[r][0];
-                                const float* kb1 = &kbs[r][6];
-                                const float T0 = r6.z, T1 = r6.w;
-                                const int d = divs[r];
-                                const float dt = dts[r];
-                                const float tc = eTc[e], tp = eTp[e];
-                                const float time0 = tc - dt;
-                                const float time1 = (b != d - 1) ? (tc + dt) - dt : closetime;
-                                const float mixc = (tc - T0) / (T1 - T0);
-                                const float mixp = (tp - T0) / (T1 - T0);
-                                float sb[5];
-#pragma unroll
-                                for (int c = 0; c < 5; ++c) {
-                                    const float cur = kb0[c] + mixc * (kb1[c] - kb0[c]);
-                                    const float prev = b == 0 ? kb0[c] : kb0[c] + mixp * (kb1[c] - kb0[c]);
-                                    sb[c] = c < 3 ? fmin_ref(prev, cur) : fmax_ref(prev, cur);
-                                }
-                                const bool inb = !(sx < sb[0] || sx > sb[3] || sy < sb[1] || sy > sb[4]);
-                                if (!(stime < time0 || stime > time1) && inb && !(sb[2] > occlZ) && lodOk) {
-                                    ++nBound;
-                                    // CqMicroPolygonMotion::Sample: the key span of the time
+                                bool tight = true;
+                                f3d pts[4] = {K0[0], K0[1], K0[2], K0[3]};
+                                if (b != 0xff) {
+                                    // CqMicroPolygonMotion::Sample: the key span of
+                                    // the time, the tight bound and the vertices there
+                                    const float* kb0 = &kbs[r][0];
+                                    const float* kb1 = &kbs[r][6];
+                                    const float T0 = r6.z, T1 = r6.w;
                                     int iIndex = 0;
                                     float Fraction = 0.0f;
                                     bool Exact = true;
-                                    if (stime > T0) {
-                                        if (stime >= T1) {
+                                    if (ost > T0) {
+                                        if (ost >= T1) {
                                             iIndex = 1;
                                         } else {
-                                            Fraction = (stime - T0) / (T1 - T0);
-                                            Exact = (T0 == stime);
+                                            Fraction = (ost - T0) / (T1 - T0);
+                                            Exact = (T0 == ost);
                                         }
                                     }
                                     const float* kbi = iIndex ? kb1 : kb0;
@@ -1658,69 +1717,106 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                                         tb[2] = F1 * kb0[1] + Fraction * kb1[1];
                                         tb[3] = F1 * kb0[4] + Fraction * kb1[4];
                                     }
-                                    if (!(sx < tb[0] || sx > tb[1] || sy < tb[2] || sy > tb[3])) {
-                                        const float4 r0 = R[0], r1 = R[1], r2 = R[2], r3 = R[3], r4 = R[4], r5 = R[5];
-                                        const f3d K0[4] = {mk3(r0.x, r0.y, r0.z), mk3(r0.w, r1.x, r1.y),
-                                                           mk3(r1.z, r1.w, r2.x), mk3(r2.y, r2.z, r2.w)};
-                                        const f3d K1[4] = {mk3(r3.x, r3.y, r3.z), mk3(r3.w, r4.x, r4.y),
-                                                           mk3(r4.z, r4.w, r5.x), mk3(r5.y, r5.z, r5.w)};
-                                        // a key's vertices when the time is exact, else
-                                        // (1 - F) K0 + F K1; selected per component
-                                        const float F1 = 1.0f - Fraction;
-                                        const bool k1 = iIndex != 0;
-                                        f3d pts[4];
+                                    tight = !(osx < tb[0] || osx > tb[1] || osy < tb[2] || osy > tb[3]);
+                                    const float4 r3 = R[3], r4 = R[4], r5 = R[5];
+                                    const f3d K1[4] = {mk3(r3.x, r3.y, r3.z), mk3(r3.w, r4.x, r4.y),
+                                                       mk3(r4.z, r4.w, r5.x), mk3(r5.y, r5.z, r5.w)};
+                                    // a key's vertices when the time is exact, else
+                                    // (1 - F) K0 + F K1; selected per component
+                                    const float F1 = 1.0f - Fraction;
+                                    const bool k1 = iIndex != 0;
 #pragma unroll
-                                        for (int k = 0; k < 4; ++k) {
-                                            const f3d mixed = mk3((F1 * K0[k].x) + (Fraction * K1[k].x),
-                                                                  (F1 * K0[k].y) + (Fraction * K1[k].y),
-                                                                  (F1 * K0[k].z) + (Fraction * K1[k].z));
-                                            pts[k].x = Exact ? (k1 ? K1[k].x : K0[k].x) : mixed.x;
-                                            pts[k].y = Exact ? (k1 ? K1[k].y : K0[k].y) : mixed.y;
-                                            pts[k].z = Exact ? (k1 ? K1[k].z : K0[k].z) : mixed.z;
-                                        }
-                                        const MBHit h = quad_hit(pts[0], pts[1], pts[2], pts[3], code, degen, sx, sy);
-                                        if (h.in) {
-                                            ++nIn;
-                                            if (!h.linear) ++nIn2;
-                                            hit = !(occlZ <= h.D);
-                                            D = h.D; u = h.u; v = h.v;
-                                        }
+                                    for (int k = 0; k < 4; ++k) {
+                                        const f3d mixed = mk3((F1 * K0[k].x) + (Fraction * K1[k].x),
+                                                              (F1 * K0[k].y) + (Fraction * K1[k].y),
+                                                              (F1 * K0[k].z) + (Fraction * K1[k].z));
+                                        pts[k].x = Exact ? (k1 ? K1[k].x : K0[k].x) : mixed.x;
+                                        pts[k].y = Exact ? (k1 ? K1[k].y : K0[k].y) : mixed.y;
+                                        pts[k].z = Exact ? (k1 ? K1[k].z : K0[k].z) : mixed.z;
+                                    }
+                                }
+                                if (tight) {
+                                    const MBHit h = quad_hit(pts[0], pts[1], pts[2], pts[3], code, degen, osx, osy);
+                                    if (h.in) {
+                                        fl = 2u | (h.linear ? 4u : 0u);
+                                        D = h.D; u = h.u; v = h.v;
+                                    }
+                                }
+                                const int slot = p & (RH_MB_RING - 1);
+                                ringCode[wave][slot] = (uint32_t)q | (fl << 8);
+                                ringD[wave][slot] = D;
+                                ringU[wave][slot] = u;
+                                ringV[wave][slot] = v;
+                            }
+                        }
+                        done = min(done + 64, total);
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        // B: the ranks now complete, in order
+                        while (rB < maxc && rankStart[wave][rB + 1] <= done) {
+                            const bool act = cnt > rB;
+                            const unsigned long long lm = __ballot(act);
+                            bool hit = false, opaque = false;
+                            float D = 0.f, u = 0.f, v = 0.f;
+                            uint32_t hid = 0;
+                            int hgrid = 0;
+                            if (act) {
+                                const int slot = (rankStart[wave][rB] + __popcll(lm & lanesBelow)) & (RH_MB_RING - 1);
+                                const uint32_t rc = ringCode[wave][slot];
+                                const int e = buf[wave][rc & 0xffu];
+                                const uint4 h = E0[e];
+                                const float mz = __uint_as_float(h.w);
+                                const int r = (int)(h.x & 0xffu);
+                                const float4 r6 = recs[r * 8 + 6];
+                                opaque = (__float_as_uint(r6.x) >> 10) & 1u;
+                                hid = ids[r];
+                                hgrid = __float_as_int(r6.y);
+                                if (!(mz > occlZ)) {
+                                    ++nBound;
+                                    if (rc & 0x200u) {
+                                        ++nIn;
+                                        if (!(rc & 0x400u)) ++nIn2;
+                                        D = ringD[wave][slot];
+                                        u = ringU[wave][slot];
+                                        v = ringV[wave][slot];
+                                        hit = !(occlZ <= D);
                                     }
                                 }
                             }
-                        }
-                        // StoreSample (bucketprocessor.cpp:1471-1569)
-                        nHit += hit ? 1u : 0u;
-                        const bool opq = hit && opaque;
-                        occlZ = opq ? D : occlZ;
-                        win = opq ? hid : win;
-                        wu = opq ? u : wu;
-                        wv = opq ? v : wv;
-                        const bool app = hit && !opaque;
-                        const unsigned long long am = __ballot(app);
-                        if (am) {
-                            const int na = __popcll(am);
-                            if (na > wbLeft) {
-                                uint32_t base = 0;
-                                if (lane == 0) base = atomicAdd(hitCounter, 64u);
-                                wbBase = __shfl(base, 0);
-                                wbLeft = 64;
-                            }
-                            if (app) {
-                                const uint32_t slot = wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & lanesBelow);
-                                if (slot < fr->hitCapacity && nh < RH_MAX_HITS - 1) {
-                                    float4 hA, hB;
-                                    shade_hit(prims[grids[hgrid].prim], u, v, D, hid, hA, hB);
-                                    hitData[2 * (size_t)slot] = hA;
-                                    hitData[2 * (size_t)slot + 1] = hB;
-                                    hitNext[slot] = head;
-                                    head = slot;
-                                    ++nh;
-                                } else if (slot < fr->hitCapacity) {
-                                    lost = true;
+                            // StoreSample (bucketprocessor.cpp:1471-1569)
+                            nHit += hit ? 1u : 0u;
+                            const bool opq = hit && opaque;
+                            occlZ = opq ? D : occlZ;
+                            win = opq ? hid : win;
+                            wu = opq ? u : wu;
+                            wv = opq ? v : wv;
+                            const bool app = hit && !opaque;
+                            const unsigned long long am = __ballot(app);
+                            if (am) {
+                                const int na = __popcll(am);
+                                if (na > wbLeft) {
+                                    uint32_t base = 0;
+                                    if (lane == 0) base = atomicAdd(hitCounter, 64u);
+                                    wbBase = __shfl(base, 0);
+                                    wbLeft = 64;
                                 }
+                                if (app) {
+                                    const uint32_t slot = wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & lanesBelow);
+                                    if (slot < fr->hitCapacity && nh < RH_MAX_HITS - 1) {
+                                        float4 hA, hB;
+                                        shade_hit(prims[grids[hgrid].prim], u, v, D, hid, hA, hB);
+                                        hitData[2 * (size_t)slot] = hA;
+                                        hitData[2 * (size_t)slot + 1] = hB;
+                                        hitNext[slot] = head;
+                                        head = slot;
+                                        ++nh;
+                                    } else if (slot < fr->hitCapacity) {
+                                        lost = true;
+                                    }
+                                }
+                                wbLeft -= na;
                             }
-                            wbLeft -= na;
+                            ++rB;
                         }
                     }
                     // keep what the buffer holds past the first 64
