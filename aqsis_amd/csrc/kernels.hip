@@ -1614,6 +1614,7 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                     // times are unbounded).
                     unsigned long long vm = 0;
                     uint32_t nVisit = 0;
+#pragma unroll 4
                     for (int q = 0; q < nq; ++q) {
                         const int e = buf[wave][q];
                         const uint4 h = E0[e];
