@@ -3,10 +3,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdint>
 #include <mutex>
 #include <cstring>
+#include <list>
+#include <memory>
 #include <new>
 #include <string>
 #include <thread>
@@ -193,6 +196,64 @@ static void keep_pool_memory() {
     });
 }
 
+// Sample plans of recent frame shapes.  A plan (the multijitter tables, the
+// per-pixel pattern draws in bucket order and the dither draws) is a function
+// of the frame shape alone, and drawing it is the dearest part of a context
+// (about 0.24 s at 2048x2048, one sequential MT19937 stream); a renderer or server
+// that re-creates contexts of shapes it has seen takes a copy instead.
+// Least recently used plans go first beyond kPlanCacheBytes.
+namespace {
+using PlanKey = std::array<int, 14>;
+struct PlanCache {
+    std::mutex m;
+    std::list<std::pair<PlanKey, std::shared_ptr<const rh::Plan>>> lru;   // front: most recent
+    size_t bytes = 0;
+};
+PlanCache& plan_cache() {
+    static PlanCache pc;
+    return pc;
+}
+constexpr size_t kPlanCacheBytes = size_t(2) << 30;
+
+size_t plan_bytes(const rh::Plan& p) {
+    return (p.pos2d.size() + p.pos1d.size() + p.dither.size()) * sizeof(float) + p.pixPattern.size();
+}
+
+// Everything make_plan reads from a shape.
+PlanKey plan_key(const rh::Shape& s) {
+    return {s.o.xres, s.o.yres, s.cropX0, s.cropX1, s.cropY0, s.cropY1, s.o.bucket_x, s.o.bucket_y,
+            s.shiftX, s.shiftY, s.nx, s.ny, s.o.jitter ? 1 : 0, s.o.num_displays};
+}
+
+void cached_plan(const rh::Shape& s, rh::Plan& out) {
+    const PlanKey key = plan_key(s);
+    PlanCache& pc = plan_cache();
+    {
+        std::lock_guard<std::mutex> lk(pc.m);
+        for (auto it = pc.lru.begin(); it != pc.lru.end(); ++it)
+            if (it->first == key) {
+                pc.lru.splice(pc.lru.begin(), pc.lru, it);
+                out = *pc.lru.front().second;
+                return;
+            }
+    }
+    auto p = std::make_shared<rh::Plan>();
+    rh::make_plan(s, *p);
+    out = *p;
+    const size_t b = plan_bytes(*p);
+    if (b > kPlanCacheBytes / 4) return;
+    std::lock_guard<std::mutex> lk(pc.m);
+    for (const auto& e : pc.lru)
+        if (e.first == key) return;       // another thread drew the same plan
+    pc.lru.emplace_front(key, std::move(p));
+    pc.bytes += b;
+    while (pc.bytes > kPlanCacheBytes && !pc.lru.empty()) {
+        pc.bytes -= plan_bytes(*pc.lru.back().second);
+        pc.lru.pop_back();
+    }
+}
+}  // namespace
+
 extern "C" {
 
 rh_status rh_create(const rh_options* opt, rh_ctx** out) {
@@ -205,7 +266,7 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
     std::memset(&c->stats, 0, sizeof(c->stats));
     std::string err;
     if (!rh::make_shape(*opt, c->shape, err)) { delete c; return fail(RH_ERR_INVALID, err); }
-    rh::make_plan(c->shape, c->plan);
+    cached_plan(c->shape, c->plan);
     const rh::Shape& s = c->shape;
     keep_pool_memory();
     hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);

@@ -73,6 +73,7 @@ struct Job {
     std::vector<float> channels;
     std::vector<uint8_t> rgba8;
     rh_stats stats;
+    double msSetup = 0.0, msRender = 0.0, msReadback = 0.0;
     Clock::time_point tSubmit, tStart, tEnd;
 };
 
@@ -211,11 +212,14 @@ struct rh_server {
 
     void run(Job* j, Resident& r) {
         rh_status st = RH_OK;
+        const Clock::time_point t0 = Clock::now();
         if (!r.ctx) st = rh_create(&j->opt, &r.ctx);
+        const Clock::time_point t1 = Clock::now();
         if (st == RH_OK)
             st = rh_internal_post_scheduled(r.ctx, j->prims.data(), (int32_t)j->prims.size(), std::move(j->sched),
                                             j->msSchedule);
         if (st == RH_OK) st = rh_render(r.ctx);
+        const Clock::time_point t2 = Clock::now();
         if (st == RH_OK) st = rh_frame_size(r.ctx, &j->width, &j->height, nullptr, nullptr, nullptr);
         if (st == RH_OK) {
             j->channels.resize((size_t)j->width * j->height * 9);
@@ -223,6 +227,9 @@ struct rh_server {
             st = rh_read_channels(r.ctx, j->channels.data());
         }
         if (st == RH_OK) st = rh_read_rgba8(r.ctx, j->rgba8.data());
+        j->msSetup = ms_between(t0, t1);
+        j->msRender = ms_between(t1, t2);
+        j->msReadback = ms_between(t2, Clock::now());
         if (st == RH_OK) st = rh_get_stats(r.ctx, &j->stats);
         if (st != RH_OK) j->error = rh_last_error();
         j->status = st;
@@ -306,6 +313,10 @@ struct rh_server {
             out->ms_total = ms_between(j.tSubmit, j.tEnd);
             out->ms_device = j.stats.ms_device;
             out->spl_count = j.stats.spl_count;
+            out->ms_setup = j.msSetup;
+            out->ms_upload = j.stats.ms_upload;
+            out->ms_render = j.msRender;
+            out->ms_readback = j.msReadback;
         }
     }
 };

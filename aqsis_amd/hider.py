@@ -222,7 +222,8 @@ class RhJobInfo(C.Structure):
     _fields_ = [("id", C.c_uint64), ("status", C.c_int32), ("state", C.c_int32), ("width", C.c_int32),
                 ("height", C.c_int32), ("device_bytes", C.c_uint64), ("spl_count", C.c_uint64),
                 ("ms_queue", C.c_double), ("ms_run", C.c_double), ("ms_total", C.c_double),
-                ("ms_device", C.c_double)]
+                ("ms_device", C.c_double), ("ms_setup", C.c_double), ("ms_upload", C.c_double),
+                ("ms_render", C.c_double), ("ms_readback", C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

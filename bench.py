@@ -235,7 +235,12 @@ def serving(args):
         # set-up, upload, frame, read-back) and the device frame
         "mean_ms": {"queue_incl_schedule": mean([infos[i]["ms_queue"] for i in done]),
                     "run": mean([infos[i]["ms_run"] for i in done]),
+                    "run_setup": mean([infos[i]["ms_setup"] for i in done]),
+                    "run_upload": mean([infos[i]["ms_upload"] for i in done]),
+                    "run_render": mean([infos[i]["ms_render"] for i in done]),
+                    "run_readback": mean([infos[i]["ms_readback"] for i in done]),
                     "device": mean([infos[i]["ms_device"] for i in done])},
+        "contexts_created": sum(1 for i in done if infos[i]["ms_setup"] > 1.0),
         "device_memory_high_water_gb": high / 2 ** 30,
         "failed": len(errs),
         "errors": errs[:5],

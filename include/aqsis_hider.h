@@ -282,6 +282,11 @@ typedef struct rh_job_info {
     double ms_run;             /* admitted -> done (host schedule + device)     */
     double ms_total;           /* submit -> done                                */
     double ms_device;          /* device frame time                             */
+    double ms_setup;           /* frame-shape context created for the job (0 when
+                                  a resident one was reused)                    */
+    double ms_upload;          /* host->device copies of the job's inputs       */
+    double ms_render;          /* inputs posted, uploaded and rendered          */
+    double ms_readback;        /* channels and 8-bit image copied back          */
 } rh_job_info;
 
 typedef struct rh_server rh_server;

@@ -65,3 +65,22 @@ def test_rerender_identical(oracle):
         b = (h.channels(), h.rgba8(), h.samples()[0])
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.gpu
+def test_contexts_of_one_shape_render_alike(oracle):
+    """A second context of a frame shape takes its sample plan from the
+    process's plan cache; it renders exactly what the first did and what the
+    oracle draws for that shape."""
+    from aqsis_amd.hider import Hider
+    opts, prims = CASES["patches_transparent"]()
+    outs = []
+    for _ in range(2):
+        with Hider(opts) as h:
+            h.post_surfaces(prims)
+            h.render()
+            outs.append((h.channels(), h.rgba8(), h.samples()[0]))
+    for x, y in zip(*outs):
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+    ref = oracle.render(opts, prims, occlusion_cull=False)
+    assert np.array_equal(outs[1][2], ref["mp_id"])
