@@ -27,7 +27,8 @@ def _same(a, b):
             assert np.array_equal(a[k], b[k]), k
 
 
-@pytest.mark.parametrize("case", ["patches_small", "patches_transparent", "spheres_config", "patches_8x8"])
+@pytest.mark.parametrize("case", ["patches_small", "patches_transparent", "spheres_config", "patches_8x8",
+                                  "motion_mixed_opaque"])
 def test_layouts_give_identical_frames(case):
     from aqsis_amd.hider import Hider
     opts, prims = CASES[case]()
