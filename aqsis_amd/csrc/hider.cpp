@@ -20,7 +20,7 @@
 #include "front.h"
 
 extern "C" hipError_t rh_launch_dice(hipStream_t, const FrameDev*, const GridDev*, int, const PrimDev*,
-                                     const float*, float4*, float*, float4*, float4*);
+                                     const float*, float*, float4*, float4*);
 extern "C" hipError_t rh_launch_sample_mb(hipStream_t, const SampleLaunch*, int, const float4*);
 extern "C" hipError_t rh_launch_bin(hipStream_t, const FrameDev*, const BucketDev*, int, const TaskDev*,
                                     const GridDev*, const uint32_t*, const uint32_t*, const uint32_t*, size_t,
@@ -115,7 +115,7 @@ struct rh_ctx {
     // schedule
     DBuf dGrids, dParams, dPrims, dCandStart, dCandGrid, dCapStart, dSegBucket, dSegCounts;
     // per frame
-    DBuf dVtx, dMprec, dMpbound, dLists, dListCount, dSamples, dChannels, dRgba, dCounters, dBucketRefs;
+    DBuf dMprec, dMpbound, dLists, dListCount, dSamples, dChannels, dRgba, dCounters, dBucketRefs;
     DBuf dMprecMB;                        // motion-frame micropolygon records
     // transparent hits: arena of (Ci, depth), (Oi, id) pairs and list links
     DBuf dHitData, dHitNext, dHitCtl, dCombine, dFrontDepth;
@@ -138,7 +138,7 @@ struct rh_ctx {
     template <typename F> void each_buffer(F f) {
         DBuf* all[] = {&dFrame, &dBuckets, &dTasks, &dPos2d, &dPos1d, &dPix, &dDither, &dFilter, &dBoxMask,
                        &dGrids, &dParams,
-                       &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dVtx, &dMprec, &dMpbound, &dLists,
+                       &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dMprec, &dMpbound, &dLists,
                        &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dBucketRefs,
                        &dHitData, &dHitNext, &dHitCtl, &dCombine, &dFrontDepth, &dItemsStatic, &dItems,
                        &dItemCount, &dQueue, &dRows, &dSegBucket, &dSegCounts, &dMprecMB};
@@ -476,7 +476,6 @@ static rh_status upload_schedule(rh_ctx* c) {
     HIP_TRY(upload(c->dSegBucket, S.segBucket, c->st));
     HIP_TRY(c->dSegCounts.ensure(S.candGrid.size() * 17 * sizeof(uint32_t)));
     HIP_TRY(upload(c->dCapStart, cap, c->st));
-    HIP_TRY(c->dVtx.ensure(S.vertices * sizeof(float4)));
     HIP_TRY(c->dMprec.ensure(S.micropolygons * RH_MPREC_FLOATS * sizeof(float)));
     HIP_TRY(c->dMpbound.ensure(S.micropolygons * sizeof(float4)));
     HIP_TRY(c->dLists.ensure((size_t)S.capStart.back() * sizeof(uint32_t)));
@@ -519,7 +518,7 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
     if (timed) ENQ(hipEventRecord(c->ev[0], c->st));
     ENQ(rh_launch_frame_begin(c->st, c->dCounters.as<unsigned long long>(), c->dHitCtl.as<uint32_t>()));
     ENQ(rh_launch_dice(c->st, f, c->dGrids.as<GridDev>(), (int)S.grids.size(), c->dPrims.as<PrimDev>(),
-                       c->dParams.as<float>(), c->dVtx.as<float4>(), c->dMprec.as<float>(), c->dMpbound.as<float4>(),
+                       c->dParams.as<float>(), c->dMprec.as<float>(), c->dMpbound.as<float4>(),
                        S.motion ? c->dMprecMB.as<float4>() : nullptr));
     if (timed) ENQ(hipEventRecord(c->ev[1], c->st));
     ENQ(rh_launch_bin(c->st, f, c->dBuckets.as<BucketDev>(), nb, c->dTasks.as<TaskDev>(), c->dGrids.as<GridDev>(),
@@ -803,7 +802,7 @@ void rh_destroy(rh_ctx* c) { delete c; }
 uint64_t rh_internal_device_bytes(const rh_ctx* c) {
     const DBuf* all[] = {&c->dFrame, &c->dBuckets, &c->dTasks, &c->dPos2d, &c->dPos1d, &c->dPix, &c->dDither,
                          &c->dFilter, &c->dBoxMask, &c->dGrids, &c->dParams, &c->dPrims, &c->dCandStart,
-                         &c->dCandGrid, &c->dCapStart, &c->dSegBucket, &c->dSegCounts, &c->dVtx, &c->dMprec, &c->dMpbound, &c->dLists,
+                         &c->dCandGrid, &c->dCapStart, &c->dSegBucket, &c->dSegCounts, &c->dMprec, &c->dMpbound, &c->dLists,
                          &c->dListCount, &c->dSamples, &c->dChannels, &c->dRgba, &c->dCounters,
                          &c->dBucketRefs, &c->dHitData, &c->dHitNext, &c->dHitCtl, &c->dCombine, &c->dFrontDepth,
                          &c->dItemsStatic, &c->dItems, &c->dItemCount, &c->dQueue, &c->dRows, &c->dMprecMB};
@@ -825,7 +824,7 @@ static uint64_t frame_bytes(const rh::Shape& s) {
 // Device bytes of one frame of a scheduled scene.
 static uint64_t footprint_bytes(const rh::Shape& s, const rh::Schedule& S, const rh_prim* prims, int32_t n) {
     uint64_t bytes = frame_bytes(s) + S.grids.size() * sizeof(GridDev) + S.params.size() * 4 +
-                     (uint64_t)n * sizeof(PrimDev) + S.candGrid.size() * (4 + 4 + 17 * 4) + S.vertices * 16 +
+                     (uint64_t)n * sizeof(PrimDev) + S.candGrid.size() * (4 + 4 + 17 * 4) +
                      S.micropolygons * (RH_MPREC_FLOATS * 4 + 16) + S.capStart.back() * 4 +
                      (S.motion ? S.micropolygons * RH_MPREC_MB_FLOATS * 4 : 0);
     bool transparent = false;

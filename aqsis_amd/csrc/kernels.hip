@@ -252,7 +252,7 @@ __device__ __forceinline__ int key_param_floats(int type, int cu, int cv) {
 
 __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__ fr, const GridDev* __restrict__ grids,
                                                     const PrimDev* __restrict__ prims,
-                                                    const float* __restrict__ params, float4* __restrict__ vtx,
+                                                    const float* __restrict__ params,
                                                     float* __restrict__ mprec, float4* __restrict__ mpbound,
                                                     float4* __restrict__ mprecMB) {
     __shared__ float4 V[RH_MAX_GRID_VERTS];
@@ -271,9 +271,7 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
     for (int i = tid; i < nv; i += blockDim.x) {
         float4 p = V[i];
         f3d r = xform_point(fr->camToRaster, false, mk3(p.x, p.y, p.z));
-        float4 q = make_float4(r.x, r.y, p.z, 0.0f);
-        V[i] = q;
-        vtx[G.vtxBase + i] = q;
+        V[i] = make_float4(r.x, r.y, p.z, 0.0f);
     }
     // A deforming surface's grid: key 1 diced with key 0's sizes and
     // projected the same way (CqDeformingSurface::Dice, surface.h:643-655;
@@ -286,9 +284,7 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
         for (int i = tid; i < nv; i += blockDim.x) {
             float4 p = V1[i];
             f3d r = xform_point(fr->camToRaster, false, mk3(p.x, p.y, p.z));
-            float4 q = make_float4(r.x, r.y, p.z, 0.0f);
-            V1[i] = q;
-            vtx[G.vtxBase + nv + i] = q;
+            V1[i] = make_float4(r.x, r.y, p.z, 0.0f);
         }
     }
     __syncthreads();
@@ -2295,10 +2291,10 @@ __global__ void __launch_bounds__(128) rh_filter_tiled(const FrameDev* __restric
 // Host-side launchers
 // ---------------------------------------------------------------------------
 extern "C" hipError_t rh_launch_dice(hipStream_t st, const FrameDev* fr, const GridDev* grids, int ngrids,
-                                     const PrimDev* prims, const float* params, float4* vtx, float* mprec,
+                                     const PrimDev* prims, const float* params, float* mprec,
                                      float4* mpbound, float4* mprecMB) {
     if (ngrids <= 0) return hipSuccess;
-    hipLaunchKernelGGL(rh_dice_bust, dim3(ngrids), dim3(256), 0, st, fr, grids, prims, params, vtx, mprec,
+    hipLaunchKernelGGL(rh_dice_bust, dim3(ngrids), dim3(256), 0, st, fr, grids, prims, params, mprec,
                        mpbound, mprecMB);
     return hipGetLastError();
 }
