@@ -192,6 +192,9 @@ def serving(args):
                 with lock:
                     k = ids[info["id"]]
                 finished += 1
+                if finished % 100 == 0:
+                    print(f"serving: {finished} of {len(reqs)} jobs done at {now - t_start:.1f} s", file=sys.stderr,
+                          flush=True)
                 if info["status"] != 0:
                     errs.append(f"job {k}: status {info['status']}")
                 else:
