@@ -64,6 +64,15 @@ rh_status fail(rh_status code, const std::string& msg) {
 // device's memory pool: growing or freeing one does not wait for other
 // contexts' work, and freed memory is reused without returning it to the
 // driver (keep_pool_memory).
+// Hand the device pool's unused memory back to the driver until the pool
+// holds at most `keep` bytes.
+void release_pool_memory(uint64_t keep) {
+    int dev = 0;
+    hipMemPool_t pool;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
+        (void)hipMemPoolTrimTo(pool, (size_t)keep);
+}
+
 struct DBuf {
     void* p = nullptr;
     size_t cap = 0;
@@ -76,6 +85,14 @@ struct DBuf {
         cap = 0;
         size_t n = bytes + bytes / 8 + 256;
         hipError_t e = hipMallocAsync(&p, n, st);
+        if (e == hipErrorOutOfMemory) {
+            // memory the pool holds unused (freed by other buffers) goes
+            // back to the driver first
+            (void)hipGetLastError();
+            (void)hipStreamSynchronize(st);
+            release_pool_memory(0);
+            e = hipMallocAsync(&p, n, st);
+        }
         if (e == hipSuccess) cap = n;
         return e;
     }
@@ -798,7 +815,14 @@ void rh_destroy(rh_ctx* c) { delete c; }
 
 }  // extern "C"
 
-// Used by the serving loop (server.cpp).
+// Used by the serving loop (server.cpp): hand the device pool's unused memory
+// back to the driver until the pool holds at most `keep` bytes.  The pool
+// keeps freed memory for reuse; a server trims it to its budget after each
+// job and when it destroys contexts, so that neither its buffers nor what the
+// driver allocates itself (kernel scratch at launch) find the device full of
+// freed blocks.
+void rh_internal_trim_pool(uint64_t keep) { release_pool_memory(keep); }
+
 uint64_t rh_internal_device_bytes(const rh_ctx* c) {
     const DBuf* all[] = {&c->dFrame, &c->dBuckets, &c->dTasks, &c->dPos2d, &c->dPos1d, &c->dPix, &c->dDither,
                          &c->dFilter, &c->dBoxMask, &c->dGrids, &c->dParams, &c->dPrims, &c->dCandStart,

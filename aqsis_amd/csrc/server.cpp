@@ -28,6 +28,7 @@
 #include "front.h"
 
 uint64_t rh_internal_device_bytes(const rh_ctx* c);
+void rh_internal_trim_pool(uint64_t keep);
 bool rh_internal_schedule(const rh_options& opt, const rh_prim* prims, int32_t n, int threads, rh::Schedule& S,
                           uint64_t& bytes, double& ms, std::string& err);
 rh_status rh_internal_post_scheduled(rh_ctx* c, const rh_prim* prims, int32_t n, rh::Schedule&& S, double ms);
@@ -160,6 +161,8 @@ struct rh_server {
         }
         if (dead.empty()) return;
         for (auto& d : dead) rh_destroy(d.first);
+        // the pool holds what the dead contexts freed: keep only the budget
+        rh_internal_trim_pool(opt.memory_budget);
         {
             std::lock_guard<std::mutex> lk(m);
             for (auto& d : dead) pendingBytes -= d.second;
@@ -293,6 +296,9 @@ struct rh_server {
                 j->tEnd = Clock::now();
                 j->state = 2;
             }
+            // blocks the job's buffers outgrew stay in the pool: keep the
+            // pool within the budget
+            rh_internal_trim_pool(opt.memory_budget);
             cvDone.notify_all();
             cvWork.notify_all();
             bury();

@@ -196,7 +196,12 @@ def serving(args):
                     print(f"serving: {finished} of {len(reqs)} jobs done at {now - t_start:.1f} s", file=sys.stderr,
                           flush=True)
                 if info["status"] != 0:
-                    errs.append(f"job {k}: status {info['status']}")
+                    try:
+                        srv.result(info["id"], info["width"], info["height"])
+                        msg = ""
+                    except Exception as e:          # the job's own error message
+                        msg = str(e)
+                    errs.append(f"job {k}: status {info['status']} {msg}")
                 else:
                     lat[k] = (now - (t_start + reqs[k][2])) * 1e3   # from the scheduled arrival
                     infos[k] = info
