@@ -139,19 +139,21 @@ RH_HD int partition_pivot(float* d, uint32_t* x, int first, int last) {
     }
 }
 
-// std::sort(d, d + n) with x carried along; n <= 64.
-RH_HD void sort(float* d, uint32_t* x, int n) {
+// std::sort(d, d + n) with x carried along; n <= 64.  `stk` holds 24 ints
+// of work stack (caller storage, e.g. LDS).
+RH_HD void sort_with(float* d, uint32_t* x, int n, int* stk) {
     if (n <= 0) return;
     int lg = 0;
     while ((2 << lg) <= n) ++lg;          // __lg(n)
     // __introsort_loop: the two parts of a partition are disjoint, so they
     // may be finished in any order; each carries the decremented limit.
-    int stFirst[24], stLast[24], stDepth[24];
+    // A stack entry packs first | last << 8 | depth limit << 16.
     int sp = 0;
-    stFirst[0] = 0; stLast[0] = n; stDepth[0] = 2 * lg; sp = 1;
+    stk[0] = 0 | (n << 8) | ((2 * lg) << 16);
+    sp = 1;
     while (sp > 0) {
         --sp;
-        int first = stFirst[sp], last = stLast[sp], depth = stDepth[sp];
+        int first = stk[sp] & 0xff, last = (stk[sp] >> 8) & 0xff, depth = stk[sp] >> 16;
         while (last - first > 16) {
             if (depth == 0) {
                 heap_sort(d, x, first, last);
@@ -159,7 +161,8 @@ RH_HD void sort(float* d, uint32_t* x, int n) {
             }
             --depth;
             const int cut = partition_pivot(d, x, first, last);
-            stFirst[sp] = cut; stLast[sp] = last; stDepth[sp] = depth; ++sp;
+            stk[sp] = cut | (last << 8) | (depth << 16);
+            ++sp;
             last = cut;
         }
     }
@@ -170,6 +173,12 @@ RH_HD void sort(float* d, uint32_t* x, int n) {
     } else {
         insertion_sort(d, x, 0, n);
     }
+}
+
+// std::sort(d, d + n) with x carried along; n <= 64.
+RH_HD void sort(float* d, uint32_t* x, int n) {
+    int stk[24];
+    sort_with(d, x, n, stk);
 }
 
 }  // namespace rhsort

@@ -1944,13 +1944,17 @@ __global__ void rh_frame_begin(unsigned long long* __restrict__ counters, uint32
 // depth as std::sort does, then composited back to front; the depth is that
 // of the nearest hit whose opacity reaches zthreshold, else the occluding
 // depth; the front-most hit gives the id, and its depth goes to frontDepth.
-// One thread per listed sample.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+// One thread per listed sample, one wave per workgroup.
+#define RH_COMBINE_WG 64
+__global__ void __launch_bounds__(RH_COMBINE_WG)
 rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
                                                   const uint32_t* __restrict__ hitCounter,
                                                   const float4* __restrict__ hitData,
                                                   const uint32_t* __restrict__ hitNext, float4* __restrict__ samples,
                                                   float* __restrict__ frontDepth) {
+    __shared__ float cmbD[RH_COMBINE_WG * (RH_MAX_HITS + 1)];
+    __shared__ uint32_t cmbX[RH_COMBINE_WG * (RH_MAX_HITS + 1)];
+    __shared__ int cmbS[RH_COMBINE_WG * 25];
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= hitCounter[2]) return;
     const uint4 e = list[t];
@@ -1959,8 +1963,11 @@ rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
     int n = (int)e.w;
     const float4 oA = samples[2 * sidx], oB = samples[2 * sidx + 1];
     const float occlZ = oA.w;
-    float d[RH_MAX_HITS];
-    uint32_t x[RH_MAX_HITS];
+    // the hit list and the sort's work stack in LDS (no scratch memory),
+    // each lane's array an odd number of words from the next
+    float* d = cmbD + threadIdx.x * (RH_MAX_HITS + 1);
+    uint32_t* x = cmbX + threadIdx.x * (RH_MAX_HITS + 1);
+    int* stk = cmbS + threadIdx.x * 25;
     // the list is linked newest first; rebuild insertion order
     for (int q = n - 1; q >= 0; --q) {
         x[q] = head;
@@ -1972,7 +1979,7 @@ rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
         d[n] = oA.w;
         ++n;
     }
-    rhsort::sort(d, x, n);
+    rhsort::sort_with(d, x, n, stk);
     float col[3] = {0.0f, 0.0f, 0.0f}, opa[3] = {0.0f, 0.0f, 0.0f};
     float opaqueDepth = occlZ;
     const float zt0 = fr->zThreshold[0], zt1 = fr->zThreshold[1], zt2 = fr->zThreshold[2];
@@ -2413,7 +2420,8 @@ extern "C" hipError_t rh_launch_combine(hipStream_t st, const FrameDev* fr, size
                                         const uint32_t* hitCounter, const float4* hitData, const uint32_t* hitNext,
                                         float4* samples, float* frontDepth) {
     if (maxEntries == 0) return hipSuccess;
-    hipLaunchKernelGGL(rh_combine, dim3((unsigned)((maxEntries + 255) / 256)), dim3(256), 0, st, fr, list,
+    hipLaunchKernelGGL(rh_combine, dim3((unsigned)((maxEntries + RH_COMBINE_WG - 1) / RH_COMBINE_WG)),
+                       dim3(RH_COMBINE_WG), 0, st, fr, list,
                        hitCounter, hitData, hitNext, samples, frontDepth);
     return hipGetLastError();
 }
