@@ -75,10 +75,14 @@ struct Job {
     std::vector<uint8_t> rgba8;
     rh_stats stats;
     double msSetup = 0.0, msRender = 0.0, msReadback = 0.0;
-    Clock::time_point tSubmit, tStart, tEnd;
+    Clock::time_point tSubmit, tReady, tStart, tEnd;
 };
 
 }  // namespace
+
+// Backfill limits of the admission queue (see worker()).
+constexpr double kBackfillMs = 1000.0;
+constexpr int kBackfillDepth = 16;
 
 struct rh_server {
     rh_server_options opt;
@@ -132,6 +136,7 @@ struct rh_server {
                     finish_unrun(j, RH_ERR_CAPACITY, "job needs " + std::to_string(j->estimate) +
                                                          " device bytes, more than the server's budget");
                 } else {
+                    j->tReady = Clock::now();
                     queue.push_back(j);
                 }
             }
@@ -247,7 +252,31 @@ struct rh_server {
             {
                 std::unique_lock<std::mutex> lk(m);
                 for (;;) {
-                    if (stop || (!queue.empty() && admit(queue.front(), slot))) break;
+                    if (stop) break;
+                    if (!queue.empty()) {
+                        // The head job first.  While it has waited less than
+                        // kBackfillMs for room, a later job that fits may go
+                        // ahead of it (a large frame waiting for memory does
+                        // not hold back small ones); after that, strict order,
+                        // so the head is never starved.
+                        auto pick = queue.end();
+                        if (admit(queue.front(), slot)) {
+                            pick = queue.begin();
+                        } else if (ms_between(queue.front()->tReady, Clock::now()) < kBackfillMs) {
+                            int tried = 0;
+                            for (auto it = std::next(queue.begin()); it != queue.end() && tried < kBackfillDepth;
+                                 ++it, ++tried)
+                                if (admit(*it, slot)) {
+                                    pick = it;
+                                    break;
+                                }
+                        }
+                        if (pick != queue.end()) {
+                            j = *pick;
+                            queue.erase(pick);
+                            break;
+                        }
+                    }
                     if (!graveyard.empty()) {
                         // room is being made: destroy the evicted contexts, retry
                         lk.unlock();
@@ -255,11 +284,10 @@ struct rh_server {
                         lk.lock();
                         continue;
                     }
-                    cvWork.wait(lk);
+                    cvWork.wait_for(lk, std::chrono::milliseconds(50));
                 }
-                if (stop && queue.empty()) return;
-                if (stop) {
-                    // drain: fail what was never admitted
+                if (!j) {
+                    // stopped: fail what was never admitted
                     while (!queue.empty()) {
                         finish_unrun(queue.front(), RH_ERR_STATE, "server stopped");
                         queue.pop_front();
@@ -267,8 +295,6 @@ struct rh_server {
                     cvDone.notify_all();
                     return;
                 }
-                j = queue.front();
-                queue.pop_front();
                 j->state = 1;
                 j->tStart = Clock::now();
             }
