@@ -1284,9 +1284,27 @@ __device__ __forceinline__ MBHit quad_hit(const f3d p0, const f3d p1, const f3d 
 #define RH_MB_RECS 128     // records staged per chunk (a multiple of 64)
 #endif
 #ifndef RH_MB_PAIRS
-#define RH_MB_PAIRS 2048   // (record, sub-bound) pairs per chunk at most
+#define RH_MB_PAIRS 1792   // (record, sub-bound) pairs per chunk at most
 #endif
 #define RH_MB_RING 128     // visit results held per wave
+
+// Transpose of a 64 x 64 bit matrix held one row per lane (bit c of lane r's
+// word is element (r, c)): lane c returns column c.  Six rounds exchange
+// blocks of 32, 16, ... 1 bits with the lane that many rows away.
+__device__ __forceinline__ unsigned long long transpose64(unsigned long long row, int lane) {
+    const unsigned long long keep[6] = {0x00000000ffffffffull, 0x0000ffff0000ffffull, 0x00ff00ff00ff00ffull,
+                                        0x0f0f0f0f0f0f0f0full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int s = 32 >> k;
+        const unsigned long long K = keep[k];
+        const uint32_t plo = (uint32_t)__shfl_xor((int)(uint32_t)row, s);
+        const uint32_t phi = (uint32_t)__shfl_xor((int)(uint32_t)(row >> 32), s);
+        const unsigned long long partner = ((unsigned long long)phi << 32) | plo;
+        row = (lane & s) ? (((partner >> s) & K) | (row & ~K)) : ((row & K) | ((partner & K) << s));
+    }
+    return row;
+}
 
 // The position of the k-th (from 0) set bit of m, which has more than k.
 __device__ __forceinline__ int select_bit(unsigned long long m, int k) {
@@ -1334,6 +1352,8 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
     __shared__ float4 E1[RH_MB_PAIRS];
     __shared__ float2 E2[RH_MB_PAIRS];
     __shared__ uint16_t buf[NW][128];
+    __shared__ float4 smp[NW][64];                 // the block's samples: x, y, time, LOD test
+    __shared__ unsigned long long winLanes[65];
     // per wave: the visit results ring, and per rank its first position and lanes
     __shared__ uint32_t ringCode[NW][RH_MB_RING];
     __shared__ float ringD[NW][RH_MB_RING], ringU[NW][RH_MB_RING], ringV[NW][RH_MB_RING];
@@ -1371,6 +1391,14 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
     // the task in sub-sample units of the bucket's sample region
     const int tgx0 = (T.x0 - B.sx0) * nx, tgx1 = (T.x1 - B.sx0) * nx;
     const int tgy0 = (T.y0 - B.sy0) * ny, tgy1 = (T.y1 - B.sy0) * ny;
+    // a block's lanes of sub-sample index below k, and the rows of its lanes
+    unsigned long long rowRep = 0ull;
+    for (int r = 0; r < BH; ++r) rowRep |= 1ull << (r * BW);
+    if (wave == 0)
+        for (int k = 0; k <= 64; ++k) {
+            const unsigned long long m = __ballot(laneOk && si < k);
+            if (lane == 0) winLanes[k] = m;
+        }
 
     for (int p0 = 0; p0 < nbx * nby; p0 += NW) {
         const int blk = p0 + wave;
@@ -1395,6 +1423,8 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
         float occlZ = FLT_MAX, wu = 0.f, wv = 0.f;
         uint32_t win = 0xffffffffu, head = 0xffffffffu;
         int nh = 0;
+        const unsigned long long liveM = __ballot(live);
+        smp[wave][lane] = make_float4(sx, sy, stime, lodOk ? 1.0f : 0.0f);
 
         for (uint32_t c0 = 0; c0 < n;) {
             const int cnt = (int)((n - c0) < (uint32_t)RH_MB_RECS ? (n - c0) : (uint32_t)RH_MB_RECS);
@@ -1602,29 +1632,33 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                     }
                     if (fill == 0) break;
                     const int nq = fill < 64 ? fill : 64;
-                    // The entries visiting this lane's sample (every visit
-                    // is a sample test), and of those the ones passing the
-                    // tests that do not depend on earlier visits: the
-                    // sub-bound's time window, its bound (Contains2D) and
-                    // LOD (bucketprocessor.cpp:1393-1420; a static entry's
-                    // times are unbounded).
-                    unsigned long long vm = 0;
-                    uint32_t nVisit = 0;
-#pragma unroll 4
-                    for (int q = 0; q < nq; ++q) {
-                        const int e = buf[wave][q];
+                    // Per entry (one lane each): the lanes it visits, its
+                    // rectangle's lanes and-ed with its index window's (every
+                    // visit is a sample test), and of those the ones passing
+                    // the tests that do not depend on earlier visits: the
+                    // sub-bound's time window, its bound (Contains2D) and LOD
+                    // (bucketprocessor.cpp:1393-1420; a static entry's times
+                    // are unbounded).  Then the 64 x 64 bit matrix is
+                    // transposed, so each lane holds the entries that pass
+                    // for its sample.
+                    unsigned long long pass = 0;
+                    if (lane < nq) {
+                        const int e = buf[wave][lane];
                         const uint4 h = E0[e];
                         const float4 bd = E1[e];
                         const float2 tt = E2[e];
-                        const bool visit = live & (gx >= (int)(h.y & 0xffffu)) & (gx < (int)(h.y >> 16)) &
-                                           (gy >= (int)(h.z & 0xffffu)) & (gy < (int)(h.z >> 16)) &
-                                           (si >= (int)((h.x >> 16) & 0xffu)) & (si < (int)(h.x >> 24));
-                        const bool pass = visit & !(stime < tt.x || stime > tt.y) &
-                                          !(sx < bd.x || sx > bd.y || sy < bd.z || sy > bd.w) & lodOk;
-                        nVisit += visit ? 1u : 0u;
-                        vm |= (unsigned long long)pass << q;
+                        const unsigned long long vis = block_rect_lanes(make_uint2(h.y, h.z), bgx0, bgy0, BW, BH, rowRep) &
+                                                       winLanes[(h.x >> 24)] & ~winLanes[(h.x >> 16) & 0xffu] & liveM;
+                        nTest += (uint32_t)__popcll(vis);
+                        for (unsigned long long v = vis; v != 0ull; v &= v - 1ull) {
+                            const int j = __ffsll((long long)v) - 1;
+                            const float4 sp = smp[wave][j];    // x, y, time, LOD test
+                            const bool ok = !(sp.z < tt.x || sp.z > tt.y) &
+                                            !(sp.x < bd.x || sp.x > bd.y || sp.y < bd.z || sp.y > bd.w) & (sp.w != 0.0f);
+                            pass |= (unsigned long long)ok << j;
+                        }
                     }
-                    nTest += nVisit;
+                    const unsigned long long vm = transpose64(pass, lane);
                     // Two passes over the visits of these entries.  A: every
                     // visit's tests but the occlusion test, 64 visits a step
                     // whichever samples they belong to, taken in rank-major
