@@ -129,6 +129,31 @@ def motion_spheres():
     return opts, prims
 
 
+def motion_closed_shutter():
+    # Shutter open == close (the fast-shutter path: every sub-bound visits all
+    # of a pixel's samples, all at one time between the keys, so a
+    # transparent micropolygon is stored once per sub-bound holding the
+    # sample), 2x2 samples; mostly opaque so that no sample holds more
+    # transparent hits than the device keeps.
+    opts, prims = scenes.motion_scene(seed=31, n=90, res=(96, 80), samples=(2, 2), filt=("gaussian", 2.0, 2.0),
+                                      opacity=(0.5, 1.0), shutter=(0.4, 0.4), key_times=(0.0, 1.0),
+                                      shift_sigma=0.06)
+    for i, p in enumerate(prims):
+        if i % 4:
+            p.Os[:] = [1.0, 1.0, 1.0]
+    return opts, prims
+
+
+def motion_one_sample():
+    # One sample per pixel: four sub-bounds at most (max(4, ns)), whose index
+    # windows all hold the one sample; transparent and opaque patches mixed.
+    opts, prims = scenes.motion_scene(seed=33, n=150, res=(112, 96), samples=(1, 1), filt=("box", 1.0, 1.0),
+                                      opacity=(0.4, 1.0), shift_sigma=0.08)
+    for p in prims[::2]:
+        p.Os[:] = [1.0, 1.0, 1.0]
+    return opts, prims
+
+
 CASES = {
     "spheres_config": spheres_config,
     "patches_small": patches_small,
@@ -146,4 +171,6 @@ CASES = {
     "motion_transparent_8x8": motion_transparent_8x8,
     "motion_mixed_opaque": motion_mixed_opaque,
     "motion_spheres": motion_spheres,
+    "motion_closed_shutter": motion_closed_shutter,
+    "motion_one_sample": motion_one_sample,
 }
