@@ -10,6 +10,7 @@ test_oracle_kats.py::test_occlusion_culling_changes_no_result checks that).
 import numpy as np
 import pytest
 
+from aqsis_amd import scenes
 from parity_cases import CASES
 
 pytestmark = pytest.mark.gpu
@@ -84,3 +85,30 @@ def test_contexts_of_one_shape_render_alike(oracle):
         assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
     ref = oracle.render(opts, prims, occlusion_cull=False)
     assert np.array_equal(outs[1][2], ref["mp_id"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["no_surfaces", "all_outside"])
+def test_frames_without_visible_surfaces(kind, oracle):
+    """A frame with no surfaces, and one whose surfaces all lie outside the
+    view, render the empty image: no hits, empty samples, channels and 8-bit
+    output as the oracle's."""
+    from aqsis_amd.hider import Hider
+    opts, prims = scenes.patches_scene(seed=41, n=20, res=(64, 48), samples=(2, 2))
+    if kind == "no_surfaces":
+        prims = []
+    else:
+        for p in prims:
+            for i in range(16):
+                p.P[i][0] += 1000.0           # far to the side of the frustum
+    ref = oracle.render(opts, prims, occlusion_cull=False)
+    with Hider(opts) as h:
+        h.post_surfaces(prims)
+        h.render()
+        ids, depth, _ = h.samples()
+        st = h.stats()
+        assert np.array_equal(h.channels().view(np.uint32), ref["channels"].view(np.uint32))
+        assert np.array_equal(h.rgba8(), ref["rgba8"])
+    assert np.array_equal(ids, ref["mp_id"]) and np.all(ids == 0xffffffff)
+    assert st["spl_hits"] == ref["stats"]["spl_hits"] == 0
+    assert st["spl_count"] == ref["stats"]["spl_count"]
