@@ -114,8 +114,8 @@ struct SampleLaunch {
     const float* pos1d;
     const uchar4* pixPattern;
     float4* samples;
-    float4* hitData;
-    uint32_t* hitNext;
+    float4* hitData;                      // per transparent hit: u, v, depth, id
+    uint32_t* hitNext;                    // per transparent hit: list link, surface
     uint32_t* hitCounter;
     uint4* combineList;
     float* frontDepth;

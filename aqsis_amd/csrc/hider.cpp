@@ -32,7 +32,7 @@ extern "C" hipError_t rh_launch_plan_items(hipStream_t, const FrameDev*, const T
                                            uint32_t, int, uint2*, uint32_t*, uint32_t*);
 extern "C" hipError_t rh_launch_frame_begin(hipStream_t, unsigned long long*, uint32_t*);
 extern "C" hipError_t rh_launch_combine(hipStream_t, const FrameDev*, size_t, const uint4*, const uint32_t*,
-                                        const float4*, const uint32_t*, float4*, float*);
+                                        const float4*, const uint32_t*, const PrimDev*, float4*, float*);
 extern "C" hipError_t rh_launch_reduce_counters(hipStream_t, const uint32_t*, int, const uint32_t*, int,
                                                 unsigned long long*);
 extern "C" size_t rh_filter_tile_lds(int, int, int);
@@ -134,7 +134,7 @@ struct rh_ctx {
     // per frame
     DBuf dMprec, dMpbound, dLists, dListCount, dSamples, dChannels, dRgba, dCounters, dBucketRefs;
     DBuf dMprecMB;                        // motion-frame micropolygon records
-    // transparent hits: arena of (Ci, depth), (Oi, id) pairs and list links
+    // transparent hits: arena of (u, v, depth, id) and (link, surface) pairs
     DBuf dHitData, dHitNext, dHitCtl, dCombine, dFrontDepth;
     bool transparent = false;             // some posted surface has Os < 1
     // work items: one per task (grid and per-bucket forms), cost-split (batched)
@@ -440,8 +440,8 @@ static uint64_t arena_guess(const rh::Shape& s, const rh::Schedule& S, const rh_
 static rh_status set_hit_capacity(rh_ctx* c, uint64_t n) {
     if (n < 64) n = 64;
     if (n > 0x7fffffffull) return fail(RH_ERR_CAPACITY, "transparent hits exceed 2^31 entries");
-    HIP_TRY(c->dHitData.ensure(n * 2 * sizeof(float4)));
-    HIP_TRY(c->dHitNext.ensure(n * sizeof(uint32_t)));
+    HIP_TRY(c->dHitData.ensure(n * sizeof(float4)));
+    HIP_TRY(c->dHitNext.ensure(n * 2 * sizeof(uint32_t)));
     c->frame.hitCapacity = (uint32_t)n;
     c->dropGraph();
     std::vector<FrameDev> fv(1, c->frame);
@@ -590,7 +590,7 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
     if (c->transparent)
         ENQ(rh_launch_combine(c->st, f, (size_t)s.extW * s.extH * s.ns, c->dCombine.as<uint4>(),
                               c->dHitCtl.as<uint32_t>(), c->dHitData.as<float4>(), c->dHitNext.as<uint32_t>(),
-                              c->dSamples.as<float4>(), c->dFrontDepth.as<float>()));
+                              c->dPrims.as<PrimDev>(), c->dSamples.as<float4>(), c->dFrontDepth.as<float>()));
     if (timed) ENQ(hipEventRecord(c->ev[3], c->st));
     ENQ(rh_launch_reduce_counters(c->st, c->dRows.as<uint32_t>(), rows, c->dBucketRefs.as<uint32_t>(), nb,
                                   c->dCounters.as<unsigned long long>()));
@@ -855,9 +855,9 @@ static uint64_t footprint_bytes(const rh::Shape& s, const rh::Schedule& S, const
     for (int32_t i = 0; i < n; ++i)
         if (!(prims[i].Os[0] >= 1.0f && prims[i].Os[1] >= 1.0f && prims[i].Os[2] >= 1.0f)) transparent = true;
     if (transparent) {
-        // hit arena (36 B an entry), combine list and front depths
+        // hit arena (24 B an entry), combine list and front depths
         const uint64_t nsamp = (uint64_t)s.extW * s.extH * s.ns;
-        bytes += arena_guess(s, S, prims) * 36 + nsamp * 20;
+        bytes += arena_guess(s, S, prims) * 24 + nsamp * 20;
     }
     // device buffers are allocated with 1/8 slack (DBuf::ensure)
     return bytes + bytes / 8 + 64 * 256;

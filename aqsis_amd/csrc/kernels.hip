@@ -1099,11 +1099,11 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                                 const uint32_t slot =
                                     wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & ((1ull << lane) - 1ull));
                                 if (slot < fr->hitCapacity && nh[k] < RH_MAX_HITS - 1) {
-                                    float4 hA, hB;
-                                    shade_hit(prims[grids[hgrid].prim], u, v, D, hid, hA, hB);
-                                    hitData[2 * (size_t)slot] = hA;
-                                    hitData[2 * (size_t)slot + 1] = hB;
-                                    hitNext[slot] = head[k];
+                                    // the hit's (u, v, depth, id) and surface; the
+                                    // combine shades it
+                                    hitData[slot] = make_float4(u, v, D, __uint_as_float(hid));
+                                    hitNext[2 * (size_t)slot] = head[k];
+                                    hitNext[2 * (size_t)slot + 1] = grids[hgrid].prim;
                                     head[k] = slot;
                                     ++nh[k];
                                 } else if (slot < fr->hitCapacity) {
@@ -1834,11 +1834,9 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                                 if (app) {
                                     const uint32_t slot = wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & lanesBelow);
                                     if (slot < fr->hitCapacity && nh < RH_MAX_HITS - 1) {
-                                        float4 hA, hB;
-                                        shade_hit(prims[grids[hgrid].prim], u, v, D, hid, hA, hB);
-                                        hitData[2 * (size_t)slot] = hA;
-                                        hitData[2 * (size_t)slot + 1] = hB;
-                                        hitNext[slot] = head;
+                                        hitData[slot] = make_float4(u, v, D, __uint_as_float(hid));
+                                        hitNext[2 * (size_t)slot] = head;
+                                        hitNext[2 * (size_t)slot + 1] = grids[hgrid].prim;
                                         head = slot;
                                         ++nh;
                                     } else if (slot < fr->hitCapacity) {
@@ -1984,7 +1982,8 @@ __global__ void __launch_bounds__(RH_COMBINE_WG)
 rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
                                                   const uint32_t* __restrict__ hitCounter,
                                                   const float4* __restrict__ hitData,
-                                                  const uint32_t* __restrict__ hitNext, float4* __restrict__ samples,
+                                                  const uint32_t* __restrict__ hitNext,
+                                                  const PrimDev* __restrict__ prims, float4* __restrict__ samples,
                                                   float* __restrict__ frontDepth) {
     __shared__ float cmbD[RH_COMBINE_WG * (RH_MAX_HITS + 1)];
     __shared__ uint32_t cmbX[RH_COMBINE_WG * (RH_MAX_HITS + 1)];
@@ -2005,8 +2004,8 @@ rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
     // the list is linked newest first; rebuild insertion order
     for (int q = n - 1; q >= 0; --q) {
         x[q] = head;
-        d[q] = hitData[2 * (size_t)head].w;
-        head = hitNext[head];
+        d[q] = hitData[head].z;
+        head = hitNext[2 * (size_t)head];
     }
     if (__float_as_uint(oB.w) != 0xffffffffu) {
         x[n] = 0xffffffffu;
@@ -2018,8 +2017,12 @@ rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
     float opaqueDepth = occlZ;
     const float zt0 = fr->zThreshold[0], zt1 = fr->zThreshold[1], zt2 = fr->zThreshold[2];
     for (int q = n - 1; q >= 0; --q) {
-        const float4 hA = x[q] == 0xffffffffu ? oA : hitData[2 * (size_t)x[q]];
-        const float4 hB = x[q] == 0xffffffffu ? oB : hitData[2 * (size_t)x[q] + 1];
+        float4 hA = oA, hB = oB;
+        if (x[q] != 0xffffffffu) {
+            // InterpolateOutputs of the hit's surface at its (u, v)
+            const float4 h = hitData[x[q]];
+            shade_hit(prims[hitNext[2 * (size_t)x[q] + 1]], h.x, h.y, h.z, __float_as_uint(h.w), hA, hB);
+        }
         const float Ci[3] = {hA.x, hA.y, hA.z}, Oi[3] = {hB.x, hB.y, hB.z};
         for (int c = 0; c < 3; ++c) {
             const float oc = Oi[c] < 0.0f ? 0.0f : (Oi[c] > 1.0f ? 1.0f : Oi[c]);
@@ -2028,7 +2031,7 @@ rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
         }
         if (Oi[0] >= zt0 && Oi[1] >= zt1 && Oi[2] >= zt2) opaqueDepth = hA.w;
     }
-    const uint32_t front = x[0] == 0xffffffffu ? __float_as_uint(oB.w) : __float_as_uint(hitData[2 * (size_t)x[0] + 1].w);
+    const uint32_t front = x[0] == 0xffffffffu ? __float_as_uint(oB.w) : __float_as_uint(hitData[x[0]].w);
     samples[2 * sidx] = make_float4(col[0], col[1], col[2], opaqueDepth);
     frontDepth[sidx] = d[0];
     samples[2 * sidx + 1] = make_float4(opa[0], opa[1], opa[2], __uint_as_float(front));
@@ -2452,11 +2455,12 @@ extern "C" hipError_t rh_launch_plan_items(hipStream_t st, const FrameDev* fr, c
 // the kernel reads the actual count from hitCounter[2].
 extern "C" hipError_t rh_launch_combine(hipStream_t st, const FrameDev* fr, size_t maxEntries, const uint4* list,
                                         const uint32_t* hitCounter, const float4* hitData, const uint32_t* hitNext,
+                                        const PrimDev* prims,
                                         float4* samples, float* frontDepth) {
     if (maxEntries == 0) return hipSuccess;
     hipLaunchKernelGGL(rh_combine, dim3((unsigned)((maxEntries + RH_COMBINE_WG - 1) / RH_COMBINE_WG)),
                        dim3(RH_COMBINE_WG), 0, st, fr, list,
-                       hitCounter, hitData, hitNext, samples, frontDepth);
+                       hitCounter, hitData, hitNext, prims, samples, frontDepth);
     return hipGetLastError();
 }
 
