@@ -266,8 +266,9 @@ def main():
                     help="device layout: auto = autotuned batched form")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="frames of the one-launch-per-bucket form timed beside (0: skip)")
-    ap.add_argument("--requests", type=int, default=200, help="serving: number of requests")
-    ap.add_argument("--rate", type=float, default=40.0, help="serving: Poisson arrival rate (1/s)")
+    ap.add_argument("--requests", type=int, default=2000, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=20.0,
+                    help="serving: Poisson arrival rate (1/s); 20/s keeps the GPU about 90%% busy")
     ap.add_argument("--slots", type=int, default=4, help="serving: jobs in flight")
     ap.add_argument("--budget-gb", type=int, default=160, help="serving: device memory budget")
     ap.add_argument("--copy-kernel", type=int, default=0, metavar="BYTES",
