@@ -780,6 +780,40 @@ rh_status rh_read_rgba8(rh_ctx* c, uint8_t* out) {
     return RH_OK;
 }
 
+rh_status rh_display(rh_ctx* c, rh_dspy_data_fn data, void* image, int32_t scanline_order) {
+    if (!c || !data) return fail(RH_ERR_INVALID, "bad arguments");
+    if (!c->rendered) return fail(RH_ERR_STATE, "no rendered frame");
+    const rh::Shape& s = c->shape;
+    const int W = s.width(), H = s.height();
+    if (s.cropX0 != 0 || s.cropY0 != 0 || W != s.o.xres || H != s.o.yres)
+        return fail(RH_ERR_INVALID, "rh_display takes frames without a crop window");
+    std::vector<uint8_t> img((size_t)W * H * 4);
+    HIP_TRY(hipMemcpy(img.data(), c->dRgba.p, img.size(), hipMemcpyDeviceToHost));
+    std::vector<uint8_t> bucket, rows;
+    if (scanline_order) rows.resize((size_t)W * 4 * s.o.bucket_y);
+    for (const auto& b : s.buckets) {
+        const int x0 = b.x, x1 = b.x + b.w, y0 = b.y, y1 = b.y + b.h;
+        if (!scanline_order) {
+            // the bucket's pixels, row by row (FormatBucketForDisplay)
+            bucket.resize((size_t)b.w * b.h * 4);
+            for (int y = y0; y < y1; ++y)
+                std::memcpy(&bucket[(size_t)(y - y0) * b.w * 4], &img[((size_t)y * W + x0) * 4], (size_t)b.w * 4);
+            if (data(image, x0, x1, y0, y1, 4, bucket.data()) != 0)
+                return fail(RH_ERR_STATE, "the display driver stopped the delivery");
+            continue;
+        }
+        // CollapseBucketsToScanlines: gather the bucket row, send whole rows
+        // when its last bucket has arrived (SendToDisplay)
+        for (int y = y0; y < y1; ++y)
+            std::memcpy(&rows[((size_t)(y - y0) * W + x0) * 4], &img[((size_t)y * W + x0) * 4], (size_t)b.w * 4);
+        if (x1 >= W)
+            for (int y = y0; y < y1; ++y)
+                if (data(image, 0, W, y, y + 1, 4, &rows[(size_t)(y - y0) * W * 4]) != 0)
+                    return fail(RH_ERR_STATE, "the display driver stopped the delivery");
+    }
+    return RH_OK;
+}
+
 rh_status rh_read_samples(rh_ctx* c, uint32_t* mp_id, float* depth, float* rgbaz) {
     if (!c) return fail(RH_ERR_INVALID, "null context");
     if (!c->rendered) return fail(RH_ERR_STATE, "no rendered frame");

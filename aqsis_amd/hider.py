@@ -41,9 +41,13 @@ class RhTuning(C.Structure):
 MODES = {"grid": 0, "plain": 1, "batched": 2}
 
 EXPORTS = ["rh_options_default", "rh_create", "rh_post_surfaces", "rh_clear_surfaces", "rh_render",
-           "rh_render_device", "rh_render_stages", "rh_frame_size", "rh_read_channels", "rh_read_rgba8", "rh_read_samples",
+           "rh_render_device", "rh_render_stages", "rh_frame_size", "rh_read_channels", "rh_read_rgba8", "rh_display",
+           "rh_read_samples",
            "rh_get_stats", "rh_destroy", "rh_last_error", "rh_estimate", "rh_sample_positions",
            "rh_set_tuning", "rh_get_tuning", "rh_autotune"]
+
+
+DSPY_DATA_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p)
 
 
 def lib():
@@ -66,6 +70,7 @@ def lib():
         L.rh_read_channels.argtypes = [C.c_void_p, C.c_void_p]
         L.rh_read_rgba8.argtypes = [C.c_void_p, C.c_void_p]
         L.rh_read_samples.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rh_display.argtypes = [C.c_void_p, DSPY_DATA_FN, C.c_void_p, C.c_int32]
         L.rh_get_stats.argtypes = [C.c_void_p, P(RhStats)]
         L.rh_destroy.argtypes = [C.c_void_p]
         L.rh_destroy.restype = None
@@ -136,6 +141,16 @@ class Hider:
         out = np.zeros((self.height, self.width, 4), np.uint8)
         _check(lib().rh_read_rgba8(self._ctx, _ptr(out)))
         return out
+
+    def display(self, data, scanline_order=False):
+        """Deliver the 8-bit rgba to `data(xmin, xmaxplus1, ymin, ymaxplus1,
+        entry_size, bytes)` as a display driver's DspyImageData receives it
+        (rh_display); a true return value from `data` stops the delivery."""
+        def cb(_image, x0, x1, y0, y1, es, ptr):
+            n = (x1 - x0) * (y1 - y0) * es
+            return 1 if data(x0, x1, y0, y1, es, C.string_at(ptr, n)) else 0
+        fn = DSPY_DATA_FN(cb)
+        _check(lib().rh_display(self._ctx, fn, None, 1 if scanline_order else 0))
 
     def samples(self):
         n = self.ext_width * self.ext_height * self.spp

@@ -225,6 +225,23 @@ rh_status rh_read_channels(rh_ctx* ctx, float* out);
  * ddmanager.cpp:1022-1118). */
 rh_status rh_read_rgba8(rh_ctx* ctx, uint8_t* out);
 
+/* A display driver's image-data entry point, with DspyImageData's argument
+ * meaning (include/aqsis/ri/ndspy.h:158): the rectangle [xmin, xmaxplus1) x
+ * [ymin, ymaxplus1), the bytes of one pixel, and the pixels row by row.  A
+ * non-zero return stops the delivery. */
+typedef int (*rh_dspy_data_fn)(void* image, int xmin, int xmaxplus1, int ymin, int ymaxplus1, int entry_size,
+                               const unsigned char* data);
+
+/* Hand the rendered frame's 8-bit rgba to a display driver the way
+ * CqDisplayRequest::DisplayBucket does (ddmanager.cpp:984-1019, 1129-1175):
+ * bucket by bucket in the render's bucket order, each bucket's display
+ * rectangle (imagebuffer.cpp:745-748, bucketprocessor.cpp:107); or, with
+ * scanline_order (PkDspyFlagsWantsScanLineOrder), one full-width row at a
+ * time as each row of buckets completes.  Entries are 4 bytes (r, g, b, a).
+ * Frames with a crop window other than the whole image are refused
+ * (RH_ERR_INVALID). */
+rh_status rh_display(rh_ctx* ctx, rh_dspy_data_fn data, void* image, int32_t scanline_order);
+
 /* Per-sample visibility of the extended region, sample s of pixel
  * (x, y) at index (y*ext_width + x)*samples_per_pixel + s:
  *   mp_id[]  id of the front-most micropolygon hit (0xffffffff: none);

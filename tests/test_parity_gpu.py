@@ -112,3 +112,31 @@ def test_frames_without_visible_surfaces(kind, oracle):
     assert np.array_equal(ids, ref["mp_id"]) and np.all(ids == 0xffffffff)
     assert st["spl_hits"] == ref["stats"]["spl_hits"] == 0
     assert st["spl_count"] == ref["stats"]["spl_count"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scanline_order", [False, True])
+def test_display_delivery(scanline_order):
+    """rh_display hands the 8-bit image to a display driver as
+    CqDisplayRequest::DisplayBucket does: bucket rectangles in the render's
+    bucket order (16x16, clipped at the image edge), or full-width rows in
+    order once each row of buckets is complete; the pieces rebuild the
+    image exactly."""
+    from aqsis_amd.hider import Hider
+    opts, prims = scenes.patches_scene(seed=43, n=40, res=(72, 40), samples=(2, 2))
+    calls = []
+    with Hider(opts) as h:
+        h.post_surfaces(prims)
+        h.render()
+        img = h.rgba8()
+        h.display(lambda *a: calls.append(a), scanline_order=scanline_order)
+    rebuilt = np.zeros_like(img)
+    for x0, x1, y0, y1, es, data in calls:
+        assert es == 4
+        rebuilt[y0:y1, x0:x1] = np.frombuffer(data, np.uint8).reshape(y1 - y0, x1 - x0, 4)
+    assert np.array_equal(rebuilt, img)
+    if scanline_order:
+        assert [(c[0], c[1], c[2], c[3]) for c in calls] == [(0, 72, y, y + 1) for y in range(40)]
+    else:
+        want = [(bx, min(bx + 16, 72), by, min(by + 16, 40)) for by in range(0, 40, 16) for bx in range(0, 72, 16)]
+        assert [(c[0], c[1], c[2], c[3]) for c in calls] == want
