@@ -116,6 +116,11 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 
 }  // namespace
 
+namespace rh {
+// Error text of the host-only entry points compiled in other files.
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace rh
+
 struct rh_ctx {
     rh::Shape shape;
     rh::Plan plan;

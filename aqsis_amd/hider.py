@@ -41,7 +41,7 @@ class RhTuning(C.Structure):
 MODES = {"grid": 0, "plain": 1, "batched": 2}
 
 EXPORTS = ["rh_options_default", "rh_create", "rh_post_surfaces", "rh_clear_surfaces", "rh_render",
-           "rh_render_device", "rh_render_stages", "rh_frame_size", "rh_read_channels", "rh_read_rgba8", "rh_display",
+           "rh_render_device", "rh_render_stages", "rh_frame_size", "rh_read_channels", "rh_read_rgba8", "rh_display", "rh_tiff_write",
            "rh_read_samples",
            "rh_get_stats", "rh_destroy", "rh_last_error", "rh_estimate", "rh_sample_positions",
            "rh_set_tuning", "rh_get_tuning", "rh_autotune"]
@@ -71,6 +71,8 @@ def lib():
         L.rh_read_rgba8.argtypes = [C.c_void_p, C.c_void_p]
         L.rh_read_samples.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.rh_display.argtypes = [C.c_void_p, DSPY_DATA_FN, C.c_void_p, C.c_int32]
+        L.rh_tiff_write.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                    C.c_void_p, C.c_void_p, C.c_char_p]
         L.rh_get_stats.argtypes = [C.c_void_p, P(RhStats)]
         L.rh_destroy.argtypes = [C.c_void_p]
         L.rh_destroy.restype = None
@@ -152,6 +154,18 @@ class Hider:
         fn = DSPY_DATA_FN(cb)
         _check(lib().rh_display(self._ctx, fn, None, 1 if scanline_order else 0))
 
+    def write_tiff(self, path, compression="lzw", scanline_order=False):
+        """Write the frame to a TIFF file as the "file" display driver does:
+        the driver's image buffer is filled from the rh_display delivery
+        (DspyImageData), then written with rh_tiff_write."""
+        img = np.zeros((self.height, self.width, 4), np.uint8)
+
+        def data(x0, x1, y0, y1, es, buf):
+            img[y0:y1, x0:x1] = np.frombuffer(buf, np.uint8).reshape(y1 - y0, x1 - x0, es)
+            return False
+        self.display(data, scanline_order)
+        write_tiff(path, img, compression)
+
     def samples(self):
         n = self.ext_width * self.ext_height * self.spp
         ids = np.zeros(n, np.uint32)
@@ -208,6 +222,22 @@ class Hider:
 
     def __exit__(self, *exc):
         self.close()
+
+
+TIFF_COMPRESSION = {"none": 1, "lzw": 5}
+
+
+def write_tiff(path, pixels, compression="lzw", origin=None, full_size=None, description=None):
+    """Write an 8-bit (height, width, 3 or 4) image to `path` as the "file"
+    display driver's integer branch does (rh_tiff_write)."""
+    a = np.ascontiguousarray(pixels, dtype=np.uint8)
+    if a.ndim != 3:
+        raise HiderError("write_tiff: pixels must be (height, width, channels)")
+    o = None if origin is None else (C.c_int32 * 2)(*origin)
+    f = None if full_size is None else (C.c_int32 * 2)(*full_size)
+    d = None if description is None else description.encode()
+    _check(lib().rh_tiff_write(os.fsencode(path), _ptr(a), a.shape[1], a.shape[0], a.shape[2],
+                               TIFF_COMPRESSION[compression], o, f, d))
 
 
 def estimate(opts, prims):

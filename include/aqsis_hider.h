@@ -242,6 +242,24 @@ typedef int (*rh_dspy_data_fn)(void* image, int xmin, int xmaxplus1, int ymin, i
  * (RH_ERR_INVALID). */
 rh_status rh_display(rh_ctx* ctx, rh_dspy_data_fn data, void* image, int32_t scanline_order);
 
+/* Compression of rh_tiff_write: TIFF's Compression tag values. */
+enum { RH_TIFF_NONE = 1, RH_TIFF_LZW = 5 };
+
+/* Write an 8-bit image to a TIFF file as the "file"/"tiff" display driver's
+ * integer branch does (tools/displays/file/display.cpp:227-350, WriteTIFF,
+ * with the origin and original size set at open, :549-551, 590-593): rgb or rgba
+ * (channels 3 or 4; rgba carries ExtraSamples = associated alpha), rows top
+ * to bottom, contiguous samples, strips of libtiff's default height,
+ * uncompressed or LZW (the driver's default), resolution unit none at 1/1,
+ * XPosition/YPosition = origin, PIXAR_IMAGEFULLWIDTH/LENGTH = full_size.
+ * origin and full_size may be NULL (0,0 and width,height); description NULL
+ * gives a fixed text.  Host only: needs no device.  The pixel data are the
+ * driver's; the tags it takes from the scene (world-to-camera and
+ * world-to-screen matrices, host name) are not written. */
+rh_status rh_tiff_write(const char* path, const uint8_t* pixels, int32_t width, int32_t height,
+                        int32_t channels, int32_t compression, const int32_t* origin,
+                        const int32_t* full_size, const char* description);
+
 /* Per-sample visibility of the extended region, sample s of pixel
  * (x, y) at index (y*ext_width + x)*samples_per_pixel + s:
  *   mp_id[]  id of the front-most micropolygon hit (0xffffffff: none);

@@ -140,3 +140,25 @@ def test_display_delivery(scanline_order):
     else:
         want = [(bx, min(bx + 16, 72), by, min(by + 16, 40)) for by in range(0, 40, 16) for bx in range(0, 72, 16)]
         assert [(c[0], c[1], c[2], c[3]) for c in calls] == want
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("compression", ["none", "lzw"])
+def test_tiff_file_of_frame(tmp_path, compression):
+    """The frame written as the file display driver writes it (the driver's
+    buffer filled from rh_display, then rh_tiff_write) holds the frame's
+    8-bit rgba."""
+    from aqsis_amd.hider import Hider
+    from test_tiff import _raw_pixels, _read, _same
+    opts, prims = scenes.patches_scene(seed=44, n=60, res=(80, 48), samples=(2, 2))
+    path = tmp_path / "frame.tif"
+    with Hider(opts) as h:
+        h.post_surfaces(prims)
+        h.render()
+        img = h.rgba8()
+        h.write_tiff(path, compression)
+    got, tags = _read(path)
+    assert tags[256] == 80 and tags[257] == 48
+    assert _same(got, img)
+    if compression == "none":
+        assert _raw_pixels(path) == img.tobytes()
