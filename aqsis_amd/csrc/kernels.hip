@@ -51,8 +51,11 @@ __device__ __forceinline__ int lceil_i(float x) {
     return (int)lceil_f(x);
 }
 __device__ __forceinline__ long lfloor_d(double x) {  // math.h:47-51
-    long t = (long)x;
-    return t - (x < 0 && x != (double)t);
+    // static_cast<long> as x86-64 performs it (cvttsd2si): NaN and values
+    // outside the long range give LONG_MIN; the subtraction wraps.  So a NaN
+    // colour (pow of a negative filtered value) quantises to the minimum.
+    const long t = (x >= -9223372036854775808.0 && x < 9223372036854775808.0) ? (long)x : (long)(1ull << 63);
+    return (long)((unsigned long)t - (unsigned long)(x < 0 && x != (double)t));
 }
 // lfloor / lceil of math.h:47-61 to int where the result fits.
 __device__ __forceinline__ int lfloor_i(float x) {

@@ -47,6 +47,26 @@ def spheres_exposure_mitchell():
     return opts, prims
 
 
+def spheres_disk_clip_aspect():
+    # Disk filter, PixelSamples 3x2, non-square pixels, clipping planes that
+    # cut through the spheres' depth range (surfaces wholly outside are
+    # culled, the rest are sampled), and two display requests drawing dither
+    # values from the shared random stream.
+    opts, prims = scenes.spheres_scene(seed=29, n=56, res=(168, 120), samples=(3, 2), filt=("disk", 2.0, 2.0))
+    opts.pixel_aspect = 1.25
+    opts.clip_near, opts.clip_far = 11.5, 17.0
+    opts.num_displays = 2
+    return opts, prims
+
+
+def patches_bessel_1x1():
+    # Bessel 3x3 with one sample per pixel, exposure gamma, no dither.
+    opts, prims = scenes.patches_scene(seed=31, n=300, res=(176, 112), samples=(1, 1), filt=("bessel", 3.0, 3.0))
+    opts.exposure_gamma = 2.2
+    opts.quantize_dither = 0.0
+    return opts, prims
+
+
 def spheres_wide_gaussian():
     # A 5x5 filter footprint (DiscreteShift 2) with 3x3 samples and a frame
     # that is not a whole number of 8x8 pixel tiles.
@@ -163,6 +183,8 @@ CASES = {
     "patches_gridsampler": patches_gridsampler,
     "spheres_exposure_mitchell": spheres_exposure_mitchell,
     "spheres_wide_gaussian": spheres_wide_gaussian,
+    "spheres_disk_clip_aspect": spheres_disk_clip_aspect,
+    "patches_bessel_1x1": patches_bessel_1x1,
     "patches_transparent": patches_transparent,
     "patches_transparent_8x8_zthreshold": patches_transparent_8x8_zthreshold,
     "spheres_mixed_opacity": spheres_mixed_opacity,

@@ -54,6 +54,20 @@ def test_rounding_known_answers(oracle):
     assert [ro[1], ro[7], ro[8], ro[9]] == [-1, -1, -1, -2]
 
 
+def test_rounding_of_nan_and_overflow(oracle):
+    """lround(x) = lfloor(x - 0.5) + 1 with lfloor's static_cast<long>
+    (math.h:47-67) as x86-64 executes it: NaN and values beyond the long
+    range convert to LONG_MIN, so a NaN colour quantises to the clamp's
+    minimum (ddmanager.cpp:1068-1069).  The device's quantise reproduces
+    this; the parity case with gamma over a negative-lobed filter checks it."""
+    x = np.array([np.nan, np.inf, 3e19], np.float32)
+    fl, ce, ro = (np.zeros(len(x), np.int64) for _ in range(3))
+    oracle.lib().oracle_round_f(x.ctypes.data_as(C.c_void_p), len(x), *(a.ctypes.data_as(C.c_void_p) for a in (fl, ce, ro)))
+    lmin = -2 ** 63
+    assert list(fl) == [lmin] * 3
+    assert list(ro) == [lmin + 1] * 3
+
+
 # --------------------------------------------------------------------------
 # Inverse bilinear: the 11 cases of libs/core/bilinear_test.cpp:156-261 with
 # the fixture of :104-146 and the tolerances each case uses.
