@@ -154,17 +154,18 @@ class Hider:
         fn = DSPY_DATA_FN(cb)
         _check(lib().rh_display(self._ctx, fn, None, 1 if scanline_order else 0))
 
-    def write_tiff(self, path, compression="lzw", scanline_order=False):
+    def write_tiff(self, path, compression="lzw", scanline_order=False, channels=4):
         """Write the frame to a TIFF file as the "file" display driver does:
         the driver's image buffer is filled from the rh_display delivery
-        (DspyImageData), then written with rh_tiff_write."""
+        (DspyImageData), then written with rh_tiff_write; channels 4 (rgba)
+        or 3 (rgb)."""
         img = np.zeros((self.height, self.width, 4), np.uint8)
 
         def data(x0, x1, y0, y1, es, buf):
             img[y0:y1, x0:x1] = np.frombuffer(buf, np.uint8).reshape(y1 - y0, x1 - x0, es)
             return False
         self.display(data, scanline_order)
-        write_tiff(path, img, compression)
+        write_tiff(path, img[..., :channels], compression)
 
     def samples(self):
         n = self.ext_width * self.ext_height * self.spp

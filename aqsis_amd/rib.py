@@ -8,7 +8,7 @@ token rules: strings in double quotes, numbers, [ ] arrays, # comments):
   Format, PixelSamples, PixelFilter, ShadingRate, Exposure, Quantize "rgba",
   Projection "perspective" "fov", Clipping, CropWindow, Shutter,
   Option "limits" "bucketsize" / "eyesplits" / "zthreshold",
-  Hider "hidden" "jitter", Display (ignored), FrameBegin/End, WorldBegin/End,
+  Hider "hidden" "jitter", Display, FrameBegin/End, WorldBegin/End,
   AttributeBegin/End, TransformBegin/End, Identity, Transform,
   ConcatTransform, Translate, Rotate, Scale, Color, Opacity,
   Surface "constant", Sphere, Patch "bicubic" / "bilinear" "P",
@@ -24,6 +24,11 @@ or given as two motion keys, is a two-key deforming surface (nkeys = 2):
 patches carry key 1's camera-space points in P1, spheres key 1's
 object->camera matrix in xform[1].  Anything else raises RibError naming the
 request and line.
+
+Display requests are counted into num_displays, since each display request
+draws one dither value per pixel from the shared random stream
+(ddmanager.cpp:1050); render_file writes the first display as a TIFF file
+when its type is "file" or "tiff" (tools/displays/file/display.cpp).
 """
 import math
 import re
@@ -177,9 +182,12 @@ def _set_keys(prim, times, P1=None, xform1=None):
         prim.xform[1][:] = [float(v) for v in xform1.reshape(-1)]
 
 
-def parse(text):
-    """RIB text -> (rh_options, [rh_prim]) for the hider."""
+def parse(text, displays=None):
+    """RIB text -> (rh_options, [rh_prim]) for the hider.  The Display
+    requests are appended to `displays` when a list is given, as dicts with
+    the name, type, mode, parameters and line."""
     opts = default_options()
+    ndisplays = 0
     prims = []
     # The current transformation: one matrix, or one per motion key when it
     # moves (`keys`, with the key times in `ktimes`); likewise the camera.
@@ -252,7 +260,14 @@ def parse(text):
                 p = _params(args[1:])
                 if "jitter" in p:
                     opts.jitter = int(p["jitter"][0])
-            elif req in ("Display", "FrameBegin", "FrameEnd", "Declare", "Sides"):
+            elif req == "Display":
+                if len(args) < 3 or not all(isinstance(a, str) for a in args[:3]):
+                    raise RibError(f"line {line}: Display needs a name, a type and a mode")
+                ndisplays += 1
+                if displays is not None:
+                    displays.append({"name": args[0], "type": args[1], "mode": args[2],
+                                     "params": _params(args[3:]), "line": line})
+            elif req in ("FrameBegin", "FrameEnd", "Declare", "Sides"):
                 pass
             elif req == "MotionBegin":
                 if len(flat) != 2 or not flat[0] < flat[1]:
@@ -346,9 +361,52 @@ def parse(text):
             raise RibError(f"line {line}: malformed {req}: {e}") from None
     if motion is not None:
         raise RibError(f"line {motion['line']}: MotionBegin without MotionEnd")
+    if ndisplays:
+        opts.num_displays = ndisplays
     return opts, prims
 
 
-def read(path):
+def read(path, displays=None):
     with open(path) as f:
-        return parse(f.read())
+        return parse(f.read(), displays)
+
+def render_file(path, out_dir=None):
+    """Render a RIB file and write its first display, when that is a "file"
+    or "tiff" display of mode "rgb" or "rgba", as a TIFF file (the name
+    taken relative to `out_dir`, default the RIB's folder; the "compression"
+    parameter "lzw", the driver's default, or "none").  Other display types
+    are rendered and not written.  Returns the paths written."""
+    import os
+    from .hider import Hider
+    displays = []
+    opts, prims = read(path, displays)
+    out_dir = os.path.dirname(os.path.abspath(path)) if out_dir is None else out_dir
+    files = [d for d in displays if d["type"] in ("file", "tiff")]
+    for d in files:
+        if d is not displays[0]:
+            raise RibError(f"line {d['line']}: only the first display request can be written to a file")
+        if d["mode"] not in ("rgb", "rgba"):
+            raise RibError(f"line {d['line']}: file display mode {d['mode']!r} is not rgb or rgba")
+        comp = d["params"].get("compression", ["lzw"])[0]
+        if comp not in ("lzw", "none"):
+            raise RibError(f"line {d['line']}: file display compression {comp!r} is not lzw or none")
+    written = []
+    with Hider(opts) as h:
+        h.post_surfaces(prims)
+        h.render()
+        for d in files:
+            target = os.path.join(out_dir, d["name"].lstrip("+"))
+            h.write_tiff(target, d["params"].get("compression", ["lzw"])[0], channels=len(d["mode"]))
+            written.append(target)
+    return written
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="Render RIB files with the hider and write their file displays.")
+    ap.add_argument("rib", nargs="+")
+    ap.add_argument("-o", "--out-dir", default=None)
+    a = ap.parse_args()
+    for f in a.rib:
+        for w in render_file(f, a.out_dir):
+            print(w)

@@ -168,3 +168,25 @@ def test_tiff_file_of_frame(tmp_path, compression):
     assert _same(got, img)
     if compression == "none":
         assert _raw_pixels(path) == img.tobytes()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["rgba", "rgb"])
+def test_rib_file_display(tmp_path, mode):
+    """A RIB with a "file" display renders to that TIFF file: the pixels of
+    the frame the same scene gives through the ABI."""
+    from aqsis_amd import rib
+    from aqsis_amd.hider import Hider
+    from test_tiff import _read, _same
+    opts, prims = scenes.patches_scene(seed=45, n=50, res=(64, 40), samples=(2, 2))
+    path = tmp_path / "scene.rib"
+    path.write_text(f'Display "+out.tif" "file" "{mode}"\n' + scenes.to_rib(opts, prims))
+    written = rib.render_file(str(path))
+    assert written == [str(tmp_path / "out.tif")]
+    with Hider(opts) as h:
+        h.post_surfaces(prims)
+        h.render()
+        img = h.rgba8()[..., :len(mode)]
+    got, tags = _read(written[0])
+    assert tags[259] == 5 and tags[277] == len(mode)
+    assert _same(got, img)

@@ -124,3 +124,32 @@ def test_motion_blocks():
 def test_refused(text, msg):
     with pytest.raises(rib.RibError, match=msg):
         rib.parse(text)
+
+
+def test_display_requests():
+    """Display requests are collected and counted into num_displays (each
+    draws one dither value per pixel from the shared stream)."""
+    text = ('Display "a.tif" "file" "rgba" "compression" "none"\n'
+            'Display "+b" "framebuffer" "rgb"\n'
+            'Format 32 24 1\nWorldBegin\nSphere 1 -1 1 360\nWorldEnd\n')
+    displays = []
+    opts, _ = rib.parse(text, displays)
+    assert opts.num_displays == 2
+    assert [(d["name"], d["type"], d["mode"]) for d in displays] == [("a.tif", "file", "rgba"), ("+b", "framebuffer", "rgb")]
+    assert displays[0]["params"] == {"compression": ["none"]}
+    assert rib.parse(text.split("\n", 2)[2])[0].num_displays == 1  # none given: the default one
+    with pytest.raises(rib.RibError, match="line 1: Display needs"):
+        rib.parse('Display "a.tif" "file"\n')
+
+
+@pytest.mark.parametrize("first,second,msg", [
+    ('"a.tif" "file" "z"', None, "mode 'z'"),
+    ('"a.tif" "tiff" "rgba" "compression" "jpeg"', None, "compression 'jpeg'"),
+    ('"a" "framebuffer" "rgba"', '"+b.tif" "file" "rgba"', "only the first display"),
+])
+def test_file_displays_refused_before_rendering(tmp_path, first, second, msg):
+    lines = [f"Display {first}"] + ([f"Display {second}"] if second else [])
+    path = tmp_path / "s.rib"
+    path.write_text("\n".join(lines) + "\nFormat 16 16 1\nWorldBegin\nWorldEnd\n")
+    with pytest.raises(rib.RibError, match=msg):
+        rib.render_file(str(path))
