@@ -67,6 +67,14 @@ def patches_bessel_1x1():
     return opts, prims
 
 
+def spheres_quantize_range():
+    # Quantize "rgba" 200 10 230 0.5: a different one value and a clamp
+    # range inside the byte.
+    opts, prims = scenes.spheres_scene(seed=37, n=48, res=(128, 96), samples=(2, 2))
+    opts.quantize_one, opts.quantize_min, opts.quantize_max, opts.quantize_dither = 200.0, 10.0, 230.0, 0.5
+    return opts, prims
+
+
 def spheres_wide_gaussian():
     # A 5x5 filter footprint (DiscreteShift 2) with 3x3 samples and a frame
     # that is not a whole number of 8x8 pixel tiles.
@@ -185,6 +193,7 @@ CASES = {
     "spheres_wide_gaussian": spheres_wide_gaussian,
     "spheres_disk_clip_aspect": spheres_disk_clip_aspect,
     "patches_bessel_1x1": patches_bessel_1x1,
+    "spheres_quantize_range": spheres_quantize_range,
     "patches_transparent": patches_transparent,
     "patches_transparent_8x8_zthreshold": patches_transparent_8x8_zthreshold,
     "spheres_mixed_opacity": spheres_mixed_opacity,
