@@ -1,5 +1,5 @@
 """Pins the oracle (CPU restatement of the reference hider) before it is used
-to judge the HIP path: the reference's own unit-test known answers
+to check the HIP path: the reference's own unit-test known answers
 (libs/math/math_test.cpp, libs/core/bilinear_test.cpp,
 libs/core/occlusion_test.cpp), the C++ standard's MT19937 known answers,
 closed forms and invariants chosen so that a dropped term, a swapped
