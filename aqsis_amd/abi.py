@@ -66,6 +66,7 @@ class RhPrim(C.Structure):
         ("xform", (C.c_float * 16) * 2),
         ("P", (C.c_float * 3) * 16),
         ("P1", (C.c_float * 3) * 16),
+        ("no_cull_hidden", C.c_int32),
     ]
 
 
@@ -79,6 +80,7 @@ class RhStats(C.Structure):
         ("ms_schedule", C.c_double), ("ms_upload", C.c_double),
         ("ms_dice", C.c_double), ("ms_bin", C.c_double), ("ms_sample", C.c_double),
         ("ms_filter", C.c_double), ("ms_device", C.c_double), ("ms_composite", C.c_double),
+        ("cull_probes", C.c_uint64), ("cull_passes", C.c_uint64), ("ms_cull", C.c_double),
     ]
 
     def as_dict(self):

@@ -121,6 +121,8 @@ struct SampleLaunch {
     float* frontDepth;
     uint32_t* rowCounts;
     int rowBase;
+    uint32_t idCut;                       // only list entries below this id (0xffffffff: all)
+    int depthOnly;                        // opaque depths only: no transparent hits are kept
 };
 
 // Sampling kernel variants: LDS chunk of CH records, SLOTS blocks per wave.

@@ -355,6 +355,7 @@ struct Node {
     // key 1's control points (spheres: key 1's matrix, Ctx::objToCam1).
     bool motion = false;
     f3 cp1[16];
+    float subMin = FLT_MAX;     // least cached minimum depth in the subtree
 };
 
 // sinCosGrid (quadrics.cpp:1493-1518)
@@ -702,7 +703,7 @@ void build_tree(const Ctx& c, int prim, std::vector<Node>& t) {
 struct HeapEnt {
     float z;                    // cached raster bound's minimum z
     bool hasBound;
-    Node* nd;
+    const Node* nd;
 };
 
 struct HeapCmp {  // closest_surface (bucket.h:164-176)
@@ -714,8 +715,19 @@ struct HeapCmp {  // closest_surface (bucket.h:164-176)
 
 }  // namespace
 
-bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, std::string& err, int threads) {
+struct Forest {
     Ctx c;
+    std::vector<std::vector<Node>> trees;  // one split tree per primitive, parents first
+    std::vector<uint64_t> nodeBase;        // primitives + 1: global index of each tree's root
+};
+
+void ForestFree::operator()(Forest* f) const { delete f; }
+
+bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, std::string& err, int threads) {
+    ForestPtr fp(new Forest());
+    Forest& f = *fp;
+
+    Ctx& c = f.c;
     c.s = &s;
     c.prims = prims;
     c.objToCam.assign(n, m_identity());
@@ -764,15 +776,24 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
             }
         }
     }
-    // 1) Split trees, one per primitive, in parallel.
-    std::vector<std::vector<Node>> trees(n);
+    // Split trees, one per primitive, in parallel.
+    f.trees.assign(n, std::vector<Node>());
     int nt = std::max(1, std::min(threads, n));
     std::atomic<int> next(0);
     auto work = [&]() {
         for (;;) {
             int i = next.fetch_add(1);
             if (i >= n) break;
-            build_tree(c, i, trees[i]);
+            std::vector<Node>& t = f.trees[i];
+            build_tree(c, i, t);
+            // the least minimum depth of each node's subtree (children follow
+            // their parents in the tree)
+            for (size_t j = t.size(); j-- > 0;) {
+                Node& nd = t[j];
+                float m = nd.hasBound ? nd.rb.lo.z : FLT_MAX;
+                for (int q = 0; q < nd.nkids; ++q) m = mn(m, t[nd.kid0 + q].subMin);
+                nd.subMin = m;
+            }
         }
     };
     if (nt == 1) work();
@@ -781,13 +802,99 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
         for (int k = 0; k < nt; ++k) pool.emplace_back(work);
         for (auto& th : pool) th.join();
     }
+    f.nodeBase.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) f.nodeBase[i + 1] = f.nodeBase[i] + f.trees[i].size();
+    // the caller's shape and surfaces are passed again to simulate
+    c.s = nullptr;
+    c.prims = nullptr;
+    out = std::move(fp);
+    return true;
+}
 
-    // 2) Bucket loop: post roots in order, pop nearest first, post children.
+void cull_pairs(const Shape& s, const Forest& f, CullPairs& out) {
+    const size_t nn = f.nodeBase.back();
+    out.bound.assign(nn * 4, 0.0f);
+    out.range.assign(nn * 4, 0);
+    out.start.assign(nn + 1, 0);
+    for (size_t p = 0; p + 1 < f.nodeBase.size(); ++p) {
+        const auto& t = f.trees[p];
+        for (size_t j = 0; j < t.size(); ++j) {
+            const size_t g = f.nodeBase[p] + j;
+            const Node& nd = t[j];
+            int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
+            if (nd.hasBound && !nd.culled) {
+                // PostSurface's bucket range of a diceable surface (imagebuffer.cpp:398-407)
+                x0 = clampv(nd.xminb, s.bx0, s.bx1 - 1); y0 = clampv(nd.yminb, s.by0, s.by1 - 1);
+                x1 = clampv(nd.xmaxb, s.bx0, s.bx1 - 1); y1 = clampv(nd.ymaxb, s.by0, s.by1 - 1);
+                out.bound[4 * g] = nd.rb.lo.x; out.bound[4 * g + 1] = nd.rb.lo.y;
+                out.bound[4 * g + 2] = nd.rb.hi.x; out.bound[4 * g + 3] = nd.rb.hi.y;
+            }
+            out.range[4 * g] = x0; out.range[4 * g + 1] = y0; out.range[4 * g + 2] = x1; out.range[4 * g + 3] = y1;
+            const uint64_t cnt = (x1 >= x0 && y1 >= y0) ? (uint64_t)(x1 - x0 + 1) * (uint64_t)(y1 - y0 + 1) : 0;
+            out.start[g + 1] = out.start[g] + cnt;
+        }
+    }
+}
+
+uint64_t cull_trees(const Shape& s, std::vector<CullTreeDev>& out) {
+    out.assign(s.buckets.size(), CullTreeDev());
+    uint64_t pyr = 0;
+    for (size_t k = 0; k < s.buckets.size(); ++k) {
+        const auto& b = s.buckets[k];
+        CullTreeDev& t = out[k];
+        // CqOcclusionTree::setupTree (occlusion.cpp:54-107)
+        const int w = b.sx1 - b.sx0, h = b.sy1 - b.sy0;
+        const int numX = w * s.nx, numY = h * s.ny;
+        int depthX = 0, depthY = 0;
+        if (numX > 0 && numY > 0) {
+            // aqsis's log2: 1.4426950408889633 * std::log(float) (math.h:156-160)
+            depthX = (int)ceil_l((float)(1.4426950408889633 * (double)std::log((float)numX)));
+            depthY = (int)ceil_l((float)(1.4426950408889633 * (double)std::log((float)numY)));
+            if (depthX < depthY) depthX = depthY - 1;
+            else if (depthY < depthX) depthY = depthX - 1;
+        }
+        t.dx = depthX; t.dy = depthY;
+        t.cw = numX; t.ch = numY;
+        t.sx0 = b.sx0; t.sy0 = b.sy0;
+        t.x0 = (float)b.sx0; t.y0 = (float)b.sy0;
+        if (numX > 0 && numY > 0) {
+            const float scx = float(1 << depthX) / numX, scy = float(1 << depthY) / numY;
+            t.x1 = t.x0 + (float)w * scx;
+            t.y1 = t.y0 + (float)h * scy;
+        } else {
+            t.x1 = t.x0; t.y1 = t.y0;
+        }
+        t.pyr = (uint32_t)pyr;
+        t.levels = 0;
+        if (numX > 0 && numY > 0) {
+            for (int l = 0;; ++l) {
+                const int lw = cull_level_dim(numX, l), lh = cull_level_dim(numY, l);
+                pyr += (uint64_t)lw * lh;
+                t.levels = l + 1;
+                if (lw == 1 && lh == 1) break;
+            }
+        }
+    }
+    return pyr;
+}
+
+bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, const CullInput* cull,
+              const CullPairs* pairs, Schedule& out, std::string& err) {
+    const Ctx& c = f.c;
+    const auto& trees = f.trees;
+    std::vector<CullTreeDev> ctrees;
+    if (cull) cull_trees(s, ctrees);
+    // Bucket loop: post roots in order, pop nearest first, cull (against
+    // the bucket's occlusion tree) or dice or split, post children.
     const int nb = (int)s.buckets.size();
     std::vector<std::vector<HeapEnt>> heaps(nb);
     std::vector<uint8_t> processed(nb, 0);
     out = Schedule();
-    auto post = [&](Node* nd) {
+    auto push = [&](int k, const HeapEnt& ent) {
+        heaps[k].push_back(ent);
+        std::push_heap(heaps[k].begin(), heaps[k].end(), HeapCmp());
+    };
+    auto post = [&](const Node* nd) {
         out.surfaces++;
         if (nd->culled) return;
         int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
@@ -797,8 +904,7 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
         int k = s.bucketIndex(x0, y0);
         const HeapEnt ent{nd->rb.lo.z, nd->hasBound, nd};
         if (!processed[k]) {
-            heaps[k].push_back(ent);
-            std::push_heap(heaps[k].begin(), heaps[k].end(), HeapCmp());
+            push(k, ent);
             return;
         }
         int yb = y0, xb = x0 + 1;
@@ -806,8 +912,7 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
             while (xb <= x1) {
                 int kk = s.bucketIndex(xb, yb);
                 if (!processed[kk]) {
-                    heaps[kk].push_back(ent);
-                    std::push_heap(heaps[kk].begin(), heaps[kk].end(), HeapCmp());
+                    push(kk, ent);
                     return;
                 }
                 ++xb;
@@ -816,15 +921,89 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
             ++yb;
         }
     };
+    // CqImageBuffer::RepostSurface (imagebuffer.cpp:449-483): the bucket to
+    // the right when the bound reaches it, else the next row's bucket under
+    // the bound's left side; dropped past the last row.
+    auto repost = [&](int k, const Node* nd) {
+        const auto& b = s.buckets[k];
+        const int col = s.bx0 + k % s.nbr, row = s.by0 + k / s.nbr;
+        int nextX = col + 1, nextY = row;
+        const HeapEnt ent{nd->rb.lo.z, nd->hasBound, nd};
+        if (nextX < s.bx1 && nd->rb.hi.x >= (float)(b.x + b.w)) {
+            push(s.bucketIndex(nextX, nextY), ent);
+            return;
+        }
+        ++nextY;
+        nextX = (int)mx<long>((long)s.bx0, floor_l(nd->rb.lo.x) / s.o.bucket_x);
+        if (nextX < s.bx1 && nextY < s.by1 && nd->rb.hi.y >= (float)(b.y + b.h)) push(s.bucketIndex(nextX, nextY), ent);
+    };
+    auto node_index = [&](const Node* nd) -> uint64_t {
+        return f.nodeBase[nd->prim] + (uint64_t)(nd - trees[nd->prim].data());
+    };
     for (int i = 0; i < n; ++i) post(&trees[i][0]);
     uint32_t mpCount = 0, vtxCount = 0;
     std::vector<float>& P = out.params;
     for (int k = 0; k < nb; ++k) {
         auto& h = heaps[k];
         while (!h.empty()) {
-            Node* nd = h.front().nd;
+            const Node* nd = h.front().nd;
             std::pop_heap(h.begin(), h.end(), HeapCmp());
             h.pop_back();
+            // RenderSurface's occlusion cull (bucketprocessor.cpp:945-957):
+            // Attribute "cull" "hidden" (default 1) and a cached bound.
+            if (cull && nd->hasBound && !prims[nd->prim].no_cull_hidden) {
+                const uint64_t g = node_index(nd);
+                const int32_t* R = &pairs->range[4 * g];
+                const int col = s.bx0 + k % s.nbr, row = s.by0 + k / s.nbr;
+                if (col < R[0] || col > R[2] || row < R[1] || row > R[3]) {
+                    err = "internal: occlusion test outside a surface's bucket range";
+                    return false;
+                }
+                const uint64_t pi = pairs->start[g] + (uint64_t)(row - R[1]) * (uint64_t)(R[2] - R[0] + 1) +
+                                    (uint64_t)(col - R[0]);
+                const float z = nd->rb.lo.z, M = cull->pairMax[pi];
+                bool culled = M < z;
+                auto fx = std::lower_bound(cull->fixed.begin(), cull->fixed.end(), std::make_pair(pi, (uint8_t)0));
+                if (fx != cull->fixed.end() && fx->first == pi) {
+                    culled = fx->second != 0;
+                } else {
+                    // The final depths settle the test unless they lie within
+                    // the margin of z, or of the least depth a surface still in
+                    // the heap (or a descendant) can reach; a heap holding a
+                    // surface without a bound has no depth order at all.
+                    bool settled;
+                    if (culled) {
+                        const float margin = kCullMargin * std::fabs(z);
+                        float lim = z;
+                        bool unordered = false;
+                        for (const HeapEnt& e : h) {
+                            if (!e.hasBound) unordered = true;
+                            else lim = mn(lim, e.nd->subMin);
+                        }
+                        settled = !unordered && M < lim - margin;
+                    } else {
+                        settled = M >= z + kKeepMargin * std::fabs(z);
+                    }
+                    if (!settled) {
+                        CullProbe pr;
+                        pr.pair = pi;
+                        pr.bucket = k;
+                        pr.mpCut = mpCount;
+                        pr.z = z;
+                        pr.cx0 = pr.cx1 = pr.cy0 = pr.cy1 = 0;
+                        pr.kind = ctrees[k].levels ? cull_cells(ctrees[k], nd->rb.lo.x, nd->rb.lo.y, nd->rb.hi.x,
+                                                                nd->rb.hi.y, pr.cx0, pr.cx1, pr.cy0, pr.cy1)
+                                                   : 0;
+                        pr.culled = culled ? 1 : 0;
+                        out.probes.push_back(pr);
+                    }
+                }
+                if (culled) {
+                    out.culled++;
+                    repost(k, nd);
+                    continue;
+                }
+            }
             if (nd->leaf) {
                 GridRec g;
                 g.type = nd->type; g.cu = nd->uDice; g.cv = nd->vDice; g.prim = nd->prim;
@@ -869,16 +1048,15 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
                 }
                 if (nd->motion) {
                     g.type |= GRID_MOTION;
-                    P.push_back(c.prims[nd->prim].key_time[0]);
-                    P.push_back(c.prims[nd->prim].key_time[1]);
+                    P.push_back(prims[nd->prim].key_time[0]);
+                    P.push_back(prims[nd->prim].key_time[1]);
                     out.motion = true;
                 }
                 out.grids.push_back(g);
             } else if (nd->nkids > 0) {
-                std::vector<Node>* tree = nullptr;
                 // children live in the same tree as their parent
-                tree = &trees[nd->prim];
-                for (int q = 0; q < nd->nkids; ++q) post(&(*tree)[nd->kid0 + q]);
+                const std::vector<Node>& tree = trees[nd->prim];
+                for (int q = 0; q < nd->nkids; ++q) post(&tree[nd->kid0 + q]);
             }
         }
         processed[k] = 1;
@@ -886,7 +1064,7 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
     out.micropolygons = mpCount;
     out.vertices = vtxCount;
 
-    // 3) Candidate lists per bucket (grids in dicing order), only buckets not
+    // Candidate lists per bucket (grids in dicing order), only buckets not
     // yet processed when the grid was diced.
     out.candStart.assign(nb + 1, 0);
     for (const auto& g : out.grids)
@@ -927,6 +1105,13 @@ bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, s
     for (int k = 0; k < nb; ++k)
         for (uint32_t c = out.candStart[k]; c < out.candStart[k + 1]; ++c) out.segBucket[c] = (uint32_t)k;
     return true;
+}
+
+
+bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, std::string& err, int threads) {
+    ForestPtr f;
+    if (!build_forest(s, prims, n, f, err, threads)) return false;
+    return simulate(s, prims, n, *f, nullptr, nullptr, out, err);
 }
 
 }  // namespace rh

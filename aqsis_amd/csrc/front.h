@@ -3,10 +3,12 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "../../include/aqsis_hider.h"
+#include "cullgeom.h"
 #include "hmath.h"
 
 namespace rh {
@@ -73,6 +75,20 @@ struct GridRec {
     float px0, py0, px1, py1;             // raster bound with a 2-pixel guard band
 };
 
+// An occlusion test the final depths cannot settle (a surface's minimum
+// depth within the margin of the leaves' deepest final depth, or a heap
+// whose order does not bound the depths still to come): the tree's depths at
+// the moment of the test decide it.
+struct CullProbe {
+    uint64_t pair;                        // (surface, bucket) pair of the test
+    int32_t bucket;                       // region bucket index
+    uint32_t mpCut;                       // micropolygons busted before the test
+    float z;                              // the surface's minimum depth
+    int32_t kind;                         // cull_cells: 0 no leaf, 1 cells, 2 cells + padding
+    int32_t cx0, cx1, cy0, cy1;           // the test's sampled cells (inclusive)
+    uint8_t culled;                       // the decision the schedule took
+};
+
 struct Schedule {
     std::vector<GridRec> grids;
     std::vector<float> params;            // per-grid dicing parameters
@@ -85,7 +101,55 @@ struct Schedule {
     std::vector<uint32_t> candGrid;
     std::vector<uint32_t> segBucket;      // bucket of each candidate entry (a binning segment)
     std::vector<uint64_t> capStart;       // tasks + 1, per-task list capacity offsets
+    // occlusion culling (RenderSurface, bucketprocessor.cpp:945-957)
+    uint64_t culled = 0;                  // surfaces culled by a bucket's occlusion tree
+    std::vector<CullProbe> probes;        // tests left to the depths at their moment, in order
 };
+
+// The split trees of a scene's surfaces (CqSurface::Split / Diceable per
+// primitive), built once and scheduled with or without occlusion culling.
+struct Forest;
+struct ForestFree { void operator()(Forest* f) const; };
+using ForestPtr = std::unique_ptr<Forest, ForestFree>;
+
+bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, std::string& err, int threads);
+
+// The (surface, bucket) pairs an occlusion test may ask about: every surface
+// with a cached raster bound, in every bucket of its bucket range (posting
+// and reposting only ever move a surface inside it).
+struct CullPairs {
+    std::vector<float> bound;             // per surface: raster bound min x, min y, max x, max y
+    std::vector<int32_t> range;           // per surface: region bucket range x0, y0, x1, y1 (empty: x1 < x0)
+    std::vector<uint64_t> start;          // surfaces + 1: each surface's first pair
+};
+void cull_pairs(const Shape& s, const Forest& f, CullPairs& out);
+
+// Each bucket's occlusion tree (setupTree) and depth-pyramid offsets; returns
+// the pyramid floats of all buckets.
+uint64_t cull_trees(const Shape& s, std::vector<CullTreeDev>& out);
+
+// The answers a culling schedule takes: per pair, the deepest final depth of
+// the leaves the surface's bound meets (negative: none), and decisions
+// already fixed by probing (pair, culled), sorted by pair.
+struct CullInput {
+    const float* pairMax = nullptr;
+    std::vector<std::pair<uint64_t, uint8_t>> fixed;
+};
+
+// The bucket loop of a forest: pop nearest first, cull (when `cull` is given
+// and the surface's cull flag is set), dice or split, post the children.
+bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, const CullInput* cull,
+              const CullPairs* pairs, Schedule& out, std::string& err);
+
+// Relative margins around a cull test's depth inside which the final depths
+// do not settle it.  A cull needs the final depths below the surface's
+// minimum depth (and below the least one still to come from the heap) by
+// more than micropolygon depths can fall below their surface's bound through
+// dicing and bilinear rounding; keeping a surface needs them above it by more
+// than the final depths of two dicing orders can differ (a bilinear depth
+// below its micropolygon's least vertex depth, a few units in the last place).
+constexpr float kCullMargin = 1.0f / 16384.0f;
+constexpr float kKeepMargin = 1.0f / 65536.0f;
 
 // Parameter layouts in Schedule::params (all floats):
 //  sphere:   [0] radius, [1..16] object->camera matrix, [17] identity flag,
@@ -94,6 +158,7 @@ struct Schedule {
 //  bilinear: 4 corners xyz (12 floats)
 enum { SPH_RADIUS = 0, SPH_MAT = 1, SPH_IDENT = 17, SPH_TABLES = 18 };
 
+// build_forest + simulate without occlusion culling.
 bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, std::string& err,
                    int threads);
 

@@ -41,6 +41,11 @@ extern "C" hipError_t rh_launch_box_masks(hipStream_t, const FrameDev*, size_t, 
 extern "C" hipError_t rh_launch_filter(hipStream_t, const FrameDev*, int, int, int, int, int, const float4*,
                                        const uint16_t*, const uchar4*, const float*, const float*, const float*,
                                        float*, uchar4*);
+extern "C" hipError_t rh_launch_cull_pyramid(hipStream_t, const FrameDev*, const CullTreeDev*, int, const float4*, float*);
+extern "C" hipError_t rh_launch_cull_pairs(hipStream_t, const CullTreeDev*, int, int, int, const float4*, const int4*,
+                                           const unsigned long long*, int, unsigned long long, const float*, float*);
+extern "C" hipError_t rh_launch_cull_probe_max(hipStream_t, const FrameDev*, const CullTreeDev*, int, int, int, int,
+                                               int, int, const float4*, float*);
 
 static_assert(rh::GRID_MOTION == RH_GRID_MOTION, "motion grid flag differs between host and device");
 
@@ -144,6 +149,8 @@ struct rh_ctx {
     bool transparent = false;             // some posted surface has Os < 1
     // work items: one per task (grid and per-bucket forms), cost-split (batched)
     DBuf dItemsStatic, dItems, dItemCount, dQueue, dRows;
+    // occlusion culling: bucket trees, depth pyramids, (surface, bucket) tests, probe results
+    DBuf dCullTrees, dCullPyr, dCullBounds, dCullRanges, dCullStart, dCullPairMax, dProbeOut;
     int maxItems = 0, persistentGrid = 0, maxRows = 0;
     rh_tuning tuning;
     hipGraph_t graph = nullptr;
@@ -163,7 +170,8 @@ struct rh_ctx {
                        &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dMprec, &dMpbound, &dLists,
                        &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dBucketRefs,
                        &dHitData, &dHitNext, &dHitCtl, &dCombine, &dFrontDepth, &dItemsStatic, &dItems,
-                       &dItemCount, &dQueue, &dRows, &dSegBucket, &dSegCounts, &dMprecMB};
+                       &dItemCount, &dQueue, &dRows, &dSegBucket, &dSegCounts, &dMprecMB, &dCullTrees, &dCullPyr,
+                       &dCullBounds, &dCullRanges, &dCullStart, &dCullPairMax, &dProbeOut};
         for (DBuf* b : all) f(*b);
     }
     ~rh_ctx() {
@@ -416,6 +424,7 @@ rh_status rh_post_surfaces(rh_ctx* c, const rh_prim* prims, int32_t n) {
     if (!c || (n > 0 && !prims) || n < 0) return fail(RH_ERR_INVALID, "bad arguments");
     c->prims.insert(c->prims.end(), prims, prims + n);
     c->scheduled = false;
+    c->rendered = false;
     return RH_OK;
 }
 
@@ -455,6 +464,33 @@ static rh_status set_hit_capacity(rh_ctx* c, uint64_t n) {
 }
 
 static rh_status upload_schedule(rh_ctx* c);
+static rh_status cull_schedule(rh_ctx* c, const rh::Forest& f);
+
+// Some surface may be occlusion culled (Attribute "cull" "hidden", default 1).
+static bool any_cullable(const std::vector<rh_prim>& prims) {
+    for (const auto& q : prims)
+        if (!q.no_cull_hidden) return true;
+    return false;
+}
+
+// The frame's split/dice schedule: the culling-free one first, then, when
+// surfaces may be culled, the schedule the reference's occlusion culling
+// gives (cull_schedule).
+static rh_status install_schedule(rh_ctx* c, rh::Schedule&& S, const rh::Forest& f,
+                                  std::chrono::steady_clock::time_point t0) {
+    c->sched = std::move(S);
+    c->stats.cull_passes = 1;
+    c->stats.cull_probes = 0;
+    c->stats.ms_cull = 0.0;
+    c->stats.ms_schedule = ms_since(t0);
+    rh_status r = upload_schedule(c);
+    if (r != RH_OK || c->prims.empty() || !any_cullable(c->prims)) return r;
+    const auto tc = std::chrono::steady_clock::now();
+    r = cull_schedule(c, f);
+    c->stats.ms_cull = ms_since(tc);
+    c->stats.ms_schedule = ms_since(t0);
+    return r;
+}
 
 static rh_status schedule(rh_ctx* c) {
     c->dropGraph();
@@ -463,10 +499,12 @@ static rh_status schedule(rh_ctx* c) {
     int threads = (int)std::thread::hardware_concurrency();
     if (threads > 16) threads = 16;
     if (c->scheduleThreads > 0) threads = c->scheduleThreads;
-    if (!rh::make_schedule(c->shape, c->prims.data(), (int)c->prims.size(), c->sched, err, threads))
+    rh::ForestPtr f;
+    rh::Schedule S;
+    if (!rh::build_forest(c->shape, c->prims.data(), (int)c->prims.size(), f, err, threads) ||
+        !rh::simulate(c->shape, c->prims.data(), (int)c->prims.size(), *f, nullptr, nullptr, S, err))
         return fail(RH_ERR_INVALID, err);
-    c->stats.ms_schedule = ms_since(t0);
-    return upload_schedule(c);
+    return install_schedule(c, std::move(S), *f, t0);
 }
 
 // Device copies of a computed schedule.
@@ -562,6 +600,7 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
     a.hitNext = c->dHitNext.as<uint32_t>(); a.hitCounter = c->dHitCtl.as<uint32_t>();
     a.combineList = c->dCombine.as<uint4>(); a.frontDepth = c->transparent ? c->dFrontDepth.as<float>() : nullptr;
     a.rowCounts = c->dRows.as<uint32_t>(); a.rowBase = 0;
+    a.idCut = 0xffffffffu; a.depthOnly = 0;
     const int variant = variant_of(tu);
     int rows = 0;
     if (S.motion) {
@@ -608,12 +647,160 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
     return hipSuccess;
 }
 
+// The sampling launch arguments of a frame's depth pass: opaque depths only
+// (no transparent hit is kept), written to the frame's sample records.
+static SampleLaunch depth_launch(rh_ctx* c) {
+    SampleLaunch a;
+    std::memset(&a, 0, sizeof(a));
+    a.fr = c->dFrame.as<FrameDev>(); a.buckets = c->dBuckets.as<BucketDev>(); a.tasks = c->dTasks.as<TaskDev>();
+    a.items = c->dItemsStatic.as<uint2>(); a.itemBase = 0; a.nStatic = (int)c->shape.tasks.size();
+    a.nDevice = c->dItemCount.as<uint32_t>(); a.persistent = 0; a.queueHead = c->dQueue.as<uint32_t>();
+    a.capStart = c->dCapStart.as<unsigned long long>(); a.listCount = c->dListCount.as<uint32_t>();
+    a.lists = c->dLists.as<uint32_t>(); a.mprec = c->dMprec.as<float>(); a.grids = c->dGrids.as<GridDev>();
+    a.prims = c->dPrims.as<PrimDev>(); a.pos2d = c->dPos2d.as<float>(); a.pos1d = c->dPos1d.as<float>();
+    a.pixPattern = c->dPix.as<uchar4>(); a.samples = c->dSamples.as<float4>(); a.hitData = c->dHitData.as<float4>();
+    a.hitNext = c->dHitNext.as<uint32_t>(); a.hitCounter = c->dHitCtl.as<uint32_t>();
+    a.combineList = c->dCombine.as<uint4>(); a.frontDepth = nullptr;
+    a.rowCounts = c->dRows.as<uint32_t>(); a.rowBase = 0;
+    a.idCut = 0xffffffffu; a.depthOnly = 1;
+    return a;
+}
+
+// Dice, bust and bin the resident schedule (its task lists), then, with
+// `sample`, sample every task for opaque depths.
+static hipError_t enqueue_depth_pass(rh_ctx* c, bool sample) {
+    const rh::Schedule& S = c->sched;
+    const rh::Shape& s = c->shape;
+    const FrameDev* f = c->dFrame.as<FrameDev>();
+    hipError_t e;
+    if ((e = rh_launch_frame_begin(c->st, c->dCounters.as<unsigned long long>(), c->dHitCtl.as<uint32_t>())) != hipSuccess)
+        return e;
+    if ((e = rh_launch_dice(c->st, f, c->dGrids.as<GridDev>(), (int)S.grids.size(), c->dPrims.as<PrimDev>(),
+                            c->dParams.as<float>(), c->dMprec.as<float>(), c->dMpbound.as<float4>(),
+                            S.motion ? c->dMprecMB.as<float4>() : nullptr)) != hipSuccess)
+        return e;
+    if ((e = rh_launch_bin(c->st, f, c->dBuckets.as<BucketDev>(), (int)s.buckets.size(), c->dTasks.as<TaskDev>(),
+                           c->dGrids.as<GridDev>(), c->dCandStart.as<uint32_t>(), c->dCandGrid.as<uint32_t>(),
+                           c->dSegBucket.as<uint32_t>(), S.candGrid.size(), c->dCapStart.as<unsigned long long>(),
+                           c->dMpbound.as<float4>(), c->dSegCounts.as<uint32_t>(), c->dLists.as<uint32_t>(),
+                           c->dListCount.as<uint32_t>(), c->dBucketRefs.as<uint32_t>(),
+                           c->dCounters.as<unsigned long long>(), c->binTaskSide,
+                           ((c->frame.bucketX & (c->frame.bucketX - 1)) == 0 &&
+                            (c->frame.bucketY & (c->frame.bucketY - 1)) == 0) ? 1 : 0,
+                           S.motion ? c->dMprecMB.as<float4>() : nullptr)) != hipSuccess)
+        return e;
+    if (!sample) return hipSuccess;
+    SampleLaunch a = depth_launch(c);
+    const int nt = (int)s.tasks.size();
+    if (S.motion) return rh_launch_sample_mb(c->st, &a, nt, c->dMprecMB.as<float4>());
+    return rh_launch_sample(c->st, 0, nt, &a);
+}
+
+// The schedule aqsis's occlusion culling gives (RenderSurface,
+// bucketprocessor.cpp:945-957; RepostSurface, imagebuffer.cpp:449-483;
+// CqOcclusionTree::canCull, occlusion.cpp:143-196).  A bucket's tree holds
+// the opaque depths of its samples at the moment a surface is taken from
+// its heap; depths only decrease, and a surface popped later than another
+// cannot reach below that one's minimum depth except by rounding, so the
+// frame's final depths (from the culling-free schedule, which the resident
+// buffers hold) decide every test whose depths lie clear of the surface's
+// minimum depth by a margin.  The others (CullProbe) are decided by the
+// bucket's depths at that moment: the bucket's lists are rendered up to the
+// test's first micropolygon and the leaves read back.  A probe that
+// overturns a decision fixes it and the schedule is computed again.
+static rh_status cull_schedule(rh_ctx* c, const rh::Forest& forest) {
+    const rh::Shape& s = c->shape;
+    const int n = (int)c->prims.size();
+    // 1) the final opaque depths of the culling-free schedule
+    HIP_TRY(enqueue_depth_pass(c, true));
+    // 2) every bucket's depth pyramid and every (surface, bucket) test
+    rh::CullPairs pairs;
+    rh::cull_pairs(s, forest, pairs);
+    std::vector<CullTreeDev> trees;
+    const uint64_t pyrFloats = rh::cull_trees(s, trees);
+    const size_t nnodes = pairs.start.size() - 1;
+    const uint64_t npairs = pairs.start.back();
+    std::vector<float> pairMax(npairs, -1.0f);
+    if (npairs > 0) {
+        HIP_TRY(upload(c->dCullTrees, trees, c->st));
+        HIP_TRY(c->dCullPyr.ensure(pyrFloats * sizeof(float)));
+        HIP_TRY(rh_launch_cull_pyramid(c->st, c->dFrame.as<FrameDev>(), c->dCullTrees.as<CullTreeDev>(),
+                                       (int)trees.size(), c->dSamples.as<float4>(), c->dCullPyr.as<float>()));
+        std::vector<unsigned long long> st(pairs.start.begin(), pairs.start.end());
+        HIP_TRY(upload(c->dCullBounds, pairs.bound, c->st));
+        HIP_TRY(upload(c->dCullRanges, pairs.range, c->st));
+        HIP_TRY(upload(c->dCullStart, st, c->st));
+        HIP_TRY(c->dCullPairMax.ensure(npairs * sizeof(float)));
+        HIP_TRY(rh_launch_cull_pairs(c->st, c->dCullTrees.as<CullTreeDev>(), s.bx0, s.by0, s.nbr,
+                                     c->dCullBounds.as<float4>(), c->dCullRanges.as<int4>(),
+                                     c->dCullStart.as<unsigned long long>(), (int)nnodes, npairs,
+                                     c->dCullPyr.as<float>(), c->dCullPairMax.as<float>()));
+        HIP_TRY(hipMemcpyAsync(pairMax.data(), c->dCullPairMax.p, npairs * sizeof(float), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+    }
+    // 3) the culling schedule, its probes, and again while a probe overturns a decision
+    rh::CullInput in;
+    in.pairMax = pairMax.data();
+    std::string err;
+    for (int pass = 0;; ++pass) {
+        rh::Schedule S;
+        if (!rh::simulate(s, c->prims.data(), n, forest, &in, &pairs, S, err)) return fail(RH_ERR_INVALID, err);
+        c->sched = std::move(S);
+        rh_status r = upload_schedule(c);
+        if (r != RH_OK) return r;
+        c->stats.cull_passes++;
+        const std::vector<rh::CullProbe>& probes = c->sched.probes;
+        if (probes.empty()) break;
+        if (pass > 64) return fail(RH_ERR_STATE, "occlusion culling: probes kept overturning decisions");
+        // the schedule's task lists, then each probe's bucket up to its moment
+        HIP_TRY(enqueue_depth_pass(c, false));
+        HIP_TRY(c->dProbeOut.ensure(probes.size() * sizeof(float)));
+        for (size_t i = 0; i < probes.size(); ++i) {
+            const rh::CullProbe& pr = probes[i];
+            const int t0 = s.taskStart[pr.bucket], t1 = s.taskStart[pr.bucket + 1];
+            if (t1 > t0) {
+                SampleLaunch a = depth_launch(c);
+                a.itemBase = t0;
+                a.nStatic = t1 - t0;
+                a.rowBase = t0;
+                a.idCut = pr.mpCut;
+                if (c->sched.motion) HIP_TRY(rh_launch_sample_mb(c->st, &a, t1 - t0, c->dMprecMB.as<float4>()));
+                else HIP_TRY(rh_launch_sample(c->st, 0, t1 - t0, &a));
+            }
+            HIP_TRY(rh_launch_cull_probe_max(c->st, c->dFrame.as<FrameDev>(), c->dCullTrees.as<CullTreeDev>(),
+                                             pr.bucket, t1 > t0 ? pr.kind : 0, pr.cx0, pr.cx1, pr.cy0, pr.cy1,
+                                             c->dSamples.as<float4>(), c->dProbeOut.as<float>() + i));
+        }
+        std::vector<float> res(probes.size());
+        HIP_TRY(hipMemcpyAsync(res.data(), c->dProbeOut.p, res.size() * sizeof(float), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+        c->stats.cull_probes += probes.size();
+        // Decisions up to the first overturned one stand (the schedule before
+        // it is the reference's); that one is fixed and the rest recomputed.
+        bool overturned = false;
+        for (size_t i = 0; i < probes.size() && !overturned; ++i) {
+            const uint8_t exact = res[i] < probes[i].z ? 1 : 0;
+            in.fixed.emplace_back(probes[i].pair, exact);
+            overturned = exact != probes[i].culled;
+        }
+        std::sort(in.fixed.begin(), in.fixed.end());
+        if (!overturned) break;
+    }
+    // the scratch of the tests goes back to the pool
+    for (DBuf* b : {&c->dCullPyr, &c->dCullBounds, &c->dCullRanges, &c->dCullStart, &c->dCullPairMax})
+        b->release();
+    return RH_OK;
+}
+
 // One frame on the resident inputs.  The batched form replays its frame
 // graph unless `eager` asks for the same launches issued one by one with a
 // timestamp between the stages.
 static rh_status render_frame(rh_ctx* c, bool eager) {
     if (!c) return fail(RH_ERR_INVALID, "null context");
     if (!c->scheduled) return fail(RH_ERR_STATE, "rh_render must run before rh_render_device");
+    // The kernels overwrite the frame's buffers: until this frame completes,
+    // nothing may be read back.
+    c->rendered = false;
     const rh::Schedule& S = c->sched;
     const bool graphed = c->tuning.mode == RH_MODE_BATCHED && !eager;
     unsigned long long cnt[RH_NCOUNTERS];
@@ -670,7 +857,7 @@ static rh_status render_frame(rh_ctx* c, bool eager) {
     c->stats.grids = S.grids.size();
     c->stats.micropolygons = S.micropolygons;
     c->stats.surfaces = S.surfaces;
-    c->stats.occlusion_culled = 0;
+    c->stats.occlusion_culled = S.culled;
     c->stats.ms_dice = t[0];
     c->stats.ms_bin = t[1];
     c->stats.ms_sample = t[2];
@@ -868,7 +1055,9 @@ uint64_t rh_internal_device_bytes(const rh_ctx* c) {
                          &c->dCandGrid, &c->dCapStart, &c->dSegBucket, &c->dSegCounts, &c->dMprec, &c->dMpbound, &c->dLists,
                          &c->dListCount, &c->dSamples, &c->dChannels, &c->dRgba, &c->dCounters,
                          &c->dBucketRefs, &c->dHitData, &c->dHitNext, &c->dHitCtl, &c->dCombine, &c->dFrontDepth,
-                         &c->dItemsStatic, &c->dItems, &c->dItemCount, &c->dQueue, &c->dRows, &c->dMprecMB};
+                         &c->dItemsStatic, &c->dItems, &c->dItemCount, &c->dQueue, &c->dRows, &c->dMprecMB,
+                         &c->dCullTrees, &c->dCullPyr, &c->dCullBounds, &c->dCullRanges, &c->dCullStart,
+                         &c->dCullPairMax, &c->dProbeOut};
     uint64_t total = 0;
     for (const DBuf* b : all) total += b->cap;
     return total;
@@ -903,23 +1092,40 @@ static uint64_t footprint_bytes(const rh::Shape& s, const rh::Schedule& S, const
 }
 
 // Used by the serving loop: the split/dice schedule of a job, computed once
-// when it is submitted, with its footprint; and installing it in a context.
+// when it is submitted, with its footprint; and installing it in a context
+// (which, when surfaces may be occlusion culled, computes the culling
+// schedule there on the device).
 bool rh_internal_schedule(const rh_options& opt, const rh_prim* prims, int32_t n, int threads, rh::Schedule& S,
-                          uint64_t& bytes, double& ms, std::string& err) {
+                          rh::ForestPtr& forest, uint64_t& bytes, double& ms, std::string& err) {
     auto t0 = std::chrono::steady_clock::now();
     rh::Shape s;
     if (!rh::make_shape(opt, s, err)) return false;
-    if (!rh::make_schedule(s, prims, n, S, err, threads)) return false;
+    if (!rh::build_forest(s, prims, n, forest, err, threads)) return false;
+    if (!rh::simulate(s, prims, n, *forest, nullptr, nullptr, S, err)) return false;
     bytes = footprint_bytes(s, S, prims, n);
+    bool cullable = false;
+    for (int32_t i = 0; i < n && !cullable; ++i) cullable = !prims[i].no_cull_hidden;
+    if (cullable) {
+        // the tests' scratch: depth pyramids, surface bounds and ranges, pair results
+        rh::CullPairs pairs;
+        rh::cull_pairs(s, *forest, pairs);
+        std::vector<CullTreeDev> trees;
+        const uint64_t pyr = rh::cull_trees(s, trees);
+        const uint64_t extra = pyr * 4 + trees.size() * sizeof(CullTreeDev) + (pairs.start.size() - 1) * 40 +
+                               pairs.start.back() * 4;
+        bytes += extra + extra / 8;
+    }
     ms = ms_since(t0);
     return true;
 }
 
-rh_status rh_internal_post_scheduled(rh_ctx* c, const rh_prim* prims, int32_t n, rh::Schedule&& S, double ms) {
+rh_status rh_internal_post_scheduled(rh_ctx* c, const rh_prim* prims, int32_t n, rh::Schedule&& S,
+                                     rh::ForestPtr&& forest, double ms) {
     c->prims.assign(prims, prims + n);
-    c->sched = std::move(S);
-    c->stats.ms_schedule = ms;
-    rh_status r = upload_schedule(c);
+    c->dropGraph();
+    const auto t0 = std::chrono::steady_clock::now();
+    rh_status r = install_schedule(c, std::move(S), *forest, t0);
+    c->stats.ms_schedule += ms;
     if (r == RH_OK) c->rendered = false;
     return r;
 }
@@ -928,19 +1134,21 @@ extern "C" {
 
 rh_status rh_estimate(const rh_options* opt, const rh_prim* prims, int32_t n, rh_footprint* out) {
     if (!opt || !out || n < 0 || (n > 0 && !prims)) return fail(RH_ERR_INVALID, "bad arguments");
-    rh::Shape s;
     std::string err;
-    if (!rh::make_shape(*opt, s, err)) return fail(RH_ERR_INVALID, err);
     rh::Schedule S;
+    rh::ForestPtr f;
     int threads = (int)std::thread::hardware_concurrency();
     if (threads > 16) threads = 16;
-    if (!rh::make_schedule(s, prims, n, S, err, threads)) return fail(RH_ERR_INVALID, err);
+    uint64_t bytes = 0;
+    double ms = 0.0;
+    if (!rh_internal_schedule(*opt, prims, n, threads, S, f, bytes, ms, err)) return fail(RH_ERR_INVALID, err);
+    // (the culling-free schedule: occlusion culling only removes work)
     out->surfaces = S.surfaces;
     out->grids = S.grids.size();
     out->micropolygons = S.micropolygons;
     out->vertices = S.vertices;
     out->list_capacity = S.capStart.back();
-    out->device_bytes = footprint_bytes(s, S, prims, n);
+    out->device_bytes = bytes;
     return RH_OK;
 }
 

@@ -34,6 +34,7 @@
 
 #include "devtypes.h"
 #include "hitsort.h"
+#include "cullgeom.h"
 
 namespace {
 
@@ -788,6 +789,20 @@ __device__ __forceinline__ void shade_hit(const PrimDev& P, float u, float v, fl
     B.w = __uint_as_float(id);
 }
 
+// The part of a task's list (ascending micropolygon ids, the busting order)
+// below idCut: the micropolygons rendered in a bucket before a surface is
+// taken from its heap (0xffffffff: the whole list).
+__device__ __forceinline__ uint32_t list_cut(const uint32_t* __restrict__ L, uint32_t n, uint32_t idCut) {
+    if (idCut == 0xffffffffu) return n;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (L[mid] < idCut) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 #define RH_S_WAVES 4
 #define RH_S_GROUP 16
 
@@ -809,7 +824,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
           const float* __restrict__ pos2d, const float* __restrict__ pos1d, const uchar4* __restrict__ pixPattern,
           float4* __restrict__ samples, float4* __restrict__ hitData, uint32_t* __restrict__ hitNext,
           uint32_t* __restrict__ hitCounter, uint4* __restrict__ combineList, float* __restrict__ frontDepth,
-          uint32_t* __restrict__ rowCounts, int rowBase) {
+          uint32_t* __restrict__ rowCounts, int rowBase, uint32_t idCut, int depthOnly) {
     __shared__ float4 recs[CH * 9];
     __shared__ uint32_t wsum[RH_S_WAVES][5];
     __shared__ uint2 meta[CH];
@@ -861,7 +876,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
     const int tw = T.x1 - T.x0;
     const int nbx = (tw + bw - 1) / bw;
     const uint32_t* L = lists + capStart[task];
-    const uint32_t n = listCount[task];
+    const uint32_t n = list_cut(L, listCount[task], idCut);
 
     for (int p0 = 0; p0 < blkN; p0 += RH_S_WAVES * SLOTS) {
         bool has[SLOTS], live[SLOTS], lodOk[SLOTS];
@@ -1086,7 +1101,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                         wu[k] = opq ? u : wu[k];
                         wv[k] = opq ? v : wv[k];
                         // transparent: a new entry of the sample's hit list
-                        const bool app = hit && (hflags & 2u) == 0;
+                        const bool app = hit && (hflags & 2u) == 0 && !depthOnly;
                         // Append the wave's transparent hits (converged): entries come
                         // from 64-entry blocks of the frame's arena, one atomic per block.
                         const unsigned long long am = __ballot(app);
@@ -1341,7 +1356,8 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                                                         float4* __restrict__ samples, float4* __restrict__ hitData,
                                                         uint32_t* __restrict__ hitNext, uint32_t* __restrict__ hitCounter,
                                                         uint4* __restrict__ combineList, float* __restrict__ frontDepth,
-                                                        uint32_t* __restrict__ rowCounts, int rowBase) {
+                                                        uint32_t* __restrict__ rowCounts, int rowBase, int taskBase,
+                                                        uint32_t idCut, int depthOnly) {
     __shared__ float4 recs[RH_MB_RECS * 8];
     __shared__ float kbs[RH_MB_RECS][12];          // key bounds, kb[k][c] at [k * 6 + c]
     __shared__ float dts[RH_MB_RECS];
@@ -1365,7 +1381,7 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
     __shared__ uint32_t wtot[NW];
     __shared__ int nfitShared;
     __shared__ uint32_t wsum[NW][5];
-    const int task = blockIdx.x;
+    const int task = taskBase + (int)blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned long long lanesBelow = (1ull << lane) - 1ull;
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
@@ -1383,7 +1399,7 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
     const BucketDev B = buckets[T.bucket];
     const int nbx = (T.x1 - T.x0 + bw - 1) / bw, nby = (T.y1 - T.y0 + bh - 1) / bh;
     const uint32_t* L = lists + capStart[task];
-    const uint32_t n = listCount[task];
+    const uint32_t n = list_cut(L, listCount[task], idCut);
     const float opentime = fr->shutterOpen, closetime = fr->shutterClose;
     // isClose (math.h:174-182)
     const float dsh = fabsf(closetime - opentime);
@@ -1824,7 +1840,7 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                             win = opq ? hid : win;
                             wu = opq ? u : wu;
                             wv = opq ? v : wv;
-                            const bool app = hit && !opaque;
+                            const bool app = hit && !opaque && !depthOnly;
                             const unsigned long long am = __ballot(app);
                             if (am) {
                                 const int na = __popcll(am);
@@ -2337,6 +2353,139 @@ __global__ void __launch_bounds__(128) rh_filter_tiled(const FrameDev* __restric
 // ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Occlusion culling: each bucket's tree of sub-sample depths
+// (CqOcclusionTree, libs/core/occlusion.cpp) and the cull tests of every
+// (surface, bucket) pair of a scene, on the depths of a frame.
+// ---------------------------------------------------------------------------
+// The opaque depth (occlZ) of sub-sample cell (cx, cy) of a bucket's sample
+// region, from a frame's sample records (the samplers store it in the first
+// record's w before any combine).
+__device__ __forceinline__ float cell_depth(const FrameDev* __restrict__ fr, const CullTreeDev& t,
+                                            const float4* __restrict__ samples, int cx, int cy) {
+    const int nx = fr->nx, ny = fr->ny;
+    const int px = t.sx0 + cx / nx, py = t.sy0 + cy / ny;
+    const int si = (cy - (cy / ny) * ny) * nx + (cx - (cx / nx) * nx);
+    const size_t sidx = ((size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)) * fr->ns + si;
+    return samples[2 * sidx].w;
+}
+
+// One workgroup per bucket: its depth pyramid, level 0 the cells, each
+// further level the maximum of 2x2 cells of the one below.
+__global__ void __launch_bounds__(256) rh_cull_pyramid(const FrameDev* __restrict__ fr,
+                                                       const CullTreeDev* __restrict__ trees,
+                                                       const float4* __restrict__ samples, float* __restrict__ pyr) {
+    const CullTreeDev t = trees[blockIdx.x];
+    if (t.levels == 0) return;
+    float* L = pyr + t.pyr;
+    const int n0 = t.cw * t.ch;
+    for (int i = threadIdx.x; i < n0; i += blockDim.x) L[i] = cell_depth(fr, t, samples, i % t.cw, i / t.cw);
+    int w = t.cw, h = t.ch;
+    for (int l = 1; l < t.levels; ++l) {
+        __syncthreads();
+        float* U = L + (size_t)w * h;
+        const int uw = cull_level_dim(t.cw, l), uh = cull_level_dim(t.ch, l);
+        for (int i = threadIdx.x; i < uw * uh; i += blockDim.x) {
+            const int x = i % uw, y = i / uw;
+            const int x0 = 2 * x, y0 = 2 * y;
+            float m = L[(size_t)y0 * w + x0];
+            if (x0 + 1 < w) m = fmaxf(m, L[(size_t)y0 * w + x0 + 1]);
+            if (y0 + 1 < h) {
+                m = fmaxf(m, L[(size_t)(y0 + 1) * w + x0]);
+                if (x0 + 1 < w) m = fmaxf(m, L[(size_t)(y0 + 1) * w + x0 + 1]);
+            }
+            U[i] = m;
+        }
+        L = U;
+        w = uw;
+        h = uh;
+    }
+}
+
+// One thread per (surface, bucket) pair: the largest depth among the tree
+// leaves the surface's raster bound meets (padding leaves count as 0), or
+// -1 when it meets none; the surface is culled there when this lies below
+// its minimum depth (canCull, occlusion.cpp:143-196).
+__global__ void __launch_bounds__(256) rh_cull_pairs(const CullTreeDev* __restrict__ trees, int bx0, int by0, int nbr,
+                                                     const float4* __restrict__ bounds,
+                                                     const int4* __restrict__ ranges,
+                                                     const unsigned long long* __restrict__ start, int nnodes,
+                                                     unsigned long long npairs, const float* __restrict__ pyr,
+                                                     float* __restrict__ pairMax) {
+    const unsigned long long pi = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= npairs) return;
+    // the surface: the last one whose first pair is at or before pi
+    int lo = 0, hi = nnodes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= pi) lo = mid;
+        else hi = mid - 1;
+    }
+    const int4 R = ranges[lo];
+    const float4 bd = bounds[lo];
+    const unsigned long long off = pi - start[lo];
+    const int nbx = R.z - R.x + 1;
+    const int col = R.x + (int)(off % (unsigned long long)nbx), row = R.y + (int)(off / (unsigned long long)nbx);
+    const CullTreeDev t = trees[(row - by0) * nbr + (col - bx0)];
+    int cx0 = 0, cx1 = -1, cy0 = 0, cy1 = -1;
+    const int kind = t.levels ? cull_cells(t, bd.x, bd.y, bd.z, bd.w, cx0, cx1, cy0, cy1) : 0;
+    float m = -1.0f;
+    if (kind == 2) m = 0.0f;
+    if (kind != 0 && cx0 <= cx1 && cy0 <= cy1) m = fmaxf(m, pyramid_max(pyr + t.pyr, t.cw, t.ch, cx0, cx1, cy0, cy1));
+    pairMax[pi] = m;
+}
+
+// One workgroup per probe: the largest depth over a probe's cells, from a
+// frame's sample records rendered up to the probe's moment.
+__global__ void __launch_bounds__(256) rh_cull_probe_max(const FrameDev* __restrict__ fr,
+                                                         const CullTreeDev* __restrict__ trees, int bucket, int kind,
+                                                         int cx0, int cx1, int cy0, int cy1,
+                                                         const float4* __restrict__ samples,
+                                                         float* __restrict__ out) {
+    __shared__ float part[4];
+    const CullTreeDev t = trees[bucket];
+    float m = -FLT_MAX;
+    const int w = cx1 - cx0 + 1, h = cy1 - cy0 + 1;
+    if (kind != 0 && w > 0 && h > 0)
+        for (int i = threadIdx.x; i < w * h; i += blockDim.x)
+            m = fmaxf(m, cell_depth(fr, t, samples, cx0 + i % w, cy0 + i / w));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+        if (kind == 2) r = fmaxf(r, 0.0f);
+        if (kind == 0 || r == -FLT_MAX) r = kind == 2 ? 0.0f : -1.0f;
+        *out = r;
+    }
+}
+
+extern "C" hipError_t rh_launch_cull_pyramid(hipStream_t st, const FrameDev* fr, const CullTreeDev* trees, int nbuckets,
+                                             const float4* samples, float* pyr) {
+    if (nbuckets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rh_cull_pyramid, dim3(nbuckets), dim3(256), 0, st, fr, trees, samples, pyr);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rh_launch_cull_pairs(hipStream_t st, const CullTreeDev* trees, int bx0, int by0, int nbr,
+                                           const float4* bounds, const int4* ranges, const unsigned long long* start,
+                                           int nnodes, unsigned long long npairs, const float* pyr, float* pairMax) {
+    if (npairs == 0) return hipSuccess;
+    const unsigned long long blocks = (npairs + 255) / 256;
+    if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(rh_cull_pairs, dim3((unsigned)blocks), dim3(256), 0, st, trees, bx0, by0, nbr, bounds, ranges,
+                       start, nnodes, npairs, pyr, pairMax);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rh_launch_cull_probe_max(hipStream_t st, const FrameDev* fr, const CullTreeDev* trees, int bucket,
+                                               int kind, int cx0, int cx1, int cy0, int cy1, const float4* samples,
+                                               float* out) {
+    hipLaunchKernelGGL(rh_cull_probe_max, dim3(1), dim3(256), 0, st, fr, trees, bucket, kind, cx0, cx1, cy0, cy1,
+                       samples, out);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t rh_launch_dice(hipStream_t st, const FrameDev* fr, const GridDev* grids, int ngrids,
                                      const PrimDev* prims, const float* params, float* mprec,
                                      float4* mpbound, float4* mprecMB) {
@@ -2406,7 +2555,7 @@ static void launch_sample_t(hipStream_t st, int grid, const SampleLaunch& a) {
                        a.itemBase, a.nStatic, a.nDevice, a.persistent, a.queueHead, a.capStart, a.listCount, a.lists,
                        reinterpret_cast<const float4*>(a.mprec), a.grids, a.prims, a.pos2d, a.pos1d, a.pixPattern,
                        a.samples, a.hitData, a.hitNext, a.hitCounter, a.combineList, a.frontDepth, a.rowCounts,
-                       a.rowBase);
+                       a.rowBase, a.idCut, a.depthOnly);
 }
 
 // One sampling launch of `grid` workgroups with kernel variant `variant`
@@ -2423,15 +2572,16 @@ extern "C" hipError_t rh_launch_sample(hipStream_t st, int variant, int grid, co
     return hipGetLastError();
 }
 
-// The motion form of the sampler: one workgroup of RH_MB_WAVES waves per task.
+// The motion form of the sampler: one workgroup of RH_MB_WAVES waves per
+// task, tasks [a->itemBase, a->itemBase + ntasks).
 #define RH_MB_WAVES 16
 extern "C" hipError_t rh_launch_sample_mb(hipStream_t st, const SampleLaunch* a, int ntasks,
                                           const float4* mprecMB) {
     if (ntasks <= 0) return hipSuccess;
-    hipLaunchKernelGGL((rh_sample_mb<RH_MB_WAVES>), dim3(ntasks), dim3(RH_MB_WAVES * 64), 0, st, a->fr, a->buckets, a->tasks, ntasks,
-                       a->capStart, a->listCount, a->lists, mprecMB, a->grids, a->prims, a->pos2d, a->pos1d,
-                       a->pixPattern, a->samples, a->hitData, a->hitNext, a->hitCounter, a->combineList,
-                       a->frontDepth, a->rowCounts, a->rowBase);
+    hipLaunchKernelGGL((rh_sample_mb<RH_MB_WAVES>), dim3(ntasks), dim3(RH_MB_WAVES * 64), 0, st, a->fr, a->buckets, a->tasks,
+                       a->itemBase + ntasks, a->capStart, a->listCount, a->lists, mprecMB, a->grids, a->prims,
+                       a->pos2d, a->pos1d, a->pixPattern, a->samples, a->hitData, a->hitNext, a->hitCounter,
+                       a->combineList, a->frontDepth, a->rowCounts, a->rowBase, a->itemBase, a->idCut, a->depthOnly);
     return hipGetLastError();
 }
 

@@ -30,8 +30,9 @@
 uint64_t rh_internal_device_bytes(const rh_ctx* c);
 void rh_internal_trim_pool(uint64_t keep);
 bool rh_internal_schedule(const rh_options& opt, const rh_prim* prims, int32_t n, int threads, rh::Schedule& S,
-                          uint64_t& bytes, double& ms, std::string& err);
-rh_status rh_internal_post_scheduled(rh_ctx* c, const rh_prim* prims, int32_t n, rh::Schedule&& S, double ms);
+                          rh::ForestPtr& forest, uint64_t& bytes, double& ms, std::string& err);
+rh_status rh_internal_post_scheduled(rh_ctx* c, const rh_prim* prims, int32_t n, rh::Schedule&& S,
+                                     rh::ForestPtr&& forest, double ms);
 
 namespace {
 
@@ -65,6 +66,7 @@ struct Job {
     std::vector<rh_prim> prims;
     uint64_t estimate = 0;                // device bytes of the scheduled frame
     rh::Schedule sched;                   // computed at submission, used once
+    rh::ForestPtr forest;                 // its split trees (for the culling schedule)
     double msSchedule = 0.0;
     int32_t status = RH_OK;
     int state = 0;                        // 0 queued, 1 running, 2 done
@@ -127,7 +129,8 @@ struct rh_server {
             }
             std::string err;
             const bool ok = rh_internal_schedule(j->opt, j->prims.data(), (int32_t)j->prims.size(),
-                                                 threadsPerSchedule, j->sched, j->estimate, j->msSchedule, err);
+                                                 threadsPerSchedule, j->sched, j->forest, j->estimate, j->msSchedule,
+                                                 err);
             {
                 std::lock_guard<std::mutex> lk(m);
                 if (!ok) {
@@ -225,7 +228,7 @@ struct rh_server {
         const Clock::time_point t1 = Clock::now();
         if (st == RH_OK)
             st = rh_internal_post_scheduled(r.ctx, j->prims.data(), (int32_t)j->prims.size(), std::move(j->sched),
-                                            j->msSchedule);
+                                            std::move(j->forest), j->msSchedule);
         if (st == RH_OK) st = rh_render(r.ctx);
         const Clock::time_point t2 = Clock::now();
         if (st == RH_OK) st = rh_frame_size(r.ctx, &j->width, &j->height, nullptr, nullptr, nullptr);

@@ -27,8 +27,11 @@ request and line.
 
 Display requests are counted into num_displays, since each display request
 draws one dither value per pixel from the shared random stream
-(ddmanager.cpp:1050); render_file writes the first display as a TIFF file
-when its type is "file" or "tiff" (tools/displays/file/display.cpp).
+(ddmanager.cpp:1050).  As RiDisplay does (libs/core/api/ri.cpp:995-1004), a
+request whose name starts with '+' adds a display and any other replaces the
+requests made so far; the name is kept without its '+'.  render_file writes
+the first display as a TIFF file when its type is "file" or "tiff"
+(tools/displays/file/display.cpp).
 """
 import math
 import re
@@ -187,7 +190,7 @@ def parse(text, displays=None):
     requests are appended to `displays` when a list is given, as dicts with
     the name, type, mode, parameters and line."""
     opts = default_options()
-    ndisplays = 0
+    requests = []          # the display requests in force
     prims = []
     # The current transformation: one matrix, or one per motion key when it
     # moves (`keys`, with the key times in `ktimes`); likewise the camera.
@@ -263,10 +266,13 @@ def parse(text, displays=None):
             elif req == "Display":
                 if len(args) < 3 or not all(isinstance(a, str) for a in args[:3]):
                     raise RibError(f"line {line}: Display needs a name, a type and a mode")
-                ndisplays += 1
-                if displays is not None:
-                    displays.append({"name": args[0], "type": args[1], "mode": args[2],
-                                     "params": _params(args[3:]), "line": line})
+                name = args[0]
+                if name.startswith("+"):
+                    name = name[1:]
+                else:
+                    requests = []          # ClearDisplayRequests
+                requests.append({"name": name, "type": args[1], "mode": args[2],
+                                 "params": _params(args[3:]), "line": line})
             elif req in ("FrameBegin", "FrameEnd", "Declare", "Sides"):
                 pass
             elif req == "MotionBegin":
@@ -361,8 +367,10 @@ def parse(text, displays=None):
             raise RibError(f"line {line}: malformed {req}: {e}") from None
     if motion is not None:
         raise RibError(f"line {motion['line']}: MotionBegin without MotionEnd")
-    if ndisplays:
-        opts.num_displays = ndisplays
+    if requests:
+        opts.num_displays = len(requests)
+    if displays is not None:
+        displays.extend(requests)
     return opts, prims
 
 
@@ -395,7 +403,7 @@ def render_file(path, out_dir=None):
         h.post_surfaces(prims)
         h.render()
         for d in files:
-            target = os.path.join(out_dir, d["name"].lstrip("+"))
+            target = os.path.join(out_dir, d["name"])
             h.write_tiff(target, d["params"].get("compression", ["lzw"])[0], channels=len(d["mode"]))
             written.append(target)
     return written

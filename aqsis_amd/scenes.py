@@ -151,6 +151,86 @@ def patches_scene(seed=1, n=10000, res=(1920, 1080), fov=45.0, znear=8.0,
     return opts, prims
 
 
+def raster_quad(opts, rect, z, Cs, Os=(1.0, 1.0, 1.0), shading_rate=1.0):
+    """A bilinear patch facing the camera at camera depth z whose raster
+    projection is the rectangle rect = (x0, x1, y0, y1) of the frame `opts`
+    describes (perspective, pixel aspect 1)."""
+    W, H = opts.xres, opts.yres
+    aspect = W / H
+    tanh = math.tan(math.radians(opts.fov) / 2.0)
+    sx = tanh * max(aspect, 1.0)
+    sy = tanh * max(1.0 / aspect, 1.0)
+
+    def cam(rx, ry):
+        return [(rx / W * 2.0 - 1.0) * sx * z, (1.0 - ry / H * 2.0) * sy * z, z]
+
+    x0, x1, y0, y1 = rect
+    P = [cam(x0, y0), cam(x1, y0), cam(x0, y1), cam(x1, y1)]
+    return bilinear(P, Cs, Os, shading_rate)
+
+
+def coplanar_scene(res=(64, 32), samples=(2, 2), filt=("box", 1.0, 1.0)):
+    """Exact depth ties that occlusion culling reorders: a near quad covering
+    bucket 0, and two coplanar quads at z = 10 that overlap in bucket 1, the
+    first starting in bucket 0 (behind the near quad there, so aqsis culls it
+    and reposts it to bucket 1) and the second starting in bucket 1."""
+    opts = default_options(xres=res[0], yres=res[1], fov=90.0, xsamples=samples[0], ysamples=samples[1],
+                           filter=filt[0], filter_xwidth=filt[1], filter_ywidth=filt[2])
+    prims = [raster_quad(opts, (-4.0, 18.0, -4.0, 18.0), 2.0, (0.9, 0.1, 0.1)),
+             raster_quad(opts, (4.0, 28.0, 4.0, 12.0), 10.0, (0.1, 0.9, 0.1)),
+             raster_quad(opts, (20.0, 40.0, 2.0, 14.0), 10.0, (0.1, 0.1, 0.9))]
+    return opts, prims
+
+
+def decal_scene(seed=1, n=60, res=(128, 96), samples=(2, 2), filt=("gaussian", 2.0, 2.0), planes=(6.0, 9.0, 12.0),
+                opacity=None, occluders=6):
+    """Coplanar geometry at many places: `n` camera-facing quads of random
+    raster size, each on one of a few depth planes (so quads on one plane tie
+    exactly wherever they overlap), behind `occluders` nearer quads that
+    cover whole buckets, so occlusion culling reposts surfaces.  opacity=(lo,
+    hi) makes the decals transparent (equal-depth transparent hits)."""
+    rng = np.random.default_rng(seed)
+    opts = default_options(xres=res[0], yres=res[1], fov=60.0, xsamples=samples[0], ysamples=samples[1],
+                           filter=filt[0], filter_xwidth=filt[1], filter_ywidth=filt[2])
+    prims = []
+    for _ in range(occluders):
+        bx = 16.0 * rng.integers(0, max(1, res[0] // 16))
+        by = 16.0 * rng.integers(0, max(1, res[1] // 16))
+        w = 16.0 * rng.integers(1, 3)
+        prims.append(raster_quad(opts, (bx - 3.0, bx + w + 3.0, by - 3.0, by + 19.0), 3.0,
+                                 np.float32(rng.uniform(0, 1, 3))))
+    for _ in range(n):
+        w, h = rng.uniform(6.0, 40.0, 2)
+        x0 = rng.uniform(-10.0, res[0] - 5.0)
+        y0 = rng.uniform(-10.0, res[1] - 5.0)
+        z = float(planes[rng.integers(0, len(planes))])
+        if opacity is None:
+            os_ = (1.0, 1.0, 1.0)
+        else:
+            o = rng.uniform(opacity[0], opacity[1])
+            os_ = (o, o, o)
+        prims.append(raster_quad(opts, (x0, x0 + w, y0, y0 + h), z, np.float32(rng.uniform(0, 1, 3)),
+                                 np.float32(os_)))
+    return opts, prims
+
+
+def eye_plane_scene(seed=1, res=(96, 64), samples=(2, 2), filt=("gaussian", 2.0, 2.0), n=40):
+    """Surfaces that cross the eye plane: two ground planes (a bicubic and a
+    bilinear patch) running from behind the camera (z = -6) to z = 30 below
+    the view axis, which CullSurface marks undiceable and splits until the
+    pieces lie in front of the camera or behind it, or exceed the eye-split
+    limit (imagebuffer.cpp:284-300); plus `n` ordinary patches."""
+    rng = np.random.default_rng(seed)
+    opts, prims = patches_scene(seed=seed, n=n, res=res, samples=samples, filt=filt, znear=6.0, zfar=20.0)
+    xs = np.linspace(-12.0, 12.0, 4, dtype=np.float32)
+    zs = np.linspace(-6.0, 30.0, 4, dtype=np.float32)
+    P = np.array([[x, -2.5 + 0.3 * np.sin(x + z), z] for z in zs for x in xs], np.float32)
+    prims.append(bicubic(P, np.float32(rng.uniform(0, 1, 3))))
+    Q = np.array([[-8.0, 2.0, -4.0], [8.0, 2.0, -4.0], [-8.0, 1.5, 25.0], [8.0, 1.5, 25.0]], np.float32)
+    prims.append(bilinear(Q, np.float32(rng.uniform(0, 1, 3))))
+    return opts, prims
+
+
 def motion_scene(seed=1, n=600, res=(256, 256), fov=45.0, znear=8.0, zfar=40.0, size_px=28.0,
                  jitter=0.1, samples=(8, 8), filt=("gaussian", 2.0, 2.0), opacity=(0.2, 0.8),
                  shift_sigma=0.05, shutter=(0.0, 1.0), key_times=(0.0, 1.0), shading_rate=1.0):

@@ -98,6 +98,9 @@ typedef struct rh_options {
  * / RiMotionEnd: CqDeformingSurface, geometry/surface.h:617-720) with keys at
  * key_time[0] < key_time[1]: patches have key 1's camera-space points in P1
  * (same layout as P), spheres key 1's object->camera matrix in xform[1].
+ * no_cull_hidden = 1 is Attribute "cull" "hidden" [0]: the surface is never
+ * occlusion culled (RenderSurface, bucketprocessor.cpp:945-957); 0, the
+ * reference's default, lets a bucket's occlusion tree cull it.
  */
 typedef struct rh_prim {
     int32_t type;
@@ -110,6 +113,7 @@ typedef struct rh_prim {
     float xform[2][16];
     float P[16][3];
     float P1[16][3];
+    int32_t no_cull_hidden;
 } rh_prim;
 
 /* Counters of one frame.  spl_* are the reference's own hider counters
@@ -125,7 +129,7 @@ typedef struct rh_stats {
     uint64_t micropolygons;    /* busted micropolygons                          */
     uint64_t mp_bucket_refs;   /* (micropolygon, bucket) list entries           */
     uint64_t surfaces;         /* surfaces posted, including split children     */
-    uint64_t occlusion_culled; /* surfaces rejected by the occlusion test       */
+    uint64_t occlusion_culled; /* occlusion tests that culled a surface         */
     double ms_schedule;        /* host: split/dice scheduling                   */
     double ms_upload;          /* host->device copies of the frame inputs       */
     double ms_dice;            /* device: dice + bust                           */
@@ -135,6 +139,11 @@ typedef struct rh_stats {
     double ms_device;          /* device: whole frame, event to event           */
     double ms_composite;       /* device: transparent hit combine, a part of
                                   ms_sample (0 for an opaque frame)             */
+    uint64_t cull_probes;      /* occlusion tests decided by the depths at their
+                                  moment (the final depths did not settle them) */
+    uint64_t cull_passes;      /* schedules computed for the frame              */
+    double ms_cull;            /* host + device: the culling schedule beyond the
+                                  first split/dice schedule (a part of ms_schedule) */
 } rh_stats;
 
 /* Work and device memory one frame of a scene needs, from the host

@@ -38,6 +38,8 @@ def lib():
         L.oracle_bilerp2.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
         L.oracle_tree_index.argtypes = [C.c_int32] * 4
         L.oracle_tree_index.restype = C.c_int32
+        L.oracle_can_cull.argtypes = [C.c_int32] * 6 + [C.c_void_p, C.c_void_p]
+        L.oracle_can_cull.restype = C.c_int32
         L.oracle_filter.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float]
         L.oracle_filter.restype = C.c_float
         L.oracle_jitter_tables.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]

@@ -780,6 +780,31 @@ struct OcclTree {
     }
 };
 
+// setupTree's geometry for a sample region of w x h pixels at (sMinX, sMinY)
+// with xS x yS samples per pixel: every sampled leaf at FLT_MAX, the padding
+// leaves at 0, depths propagated (occlusion.cpp:54-107).
+static void init_tree(OcclTree& tree, int sMinX, int sMinY, int w, int h, int xS, int yS) {
+    int numX = w * xS, numY = h * yS;
+    int depthX = (int)lceil(aq_log2((float)numX));
+    int depthY = (int)lceil(aq_log2((float)numY));
+    if (depthX < depthY) depthX = depthY - 1;
+    else if (depthY < depthX) depthY = depthX - 1;
+    tree.splitXFirst = depthX >= depthY;
+    int depth = depthX + depthY;
+    tree.numLevels = depth + 1;
+    int numLeaf = 1 << depth;
+    tree.firstLeaf = numLeaf - 1;
+    tree.depth.assign(2 * numLeaf - 1, 0.0f);
+    tree.bmin = v2((float)sMinX, (float)sMinY);
+    V2 diag = v2((float)w, (float)h);
+    V2 scale = v2(float(1 << depthX) / numX, float(1 << depthY) / numY);
+    tree.bmax = tree.bmin + v2(diag.x * scale.x, diag.y * scale.y);
+    for (int y = 0; y < numY; ++y)
+        for (int x = 0; x < numX; ++x)
+            tree.depth[OcclTree::index_for_point(tree.numLevels, tree.splitXFirst, x, y)] = FLT_MAX;
+    tree.propagate();
+}
+
 // ---------------------------------------------------------------------------
 struct Bucket {
     int col = 0, row = 0, xPos = 0, yPos = 0, xSize = 0, ySize = 0;
@@ -1468,33 +1493,14 @@ struct Hider {
 
     // CqOcclusionTree::setupTree (occlusion.cpp:54-107)
     void setup_tree() {
-        int xS = F.nx, yS = F.ny;
-        int w = sMaxX - sMinX, h = sMaxY - sMinY;
-        int numX = w * xS, numY = h * yS;
-        int depthX = (int)lceil(aq_log2((float)numX));
-        int depthY = (int)lceil(aq_log2((float)numY));
-        if (depthX < depthY) depthX = depthY - 1;
-        else if (depthY < depthX) depthY = depthX - 1;
-        tree.splitXFirst = depthX >= depthY;
-        int depth = depthX + depthY;
-        tree.numLevels = depth + 1;
-        int numLeaf = 1 << depth;
-        tree.firstLeaf = numLeaf - 1;
-        tree.depth.assign(2 * numLeaf - 1, 0.0f);
-        tree.bmin = v2((float)sMinX, (float)sMinY);
-        V2 diag = v2((float)w, (float)h);
-        V2 scale = v2(float(1 << depthX) / numX, float(1 << depthY) / numY);
-        tree.bmax = tree.bmin + v2(diag.x * scale.x, diag.y * scale.y);
+        init_tree(tree, sMinX, sMinY, sMaxX - sMinX, sMaxY - sMinY, F.nx, F.ny);
         for (int y = sMinY; y < sMaxY; ++y)
             for (int x = sMinX; x < sMaxX; ++x)
                 for (int i = 0; i < F.nsamp; ++i) {
-                    int spx = xS * x + i % xS, spy = yS * y + i / xS;
-                    int idx = OcclTree::index_for_point(tree.numLevels, tree.splitXFirst, spx - xS * sMinX,
-                                                        spy - yS * sMinY);
-                    sample_at(x, y, i).occlIndex = idx;
-                    tree.depth[idx] = FLT_MAX;
+                    int spx = F.nx * x + i % F.nx, spy = F.ny * y + i / F.nx;
+                    sample_at(x, y, i).occlIndex = OcclTree::index_for_point(tree.numLevels, tree.splitXFirst,
+                                                                             spx - F.nx * sMinX, spy - F.ny * sMinY);
                 }
-        tree.propagate();
     }
 
     void render_waiting(Bucket& b) {
@@ -1508,9 +1514,10 @@ struct Hider {
         if (cull) tree.update();
     }
 
-    // RenderSurface (bucketprocessor.cpp:942-1059)
+    // RenderSurface (bucketprocessor.cpp:942-1059); the cull test applies to
+    // surfaces whose Attribute "cull" "hidden" is 1, the default (:950).
     void render_surface(Bucket& b, Surf* s) {
-        if (cull && s->cachedBound && tree.can_cull(s->rasterBound)) {
+        if (cull && s->cachedBound && !prims[s->prim].no_cull_hidden && tree.can_cull(s->rasterBound)) {
             st.occlusion_culled++;
             repost_surface(b, s);
             return;
@@ -1900,6 +1907,20 @@ void oracle_bilerp2(const float* abcd, const float* uv, int32_t n, float* out) {
 
 int32_t oracle_tree_index(int32_t numLevels, int32_t splitXFirst, int32_t x, int32_t y) {
     return OcclTree::index_for_point(numLevels, splitXFirst != 0, x, y);
+}
+
+int32_t oracle_can_cull(int32_t sMinX, int32_t sMinY, int32_t w, int32_t h, int32_t xS, int32_t yS,
+                        const float* cellDepth, const float* bound) {
+    OcclTree t;
+    init_tree(t, sMinX, sMinY, w, h, xS, yS);
+    for (int y = 0; y < h * yS; ++y)
+        for (int x = 0; x < w * xS; ++x)
+            t.depth[OcclTree::index_for_point(t.numLevels, t.splitXFirst, x, y)] = cellDepth[(size_t)y * w * xS + x];
+    t.propagate();
+    Bound b;
+    b.mn = v3(bound[0], bound[1], bound[4]);
+    b.mx = v3(bound[2], bound[3], bound[5]);
+    return t.can_cull(b) ? 1 : 0;
 }
 
 float oracle_filter(int32_t filter, float x, float y, float xw, float yw) {

@@ -50,6 +50,12 @@ void oracle_inv_bilinear(const float* abcd /* 8 */, const float* p, int32_t n, f
 void oracle_bilerp2(const float* abcd /* 8 */, const float* uv, int32_t n, float* out);
 /* CqOcclusionTree::treeIndexForPoint of occlusion.cpp:249-281. */
 int32_t oracle_tree_index(int32_t numLevels, int32_t splitXFirst, int32_t x, int32_t y);
+/* CqOcclusionTree::canCull on the tree of a w x h pixel sample region at
+ * (sMinX, sMinY) with xS x yS samples per pixel whose leaves hold
+ * cellDepth[y * w * xS + x] (sub-sample cells); bound = min x, min y, max x,
+ * max y, min z, max z. */
+int32_t oracle_can_cull(int32_t sMinX, int32_t sMinY, int32_t w, int32_t h, int32_t xS, int32_t yS,
+                        const float* cellDepth, const float* bound);
 /* RiXxxFilter(x, y, xw, yw) of filters.cpp. */
 float oracle_filter(int32_t filter, float x, float y, float xw, float yw);
 /* The 250-pattern multijitter tables built after Reseed(545)

@@ -4,7 +4,8 @@ The reference hider cannot be built in this environment (it needs Boost,
 flex and bison, and reaches into the renderer's singletons), so these
 fixtures are regression pins of the oracle's restatement, not reference
 output: per-sample winning micropolygon ids and depths, filtered channels, 8-bit
-pixels and the sample-test count (occlusion culling off) of small scenes.
+pixels and the sample-test count of small scenes, with occlusion culling on
+(the reference's default, Attribute "cull" "hidden" 1).
 
 Usage: python tests/golden/make_golden.py
 """
@@ -30,7 +31,7 @@ def golden_cases():
 
 def main():
     for name, (opts, prims) in golden_cases().items():
-        r = loader.render(opts, prims, occlusion_cull=False)
+        r = loader.render(opts, prims, occlusion_cull=True)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), mp_id=r["mp_id"], depth=r["depth"],
                             channels=r["channels"], rgba8=r["rgba8"],
                             spl_count=np.uint64(r["stats"]["spl_count"]))

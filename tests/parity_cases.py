@@ -182,6 +182,55 @@ def motion_one_sample():
     return opts, prims
 
 
+def coplanar_ties():
+    # Exact depth ties that occlusion culling reorders (RepostSurface): the
+    # culled quad is diced after the other and loses the ties.
+    return scenes.coplanar_scene()
+
+
+def decal_ties():
+    # Many camera-facing quads on three depth planes (equal depths wherever
+    # two overlap) behind near quads that cover whole buckets.
+    return scenes.decal_scene(seed=1)
+
+
+def decal_ties_transparent():
+    # The same with transparent quads: equal-depth transparent hits, whose
+    # order into the depth sort follows the dicing order.
+    return scenes.decal_scene(seed=2, opacity=(0.3, 0.7), n=50)
+
+
+def decal_ties_motion():
+    # Ties and culling in a frame with moving surfaces (the motion sampler's
+    # probes): the decal quads as two-key deforming surfaces, half of them
+    # moved in the image plane.
+    opts, prims = scenes.decal_scene(seed=3, n=45, samples=(2, 2))
+    opts.shutter_open, opts.shutter_close = 0.0, 1.0
+    rng = np.random.default_rng(3)
+    for i, p in enumerate(prims):
+        p.nkeys = 2
+        p.key_time[:] = [0.0, 1.0]
+        d = rng.normal(0.0, 0.3, 2) if i % 2 else np.zeros(2)
+        for k in range(4):
+            p.P1[k][:] = [float(np.float32(p.P[k][0] + d[0])), float(np.float32(p.P[k][1] + d[1])), p.P[k][2]]
+    return opts, prims
+
+
+def patches_cull_flag_mixed():
+    # Attribute "cull" "hidden" [0] on every other surface: only the others
+    # may be occlusion culled.
+    opts, prims = scenes.patches_scene(seed=47, n=400, res=(240, 136))
+    for p in prims[::2]:
+        p.no_cull_hidden = 1
+    return opts, prims
+
+
+def eye_plane_split():
+    # Ground planes crossing the eye plane (eye splits, pieces behind the
+    # camera culled) under ordinary patches.
+    return scenes.eye_plane_scene(seed=5)
+
+
 CASES = {
     "spheres_config": spheres_config,
     "patches_small": patches_small,
@@ -204,4 +253,10 @@ CASES = {
     "motion_spheres": motion_spheres,
     "motion_closed_shutter": motion_closed_shutter,
     "motion_one_sample": motion_one_sample,
+    "coplanar_ties": coplanar_ties,
+    "decal_ties": decal_ties,
+    "decal_ties_transparent": decal_ties_transparent,
+    "decal_ties_motion": decal_ties_motion,
+    "patches_cull_flag_mixed": patches_cull_flag_mixed,
+    "eye_plane_split": eye_plane_split,
 }

@@ -1,0 +1,125 @@
+"""The host's occlusion-culling schedule against the oracle, on the CPU.
+
+The product computes the schedule aqsis's occlusion culling gives from the
+frame's final opaque depths (rh::simulate with cull answers; the device
+supplies the depths and the per-test maxima).  Here a C++ harness compiled
+from the host front end (aqsis_amd/csrc/front.cpp) takes the final depths
+from the oracle's own render instead, answers every (surface, bucket) test
+with the largest depth over its leaf cells, and runs the culling schedule.
+For scenes whose tests all lie clear of the margin (no probes needed) the
+schedule must dice exactly the grids and micropolygons the oracle's culling
+render dices, post as many surfaces, and cull as many times.  Tests that the
+final depths leave open (probes, settled on the device by rendering the
+bucket up to the test) are few: the harness takes their tentative
+decisions, which for these scenes are the oracle's.  (Buckets whose heaps
+hold surfaces spanning the eye plane have no depth order, so all their culls
+are probed; the GPU parity case eye_plane_split covers them.)"""
+import ctypes as C
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from aqsis_amd import scenes
+from aqsis_amd.scenes import prim_array
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "aqsis_amd", "csrc")
+
+HARNESS = r'''
+#include "front.h"
+#include <cstdio>
+#include <vector>
+// argv: options file, prims file (n records), depths file (ext samples), n
+int main(int argc, char** argv) {
+    rh_options o;
+    FILE* f = std::fopen(argv[1], "rb");
+    if (std::fread(&o, sizeof o, 1, f) != 1) return 2;
+    std::fclose(f);
+    const int n = std::atoi(argv[4]);
+    std::vector<rh_prim> p(n);
+    f = std::fopen(argv[2], "rb");
+    if (n && std::fread(p.data(), sizeof(rh_prim), n, f) != (size_t)n) return 2;
+    std::fclose(f);
+    rh::Shape s;
+    std::string err;
+    if (!rh::make_shape(o, s, err)) return 3;
+    std::vector<float> depth((size_t)s.extW * s.extH * s.ns);
+    f = std::fopen(argv[3], "rb");
+    if (std::fread(depth.data(), 4, depth.size(), f) != depth.size()) return 2;
+    std::fclose(f);
+    rh::ForestPtr fo;
+    if (!rh::build_forest(s, p.data(), n, fo, err, 1)) return 4;
+    rh::CullPairs pairs;
+    rh::cull_pairs(s, *fo, pairs);
+    std::vector<CullTreeDev> trees;
+    rh::cull_trees(s, trees);
+    std::vector<float> pm(pairs.start.back(), -1.0f);
+    for (size_t g = 0; g + 1 < pairs.start.size(); ++g) {
+        const int* R = &pairs.range[4 * g];
+        const float* b = &pairs.bound[4 * g];
+        for (uint64_t q = pairs.start[g]; q < pairs.start[g + 1]; ++q) {
+            const int off = (int)(q - pairs.start[g]), nbx = R[2] - R[0] + 1;
+            const int col = R[0] + off % nbx, row = R[1] + off / nbx;
+            const CullTreeDev& t = trees[(row - s.by0) * s.nbr + (col - s.bx0)];
+            int cx0 = 0, cx1 = -1, cy0 = 0, cy1 = -1;
+            const int kind = t.levels ? cull_cells(t, b[0], b[1], b[2], b[3], cx0, cx1, cy0, cy1) : 0;
+            float m = kind == 2 ? 0.0f : -1.0f;
+            for (int cy = cy0; kind && cy <= cy1; ++cy)
+                for (int cx = cx0; cx <= cx1; ++cx) {
+                    const int px = t.sx0 + cx / s.nx, py = t.sy0 + cy / s.ny;
+                    const int si = (cy % s.ny) * s.nx + cx % s.nx;
+                    const float d = depth[((size_t)(py - s.extY0) * s.extW + (px - s.extX0)) * s.ns + si];
+                    m = m < d ? d : m;
+                }
+            pm[q] = m;
+        }
+    }
+    rh::CullInput in;
+    in.pairMax = pm.data();
+    rh::Schedule S;
+    if (!rh::simulate(s, p.data(), n, *fo, &in, &pairs, S, err)) { std::printf("%s\n", err.c_str()); return 5; }
+    std::printf("%zu %llu %llu %llu %zu\n", S.grids.size(), (unsigned long long)S.micropolygons,
+                (unsigned long long)S.surfaces, (unsigned long long)S.culled, S.probes.size());
+    return 0;
+}
+'''
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not installed")
+    d = tmp_path_factory.mktemp("cullsched")
+    src = d / "h.cpp"
+    src.write_text(HARNESS)
+    exe = d / "h"
+    subprocess.run([gxx, "-O2", "-std=c++17", "-ffp-contract=off", "-I", CSRC, str(src),
+                    os.path.join(CSRC, "front.cpp"), "-o", str(exe), "-lpthread"], check=True)
+    return str(exe), d
+
+
+@pytest.mark.parametrize("make", [
+    lambda: scenes.spheres_scene(seed=1),
+    lambda: scenes.patches_scene(seed=5, n=400, res=(320, 180)),
+    lambda: scenes.patches_scene(seed=9, n=300, res=(200, 120), samples=(3, 3), filt=("catmull-rom", 3.0, 3.0)),
+], ids=["spheres", "patches", "patches_3x3"])
+def test_cull_schedule_matches_oracle(oracle, harness, make):
+    exe, d = harness
+    opts, prims = make()
+    ref = oracle.render(opts, prims, occlusion_cull=True)
+    st = ref["stats"]
+    # opaque scenes: the front-most hit's depth is the sample's occluding depth
+    arr = prim_array(prims)
+    (d / "o.bin").write_bytes(bytes(opts))
+    (d / "p.bin").write_bytes(bytes(arr))
+    (d / "z.bin").write_bytes(ref["depth"].astype(np.float32).tobytes())
+    out = subprocess.run([exe, str(d / "o.bin"), str(d / "p.bin"), str(d / "z.bin"), str(len(prims))],
+                         capture_output=True, text=True, check=True).stdout.split()
+    grids, mps, surfaces, culled, probes = (int(v) for v in out)
+    assert probes <= max(2, culled // 20)
+    assert culled == st["occlusion_culled"] > 0
+    assert (grids, mps, surfaces) == (st["grids"], st["micropolygons"], st["surfaces"])

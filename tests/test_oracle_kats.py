@@ -320,10 +320,11 @@ def test_exposure_and_quantise_closed_forms(oracle):
 
 
 # --------------------------------------------------------------------------
-# Occlusion culling only skips work: identical samples and pixels.
+# Occlusion culling defers surfaces to later buckets (RepostSurface), which
+# changes the dicing order.  Without equal depths that changes no hit...
 @pytest.mark.parametrize("make", [lambda: scenes.spheres_scene(seed=1),
                                   lambda: scenes.patches_scene(seed=5, n=400, res=(320, 180))])
-def test_occlusion_culling_changes_no_result(oracle, make):
+def test_occlusion_culling_without_ties_changes_no_hit(oracle, make):
     opts, prims = make()
     a = oracle.render(opts, prims, occlusion_cull=True)
     b = oracle.render(opts, prims, occlusion_cull=False)
@@ -333,6 +334,40 @@ def test_occlusion_culling_changes_no_result(oracle, make):
     assert np.array_equal(a["rgba8"], b["rgba8"])
     assert a["stats"]["occlusion_culled"] > 0
     assert a["stats"]["spl_count"] < b["stats"]["spl_count"]
+
+
+def test_occlusion_culling_reorders_equal_depth_hits(oracle):
+    """...but at exact depth ties it decides the winner.  A near quad covers
+    bucket 0; two coplanar quads at z = 10 overlap in bucket 1, the first
+    starting in bucket 0.  Culled there, the first is reposted to bucket 1
+    (imagebuffer.cpp:449-483) and diced after the second, which therefore wins
+    the ties (first come wins, bucketprocessor.cpp:1475); without culling the
+    first is diced in bucket 0 and wins them."""
+    opts, prims = scenes.coplanar_scene()
+    on = oracle.render(opts, prims, occlusion_cull=True)
+    off = oracle.render(opts, prims, occlusion_cull=False)
+    assert on["stats"]["occlusion_culled"] == 1
+    assert np.array_equal(on["depth"].view(np.uint32), off["depth"].view(np.uint32))
+    W, H, ew, eh, spp = oracle.frame_size(opts)
+    c_on = on["rgbaz"].reshape(eh, ew, spp, 7)[..., :3]
+    c_off = off["rgbaz"].reshape(eh, ew, spp, 7)[..., :3]
+    green, blue = np.float32([0.1, 0.9, 0.1]), np.float32([0.1, 0.1, 0.9])
+    # samples in the overlap [20, 28] x [4, 12], away from its edges: where
+    # the two depths round alike, the first diced quad wins (elsewhere the
+    # nearer of the two rounded depths does)
+    ov_on, ov_off = c_on[5:11, 21:27].reshape(-1, 3), c_off[5:11, 21:27].reshape(-1, 3)
+    blue_on = np.all(np.abs(ov_on - blue) < 1e-6, axis=1).mean()
+    green_off = np.all(np.abs(ov_off - green) < 1e-6, axis=1).mean()
+    assert blue_on > 0.6 and green_off > 0.6, (blue_on, green_off)
+    assert not np.array_equal(on["rgba8"], off["rgba8"])
+    # the culling flag of the surfaces: Attribute "cull" "hidden" [0] on all
+    # of them renders the culling-free frame
+    for p in prims:
+        p.no_cull_hidden = 1
+    none = oracle.render(opts, prims, occlusion_cull=True)
+    assert none["stats"]["occlusion_culled"] == 0
+    assert np.array_equal(none["rgbaz"].view(np.uint32), off["rgbaz"].view(np.uint32))
+    assert np.array_equal(none["mp_id"], off["mp_id"])
 
 
 # --- Motion blur (deformation keys) -----------------------------------------
@@ -437,3 +472,102 @@ def test_moving_quad_hits_match_brute_force(oracle):
     assert checked > 40000 and mism == 0
     dz = r["depth"][r["mp_id"] != 0xffffffff]
     assert dz.size > 500 and np.allclose(dz, z, rtol=1e-5)
+
+
+# --------------------------------------------------------------------------
+# Filters whose forms need more than a polynomial: closed forms written from
+# the definitions (Mitchell & Netravali 1988 with B = C = 1/3 on t = 2x/width;
+# the half-cosine-windowed Bessel J1 of RiBesselFilter; the cosine-windowed
+# sinc of RiSincFilter, libs/core/filters.cpp:36-58, 117-122, 233-342),
+# evaluated in double with scipy's independent Bessel function.
+def _mitchell_1d(t):
+    B = C = 1.0 / 3.0
+    t = abs(t)
+    if t < 1.0:
+        return ((12 - 9 * B - 6 * C) * t ** 3 + (-18 + 12 * B + 6 * C) * t ** 2 + (6 - 2 * B)) / 6.0
+    if t < 2.0:
+        return ((-B - 6 * C) * t ** 3 + (6 * B + 30 * C) * t ** 2 + (-12 * B - 48 * C) * t + (8 * B + 24 * C)) / 6.0
+    return 0.0
+
+
+def test_mitchell_closed_form(oracle):
+    f = oracle.lib().oracle_filter
+    g = FILTERS
+    # the kernel's own values at |2x/w| = 0, 0.5, 1, 1.5, 2: 8/9, 77/144, 1/18, -5/144, 0
+    assert _mitchell_1d(0.0) == pytest.approx(8.0 / 9.0, abs=1e-15)
+    assert _mitchell_1d(0.5) == pytest.approx(77.0 / 144.0, abs=1e-15)
+    assert _mitchell_1d(1.0) == pytest.approx(1.0 / 18.0, abs=1e-15)
+    assert _mitchell_1d(1.5) == pytest.approx(-5.0 / 144.0, abs=1e-15)
+    assert _mitchell_1d(2.0) == pytest.approx(0.0, abs=1e-15)
+    for w in (3.0, 4.0):
+        for t in (0.0, 0.5, 1.0, 1.5):
+            x = t * w / 2.0
+            got = f(g["mitchell"], x, 0.0, w, w)
+            assert got == pytest.approx(_mitchell_1d(t) * _mitchell_1d(0.0), rel=2e-6, abs=1e-7), (w, t)
+        for x, y in [(0.3, -0.7), (1.1, 0.4), (-1.4, -1.2)]:
+            want = _mitchell_1d(2 * x / w) * _mitchell_1d(2 * y / w)
+            assert f(g["mitchell"], x, y, w, w) == pytest.approx(want, rel=2e-6, abs=1e-7)
+
+
+def test_bessel_closed_form(oracle):
+    from scipy.special import j1
+    f = oracle.lib().oracle_filter
+    g = FILTERS
+    assert f(g["bessel"], 0.0, 0.0, 3.0, 3.0) == np.float32(math.pi)
+    for (x, y, xw, yw) in [(0.3, 0.2, 3.0, 3.0), (1.0, 0.0, 3.0, 3.0), (0.7, -0.9, 3.0, 2.5),
+                           (-1.2, 0.5, 4.0, 4.0), (0.05, 0.0, 2.0, 2.0)]:
+        xx, yy = x * x, y * y
+        w = xx / (xw / 2) ** 2 + yy / (yw / 2) ** 2
+        d = math.hypot(x, y)
+        want = math.cos(0.5 * math.pi * math.sqrt(w)) * 2.0 * j1(math.pi * d) / d if w < 1 else 0.0
+        assert f(g["bessel"], x, y, xw, yw) == pytest.approx(want, rel=2e-6, abs=1e-7), (x, y)
+    # outside the ellipse
+    assert f(g["bessel"], 1.6, 0.0, 3.0, 3.0) == 0.0
+    # the window's first zero: 2 J1(pi d)/d vanishes at d = 1.2197 (j1's first zero / pi)
+    assert abs(f(g["bessel"], 3.8317059702 / math.pi, 0.0, 3.0, 3.0)) < 1e-5
+
+
+def test_sinc_closed_form(oracle):
+    f = oracle.lib().oracle_filter
+    g = FILTERS
+
+    def s1(x, w):
+        if x == 0:
+            return 1.0
+        return math.cos(0.5 * math.pi * x / w) * math.sin(math.pi * x) / (math.pi * x)
+
+    for (x, y, xw, yw) in [(0.5, 0.0, 3.0, 3.0), (1.0, 0.0, 3.0, 3.0), (0.3, -0.8, 3.0, 3.0),
+                           (1.4, 1.1, 4.0, 3.0), (-0.25, 0.75, 2.0, 2.0)]:
+        want = s1(x, xw) * s1(y, yw)
+        assert f(g["sinc"], x, y, xw, yw) == pytest.approx(want, rel=2e-6, abs=1e-7), (x, y)
+    # integer x: sin(pi x) = 0
+    assert abs(f(g["sinc"], 1.0, 0.3, 3.0, 3.0)) < 1e-6
+    # symmetric
+    assert f(g["sinc"], -0.6, 0.2, 3.0, 3.0) == f(g["sinc"], 0.6, -0.2, 3.0, 3.0)
+
+
+def test_gxx_evaluates_constructor_arguments_right_to_left(tmp_path):
+    """The multijitter set-up passes two RandomFloat() calls as the arguments
+    of one CqVector2D constructor (libs/core/multijitter.cpp:139, 166-168),
+    whose evaluation order C++ leaves unspecified.  The oracle and the host
+    plan draw the y argument first, as g++ (the reference's compiler here)
+    does at -O2 for x86-64; this checks that fact on the installed g++."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not installed")
+    src = tmp_path / "order.cpp"
+    src.write_text(
+        "#include <cstdio>\n"
+        "static int counter = 0;\n"
+        "static float next() { return (float)(++counter); }\n"
+        "struct V { float x, y; V(float a, float b) : x(a), y(b) {} };\n"
+        "int main() { V v(next(), next()); V w((1 + next()) * 0.5f, (2 + next()) * 0.5f);\n"
+        "  std::printf(\"%g %g %g %g\\n\", v.x, v.y, w.x, w.y); return 0; }\n")
+    exe = tmp_path / "order"
+    subprocess.run([gxx, "-O2", "-std=c++17", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    vx, vy, wx, wy = (float(v) for v in out)
+    assert (vx, vy) == (2.0, 1.0)             # y argument drawn first
+    assert (wx, wy) == ((1 + 4) * 0.5, (2 + 3) * 0.5)

@@ -3,9 +3,11 @@
 The bar: per-sample winning micropolygon id, depth and combined colour
 bit-exact; filtered channels within 1e-5 absolute (they come out
 bit-identical); 8-bit output identical; the hider counters SPL_count,
-SPL_bound_hits and SPL_hits equal to the oracle's with occlusion culling
-off (culling changes which occluded surfaces are diced, never a result;
-test_oracle_kats.py::test_occlusion_culling_changes_no_result checks that).
+SPL_bound_hits and SPL_hits and the number of occlusion-culled surfaces
+equal to the oracle's.  Both run the reference's default, occlusion culling
+on for every surface whose Attribute "cull" "hidden" is 1: culling defers
+surfaces to later buckets, which decides the winners at equal depths
+(coplanar and decal cases).
 """
 import numpy as np
 import pytest
@@ -33,7 +35,7 @@ def test_parity(case, oracle):
     opts, prims = CASES[case]()
     print(f"case {case}: {opts.xres}x{opts.yres} spp {opts.xsamples}x{opts.ysamples} prims {len(prims)}",
           flush=True)
-    ref = oracle.render(opts, prims, occlusion_cull=False)
+    ref = oracle.render(opts, prims, occlusion_cull=True)
     got = _render(opts, prims)
     assert np.array_equal(got["mp_id"], ref["mp_id"]), \
         f"winner id mismatch at {np.count_nonzero(got['mp_id'] != ref['mp_id'])} samples"
@@ -53,7 +55,8 @@ def test_parity(case, oracle):
     diff = np.abs(got["channels"][fin].astype(np.float64) - ref["channels"][fin].astype(np.float64))
     assert diff.size == 0 or diff.max() <= PIXEL_TOL
     assert np.array_equal(got["rgba8"], ref["rgba8"])
-    for k in ("spl_count", "spl_bound_hits", "spl_hits", "inside", "inside_nonlinear", "grids", "micropolygons"):
+    for k in ("spl_count", "spl_bound_hits", "spl_hits", "inside", "inside_nonlinear", "grids", "micropolygons",
+              "occlusion_culled"):
         assert got["stats"][k] == ref["stats"][k], k
     hit = np.count_nonzero(ref["mp_id"] != 0xFFFFFFFF)
     assert hit > 0.05 * ref["mp_id"].size  # the scene covers the frame
@@ -89,7 +92,7 @@ def test_contexts_of_one_shape_render_alike(oracle):
             outs.append((h.channels(), h.rgba8(), h.samples()[0]))
     for x, y in zip(*outs):
         assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
-    ref = oracle.render(opts, prims, occlusion_cull=False)
+    ref = oracle.render(opts, prims, occlusion_cull=True)
     assert np.array_equal(outs[1][2], ref["mp_id"])
 
 
@@ -107,7 +110,7 @@ def test_frames_without_visible_surfaces(kind, oracle):
         for p in prims:
             for i in range(16):
                 p.P[i][0] += 1000.0           # far to the side of the frustum
-    ref = oracle.render(opts, prims, occlusion_cull=False)
+    ref = oracle.render(opts, prims, occlusion_cull=True)
     with Hider(opts) as h:
         h.post_surfaces(prims)
         h.render()

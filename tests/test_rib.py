@@ -128,18 +128,38 @@ def test_refused(text, msg):
 
 def test_display_requests():
     """Display requests are collected and counted into num_displays (each
-    draws one dither value per pixel from the shared stream)."""
+    draws one dither value per pixel from the shared stream); a '+' name adds
+    a display and is recorded without the '+'."""
     text = ('Display "a.tif" "file" "rgba" "compression" "none"\n'
             'Display "+b" "framebuffer" "rgb"\n'
             'Format 32 24 1\nWorldBegin\nSphere 1 -1 1 360\nWorldEnd\n')
     displays = []
     opts, _ = rib.parse(text, displays)
     assert opts.num_displays == 2
-    assert [(d["name"], d["type"], d["mode"]) for d in displays] == [("a.tif", "file", "rgba"), ("+b", "framebuffer", "rgb")]
+    assert [(d["name"], d["type"], d["mode"]) for d in displays] == [("a.tif", "file", "rgba"), ("b", "framebuffer", "rgb")]
     assert displays[0]["params"] == {"compression": ["none"]}
     assert rib.parse(text.split("\n", 2)[2])[0].num_displays == 1  # none given: the default one
     with pytest.raises(rib.RibError, match="line 1: Display needs"):
         rib.parse('Display "a.tif" "file"\n')
+
+
+def test_display_without_plus_replaces_earlier_requests():
+    """RiDisplay with a name that has no leading '+' clears the display
+    requests made before it (libs/core/api/ri.cpp:995-1004): two plain
+    Display lines leave one display, the second, and one dither draw per
+    pixel."""
+    text = ('Display "a.tif" "file" "rgba"\n'
+            'Display "b.tif" "tiff" "rgb"\n'
+            'Format 32 24 1\nWorldBegin\nSphere 1 -1 1 360\nWorldEnd\n')
+    displays = []
+    opts, _ = rib.parse(text, displays)
+    assert opts.num_displays == 1
+    assert [(d["name"], d["type"], d["mode"], d["line"]) for d in displays] == [("b.tif", "tiff", "rgb", 2)]
+    text3 = 'Display "a" "file" "rgba"\nDisplay "+b" "file" "rgb"\nDisplay "c" "framebuffer" "rgb"\nDisplay "+d" "file" "rgb"\n'
+    displays = []
+    opts, _ = rib.parse(text3, displays)
+    assert opts.num_displays == 2
+    assert [d["name"] for d in displays] == ["c", "d"]
 
 
 @pytest.mark.parametrize("first,second,msg", [
