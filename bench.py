@@ -99,18 +99,47 @@ def profiled_traffic(kernel):
     return None
 
 
-def cpu_baseline(opts, prims):
-    """The oracle on the host cores: (i) one thread on the CropWindow
-    [0,0.5]x[0,0.5] quarter of the frame, as the reference's single-threaded
-    bucket loop runs; (ii) all host cores, the whole frame cut into one
-    horizontal crop band per thread (ctypes releases the GIL, the oracle has
-    no shared state), timed wall to wall.  Occlusion culling on as in the
-    reference.  The reported value is (ii); (i) is kept beside it."""
+def host_cpus():
+    """The host cores this process may use: the affinity set, narrowed by a
+    cgroup CPU quota when one is set; with the count, nproc and the CPU
+    model name (/proc/cpuinfo, the lscpu "Model name")."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    threads = aff if quota is None else max(1, min(aff, int(quota)))
+    return {"threads": threads, "nproc": os.cpu_count(), "affinity": aff, "cgroup_cpu_quota": quota,
+            "model": model}
+
+
+def cpu_baseline(opts, prims, quarter_single=True):
+    """The oracle (a C++ restatement of the reference hider, occlusion
+    culling on as in the reference) on the host cores: (i) all the cores
+    this process may use, the whole frame cut into one horizontal CropWindow
+    band per thread (ctypes releases the GIL, the oracle has no shared
+    state), timed wall to wall -- the reported value; (ii) one thread, as the
+    reference's single-threaded bucket loop runs, on the CropWindow
+    [0,0.5]x[0,0.5] quarter of the frame (or the whole frame when
+    quarter_single is False)."""
     from oracle import loader
     import copy
     import threading
+    cpus = host_cpus()
     o = copy.copy(opts)
-    o.crop[:] = [0.0, 0.5, 0.0, 0.5]
+    if quarter_single:
+        o.crop[:] = [0.0, 0.5, 0.0, 0.5]
     loader.render(o, prims, occlusion_cull=True, samples=False)   # warm-up / page-in
     t0 = time.perf_counter()
     r = loader.render(o, prims, occlusion_cull=True, samples=False)
@@ -118,8 +147,7 @@ def cpu_baseline(opts, prims):
     st = r["stats"]
     one_frame_s = st["ms_device"] / 1e3
     one_rate = st["spl_count"] / one_frame_s
-
-    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    threads = cpus["threads"]
     bands = []
     for i in range(threads):
         ob = copy.copy(opts)
@@ -138,20 +166,78 @@ def cpu_baseline(opts, prims):
         t.join()
     wall_all = time.perf_counter() - t0
     tests_all = sum(x["spl_count"] for x in results)
+    part = "the CropWindow [0,0.5]x[0,0.5] quarter" if quarter_single else "the whole frame"
     return {
         "value": tests_all / wall_all,
         "unit": "sample-tests/s",
         "cores": threads,
         "kind": "port",
         "sample": f"oracle (C++ restatement of the reference hider, occlusion culling on): the whole frame as "
-                  f"{threads} horizontal CropWindow bands, one thread each, wall time; single-thread figures "
-                  f"on the CropWindow [0,0.5]x[0,0.5] quarter beside it",
+                  f"{threads} horizontal CropWindow bands, one thread each, wall time; one thread on {part} beside it",
+        "host": cpus,
         "frames_per_s": 1.0 / wall_all,
         "wall_s": wall_all,
-        "single_thread": {"value": one_rate, "frame_ms_quarter": st["ms_device"],
-                          "frames_per_s_extrapolated": 0.25 / one_frame_s, "spl_count_quarter": st["spl_count"],
-                          "wall_s": wall1},
+        "single_thread": {"value": one_rate, "frame_ms": st["ms_device"],
+                          "frames_per_s_extrapolated": (0.25 if quarter_single else 1.0) / one_frame_s,
+                          "spl_count": st["spl_count"], "wall_s": wall1, "part": part},
     }
+
+
+def e2e_frames(h, prims, steps):
+    """A new scene per frame on a resident context: surfaces posted, the
+    split/dice schedule (occlusion culling included), upload and the device
+    frame, timed wall to wall.  Returns the median and the parts of the last."""
+    times, parts = [], None
+    for _ in range(steps):
+        h.clear_surfaces()
+        t0 = time.perf_counter()
+        h.post_surfaces(prims)
+        h.render()
+        times.append((time.perf_counter() - t0) * 1e3)
+        st = h.stats()
+        parts = {"schedule_ms": st["ms_schedule"], "cull_ms": st["ms_cull"], "upload_ms": st["ms_upload"],
+                 "device_ms": st["ms_device"], "cull_passes": st["cull_passes"], "cull_probes": st["cull_probes"]}
+    times.sort()
+    med = times[len(times) // 2]
+    return {"ms_per_frame": med, "frames_per_s": 1e3 / med, "steps": steps, "last": parts}
+
+
+def config1_target(args):
+    """The north-star target on the CPU-runnable configuration (256x256, 64
+    spheres, 4x4, gaussian 2x2): the device hider against the oracle on all
+    host cores, end to end (new scene per frame) and on resident inputs, with
+    the frame's parity checked in the same run."""
+    import numpy as np
+    from aqsis_amd import scenes
+    from aqsis_amd.hider import Hider
+    from oracle import loader
+    opts, prims = scenes.spheres_scene(seed=1)
+    with Hider(opts) as h:
+        h.post_surfaces(prims)
+        h.render()
+        ids, depth, _ = h.samples()
+        ch, rgba = h.channels(), h.rgba8()
+        e2e = e2e_frames(h, prims, max(3, args.e2e_steps))
+        for _ in range(3):
+            h.render_device()
+        t0 = time.perf_counter()
+        n = 20
+        for _ in range(n):
+            h.render_device()
+        dev_fps = n / (time.perf_counter() - t0)
+    ref = loader.render(opts, prims, occlusion_cull=True)
+    exact = bool(np.array_equal(ids, ref["mp_id"]) and np.array_equal(depth.view(np.uint32), ref["depth"].view(np.uint32)))
+    err = float(np.max(np.abs(ch.astype(np.float64) - ref["channels"])))
+    cb = cpu_baseline(opts, prims, quarter_single=False)
+    return {"config": "256x256, 16x16 buckets, PixelSamples 4x4, gaussian 2x2, 64 spheres (seed 1)",
+            "gpu_frames_per_s_e2e": e2e["frames_per_s"], "gpu_frames_per_s_device": dev_fps,
+            "cpu_frames_per_s_all_cores": cb["frames_per_s"], "cpu_cores": cb["cores"],
+            "cpu_frames_per_s_1thread": 1.0 / cb["single_thread"]["wall_s"],
+            "ratio_e2e_vs_cpu_all_cores": e2e["frames_per_s"] / cb["frames_per_s"],
+            "ratio_device_vs_cpu_all_cores": dev_fps / cb["frames_per_s"],
+            "ratio_e2e_vs_cpu_1thread": e2e["frames_per_s"] * cb["single_thread"]["wall_s"],
+            "visibility_bit_exact": exact, "max_pixel_error": err,
+            "rgba8_identical": bool(np.array_equal(rgba, ref["rgba8"])), "e2e_parts": e2e["last"]}
 
 
 def serving(args):
@@ -271,6 +357,8 @@ def main():
                     help="serving: Poisson arrival rate (1/s); 20/s keeps the GPU about 90%% busy")
     ap.add_argument("--slots", type=int, default=4, help="serving: jobs in flight")
     ap.add_argument("--budget-gb", type=int, default=160, help="serving: device memory budget")
+    ap.add_argument("--e2e-steps", type=int, default=5, help="new-scene frames timed end to end")
+    ap.add_argument("--no-targets", action="store_true", help="skip the CPU-runnable configuration's target line")
     ap.add_argument("--copy-kernel", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -356,6 +444,8 @@ def main():
         "data": "synthetic",
         "config": desc,
         "frames_per_s": fps,
+        "frames_per_s_note": "device frames on resident inputs (schedule and upload outside); "
+                             "frames_per_s_e2e has them in",
         "frame_ms_p50": dev_sorted[len(dev_sorted) // 2],
         "frame_ms_p99": dev_sorted[min(len(dev_sorted) - 1, int(0.99 * len(dev_sorted)))],
         "stage_ms": {"dice_bust": sum(dice_ms) / len(dice_ms), "bin": sum(bin_ms) / len(bin_ms),
@@ -369,7 +459,8 @@ def main():
         "e2e_ms_per_frame": st0["ms_schedule"] + st0["ms_upload"] + ms_per_step,
         "first_frame_ms": first_ms,
         "counts": {k: st[k] for k in ("spl_count", "spl_bound_hits", "spl_hits", "inside", "inside_nonlinear",
-                                      "grids", "micropolygons", "mp_bucket_refs")},
+                                      "grids", "micropolygons", "mp_bucket_refs", "occlusion_culled")},
+        "counts_note": "occlusion culling on, as the reference's default: SPL_count in the reference's units",
         "roofline": {
             "bound": "valu",
             "kernel": "rh_sample_mb" if motion else "rh_sample",
@@ -397,13 +488,29 @@ def main():
         out["plain_form"] = {"frames_per_s": args.plain_steps / plain_wall,
                              "ms_per_frame": plain_wall * 1e3 / args.plain_steps}
         out["speedup_vs_plain_form_frames"] = fps / out["plain_form"]["frames_per_s"]
+    if args.e2e_steps > 0:
+        h.set_tuning(mode=["grid", "plain", "batched"][tuning["mode"]], lds_records=tuning["lds_records"],
+                     blocks_per_wave=tuning["blocks_per_wave"], split_cost=tuning["split_cost"])
+        e2e = e2e_frames(h, prims, args.e2e_steps)
+        out["frames_per_s_e2e"] = e2e["frames_per_s"]
+        out["e2e"] = e2e
     if not args.no_cpu:
         cb = cpu_baseline(opts, prims)
         out["cpu_baseline"] = cb
-        out["speedup_vs_cpu_sample_tests"] = tests_per_s / cb["value"]
-        out["speedup_vs_cpu_frames"] = fps / cb["frames_per_s"]
-        out["speedup_vs_cpu_1thread_sample_tests"] = tests_per_s / cb["single_thread"]["value"]
+        sp = {"sample_tests": tests_per_s / cb["value"],
+              "frames_device": fps / cb["frames_per_s"],
+              "sample_tests_vs_1thread": tests_per_s / cb["single_thread"]["value"]}
+        if "frames_per_s_e2e" in out:
+            sp["frames_e2e"] = out["frames_per_s_e2e"] / cb["frames_per_s"]
+        out["speedup_vs_cpu"] = sp
+        out["speedup_note"] = ("both sides count sample tests with occlusion culling on; frames_e2e puts the "
+                               "host schedule and upload into the device side")
     h.close()
+    if not args.no_targets and not args.no_cpu:
+        t = {"config1": config1_target(args)}
+        if "speedup_vs_plain_form_frames" in out:
+            t["batched_vs_plain_frames"] = out["speedup_vs_plain_form_frames"]
+        out["targets"] = t
     print(json.dumps(out))
 
 
