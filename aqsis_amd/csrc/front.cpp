@@ -352,10 +352,17 @@ struct Node {
     float radius = 1, phiMin = 0, phiMax = 0, thetaMin = 0, thetaMax = 360;
     f3 cp[16];
     // A deforming surface (CqDeformingSurface, geometry/surface.h:617-720):
-    // key 1's control points (spheres: key 1's matrix, Ctx::objToCam1).
+    // key 1's control points are in its tree's cp1 (spheres: key 1's matrix,
+    // Ctx::objToCam1).
     bool motion = false;
-    f3 cp1[16];
     float subMin = FLT_MAX;     // least cached minimum depth in the subtree
+};
+
+// One primitive's split tree: nodes, parents before children, and for a
+// deforming surface key 1's control points, 16 per node.
+struct Tree {
+    std::vector<Node> n;
+    std::vector<f3> cp1;
 };
 
 // sinCosGrid (quadrics.cpp:1493-1518)
@@ -574,7 +581,7 @@ void split_bilinear(const f3* P, f3* A, f3* B, bool u) {
 }
 
 // CullSurface + bucket range (imagebuffer.cpp:268-345, 353-446)
-void post_geometry(Node& n, const Ctx& c) {
+void post_geometry(Node& n, const f3* cp1, const Ctx& c) {
     const Shape& s = *c.s;
     const rh_options& o = s.o;
     box3 B = (n.type == RH_PRIM_SPHERE) ? sphere_camera_bound(n, c.objToCam[n.prim])
@@ -583,7 +590,7 @@ void post_geometry(Node& n, const Ctx& c) {
         // CqDeformingSurface::Bound (surface.h:629-640): the keys' bounds
         // encapsulated into an empty box
         Node k1 = n;
-        for (int i = 0; i < 16; ++i) k1.cp[i] = n.cp1[i];
+        for (int i = 0; i < 16; ++i) k1.cp[i] = cp1[i];
         box3 B1 = (n.type == RH_PRIM_SPHERE) ? sphere_camera_bound(n, c.objToCam1[n.prim])
                                              : patch_camera_bound(k1, n.type == RH_PRIM_BICUBIC ? 16 : 4);
         box3 u;
@@ -619,11 +626,15 @@ void post_geometry(Node& n, const Ctx& c) {
 
 // Expand the whole split tree of one primitive (everything that does not
 // depend on the bucket being processed).
-void build_tree(const Ctx& c, int prim, std::vector<Node>& t) {
+void build_tree(const Ctx& c, int prim, Tree& tr) {
     const rh_prim& p = c.prims[prim];
+    std::vector<Node>& t = tr.n;
     Node root;
     root.type = p.type;
     root.prim = prim;
+    root.motion = p.nkeys == 2;
+    t.clear();
+    tr.cp1.clear();
     if (p.type == RH_PRIM_SPHERE) {
         root.radius = p.sphere[0];
         root.thetaMin = 0.0f;
@@ -633,14 +644,16 @@ void build_tree(const Ctx& c, int prim, std::vector<Node>& t) {
     } else {
         int k = p.type == RH_PRIM_BICUBIC ? 16 : 4;
         for (int i = 0; i < k; ++i) root.cp[i] = mk(p.P[i][0], p.P[i][1], p.P[i][2]);
-        if (p.nkeys == 2)
-            for (int i = 0; i < k; ++i) root.cp1[i] = mk(p.P1[i][0], p.P1[i][1], p.P1[i][2]);
     }
-    root.motion = p.nkeys == 2;
-    t.clear();
+    if (root.motion) {
+        tr.cp1.assign(16, mk(0, 0, 0));
+        if (p.type != RH_PRIM_SPHERE)
+            for (int i = 0; i < (p.type == RH_PRIM_BICUBIC ? 16 : 4); ++i) tr.cp1[i] = mk(p.P1[i][0], p.P1[i][1], p.P1[i][2]);
+    }
+    t.reserve(16);
     t.push_back(root);
     for (size_t i = 0; i < t.size(); ++i) {
-        post_geometry(t[i], c);
+        post_geometry(t[i], root.motion ? &tr.cp1[16 * i] : nullptr, c);
         if (t[i].culled) continue;
         Node& n = t[i];
         bool d;
@@ -658,7 +671,7 @@ void build_tree(const Ctx& c, int prim, std::vector<Node>& t) {
         if (n.discard) continue;
         // CqSurface::Split (surface.cpp:529-577): two halves; an undiceable
         // (eye-split) surface splits each half once more.
-        auto halves = [&](const Node& src, Node& a, Node& b) {
+        auto halves = [&](const Node& src, const f3* src1, Node& a, Node& b, f3* a1, f3* b1) {
             a = src; b = src;
             bool u = src.splitDir == DIR_U;
             if (src.type == RH_PRIM_SPHERE) {
@@ -667,10 +680,10 @@ void build_tree(const Ctx& c, int prim, std::vector<Node>& t) {
                 if (u) { a.thetaMax = tc; b.thetaMin = tc; } else { a.phiMax = pc; b.phiMin = pc; }
             } else if (src.type == RH_PRIM_BICUBIC) {
                 split_bicubic(src.cp, a.cp, b.cp, u);
-                if (src.motion) split_bicubic(src.cp1, a.cp1, b.cp1, u);
+                if (src.motion) split_bicubic(src1, a1, b1, u);
             } else {
                 split_bilinear(src.cp, a.cp, b.cp, u);
-                if (src.motion) split_bilinear(src.cp1, a.cp1, b.cp1, u);
+                if (src.motion) split_bilinear(src1, a1, b1, u);
             }
             for (Node* k : {&a, &b}) {
                 k->splitDir = u ? DIR_V : DIR_U;
@@ -679,31 +692,51 @@ void build_tree(const Ctx& c, int prim, std::vector<Node>& t) {
                 k->hasBound = false; k->leaf = false; k->kid0 = -1; k->nkids = 0;
             }
         };
-        Node a, b;
-        halves(t[i], a, b);
-        std::vector<Node> kids;
+        Node kids[4];
+        f3 kid1[4][16];
+        int nk;
+        const f3* own1 = root.motion ? &tr.cp1[16 * i] : nullptr;
         // (a deforming surface's keys split once: CqDeformingSurface::Split,
         // surface.h:657-692)
         if (!t[i].diceableFlag && t[i].type != RH_PRIM_BILINEAR && !t[i].motion) {
-            Node a0, a1, b0, b1;
-            halves(a, a0, a1);
-            halves(b, b0, b1);
-            kids = {a0, a1, b0, b1};
+            Node a, b;
+            f3 a1[16], b1[16];
+            halves(t[i], own1, a, b, a1, b1);
+            halves(a, a1, kids[0], kids[1], kid1[0], kid1[1]);
+            halves(b, b1, kids[2], kids[3], kid1[2], kid1[3]);
+            nk = 4;
         } else {
-            kids = {a, b};
+            halves(t[i], own1, kids[0], kids[1], kid1[0], kid1[1]);
+            nk = 2;
         }
         t[i].kid0 = (int)t.size();
-        t[i].nkids = (int)kids.size();
-        for (auto& k : kids) t.push_back(k);
+        t[i].nkids = nk;
+        for (int q = 0; q < nk; ++q) {
+            t.push_back(kids[q]);
+            if (root.motion) tr.cp1.insert(tr.cp1.end(), kid1[q], kid1[q] + 16);
+        }
     }
 }
+
+// The bucket loop's view of a node: what posting, popping, culling and
+// reposting read, packed so that the loop walks a few megabytes.
+enum { SN_LEAF = 1, SN_BOUND = 2, SN_CULLED = 4, SN_MOTION = 8 };
+struct SimNode {
+    float z, subMin;            // cached bound's minimum z; least in the subtree
+    float bx0, by0, bx1, by1;   // cached raster bound in x and y
+    int16_t x0, y0, x1, y1;     // PostSurface's bucket range, clamped to the region
+    uint32_t kid0;              // forest index of the first child
+    int32_t prim;
+    uint16_t cu, cv;            // dice sizes of a leaf
+    uint8_t nkids, flags;
+};
 
 // A bucket heap entry carries the comparator's inputs so that heap moves
 // and comparisons do not touch the nodes.
 struct HeapEnt {
     float z;                    // cached raster bound's minimum z
     bool hasBound;
-    const Node* nd;
+    uint32_t nd;                // forest index
 };
 
 struct HeapCmp {  // closest_surface (bucket.h:164-176)
@@ -717,8 +750,10 @@ struct HeapCmp {  // closest_surface (bucket.h:164-176)
 
 struct Forest {
     Ctx c;
-    std::vector<std::vector<Node>> trees;  // one split tree per primitive, parents first
-    std::vector<uint64_t> nodeBase;        // primitives + 1: global index of each tree's root
+    std::vector<Tree> trees;               // one split tree per primitive, parents first
+    std::vector<uint64_t> nodeBase;        // primitives + 1: forest index of each tree's root
+    std::vector<SimNode> sim;              // every node, in forest order
+    int threads = 1;
 };
 
 void ForestFree::operator()(Forest* f) const { delete f; }
@@ -777,15 +812,16 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
         }
     }
     // Split trees, one per primitive, in parallel.
-    f.trees.assign(n, std::vector<Node>());
+    f.trees.assign(n, Tree());
+    f.threads = std::max(1, threads);
     int nt = std::max(1, std::min(threads, n));
     std::atomic<int> next(0);
     auto work = [&]() {
         for (;;) {
             int i = next.fetch_add(1);
             if (i >= n) break;
-            std::vector<Node>& t = f.trees[i];
-            build_tree(c, i, t);
+            build_tree(c, i, f.trees[i]);
+            std::vector<Node>& t = f.trees[i].n;
             // the least minimum depth of each node's subtree (children follow
             // their parents in the tree)
             for (size_t j = t.size(); j-- > 0;) {
@@ -803,7 +839,44 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
         for (auto& th : pool) th.join();
     }
     f.nodeBase.assign(n + 1, 0);
-    for (int i = 0; i < n; ++i) f.nodeBase[i + 1] = f.nodeBase[i] + f.trees[i].size();
+    for (int i = 0; i < n; ++i) f.nodeBase[i + 1] = f.nodeBase[i] + f.trees[i].n.size();
+    if (f.nodeBase[n] >= 0xffffffffull) {
+        err = "the scene splits into more than 2^32 surfaces";
+        return false;
+    }
+    // the bucket loop's packed nodes
+    f.sim.assign(f.nodeBase[n], SimNode());
+    next = 0;
+    auto pack = [&]() {
+        for (;;) {
+            int i = next.fetch_add(1);
+            if (i >= n) break;
+            const std::vector<Node>& t = f.trees[i].n;
+            for (size_t j = 0; j < t.size(); ++j) {
+                const Node& nd = t[j];
+                SimNode& q = f.sim[f.nodeBase[i] + j];
+                q.z = nd.rb.lo.z;
+                q.subMin = nd.subMin;
+                q.bx0 = nd.rb.lo.x; q.by0 = nd.rb.lo.y; q.bx1 = nd.rb.hi.x; q.by1 = nd.rb.hi.y;
+                int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+                if (nd.diceableFlag) { x0 = nd.xminb; y0 = nd.yminb; x1 = nd.xmaxb; y1 = nd.ymaxb; }
+                q.x0 = (int16_t)clampv(x0, s.bx0, s.bx1 - 1); q.y0 = (int16_t)clampv(y0, s.by0, s.by1 - 1);
+                q.x1 = (int16_t)clampv(x1, s.bx0, s.bx1 - 1); q.y1 = (int16_t)clampv(y1, s.by0, s.by1 - 1);
+                q.kid0 = nd.nkids ? (uint32_t)(f.nodeBase[i] + nd.kid0) : 0u;
+                q.prim = nd.prim;
+                q.cu = (uint16_t)nd.uDice; q.cv = (uint16_t)nd.vDice;
+                q.nkids = (uint8_t)nd.nkids;
+                q.flags = (uint8_t)((nd.leaf ? SN_LEAF : 0) | (nd.hasBound ? SN_BOUND : 0) |
+                                    (nd.culled ? SN_CULLED : 0) | (nd.motion ? SN_MOTION : 0));
+            }
+        }
+    };
+    if (nt == 1) pack();
+    else {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < nt; ++k) pool.emplace_back(pack);
+        for (auto& th : pool) th.join();
+    }
     // the caller's shape and surfaces are passed again to simulate
     c.s = nullptr;
     c.prims = nullptr;
@@ -817,7 +890,7 @@ void cull_pairs(const Shape& s, const Forest& f, CullPairs& out) {
     out.range.assign(nn * 4, 0);
     out.start.assign(nn + 1, 0);
     for (size_t p = 0; p + 1 < f.nodeBase.size(); ++p) {
-        const auto& t = f.trees[p];
+        const auto& t = f.trees[p].n;
         for (size_t j = 0; j < t.size(); ++j) {
             const size_t g = f.nodeBase[p] + j;
             const Node& nd = t[j];
@@ -878,10 +951,30 @@ uint64_t cull_trees(const Shape& s, std::vector<CullTreeDev>& out) {
     return pyr;
 }
 
+// Run fn(i) for i in [0, n) on up to `threads` threads.
+template <typename Fn> static void parallel_for(int n, int threads, Fn fn) {
+    const int nt = std::max(1, std::min(threads, n));
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::atomic<int> next(0);
+    std::vector<std::thread> pool;
+    for (int k = 0; k < nt; ++k)
+        pool.emplace_back([&]() {
+            for (;;) {
+                const int i = next.fetch_add(1);
+                if (i >= n) break;
+                fn(i);
+            }
+        });
+    for (auto& th : pool) th.join();
+}
+
 bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, const CullInput* cull,
               const CullPairs* pairs, Schedule& out, std::string& err) {
     const Ctx& c = f.c;
-    const auto& trees = f.trees;
+    const std::vector<SimNode>& sn = f.sim;
     std::vector<CullTreeDev> ctrees;
     if (cull) cull_trees(s, ctrees);
     // Bucket loop: post roots in order, pop nearest first, cull (against
@@ -894,15 +987,13 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
         heaps[k].push_back(ent);
         std::push_heap(heaps[k].begin(), heaps[k].end(), HeapCmp());
     };
-    auto post = [&](const Node* nd) {
+    auto post = [&](uint32_t g) {
         out.surfaces++;
-        if (nd->culled) return;
-        int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
-        if (nd->diceableFlag) { x0 = nd->xminb; y0 = nd->yminb; x1 = nd->xmaxb; y1 = nd->ymaxb; }
-        x0 = clampv(x0, s.bx0, s.bx1 - 1); y0 = clampv(y0, s.by0, s.by1 - 1);
-        x1 = clampv(x1, s.bx0, s.bx1 - 1); y1 = clampv(y1, s.by0, s.by1 - 1);
+        const SimNode& q = sn[g];
+        if (q.flags & SN_CULLED) return;
+        const int x0 = q.x0, y0 = q.y0, x1 = q.x1, y1 = q.y1;
         int k = s.bucketIndex(x0, y0);
-        const HeapEnt ent{nd->rb.lo.z, nd->hasBound, nd};
+        const HeapEnt ent{q.z, (q.flags & SN_BOUND) != 0, g};
         if (!processed[k]) {
             push(k, ent);
             return;
@@ -924,36 +1015,35 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
     // CqImageBuffer::RepostSurface (imagebuffer.cpp:449-483): the bucket to
     // the right when the bound reaches it, else the next row's bucket under
     // the bound's left side; dropped past the last row.
-    auto repost = [&](int k, const Node* nd) {
+    auto repost = [&](int k, uint32_t g) {
+        const SimNode& q = sn[g];
         const auto& b = s.buckets[k];
         const int col = s.bx0 + k % s.nbr, row = s.by0 + k / s.nbr;
         int nextX = col + 1, nextY = row;
-        const HeapEnt ent{nd->rb.lo.z, nd->hasBound, nd};
-        if (nextX < s.bx1 && nd->rb.hi.x >= (float)(b.x + b.w)) {
+        const HeapEnt ent{q.z, (q.flags & SN_BOUND) != 0, g};
+        if (nextX < s.bx1 && q.bx1 >= (float)(b.x + b.w)) {
             push(s.bucketIndex(nextX, nextY), ent);
             return;
         }
         ++nextY;
-        nextX = (int)mx<long>((long)s.bx0, floor_l(nd->rb.lo.x) / s.o.bucket_x);
-        if (nextX < s.bx1 && nextY < s.by1 && nd->rb.hi.y >= (float)(b.y + b.h)) push(s.bucketIndex(nextX, nextY), ent);
+        nextX = (int)mx<long>((long)s.bx0, floor_l(q.bx0) / s.o.bucket_x);
+        if (nextX < s.bx1 && nextY < s.by1 && q.by1 >= (float)(b.y + b.h)) push(s.bucketIndex(nextX, nextY), ent);
     };
-    auto node_index = [&](const Node* nd) -> uint64_t {
-        return f.nodeBase[nd->prim] + (uint64_t)(nd - trees[nd->prim].data());
-    };
-    for (int i = 0; i < n; ++i) post(&trees[i][0]);
+    for (int i = 0; i < n; ++i) post((uint32_t)f.nodeBase[i]);
+    // the dicing order: (node, bucket) of each grid
+    std::vector<std::pair<uint32_t, int32_t>> diced;
     uint32_t mpCount = 0, vtxCount = 0;
-    std::vector<float>& P = out.params;
     for (int k = 0; k < nb; ++k) {
         auto& h = heaps[k];
         while (!h.empty()) {
-            const Node* nd = h.front().nd;
+            const uint32_t g = h.front().nd;
             std::pop_heap(h.begin(), h.end(), HeapCmp());
             h.pop_back();
+            const SimNode& q = sn[g];
             // RenderSurface's occlusion cull (bucketprocessor.cpp:945-957):
             // Attribute "cull" "hidden" (default 1) and a cached bound.
-            if (cull && nd->hasBound && !prims[nd->prim].no_cull_hidden) {
-                const uint64_t g = node_index(nd);
-                const int32_t* R = &pairs->range[4 * g];
+            if (cull && (q.flags & SN_BOUND) && !prims[q.prim].no_cull_hidden) {
+                const int32_t* R = &pairs->range[4 * (size_t)g];
                 const int col = s.bx0 + k % s.nbr, row = s.by0 + k / s.nbr;
                 if (col < R[0] || col > R[2] || row < R[1] || row > R[3]) {
                     err = "internal: occlusion test outside a surface's bucket range";
@@ -961,7 +1051,7 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
                 }
                 const uint64_t pi = pairs->start[g] + (uint64_t)(row - R[1]) * (uint64_t)(R[2] - R[0] + 1) +
                                     (uint64_t)(col - R[0]);
-                const float z = nd->rb.lo.z, M = cull->pairMax[pi];
+                const float z = q.z, M = cull->pairMax[pi];
                 bool culled = M < z;
                 auto fx = std::lower_bound(cull->fixed.begin(), cull->fixed.end(), std::make_pair(pi, (uint8_t)0));
                 if (fx != cull->fixed.end() && fx->first == pi) {
@@ -978,7 +1068,7 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
                         bool unordered = false;
                         for (const HeapEnt& e : h) {
                             if (!e.hasBound) unordered = true;
-                            else lim = mn(lim, e.nd->subMin);
+                            else lim = mn(lim, sn[e.nd].subMin);
                         }
                         settled = !unordered && M < lim - margin;
                     } else {
@@ -991,8 +1081,8 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
                         pr.mpCut = mpCount;
                         pr.z = z;
                         pr.cx0 = pr.cx1 = pr.cy0 = pr.cy1 = 0;
-                        pr.kind = ctrees[k].levels ? cull_cells(ctrees[k], nd->rb.lo.x, nd->rb.lo.y, nd->rb.hi.x,
-                                                                nd->rb.hi.y, pr.cx0, pr.cx1, pr.cy0, pr.cy1)
+                        pr.kind = ctrees[k].levels ? cull_cells(ctrees[k], q.bx0, q.by0, q.bx1, q.by1, pr.cx0,
+                                                                pr.cx1, pr.cy0, pr.cy1)
                                                    : 0;
                         pr.culled = culled ? 1 : 0;
                         out.probes.push_back(pr);
@@ -1000,69 +1090,106 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
                 }
                 if (culled) {
                     out.culled++;
-                    repost(k, nd);
+                    repost(k, g);
                     continue;
                 }
             }
-            if (nd->leaf) {
-                GridRec g;
-                g.type = nd->type; g.cu = nd->uDice; g.cv = nd->vDice; g.prim = nd->prim;
-                g.mpBase = mpCount; g.vtxBase = vtxCount; g.paramOff = (uint32_t)P.size();
-                g.diceBucket = k;
-                // Candidate buckets: the cached raster bound (already grown by
-                // half the filter width) plus a one-pixel guard band.
-                const rh_options& o = s.o;
-                auto bidx = [](float v, int bs) { return (int)std::floor((double)v / bs); };
-                g.cbx0 = (int16_t)clampv(bidx(nd->rb.lo.x - 1.0f, o.bucket_x), s.bx0, s.bx1 - 1);
-                g.cby0 = (int16_t)clampv(bidx(nd->rb.lo.y - 1.0f, o.bucket_y), s.by0, s.by1 - 1);
-                g.cbx1 = (int16_t)clampv(bidx(nd->rb.hi.x + 1.0f, o.bucket_x), s.bx0, s.bx1 - 1);
-                g.cby1 = (int16_t)clampv(bidx(nd->rb.hi.y + 1.0f, o.bucket_y), s.by0, s.by1 - 1);
-                g.px0 = nd->rb.lo.x - 2.0f; g.py0 = nd->rb.lo.y - 2.0f;
-                g.px1 = nd->rb.hi.x + 2.0f; g.py1 = nd->rb.hi.y + 2.0f;
-                if ((g.cu + 1) * (g.cv + 1) > 514 || g.cu < 1 || g.cv < 1) {
+            if (q.flags & SN_LEAF) {
+                if ((q.cu + 1) * (q.cv + 1) > 514 || q.cu < 1 || q.cv < 1) {
                     err = "internal: grid size outside the dicing limits";
                     return false;
                 }
-                mpCount += (uint32_t)(g.cu * g.cv);
-                vtxCount += (uint32_t)((g.cu + 1) * (g.cv + 1)) * (nd->motion ? 2u : 1u);
-                // parameters of each key in turn; a deforming surface's keys
-                // share the dice sizes (CopySplitInfo) and its key times follow
-                for (int key = 0; key < (nd->motion ? 2 : 1); ++key) {
-                    if (nd->type == RH_PRIM_SPHERE) {
-                        const m44& M = key ? c.objToCam1[nd->prim] : c.objToCam[nd->prim];
-                        P.push_back(nd->radius);
-                        for (int r = 0; r < 4; ++r)
-                            for (int cc = 0; cc < 4; ++cc) P.push_back(M.e[r][cc]);
-                        P.push_back(M.id ? 1.0f : 0.0f);
-                        size_t o0 = P.size();
-                        P.resize(o0 + 2 * (g.cu + 1) + 2 * (g.cv + 1));
-                        sincos_grid(deg2rad(nd->thetaMin), deg2rad(nd->thetaMax), g.cu + 1, &P[o0],
-                                    &P[o0 + g.cu + 1]);
-                        sincos_grid(nd->phiMin, nd->phiMax, g.cv + 1, &P[o0 + 2 * (g.cu + 1)],
-                                    &P[o0 + 2 * (g.cu + 1) + g.cv + 1]);
-                    } else {
-                        int kk = nd->type == RH_PRIM_BICUBIC ? 16 : 4;
-                        const f3* cp = key ? nd->cp1 : nd->cp;
-                        for (int i = 0; i < kk; ++i) { P.push_back(cp[i].x); P.push_back(cp[i].y); P.push_back(cp[i].z); }
-                    }
-                }
-                if (nd->motion) {
-                    g.type |= GRID_MOTION;
-                    P.push_back(prims[nd->prim].key_time[0]);
-                    P.push_back(prims[nd->prim].key_time[1]);
-                    out.motion = true;
-                }
-                out.grids.push_back(g);
-            } else if (nd->nkids > 0) {
+                diced.emplace_back(g, k);
+                mpCount += (uint32_t)(q.cu * q.cv);
+                vtxCount += (uint32_t)((q.cu + 1) * (q.cv + 1)) * ((q.flags & SN_MOTION) ? 2u : 1u);
+            } else {
                 // children live in the same tree as their parent
-                const std::vector<Node>& tree = trees[nd->prim];
-                for (int q = 0; q < nd->nkids; ++q) post(&tree[nd->kid0 + q]);
+                for (int j = 0; j < q.nkids; ++j) post(q.kid0 + (uint32_t)j);
             }
         }
         processed[k] = 1;
     }
     out.micropolygons = mpCount;
     out.vertices = vtxCount;
+
+    // The grids in dicing order: ids, vertex and parameter offsets first
+    // (running sums), then the records and parameters in parallel.
+    const size_t ng = diced.size();
+    out.grids.resize(ng);
+    std::vector<uint64_t> poff(ng + 1, 0);
+    {
+        uint32_t mp = 0, vx = 0;
+        for (size_t i = 0; i < ng; ++i) {
+            const SimNode& q = sn[diced[i].first];
+            const int keys = (q.flags & SN_MOTION) ? 2 : 1;
+            const int type = f.trees[q.prim].n.empty() ? 0 : f.trees[q.prim].n[0].type;
+            uint64_t pk = type == RH_PRIM_SPHERE ? (uint64_t)(18 + 2 * (q.cu + 1) + 2 * (q.cv + 1))
+                                                 : (type == RH_PRIM_BICUBIC ? 48u : 12u);
+            poff[i + 1] = poff[i] + pk * keys + (keys == 2 ? 2 : 0);
+            GridRec& gr = out.grids[i];
+            gr.mpBase = mp;
+            gr.vtxBase = vx;
+            mp += (uint32_t)(q.cu * q.cv);
+            vx += (uint32_t)((q.cu + 1) * (q.cv + 1)) * (uint32_t)keys;
+            if (keys == 2) out.motion = true;
+        }
+    }
+    if (poff[ng] >= 0xffffffffull) {
+        err = "the schedule's dicing parameters exceed 2^32 floats";
+        return false;
+    }
+    std::vector<float>& P = out.params;
+    P.assign(poff[ng], 0.0f);
+    const int chunks = (int)((ng + 255) / 256);
+    parallel_for(chunks, f.threads, [&](int ch) {
+        const size_t i0 = (size_t)ch * 256, i1 = std::min(ng, i0 + 256);
+        for (size_t i = i0; i < i1; ++i) {
+            const uint32_t g = diced[i].first;
+            const SimNode& q = sn[g];
+            const Tree& tr = f.trees[q.prim];
+            const size_t local = g - f.nodeBase[q.prim];
+            const Node& nd = tr.n[local];
+            GridRec& gr = out.grids[i];
+            gr.type = nd.type; gr.cu = nd.uDice; gr.cv = nd.vDice; gr.prim = nd.prim;
+            gr.paramOff = (uint32_t)poff[i];
+            gr.diceBucket = diced[i].second;
+            // Candidate buckets: the cached raster bound (already grown by
+            // half the filter width) plus a one-pixel guard band.
+            const rh_options& o = s.o;
+            auto bidx = [](float v, int bs) { return (int)std::floor((double)v / bs); };
+            gr.cbx0 = (int16_t)clampv(bidx(nd.rb.lo.x - 1.0f, o.bucket_x), s.bx0, s.bx1 - 1);
+            gr.cby0 = (int16_t)clampv(bidx(nd.rb.lo.y - 1.0f, o.bucket_y), s.by0, s.by1 - 1);
+            gr.cbx1 = (int16_t)clampv(bidx(nd.rb.hi.x + 1.0f, o.bucket_x), s.bx0, s.bx1 - 1);
+            gr.cby1 = (int16_t)clampv(bidx(nd.rb.hi.y + 1.0f, o.bucket_y), s.by0, s.by1 - 1);
+            gr.px0 = nd.rb.lo.x - 2.0f; gr.py0 = nd.rb.lo.y - 2.0f;
+            gr.px1 = nd.rb.hi.x + 2.0f; gr.py1 = nd.rb.hi.y + 2.0f;
+            // parameters of each key in turn; a deforming surface's keys
+            // share the dice sizes (CopySplitInfo) and its key times follow
+            float* W = &P[poff[i]];
+            for (int key = 0; key < (nd.motion ? 2 : 1); ++key) {
+                if (nd.type == RH_PRIM_SPHERE) {
+                    const m44& M = key ? c.objToCam1[nd.prim] : c.objToCam[nd.prim];
+                    *W++ = nd.radius;
+                    for (int r = 0; r < 4; ++r)
+                        for (int cc = 0; cc < 4; ++cc) *W++ = M.e[r][cc];
+                    *W++ = M.id ? 1.0f : 0.0f;
+                    sincos_grid(deg2rad(nd.thetaMin), deg2rad(nd.thetaMax), gr.cu + 1, W, W + gr.cu + 1);
+                    W += 2 * (gr.cu + 1);
+                    sincos_grid(nd.phiMin, nd.phiMax, gr.cv + 1, W, W + gr.cv + 1);
+                    W += 2 * (gr.cv + 1);
+                } else {
+                    const int kk = nd.type == RH_PRIM_BICUBIC ? 16 : 4;
+                    const f3* cp = key ? &tr.cp1[16 * local] : nd.cp;
+                    for (int j = 0; j < kk; ++j) { *W++ = cp[j].x; *W++ = cp[j].y; *W++ = cp[j].z; }
+                }
+            }
+            if (nd.motion) {
+                gr.type |= GRID_MOTION;
+                *W++ = prims[nd.prim].key_time[0];
+                *W++ = prims[nd.prim].key_time[1];
+            }
+        }
+    });
 
     // Candidate lists per bucket (grids in dicing order), only buckets not
     // yet processed when the grid was diced.
@@ -1075,38 +1202,40 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
             }
     for (int k = 0; k < nb; ++k) out.candStart[k + 1] += out.candStart[k];
     out.candGrid.assign(out.candStart[nb], 0);
-    const int ntask = (int)s.tasks.size();
-    out.capStart.assign(ntask + 1, 0);
     std::vector<uint32_t> fill(out.candStart.begin(), out.candStart.end() - 1);
     for (uint32_t gi = 0; gi < out.grids.size(); ++gi) {
         const auto& g = out.grids[gi];
         for (int by = g.cby0; by <= g.cby1; ++by)
             for (int bx = g.cbx0; bx <= g.cbx1; ++bx) {
                 int k = s.bucketIndex(bx, by);
-                if (k >= g.diceBucket) {
-                    out.candGrid[fill[k]++] = gi;
-                    // A micropolygon enters a task's list only if its pixel
-                    // window meets the task, and a static one's window lies
-                    // inside the grid's guarded bound.  A moving one's
-                    // window (its sub-bounds) may reach past its keys'
-                    // bound to anywhere in the bucket.
-                    const bool anyTask = (g.type & GRID_MOTION) != 0;
-                    for (int t = s.taskStart[k]; t < s.taskStart[k + 1]; ++t) {
-                        const auto& T = s.tasks[t];
-                        if (anyTask || (g.px0 < (float)T.x1 && g.px1 > (float)T.x0 && g.py0 < (float)T.y1 &&
-                                        g.py1 > (float)T.y0))
-                            out.capStart[t + 1] += (uint64_t)(g.cu * g.cv);
-                    }
-                }
+                if (k >= g.diceBucket) out.candGrid[fill[k]++] = gi;
             }
     }
+    // Each task's list capacity, bucket by bucket in parallel: a
+    // micropolygon enters a task's list only if its pixel window meets the
+    // task, and a static one's window lies inside the grid's guarded bound.
+    // A moving one's window (its sub-bounds) may reach past its keys' bound
+    // to anywhere in the bucket.
+    const int ntask = (int)s.tasks.size();
+    out.capStart.assign(ntask + 1, 0);
+    parallel_for(nb, f.threads, [&](int k) {
+        for (uint32_t e = out.candStart[k]; e < out.candStart[k + 1]; ++e) {
+            const auto& g = out.grids[out.candGrid[e]];
+            const bool anyTask = (g.type & GRID_MOTION) != 0;
+            for (int t = s.taskStart[k]; t < s.taskStart[k + 1]; ++t) {
+                const auto& T = s.tasks[t];
+                if (anyTask || (g.px0 < (float)T.x1 && g.px1 > (float)T.x0 && g.py0 < (float)T.y1 &&
+                                g.py1 > (float)T.y0))
+                    out.capStart[t + 1] += (uint64_t)(g.cu * g.cv);
+            }
+        }
+    });
     for (int t = 0; t < ntask; ++t) out.capStart[t + 1] += out.capStart[t];
     out.segBucket.resize(out.candGrid.size());
     for (int k = 0; k < nb; ++k)
-        for (uint32_t c = out.candStart[k]; c < out.candStart[k + 1]; ++c) out.segBucket[c] = (uint32_t)k;
+        for (uint32_t e = out.candStart[k]; e < out.candStart[k + 1]; ++e) out.segBucket[e] = (uint32_t)k;
     return true;
 }
-
 
 bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, std::string& err, int threads) {
     ForestPtr f;
