@@ -51,6 +51,7 @@ class RhOptions(C.Structure):
         ("jitter", C.c_int32),
         ("num_displays", C.c_int32),
         ("zthreshold", C.c_float * 3),
+        ("dof", C.c_float * 3),
     ]
 
 
@@ -67,6 +68,7 @@ class RhPrim(C.Structure):
         ("P", (C.c_float * 3) * 16),
         ("P1", (C.c_float * 3) * 16),
         ("no_cull_hidden", C.c_int32),
+        ("focus_factor", C.c_float),
     ]
 
 
@@ -107,10 +109,11 @@ def default_options(**kw):
     o.jitter = 1
     o.num_displays = 1
     o.zthreshold[:] = [1.0, 1.0, 1.0]
+    o.dof[:] = [FLT_MAX, FLT_MAX, FLT_MAX]
     for k, v in kw.items():
         if k == "filter" and isinstance(v, str):
             v = FILTERS[v]
-        if k in ("crop", "zthreshold"):
+        if k in ("crop", "zthreshold", "dof"):
             getattr(o, k)[:] = list(v)
         else:
             setattr(o, k, v)

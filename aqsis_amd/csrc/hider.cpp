@@ -205,6 +205,7 @@ void rh_options_default(rh_options* o) {
     o->jitter = 1;
     o->num_displays = 1;
     o->zthreshold[0] = o->zthreshold[1] = o->zthreshold[2] = 1.0f;
+    o->dof[0] = o->dof[1] = o->dof[2] = FLT_MAX;   // pinhole camera (options.cpp:297)
 }
 
 const char* rh_last_error(void) { return g_err.c_str(); }

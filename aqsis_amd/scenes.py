@@ -37,6 +37,7 @@ def sphere(radius, centre, Cs, Os=(1.0, 1.0, 1.0), shading_rate=1.0):
     p.type = RH_PRIM_SPHERE
     p.nkeys = 1
     p.shading_rate = shading_rate
+    p.focus_factor = 1.0
     p.Cs[:] = [float(c) for c in Cs]
     p.Os[:] = [float(c) for c in Os]
     p.sphere[:] = [float(radius), -float(radius), float(radius), 360.0]
@@ -50,6 +51,7 @@ def bicubic(P, Cs, Os=(1.0, 1.0, 1.0), shading_rate=1.0):
     p.type = RH_PRIM_BICUBIC
     p.nkeys = 1
     p.shading_rate = shading_rate
+    p.focus_factor = 1.0
     p.Cs[:] = [float(c) for c in Cs]
     p.Os[:] = [float(c) for c in Os]
     P = np.asarray(P, dtype=np.float32).reshape(16, 3)
@@ -65,6 +67,7 @@ def bilinear(P, Cs, Os=(1.0, 1.0, 1.0), shading_rate=1.0):
     p.type = RH_PRIM_BILINEAR
     p.nkeys = 1
     p.shading_rate = shading_rate
+    p.focus_factor = 1.0
     p.Cs[:] = [float(c) for c in Cs]
     p.Os[:] = [float(c) for c in Os]
     P = np.asarray(P, dtype=np.float32).reshape(4, 3)
@@ -260,9 +263,11 @@ def motion_scene(seed=1, n=600, res=(256, 256), fov=45.0, znear=8.0, zfar=40.0, 
 
 
 def _prim_records(n):
-    """A zeroed ctypes array of n RhPrim and a numpy view of its bytes."""
+    """A ctypes array of n RhPrim, zeroed but for focus_factor (1), and a numpy
+    view of its bytes."""
     arr = (RhPrim * n)()
     raw = np.frombuffer(arr, dtype=np.uint8).reshape(n, C.sizeof(RhPrim))
+    _field(raw, "focus_factor", ())[:] = 1.0   # GeometricApproximation "focusfactor" default
     return arr, raw
 
 

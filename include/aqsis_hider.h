@@ -60,8 +60,8 @@ enum {
 /*
  * The hot options.  Field meanings and defaults follow RiFormat,
  * RiCropWindow, RiProjection, RiClipping, RiPixelSamples, RiPixelFilter,
- * RiExposure, RiQuantize "rgba", RiShutter and Option "limits"
- * (libs/core/api/ri.cpp:730-930, libs/core/options.cpp:273-310,
+ * RiExposure, RiQuantize "rgba", RiShutter, RiDepthOfField and Option
+ * "limits" (libs/core/api/ri.cpp:730-930, libs/core/options.cpp:273-310,
  * libs/core/optioncache.cpp:48-117).  rh_options_default() fills them in.
  */
 typedef struct rh_options {
@@ -81,6 +81,9 @@ typedef struct rh_options {
     int32_t jitter;              /* Hider "hidden" "jitter" (1)              */
     int32_t num_displays;        /* display requests that consume dither draws (1) */
     float zthreshold[3];         /* Option "limits" "zthreshold" (1 1 1)     */
+    float dof[3];                /* DepthOfField fstop focallength focaldistance
+                                    (FLT_MAX x3: pinhole; depth of field is on
+                                    when fstop < FLT_MAX, renderer.h:368-377) */
 } rh_options;
 
 /*
@@ -101,6 +104,10 @@ typedef struct rh_options {
  * no_cull_hidden = 1 is Attribute "cull" "hidden" [0]: the surface is never
  * occlusion culled (RenderSurface, bucketprocessor.cpp:945-957); 0, the
  * reference's default, lets a bucket's occlusion tree cull it.
+ * focus_factor is GeometricApproximation "focusfactor" (the reference's
+ * default is 1; a record built zeroed must set it): with depth of field the
+ * shading rate grows by max(1, 0.025 * focus_factor * CoC^2)
+ * (CqSurface::AdjustedShadingRate, geometry/surface.cpp:688-724).
  */
 typedef struct rh_prim {
     int32_t type;
@@ -114,6 +121,7 @@ typedef struct rh_prim {
     float P[16][3];
     float P1[16][3];
     int32_t no_cull_hidden;
+    float focus_factor;
 } rh_prim;
 
 /* Counters of one frame.  spl_* are the reference's own hider counters

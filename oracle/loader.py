@@ -71,8 +71,10 @@ def render(opts, prims, occlusion_cull=False, samples=True):
     depth = np.zeros(ns, np.float32) if samples else None
     rgbaz = np.zeros((ns, 7), np.float32) if samples else None
     st = RhStats()
-    lib().oracle_render(C.byref(opts), arr, len(prims), 1 if occlusion_cull else 0,
-                        _ptr(ch), _ptr(rgba), _ptr(ids), _ptr(depth), _ptr(rgbaz), C.byref(st))
+    rc = lib().oracle_render(C.byref(opts), arr, len(prims), 1 if occlusion_cull else 0,
+                             _ptr(ch), _ptr(rgba), _ptr(ids), _ptr(depth), _ptr(rgbaz), C.byref(st))
+    if rc != 0:
+        raise ValueError(f"oracle_render: unsupported scene (code {rc}: depth of field with motion keys)")
     return {"channels": ch, "rgba8": rgba, "mp_id": ids, "depth": depth, "rgbaz": rgbaz,
             "stats": st.as_dict(), "ext": (ew, eh, spp)}
 
@@ -94,6 +96,33 @@ def sample_times(opts):
     t = np.zeros((eh, ew, spp), np.float32)
     L.oracle_sample_times(C.byref(opts), _ptr(t))
     return t
+
+
+def sample_lens(opts):
+    """Lens offset of every sample of the extended image (eh, ew, spp, 2) and
+    the lens cell each sample is visited through (eh, ew, spp)."""
+    w, h, ew, eh, spp = frame_size(opts)
+    L = lib()
+    L.oracle_sample_lens.argtypes = [C.POINTER(RhOptions), C.c_void_p, C.c_void_p]
+    xy = np.zeros((eh, ew, spp, 2), np.float32)
+    cell = np.zeros((eh, ew, spp), np.int32)
+    L.oracle_sample_lens(C.byref(opts), _ptr(xy), _ptr(cell))
+    return xy, cell
+
+
+def dof(opts, z):
+    """The frame's circle of confusion at camera depths z (n, 2), its lens-cell
+    bounds (spp, 4) and MinCoCForBound of consecutive depth pairs (n - 1)."""
+    w, h, ew, eh, spp = frame_size(opts)
+    L = lib()
+    L.oracle_dof.argtypes = [C.POINTER(RhOptions), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    z = np.ascontiguousarray(z, np.float32)
+    coc = np.zeros((len(z), 2), np.float32)
+    cells = np.zeros((spp, 4), np.float32)
+    mn = np.zeros(max(len(z) - 1, 1), np.float32)
+    if L.oracle_dof(C.byref(opts), len(z), _ptr(z), _ptr(coc), _ptr(cells), _ptr(mn)) != 0:
+        raise ValueError("the frame has no depth of field")
+    return coc, cells, mn[:len(z) - 1]
 
 
 def dice(prim, u, v):

@@ -57,6 +57,39 @@ static Bound bound_transform(const Bound& b, const Mat& M) {  // bound.cpp:75-10
 }
 
 // ---------------------------------------------------------------------------
+// Depth of field: CqRenderer::SetDepthOfFieldData / SetDepthOfFieldScale
+// (renderer.h:368-396, options.cpp:160-183).
+struct Dof {
+    bool on = false;
+    float mult = 0, oneOverF = 0;  // m_DofMultiplier, m_OneOverFocalDistance
+    V2 scale = {0, 0};             // m_DepthOfFieldScale
+};
+// GetCircleOfConfusion (renderer.h:401-406): the raster radius of the
+// circle of confusion at a camera depth.
+static V2 circle_of_confusion(const Dof& d, float depth) {
+    float c = d.mult * std::fabs(1.0f / depth - d.oneOverF);
+    return v2(d.scale.x * c, d.scale.y * c);
+}
+// MinCoCForBound (renderer.cpp:1602-1617)
+static float min_coc_for_bound(const Dof& d, float z1, float z2) {
+    float focalDist = 1 / d.oneOverF;
+    if ((z1 - focalDist) * (z2 - focalDist) < 0) return 0;
+    float minBlur = tmin(std::fabs(1 / z1 - d.oneOverF), std::fabs(1 / z2 - d.oneOverF));
+    return d.mult * tmin(d.scale.x, d.scale.y) * minBlur;
+}
+// CqImagePixel::projectToCircle (imagepixel.h:429-436): the unit square to
+// the unit disk, radially.
+static V2 project_to_circle(V2 pos) {
+    float r;
+    if (pos.y == 0.0f) r = std::fabs(pos.x);
+    else if (pos.x == 0.0f) r = std::fabs(pos.y);
+    else r = std::sqrt((pos.x * pos.x) + (pos.y * pos.y));  // CqVector2D::Magnitude
+    if (r == 0.0f) return v2(0, 0);
+    float adj = tmax(std::fabs(pos.x), std::fabs(pos.y)) / r;
+    return v2(adj * pos.x, adj * pos.y);
+}
+
+// ---------------------------------------------------------------------------
 // Frame options: SqOptionCache (optioncache.cpp:48-117), camera
 // (options.cpp:40-160, ri.cpp:586-617), crop window (renderer.cpp:1619-1626).
 struct Frame {
@@ -71,6 +104,8 @@ struct Frame {
     Mat diceCoords;              // camToRaster with the z column zeroed
     int extX0, extY0, extW, extH;  // extended (sampled) image
     std::vector<float> filterValues;  // InitialiseFilterValues
+    Dof dof;                     // lens (CqRenderer's depth-of-field state)
+    std::vector<Bound> dofBounds;  // CalculateDofBounds: one per lens cell
 };
 
 static void setup_frame(const rh_options& o, Frame& f) {
@@ -137,6 +172,56 @@ static void setup_frame(const rh_options& o, Frame& f) {
     f.diceCoords = f.camToRaster;
     f.diceCoords.m[0][2] = f.diceCoords.m[1][2] = 0;
     f.diceCoords.m[2][2] = f.diceCoords.m[3][2] = 0;
+
+    // The CoC scale: matVSpaceToSpace("camera", "raster") -- the same product
+    // with its translation and projective column cleared (renderer.cpp:
+    // 897-957) -- applied to (1,1,-1) and (0,0,-1) (options.cpp:160-171).
+    {
+        Mat dofm = f.camToRaster;
+        dofm.m[3][0] = dofm.m[3][1] = dofm.m[3][2] = dofm.m[0][3] = dofm.m[1][3] = dofm.m[2][3] = 0.0f;
+        dofm.m[3][3] = 1.0f;
+        V3 dofe = mat_vec3(dofm, v3(1, 1, -1));
+        V3 dofc = mat_vec3(dofm, v3(0, 0, -1));
+        f.dof.scale = v2(std::fabs(dofe.x - dofc.x), std::fabs(dofe.y - dofc.y));
+        // SetDepthOfFieldData (renderer.h:368-377), double intermediates
+        const float fstop = o.dof[0], focalLength = o.dof[1], focalDistance = o.dof[2];
+        f.dof.on = fstop < FLT_MAX;
+        if (f.dof.on) {
+            float lensDiameter = focalLength / fstop;
+            f.dof.mult = (float)(0.5 * lensDiameter * focalDistance / (focalDistance + lensDiameter));
+            f.dof.oneOverF = (float)(1.0 / focalDistance);
+        }
+    }
+    // CalculateDofBounds (bucketprocessor.cpp:858-911): the bound of the lens
+    // offsets of each of the xs * ys cells of [-1,1]^2 after projectToCircle.
+    if (f.dof.on) {
+        f.dofBounds.assign(f.nsamp, Bound());
+        float dx = (float)(2.0 / f.nx), dy = (float)(2.0 / f.ny);
+        float minX = -1.0f, minY = -1.0f;
+        int which = 0;
+        for (int j = 0; j < f.ny; ++j) {
+            for (int i = 0; i < f.nx; ++i) {
+                V2 tl = project_to_circle(v2(minX, minY)), tr = project_to_circle(v2(minX + dx, minY));
+                V2 bl = project_to_circle(v2(minX, minY + dy)), br = project_to_circle(v2(minX + dx, minY + dy));
+                if ((tl.y > 0.0f && bl.y < 0.0f) || (tl.y < 0.0f && bl.y > 0.0f)) {
+                    tl.x = minX; bl.x = minX; tr.x = minX + dx; br.x = minX + dx;
+                }
+                if ((tl.x > 0.0f && tr.x < 0.0f) || (tl.x < 0.0f && tr.x > 0.0f)) {
+                    tl.y = minY; bl.y = minY + dy; tr.y = minY; br.y = minY + dy;
+                }
+                Bound& b = f.dofBounds[which];
+                b.mn = v3(tl.x, tl.y, 0.0f);
+                b.mx = v3(tl.x, tl.y, 0.0f);
+                encapsulate(b, v3(tr.x, tr.y, 0.0f));
+                encapsulate(b, v3(bl.x, bl.y, 0.0f));
+                encapsulate(b, v3(br.x, br.y, 0.0f));
+                ++which;
+                minX += dx;
+            }
+            minX = -1.0f;
+            minY += dy;
+        }
+    }
 
     f.extX0 = f.cropXMin - f.shiftX;
     f.extY0 = f.cropYMin - f.shiftY;
@@ -407,9 +492,20 @@ static void sphere_dice_points(const Surf& s, int uDice, int vDice, std::vector<
         }
 }
 
-static float adjusted_shading_rate(const rh_prim& p) {
-    // CqSurface::AdjustedShadingRate (surface.cpp:690-...): no DoF, no motion.
-    return p.shading_rate;
+// CqSurface::AdjustedShadingRate (geometry/surface.cpp:688-724) without
+// the motion factor: with depth of field the rate grows with the least
+// circle of confusion over the surface's cached raster bound (its camera z
+// range), GeometricFocusFactor times 0.025 of its area.
+static thread_local const Dof* g_dof = nullptr;  // the frame this thread renders
+static float adjusted_shading_rate(const Surf& s, const rh_prim& p) {
+    float shadingRate = p.shading_rate;
+    if (g_dof && g_dof->on) {
+        const float focusFactor = p.focus_factor;
+        const float minCoC = min_coc_for_bound(*g_dof, s.rasterBound.mn.z, s.rasterBound.mx.z);
+        const float areaRatio = 0.025f;
+        shadingRate *= tmax(1.0f, areaRatio * focusFactor * minCoC * minCoC);
+    }
+    return shadingRate;
 }
 
 // CqQuadric::EstimateGridSize + Diceable (quadrics.cpp:259-346)
@@ -436,7 +532,7 @@ static bool sphere_diceable(Surf& s, const Mat& matCtoR, const rh_prim& prim) {
         }
     maxusize = std::sqrt(maxusize);
     maxvsize = std::sqrt(maxvsize);
-    float sqrtShadingRate = std::sqrt(adjusted_shading_rate(prim));
+    float sqrtShadingRate = std::sqrt(adjusted_shading_rate(s, prim));
     s.uDice = (int)lceil(EST * maxusize / sqrtShadingRate);
     s.vDice = (int)lceil(EST * maxvsize / sqrtShadingRate);
     unsigned long toomuch = (unsigned long)s.uDice * s.vDice;
@@ -466,7 +562,7 @@ static bool bicubic_diceable(Surf& s, const Mat& matCtoR, const rh_prim& prim) {
         if (mag2(V2) > vLen) vLen = mag2(V2);
         if (mag2(V3_) > vLen) vLen = mag2(V3_);
     }
-    float shadingRate = adjusted_shading_rate(prim);
+    float shadingRate = adjusted_shading_rate(s, prim);
     uLen = std::sqrt(uLen / shadingRate);
     vLen = std::sqrt(vLen / shadingRate);
     s.splitDir = (uLen > vLen) ? SplitDir_U : SplitDir_V;
@@ -489,7 +585,7 @@ static bool bilinear_diceable(Surf& s, const Mat& matCtoR, const rh_prim& prim) 
     float uLen = (mag2(V1) > mag2(V2)) ? mag2(V1) : mag2(V2);
     V1 = hull[2] - hull[0]; V2 = hull[3] - hull[1];
     float vLen = (mag2(V1) > mag2(V2)) ? mag2(V1) : mag2(V2);
-    float shadingRate = adjusted_shading_rate(prim);
+    float shadingRate = adjusted_shading_rate(s, prim);
     uLen = std::sqrt(uLen / shadingRate);
     vLen = std::sqrt(vLen / shadingRate);
     s.splitDir = (uLen > vLen) ? SplitDir_U : SplitDir_V;
@@ -720,6 +816,7 @@ struct Hit {
 };
 struct Sample {
     V2 pos;
+    V2 dofOffset;   // lens position in the unit disk (setSamples, imagepixel.cpp:357)
     float time, lod;
     float occlZ;
     int occlIndex;
@@ -829,6 +926,7 @@ struct Hider {
     SamplerTables samp;
     std::vector<Bucket> buckets;  // row-major, cYBuckets * cXBuckets
     std::vector<Sample> samples;  // extended image, extW*extH*nsamp
+    std::vector<int> dofIndex;    // per pixel: m_DofOffsetIndices (lens cell -> sample)
     std::vector<uint8_t> pixelHasValid;
     std::vector<float> channels;  // display region, 9 per pixel
     std::vector<uint8_t> rgba8;   // display region
@@ -874,6 +972,15 @@ struct Hider {
         }
         float minz = B.mn.z, maxz = B.mx.z;
         B = bound_transform(B, F.camToRaster);
+        if (F.dof.on) {
+            // grown by the larger circle of confusion at its depths (imagebuffer.cpp:312-322)
+            const V2 minZCoc = circle_of_confusion(F.dof, minz), maxZCoc = circle_of_confusion(F.dof, maxz);
+            const float cocX = tmax(minZCoc.x, maxZCoc.x), cocY = tmax(minZCoc.y, maxZCoc.y);
+            B.mn.x = B.mn.x - cocX;
+            B.mn.y = B.mn.y - cocY;
+            B.mx.x = B.mx.x + cocX;
+            B.mx.y = B.mx.y + cocY;
+        }
         B.mn.x = B.mn.x - o.filter_xwidth / 2.0f;
         B.mn.y = B.mn.y - o.filter_ywidth / 2.0f;
         B.mx.x = B.mx.x + o.filter_xwidth / 2.0f;
@@ -945,6 +1052,13 @@ struct Hider {
     void add_mpg(MP* mp) {
         const rh_options& o = F.o;
         Bound B = mp->bound;
+        if (F.dof.on) {
+            // the larger circle of confusion at its two depths (imagebuffer.cpp:519-530)
+            const V2 c0 = circle_of_confusion(F.dof, B.mn.z), c1 = circle_of_confusion(F.dof, B.mx.z);
+            const V2 maxCoC = v2(tmax(c0.x, c1.x), tmax(c0.y, c1.y));
+            B.mn.x = B.mn.x - maxCoC.x; B.mn.y = B.mn.y - maxCoC.y; B.mn.z = B.mn.z - 0.0f;
+            B.mx.x = B.mx.x + maxCoC.x; B.mx.y = B.mx.y + maxCoC.y; B.mx.z = B.mx.z + 0.0f;
+        }
         if (B.mx.x < F.cropXMin - o.filter_xwidth / 2.0f || B.mx.y < F.cropYMin - o.filter_ywidth / 2.0f ||
             B.mn.x > F.cropXMax + o.filter_xwidth / 2.0f || B.mn.y > F.cropYMax + o.filter_ywidth / 2.0f)
             return;
@@ -1269,6 +1383,85 @@ struct Hider {
             }
     }
 
+    // RenderMPG_MBOrDof for a static micropolygon with depth of field
+    // (bucketprocessor.cpp:1221-1469, the UsingDof branches).  Each lens cell
+    // shifts the micropolygon's bound by the cell's bound of lens offsets
+    // times the larger circle of confusion at its depths; a pixel of the
+    // shifted bound offers the one sample whose lens offset lies in that
+    // cell (GetDofOffsetIndex).  Every visit counts into SPL_count.
+    void render_mp_dof(MP* mp) {
+        const Grid& g = *mp->g;
+        HitCache hc;
+        const int cu = g.cu;
+        // CacheHitTestValues with DoF (micropolygon.cpp:1394-1421)
+        hc.P[0] = g.P[mp->index];
+        hc.P[1] = g.P[mp->index + 1];
+        hc.P[2] = g.P[mp->index + cu + 1];
+        hc.P[3] = g.P[mp->index + cu + 2];
+        V2 cocMult[4];
+        for (int i = 0; i < 4; ++i) cocMult[i] = circle_of_confusion(F.dof, hc.P[i].z);
+        const V2 cm01 = v2(tmin(cocMult[0].x, cocMult[1].x), tmin(cocMult[0].y, cocMult[1].y));
+        const V2 cm23 = v2(tmin(cocMult[2].x, cocMult[3].x), tmin(cocMult[2].y, cocMult[3].y));
+        const V2 cocMultMin = v2(tmin(cm01.x, cm23.x), tmin(cm01.y, cm23.y));
+        const V2 cx01 = v2(tmax(cocMult[0].x, cocMult[1].x), tmax(cocMult[0].y, cocMult[1].y));
+        const V2 cx23 = v2(tmax(cocMult[2].x, cocMult[3].x), tmax(cocMult[2].y, cocMult[3].y));
+        const V2 cocMultMax = v2(tmax(cx01.x, cx23.x), tmax(cx01.y, cx23.y));
+        const bool isOpaque = g.Oi[0] >= 1.0f && g.Oi[1] >= 1.0f && g.Oi[2] >= 1.0f;
+        const Bound& Bd = mp->bound;   // SubBound(0) of a static micropolygon
+        const float bminz = Bd.mn.z, bmaxz = Bd.mx.z;
+        if (bminz > F.o.clip_far || bmaxz < F.o.clip_near) return;
+        const float mpgbminx = Bd.mn.x, mpgbmaxx = Bd.mx.x, mpgbminy = Bd.mn.y, mpgbmaxy = Bd.mx.y;
+        const V2 minZCoc = circle_of_confusion(F.dof, bminz), maxZCoc = circle_of_confusion(F.dof, bmaxz);
+        const float maxCocX = tmax(minZCoc.x, maxZCoc.x), maxCocY = tmax(minZCoc.y, maxZCoc.y);
+        for (int cell = 0; cell < F.nsamp; ++cell) {
+            const Bound& DB = F.dofBounds[cell];
+            const float leftOffset = DB.mx.x * maxCocX, rightOffset = DB.mn.x * maxCocX;
+            const float topOffset = DB.mx.y * maxCocY, bottomOffset = DB.mn.y * maxCocY;
+            const float bminx = mpgbminx - leftOffset, bmaxx = mpgbmaxx - rightOffset;
+            const float bminy = mpgbminy - topOffset, bmaxy = mpgbmaxy - bottomOffset;
+            int eX = (int)lceil(bmaxx), eY = (int)lceil(bmaxy);
+            if (eX > sMaxX) eX = sMaxX;
+            if (eY > sMaxY) eY = sMaxY;
+            int sX = static_cast<int>(std::floor(bminx)), sY = static_cast<int>(std::floor(bminy));
+            if (sY < sMinY) sY = sMinY;
+            if (sX < sMinX) sX = sMinX;
+            if (sX >= eX || sY >= eY) continue;
+            for (int iY = sY; iY < eY; ++iY)
+                for (int iX = sX; iX < eX; ++iX) {
+                    const int index = dofIndex[((size_t)(iY - F.extY0) * F.extW + (iX - F.extX0)) * F.nsamp + cell];
+                    Sample& sd = sample_at(iX, iY, index);
+                    const V2 vecP = sd.pos;
+                    st.spl_count++;
+                    // the shifted bound's Contains2D (bound.h:144-151)
+                    if (vecP.x < bminx || vecP.x > bmaxx || vecP.y < bminy || vecP.y > bmaxy) continue;
+                    if (Bd.mn.z > sd.occlZ) continue;
+                    if (0.0f > sd.lod || sd.lod >= 1.0f) continue;
+                    st.spl_bound_hits++;
+                    // CqMicroPolygon::Sample with DoF (micropolygon.cpp:1561-1593):
+                    // dofSampleInBound (:1531-1550) on the micropolygon's bound
+                    const V2 o = sd.dofOffset;
+                    V2 cocMin = v2(vecP.x + cocMultMin.x * o.x, vecP.y + cocMultMin.y * o.y);
+                    V2 cocMax = v2(vecP.x + cocMultMax.x * o.x, vecP.y + cocMultMax.y * o.y);
+                    if (o.x < 0) std::swap(cocMin.x, cocMax.x);
+                    if (o.y < 0) std::swap(cocMin.y, cocMax.y);
+                    if (cocMin.x > Bd.mx.x || cocMin.y > Bd.mx.y || cocMax.x < Bd.mn.x || cocMax.y < Bd.mn.y) continue;
+                    // the vertices moved against the lens offset, then the
+                    // point-in-polygon set-up on them
+                    V3 pts[4];
+                    for (int i = 0; i < 4; ++i)
+                        pts[i] = v3(hc.P[i].x - cocMult[i].x * o.x, hc.P[i].y - cocMult[i].y * o.y, hc.P[i].z - 0.0f);
+                    cache_point_in_poly(hc, pts, mp->indexCode);
+                    float D;
+                    V2 uv;
+                    if (contains(hc, vecP, D, uv)) {
+                        st.inside++;
+                        if (!hc.linear) st.inside_nonlinear++;
+                        store_sample(mp, iX, iY, index, D, uv, isOpaque);
+                    }
+                }
+        }
+    }
+
     // CqMicroPolygonMotion::BuildBoundList (micropolygon.cpp:1689-1756) for
     // two keys: the number of sub-bounds is the micropolygon's raster move in
     // its own widths, at most timeRanges; each sub-bound encapsulates the
@@ -1464,20 +1657,22 @@ struct Hider {
             for (int x = sMinX; x < sMaxX; ++x) {
                 const int* shuffled;
                 const V2* positions;
+                const V2* dofOffsets;
                 const float* times;
                 const float* lods;
                 if (samp.cache == 1) {
-                    shuffled = &samp.shuffled[0]; positions = &samp.pos2d[0];
+                    shuffled = &samp.shuffled[0]; positions = &samp.pos2d[0]; dofOffsets = &samp.pos2d[0];
                     times = &samp.pos1d[0]; lods = &samp.pos1d[0];
                 } else {
                     shuffled = &samp.shuffled[(size_t)n * rng.rint(250)];
                     positions = &samp.pos2d[(size_t)n * rng.rint(250)];
-                    (void)rng.rint(250);  // dof offsets (unused without DoF)
+                    dofOffsets = &samp.pos2d[(size_t)n * rng.rint(250)];
                     times = &samp.pos1d[(size_t)n * rng.rint(250)];
                     lods = &samp.pos1d[(size_t)n * rng.rint(250)];
                 }
-                (void)shuffled;
                 V2 off = v2((float)x, (float)y);
+                int* dofIdx = &dofIndex[((size_t)(y - F.extY0) * F.extW + (x - F.extX0)) * n];
+                for (int i = 0; i < n; ++i) dofIdx[i] = shuffled[i];
                 for (int i = 0; i < n; ++i) {
                     Sample& s = sample_at(x, y, i);
                     s.pos = off + positions[i];
@@ -1486,6 +1681,12 @@ struct Hider {
                     s.occlZ = FLT_MAX;
                     s.occl.flags = 0;
                     s.hits.clear();
+                }
+                // sample shuffled[i] takes lens offset i (imagepixel.cpp:357)
+                for (int i = 0; i < n; ++i) {
+                    const V2 d = dofOffsets[i];
+                    sample_at(x, y, shuffled[i]).dofOffset =
+                        project_to_circle(v2(-1.0f + 2.0f * d.x, -1.0f + 2.0f * d.y));
                 }
             }
         if (cull) setup_tree();
@@ -1504,9 +1705,10 @@ struct Hider {
     }
 
     void render_waiting(Bucket& b) {
-        // RenderMicroPoly (bucketprocessor.cpp:1067-1091): no DoF here
+        // RenderMicroPoly (bucketprocessor.cpp:1067-1091)
         for (size_t i = 0; i < b.mps.size(); ++i) {
             if (b.mps[i]->motion) render_mp_motion(b.mps[i]);
+            else if (F.dof.on) render_mp_dof(b.mps[i]);
             else render_mp_static(b.mps[i]);
         }
         for (MP* mp : b.mps) release_mp(mp);
@@ -1722,8 +1924,18 @@ int oracle_render(const rh_options* opt, const rh_prim* prims, int32_t nprims, i
     Frame& F = H->F;
     H->prims = prims;
     H->cull = occlusion_cull != 0;
+    // Depth of field together with motion blur (the motion Sample's DoF
+    // branch, micropolygon.cpp:1811-1870) is not restated.
+    if (F.dof.on)
+        for (int32_t i = 0; i < nprims; ++i)
+            if (prims[i].nkeys != 1) return -2;
+    g_dof = &F.dof;
+    struct DofReset { ~DofReset() { g_dof = nullptr; } } dofReset;
     H->samples.resize((size_t)F.extW * F.extH * F.nsamp);
-    for (auto& s : H->samples) { s.occlZ = FLT_MAX; s.occl.flags = 0; s.occlIndex = 0; s.pos = v2(0, 0); }
+    H->dofIndex.assign((size_t)F.extW * F.extH * F.nsamp, 0);
+    for (auto& s : H->samples) {
+        s.occlZ = FLT_MAX; s.occl.flags = 0; s.occlIndex = 0; s.pos = v2(0, 0); s.dofOffset = v2(0, 0);
+    }
     H->pixelHasValid.assign((size_t)F.extW * F.extH, 0);
     const int W = F.cropXMax - F.cropXMin, Hh = F.cropYMax - F.cropYMin;
     H->channels.assign((size_t)W * Hh * 9, 0.0f);
@@ -1831,6 +2043,7 @@ int oracle_sample_positions(const rh_options* opt, float* xy, float* dither) {
     H->prims = nullptr;
     H->cull = false;
     H->samples.resize((size_t)F.extW * F.extH * F.nsamp);
+    H->dofIndex.assign((size_t)F.extW * F.extH * F.nsamp, 0);
     H->pixelHasValid.assign((size_t)F.extW * F.extH, 0);
     const int W = F.cropXMax - F.cropXMin, Hh = F.cropYMax - F.cropYMin;
     H->channels.assign((size_t)W * Hh * 9, 0.0f);
@@ -1854,6 +2067,7 @@ int oracle_sample_times(const rh_options* opt, float* times) {
     H->prims = nullptr;
     H->cull = false;
     H->samples.resize((size_t)F.extW * F.extH * F.nsamp);
+    H->dofIndex.assign((size_t)F.extW * F.extH * F.nsamp, 0);
     H->pixelHasValid.assign((size_t)F.extW * F.extH, 0);
     const int W = F.cropXMax - F.cropXMin, Hh = F.cropYMax - F.cropYMin;
     H->channels.assign((size_t)W * Hh * 9, 0.0f);
@@ -1862,6 +2076,56 @@ int oracle_sample_times(const rh_options* opt, float* times) {
     H->set_image();
     H->render();
     for (size_t i = 0; i < H->samples.size(); ++i) times[i] = H->samples[i].time;
+    return 0;
+}
+
+// Per-sample lens offsets of the extended image (setSamples,
+// imagepixel.cpp:338-357): 2 floats per sample, in the unit disk, and the
+// lens cell of each sample.
+int oracle_sample_lens(const rh_options* opt, float* xy, int32_t* cell) {
+    std::unique_ptr<Hider> H(new Hider());
+    memset(&H->st, 0, sizeof(H->st));
+    setup_frame(*opt, H->F);
+    Frame& F = H->F;
+    H->prims = nullptr;
+    H->cull = false;
+    H->samples.resize((size_t)F.extW * F.extH * F.nsamp);
+    H->dofIndex.assign((size_t)F.extW * F.extH * F.nsamp, 0);
+    H->pixelHasValid.assign((size_t)F.extW * F.extH, 0);
+    const int W = F.cropXMax - F.cropXMin, Hh = F.cropYMax - F.cropYMin;
+    H->channels.assign((size_t)W * Hh * 9, 0.0f);
+    H->rgba8.assign((size_t)W * Hh * 4, 0);
+    H->ditherRec.assign((size_t)W * Hh, 0.0f);
+    H->set_image();
+    H->render();
+    for (size_t i = 0; i < H->samples.size(); ++i) {
+        xy[2 * i] = H->samples[i].dofOffset.x;
+        xy[2 * i + 1] = H->samples[i].dofOffset.y;
+    }
+    // the lens cell through which each sample is visited (GetDofOffsetIndex inverted)
+    if (cell)
+        for (size_t px = 0; px < (size_t)F.extW * F.extH; ++px)
+            for (int k = 0; k < F.nsamp; ++k) cell[px * F.nsamp + H->dofIndex[px * F.nsamp + k]] = k;
+    return 0;
+}
+
+// The frame's circle of confusion (x and y raster radii) at n camera
+// depths, its lens-cell bounds (4 floats per cell: min x, min y, max x,
+// max y) and MinCoCForBound of [z[i], z[i+1]] for i < n - 1.
+int oracle_dof(const rh_options* opt, int32_t n, const float* z, float* coc, float* cells, float* minCoc) {
+    Frame F;
+    setup_frame(*opt, F);
+    if (!F.dof.on) return -1;
+    for (int i = 0; i < n; ++i) {
+        V2 c = circle_of_confusion(F.dof, z[i]);
+        coc[2 * i] = c.x;
+        coc[2 * i + 1] = c.y;
+        if (i + 1 < n) minCoc[i] = min_coc_for_bound(F.dof, z[i], z[i + 1]);
+    }
+    for (int k = 0; k < F.nsamp; ++k) {
+        cells[4 * k] = F.dofBounds[k].mn.x; cells[4 * k + 1] = F.dofBounds[k].mn.y;
+        cells[4 * k + 2] = F.dofBounds[k].mx.x; cells[4 * k + 3] = F.dofBounds[k].mx.y;
+    }
     return 0;
 }
 

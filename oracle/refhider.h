@@ -19,7 +19,8 @@ int oracle_frame_size(const rh_options* opt, int32_t* width, int32_t* height,
 /* Render one frame on the CPU.  occlusion_cull = 1 follows the reference
  * exactly, including CqOcclusionTree surface culling and reposting
  * (bucketprocessor.cpp:947-960); 0 turns that culling off, which changes
- * only which occluded surfaces get diced and in what order.  Output
+ * only which occluded surfaces get diced and in what order.  Returns -2 for
+ * depth of field together with motion keys (not restated).  Output
  * buffers follow the layouts of rh_read_channels / rh_read_rgba8 /
  * rh_read_samples in aqsis_hider.h; any output pointer may be NULL. */
 int oracle_render(const rh_options* opt, const rh_prim* prims, int32_t nprims,
@@ -32,6 +33,14 @@ int oracle_render(const rh_options* opt, const rh_prim* prims, int32_t nprims,
 int oracle_sample_positions(const rh_options* opt, float* xy, float* dither);
 /* Shutter time of every sample of the extended image (ext pixels x spp). */
 int oracle_sample_times(const rh_options* opt, float* times);
+
+/* Lens offset (depth of field) of every sample of the extended image,
+ * 2 floats per sample, and the lens cell it is visited through (may be NULL). */
+int oracle_sample_lens(const rh_options* opt, float* xy, int32_t* cell);
+/* Depth of field of the frame: circle of confusion at n camera depths
+ * (coc[2n]), the lens-cell bounds (cells[4 * spp]) and MinCoCForBound of
+ * [z[i], z[i+1]] (minCoc[n-1]).  -1 when the frame has no depth of field. */
+int oracle_dof(const rh_options* opt, int32_t n, const float* z, float* coc, float* cells, float* minCoc);
 
 /* Camera-space grid points of one surface diced at u x v
  * (CqSurface::Dice): out[3*((v+1)*(u+1))]. */
