@@ -66,6 +66,10 @@ struct FrameDev {
     uint32_t hitCapacity;         // transparent-hit arena entries
     float shutterOpen, shutterClose;
     float clipNear, clipFar;
+    // depth of field (CqRenderer's lens state): on when dofOn != 0
+    int dofOn;
+    float dofMult, dofOneOverF;   // m_DofMultiplier, m_OneOverFocalDistance
+    float dofScaleX, dofScaleY;   // m_DepthOfFieldScale
 };
 
 struct GridDev {
@@ -123,6 +127,18 @@ struct SampleLaunch {
     int rowBase;
     uint32_t idCut;                       // only list entries below this id (0xffffffff: all)
     int depthOnly;                        // opaque depths only: no transparent hits are kept
+};
+
+// Lens tables of a frame with depth of field: per pattern the
+// getShuffledIndices table (lens cell -> sample index) and the lens offsets
+// projectToCircle(2p - 1); per extended pixel its lens-offset pattern; per
+// lens cell the bound of its offsets (CalculateDofBounds: min x, min y,
+// max x, max y).
+struct DofTables {
+    const uint8_t* shuffled;
+    const float2* lens;
+    const uint8_t* pixDof;
+    const float4* cells;
 };
 
 // Sampling kernel variants: LDS chunk of CH records, SLOTS blocks per wave.

@@ -29,6 +29,10 @@ bool make_shape(const rh_options& o, Shape& s, std::string& err) {
     if (!(o.fov > 0 && o.fov < 180)) { err = "fov must be in (0, 180)"; return false; }
     if (o.num_displays < 0 || o.num_displays > 16) { err = "num_displays out of range"; return false; }
     if (!(o.pixel_aspect > 0)) { err = "pixel aspect must be positive"; return false; }
+    if (o.dof[0] < FLT_MAX && !(o.dof[0] > 0 && o.dof[1] > 0 && o.dof[2] > 0)) {
+        err = "DepthOfField fstop, focal length and focal distance must be positive";
+        return false;
+    }
     s.o = o;
     // CqRenderer::initialiseCropWindow (renderer.cpp:1619-1626)
     s.cropX0 = (int)clampv<long>(ceil_l((float)o.xres * o.crop[0]), 0, o.xres);
@@ -76,6 +80,58 @@ bool make_shape(const rh_options& o, Shape& s, std::string& err) {
     s.camToRaster = m_mul(raster, m_identity());
     s.diceM = s.camToRaster;
     s.diceM.e[0][2] = s.diceM.e[1][2] = s.diceM.e[2][2] = s.diceM.e[3][2] = 0;
+
+    // Depth of field.  The CoC scale maps (1,1,-1) and (0,0,-1) through
+    // matVSpaceToSpace("camera", "raster"): the same product with its
+    // translation and projective column cleared (renderer.cpp:897-957,
+    // options.cpp:160-171).  The multiplier and inverse focal distance are
+    // computed in double and stored as floats (renderer.h:368-377).
+    {
+        m44 dm = s.camToRaster;
+        dm.e[3][0] = dm.e[3][1] = dm.e[3][2] = dm.e[0][3] = dm.e[1][3] = dm.e[2][3] = 0.0f;
+        dm.e[3][3] = 1.0f;
+        const f3 de = m_point(dm, mk(1, 1, -1)), dc = m_point(dm, mk(0, 0, -1));
+        s.dofScaleX = std::fabs(de.x - dc.x);
+        s.dofScaleY = std::fabs(de.y - dc.y);
+        s.dof = o.dof[0] < FLT_MAX;
+        s.dofCells.clear();
+        if (s.dof) {
+            const float fstop = o.dof[0], focalLength = o.dof[1], focalDistance = o.dof[2];
+            const float lensDiameter = focalLength / fstop;
+            s.dofMult = (float)(0.5 * lensDiameter * focalDistance / (focalDistance + lensDiameter));
+            s.dofOneOverF = (float)(1.0 / focalDistance);
+            // CalculateDofBounds (bucketprocessor.cpp:858-911): each cell of
+            // [-1,1]^2 bounded by its projected corners, or by the square's
+            // own extent across an axis it straddles.
+            s.dofCells.assign((size_t)s.ns * 4, 0.0f);
+            const float dx = (float)(2.0 / s.nx), dy = (float)(2.0 / s.ny);
+            float minX = -1.0f, minY = -1.0f;
+            for (int j = 0, which = 0; j < s.ny; ++j) {
+                for (int i = 0; i < s.nx; ++i, ++which) {
+                    float tl[2], tr[2], bl[2], br[2];
+                    project_to_circle(minX, minY, tl[0], tl[1]);
+                    project_to_circle(minX + dx, minY, tr[0], tr[1]);
+                    project_to_circle(minX, minY + dy, bl[0], bl[1]);
+                    project_to_circle(minX + dx, minY + dy, br[0], br[1]);
+                    if ((tl[1] > 0.0f && bl[1] < 0.0f) || (tl[1] < 0.0f && bl[1] > 0.0f)) {
+                        tl[0] = minX; bl[0] = minX; tr[0] = minX + dx; br[0] = minX + dx;
+                    }
+                    if ((tl[0] > 0.0f && tr[0] < 0.0f) || (tl[0] < 0.0f && tr[0] > 0.0f)) {
+                        tl[1] = minY; bl[1] = minY + dy; tr[1] = minY; br[1] = minY + dy;
+                    }
+                    float* c = &s.dofCells[4 * (size_t)which];
+                    c[0] = tl[0]; c[1] = tl[1]; c[2] = tl[0]; c[3] = tl[1];
+                    for (const float* q : {tr, bl, br}) {
+                        c[2] = mx(c[2], q[0]); c[3] = mx(c[3], q[1]);
+                        c[0] = mn(c[0], q[0]); c[1] = mn(c[1], q[1]);
+                    }
+                    minX += dx;
+                }
+                minX = -1.0f;
+                minY += dy;
+            }
+        }
+    }
 
     // Filter weights at subpixel centres (bucketprocessor.cpp:811-856)
     const int xm = s.shiftX, ym = s.shiftY;
@@ -154,6 +210,17 @@ bool make_shape(const rh_options& o, Shape& s, std::string& err) {
         s.taskStart.push_back((int)s.tasks.size());
     }
     return true;
+}
+
+void project_to_circle(float x, float y, float& ox, float& oy) {
+    float r;   // CqVector2D::Magnitude
+    if (y == 0.0f) r = std::fabs(x);
+    else if (x == 0.0f) r = std::fabs(y);
+    else r = std::sqrt((x * x) + (y * y));
+    if (r == 0.0f) { ox = oy = 0.0f; return; }
+    const float adj = mx(std::fabs(x), std::fabs(y)) / r;
+    ox = adj * x;
+    oy = adj * y;
 }
 
 // RiXxxFilter functions (libs/core/filters.cpp:72-342)
@@ -285,6 +352,7 @@ void make_plan(const Shape& s, Plan& p) {
             }
         }
         st.g.seed(19);
+        p.shuffled.assign(shuffled.begin(), shuffled.end());
     } else {
         // CqGridSampler (grid.cpp:37-63): no random draws at all.
         p.cache = 1;
@@ -298,11 +366,19 @@ void make_plan(const Shape& s, Plan& p) {
             }
         float dt = (float)(1 / ns), t = (float)(dt * 0.5);
         for (int i = 0; i < ns; ++i) { p.pos1d[i] = t; t += dt; }
+        p.shuffled.resize(ns);
+        for (int i = 0; i < ns; ++i) p.shuffled[i] = (uint8_t)i;
     }
+    // Lens offsets of each pattern: projectToCircle(-1 + 2 * offset)
+    // (imagepixel.cpp:357), offset k going to the sample shuffled[k].
+    p.lens.assign(p.pos2d.size(), 0.0f);
+    for (size_t k = 0; k + 1 < p.pos2d.size(); k += 2)
+        project_to_circle(-1.0f + 2.0f * p.pos2d[k], -1.0f + 2.0f * p.pos2d[k + 1], p.lens[k], p.lens[k + 1]);
     // Bucket loop order: per bucket, 5 draws per sample-region pixel
     // (imagepixel.cpp:338-347), then one draw per display pixel per display
     // request (ddmanager.cpp:1050).
     p.pixPattern.assign((size_t)s.extW * s.extH * 4, 0);
+    p.pixDof.assign((size_t)s.extW * s.extH, 0);
     p.dither.assign((size_t)s.width() * s.height(), 0.0f);
     for (const auto& b : s.buckets) {
         if (s.o.jitter) {
@@ -311,10 +387,11 @@ void make_plan(const Shape& s, Plan& p) {
                     uint8_t* q = &p.pixPattern[((size_t)(y - s.extY0) * s.extW + (x - s.extX0)) * 4];
                     int shuf = st.i(250);
                     int pos = st.i(250);
-                    (void)st.i(250);  // depth-of-field offsets
+                    int dof = st.i(250);
                     int tim = st.i(250);
                     int lod = st.i(250);
                     q[0] = (uint8_t)pos; q[1] = (uint8_t)tim; q[2] = (uint8_t)lod; q[3] = (uint8_t)shuf;
+                    p.pixDof[(size_t)(y - s.extY0) * s.extW + (x - s.extX0)] = (uint8_t)dof;
                 }
         }
         bool outside = (b.x + b.w) <= s.cropX0 || (b.y + b.h) <= s.cropY0 || b.x > s.cropX1 || b.y > s.cropY1;
@@ -473,6 +550,29 @@ struct Ctx {
     std::vector<m44> objToCam1;  // key 1 of moving spheres
 };
 
+// CqSurface::AdjustedShadingRate (geometry/surface.cpp:688-724) without the
+// motion factor: with depth of field, ShadingRate times max(1, 0.025 *
+// GeometricFocusFactor * MinCoCForBound^2) over the cached raster bound's
+// depths (MinCoCForBound, renderer.cpp:1602-1617: 0 across the focal plane,
+// else the nearer end's least radius).
+float adjusted_rate(const Node& n, const Ctx& c) {
+    const Shape& s = *c.s;
+    const rh_prim& p = c.prims[n.prim];
+    float rate = p.shading_rate;
+    if (s.dof) {
+        const float z1 = n.rb.lo.z, z2 = n.rb.hi.z;
+        const float focalDist = 1 / s.dofOneOverF;
+        float minCoC = 0.0f;
+        if (!((z1 - focalDist) * (z2 - focalDist) < 0)) {
+            const float minBlur = mn(std::fabs(1 / z1 - s.dofOneOverF), std::fabs(1 / z2 - s.dofOneOverF));
+            minCoC = s.dofMult * mn(s.dofScaleX, s.dofScaleY) * minBlur;
+        }
+        const float focusFactor = p.focus_factor, areaRatio = 0.025f;
+        rate *= mx(1.0f, areaRatio * focusFactor * minCoC * minCoC);
+    }
+    return rate;
+}
+
 // CqQuadric::EstimateGridSize + Diceable (quadrics.cpp:259-346)
 bool sphere_diceable(Node& n, const Ctx& c) {
     if (!n.diceableFlag) return false;
@@ -495,7 +595,7 @@ bool sphere_diceable(Node& n, const Ctx& c) {
             prev = p;
         }
     su = std::sqrt(su); sv = std::sqrt(sv);
-    float srt = std::sqrt(c.prims[n.prim].shading_rate);
+    float srt = std::sqrt(adjusted_rate(n, c));
     n.uDice = (int)ceil_l(E * su / srt);
     n.vDice = (int)ceil_l(E * sv / srt);
     unsigned long too = (unsigned long)n.uDice * n.vDice;
@@ -521,7 +621,7 @@ bool bicubic_diceable(Node& n, const Ctx& c) {
             float d = len2(sub(h[v + 4 * k + 4], h[v + 4 * k]));
             if (d > vl) vl = d;
         }
-    float sr = c.prims[n.prim].shading_rate;
+    float sr = adjusted_rate(n, c);
     ul = std::sqrt(ul / sr); vl = std::sqrt(vl / sr);
     n.splitDir = ul > vl ? DIR_U : DIR_V;
     ul *= 3; vl *= 3;
@@ -541,7 +641,7 @@ bool bilinear_diceable(Node& n, const Ctx& c) {
     float ul = a > b ? a : b;
     a = len2(sub(h[2], h[0])); b = len2(sub(h[3], h[1]));
     float vl = a > b ? a : b;
-    float sr = c.prims[n.prim].shading_rate;
+    float sr = adjusted_rate(n, c);
     ul = std::sqrt(ul / sr); vl = std::sqrt(vl / sr);
     n.splitDir = ul > vl ? DIR_U : DIR_V;
     ul = (float)mx<long>(round_l(ul), 1);
@@ -610,6 +710,17 @@ void post_geometry(Node& n, const f3* cp1, const Ctx& c) {
     }
     float z0 = B.lo.z, z1 = B.hi.z;
     B = box_xform(B, s.camToRaster);
+    if (s.dof) {
+        // grown by the larger circle of confusion at its depths (imagebuffer.cpp:312-322)
+        float c0x, c0y, c1x, c1y;
+        circle_of_confusion(s, z0, c0x, c0y);
+        circle_of_confusion(s, z1, c1x, c1y);
+        const float cocX = mx(c0x, c1x), cocY = mx(c0y, c1y);
+        B.lo.x = B.lo.x - cocX;
+        B.lo.y = B.lo.y - cocY;
+        B.hi.x = B.hi.x + cocX;
+        B.hi.y = B.hi.y + cocY;
+    }
     B.lo.x = B.lo.x - o.filter_xwidth / 2.0f;
     B.lo.y = B.lo.y - o.filter_ywidth / 2.0f;
     B.hi.x = B.hi.x + o.filter_xwidth / 2.0f;
@@ -774,6 +885,10 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
             return false;
         }
         if (p.nkeys != 1 && p.nkeys != 2) { err = "nkeys must be 1 or 2"; return false; }
+        if (p.nkeys == 2 && s.dof) {
+            err = "depth of field together with motion keys is not supported";
+            return false;
+        }
         if (p.nkeys == 2 && !(p.key_time[0] < p.key_time[1])) {
             err = "motion key times must increase";
             return false;

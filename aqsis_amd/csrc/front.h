@@ -2,6 +2,7 @@
 // schedule that the HIP kernels consume.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -46,9 +47,24 @@ struct Shape {
     };
     std::vector<TaskInfo> tasks;
     std::vector<int> taskStart;           // buckets + 1
+    // Depth of field (RiDepthOfField; CqRenderer's lens state,
+    // renderer.h:368-396): on when fstop < FLT_MAX.
+    bool dof = false;
+    float dofMult = 0, dofOneOverF = 0;   // m_DofMultiplier, m_OneOverFocalDistance
+    float dofScaleX = 0, dofScaleY = 0;   // m_DepthOfFieldScale
+    std::vector<float> dofCells;          // ns x (min x, min y, max x, max y): CalculateDofBounds
 };
 
 bool make_shape(const rh_options& o, Shape& s, std::string& err);
+
+// GetCircleOfConfusion (renderer.h:401-406): raster radii at a camera depth.
+inline void circle_of_confusion(const Shape& s, float depth, float& cx, float& cy) {
+    const float c = s.dofMult * std::fabs(1.0f / depth - s.dofOneOverF);
+    cx = s.dofScaleX * c;
+    cy = s.dofScaleY * c;
+}
+// CqImagePixel::projectToCircle (imagepixel.h:429-436)
+void project_to_circle(float x, float y, float& ox, float& oy);
 
 // Sample plan: the pattern tables and per-pixel pattern choices that
 // aqsis's global random stream produces for this frame shape.
@@ -58,6 +74,10 @@ struct Plan {
     std::vector<float> pos1d;             // cache * ns
     std::vector<uint8_t> pixPattern;      // ext pixels * 4: position, time, lod, shuffle
     std::vector<float> dither;            // display pixels: display 0's RandomFloat()
+    // depth of field
+    std::vector<uint8_t> shuffled;        // cache * ns: getShuffledIndices tables
+    std::vector<float> lens;              // cache * ns * 2: projectToCircle(2 * pos2d - 1)
+    std::vector<uint8_t> pixDof;          // ext pixels: the pattern of the lens offsets
 };
 
 void make_plan(const Shape& s, Plan& p);

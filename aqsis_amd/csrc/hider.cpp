@@ -27,6 +27,7 @@ extern "C" hipError_t rh_launch_bin(hipStream_t, const FrameDev*, const BucketDe
                                     const unsigned long long*, const float4*, uint32_t*, uint32_t*, uint32_t*,
                                     uint32_t*, unsigned long long*, int, int, const float4*);
 extern "C" hipError_t rh_launch_sample(hipStream_t, int, int, const SampleLaunch*);
+extern "C" hipError_t rh_launch_sample_dof(hipStream_t, const SampleLaunch*, int, const DofTables*);
 extern "C" hipError_t rh_sample_occupancy(int, int*);
 extern "C" hipError_t rh_launch_plan_items(hipStream_t, const FrameDev*, const TaskDev*, int, const uint32_t*,
                                            uint32_t, int, uint2*, uint32_t*, uint32_t*);
@@ -138,6 +139,7 @@ struct rh_ctx {
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // frame shape
     DBuf dFrame, dBuckets, dTasks, dPos2d, dPos1d, dPix, dDither, dFilter, dBoxMask;
+    DBuf dShuffled, dLens, dPixDof, dDofCells;   // depth of field: lens tables (DofTables)
     bool tiledFilter = false;
     // schedule
     DBuf dGrids, dParams, dPrims, dCandStart, dCandGrid, dCapStart, dSegBucket, dSegCounts;
@@ -164,8 +166,17 @@ struct rh_ctx {
     }
     FrameDev frame;                       // host copy of dFrame
     rh_stats stats;
+    DofTables dof() const {
+        DofTables t;
+        t.shuffled = dShuffled.as<uint8_t>();
+        t.lens = dLens.as<float2>();
+        t.pixDof = dPixDof.as<uint8_t>();
+        t.cells = dDofCells.as<float4>();
+        return t;
+    }
     template <typename F> void each_buffer(F f) {
         DBuf* all[] = {&dFrame, &dBuckets, &dTasks, &dPos2d, &dPos1d, &dPix, &dDither, &dFilter, &dBoxMask,
+                       &dShuffled, &dLens, &dPixDof, &dDofCells,
                        &dGrids, &dParams,
                        &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dMprec, &dMpbound, &dLists,
                        &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dBucketRefs,
@@ -247,7 +258,8 @@ PlanCache& plan_cache() {
 constexpr size_t kPlanCacheBytes = size_t(2) << 30;
 
 size_t plan_bytes(const rh::Plan& p) {
-    return (p.pos2d.size() + p.pos1d.size() + p.dither.size()) * sizeof(float) + p.pixPattern.size();
+    return (p.pos2d.size() + p.pos1d.size() + p.dither.size() + p.lens.size()) * sizeof(float) +
+           p.pixPattern.size() + p.pixDof.size() + p.shuffled.size();
 }
 
 // Everything make_plan reads from a shape.
@@ -323,6 +335,9 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
     f.hitCapacity = 0;
     f.shutterOpen = opt->shutter_open; f.shutterClose = opt->shutter_close;
     f.clipNear = opt->clip_near; f.clipFar = opt->clip_far;
+    f.dofOn = s.dof ? 1 : 0;
+    f.dofMult = s.dofMult; f.dofOneOverF = s.dofOneOverF;
+    f.dofScaleX = s.dofScaleX; f.dofScaleY = s.dofScaleY;
     c->frame = f;
 
     std::vector<BucketDev> bk;
@@ -361,7 +376,11 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
         (e = c->dListCount.ensure(s.tasks.size() * sizeof(uint32_t))) != hipSuccess ||
         (e = c->dBucketRefs.ensure(s.buckets.size() * sizeof(uint32_t))) != hipSuccess ||
         (e = c->dHitCtl.ensure(4 * sizeof(uint32_t))) != hipSuccess ||
-        (e = upload(c->dCounters, zero, c->st)) != hipSuccess) {
+        (e = upload(c->dCounters, zero, c->st)) != hipSuccess ||
+        (s.dof && ((e = upload(c->dShuffled, c->plan.shuffled, c->st)) != hipSuccess ||
+                   (e = upload(c->dLens, c->plan.lens, c->st)) != hipSuccess ||
+                   (e = upload(c->dPixDof, c->plan.pixDof, c->st)) != hipSuccess ||
+                   (e = upload(c->dDofCells, s.dofCells, c->st)) != hipSuccess))) {
         delete c;
         return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_DEVICE, std::string("device set-up: ") + hipGetErrorString(e));
     }
@@ -609,6 +628,11 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
         // sampler, whatever the layout.
         ENQ(rh_launch_sample_mb(c->st, &a, nt, c->dMprecMB.as<float4>()));
         rows = nt;
+    } else if (s.dof) {
+        // Depth of field: the lens form of the sampler, whatever the layout.
+        const DofTables dt = c->dof();
+        ENQ(rh_launch_sample_dof(c->st, &a, nt, &dt));
+        rows = nt;
     } else if (tu.mode == RH_MODE_PLAIN) {
         // One launch per bucket, in bucket order.
         for (int b = 0; b < nb; ++b) {
@@ -694,6 +718,10 @@ static hipError_t enqueue_depth_pass(rh_ctx* c, bool sample) {
     SampleLaunch a = depth_launch(c);
     const int nt = (int)s.tasks.size();
     if (S.motion) return rh_launch_sample_mb(c->st, &a, nt, c->dMprecMB.as<float4>());
+    if (s.dof) {
+        const DofTables dt = c->dof();
+        return rh_launch_sample_dof(c->st, &a, nt, &dt);
+    }
     return rh_launch_sample(c->st, 0, nt, &a);
 }
 
@@ -765,8 +793,14 @@ static rh_status cull_schedule(rh_ctx* c, const rh::Forest& forest) {
                 a.nStatic = t1 - t0;
                 a.rowBase = t0;
                 a.idCut = pr.mpCut;
-                if (c->sched.motion) HIP_TRY(rh_launch_sample_mb(c->st, &a, t1 - t0, c->dMprecMB.as<float4>()));
-                else HIP_TRY(rh_launch_sample(c->st, 0, t1 - t0, &a));
+                if (c->sched.motion) {
+                    HIP_TRY(rh_launch_sample_mb(c->st, &a, t1 - t0, c->dMprecMB.as<float4>()));
+                } else if (s.dof) {
+                    const DofTables dt = c->dof();
+                    HIP_TRY(rh_launch_sample_dof(c->st, &a, t1 - t0, &dt));
+                } else {
+                    HIP_TRY(rh_launch_sample(c->st, 0, t1 - t0, &a));
+                }
             }
             HIP_TRY(rh_launch_cull_probe_max(c->st, c->dFrame.as<FrameDev>(), c->dCullTrees.as<CullTreeDev>(),
                                              pr.bucket, t1 > t0 ? pr.kind : 0, pr.cx0, pr.cx1, pr.cy0, pr.cy1,

@@ -351,7 +351,19 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
         out[3] = make_float4(bminx, bmaxx, bminy, bmaxy);
         out[4] = make_float4(bminz, __uint_as_float(flags), __int_as_float((int)blockIdx.x), 0.0f);
         if (!mprecMB) {
-            mpbound[G.mpBase + m] = make_float4(bminx, bmaxx, bminy, bmaxy);
+            if (fr->dofOn) {
+                // AddMPG grows the bound by the larger circle of confusion at
+                // its two depths (imagebuffer.cpp:519-530); binning reads it
+                // for the bucket test and the pixel window.
+                const float bmaxz = fmax_ref(fmax_ref(fmax_ref(pA.z, pB.z), pC.z), pD.z);
+                const float c0 = fr->dofMult * fabsf(1.0f / bminz - fr->dofOneOverF);
+                const float c1 = fr->dofMult * fabsf(1.0f / bmaxz - fr->dofOneOverF);
+                const float cx = fmax_ref(fr->dofScaleX * c0, fr->dofScaleX * c1);
+                const float cy = fmax_ref(fr->dofScaleY * c0, fr->dofScaleY * c1);
+                mpbound[G.mpBase + m] = make_float4(bminx - cx, bmaxx + cx, bminy - cy, bmaxy + cy);
+            } else {
+                mpbound[G.mpBase + m] = make_float4(bminx, bmaxx, bminy, bmaxy);
+            }
             continue;
         }
         // Motion frames: every micropolygon also gets the record the motion
@@ -1922,6 +1934,365 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Sampling with depth of field
+// ---------------------------------------------------------------------------
+// GetCircleOfConfusion (renderer.h:401-406): raster radii at a camera depth.
+__device__ __forceinline__ float2 dof_coc(const FrameDev* __restrict__ fr, float depth) {
+    const float c = fr->dofMult * fabsf(1.0f / depth - fr->dofOneOverF);
+    return make_float2(fr->dofScaleX * c, fr->dofScaleY * c);
+}
+
+// CqMicroPolygon::Sample with depth of field (micropolygon.cpp:1561-1593)
+// after dofSampleInBound: the four grid-order vertices moved against the
+// sample's lens offset, cachePointInPolyTest's edges and CqInvBilinear's
+// set-up on them (micropolygon.cpp:1346-1392, bilinear.h:260-275; the
+// linear test of the moved quad decides one or two Newton steps), fContains
+// (:1295-1342), the inverse bilinear (bilinear.h:277-305) and the depth
+// bilerp (:228-236).
+__device__ __forceinline__ bool dof_quad_hit(const float* qx, const float* qy, const float4 z, uint32_t code,
+                                             bool degen, float px, float py, float& u, float& v, float& D,
+                                             bool& linear) {
+    float ox[4], oy[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int o = i < 3 ? (code >> (2 * (i + 1))) & 3 : code & 3;
+        ox[i] = pick4(o, qx[0], qx[1], qx[2], qx[3]);
+        oy[i] = pick4(o, qy[0], qy[1], qy[2], qy[3]);
+    }
+    float YM[4], XM[4], X[4], Y[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int j = (i + 3) & 3;
+        YM[i] = ox[i] - ox[j];
+        XM[i] = oy[i] - oy[j];
+        X[i] = ox[j];
+        Y[i] = oy[j];
+    }
+    if (degen) {
+#pragma unroll
+        for (int i = 2; i < 4; ++i) {
+            YM[i] = ox[3] - ox[1];
+            XM[i] = oy[3] - oy[1];
+            X[i] = ox[1];
+            Y[i] = oy[1];
+        }
+    }
+    const float t0 = ((py - Y[0]) * YM[0]) - ((px - X[0]) * XM[0]);
+    const float t1 = ((py - Y[1]) * YM[1]) - ((px - X[1]) * XM[1]);
+    const float t2 = ((py - Y[2]) * YM[2]) - ((px - X[2]) * XM[2]);
+    const float t3 = ((py - Y[3]) * YM[3]) - ((px - X[3]) * XM[3]);
+    if (t0 <= 0 || t1 <= 0 || t2 < 0 || t3 < 0) return false;
+    const float Ax = qx[0], Ay = qy[0];
+    const float Ex = qx[1] - qx[0], Ey = qy[1] - qy[0];
+    const float Fx = qx[2] - qx[0], Fy = qy[2] - qy[0];
+    const float Gx = ((-Ex) - qx[2]) + qx[3], Gy = ((-Ey) - qy[2]) + qy[3];
+    const float patchSize = fmax_ref(fmax_ref(fabsf(Fx), fabsf(Fy)), fmax_ref(fabsf(Ex), fabsf(Ey)));
+    const float irregularity = fmax_ref(fabsf(Gx), fabsf(Gy));
+    linear = (double)irregularity < 1e-2 * (double)patchSize;
+    u = 0.5f;
+    v = 0.5f;
+    {
+        const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
+        const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
+        const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px;
+        const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py;
+        float det = M1x * M2y - M1y * M2x;
+        det = 1 / det;
+        u = u - det * (bx * M2y - by * M2x);
+        v = v - det * (-(bx * M1y - by * M1x));
+    }
+    if (!linear) {
+        const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
+        const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
+        const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px;
+        const float by = (((Ay + Ey * u) + Fy * v) + (Gy * u) * v) - py;
+        float det = M1x * M2y - M1y * M2x;
+        if (det != 0) det = 1 / det;
+        u = u - det * (bx * M2y - by * M2x);
+        v = v - det * (-(bx * M1y - by * M1x));
+    }
+    const float w0 = (1 - v) * (1 - u), w1 = (1 - v) * u, w2 = v * (1 - u), w3 = v * u;
+    D = ((w0 * z.x + w1 * z.y) + w2 * z.z) + w3 * z.w;
+    return true;
+}
+
+#define RH_DOF_CH 128
+#define RH_DOF_WAVES 4
+// RenderMPG_MBOrDof for static micropolygons with depth of field
+// (bucketprocessor.cpp:1221-1469, the UsingDof branches).  A wave block is
+// one lens cell of an 8 x 8 pixel tile of a task: lane = pixel, and the
+// pixel's sample is the one visited through that cell (GetDofOffsetIndex:
+// the pixel's shuffle table entry of the cell), whose lens offset is the
+// cell's entry of the pixel's dof pattern.  Per block and micropolygon, in
+// list order: the bound shifted by the cell's bound of offsets times the
+// larger circle of confusion at the micropolygon's depths gives the pixel
+// window (every pixel of it counts into SPL_count); the shifted bound's
+// Contains2D and the LOD range are order-free; then each lane, in list
+// order: occlusion skip, SPL_bound_hits, dofSampleInBound (micropolygon.cpp:
+// 1531-1550) against the micropolygon's bound, the moved-quad hit test and
+// StoreSample.  One workgroup of four waves per task.
+__global__ void __launch_bounds__(RH_DOF_WAVES * 64)
+rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets, const TaskDev* __restrict__ tasks,
+              int itemBase, const unsigned long long* __restrict__ capStart, const uint32_t* __restrict__ listCount,
+              const uint32_t* __restrict__ lists, const float4* __restrict__ mprec4, const GridDev* __restrict__ grids,
+              const PrimDev* __restrict__ prims, const float* __restrict__ pos2d, const float* __restrict__ pos1d,
+              const uchar4* __restrict__ pixPattern, const uint8_t* __restrict__ shuffled,
+              const float2* __restrict__ lensTab, const uint8_t* __restrict__ pixDof, const float4* __restrict__ cells,
+              float4* __restrict__ samples, float4* __restrict__ hitData, uint32_t* __restrict__ hitNext,
+              uint32_t* __restrict__ hitCounter, uint4* __restrict__ combineList, float* __restrict__ frontDepth,
+              uint32_t* __restrict__ rowCounts, int rowBase, uint32_t idCut, int depthOnly) {
+    __shared__ float4 recs[RH_DOF_CH * 9];
+    __shared__ uint8_t matched[RH_DOF_WAVES][RH_DOF_CH];
+    __shared__ unsigned long long matM[RH_DOF_WAVES][RH_DOF_CH];
+    __shared__ float4 matB[RH_DOF_WAVES][RH_DOF_CH];
+    __shared__ uint32_t wsum[RH_DOF_WAVES][5];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ns = fr->ns;
+    const TaskDev T = tasks[itemBase + blockIdx.x];
+    const BucketDev B = buckets[T.bucket];
+    const int tilesX = (T.x1 - T.x0 + 7) >> 3, tilesY = (T.y1 - T.y0 + 7) >> 3;
+    const int nblk = tilesX * tilesY * ns;
+    const uint32_t* L = lists + capStart[itemBase + blockIdx.x];
+    const uint32_t n = list_cut(L, listCount[itemBase + blockIdx.x], idCut);
+    const unsigned long long rowRep = 0x0101010101010101ull;   // bit 8r of every row r
+    const int lx = lane & 7, ly = lane >> 3;
+    uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
+    uint32_t wbBase = 0;
+    int wbLeft = 0;
+    bool lost = false;
+
+    // every wave runs the same number of passes (the staging barriers)
+    for (int p0 = 0; p0 < nblk; p0 += RH_DOF_WAVES) {
+        const int blk = p0 + wave;
+        const bool has = blk < nblk;                     // uniform across the wave
+        const int tile = has ? blk / ns : 0, cell = has ? blk - tile * ns : 0;
+        const int tx0 = T.x0 + (tile % tilesX) * 8, ty0 = T.y0 + (tile / tilesX) * 8;
+        const int px = tx0 + lx, py = ty0 + ly;
+        const bool live = has && px < T.x1 && py < T.y1;
+        float sx = 0.f, sy = 0.f, lox = 0.f, loy = 0.f;
+        bool lodOk = false;
+        size_t sidx = 0;
+        if (live) {
+            const size_t pix = (size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0);
+            const uchar4 pat = pixPattern[pix];
+            const int si = shuffled[(size_t)pat.w * ns + cell];
+            sx = (float)px + pos2d[2 * ((size_t)pat.x * ns + si)];
+            sy = (float)py + pos2d[2 * ((size_t)pat.x * ns + si) + 1];
+            const float lod = pos1d[(size_t)pat.z * ns + si];
+            lodOk = !(0.0f > lod || lod >= 1.0f);
+            const float2 lo = lensTab[(size_t)pixDof[pix] * ns + cell];
+            lox = lo.x;
+            loy = lo.y;
+            sidx = pix * ns + si;
+        }
+        const float4 DB = has ? cells[cell] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const unsigned long long liveM = __ballot(live), inM = __ballot(live && lodOk);
+        const int BX0 = tx0 - B.sx0, BY0 = ty0 - B.sy0;
+        float occlZ = FLT_MAX, wu = 0.f, wv = 0.f;
+        uint32_t win = 0xffffffffu, head = 0xffffffffu;
+        int nh = 0;
+        for (uint32_t c0 = 0; c0 < n; c0 += RH_DOF_CH) {
+            const int cnt = (int)((n - c0) < (uint32_t)RH_DOF_CH ? (n - c0) : (uint32_t)RH_DOF_CH);
+            __syncthreads();
+            if (tid < cnt) {
+                const uint32_t id = L[c0 + tid];
+                const float4* src = mprec4 + (size_t)id * (RH_MPREC_FLOATS / 4);
+                const float4 a = src[0], b = src[1], zz = src[2], bd = src[3], r4 = src[4];
+                // CacheHitTestValues with DoF (micropolygon.cpp:1394-1421):
+                // each vertex's CoC, their component-wise least and largest
+                const float2 c0v = dof_coc(fr, zz.x), c1v = dof_coc(fr, zz.y), c2v = dof_coc(fr, zz.z),
+                             c3v = dof_coc(fr, zz.w);
+                const float cminx = fmin_ref(fmin_ref(c0v.x, c1v.x), fmin_ref(c2v.x, c3v.x));
+                const float cminy = fmin_ref(fmin_ref(c0v.y, c1v.y), fmin_ref(c2v.y, c3v.y));
+                const float cmaxx = fmax_ref(fmax_ref(c0v.x, c1v.x), fmax_ref(c2v.x, c3v.x));
+                const float cmaxy = fmax_ref(fmax_ref(c0v.y, c1v.y), fmax_ref(c2v.y, c3v.y));
+                // the larger CoC at the bound's depths (CalculateBound's
+                // order: A, B, C = vertex 3, D = vertex 2)
+                const float bmaxz = fmax_ref(fmax_ref(fmax_ref(zz.x, zz.y), zz.w), zz.z);
+                const float2 cz0 = dof_coc(fr, r4.x), cz1 = dof_coc(fr, bmaxz);
+                float4* R = &recs[tid * 9];
+                R[0] = a; R[1] = b; R[2] = zz; R[3] = bd;
+                R[4] = make_float4(r4.x, r4.y, r4.z, __uint_as_float(id));
+                R[5] = make_float4(c0v.x, c0v.y, c1v.x, c1v.y);
+                R[6] = make_float4(c2v.x, c2v.y, c3v.x, c3v.y);
+                R[7] = make_float4(cminx, cminy, cmaxx, cmaxy);
+                R[8] = make_float4(fmax_ref(cz0.x, cz1.x), fmax_ref(cz0.y, cz1.y),
+                                   (bmaxz < fr->clipNear || r4.x > fr->clipFar) ? 1.0f : 0.0f, 0.0f);
+            }
+            __syncthreads();
+            if (!has) continue;
+            float zmax = live ? occlZ : -FLT_MAX;
+            for (int off = 32; off > 0; off >>= 1) zmax = fmaxf(zmax, __shfl_xor(zmax, off));
+            // (1) the chunk's micropolygons whose window for this cell meets
+            // the tile, in list order, with their lanes and shifted bounds;
+            // those beyond every lane's opaque depth are only counted
+            int nm = 0;
+            for (int j0 = 0; j0 < cnt; j0 += 64) {
+                const int j = j0 + lane;
+                unsigned long long m = 0ull;
+                float4 sb = make_float4(0.f, 0.f, 0.f, 0.f);
+                bool ok = false;
+                if (j < cnt) {
+                    const float4 bd = recs[j * 9 + 3], r8 = recs[j * 9 + 8];
+                    if (r8.z == 0.0f) {            // outside the clipping range: skipped
+                        const float bminx = bd.x - DB.z * r8.x, bmaxx = bd.y - DB.x * r8.x;
+                        const float bminy = bd.z - DB.w * r8.y, bmaxy = bd.w - DB.y * r8.y;
+                        int eX = lceil_i(bmaxx), eY = lceil_i(bmaxy);
+                        if (eX > B.sx1) eX = B.sx1;
+                        if (eY > B.sy1) eY = B.sy1;
+                        int sX = floor_i(bminx), sY = floor_i(bminy);
+                        if (sY < B.sy0) sY = B.sy0;
+                        if (sX < B.sx0) sX = B.sx0;
+                        if (sX < eX && sY < eY) {
+                            const uint2 mt = make_uint2((uint32_t)(sX - B.sx0) | ((uint32_t)(eX - B.sx0) << 16),
+                                                        (uint32_t)(sY - B.sy0) | ((uint32_t)(eY - B.sy0) << 16));
+                            m = block_rect_lanes(mt, BX0, BY0, 8, 8, rowRep) & liveM;
+                        }
+                        sb = make_float4(bminx, bmaxx, bminy, bmaxy);
+                    }
+                    nTest += (uint32_t)__popcll(m);
+                    ok = m != 0ull && !(recs[j * 9 + 4].x > zmax);
+                }
+                const unsigned long long msk = __ballot(ok);
+                if (ok) {
+                    const int slot = nm + __popcll(msk & ((1ull << lane) - 1ull));
+                    matched[wave][slot] = (uint8_t)j;
+                    matM[wave][slot] = m & inM;
+                    matB[wave][slot] = sb;
+                }
+                nm += __popcll(msk);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int i0 = 0; i0 < nm; i0 += 64) {
+                const int ni = (nm - i0) < 64 ? (nm - i0) : 64;
+                // (2) order-free: the lane's sample in the micropolygon's
+                // window, inside its shifted bound, LOD in range
+                unsigned long long pb = 0ull;
+                for (int i = 0; i < ni; ++i) {
+                    const unsigned long long mm = matM[wave][i0 + i];
+                    const float4 sb = matB[wave][i0 + i];
+                    const bool inB = !(sx < sb.x) & !(sx > sb.y) & !(sy < sb.z) & !(sy > sb.w);
+                    const bool in = ((mm >> lane) & 1ull) != 0ull && inB;
+                    pb |= (unsigned long long)in << i;
+                }
+                // (3) ordered: each lane its own candidates in list order
+                while (__ballot(pb != 0ull)) {
+                    bool hit = false;
+                    float D = 0.f, u = 0.5f, v = 0.5f;
+                    uint32_t hflags = 0u, hid = 0u, hgrid = 0u;
+                    if (pb != 0ull) {
+                        const int i = __ffsll((long long)pb) - 1;
+                        pb &= pb - 1ull;
+                        const float4* R = &recs[matched[wave][i0 + i] * 9];
+                        const float4 r4 = R[4];
+                        // occlusion skip against the current opaque hit (bucketprocessor.cpp:1396-1397)
+                        if (!(r4.x > occlZ)) {
+                            ++nBound;
+                            // dofSampleInBound (micropolygon.cpp:1531-1550)
+                            const float4 cm = R[7], bd = R[3];
+                            float cMinX = sx + cm.x * lox, cMinY = sy + cm.y * loy;
+                            float cMaxX = sx + cm.z * lox, cMaxY = sy + cm.w * loy;
+                            if (lox < 0) { const float t = cMinX; cMinX = cMaxX; cMaxX = t; }
+                            if (loy < 0) { const float t = cMinY; cMinY = cMaxY; cMaxY = t; }
+                            if (!(cMinX > bd.y || cMinY > bd.w || cMaxX < bd.x || cMaxY < bd.z)) {
+                                const float4 a = R[0], b = R[1], c01 = R[5], c23 = R[6];
+                                // the vertices moved against the lens offset
+                                const float qx[4] = {a.x - c01.x * lox, a.z - c01.z * lox, b.x - c23.x * lox,
+                                                     b.z - c23.z * lox};
+                                const float qy[4] = {a.y - c01.y * loy, a.w - c01.w * loy, b.y - c23.y * loy,
+                                                     b.w - c23.w * loy};
+                                hflags = __float_as_uint(r4.y);
+                                bool lin = false;
+                                if (dof_quad_hit(qx, qy, R[2], hflags & 0xffu, ((hflags >> 8) & 1u) != 0u, sx, sy,
+                                                 u, v, D, lin)) {
+                                    ++nIn;
+                                    if (!lin) ++nIn2;
+                                    // StoreSample (bucketprocessor.cpp:1471-1569): reject when occlZ <= D
+                                    hit = !(occlZ <= D);
+                                    hid = __float_as_uint(r4.w);
+                                    hgrid = __float_as_uint(r4.z);
+                                }
+                            }
+                        }
+                    }
+                    nHit += hit ? 1u : 0u;
+                    const bool opq = hit && (hflags & (1u << 10)) != 0;
+                    occlZ = opq ? D : occlZ;
+                    win = opq ? hid : win;
+                    wu = opq ? u : wu;
+                    wv = opq ? v : wv;
+                    const bool app = hit && (hflags & (1u << 10)) == 0 && !depthOnly;
+                    const unsigned long long am = __ballot(app);
+                    if (am) {
+                        const int na = __popcll(am);
+                        if (na > wbLeft) {
+                            uint32_t base = 0;
+                            if (lane == 0) base = atomicAdd(hitCounter, 64u);
+                            wbBase = __shfl(base, 0);
+                            wbLeft = 64;
+                        }
+                        if (app) {
+                            const uint32_t slot =
+                                wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & ((1ull << lane) - 1ull));
+                            if (slot < fr->hitCapacity && nh < RH_MAX_HITS - 1) {
+                                hitData[slot] = make_float4(u, v, D, __uint_as_float(hid));
+                                hitNext[2 * (size_t)slot] = head;
+                                hitNext[2 * (size_t)slot + 1] = grids[hgrid].prim;
+                                head = slot;
+                                ++nh;
+                            } else if (slot < fr->hitCapacity) {
+                                lost = true;
+                            }
+                        }
+                        wbLeft -= na;
+                    }
+                }
+            }
+        }
+        if (!live) continue;
+        float4 A = make_float4(0.f, 0.f, 0.f, FLT_MAX), Bv = make_float4(0.f, 0.f, 0.f, __uint_as_float(win));
+        if (win != 0xffffffffu) {
+            const float4 r4 = mprec4[(size_t)win * (RH_MPREC_FLOATS / 4) + 4];
+            shade_hit(prims[grids[__float_as_int(r4.z)].prim], wu, wv, occlZ, win, A, Bv);
+        }
+        samples[2 * sidx] = A;
+        samples[2 * sidx + 1] = Bv;
+        if (frontDepth) frontDepth[sidx] = occlZ;
+        const bool multi = nh > 0;
+        const unsigned long long cmk = __ballot(multi);
+        if (cmk) {
+            uint32_t base = 0;
+            if (lane == __ffsll((long long)cmk) - 1) base = atomicAdd(&hitCounter[2], (uint32_t)__popcll(cmk));
+            base = __shfl(base, __ffsll((long long)cmk) - 1);
+            if (multi) {
+                const uint32_t e = base + (uint32_t)__popcll(cmk & ((1ull << lane) - 1ull));
+                combineList[e] = make_uint4((uint32_t)sidx, (uint32_t)(sidx >> 32), head, (uint32_t)nh);
+            }
+        }
+    }
+    if (__ballot(lost) && lane == 0) atomicAdd(&hitCounter[1], 1u);
+    for (int off = 32; off > 0; off >>= 1) {
+        nTest += __shfl_xor(nTest, off);
+        nBound += __shfl_xor(nBound, off);
+        nHit += __shfl_xor(nHit, off);
+        nIn += __shfl_xor(nIn, off);
+        nIn2 += __shfl_xor(nIn2, off);
+    }
+    if (lane == 0) {
+        wsum[wave][0] = nTest; wsum[wave][1] = nBound; wsum[wave][2] = nHit; wsum[wave][3] = nIn;
+        wsum[wave][4] = nIn2;
+    }
+    __syncthreads();
+    if (tid < 5) {
+        uint32_t v = 0;
+        for (int w = 0; w < RH_DOF_WAVES; ++w) v += wsum[w][tid];
+        rowCounts[(size_t)(rowBase + blockIdx.x) * 5 + tid] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Batched form: work items ordered by cost
 // ---------------------------------------------------------------------------
 // One workgroup.  A task's cost is (list length + 16) x blocks; a task is cut
@@ -2582,6 +2953,18 @@ extern "C" hipError_t rh_launch_sample_mb(hipStream_t st, const SampleLaunch* a,
                        a->itemBase + ntasks, a->capStart, a->listCount, a->lists, mprecMB, a->grids, a->prims,
                        a->pos2d, a->pos1d, a->pixPattern, a->samples, a->hitData, a->hitNext, a->hitCounter,
                        a->combineList, a->frontDepth, a->rowCounts, a->rowBase, a->itemBase, a->idCut, a->depthOnly);
+    return hipGetLastError();
+}
+
+// The depth-of-field form of the sampler: one workgroup per task, tasks
+// [a->itemBase, a->itemBase + ntasks).
+extern "C" hipError_t rh_launch_sample_dof(hipStream_t st, const SampleLaunch* a, int ntasks, const DofTables* t) {
+    if (ntasks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rh_sample_dof, dim3(ntasks), dim3(RH_DOF_WAVES * 64), 0, st, a->fr, a->buckets, a->tasks,
+                       a->itemBase, a->capStart, a->listCount, a->lists, reinterpret_cast<const float4*>(a->mprec),
+                       a->grids, a->prims, a->pos2d, a->pos1d, a->pixPattern, t->shuffled, t->lens, t->pixDof,
+                       t->cells, a->samples, a->hitData, a->hitNext, a->hitCounter, a->combineList, a->frontDepth,
+                       a->rowCounts, a->rowBase, a->idCut, a->depthOnly);
     return hipGetLastError();
 }
 

@@ -7,6 +7,8 @@ token rules: strings in double quotes, numbers, [ ] arrays, # comments):
 
   Format, PixelSamples, PixelFilter, ShadingRate, Exposure, Quantize "rgba",
   Projection "perspective" "fov", Clipping, CropWindow, Shutter,
+  DepthOfField (no arguments: the pinhole camera), GeometricApproximation
+  "focusfactor",
   Option "limits" "bucketsize" / "eyesplits" / "zthreshold",
   Hider "hidden" "jitter", Display, FrameBegin/End, WorldBegin/End,
   AttributeBegin/End, TransformBegin/End, Identity, Transform,
@@ -38,7 +40,7 @@ import re
 
 import numpy as np
 
-from .abi import FILTERS, default_options
+from .abi import FILTERS, FLT_MAX, default_options
 from . import scenes
 
 
@@ -198,7 +200,7 @@ def parse(text, displays=None):
     camera, camkeys = np.eye(4), None
     ktimes = None
     in_world = False
-    attr = {"Cs": (1.0, 1.0, 1.0), "Os": (1.0, 1.0, 1.0), "rate": 1.0, "surface": "constant"}
+    attr = {"Cs": (1.0, 1.0, 1.0), "Os": (1.0, 1.0, 1.0), "rate": 1.0, "surface": "constant", "focus": 1.0}
     astack, tstack = [], []
     motion = None          # the open MotionBegin block: times, line, requests
 
@@ -250,6 +252,15 @@ def parse(text, displays=None):
                 opts.crop[:] = flat[:4]
             elif req == "Shutter":
                 opts.shutter_open, opts.shutter_close = flat[0], flat[1]
+            elif req == "DepthOfField":
+                # without arguments: back to the pinhole camera (ribparser.cpp:406-420)
+                opts.dof[:] = [FLT_MAX, FLT_MAX, FLT_MAX] if not flat else [float(v) for v in flat[:3]]
+                if flat and len(flat) != 3:
+                    raise RibError(f"line {line}: DepthOfField needs fstop, focal length and focal distance")
+            elif req == "GeometricApproximation":
+                if args[0] != "focusfactor":
+                    raise RibError(f"line {line}: only GeometricApproximation \"focusfactor\" is supported")
+                attr["focus"] = float(flat[0])
             elif req == "Option":
                 p = _params(args[1:])
                 if args[0] == "limits":
@@ -306,6 +317,7 @@ def parse(text, displays=None):
                         raise RibError(f"line {block['line']}: the motion keys are patches of different types")
                     make = scenes.bicubic if k0 == "bicubic" else scenes.bilinear
                     p = make(P0, attr["Cs"], attr["Os"], attr["rate"])
+                    p.focus_factor = attr["focus"]
                     _set_keys(p, block["times"], P1=P1)
                     prims.append(p)
                 else:
@@ -345,6 +357,7 @@ def parse(text, displays=None):
                 mk = key_matrices()
                 m = ctm @ camera if mk is None else mk[0]
                 p = scenes.sphere(flat[0], (0.0, 0.0, 0.0), attr["Cs"], attr["Os"], attr["rate"])
+                p.focus_factor = attr["focus"]
                 p.sphere[:] = [float(v) for v in flat[:4]]
                 p.xform[0][:] = [float(v) for v in m.reshape(-1)]
                 p.xform[1][:] = [float(v) for v in m.reshape(-1)]
@@ -358,6 +371,7 @@ def parse(text, displays=None):
                 kind, P = _patch_points(line, args, ctm @ camera if mk is None else mk[0])
                 make = scenes.bicubic if kind == "bicubic" else scenes.bilinear
                 p = make(P, attr["Cs"], attr["Os"], attr["rate"])
+                p.focus_factor = attr["focus"]
                 if mk is not None:
                     _set_keys(p, ktimes, P1=_patch_points(line, args, mk[1])[1])
                 prims.append(p)

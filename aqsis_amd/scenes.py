@@ -11,7 +11,7 @@ import math
 
 import numpy as np
 
-from .abi import (RhPrim, default_options, RH_PRIM_SPHERE, RH_PRIM_BICUBIC,
+from .abi import (RhPrim, default_options, FLT_MAX, RH_PRIM_SPHERE, RH_PRIM_BICUBIC,
                   RH_PRIM_BILINEAR)
 
 
@@ -398,6 +398,7 @@ def to_rib(opts, prims):
            f'Option "limits" "zthreshold" [{opts.zthreshold[0]} {opts.zthreshold[1]} {opts.zthreshold[2]}]',
            f'Hider "hidden" "jitter" [{opts.jitter}]',
            f'Shutter {opts.shutter_open} {opts.shutter_close}',
+           (f'DepthOfField {opts.dof[0]} {opts.dof[1]} {opts.dof[2]}' if opts.dof[0] < FLT_MAX else 'DepthOfField'),
            f'Projection "perspective" "fov" [{opts.fov}]',
            'WorldBegin', 'Surface "constant"']
     for p in prims:
@@ -405,6 +406,7 @@ def to_rib(opts, prims):
         out.append(f'Color [{p.Cs[0]} {p.Cs[1]} {p.Cs[2]}]')
         out.append(f'Opacity [{p.Os[0]} {p.Os[1]} {p.Os[2]}]')
         out.append(f'ShadingRate {p.shading_rate}')
+        out.append(f'GeometricApproximation "focusfactor" {p.focus_factor}')
         moving = p.nkeys == 2
         if moving:
             motion = f'MotionBegin [{p.key_time[0]} {p.key_time[1]}]'

@@ -231,6 +231,48 @@ def eye_plane_split():
     return scenes.eye_plane_scene(seed=5)
 
 
+def patches_dof():
+    # Depth of field on the HD scene generator: the focal plane inside the
+    # patches' depth range (some in focus, most blurred by up to several
+    # pixels), the lens-cell sampler and the CoC-grown bounds and dicing rate.
+    opts, prims = scenes.patches_scene(seed=53, n=260, res=(160, 120), samples=(3, 3),
+                                       filt=("gaussian", 2.0, 2.0))
+    opts.dof[:] = [2.8, 1.2, 16.0]
+    return opts, prims
+
+
+def spheres_dof_transparent():
+    # Strong blur (fstop 1.4) of spheres of mixed opacity: transparent hits
+    # under the lens, a catmull-rom filter and 4x4 samples.
+    opts, prims = scenes.spheres_scene(seed=59, n=40, res=(144, 112), samples=(4, 4),
+                                       filt=("catmull-rom", 3.0, 3.0))
+    rng = np.random.default_rng(59)
+    for p in prims[1::2]:
+        o = float(rng.uniform(0.3, 0.8))
+        p.Os[:] = [o, o, o]
+    opts.dof[:] = [1.4, 1.0, 12.0]
+    return opts, prims
+
+
+def decal_dof():
+    # Equal-depth ties and occlusion culling under the lens: the decal quads
+    # with the focal plane on the middle plane (in focus there, blurred on
+    # the others), focus factor 0.5 on half of them.
+    opts, prims = scenes.decal_scene(seed=4, n=50)
+    opts.dof[:] = [2.0, 1.0, 9.0]
+    for p in prims[::2]:
+        p.focus_factor = 0.5
+    return opts, prims
+
+
+def patches_dof_1x1_crop():
+    # One sample per pixel (one lens cell) with a crop window and a box filter.
+    opts, prims = scenes.patches_scene(seed=61, n=300, res=(180, 120), samples=(1, 1), filt=("box", 1.0, 1.0))
+    opts.crop[:] = [0.1, 0.9, 0.15, 0.85]
+    opts.dof[:] = [2.0, 1.5, 20.0]
+    return opts, prims
+
+
 CASES = {
     "spheres_config": spheres_config,
     "patches_small": patches_small,
@@ -259,4 +301,8 @@ CASES = {
     "decal_ties_motion": decal_ties_motion,
     "patches_cull_flag_mixed": patches_cull_flag_mixed,
     "eye_plane_split": eye_plane_split,
+    "patches_dof": patches_dof,
+    "spheres_dof_transparent": spheres_dof_transparent,
+    "decal_dof": decal_dof,
+    "patches_dof_1x1_crop": patches_dof_1x1_crop,
 }

@@ -102,11 +102,19 @@ def harness(tmp_path_factory):
     return str(exe), d
 
 
+def _with_dof(scene, dof):
+    # depth of field: CoC-grown bounds and the focus-factor dicing rate
+    opts, prims = scene
+    opts.dof[:] = list(dof)
+    return opts, prims
+
+
 @pytest.mark.parametrize("make", [
     lambda: scenes.spheres_scene(seed=1),
     lambda: scenes.patches_scene(seed=5, n=400, res=(320, 180)),
     lambda: scenes.patches_scene(seed=9, n=300, res=(200, 120), samples=(3, 3), filt=("catmull-rom", 3.0, 3.0)),
-], ids=["spheres", "patches", "patches_3x3"])
+    lambda: _with_dof(scenes.patches_scene(seed=53, n=260, res=(160, 120), samples=(3, 3)), (2.8, 1.2, 16.0)),
+], ids=["spheres", "patches", "patches_3x3", "patches_dof"])
 def test_cull_schedule_matches_oracle(oracle, harness, make):
     exe, d = harness
     opts, prims = make()
