@@ -164,6 +164,7 @@ struct rh_ctx {
         graphExec = nullptr;
         graph = nullptr;
     }
+    void* scratch = nullptr;              // this device's scratch array (rh_render_device_noalloc)
     FrameDev frame;                       // host copy of dFrame
     rh_stats stats;
     DofTables dof() const {
@@ -297,6 +298,37 @@ void cached_plan(const rh::Shape& s, rh::Plan& out) {
 }
 }  // namespace
 
+// Device memory of rh_render_device_noalloc, for callers that cannot
+// allocate: the frame's counters at 0, its hit-arena control words at 128,
+// the sampling workgroups' counter rows from kScratchRowsOffset.  Every GPU
+// holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress gives on that device.
+static __device__ __align__(256) unsigned char scratch[RH_NOALLOC_SCRATCH_BYTES];
+static constexpr size_t kScratchRowsOffset = 256;
+
+namespace {
+std::mutex g_scratchMu;
+void* g_scratchAddr[256] = {};
+
+// The scratch array's address on the current device, looked up once per device.
+hipError_t scratch_address(void** out) {
+    *out = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 256) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchAddr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchAddr[dev] = p;
+    }
+    *out = g_scratchAddr[dev];
+    return hipSuccess;
+}
+}  // namespace
+
 extern "C" {
 
 rh_status rh_create(const rh_options* opt, rh_ctx** out) {
@@ -426,6 +458,12 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
         if (e == hipSuccess) e = c->dItemCount.ensure(16);
         if (e == hipSuccess) e = c->dQueue.ensure(16);
         if (e == hipSuccess) e = c->dRows.ensure((size_t)c->maxRows * 5 * sizeof(uint32_t));
+    }
+    // The scratch array's address on this device (for the no-allocation
+    // form only: a failed lookup leaves it null, and that form reports it).
+    if (e == hipSuccess && scratch_address(&c->scratch) != hipSuccess) {
+        c->scratch = nullptr;
+        (void)hipGetLastError();
     }
     c->tuning.mode = RH_MODE_GRID;
     c->tuning.lds_records = 216;
@@ -585,18 +623,32 @@ static int variant_of(const rh_tuning& t) {
     return (t.lds_records == 216 ? 0 : 2) + (t.blocks_per_wave == 4 ? 0 : 1);
 }
 
+// Where a frame keeps its counters (RH_NCOUNTERS), its hit-arena control
+// words (4) and the per-workgroup counter rows (5 per row).
+struct FrameCtl {
+    unsigned long long* counters;
+    uint32_t* hitCtl;
+    uint32_t* rows;
+};
+
+static FrameCtl context_ctl(rh_ctx* c) {
+    return FrameCtl{c->dCounters.as<unsigned long long>(), c->dHitCtl.as<uint32_t>(), c->dRows.as<uint32_t>()};
+}
+
 // Enqueue one device frame on the context's stream.  With `timed`, HIP
-// events bracket the stages (not inside a graph capture).
-static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
+// events bracket the stages (not inside a graph capture).  `gridOnly`
+// takes the static grid layout whatever the tuning says.
+static hipError_t enqueue_frame(rh_ctx* c, bool timed, const FrameCtl& k, bool gridOnly = false) {
     const rh::Schedule& S = c->sched;
     const rh::Shape& s = c->shape;
     const FrameDev* f = c->dFrame.as<FrameDev>();
     const int nb = (int)s.buckets.size(), nt = (int)s.tasks.size();
-    const rh_tuning& tu = c->tuning;
+    rh_tuning tu = c->tuning;
+    if (gridOnly) tu.mode = RH_MODE_GRID;
     hipError_t e;
 #define ENQ(x) do { if ((e = (x)) != hipSuccess) return e; } while (0)
     if (timed) ENQ(hipEventRecord(c->ev[0], c->st));
-    ENQ(rh_launch_frame_begin(c->st, c->dCounters.as<unsigned long long>(), c->dHitCtl.as<uint32_t>()));
+    ENQ(rh_launch_frame_begin(c->st, k.counters, k.hitCtl));
     ENQ(rh_launch_dice(c->st, f, c->dGrids.as<GridDev>(), (int)S.grids.size(), c->dPrims.as<PrimDev>(),
                        c->dParams.as<float>(), c->dMprec.as<float>(), c->dMpbound.as<float4>(),
                        S.motion ? c->dMprecMB.as<float4>() : nullptr));
@@ -605,7 +657,7 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
                       c->dCandStart.as<uint32_t>(), c->dCandGrid.as<uint32_t>(), c->dSegBucket.as<uint32_t>(),
                       S.candGrid.size(), c->dCapStart.as<unsigned long long>(), c->dMpbound.as<float4>(),
                       c->dSegCounts.as<uint32_t>(), c->dLists.as<uint32_t>(), c->dListCount.as<uint32_t>(),
-                      c->dBucketRefs.as<uint32_t>(), c->dCounters.as<unsigned long long>(), c->binTaskSide,
+                      c->dBucketRefs.as<uint32_t>(), k.counters, c->binTaskSide,
                       ((c->frame.bucketX & (c->frame.bucketX - 1)) == 0 && (c->frame.bucketY & (c->frame.bucketY - 1)) == 0) ? 1 : 0,
                       S.motion ? c->dMprecMB.as<float4>() : nullptr));
     if (timed) ENQ(hipEventRecord(c->ev[2], c->st));
@@ -617,9 +669,9 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
     a.lists = c->dLists.as<uint32_t>(); a.mprec = c->dMprec.as<float>(); a.grids = c->dGrids.as<GridDev>();
     a.prims = c->dPrims.as<PrimDev>(); a.pos2d = c->dPos2d.as<float>(); a.pos1d = c->dPos1d.as<float>();
     a.pixPattern = c->dPix.as<uchar4>(); a.samples = c->dSamples.as<float4>(); a.hitData = c->dHitData.as<float4>();
-    a.hitNext = c->dHitNext.as<uint32_t>(); a.hitCounter = c->dHitCtl.as<uint32_t>();
+    a.hitNext = c->dHitNext.as<uint32_t>(); a.hitCounter = k.hitCtl;
     a.combineList = c->dCombine.as<uint4>(); a.frontDepth = c->transparent ? c->dFrontDepth.as<float>() : nullptr;
-    a.rowCounts = c->dRows.as<uint32_t>(); a.rowBase = 0;
+    a.rowCounts = k.rows; a.rowBase = 0;
     a.idCut = 0xffffffffu; a.depthOnly = 0;
     const int variant = variant_of(tu);
     int rows = 0;
@@ -658,11 +710,10 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed) {
     if (timed) ENQ(hipEventRecord(c->ev[5], c->st));
     if (c->transparent)
         ENQ(rh_launch_combine(c->st, f, (size_t)s.extW * s.extH * s.ns, c->dCombine.as<uint4>(),
-                              c->dHitCtl.as<uint32_t>(), c->dHitData.as<float4>(), c->dHitNext.as<uint32_t>(),
+                              k.hitCtl, c->dHitData.as<float4>(), c->dHitNext.as<uint32_t>(),
                               c->dPrims.as<PrimDev>(), c->dSamples.as<float4>(), c->dFrontDepth.as<float>()));
     if (timed) ENQ(hipEventRecord(c->ev[3], c->st));
-    ENQ(rh_launch_reduce_counters(c->st, c->dRows.as<uint32_t>(), rows, c->dBucketRefs.as<uint32_t>(), nb,
-                                  c->dCounters.as<unsigned long long>()));
+    ENQ(rh_launch_reduce_counters(c->st, k.rows, rows, c->dBucketRefs.as<uint32_t>(), nb, k.counters));
     ENQ(rh_launch_filter(c->st, f, s.width(), s.height(), s.ns, s.shiftX, s.shiftY, c->dSamples.as<float4>(),
                          c->tiledFilter ? c->dBoxMask.as<uint16_t>() : nullptr, c->dPix.as<uchar4>(),
                          c->dPos2d.as<float>(), c->dFilter.as<float>(), c->dDither.as<float>(),
@@ -827,6 +878,8 @@ static rh_status cull_schedule(rh_ctx* c, const rh::Forest& forest) {
     return RH_OK;
 }
 
+static rh_status finish_frame(rh_ctx* c, const unsigned long long* cnt, bool staged);
+
 // One frame on the resident inputs.  The batched form replays its frame
 // graph unless `eager` asks for the same launches issued one by one with a
 // timestamp between the stages.
@@ -846,7 +899,7 @@ static rh_status render_frame(rh_ctx* c, bool eager) {
                 // Only the whole frame is timed in this form: events recorded
                 // inside a captured graph do not report elapsed times here.
                 HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-                hipError_t e = enqueue_frame(c, false);
+                hipError_t e = enqueue_frame(c, false, context_ctl(c));
                 hipGraph_t g = nullptr;
                 hipError_t e2 = hipStreamEndCapture(c->st, &g);
                 if (e != hipSuccess || e2 != hipSuccess) {
@@ -861,7 +914,7 @@ static rh_status render_frame(rh_ctx* c, bool eager) {
             HIP_TRY(hipGraphLaunch(c->graphExec, c->st));
             HIP_TRY(hipEventRecord(c->ev[4], c->st));
         } else {
-            HIP_TRY(enqueue_frame(c, true));
+            HIP_TRY(enqueue_frame(c, true, context_ctl(c)));
         }
         HIP_TRY(hipEventSynchronize(c->ev[4]));
         HIP_TRY(hipMemcpy(cnt, c->dCounters.p, sizeof(cnt), hipMemcpyDeviceToHost));
@@ -879,8 +932,14 @@ static rh_status render_frame(rh_ctx* c, bool eager) {
                                              " transparent hits");
         break;
     }
+    return finish_frame(c, cnt, !graphed);
+}
+
+// A completed frame's counters and stage times into the context's stats.
+static rh_status finish_frame(rh_ctx* c, const unsigned long long* cnt, bool staged) {
+    const rh::Schedule& S = c->sched;
     float t[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tComp = 0.0f;
-    if (!graphed) {
+    if (staged) {
         for (int i = 0; i < 4; ++i) HIP_TRY(hipEventElapsedTime(&t[i], c->ev[i], c->ev[i + 1]));
         HIP_TRY(hipEventElapsedTime(&tComp, c->ev[5], c->ev[3]));
     }
@@ -910,6 +969,41 @@ static rh_status render_frame(rh_ctx* c, bool eager) {
 }
 
 rh_status rh_render_device(rh_ctx* c) { return render_frame(c, false); }
+
+// rh_render_device without device allocation: the frame's counters, hit
+// control words and counter rows in the GPU's scratch array, every other
+// buffer the context's (sized by rh_render), the static grid layout, issued
+// eagerly.  A frame that would need a larger transparent-hit arena fails
+// instead of growing it.
+rh_status rh_render_device_noalloc(rh_ctx* c) {
+    if (!c) return fail(RH_ERR_INVALID, "null context");
+    if (!c->scheduled) return fail(RH_ERR_STATE, "rh_render must run before rh_render_device_noalloc");
+    if (!c->scratch) return fail(RH_ERR_STATE, "no scratch memory on this context's device");
+    const int nt = (int)c->shape.tasks.size();
+    const size_t rows = (c->sched.motion || c->shape.dof) ? (size_t)nt : (size_t)((nt + 127) / 128 * 128);
+    if (kScratchRowsOffset + rows * 5 * sizeof(uint32_t) > RH_NOALLOC_SCRATCH_BYTES)
+        return fail(RH_ERR_CAPACITY, "the frame has " + std::to_string(nt) + " sampling tasks; the no-allocation "
+                                     "form holds the counters of at most " +
+                                     std::to_string((RH_NOALLOC_SCRATCH_BYTES - kScratchRowsOffset) / 20) + " workgroups");
+    unsigned char* base = static_cast<unsigned char*>(c->scratch);
+    const FrameCtl k{reinterpret_cast<unsigned long long*>(base), reinterpret_cast<uint32_t*>(base + 128),
+                     reinterpret_cast<uint32_t*>(base + kScratchRowsOffset)};
+    c->rendered = false;
+    HIP_TRY(enqueue_frame(c, true, k, true));
+    HIP_TRY(hipEventSynchronize(c->ev[4]));
+    unsigned long long cnt[RH_NCOUNTERS];
+    uint32_t hc[2];
+    HIP_TRY(hipMemcpy(cnt, k.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hc, k.hitCtl, sizeof(hc), hipMemcpyDeviceToHost));
+    if (hc[0] > c->frame.hitCapacity)
+        return fail(RH_ERR_CAPACITY, "the frame needs " + std::to_string(hc[0]) + " transparent-hit entries and the "
+                                     "context holds " + std::to_string(c->frame.hitCapacity) +
+                                     " (rh_render_device grows the arena)");
+    if (hc[1] != 0)
+        return fail(RH_ERR_CAPACITY, "a sample collected more than " + std::to_string(RH_MAX_HITS - 1) +
+                                         " transparent hits");
+    return finish_frame(c, cnt, true);
+}
 
 rh_status rh_render_stages(rh_ctx* c) { return render_frame(c, true); }
 

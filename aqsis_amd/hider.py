@@ -41,7 +41,7 @@ class RhTuning(C.Structure):
 MODES = {"grid": 0, "plain": 1, "batched": 2}
 
 EXPORTS = ["rh_options_default", "rh_create", "rh_post_surfaces", "rh_clear_surfaces", "rh_render",
-           "rh_render_device", "rh_render_stages", "rh_frame_size", "rh_read_channels", "rh_read_rgba8", "rh_display", "rh_tiff_write",
+           "rh_render_device", "rh_render_device_noalloc", "rh_render_stages", "rh_frame_size", "rh_read_channels", "rh_read_rgba8", "rh_display", "rh_tiff_write",
            "rh_read_samples",
            "rh_get_stats", "rh_destroy", "rh_last_error", "rh_estimate", "rh_sample_positions",
            "rh_set_tuning", "rh_get_tuning", "rh_autotune"]
@@ -65,6 +65,7 @@ def lib():
         L.rh_clear_surfaces.argtypes = [C.c_void_p]
         L.rh_render.argtypes = [C.c_void_p]
         L.rh_render_device.argtypes = [C.c_void_p]
+        L.rh_render_device_noalloc.argtypes = [C.c_void_p]
         L.rh_render_stages.argtypes = [C.c_void_p]
         L.rh_frame_size.argtypes = [C.c_void_p] + [P(C.c_int32)] * 5
         L.rh_read_channels.argtypes = [C.c_void_p, C.c_void_p]
@@ -129,6 +130,10 @@ class Hider:
 
     def render_device(self):
         _check(lib().rh_render_device(self._ctx))
+
+    def render_device_noalloc(self):
+        """rh_render_device with no device allocation (scratch array of the GPU)."""
+        _check(lib().rh_render_device_noalloc(self._ctx))
 
     def render_stages(self):
         """One frame issued kernel by kernel with per-stage device times."""

@@ -209,6 +209,20 @@ rh_status rh_render(rh_ctx* ctx);
  * grids and sample plan (dice -> ... -> quantise), for timing. */
 rh_status rh_render_device(rh_ctx* ctx);
 
+/* rh_render_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops that forbid allocation).  The frame's counters and
+ * the sampling workgroups' counter rows live in a fixed device array of
+ * RH_NOALLOC_SCRATCH_BYTES that the library holds on each GPU until the
+ * process ends (frames of up to 209,702 sampling workgroups: RH_ERR_CAPACITY
+ * beyond); every other buffer is the context's, sized by rh_render.  Always
+ * the static grid layout, issued eagerly, with the per-stage times of
+ * rh_stats.  A frame whose transparent hits outgrow the context's arena
+ * fails with RH_ERR_CAPACITY (rh_render_device would grow it).  Calls that
+ * overlap on one GPU share its array and race: serialise them. */
+#define RH_NOALLOC_SCRATCH_BYTES (4u << 20)
+rh_status rh_render_device_noalloc(rh_ctx* ctx);
+
 /* rh_render_device with the kernels issued one by one (also in the batched
  * form, which otherwise replays a frame graph) and a device timestamp
  * between the stages, filling the per-stage times of rh_stats.  Same result
