@@ -39,10 +39,15 @@
 // stored hits, point-in-polygon hits, second Newton steps), 5 tasks whose
 // list overflowed, 6 micropolygon-bucket references, 7 transparent-hit
 // arena entries taken (may exceed the capacity: then the frame is redone
-// with a larger arena), 8 samples whose hit list exceeded RH_MAX_HITS.
+// with a larger arena), 8 unused.
 #define RH_NCOUNTERS 9
-// Transparent hits one sample can hold (its occluding hit included).
+// Hit-list entries (its occluding hit included) the combine kernel sorts in
+// LDS; a sample with more transparent hits goes to the long-list pass.
 #define RH_MAX_HITS 64
+// Transparent-hit arena control words: 0 entries taken, 1 samples left to
+// the long-list pass, 2 combine-list entries, 3 their hit-list entries,
+// 4 and 5 the long-list pass's work offsets.
+#define RH_HITCTL_WORDS 8
 // Sphere parameter layout (see front.h)
 #define SPH_RADIUS_D 0
 #define SPH_MAT_D 1

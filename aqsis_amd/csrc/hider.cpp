@@ -32,7 +32,10 @@ extern "C" hipError_t rh_sample_occupancy(int, int*);
 extern "C" hipError_t rh_launch_plan_items(hipStream_t, const FrameDev*, const TaskDev*, int, const uint32_t*,
                                            uint32_t, int, uint2*, uint32_t*, uint32_t*);
 extern "C" hipError_t rh_launch_frame_begin(hipStream_t, unsigned long long*, uint32_t*);
-extern "C" hipError_t rh_launch_combine(hipStream_t, const FrameDev*, size_t, const uint4*, const uint32_t*,
+extern "C" hipError_t rh_launch_combine_long(hipStream_t, const FrameDev*, size_t, const uint4*, uint32_t*,
+                                             const float4*, const uint32_t*, const PrimDev*, float4*, float*, float*,
+                                             uint32_t*, int*);
+extern "C" hipError_t rh_launch_combine(hipStream_t, const FrameDev*, size_t, const uint4*, uint32_t*,
                                         const float4*, const uint32_t*, const PrimDev*, float4*, float*);
 extern "C" hipError_t rh_launch_reduce_counters(hipStream_t, const uint32_t*, int, const uint32_t*, int,
                                                 unsigned long long*);
@@ -148,6 +151,7 @@ struct rh_ctx {
     DBuf dMprecMB;                        // motion-frame micropolygon records
     // transparent hits: arena of (u, v, depth, id) and (link, surface) pairs
     DBuf dHitData, dHitNext, dHitCtl, dCombine, dFrontDepth;
+    DBuf dLongWork;                       // the long-list combine's work arrays (grown on demand)
     bool transparent = false;             // some posted surface has Os < 1
     // work items: one per task (grid and per-bucket forms), cost-split (batched)
     DBuf dItemsStatic, dItems, dItemCount, dQueue, dRows;
@@ -181,7 +185,7 @@ struct rh_ctx {
                        &dGrids, &dParams,
                        &dPrims, &dCandStart, &dCandGrid, &dCapStart, &dMprec, &dMpbound, &dLists,
                        &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dBucketRefs,
-                       &dHitData, &dHitNext, &dHitCtl, &dCombine, &dFrontDepth, &dItemsStatic, &dItems,
+                       &dHitData, &dHitNext, &dHitCtl, &dCombine, &dFrontDepth, &dLongWork, &dItemsStatic, &dItems,
                        &dItemCount, &dQueue, &dRows, &dSegBucket, &dSegCounts, &dMprecMB, &dCullTrees, &dCullPyr,
                        &dCullBounds, &dCullRanges, &dCullStart, &dCullPairMax, &dProbeOut};
         for (DBuf* b : all) f(*b);
@@ -407,7 +411,7 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
         (e = c->dRgba.ensure((size_t)s.width() * s.height() * 4)) != hipSuccess ||
         (e = c->dListCount.ensure(s.tasks.size() * sizeof(uint32_t))) != hipSuccess ||
         (e = c->dBucketRefs.ensure(s.buckets.size() * sizeof(uint32_t))) != hipSuccess ||
-        (e = c->dHitCtl.ensure(4 * sizeof(uint32_t))) != hipSuccess ||
+        (e = c->dHitCtl.ensure(RH_HITCTL_WORDS * sizeof(uint32_t))) != hipSuccess ||
         (e = upload(c->dCounters, zero, c->st)) != hipSuccess ||
         (s.dof && ((e = upload(c->dShuffled, c->plan.shuffled, c->st)) != hipSuccess ||
                    (e = upload(c->dLens, c->plan.lens, c->st)) != hipSuccess ||
@@ -635,6 +639,15 @@ static FrameCtl context_ctl(rh_ctx* c) {
     return FrameCtl{c->dCounters.as<unsigned long long>(), c->dHitCtl.as<uint32_t>(), c->dRows.as<uint32_t>()};
 }
 
+// The filter + expose + quantise stage of a frame.
+static hipError_t enqueue_filter(rh_ctx* c) {
+    const rh::Shape& s = c->shape;
+    return rh_launch_filter(c->st, c->dFrame.as<FrameDev>(), s.width(), s.height(), s.ns, s.shiftX, s.shiftY,
+                            c->dSamples.as<float4>(), c->tiledFilter ? c->dBoxMask.as<uint16_t>() : nullptr,
+                            c->dPix.as<uchar4>(), c->dPos2d.as<float>(), c->dFilter.as<float>(),
+                            c->dDither.as<float>(), c->dChannels.as<float>(), c->dRgba.as<uchar4>());
+}
+
 // Enqueue one device frame on the context's stream.  With `timed`, HIP
 // events bracket the stages (not inside a graph capture).  `gridOnly`
 // takes the static grid layout whatever the tuning says.
@@ -714,13 +727,31 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed, const FrameCtl& k, bool g
                               c->dPrims.as<PrimDev>(), c->dSamples.as<float4>(), c->dFrontDepth.as<float>()));
     if (timed) ENQ(hipEventRecord(c->ev[3], c->st));
     ENQ(rh_launch_reduce_counters(c->st, k.rows, rows, c->dBucketRefs.as<uint32_t>(), nb, k.counters));
-    ENQ(rh_launch_filter(c->st, f, s.width(), s.height(), s.ns, s.shiftX, s.shiftY, c->dSamples.as<float4>(),
-                         c->tiledFilter ? c->dBoxMask.as<uint16_t>() : nullptr, c->dPix.as<uchar4>(),
-                         c->dPos2d.as<float>(), c->dFilter.as<float>(), c->dDither.as<float>(),
-                         c->dChannels.as<float>(), c->dRgba.as<uchar4>()));
+    ENQ(enqueue_filter(c));
     if (timed) ENQ(hipEventRecord(c->ev[4], c->st));
 #undef ENQ
     return hipSuccess;
+}
+
+// Samples with more transparent hits than the combine kernel sorts in LDS:
+// the long-list combine over work arrays sized from the frame's counts
+// (hc[1] samples, hc[3] list entries), then the filter again.
+static rh_status combine_long_lists(rh_ctx* c, const FrameCtl& k, const uint32_t* hc) {
+    const rh::Shape& s = c->shape;
+    const size_t entries = hc[3], deferred = hc[1];
+    const size_t bytes = entries * (sizeof(float) + sizeof(uint32_t)) + deferred * 192 * sizeof(int);
+    HIP_TRY(c->dLongWork.ensure(bytes));
+    unsigned char* w = static_cast<unsigned char*>(c->dLongWork.p);
+    float* workD = reinterpret_cast<float*>(w);
+    uint32_t* workX = reinterpret_cast<uint32_t*>(w + entries * sizeof(float));
+    int* workStk = reinterpret_cast<int*>(w + entries * (sizeof(float) + sizeof(uint32_t)));
+    HIP_TRY(rh_launch_combine_long(c->st, c->dFrame.as<FrameDev>(), (size_t)s.extW * s.extH * s.ns,
+                                   c->dCombine.as<uint4>(), k.hitCtl, c->dHitData.as<float4>(),
+                                   c->dHitNext.as<uint32_t>(), c->dPrims.as<PrimDev>(), c->dSamples.as<float4>(),
+                                   c->dFrontDepth.as<float>(), workD, workX, workStk));
+    HIP_TRY(enqueue_filter(c));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    return RH_OK;
 }
 
 // The sampling launch arguments of a frame's depth pass: opaque depths only
@@ -918,7 +949,7 @@ static rh_status render_frame(rh_ctx* c, bool eager) {
         }
         HIP_TRY(hipEventSynchronize(c->ev[4]));
         HIP_TRY(hipMemcpy(cnt, c->dCounters.p, sizeof(cnt), hipMemcpyDeviceToHost));
-        uint32_t hc[2];
+        uint32_t hc[4];
         HIP_TRY(hipMemcpy(hc, c->dHitCtl.p, sizeof(hc), hipMemcpyDeviceToHost));
         if (hc[0] > c->frame.hitCapacity) {
             // The arena ran out: redo the frame with room for every hit.
@@ -927,12 +958,19 @@ static rh_status render_frame(rh_ctx* c, bool eager) {
             if (r != RH_OK) return r;
             continue;
         }
-        if (hc[1] != 0)
-            return fail(RH_ERR_CAPACITY, "a sample collected more than " + std::to_string(RH_MAX_HITS - 1) +
-                                             " transparent hits");
-        break;
+        rh_status r = finish_frame(c, cnt, !graphed);
+        if (r != RH_OK || hc[1] == 0) return r;
+        // samples with more hits than the combine kernel holds
+        c->rendered = false;
+        const auto t0 = std::chrono::steady_clock::now();
+        r = combine_long_lists(c, context_ctl(c), hc);
+        if (r != RH_OK) return r;
+        const double ms = ms_since(t0);
+        c->stats.ms_composite += ms;
+        c->stats.ms_device += ms;
+        c->rendered = true;
+        return RH_OK;
     }
-    return finish_frame(c, cnt, !graphed);
 }
 
 // A completed frame's counters and stage times into the context's stats.
@@ -992,7 +1030,7 @@ rh_status rh_render_device_noalloc(rh_ctx* c) {
     HIP_TRY(enqueue_frame(c, true, k, true));
     HIP_TRY(hipEventSynchronize(c->ev[4]));
     unsigned long long cnt[RH_NCOUNTERS];
-    uint32_t hc[2];
+    uint32_t hc[4];
     HIP_TRY(hipMemcpy(cnt, k.counters, sizeof(cnt), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(hc, k.hitCtl, sizeof(hc), hipMemcpyDeviceToHost));
     if (hc[0] > c->frame.hitCapacity)
@@ -1000,8 +1038,9 @@ rh_status rh_render_device_noalloc(rh_ctx* c) {
                                      "context holds " + std::to_string(c->frame.hitCapacity) +
                                      " (rh_render_device grows the arena)");
     if (hc[1] != 0)
-        return fail(RH_ERR_CAPACITY, "a sample collected more than " + std::to_string(RH_MAX_HITS - 1) +
-                                         " transparent hits");
+        return fail(RH_ERR_CAPACITY, std::to_string(hc[1]) + " samples collected more than " +
+                                         std::to_string(RH_MAX_HITS - 1) + " transparent hits: their combine needs "
+                                         "work memory (rh_render_device allocates it)");
     return finish_frame(c, cnt, true);
 }
 

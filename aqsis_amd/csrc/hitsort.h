@@ -175,6 +175,41 @@ RH_HD void sort_with(float* d, uint32_t* x, int n, int* stk) {
     }
 }
 
+// std::sort(d, d + n) with x carried along for any n (a list longer than
+// sort_with takes).  `stk` holds 2 * (2 * lg(n) + 2) ints of work stack:
+// (first, last | depth limit << 24) per entry.
+RH_HD void sort_long(float* d, uint32_t* x, int n, int* stk) {
+    if (n <= 0) return;
+    int lg = 0;
+    while ((2 << lg) <= n) ++lg;          // __lg(n)
+    int sp = 0;
+    stk[0] = 0;
+    stk[1] = n | ((2 * lg) << 24);
+    sp = 1;
+    while (sp > 0) {
+        --sp;
+        int first = stk[2 * sp], last = stk[2 * sp + 1] & 0xffffff, depth = stk[2 * sp + 1] >> 24;
+        while (last - first > 16) {
+            if (depth == 0) {
+                heap_sort(d, x, first, last);
+                break;
+            }
+            --depth;
+            const int cut = partition_pivot(d, x, first, last);
+            stk[2 * sp] = cut;
+            stk[2 * sp + 1] = last | (depth << 24);
+            ++sp;
+            last = cut;
+        }
+    }
+    if (n > 16) {
+        insertion_sort(d, x, 0, 16);
+        for (int i = 16; i != n; ++i) linear_insert(d, x, i);
+    } else {
+        insertion_sort(d, x, 0, n);
+    }
+}
+
 // std::sort(d, d + n) with x carried along; n <= 64.
 RH_HD void sort(float* d, uint32_t* x, int n) {
     int stk[24];

@@ -858,7 +858,6 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
     uint32_t nTestW = 0;                  // wave-uniform part of nTest
     uint32_t wbBase = 0;                  // the wave's current arena block
     int wbLeft = 0;
-    bool lost = false;
     const int itemEnd = persistent ? (int)nDevice[0] : itemBase + nStatic;
 
     for (int round = 0;; ++round) {
@@ -1128,7 +1127,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                             if (app) {
                                 const uint32_t slot =
                                     wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & ((1ull << lane) - 1ull));
-                                if (slot < fr->hitCapacity && nh[k] < RH_MAX_HITS - 1) {
+                                if (slot < fr->hitCapacity) {
                                     // the hit's (u, v, depth, id) and surface; the
                                     // combine shades it
                                     hitData[slot] = make_float4(u, v, D, __uint_as_float(hid));
@@ -1136,8 +1135,6 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                                     hitNext[2 * (size_t)slot + 1] = grids[hgrid].prim;
                                     head[k] = slot;
                                     ++nh[k];
-                                } else if (slot < fr->hitCapacity) {
-                                    lost = true;   // more hits than one sample can combine
                                 }
                             }
                             wbLeft -= na;
@@ -1175,7 +1172,6 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
         }
     }
     }   // items
-    if (__ballot(lost) && lane == 0) atomicAdd(&hitCounter[1], 1u);
     if (lane == 0) nTest += nTestW;
     // The workgroup's hider counters: wave sums, then one row (summed by
     // rh_reduce_counters; same-address atomics from every wave would
@@ -1399,7 +1395,6 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
     uint32_t wbBase = 0;
     int wbLeft = 0;
-    bool lost = false;
     if (task < ntasks) {
     const int ns = fr->ns, nx = fr->nx, ny = fr->ny, bw = fr->blockW, bh = fr->blockH;
     const int BW = bw * nx, BH = bh * ny;
@@ -1864,14 +1859,12 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                                 }
                                 if (app) {
                                     const uint32_t slot = wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & lanesBelow);
-                                    if (slot < fr->hitCapacity && nh < RH_MAX_HITS - 1) {
+                                    if (slot < fr->hitCapacity) {
                                         hitData[slot] = make_float4(u, v, D, __uint_as_float(hid));
                                         hitNext[2 * (size_t)slot] = head;
                                         hitNext[2 * (size_t)slot + 1] = grids[hgrid].prim;
                                         head = slot;
                                         ++nh;
-                                    } else if (slot < fr->hitCapacity) {
-                                        lost = true;
                                     }
                                 }
                                 wbLeft -= na;
@@ -1913,7 +1906,6 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
         }
     }
     }   // task < ntasks
-    if (__ballot(lost) && lane == 0) atomicAdd(&hitCounter[1], 1u);
     for (int off = 32; off > 0; off >>= 1) {
         nTest += __shfl_xor(nTest, off);
         nBound += __shfl_xor(nBound, off);
@@ -2059,7 +2051,6 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
     uint32_t wbBase = 0;
     int wbLeft = 0;
-    bool lost = false;
 
     // every wave runs the same number of passes (the staging barriers)
     for (int p0 = 0; p0 < nblk; p0 += RH_DOF_WAVES) {
@@ -2236,14 +2227,12 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
                         if (app) {
                             const uint32_t slot =
                                 wbBase + (uint32_t)(64 - wbLeft) + (uint32_t)__popcll(am & ((1ull << lane) - 1ull));
-                            if (slot < fr->hitCapacity && nh < RH_MAX_HITS - 1) {
+                            if (slot < fr->hitCapacity) {
                                 hitData[slot] = make_float4(u, v, D, __uint_as_float(hid));
                                 hitNext[2 * (size_t)slot] = head;
                                 hitNext[2 * (size_t)slot + 1] = grids[hgrid].prim;
                                 head = slot;
                                 ++nh;
-                            } else if (slot < fr->hitCapacity) {
-                                lost = true;
                             }
                         }
                         wbLeft -= na;
@@ -2272,7 +2261,6 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
             }
         }
     }
-    if (__ballot(lost) && lane == 0) atomicAdd(&hitCounter[1], 1u);
     for (int off = 32; off > 0; off >>= 1) {
         nTest += __shfl_xor(nTest, off);
         nBound += __shfl_xor(nBound, off);
@@ -2354,7 +2342,7 @@ __global__ void __launch_bounds__(1024) rh_plan_items(const FrameDev* __restrict
 __global__ void rh_frame_begin(unsigned long long* __restrict__ counters, uint32_t* __restrict__ hitCtl) {
     const int t = threadIdx.x;
     if (t < RH_NCOUNTERS) counters[t] = 0ull;
-    if (t < 4) hitCtl[t] = 0u;
+    if (t < RH_HITCTL_WORDS) hitCtl[t] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -2367,30 +2355,19 @@ __global__ void rh_frame_begin(unsigned long long* __restrict__ counters, uint32
 // of the nearest hit whose opacity reaches zthreshold, else the occluding
 // depth; the front-most hit gives the id, and its depth goes to frontDepth.
 // One thread per listed sample, one wave per workgroup.
-#define RH_COMBINE_WG 64
-__global__ void __launch_bounds__(RH_COMBINE_WG)
-rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
-                                                  const uint32_t* __restrict__ hitCounter,
-                                                  const float4* __restrict__ hitData,
-                                                  const uint32_t* __restrict__ hitNext,
-                                                  const PrimDev* __restrict__ prims, float4* __restrict__ samples,
-                                                  float* __restrict__ frontDepth) {
-    __shared__ float cmbD[RH_COMBINE_WG * (RH_MAX_HITS + 1)];
-    __shared__ uint32_t cmbX[RH_COMBINE_WG * (RH_MAX_HITS + 1)];
-    __shared__ int cmbS[RH_COMBINE_WG * 25];
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= hitCounter[2]) return;
-    const uint4 e = list[t];
+// CqImagePixel::Combine of one listed sample (see rh_combine); d, x: room
+// for its hits and its occluding hit; stk: the sort's work stack.  Inlined
+// into each kernel, so that rh_combine keeps its lists in LDS with LDS
+// instructions.
+__device__ __forceinline__ void combine_sample(const FrameDev* __restrict__ fr, const uint4 e, float* d, uint32_t* x, int* stk,
+                               bool longList, const float4* __restrict__ hitData,
+                               const uint32_t* __restrict__ hitNext, const PrimDev* __restrict__ prims,
+                               float4* __restrict__ samples, float* __restrict__ frontDepth) {
     const size_t sidx = (size_t)e.x | ((size_t)e.y << 32);
     uint32_t head = e.z;
     int n = (int)e.w;
     const float4 oA = samples[2 * sidx], oB = samples[2 * sidx + 1];
     const float occlZ = oA.w;
-    // the hit list and the sort's work stack in LDS (no scratch memory),
-    // each lane's array an odd number of words from the next
-    float* d = cmbD + threadIdx.x * (RH_MAX_HITS + 1);
-    uint32_t* x = cmbX + threadIdx.x * (RH_MAX_HITS + 1);
-    int* stk = cmbS + threadIdx.x * 25;
     // the list is linked newest first; rebuild insertion order
     for (int q = n - 1; q >= 0; --q) {
         x[q] = head;
@@ -2402,7 +2379,8 @@ rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
         d[n] = oA.w;
         ++n;
     }
-    rhsort::sort_with(d, x, n, stk);
+    if (longList) rhsort::sort_long(d, x, n, stk);
+    else rhsort::sort_with(d, x, n, stk);
     float col[3] = {0.0f, 0.0f, 0.0f}, opa[3] = {0.0f, 0.0f, 0.0f};
     float opaqueDepth = occlZ;
     const float zt0 = fr->zThreshold[0], zt1 = fr->zThreshold[1], zt2 = fr->zThreshold[2];
@@ -2426,6 +2404,55 @@ rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
     frontDepth[sidx] = d[0];
     samples[2 * sidx + 1] = make_float4(opa[0], opa[1], opa[2], __uint_as_float(front));
 }
+
+#define RH_COMBINE_WG 64
+__global__ void __launch_bounds__(RH_COMBINE_WG)
+rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
+                                                  uint32_t* __restrict__ hitCounter,
+                                                  const float4* __restrict__ hitData,
+                                                  const uint32_t* __restrict__ hitNext,
+                                                  const PrimDev* __restrict__ prims, float4* __restrict__ samples,
+                                                  float* __restrict__ frontDepth) {
+    __shared__ float cmbD[RH_COMBINE_WG * (RH_MAX_HITS + 1)];
+    __shared__ uint32_t cmbX[RH_COMBINE_WG * (RH_MAX_HITS + 1)];
+    __shared__ int cmbS[RH_COMBINE_WG * 25];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= hitCounter[2]) return;
+    const uint4 e = list[t];
+    if (e.w >= RH_MAX_HITS) {
+        // more hits than the LDS arrays hold: combined by rh_combine_long
+        // (hitCounter[1] samples, hitCounter[3] list entries)
+        atomicAdd(&hitCounter[1], 1u);
+        atomicAdd(&hitCounter[3], e.w + 1u);
+        return;
+    }
+    // the hit list and the sort's work stack in LDS (no scratch memory),
+    // each lane's array an odd number of words from the next
+    combine_sample(fr, e, cmbD + threadIdx.x * (RH_MAX_HITS + 1), cmbX + threadIdx.x * (RH_MAX_HITS + 1),
+                   cmbS + threadIdx.x * 25, false, hitData, hitNext, prims, samples, frontDepth);
+}
+
+// The samples rh_combine left (more than RH_MAX_HITS - 1 transparent hits,
+// as many as the reference's unbounded lists hold): the same steps with the
+// list and the sort's stack in global work arrays (the sort restated for
+// any length), one thread per listed sample.  workD and workX hold
+// hitCounter[3] entries, workStk 192 ints per deferred sample
+// (hitCounter[1] of them); hitCounter[4] and [5] hand them out.
+__global__ void __launch_bounds__(64)
+rh_combine_long(const FrameDev* __restrict__ fr, const uint4* __restrict__ list, uint32_t* __restrict__ hitCounter,
+                const float4* __restrict__ hitData, const uint32_t* __restrict__ hitNext,
+                const PrimDev* __restrict__ prims, float4* __restrict__ samples, float* __restrict__ frontDepth,
+                float* __restrict__ workD, uint32_t* __restrict__ workX, int* __restrict__ workStk) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= hitCounter[2]) return;
+    const uint4 e = list[t];
+    if (e.w < RH_MAX_HITS) return;
+    const uint32_t off = atomicAdd(&hitCounter[4], e.w + 1u);
+    const uint32_t k = atomicAdd(&hitCounter[5], 1u);
+    combine_sample(fr, e, workD + off, workX + off, workStk + (size_t)k * 192, true, hitData, hitNext, prims, samples,
+                   frontDepth);
+}
+
 
 // ---------------------------------------------------------------------------
 // Filter + expose + quantise
@@ -2990,13 +3017,25 @@ extern "C" hipError_t rh_launch_plan_items(hipStream_t st, const FrameDev* fr, c
 // maxEntries: the combine list's capacity (the extended region's samples);
 // the kernel reads the actual count from hitCounter[2].
 extern "C" hipError_t rh_launch_combine(hipStream_t st, const FrameDev* fr, size_t maxEntries, const uint4* list,
-                                        const uint32_t* hitCounter, const float4* hitData, const uint32_t* hitNext,
+                                        uint32_t* hitCounter, const float4* hitData, const uint32_t* hitNext,
                                         const PrimDev* prims,
                                         float4* samples, float* frontDepth) {
     if (maxEntries == 0) return hipSuccess;
     hipLaunchKernelGGL(rh_combine, dim3((unsigned)((maxEntries + RH_COMBINE_WG - 1) / RH_COMBINE_WG)),
                        dim3(RH_COMBINE_WG), 0, st, fr, list,
                        hitCounter, hitData, hitNext, prims, samples, frontDepth);
+    return hipGetLastError();
+}
+
+// The long-list pass over the combine list (up to maxEntries entries; the
+// kernel reads the count from hitCounter[2]).
+extern "C" hipError_t rh_launch_combine_long(hipStream_t st, const FrameDev* fr, size_t maxEntries, const uint4* list,
+                                             uint32_t* hitCounter, const float4* hitData, const uint32_t* hitNext,
+                                             const PrimDev* prims, float4* samples, float* frontDepth, float* workD,
+                                             uint32_t* workX, int* workStk) {
+    if (maxEntries == 0) return hipSuccess;
+    hipLaunchKernelGGL(rh_combine_long, dim3((unsigned)((maxEntries + 63) / 64)), dim3(64), 0, st, fr, list,
+                       hitCounter, hitData, hitNext, prims, samples, frontDepth, workD, workX, workStk);
     return hipGetLastError();
 }
 

@@ -5,10 +5,14 @@ flex and bison, and reaches into the renderer's singletons), so these
 fixtures are regression pins of the oracle's restatement, not reference
 output: per-sample winning micropolygon ids and depths, filtered channels, 8-bit
 pixels and the sample-test count of small scenes, with occlusion culling on
-(the reference's default, Attribute "cull" "hidden" 1).
+(the reference's default, Attribute "cull" "hidden" 1); and, for the full-size
+HD and motion configurations, SHA-256 digests of the same outputs with the
+counters (configs.json).
 
 Usage: python tests/golden/make_golden.py
 """
+import hashlib
+import json
 import os
 import sys
 
@@ -29,7 +33,37 @@ def golden_cases():
     }
 
 
+# Full-size configurations: the frames are too large to store, so their
+# fixture is a SHA-256 of each output array's bytes plus the counters.
+COUNTERS = ("spl_count", "spl_bound_hits", "spl_hits", "inside", "inside_nonlinear", "grids", "micropolygons",
+            "occlusion_culled")
+
+
+def config_cases():
+    return {
+        # the plain hider at HD: 1920x1080, 4x4, catmull-rom 3x3, 10,000 bicubic patches (the benched frame)
+        "hd_patches": lambda: scenes.patches_scene(seed=1, n=10000, res=(1920, 1080)),
+        # transparency and motion blur: 256x256, 8x8, gaussian 2x2, 600 moving transparent patches
+        "motion_256": lambda: scenes.motion_scene(seed=1, n=600, res=(256, 256)),
+    }
+
+
+def frame_digest(ids, depth, rgbaz, channels, rgba8, stats):
+    h = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    return {"mp_id": h(ids.astype(np.uint32)), "depth": h(depth.astype(np.float32)),
+            "rgbaz": h(rgbaz.astype(np.float32)), "channels": h(channels.astype(np.float32)),
+            "rgba8": h(rgba8.astype(np.uint8)), "counters": {k: int(stats[k]) for k in COUNTERS}}
+
+
 def main():
+    digests = {}
+    for name, make in config_cases().items():
+        opts, prims = make()
+        r = loader.render(opts, prims, occlusion_cull=True)
+        digests[name] = frame_digest(r["mp_id"], r["depth"], r["rgbaz"], r["channels"], r["rgba8"], r["stats"])
+        print(name, digests[name]["counters"])
+    with open(os.path.join(HERE, "configs.json"), "w") as f:
+        json.dump(digests, f, indent=1, sort_keys=True)
     for name, (opts, prims) in golden_cases().items():
         r = loader.render(opts, prims, occlusion_cull=True)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), mp_id=r["mp_id"], depth=r["depth"],

@@ -273,6 +273,24 @@ def patches_dof_1x1_crop():
     return opts, prims
 
 
+def deep_stack_100():
+    # 100 transparent camera-facing quads over the middle of the frame, some
+    # at equal depths: samples with up to 100 transparent hits (more than
+    # the combine kernel sorts in LDS: the long-list pass, std::sort's full
+    # introsort with ties).
+    opts = default_options(xres=64, yres=48, fov=45.0, xsamples=2, ysamples=2, filter="box",
+                           filter_xwidth=1.0, filter_ywidth=1.0)
+    rng = np.random.default_rng(71)
+    prims = []
+    for i in range(100):
+        x0, y0 = rng.uniform(4.0, 14.0), rng.uniform(4.0, 12.0)
+        x1, y1 = rng.uniform(48.0, 60.0), rng.uniform(34.0, 44.0)
+        z = 6.0 + 0.05 * (i - i % 3)            # groups of three at one depth
+        o = float(rng.uniform(0.02, 0.1))
+        prims.append(scenes.raster_quad(opts, (x0, x1, y0, y1), z, tuple(rng.uniform(0.0, 1.0, 3)), (o, o, o)))
+    return opts, prims
+
+
 CASES = {
     "spheres_config": spheres_config,
     "patches_small": patches_small,
@@ -305,4 +323,5 @@ CASES = {
     "spheres_dof_transparent": spheres_dof_transparent,
     "decal_dof": decal_dof,
     "patches_dof_1x1_crop": patches_dof_1x1_crop,
+    "deep_stack_100": deep_stack_100,
 }

@@ -10,7 +10,7 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
-from make_golden import golden_cases  # noqa: E402
+from make_golden import config_cases, frame_digest, golden_cases  # noqa: E402
 
 CASES = golden_cases()
 
@@ -55,3 +55,15 @@ def test_device_reproduces_golden(name):
         assert float(np.max(np.abs(h.channels().astype(np.float64) - g["channels"]))) <= 1e-5
         assert np.array_equal(h.rgba8(), g["rgba8"])
         assert h.stats()["spl_count"] == int(g["spl_count"])
+
+
+@pytest.mark.parametrize("name", ["hd_patches", "motion_256"])
+def test_oracle_reproduces_config_digests(oracle, name):
+    """The full-size configurations' digests (checked against the device by
+    tests/test_configs_gpu.py) are the oracle's frames."""
+    import json
+    with open(os.path.join(HERE, "golden", "configs.json")) as f:
+        want = json.load(f)[name]
+    opts, prims = config_cases()[name]()
+    r = oracle.render(opts, prims, occlusion_cull=True)
+    assert frame_digest(r["mp_id"], r["depth"], r["rgbaz"], r["channels"], r["rgba8"], r["stats"]) == want
