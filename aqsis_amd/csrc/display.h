@@ -1,0 +1,24 @@
+// Display data of a rendered frame: the value type a display request is
+// given for the Quantize settings, and the per-value conversion of the
+// frame's channels into it.
+#pragma once
+
+#include "../../include/aqsis_hider.h"
+
+namespace rh {
+
+// selectDataFormat (ddmanager.cpp:249-283): RH_DSPY_FLOAT32 without
+// quantisation (one == 0), else the smallest unsigned (min >= 0) or signed
+// integer type whose range holds [min, max].
+int display_type(const rh_options& o);
+
+// Bytes of one value of a RH_DSPY_* type; 0 for a type that is not one.
+int display_type_bytes(int type);
+
+// FormatBucketForDisplay's values of one pixel (ddmanager.cpp:1055-1118):
+// r, g, b from its channels 0..2 and a from channel 6 (ch: the pixel's 9
+// channels), quantised as lround(value * one + dither * s) clamped to
+// [min, max] when one != 0, and stored as `type` at dst (4 values).
+void format_pixel(const rh_options& o, const float* ch, double s, int type, unsigned char* dst);
+
+}  // namespace rh

@@ -17,6 +17,7 @@
 
 #include "../../include/aqsis_hider.h"
 #include "devtypes.h"
+#include "display.h"
 #include "front.h"
 
 extern "C" hipError_t rh_launch_dice(hipStream_t, const FrameDev*, const GridDev*, int, const PrimDev*,
@@ -1140,38 +1141,97 @@ rh_status rh_read_rgba8(rh_ctx* c, uint8_t* out) {
     return RH_OK;
 }
 
-rh_status rh_display(rh_ctx* c, rh_dspy_data_fn data, void* image, int32_t scanline_order) {
+rh_status rh_display_format(const rh_options* opt, int32_t* type, int32_t* bytes) {
+    if (!opt) return fail(RH_ERR_INVALID, "null options");
+    const int t = rh::display_type(*opt);
+    if (type) *type = t;
+    if (bytes) *bytes = rh::display_type_bytes(t);
+    return RH_OK;
+}
+
+rh_status rh_format_display(const rh_options* opt, const float* channels, const float* dither, int64_t npix,
+                            int32_t type, void* out) {
+    if (!opt || npix < 0 || (npix > 0 && (!channels || !dither || !out)))
+        return fail(RH_ERR_INVALID, "bad arguments");
+    const int vb = rh::display_type_bytes(type);
+    if (vb == 0) return fail(RH_ERR_INVALID, "unknown display value type " + std::to_string(type));
+    unsigned char* dst = static_cast<unsigned char*>(out);
+    for (int64_t i = 0; i < npix; ++i)
+        rh::format_pixel(*opt, channels + i * 9, (double)dither[i], type, dst + i * 4 * vb);
+    return RH_OK;
+}
+
+// The frame's display values of `type`, width x height x 4 values.
+static rh_status display_values(rh_ctx* c, int32_t type, std::vector<unsigned char>& out) {
+    const rh::Shape& s = c->shape;
+    const size_t npix = (size_t)s.width() * s.height();
+    const int vb = rh::display_type_bytes(type);
+    if (vb == 0) return fail(RH_ERR_INVALID, "unknown display value type " + std::to_string(type));
+    out.resize(npix * 4 * vb);
+    if (type == RH_DSPY_UNSIGNED8 && rh::display_type(s.o) == RH_DSPY_UNSIGNED8) {
+        // the device quantised these already
+        HIP_TRY(hipMemcpy(out.data(), c->dRgba.p, npix * 4, hipMemcpyDeviceToHost));
+        return RH_OK;
+    }
+    std::vector<float> ch(npix * 9);
+    HIP_TRY(hipMemcpy(ch.data(), c->dChannels.p, ch.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < npix; ++i)
+        rh::format_pixel(s.o, &ch[i * 9], (double)c->plan.dither[i], type, &out[i * 4 * vb]);
+    return RH_OK;
+}
+
+rh_status rh_read_display(rh_ctx* c, int32_t type, void* out) {
+    if (!c || !out) return fail(RH_ERR_INVALID, "bad arguments");
+    if (!c->rendered) return fail(RH_ERR_STATE, "no rendered frame");
+    std::vector<unsigned char> v;
+    rh_status r = display_values(c, type, v);
+    if (r != RH_OK) return r;
+    std::memcpy(out, v.data(), v.size());
+    return RH_OK;
+}
+
+rh_status rh_display_as(rh_ctx* c, rh_dspy_data_fn data, void* image, int32_t scanline_order, int32_t type) {
     if (!c || !data) return fail(RH_ERR_INVALID, "bad arguments");
     if (!c->rendered) return fail(RH_ERR_STATE, "no rendered frame");
     const rh::Shape& s = c->shape;
-    const int W = s.width(), H = s.height();
-    if (s.cropX0 != 0 || s.cropY0 != 0 || W != s.o.xres || H != s.o.yres)
-        return fail(RH_ERR_INVALID, "rh_display takes frames without a crop window");
-    std::vector<uint8_t> img((size_t)W * H * 4);
-    HIP_TRY(hipMemcpy(img.data(), c->dRgba.p, img.size(), hipMemcpyDeviceToHost));
-    std::vector<uint8_t> bucket, rows;
-    if (scanline_order) rows.resize((size_t)W * 4 * s.o.bucket_y);
+    const int W = s.width();
+    std::vector<unsigned char> img;
+    rh_status r = display_values(c, type, img);
+    if (r != RH_OK) return r;
+    const size_t entry = 4 * (size_t)rh::display_type_bytes(type);
+    // a pixel (x, y) of the crop window, in raster coordinates
+    auto at = [&](int x, int y) { return &img[((size_t)(y - s.cropY0) * W + (x - s.cropX0)) * entry]; };
+    std::vector<unsigned char> bucket, rows;
+    if (scanline_order) rows.resize((size_t)W * entry * s.o.bucket_y);
     for (const auto& b : s.buckets) {
-        const int x0 = b.x, x1 = b.x + b.w, y0 = b.y, y1 = b.y + b.h;
+        // the bucket's display rectangle inside the crop window
+        const int x0 = std::max(b.x, s.cropX0), x1 = std::min(b.x + b.w, s.cropX1);
+        const int y0 = std::max(b.y, s.cropY0), y1 = std::min(b.y + b.h, s.cropY1);
+        if (x1 <= x0 || y1 <= y0) continue;
+        const size_t span = (size_t)(x1 - x0) * entry;
         if (!scanline_order) {
-            // the bucket's pixels, row by row (FormatBucketForDisplay)
-            bucket.resize((size_t)b.w * b.h * 4);
-            for (int y = y0; y < y1; ++y)
-                std::memcpy(&bucket[(size_t)(y - y0) * b.w * 4], &img[((size_t)y * W + x0) * 4], (size_t)b.w * 4);
-            if (data(image, x0, x1, y0, y1, 4, bucket.data()) != 0)
+            // the rectangle's pixels, row by row (FormatBucketForDisplay)
+            bucket.resize(span * (y1 - y0));
+            for (int y = y0; y < y1; ++y) std::memcpy(&bucket[(size_t)(y - y0) * span], at(x0, y), span);
+            if (data(image, x0, x1, y0, y1, (int)entry, bucket.data()) != 0)
                 return fail(RH_ERR_STATE, "the display driver stopped the delivery");
             continue;
         }
         // CollapseBucketsToScanlines: gather the bucket row, send whole rows
         // when its last bucket has arrived (SendToDisplay)
         for (int y = y0; y < y1; ++y)
-            std::memcpy(&rows[((size_t)(y - y0) * W + x0) * 4], &img[((size_t)y * W + x0) * 4], (size_t)b.w * 4);
-        if (x1 >= W)
+            std::memcpy(&rows[((size_t)(y - y0) * W + (x0 - s.cropX0)) * entry], at(x0, y), span);
+        if (x1 >= s.cropX1)
             for (int y = y0; y < y1; ++y)
-                if (data(image, 0, W, y, y + 1, 4, &rows[(size_t)(y - y0) * W * 4]) != 0)
+                if (data(image, s.cropX0, s.cropX1, y, y + 1, (int)entry, &rows[(size_t)(y - y0) * W * entry]) != 0)
                     return fail(RH_ERR_STATE, "the display driver stopped the delivery");
     }
     return RH_OK;
+}
+
+rh_status rh_display(rh_ctx* c, rh_dspy_data_fn data, void* image, int32_t scanline_order) {
+    if (!c) return fail(RH_ERR_INVALID, "null context");
+    return rh_display_as(c, data, image, scanline_order, rh::display_type(c->shape.o));
 }
 
 rh_status rh_read_samples(rh_ctx* c, uint32_t* mp_id, float* depth, float* rgbaz) {

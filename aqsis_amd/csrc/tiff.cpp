@@ -20,7 +20,7 @@ void set_last_error(const std::string& msg);
 
 namespace {
 
-enum { T_ASCII = 2, T_SHORT = 3, T_LONG = 4, T_RATIONAL = 5 };
+enum { T_ASCII = 2, T_SHORT = 3, T_LONG = 4, T_RATIONAL = 5, T_DOUBLE = 12 };
 
 struct Entry {
     uint16_t tag, type;
@@ -153,9 +153,20 @@ private:
 
 }  // namespace
 
-extern "C" rh_status rh_tiff_write(const char* path, const uint8_t* pixels, int32_t width, int32_t height,
-                                   int32_t channels, int32_t compression, const int32_t* origin,
-                                   const int32_t* full_size, const char* description) {
+namespace {
+
+Entry one_double(uint16_t tag, double x) {
+    Entry e{tag, T_DOUBLE, 1, std::vector<uint8_t>(8)};
+    std::memcpy(e.data.data(), &x, 8);   // little-endian host
+    return e;
+}
+
+// One image of `channels` samples of `bits` bits per pixel.  sample_format
+// 0 writes the 8-bit branch's tags (no SampleFormat, no StoNits); 1 (uint)
+// or 3 (IEEE float) the other branch's.
+rh_status write_tiff(const char* path, const uint8_t* pixels, int32_t width, int32_t height, int32_t channels,
+                     int bits, int sample_format, int32_t compression, const int32_t* origin,
+                     const int32_t* full_size, const char* description) {
     if (!path || !pixels) { rh::set_last_error("null argument"); return RH_ERR_INVALID; }
     if (width <= 0 || height <= 0 || (channels != 3 && channels != 4)) {
         rh::set_last_error("rh_tiff_write: width and height must be positive, channels 3 (rgb) or 4 (rgba)");
@@ -171,7 +182,7 @@ extern "C" rh_status rh_tiff_write(const char* path, const uint8_t* pixels, int3
         rh::set_last_error("rh_tiff_write: origin must be non-negative and the full size positive");
         return RH_ERR_INVALID;
     }
-    const size_t line = (size_t)width * channels;
+    const size_t line = (size_t)width * channels * (bits / 8);
     if ((uint64_t)line * height > 0xF0000000ull) {
         rh::set_last_error("rh_tiff_write: image exceeds a classic TIFF file");
         return RH_ERR_CAPACITY;
@@ -210,7 +221,7 @@ extern "C" rh_status rh_tiff_write(const char* path, const uint8_t* pixels, int3
     std::vector<Entry> tags;
     tags.push_back(longs(256, {(uint32_t)width}));                       // ImageWidth
     tags.push_back(longs(257, {(uint32_t)height}));                      // ImageLength
-    tags.push_back(shorts(258, std::vector<uint32_t>(channels, 8)));     // BitsPerSample
+    tags.push_back(shorts(258, std::vector<uint32_t>(channels, (uint32_t)bits)));   // BitsPerSample
     tags.push_back(shorts(259, {(uint32_t)compression}));                // Compression
     tags.push_back(shorts(262, {2}));                                    // Photometric RGB
     tags.push_back(ascii(270, description ? description : "aqsis_amd REYES hider"));
@@ -228,8 +239,10 @@ extern "C" rh_status rh_tiff_write(const char* path, const uint8_t* pixels, int3
     tags.push_back(ascii(305, "aqsis_amd"));                             // Software
     tags.push_back(ascii(306, datetime));                                // DateTime
     if (channels == 4) tags.push_back(shorts(338, {1}));                 // ExtraSamples associated alpha
+    if (sample_format) tags.push_back(shorts(339, std::vector<uint32_t>(channels, (uint32_t)sample_format)));
     tags.push_back(longs(33300, {(uint32_t)fw}));                        // PIXAR_IMAGEFULLWIDTH
     tags.push_back(longs(33301, {(uint32_t)fh}));                        // PIXAR_IMAGEFULLLENGTH
+    if (sample_format) tags.push_back(one_double(37439, 1.0));           // StoNits
 
     // Values longer than four bytes go before the IFD.
     std::vector<uint32_t> where(tags.size(), 0);
@@ -265,4 +278,33 @@ extern "C" rh_status rh_tiff_write(const char* path, const uint8_t* pixels, int3
         return RH_ERR_INVALID;
     }
     return RH_OK;
+}
+
+}  // namespace
+
+extern "C" rh_status rh_tiff_write(const char* path, const uint8_t* pixels, int32_t width, int32_t height,
+                                   int32_t channels, int32_t compression, const int32_t* origin,
+                                   const int32_t* full_size, const char* description) {
+    return write_tiff(path, pixels, width, height, channels, 8, 0, compression, origin, full_size, description);
+}
+
+// WriteTIFF's second branch (display.cpp:347-411): 16-bit uint for
+// PkDspyUnsigned16, else 32-bit IEEE float samples (the 32-bit unsigned
+// values keep the float tag, as the driver writes them).
+extern "C" rh_status rh_tiff_write_typed(const char* path, const void* pixels, int32_t width, int32_t height,
+                                         int32_t channels, int32_t type, int32_t compression, const int32_t* origin,
+                                         const int32_t* full_size, const char* description) {
+    const uint8_t* p = static_cast<const uint8_t*>(pixels);
+    switch (type) {
+    case RH_DSPY_UNSIGNED8:
+        return write_tiff(path, p, width, height, channels, 8, 0, compression, origin, full_size, description);
+    case RH_DSPY_UNSIGNED16:
+        return write_tiff(path, p, width, height, channels, 16, 1, compression, origin, full_size, description);
+    case RH_DSPY_FLOAT32: case RH_DSPY_UNSIGNED32:
+        return write_tiff(path, p, width, height, channels, 32, 3, compression, origin, full_size, description);
+    default:
+        rh::set_last_error("rh_tiff_write_typed: type must be RH_DSPY_UNSIGNED8, RH_DSPY_UNSIGNED16, "
+                           "RH_DSPY_UNSIGNED32 or RH_DSPY_FLOAT32 (the file driver widens signed types)");
+        return RH_ERR_INVALID;
+    }
 }

@@ -42,10 +42,17 @@ MODES = {"grid": 0, "plain": 1, "batched": 2}
 
 EXPORTS = ["rh_options_default", "rh_create", "rh_post_surfaces", "rh_clear_surfaces", "rh_render",
            "rh_render_device", "rh_render_device_noalloc", "rh_render_stages", "rh_frame_size", "rh_read_channels", "rh_read_rgba8", "rh_display", "rh_tiff_write",
+           "rh_display_format", "rh_read_display", "rh_format_display", "rh_display_as", "rh_tiff_write_typed",
            "rh_read_samples",
            "rh_get_stats", "rh_destroy", "rh_last_error", "rh_estimate", "rh_sample_positions",
            "rh_set_tuning", "rh_get_tuning", "rh_autotune"]
 
+
+# Display value types (RH_DSPY_*, the PkDspy* codes) and their numpy dtypes.
+DSPY = {"float32": 1, "uint32": 2, "int32": 3, "uint16": 4, "int16": 5, "uint8": 6, "int8": 7}
+DSPY_DTYPE = {1: np.float32, 2: np.uint32, 3: np.int32, 4: np.uint16, 5: np.int16, 6: np.uint8, 7: np.int8}
+# The file driver receives signed types as the unsigned type of their width.
+FILE_TYPE = {1: 1, 2: 2, 3: 2, 4: 4, 5: 4, 6: 6, 7: 6}
 
 DSPY_DATA_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p)
 
@@ -74,6 +81,12 @@ def lib():
         L.rh_display.argtypes = [C.c_void_p, DSPY_DATA_FN, C.c_void_p, C.c_int32]
         L.rh_tiff_write.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                     C.c_void_p, C.c_void_p, C.c_char_p]
+        L.rh_display_format.argtypes = [P(RhOptions), P(C.c_int32), P(C.c_int32)]
+        L.rh_read_display.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.rh_format_display.argtypes = [P(RhOptions), C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]
+        L.rh_display_as.argtypes = [C.c_void_p, DSPY_DATA_FN, C.c_void_p, C.c_int32, C.c_int32]
+        L.rh_tiff_write_typed.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_int32, C.c_void_p, C.c_void_p, C.c_char_p]
         L.rh_get_stats.argtypes = [C.c_void_p, P(RhStats)]
         L.rh_destroy.argtypes = [C.c_void_p]
         L.rh_destroy.restype = None
@@ -117,6 +130,12 @@ class Hider:
         v = [C.c_int32() for _ in range(5)]
         _check(lib().rh_frame_size(self._ctx, *[C.byref(x) for x in v]))
         self.width, self.height, self.ext_width, self.ext_height, self.spp = (x.value for x in v)
+        # the crop window's first pixel, ceil(res * crop) in single precision
+        f32 = np.float32
+        self.origin = (min(max(int(np.ceil(f32(opts.xres) * f32(opts.crop[0]))), 0), opts.xres),
+                       min(max(int(np.ceil(f32(opts.yres) * f32(opts.crop[2]))), 0), opts.yres))
+        self.full_size = (opts.xres, opts.yres)
+        self.display_type = display_format(opts)[0]
 
     def post_surfaces(self, prims):
         arr, n = _prims(prims)
@@ -149,28 +168,50 @@ class Hider:
         _check(lib().rh_read_rgba8(self._ctx, _ptr(out)))
         return out
 
-    def display(self, data, scanline_order=False):
-        """Deliver the 8-bit rgba to `data(xmin, xmaxplus1, ymin, ymaxplus1,
-        entry_size, bytes)` as a display driver's DspyImageData receives it
-        (rh_display); a true return value from `data` stops the delivery."""
+    def display_values(self, type=None):
+        """The frame's display values (height, width, 4: r, g, b, a) as
+        FormatBucketForDisplay gives them, of RH_DSPY_* `type` (default the
+        type the Quantize settings select) (rh_read_display)."""
+        t = self.display_type if type is None else (DSPY[type] if isinstance(type, str) else int(type))
+        out = np.zeros((self.height, self.width, 4), DSPY_DTYPE[t])
+        _check(lib().rh_read_display(self._ctx, t, _ptr(out)))
+        return out
+
+    def display(self, data, scanline_order=False, type=None):
+        """Deliver the display values to `data(xmin, xmaxplus1, ymin,
+        ymaxplus1, entry_size, bytes)`, raster coordinates, as a display
+        driver's DspyImageData receives them (rh_display / rh_display_as); a
+        true return value from `data` stops the delivery."""
         def cb(_image, x0, x1, y0, y1, es, ptr):
             n = (x1 - x0) * (y1 - y0) * es
             return 1 if data(x0, x1, y0, y1, es, C.string_at(ptr, n)) else 0
         fn = DSPY_DATA_FN(cb)
-        _check(lib().rh_display(self._ctx, fn, None, 1 if scanline_order else 0))
+        order = 1 if scanline_order else 0
+        if type is None:
+            _check(lib().rh_display(self._ctx, fn, None, order))
+        else:
+            t = DSPY[type] if isinstance(type, str) else int(type)
+            _check(lib().rh_display_as(self._ctx, fn, None, order, t))
 
     def write_tiff(self, path, compression="lzw", scanline_order=False, channels=4):
         """Write the frame to a TIFF file as the "file" display driver does:
-        the driver's image buffer is filled from the rh_display delivery
-        (DspyImageData), then written with rh_tiff_write; channels 4 (rgba)
-        or 3 (rgb)."""
-        img = np.zeros((self.height, self.width, 4), np.uint8)
+        the driver asks for the unsigned type of the frame's display type,
+        fills its image buffer from the delivery (DspyImageData, raster
+        coordinates less the crop window's origin), and writes it with
+        rh_tiff_write_typed; channels 4 (rgba) or 3 (rgb).  A cropped frame
+        carries its origin and the full image size."""
+        t = FILE_TYPE[self.display_type]
+        dt = np.dtype(DSPY_DTYPE[t])
+        img = np.zeros((self.height, self.width, 4), dt)
+        ox, oy = self.origin
 
         def data(x0, x1, y0, y1, es, buf):
-            img[y0:y1, x0:x1] = np.frombuffer(buf, np.uint8).reshape(y1 - y0, x1 - x0, es)
+            img[y0 - oy:y1 - oy, x0 - ox:x1 - ox] = np.frombuffer(buf, dt).reshape(y1 - y0, x1 - x0, 4)
             return False
-        self.display(data, scanline_order)
-        write_tiff(path, img[..., :channels], compression)
+        self.display(data, scanline_order, type=t)
+        cropped = (self.width, self.height) != self.full_size
+        write_tiff(path, img[..., :channels], compression, origin=self.origin if cropped else None,
+                   full_size=self.full_size if cropped else None)
 
     def samples(self):
         n = self.ext_width * self.ext_height * self.spp
@@ -234,16 +275,43 @@ TIFF_COMPRESSION = {"none": 1, "lzw": 5}
 
 
 def write_tiff(path, pixels, compression="lzw", origin=None, full_size=None, description=None):
-    """Write an 8-bit (height, width, 3 or 4) image to `path` as the "file"
-    display driver's integer branch does (rh_tiff_write)."""
-    a = np.ascontiguousarray(pixels, dtype=np.uint8)
-    if a.ndim != 3:
-        raise HiderError("write_tiff: pixels must be (height, width, channels)")
+    """Write a (height, width, 3 or 4) image to `path` as the "file" display
+    driver does: uint8 through its integer branch (rh_tiff_write); uint16,
+    uint32 and float32 through its other branch (rh_tiff_write_typed)."""
+    a = np.ascontiguousarray(pixels)
+    types = {np.dtype(np.uint8): 6, np.dtype(np.uint16): 4, np.dtype(np.uint32): 2, np.dtype(np.float32): 1}
+    if a.ndim != 3 or a.dtype not in types:
+        raise HiderError("write_tiff: pixels must be (height, width, channels) of uint8, uint16, uint32 or float32")
     o = None if origin is None else (C.c_int32 * 2)(*origin)
     f = None if full_size is None else (C.c_int32 * 2)(*full_size)
     d = None if description is None else description.encode()
-    _check(lib().rh_tiff_write(os.fsencode(path), _ptr(a), a.shape[1], a.shape[0], a.shape[2],
-                               TIFF_COMPRESSION[compression], o, f, d))
+    if a.dtype == np.uint8:
+        _check(lib().rh_tiff_write(os.fsencode(path), _ptr(a), a.shape[1], a.shape[0], a.shape[2],
+                                   TIFF_COMPRESSION[compression], o, f, d))
+    else:
+        _check(lib().rh_tiff_write_typed(os.fsencode(path), _ptr(a), a.shape[1], a.shape[0], a.shape[2],
+                                         types[a.dtype], TIFF_COMPRESSION[compression], o, f, d))
+
+
+def format_display(opts, channels, dither, type=None):
+    """Host-only FormatBucketForDisplay of (h, w, 9) channels with (h, w)
+    dither draws: (h, w, 4) values of RH_DSPY_* `type` (default the type the
+    Quantize settings select) (rh_format_display)."""
+    t = display_format(opts)[0] if type is None else (DSPY[type] if isinstance(type, str) else int(type))
+    ch = np.ascontiguousarray(channels, np.float32)
+    d = np.ascontiguousarray(dither, np.float32)
+    if ch.ndim != 3 or ch.shape[2] != 9 or d.shape != ch.shape[:2]:
+        raise HiderError("format_display: channels (h, w, 9) and dither (h, w) expected")
+    out = np.zeros(ch.shape[:2] + (4,), DSPY_DTYPE[t])
+    _check(lib().rh_format_display(C.byref(opts), _ptr(ch), _ptr(d), ch.shape[0] * ch.shape[1], t, _ptr(out)))
+    return out
+
+
+def display_format(opts):
+    """(RH_DSPY_* type, bytes per value) the Quantize settings select."""
+    t, b = C.c_int32(), C.c_int32()
+    _check(lib().rh_display_format(C.byref(opts), C.byref(t), C.byref(b)))
+    return t.value, b.value
 
 
 def estimate(opts, prims):

@@ -256,6 +256,35 @@ rh_status rh_read_channels(rh_ctx* ctx, float* out);
  * ddmanager.cpp:1022-1118). */
 rh_status rh_read_rgba8(rh_ctx* ctx, uint8_t* out);
 
+/* Value types of display data, DspyImageData's (include/aqsis/ri/ndspy.h:69-76). */
+enum {
+    RH_DSPY_FLOAT32 = 1, RH_DSPY_UNSIGNED32 = 2, RH_DSPY_SIGNED32 = 3, RH_DSPY_UNSIGNED16 = 4,
+    RH_DSPY_SIGNED16 = 5, RH_DSPY_UNSIGNED8 = 6, RH_DSPY_SIGNED8 = 7
+};
+
+/* The value type a display request gets for these options' Quantize
+ * settings (selectDataFormat, ddmanager.cpp:249-283): RH_DSPY_FLOAT32 when
+ * quantize_one is 0 (no quantisation), else the smallest unsigned
+ * (quantize_min >= 0) or signed integer type holding [quantize_min,
+ * quantize_max]; *bytes: the bytes of one value.  Host only. */
+rh_status rh_display_format(const rh_options* opt, int32_t* type, int32_t* bytes);
+
+/* The rendered frame's r, g, b, a as a display request of value type `type`
+ * (RH_DSPY_*) receives them (FormatBucketForDisplay, ddmanager.cpp:
+ * 1022-1118): lround(value * one + dither * draw) clamped to [min, max]
+ * when quantize_one != 0, then stored as `type` (a narrower type keeps the
+ * low bits, as the reference's conversions do on x86-64).  4 values per
+ * pixel, width x height pixels row by row.  For RH_DSPY_UNSIGNED8 the bytes
+ * equal rh_read_rgba8's. */
+rh_status rh_read_display(rh_ctx* ctx, int32_t type, void* out);
+
+/* rh_read_display's conversion on host data, for a caller holding channels
+ * (rh_read_channels' layout, 9 floats per pixel) and the per-pixel dither
+ * draws of a display (rh_sample_positions' dither): npix pixels, 4 values
+ * of `type` each into out.  Host only: needs no device. */
+rh_status rh_format_display(const rh_options* opt, const float* channels, const float* dither, int64_t npix,
+                            int32_t type, void* out);
+
 /* A display driver's image-data entry point, with DspyImageData's argument
  * meaning (include/aqsis/ri/ndspy.h:158): the rectangle [xmin, xmaxplus1) x
  * [ymin, ymaxplus1), the bytes of one pixel, and the pixels row by row.  A
@@ -263,15 +292,24 @@ rh_status rh_read_rgba8(rh_ctx* ctx, uint8_t* out);
 typedef int (*rh_dspy_data_fn)(void* image, int xmin, int xmaxplus1, int ymin, int ymaxplus1, int entry_size,
                                const unsigned char* data);
 
-/* Hand the rendered frame's 8-bit rgba to a display driver the way
+/* Hand the rendered frame's rgba to a display driver the way
  * CqDisplayRequest::DisplayBucket does (ddmanager.cpp:984-1019, 1129-1175):
  * bucket by bucket in the render's bucket order, each bucket's display
  * rectangle (imagebuffer.cpp:745-748, bucketprocessor.cpp:107); or, with
- * scanline_order (PkDspyFlagsWantsScanLineOrder), one full-width row at a
- * time as each row of buckets completes.  Entries are 4 bytes (r, g, b, a).
- * Frames with a crop window other than the whole image are refused
- * (RH_ERR_INVALID). */
+ * scanline_order (PkDspyFlagsWantsScanLineOrder), one row at a time as each
+ * row of buckets completes.  Values are of the type rh_display_format
+ * selects (entries of 4 values: r, g, b, a; rh_read_display's values).
+ * Rectangles are in raster coordinates of the whole image.  With a crop
+ * window, buckets wholly outside it are skipped (CqDDManager::DisplayBucket,
+ * ddmanager.cpp:146-171) and each rectangle is the bucket's part inside the
+ * crop window, rows the crop window's width: the reference also sends a
+ * bucket's pixels outside the crop window, whose values it never computes. */
 rh_status rh_display(rh_ctx* ctx, rh_dspy_data_fn data, void* image, int32_t scanline_order);
+
+/* rh_display with the values as `type` (RH_DSPY_*), for a driver that asks
+ * for another type than the one selected (the file driver turns signed
+ * types into unsigned ones, tools/displays/file/display.cpp:455-467). */
+rh_status rh_display_as(rh_ctx* ctx, rh_dspy_data_fn data, void* image, int32_t scanline_order, int32_t type);
 
 /* Compression of rh_tiff_write: TIFF's Compression tag values. */
 enum { RH_TIFF_NONE = 1, RH_TIFF_LZW = 5 };
@@ -290,6 +328,17 @@ enum { RH_TIFF_NONE = 1, RH_TIFF_LZW = 5 };
 rh_status rh_tiff_write(const char* path, const uint8_t* pixels, int32_t width, int32_t height,
                         int32_t channels, int32_t compression, const int32_t* origin,
                         const int32_t* full_size, const char* description);
+
+/* rh_tiff_write for the driver's other sample formats (WriteTIFF's second
+ * branch, display.cpp:347-411, which adds SampleFormat and StoNits = 1):
+ * type RH_DSPY_UNSIGNED8 as rh_tiff_write; RH_DSPY_UNSIGNED16 as 16-bit
+ * unsigned samples; RH_DSPY_FLOAT32 as 32-bit IEEE floating point samples;
+ * RH_DSPY_UNSIGNED32 as the driver writes it, 32-bit samples tagged as
+ * floating point.  pixels: width x height x channels values of that type,
+ * little-endian. */
+rh_status rh_tiff_write_typed(const char* path, const void* pixels, int32_t width, int32_t height,
+                              int32_t channels, int32_t type, int32_t compression, const int32_t* origin,
+                              const int32_t* full_size, const char* description);
 
 /* Per-sample visibility of the extended region, sample s of pixel
  * (x, y) at index (y*ext_width + x)*samples_per_pixel + s:

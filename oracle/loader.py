@@ -125,6 +125,33 @@ def dof(opts, z):
     return coc, cells, mn[:len(z) - 1]
 
 
+DISPLAY_DTYPE = {1: np.float32, 2: np.uint32, 3: np.int32, 4: np.uint16, 5: np.int16, 6: np.uint8, 7: np.int8}
+
+
+def display_type(opts):
+    """selectDataFormat's PkDspy* code for the frame's rgba display."""
+    L = lib()
+    L.oracle_display_type.argtypes = [C.POINTER(RhOptions)]
+    L.oracle_display_type.restype = C.c_int32
+    return L.oracle_display_type(C.byref(opts))
+
+
+def display_values(opts, channels, dither, type=None):
+    """FormatBucketForDisplay's (h, w, 4) values of PkDspy* `type` (default
+    the frame's) from the (h, w, 9) channels and (h, w) dither draws."""
+    L = lib()
+    L.oracle_display_values.argtypes = [C.POINTER(RhOptions), C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                        C.c_void_p]
+    t = display_type(opts) if type is None else type
+    ch = np.ascontiguousarray(channels, np.float32)
+    d = np.ascontiguousarray(dither, np.float32)
+    h, w = ch.shape[:2]
+    assert d.shape == (h, w)
+    out = np.zeros((h, w, 4), DISPLAY_DTYPE[t])
+    L.oracle_display_values(C.byref(opts), _ptr(ch), _ptr(d), h * w, t, _ptr(out))
+    return out
+
+
 def dice(prim, u, v):
     L = lib()
     L.oracle_dice.argtypes = [C.POINTER(RhPrim), C.c_int32, C.c_int32, C.c_void_p]

@@ -16,6 +16,7 @@
 
 #include <chrono>
 #include <climits>
+#include <limits>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -2216,6 +2217,58 @@ int32_t oracle_mp_sample(const float* P, float x, float y, float* depth) {
     V2 uv;
     if (Hider::contains(c, v2(x, y), D, uv)) { if (depth) *depth = D; return 1; }
     return 0;
+}
+
+// selectDataFormat (ddmanager.cpp:249-283) for the rgba request: the
+// PkDspy* code of the values a display is given (1 float32, 2 uint32,
+// 3 int32, 4 uint16, 5 int16, 6 uint8, 7 int8).
+int32_t oracle_display_type(const rh_options* o) {
+    if (o->quantize_one == 0) return 1;
+    if (o->quantize_min >= 0) return o->quantize_max <= 255 ? 6 : (o->quantize_max <= 65535 ? 4 : 2);
+    if (o->quantize_min >= -128 && o->quantize_max <= 127) return 7;
+    if (o->quantize_min >= -32768 && o->quantize_max <= 32767) return 5;
+    return 3;
+}
+
+// FormatBucketForDisplay's per-value path (ddmanager.cpp:1045-1118) over
+// npix pixels: r, g, b = channels 0..2 and a = channel 6 of each pixel's
+// 9, quantised with the pixel's dither draw s and stored as `type`, with
+// the reference's own static_casts (x86-64 g++ code, as its build).
+void oracle_display_values(const rh_options* o, const float* channels, const float* dither, int64_t npix,
+                           int32_t type, void* out) {
+    unsigned char* pdata = static_cast<unsigned char*>(out);
+    const float zero = 0.0f;
+    for (int64_t i = 0; i < npix; ++i) {
+        const double s = dither[i];
+        const float* ch = channels + i * 9;
+        const float vals[4] = {ch[0], ch[1], ch[2], ch[6]};
+        for (int k = 0; k < 4; ++k) {
+            double value = vals[k];
+            if (o->quantize_one != 0) {
+                // Aqsis::lround (floor(x - 0.5) + 1), not the C library's
+                value = (double)orc::lround(zero + value * (double)(o->quantize_one - zero) + ((double)o->quantize_dither * s));
+                value = tclamp<double>(value, o->quantize_min, o->quantize_max);
+            }
+            switch (type) {
+            case 1: { float v = (float)value; memcpy(pdata, &v, 4); pdata += 4; break; }
+            case 2: {
+                value = tclamp<double>(value, 0, (double)std::numeric_limits<uint32_t>::max());
+                uint32_t v = static_cast<uint32_t>(value);
+                memcpy(pdata, &v, 4); pdata += 4; break;
+            }
+            case 3: {
+                value = tclamp<double>(value, (double)std::numeric_limits<int32_t>::min(),
+                                       (double)std::numeric_limits<int32_t>::max());
+                int32_t v = static_cast<int32_t>(value);
+                memcpy(pdata, &v, 4); pdata += 4; break;
+            }
+            case 4: { uint16_t v = static_cast<uint16_t>(value); memcpy(pdata, &v, 2); pdata += 2; break; }
+            case 5: { int16_t v = static_cast<int16_t>(value); memcpy(pdata, &v, 2); pdata += 2; break; }
+            case 6: { uint8_t v = static_cast<uint8_t>(value); *pdata++ = v; break; }
+            default: { int8_t v = static_cast<int8_t>(value); memcpy(pdata, &v, 1); pdata += 1; break; }
+            }
+        }
+    }
 }
 
 }  // extern "C"

@@ -76,6 +76,14 @@ void oracle_jitter_tables(int32_t nx, int32_t ny, float* pos2d, float* pos1d,
  * (iu,iv),(iu+1,iv),(iu,iv+1),(iu+1,iv+1): returns 1 and depth if hit. */
 int32_t oracle_mp_sample(const float* P /* 12 */, float x, float y, float* depth);
 
+/* selectDataFormat's PkDspy* code for the rgba request. */
+int32_t oracle_display_type(const rh_options* opt);
+
+/* FormatBucketForDisplay's values (npix x 4 of `type`) from the 9 channels
+ * and the dither draw of each pixel. */
+void oracle_display_values(const rh_options* opt, const float* channels, const float* dither, int64_t npix,
+                           int32_t type, void* out);
+
 #ifdef __cplusplus
 }
 #endif
