@@ -83,6 +83,8 @@ class RhStats(C.Structure):
         ("ms_dice", C.c_double), ("ms_bin", C.c_double), ("ms_sample", C.c_double),
         ("ms_filter", C.c_double), ("ms_device", C.c_double), ("ms_composite", C.c_double),
         ("cull_probes", C.c_uint64), ("cull_passes", C.c_uint64), ("ms_cull", C.c_double),
+        ("ms_forest", C.c_double), ("ms_simulate", C.c_double), ("ms_cull_depth", C.c_double),
+        ("ms_cull_probes", C.c_double),
     ]
 
     def as_dict(self):

@@ -132,6 +132,12 @@ struct SampleLaunch {
     int rowBase;
     uint32_t idCut;                       // only list entries below this id (0xffffffff: all)
     int depthOnly;                        // opaque depths only: no transparent hits are kept
+    // Probing several buckets in one launch: per item (the static sampler's
+    // items, the others' workgroups from itemBase) its own cut, replacing
+    // idCut when set; for the one-workgroup-per-task samplers (motion, depth
+    // of field) the task of each such workgroup (null: consecutive tasks).
+    const uint32_t* itemCut;
+    const uint32_t* taskList;
 };
 
 // Lens tables of a frame with depth of field: per pattern the

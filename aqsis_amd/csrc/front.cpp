@@ -3,6 +3,7 @@
 #include "front.h"
 
 #include <algorithm>
+#include <functional>
 #include <atomic>
 #include <random>
 #include <thread>
@@ -831,21 +832,70 @@ void build_tree(const Ctx& c, int prim, Tree& tr) {
 
 // The bucket loop's view of a node: what posting, popping, culling and
 // reposting read, packed so that the loop walks a few megabytes.
-enum { SN_LEAF = 1, SN_BOUND = 2, SN_CULLED = 4, SN_MOTION = 8 };
+enum { SN_LEAF = 1, SN_BOUND = 2, SN_CULLED = 4, SN_MOTION = 8, SN_CULLABLE = 16 };
 struct SimNode {
     float z, subMin;            // cached bound's minimum z; least in the subtree
     float bx0, by0, bx1, by1;   // cached raster bound in x and y
     int16_t x0, y0, x1, y1;     // PostSurface's bucket range, clamped to the region
     uint32_t kid0;              // forest index of the first child
     int32_t prim;
+    uint32_t param;             // a leaf's dicing parameters in Forest::params
+    uint32_t pairStart;         // the node's first (surface, bucket) cull pair
     uint16_t cu, cv;            // dice sizes of a leaf
-    uint8_t nkids, flags;
+    uint8_t nkids, flags, type;
 };
+
+// (surface, bucket) pairs an occlusion test may ask about a node: every
+// bucket of its range when it has a bound (PostSurface's range,
+// imagebuffer.cpp:398-407, clamped to the region: posting and reposting keep
+// it inside), none otherwise.
+uint64_t node_pairs(const Shape& s, const Node& nd) {
+    if (!nd.hasBound || nd.culled) return 0;
+    const int x0 = clampv(nd.xminb, s.bx0, s.bx1 - 1), y0 = clampv(nd.yminb, s.by0, s.by1 - 1);
+    const int x1 = clampv(nd.xmaxb, s.bx0, s.bx1 - 1), y1 = clampv(nd.ymaxb, s.by0, s.by1 - 1);
+    return (x1 >= x0 && y1 >= y0) ? (uint64_t)(x1 - x0 + 1) * (uint64_t)(y1 - y0 + 1) : 0;
+}
+
+// Floats of a leaf's dicing parameters (layouts: front.h).
+uint32_t leaf_param_floats(const Node& nd) {
+    const uint32_t keys = nd.motion ? 2 : 1;
+    const uint32_t pk = nd.type == RH_PRIM_SPHERE ? (uint32_t)(18 + 2 * (nd.uDice + 1) + 2 * (nd.vDice + 1))
+                                                  : (nd.type == RH_PRIM_BICUBIC ? 48u : 12u);
+    return pk * keys + (keys == 2 ? 2 : 0);
+}
+
+// A leaf's dicing parameters: each key in turn, then a deforming surface's
+// two key times (its keys share the dice sizes, CopySplitInfo).
+void write_leaf_params(const Ctx& c, const Tree& tr, size_t local, float* W) {
+    const Node& nd = tr.n[local];
+    for (int key = 0; key < (nd.motion ? 2 : 1); ++key) {
+        if (nd.type == RH_PRIM_SPHERE) {
+            const m44& M = key ? c.objToCam1[nd.prim] : c.objToCam[nd.prim];
+            *W++ = nd.radius;
+            for (int r = 0; r < 4; ++r)
+                for (int cc = 0; cc < 4; ++cc) *W++ = M.e[r][cc];
+            *W++ = M.id ? 1.0f : 0.0f;
+            sincos_grid(deg2rad(nd.thetaMin), deg2rad(nd.thetaMax), nd.uDice + 1, W, W + nd.uDice + 1);
+            W += 2 * (nd.uDice + 1);
+            sincos_grid(nd.phiMin, nd.phiMax, nd.vDice + 1, W, W + nd.vDice + 1);
+            W += 2 * (nd.vDice + 1);
+        } else {
+            const int kk = nd.type == RH_PRIM_BICUBIC ? 16 : 4;
+            const f3* cp = key ? &tr.cp1[16 * local] : nd.cp;
+            for (int j = 0; j < kk; ++j) { *W++ = cp[j].x; *W++ = cp[j].y; *W++ = cp[j].z; }
+        }
+    }
+    if (nd.motion) {
+        *W++ = c.prims[nd.prim].key_time[0];
+        *W++ = c.prims[nd.prim].key_time[1];
+    }
+}
 
 // A bucket heap entry carries the comparator's inputs so that heap moves
 // and comparisons do not touch the nodes.
 struct HeapEnt {
     float z;                    // cached raster bound's minimum z
+    float subMin;               // least minimum z in the node's subtree
     bool hasBound;
     uint32_t nd;                // forest index
 };
@@ -864,8 +914,17 @@ struct Forest {
     std::vector<Tree> trees;               // one split tree per primitive, parents first
     std::vector<uint64_t> nodeBase;        // primitives + 1: forest index of each tree's root
     std::vector<SimNode> sim;              // every node, in forest order
+    std::unique_ptr<float[]> params;       // every leaf's dicing parameters, in forest order
+    uint64_t nparams = 0;
+    uint64_t leaves = 0;
+    uint64_t npairs = 0;                   // cull pairs of all nodes (SimNode::pairStart)
     int threads = 1;
 };
+
+const float* forest_params(const Forest& f, uint64_t& n) {
+    n = f.nparams;
+    return f.params.get();
+}
 
 void ForestFree::operator()(Forest* f) const { delete f; }
 
@@ -926,7 +985,10 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
             }
         }
     }
-    // Split trees, one per primitive, in parallel.
+    // Split trees, one per primitive, in parallel; each tree's leaf
+    // parameter floats in paramBase[i + 1], its leaves in leafCount[i].
+    std::vector<uint64_t> paramBase(n + 1, 0), pairBase(n + 1, 0);
+    std::vector<uint32_t> leafCount(n, 0);
     f.trees.assign(n, Tree());
     f.threads = std::max(1, threads);
     int nt = std::max(1, std::min(threads, n));
@@ -939,12 +1001,22 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
             std::vector<Node>& t = f.trees[i].n;
             // the least minimum depth of each node's subtree (children follow
             // their parents in the tree)
+            uint64_t pf = 0, pp = 0;
+            uint32_t nl = 0;
             for (size_t j = t.size(); j-- > 0;) {
                 Node& nd = t[j];
                 float m = nd.hasBound ? nd.rb.lo.z : FLT_MAX;
                 for (int q = 0; q < nd.nkids; ++q) m = mn(m, t[nd.kid0 + q].subMin);
                 nd.subMin = m;
+                if (nd.leaf) {
+                    pf += leaf_param_floats(nd);
+                    nl++;
+                }
+                pp += node_pairs(s, nd);
             }
+            paramBase[i + 1] = pf;
+            pairBase[i + 1] = pp;
+            leafCount[i] = nl;
         }
     };
     if (nt == 1) work();
@@ -959,6 +1031,24 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
         err = "the scene splits into more than 2^32 surfaces";
         return false;
     }
+    // each tree's leaf parameters start after the previous trees'
+    for (int i = 0; i < n; ++i) {
+        paramBase[i + 1] += paramBase[i];
+        pairBase[i + 1] += pairBase[i];
+        f.leaves += leafCount[i];
+    }
+    f.npairs = pairBase[n];
+    if (f.npairs >= 0xffffffffull) {
+        err = "the scene's occlusion tests exceed 2^32 (surface, bucket) pairs";
+        return false;
+    }
+    if (paramBase[n] >= 0xffffffffull) {
+        err = "the scene's dicing parameters exceed 2^32 floats";
+        return false;
+    }
+    // (not zeroed: every float is written below)
+    f.nparams = paramBase[n];
+    f.params.reset(new float[std::max<uint64_t>(f.nparams, 1)]);
     // the bucket loop's packed nodes
     f.sim.assign(f.nodeBase[n], SimNode());
     next = 0;
@@ -967,9 +1057,18 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
             int i = next.fetch_add(1);
             if (i >= n) break;
             const std::vector<Node>& t = f.trees[i].n;
+            uint64_t po = paramBase[i], pq = pairBase[i];
             for (size_t j = 0; j < t.size(); ++j) {
                 const Node& nd = t[j];
                 SimNode& q = f.sim[f.nodeBase[i] + j];
+                q.type = (uint8_t)nd.type;
+                q.param = (uint32_t)po;
+                q.pairStart = (uint32_t)pq;
+                pq += node_pairs(s, nd);
+                if (nd.leaf) {
+                    write_leaf_params(c, f.trees[i], j, &f.params[po]);
+                    po += leaf_param_floats(nd);
+                }
                 q.z = nd.rb.lo.z;
                 q.subMin = nd.subMin;
                 q.bx0 = nd.rb.lo.x; q.by0 = nd.rb.lo.y; q.bx1 = nd.rb.hi.x; q.by1 = nd.rb.hi.y;
@@ -982,7 +1081,8 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
                 q.cu = (uint16_t)nd.uDice; q.cv = (uint16_t)nd.vDice;
                 q.nkids = (uint8_t)nd.nkids;
                 q.flags = (uint8_t)((nd.leaf ? SN_LEAF : 0) | (nd.hasBound ? SN_BOUND : 0) |
-                                    (nd.culled ? SN_CULLED : 0) | (nd.motion ? SN_MOTION : 0));
+                                    (nd.culled ? SN_CULLED : 0) |
+                                    (prims[i].no_cull_hidden ? 0 : SN_CULLABLE) | (nd.motion ? SN_MOTION : 0));
             }
         }
     };
@@ -999,29 +1099,28 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
     return true;
 }
 
+template <typename Fn> static void parallel_for(int n, int threads, Fn fn);
+
 void cull_pairs(const Shape& s, const Forest& f, CullPairs& out) {
-    const size_t nn = f.nodeBase.back();
-    out.bound.assign(nn * 4, 0.0f);
-    out.range.assign(nn * 4, 0);
-    out.start.assign(nn + 1, 0);
-    for (size_t p = 0; p + 1 < f.nodeBase.size(); ++p) {
-        const auto& t = f.trees[p].n;
-        for (size_t j = 0; j < t.size(); ++j) {
-            const size_t g = f.nodeBase[p] + j;
-            const Node& nd = t[j];
-            int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
-            if (nd.hasBound && !nd.culled) {
-                // PostSurface's bucket range of a diceable surface (imagebuffer.cpp:398-407)
-                x0 = clampv(nd.xminb, s.bx0, s.bx1 - 1); y0 = clampv(nd.yminb, s.by0, s.by1 - 1);
-                x1 = clampv(nd.xmaxb, s.bx0, s.bx1 - 1); y1 = clampv(nd.ymaxb, s.by0, s.by1 - 1);
-                out.bound[4 * g] = nd.rb.lo.x; out.bound[4 * g + 1] = nd.rb.lo.y;
-                out.bound[4 * g + 2] = nd.rb.hi.x; out.bound[4 * g + 3] = nd.rb.hi.y;
-            }
-            out.range[4 * g] = x0; out.range[4 * g + 1] = y0; out.range[4 * g + 2] = x1; out.range[4 * g + 3] = y1;
-            const uint64_t cnt = (x1 >= x0 && y1 >= y0) ? (uint64_t)(x1 - x0 + 1) * (uint64_t)(y1 - y0 + 1) : 0;
-            out.start[g + 1] = out.start[g] + cnt;
+    (void)s;
+    const size_t nn = f.sim.size();
+    out.bound.resize(nn * 4);
+    out.range.resize(nn * 4);
+    out.start.resize(nn + 1);
+    const int nchunk = (int)std::min<size_t>((size_t)std::max(1, f.threads), nn / 4096 + 1);
+    const size_t per = (nn + nchunk - 1) / nchunk;
+    parallel_for(nchunk, f.threads, [&](int ch) {
+        for (size_t g = ch * per; g < std::min(nn, (ch + 1) * per); ++g) {
+            const SimNode& q = f.sim[g];
+            const bool tested = (q.flags & SN_BOUND) && !(q.flags & SN_CULLED);
+            out.bound[4 * g] = tested ? q.bx0 : 0.0f; out.bound[4 * g + 1] = tested ? q.by0 : 0.0f;
+            out.bound[4 * g + 2] = tested ? q.bx1 : 0.0f; out.bound[4 * g + 3] = tested ? q.by1 : 0.0f;
+            out.range[4 * g] = tested ? q.x0 : 0; out.range[4 * g + 1] = tested ? q.y0 : 0;
+            out.range[4 * g + 2] = tested ? q.x1 : -1; out.range[4 * g + 3] = tested ? q.y1 : -1;
+            out.start[g] = q.pairStart;
         }
-    }
+    });
+    out.start[nn] = f.npairs;
 }
 
 uint64_t cull_trees(const Shape& s, std::vector<CullTreeDev>& out) {
@@ -1086,9 +1185,56 @@ template <typename Fn> static void parallel_for(int n, int threads, Fn fn) {
     for (auto& th : pool) th.join();
 }
 
+static void finish_schedule(const Shape& s, const Forest& f, Schedule& out);
+
+// The grid of a leaf diced in region bucket k, its first micropolygon id
+// and vertex given; its parameters are the forest's.
+static bool leaf_ok(const SimNode& q, std::string& err) {
+    if ((q.cu + 1) * (q.cv + 1) > 514 || q.cu < 1 || q.cv < 1) {
+        err = "internal: grid size outside the dicing limits";
+        return false;
+    }
+    return true;
+}
+static uint32_t leaf_vertices(const SimNode& q) {
+    return (uint32_t)((q.cu + 1) * (q.cv + 1)) * ((q.flags & SN_MOTION) ? 2u : 1u);
+}
+static GridRec make_grid(const Shape& s, const SimNode& q, int k, uint32_t mpBase, uint32_t vtxBase) {
+    const rh_options& o = s.o;
+    auto bidx = [](float v, int bs) { return (int)std::floor((double)v / bs); };
+    const bool motion = (q.flags & SN_MOTION) != 0;
+    GridRec gr;
+    gr.type = q.type | (motion ? GRID_MOTION : 0);
+    gr.cu = q.cu; gr.cv = q.cv; gr.prim = q.prim;
+    gr.mpBase = mpBase;
+    gr.vtxBase = vtxBase;
+    gr.paramOff = q.param;
+    gr.diceBucket = k;
+    // Candidate buckets: the cached raster bound (already grown by half the
+    // filter width) plus a one-pixel guard band.
+    gr.cbx0 = (int16_t)clampv(bidx(q.bx0 - 1.0f, o.bucket_x), s.bx0, s.bx1 - 1);
+    gr.cby0 = (int16_t)clampv(bidx(q.by0 - 1.0f, o.bucket_y), s.by0, s.by1 - 1);
+    gr.cbx1 = (int16_t)clampv(bidx(q.bx1 + 1.0f, o.bucket_x), s.bx0, s.bx1 - 1);
+    gr.cby1 = (int16_t)clampv(bidx(q.by1 + 1.0f, o.bucket_y), s.by0, s.by1 - 1);
+    gr.px0 = q.bx0 - 2.0f; gr.py0 = q.by0 - 2.0f;
+    gr.px1 = q.bx1 + 2.0f; gr.py1 = q.by1 + 2.0f;
+    return gr;
+}
+
+// Record a leaf diced in region bucket k after the grids diced so far.
+static bool grid_record(const Shape& s, const SimNode& q, int k, uint32_t& mpCount, uint32_t& vtxCount,
+                        Schedule& out, std::string& err) {
+    if (!leaf_ok(q, err)) return false;
+    out.grids.push_back(make_grid(s, q, k, mpCount, vtxCount));
+    if (q.flags & SN_MOTION) out.motion = true;
+    mpCount += (uint32_t)(q.cu * q.cv);
+    vtxCount += leaf_vertices(q);
+    return true;
+}
+
 bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, const CullInput* cull,
               const CullPairs* pairs, Schedule& out, std::string& err) {
-    const Ctx& c = f.c;
+    (void)pairs;   // the pairs' ranges and starts are the packed nodes'
     const std::vector<SimNode>& sn = f.sim;
     std::vector<CullTreeDev> ctrees;
     if (cull) cull_trees(s, ctrees);
@@ -1108,7 +1254,7 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
         if (q.flags & SN_CULLED) return;
         const int x0 = q.x0, y0 = q.y0, x1 = q.x1, y1 = q.y1;
         int k = s.bucketIndex(x0, y0);
-        const HeapEnt ent{q.z, (q.flags & SN_BOUND) != 0, g};
+        const HeapEnt ent{q.z, q.subMin, (q.flags & SN_BOUND) != 0, g};
         if (!processed[k]) {
             push(k, ent);
             return;
@@ -1135,7 +1281,7 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
         const auto& b = s.buckets[k];
         const int col = s.bx0 + k % s.nbr, row = s.by0 + k / s.nbr;
         int nextX = col + 1, nextY = row;
-        const HeapEnt ent{q.z, (q.flags & SN_BOUND) != 0, g};
+        const HeapEnt ent{q.z, q.subMin, (q.flags & SN_BOUND) != 0, g};
         if (nextX < s.bx1 && q.bx1 >= (float)(b.x + b.w)) {
             push(s.bucketIndex(nextX, nextY), ent);
             return;
@@ -1145,8 +1291,9 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
         if (nextX < s.bx1 && nextY < s.by1 && q.by1 >= (float)(b.y + b.h)) push(s.bucketIndex(nextX, nextY), ent);
     };
     for (int i = 0; i < n; ++i) post((uint32_t)f.nodeBase[i]);
-    // the dicing order: (node, bucket) of each grid
-    std::vector<std::pair<uint32_t, int32_t>> diced;
+    // Each grid is recorded as it is diced: its ids and vertices follow the
+    // dicing order, its parameters are the forest's.
+    out.grids.reserve(f.leaves);
     uint32_t mpCount = 0, vtxCount = 0;
     for (int k = 0; k < nb; ++k) {
         auto& h = heaps[k];
@@ -1154,18 +1301,19 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
             const uint32_t g = h.front().nd;
             std::pop_heap(h.begin(), h.end(), HeapCmp());
             h.pop_back();
+            // the next surface's node, while this one is handled
+            if (!h.empty()) __builtin_prefetch(&sn[h.front().nd]);
             const SimNode& q = sn[g];
             // RenderSurface's occlusion cull (bucketprocessor.cpp:945-957):
             // Attribute "cull" "hidden" (default 1) and a cached bound.
-            if (cull && (q.flags & SN_BOUND) && !prims[q.prim].no_cull_hidden) {
-                const int32_t* R = &pairs->range[4 * (size_t)g];
+            if (cull && (q.flags & (SN_BOUND | SN_CULLABLE)) == (SN_BOUND | SN_CULLABLE)) {
                 const int col = s.bx0 + k % s.nbr, row = s.by0 + k / s.nbr;
-                if (col < R[0] || col > R[2] || row < R[1] || row > R[3]) {
+                if (col < q.x0 || col > q.x1 || row < q.y0 || row > q.y1) {
                     err = "internal: occlusion test outside a surface's bucket range";
                     return false;
                 }
-                const uint64_t pi = pairs->start[g] + (uint64_t)(row - R[1]) * (uint64_t)(R[2] - R[0] + 1) +
-                                    (uint64_t)(col - R[0]);
+                const uint64_t pi = (uint64_t)q.pairStart + (uint64_t)(row - q.y0) * (uint64_t)(q.x1 - q.x0 + 1) +
+                                    (uint64_t)(col - q.x0);
                 const float z = q.z, M = cull->pairMax[pi];
                 bool culled = M < z;
                 auto fx = std::lower_bound(cull->fixed.begin(), cull->fixed.end(), std::make_pair(pi, (uint8_t)0));
@@ -1183,7 +1331,7 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
                         bool unordered = false;
                         for (const HeapEnt& e : h) {
                             if (!e.hasBound) unordered = true;
-                            else lim = mn(lim, sn[e.nd].subMin);
+                            else lim = mn(lim, e.subMin);
                         }
                         settled = !unordered && M < lim - margin;
                     } else {
@@ -1210,13 +1358,7 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
                 }
             }
             if (q.flags & SN_LEAF) {
-                if ((q.cu + 1) * (q.cv + 1) > 514 || q.cu < 1 || q.cv < 1) {
-                    err = "internal: grid size outside the dicing limits";
-                    return false;
-                }
-                diced.emplace_back(g, k);
-                mpCount += (uint32_t)(q.cu * q.cv);
-                vtxCount += (uint32_t)((q.cu + 1) * (q.cv + 1)) * ((q.flags & SN_MOTION) ? 2u : 1u);
+                if (!grid_record(s, q, k, mpCount, vtxCount, out, err)) return false;
             } else {
                 // children live in the same tree as their parent
                 for (int j = 0; j < q.nkids; ++j) post(q.kid0 + (uint32_t)j);
@@ -1226,106 +1368,52 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
     }
     out.micropolygons = mpCount;
     out.vertices = vtxCount;
+    out.paramFloats = f.nparams;
+    finish_schedule(s, f, out);
+    return true;
+}
 
-    // The grids in dicing order: ids, vertex and parameter offsets first
-    // (running sums), then the records and parameters in parallel.
-    const size_t ng = diced.size();
-    out.grids.resize(ng);
-    std::vector<uint64_t> poff(ng + 1, 0);
-    {
-        uint32_t mp = 0, vx = 0;
-        for (size_t i = 0; i < ng; ++i) {
-            const SimNode& q = sn[diced[i].first];
-            const int keys = (q.flags & SN_MOTION) ? 2 : 1;
-            const int type = f.trees[q.prim].n.empty() ? 0 : f.trees[q.prim].n[0].type;
-            uint64_t pk = type == RH_PRIM_SPHERE ? (uint64_t)(18 + 2 * (q.cu + 1) + 2 * (q.cv + 1))
-                                                 : (type == RH_PRIM_BICUBIC ? 48u : 12u);
-            poff[i + 1] = poff[i] + pk * keys + (keys == 2 ? 2 : 0);
-            GridRec& gr = out.grids[i];
-            gr.mpBase = mp;
-            gr.vtxBase = vx;
-            mp += (uint32_t)(q.cu * q.cv);
-            vx += (uint32_t)((q.cu + 1) * (q.cv + 1)) * (uint32_t)keys;
-            if (keys == 2) out.motion = true;
-        }
-    }
-    if (poff[ng] >= 0xffffffffull) {
-        err = "the schedule's dicing parameters exceed 2^32 floats";
-        return false;
-    }
-    std::vector<float>& P = out.params;
-    P.assign(poff[ng], 0.0f);
-    const int chunks = (int)((ng + 255) / 256);
-    parallel_for(chunks, f.threads, [&](int ch) {
-        const size_t i0 = (size_t)ch * 256, i1 = std::min(ng, i0 + 256);
-        for (size_t i = i0; i < i1; ++i) {
-            const uint32_t g = diced[i].first;
-            const SimNode& q = sn[g];
-            const Tree& tr = f.trees[q.prim];
-            const size_t local = g - f.nodeBase[q.prim];
-            const Node& nd = tr.n[local];
-            GridRec& gr = out.grids[i];
-            gr.type = nd.type; gr.cu = nd.uDice; gr.cv = nd.vDice; gr.prim = nd.prim;
-            gr.paramOff = (uint32_t)poff[i];
-            gr.diceBucket = diced[i].second;
-            // Candidate buckets: the cached raster bound (already grown by
-            // half the filter width) plus a one-pixel guard band.
-            const rh_options& o = s.o;
-            auto bidx = [](float v, int bs) { return (int)std::floor((double)v / bs); };
-            gr.cbx0 = (int16_t)clampv(bidx(nd.rb.lo.x - 1.0f, o.bucket_x), s.bx0, s.bx1 - 1);
-            gr.cby0 = (int16_t)clampv(bidx(nd.rb.lo.y - 1.0f, o.bucket_y), s.by0, s.by1 - 1);
-            gr.cbx1 = (int16_t)clampv(bidx(nd.rb.hi.x + 1.0f, o.bucket_x), s.bx0, s.bx1 - 1);
-            gr.cby1 = (int16_t)clampv(bidx(nd.rb.hi.y + 1.0f, o.bucket_y), s.by0, s.by1 - 1);
-            gr.px0 = nd.rb.lo.x - 2.0f; gr.py0 = nd.rb.lo.y - 2.0f;
-            gr.px1 = nd.rb.hi.x + 2.0f; gr.py1 = nd.rb.hi.y + 2.0f;
-            // parameters of each key in turn; a deforming surface's keys
-            // share the dice sizes (CopySplitInfo) and its key times follow
-            float* W = &P[poff[i]];
-            for (int key = 0; key < (nd.motion ? 2 : 1); ++key) {
-                if (nd.type == RH_PRIM_SPHERE) {
-                    const m44& M = key ? c.objToCam1[nd.prim] : c.objToCam[nd.prim];
-                    *W++ = nd.radius;
-                    for (int r = 0; r < 4; ++r)
-                        for (int cc = 0; cc < 4; ++cc) *W++ = M.e[r][cc];
-                    *W++ = M.id ? 1.0f : 0.0f;
-                    sincos_grid(deg2rad(nd.thetaMin), deg2rad(nd.thetaMax), gr.cu + 1, W, W + gr.cu + 1);
-                    W += 2 * (gr.cu + 1);
-                    sincos_grid(nd.phiMin, nd.phiMax, gr.cv + 1, W, W + gr.cv + 1);
-                    W += 2 * (gr.cv + 1);
-                } else {
-                    const int kk = nd.type == RH_PRIM_BICUBIC ? 16 : 4;
-                    const f3* cp = key ? &tr.cp1[16 * local] : nd.cp;
-                    for (int j = 0; j < kk; ++j) { *W++ = cp[j].x; *W++ = cp[j].y; *W++ = cp[j].z; }
-                }
-            }
-            if (nd.motion) {
-                gr.type |= GRID_MOTION;
-                *W++ = prims[nd.prim].key_time[0];
-                *W++ = prims[nd.prim].key_time[1];
-            }
-        }
-    });
-
+// The bucket lists of a schedule's grids (in dicing order).
+static void finish_schedule(const Shape& s, const Forest& f, Schedule& out) {
+    const int nb = (int)s.buckets.size();
     // Candidate lists per bucket (grids in dicing order), only buckets not
-    // yet processed when the grid was diced.
-    out.candStart.assign(nb + 1, 0);
-    for (const auto& g : out.grids)
-        for (int by = g.cby0; by <= g.cby1; ++by)
-            for (int bx = g.cbx0; bx <= g.cbx1; ++bx) {
-                int k = s.bucketIndex(bx, by);
-                if (k >= g.diceBucket) out.candStart[k + 1]++;
-            }
-    for (int k = 0; k < nb; ++k) out.candStart[k + 1] += out.candStart[k];
-    out.candGrid.assign(out.candStart[nb], 0);
-    std::vector<uint32_t> fill(out.candStart.begin(), out.candStart.end() - 1);
-    for (uint32_t gi = 0; gi < out.grids.size(); ++gi) {
+    // yet processed when the grid was diced: counted per chunk of grids in
+    // parallel, then each chunk writes its part of every bucket's list.
+    const uint32_t ng = (uint32_t)out.grids.size();
+    const int nchunk = (int)std::min<uint32_t>((uint32_t)std::max(1, f.threads), ng / 1024 + 1);
+    const uint32_t per = (ng + nchunk - 1) / nchunk;
+    std::vector<uint32_t> cnt((size_t)nchunk * nb, 0);
+    auto each_cand = [&](uint32_t gi, auto fn) {
         const auto& g = out.grids[gi];
         for (int by = g.cby0; by <= g.cby1; ++by)
             for (int bx = g.cbx0; bx <= g.cbx1; ++bx) {
-                int k = s.bucketIndex(bx, by);
-                if (k >= g.diceBucket) out.candGrid[fill[k]++] = gi;
+                const int kk = s.bucketIndex(bx, by);
+                if (kk >= g.diceBucket) fn(kk);
             }
+    };
+    parallel_for(nchunk, f.threads, [&](int ch) {
+        uint32_t* C = &cnt[(size_t)ch * nb];
+        for (uint32_t gi = ch * per; gi < std::min(ng, (ch + 1) * per); ++gi) each_cand(gi, [&](int kk) { C[kk]++; });
+    });
+    out.candStart.assign(nb + 1, 0);
+    {
+        uint32_t run = 0;
+        for (int k = 0; k < nb; ++k) {
+            out.candStart[k] = run;
+            for (int ch = 0; ch < nchunk; ++ch) {
+                const uint32_t c0 = cnt[(size_t)ch * nb + k];
+                cnt[(size_t)ch * nb + k] = run;      // the chunk's first slot in bucket k
+                run += c0;
+            }
+        }
+        out.candStart[nb] = run;
     }
+    out.candGrid.resize(out.candStart[nb]);
+    parallel_for(nchunk, f.threads, [&](int ch) {
+        uint32_t* C = &cnt[(size_t)ch * nb];
+        for (uint32_t gi = ch * per; gi < std::min(ng, (ch + 1) * per); ++gi)
+            each_cand(gi, [&](int kk) { out.candGrid[C[kk]++] = gi; });
+    });
     // Each task's list capacity, bucket by bucket in parallel: a
     // micropolygon enters a task's list only if its pixel window meets the
     // task, and a static one's window lies inside the grid's guarded bound.
@@ -1333,8 +1421,10 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
     // to anywhere in the bucket.
     const int ntask = (int)s.tasks.size();
     out.capStart.assign(ntask + 1, 0);
+    out.segBucket.resize(out.candGrid.size());
     parallel_for(nb, f.threads, [&](int k) {
         for (uint32_t e = out.candStart[k]; e < out.candStart[k + 1]; ++e) {
+            out.segBucket[e] = (uint32_t)k;
             const auto& g = out.grids[out.candGrid[e]];
             const bool anyTask = (g.type & GRID_MOTION) != 0;
             for (int t = s.taskStart[k]; t < s.taskStart[k + 1]; ++t) {
@@ -1346,9 +1436,81 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
         }
     });
     for (int t = 0; t < ntask; ++t) out.capStart[t + 1] += out.capStart[t];
-    out.segBucket.resize(out.candGrid.size());
+}
+
+bool depth_schedule(const Shape& s, const rh_prim* prims, int n, const Forest& f, Schedule& out, std::string& err) {
+    (void)prims; (void)n;
+    const std::vector<SimNode>& sn = f.sim;
+    const int nb = (int)s.buckets.size();
+    out = Schedule();
+    out.surfaces = sn.size();
+    // Every leaf, grouped by its bucket range's first bucket (where the
+    // culling-free bucket loop dices it), in forest order within a bucket:
+    // counted per chunk of nodes, then placed.
+    const uint32_t nn = (uint32_t)sn.size();
+    const int nchunk = (int)std::min<uint32_t>((uint32_t)std::max(1, f.threads), nn / 4096 + 1);
+    const uint32_t per = (nn + nchunk - 1) / nchunk;
+    std::vector<uint32_t> cnt((size_t)nchunk * nb, 0);
+    auto leafBucket = [&](const SimNode& q) { return s.bucketIndex(q.x0, q.y0); };
+    parallel_for(nchunk, f.threads, [&](int ch) {
+        uint32_t* C = &cnt[(size_t)ch * nb];
+        for (uint32_t g = ch * per; g < std::min(nn, (ch + 1) * per); ++g)
+            if (sn[g].flags & SN_LEAF) C[leafBucket(sn[g])]++;
+    });
+    uint32_t run = 0;
     for (int k = 0; k < nb; ++k)
-        for (uint32_t e = out.candStart[k]; e < out.candStart[k + 1]; ++e) out.segBucket[e] = (uint32_t)k;
+        for (int ch = 0; ch < nchunk; ++ch) {
+            const uint32_t c0 = cnt[(size_t)ch * nb + k];
+            cnt[(size_t)ch * nb + k] = run;
+            run += c0;
+        }
+    std::vector<uint32_t> order(run);
+    parallel_for(nchunk, f.threads, [&](int ch) {
+        uint32_t* C = &cnt[(size_t)ch * nb];
+        for (uint32_t g = ch * per; g < std::min(nn, (ch + 1) * per); ++g)
+            if (sn[g].flags & SN_LEAF) order[C[leafBucket(sn[g])]++] = g;
+    });
+    // ids and vertices: running sums over chunks of the order, then the records
+    const int nrc = (int)std::min<uint32_t>((uint32_t)std::max(1, f.threads), run / 2048 + 1);
+    const uint32_t rper = (run + nrc - 1) / nrc;
+    std::vector<uint64_t> mpSum(nrc + 1, 0), vxSum(nrc + 1, 0);
+    std::vector<uint8_t> bad(nrc, 0), moving(nrc, 0);
+    parallel_for(nrc, f.threads, [&](int ch) {
+        for (uint32_t i = ch * rper; i < std::min(run, (ch + 1) * rper); ++i) {
+            const SimNode& q = sn[order[i]];
+            if ((q.cu + 1) * (q.cv + 1) > 514 || q.cu < 1 || q.cv < 1) bad[ch] = 1;
+            if (q.flags & SN_MOTION) moving[ch] = 1;
+            mpSum[ch + 1] += (uint64_t)(q.cu * q.cv);
+            vxSum[ch + 1] += leaf_vertices(q);
+        }
+    });
+    for (int ch = 0; ch < nrc; ++ch) {
+        if (bad[ch]) {
+            err = "internal: grid size outside the dicing limits";
+            return false;
+        }
+        if (moving[ch]) out.motion = true;
+        mpSum[ch + 1] += mpSum[ch];
+        vxSum[ch + 1] += vxSum[ch];
+    }
+    if (mpSum[nrc] >= 0xffffffffull || vxSum[nrc] >= 0xffffffffull) {
+        err = "the schedule's micropolygons or vertices exceed 2^32";
+        return false;
+    }
+    out.grids.resize(run);
+    parallel_for(nrc, f.threads, [&](int ch) {
+        uint32_t mp = (uint32_t)mpSum[ch], vx = (uint32_t)vxSum[ch];
+        for (uint32_t i = ch * rper; i < std::min(run, (ch + 1) * rper); ++i) {
+            const SimNode& q = sn[order[i]];
+            out.grids[i] = make_grid(s, q, leafBucket(q), mp, vx);
+            mp += (uint32_t)(q.cu * q.cv);
+            vx += leaf_vertices(q);
+        }
+    });
+    out.micropolygons = mpSum[nrc];
+    out.vertices = vxSum[nrc];
+    out.paramFloats = f.nparams;
+    finish_schedule(s, f, out);
     return true;
 }
 

@@ -110,8 +110,8 @@ struct CullProbe {
 };
 
 struct Schedule {
-    std::vector<GridRec> grids;
-    std::vector<float> params;            // per-grid dicing parameters
+    std::vector<GridRec> grids;           // GridRec::paramOff: into the forest's parameters
+    uint64_t paramFloats = 0;             // the forest's parameter floats (forest_params)
     uint64_t micropolygons = 0;
     uint64_t vertices = 0;
     uint64_t surfaces = 0;
@@ -133,6 +133,10 @@ struct ForestFree { void operator()(Forest* f) const; };
 using ForestPtr = std::unique_ptr<Forest, ForestFree>;
 
 bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, std::string& err, int threads);
+
+// Every leaf's dicing parameters, in forest order: what a schedule's grids
+// point into (GridRec::paramOff), the same for every schedule of the forest.
+const float* forest_params(const Forest& f, uint64_t& n);
 
 // The (surface, bucket) pairs an occlusion test may ask about: every surface
 // with a cached raster bound, in every bucket of its bucket range (posting
@@ -161,6 +165,16 @@ struct CullInput {
 bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, const CullInput* cull,
               const CullPairs* pairs, Schedule& out, std::string& err);
 
+// The culling-free schedule's grids for an opaque-depth pass, without its
+// bucket loop: every leaf diced in the first bucket of its bucket range, as
+// the culling-free loop dices it (a surface is posted to its range's first
+// bucket, which is never processed before it: roots are posted first, and a
+// child's range lies inside its parent's, so its first bucket comes no
+// earlier than the one its parent is split in), in forest order within a
+// bucket.  Only the order within a bucket, and so the micropolygon ids,
+// differ from simulate's.
+bool depth_schedule(const Shape& s, const rh_prim* prims, int n, const Forest& f, Schedule& out, std::string& err);
+
 // Relative margins around a cull test's depth inside which the final depths
 // do not settle it.  A cull needs the final depths below the surface's
 // minimum depth (and below the least one still to come from the heap) by
@@ -171,7 +185,7 @@ bool simulate(const Shape& s, const rh_prim* prims, int n, const Forest& f, cons
 constexpr float kCullMargin = 1.0f / 16384.0f;
 constexpr float kKeepMargin = 1.0f / 65536.0f;
 
-// Parameter layouts in Schedule::params (all floats):
+// Parameter layouts of a leaf in forest_params (all floats):
 //  sphere:   [0] radius, [1..16] object->camera matrix, [17] identity flag,
 //            then sinT[cu+1], cosT[cu+1], sinP[cv+1], cosP[cv+1]
 //  bicubic:  16 control points xyz (48 floats)

@@ -49,8 +49,8 @@ extern "C" hipError_t rh_launch_filter(hipStream_t, const FrameDev*, int, int, i
 extern "C" hipError_t rh_launch_cull_pyramid(hipStream_t, const FrameDev*, const CullTreeDev*, int, const float4*, float*);
 extern "C" hipError_t rh_launch_cull_pairs(hipStream_t, const CullTreeDev*, int, int, int, const float4*, const int4*,
                                            const unsigned long long*, int, unsigned long long, const float*, float*);
-extern "C" hipError_t rh_launch_cull_probe_max(hipStream_t, const FrameDev*, const CullTreeDev*, int, int, int, int,
-                                               int, int, const float4*, float*);
+extern "C" hipError_t rh_launch_cull_probes_max(hipStream_t, const FrameDev*, const CullTreeDev*, const int4*,
+                                                const int4*, int, const float4*, float*);
 
 static_assert(rh::GRID_MOTION == RH_GRID_MOTION, "motion grid flag differs between host and device");
 
@@ -158,6 +158,7 @@ struct rh_ctx {
     DBuf dItemsStatic, dItems, dItemCount, dQueue, dRows;
     // occlusion culling: bucket trees, depth pyramids, (surface, bucket) tests, probe results
     DBuf dCullTrees, dCullPyr, dCullBounds, dCullRanges, dCullStart, dCullPairMax, dProbeOut;
+    DBuf dProbeItems, dProbeTasks, dProbeCut, dProbeB, dProbeC;
     int maxItems = 0, persistentGrid = 0, maxRows = 0;
     rh_tuning tuning;
     hipGraph_t graph = nullptr;
@@ -188,7 +189,8 @@ struct rh_ctx {
                        &dListCount, &dSamples, &dChannels, &dRgba, &dCounters, &dBucketRefs,
                        &dHitData, &dHitNext, &dHitCtl, &dCombine, &dFrontDepth, &dLongWork, &dItemsStatic, &dItems,
                        &dItemCount, &dQueue, &dRows, &dSegBucket, &dSegCounts, &dMprecMB, &dCullTrees, &dCullPyr,
-                       &dCullBounds, &dCullRanges, &dCullStart, &dCullPairMax, &dProbeOut};
+                       &dCullBounds, &dCullRanges, &dCullStart, &dCullPairMax, &dProbeOut, &dProbeItems,
+                       &dProbeTasks, &dProbeCut, &dProbeB, &dProbeC};
         for (DBuf* b : all) f(*b);
     }
     ~rh_ctx() {
@@ -545,7 +547,14 @@ static rh_status install_schedule(rh_ctx* c, rh::Schedule&& S, const rh::Forest&
     c->stats.cull_passes = 1;
     c->stats.cull_probes = 0;
     c->stats.ms_cull = 0.0;
+    c->stats.ms_cull_depth = 0.0;
+    c->stats.ms_cull_probes = 0.0;
     c->stats.ms_schedule = ms_since(t0);
+    // every schedule of the forest dices from its parameter table
+    uint64_t nparams = 0;
+    const float* params = rh::forest_params(f, nparams);
+    HIP_TRY(c->dParams.ensure(nparams * sizeof(float)));
+    if (nparams) HIP_TRY(hipMemcpyAsync(c->dParams.p, params, nparams * sizeof(float), hipMemcpyHostToDevice, c->st));
     rh_status r = upload_schedule(c);
     if (r != RH_OK || c->prims.empty() || !any_cullable(c->prims)) return r;
     const auto tc = std::chrono::steady_clock::now();
@@ -564,9 +573,17 @@ static rh_status schedule(rh_ctx* c) {
     if (c->scheduleThreads > 0) threads = c->scheduleThreads;
     rh::ForestPtr f;
     rh::Schedule S;
-    if (!rh::build_forest(c->shape, c->prims.data(), (int)c->prims.size(), f, err, threads) ||
-        !rh::simulate(c->shape, c->prims.data(), (int)c->prims.size(), *f, nullptr, nullptr, S, err))
+    if (!rh::build_forest(c->shape, c->prims.data(), (int)c->prims.size(), f, err, threads))
         return fail(RH_ERR_INVALID, err);
+    c->stats.ms_forest = ms_since(t0);
+    const auto ts = std::chrono::steady_clock::now();
+    // A scene that may be occlusion culled first renders the culling-free
+    // grids for their opaque depths, in any order within a bucket.
+    if (!(any_cullable(c->prims)
+              ? rh::depth_schedule(c->shape, c->prims.data(), (int)c->prims.size(), *f, S, err)
+              : rh::simulate(c->shape, c->prims.data(), (int)c->prims.size(), *f, nullptr, nullptr, S, err)))
+        return fail(RH_ERR_INVALID, err);
+    c->stats.ms_simulate = ms_since(ts);
     return install_schedule(c, std::move(S), *f, t0);
 }
 
@@ -592,7 +609,6 @@ static rh_status upload_schedule(rh_ctx* c) {
     }
     std::vector<unsigned long long> cap(S.capStart.begin(), S.capStart.end());
     HIP_TRY(upload(c->dGrids, g, c->st));
-    HIP_TRY(upload(c->dParams, S.params, c->st));
     HIP_TRY(upload(c->dPrims, p, c->st));
     HIP_TRY(upload(c->dCandStart, S.candStart, c->st));
     HIP_TRY(upload(c->dCandGrid, S.candGrid, c->st));
@@ -687,6 +703,7 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed, const FrameCtl& k, bool g
     a.combineList = c->dCombine.as<uint4>(); a.frontDepth = c->transparent ? c->dFrontDepth.as<float>() : nullptr;
     a.rowCounts = k.rows; a.rowBase = 0;
     a.idCut = 0xffffffffu; a.depthOnly = 0;
+    a.itemCut = nullptr; a.taskList = nullptr;
     const int variant = variant_of(tu);
     int rows = 0;
     if (S.motion) {
@@ -823,6 +840,7 @@ static hipError_t enqueue_depth_pass(rh_ctx* c, bool sample) {
 static rh_status cull_schedule(rh_ctx* c, const rh::Forest& forest) {
     const rh::Shape& s = c->shape;
     const int n = (int)c->prims.size();
+    const auto td = std::chrono::steady_clock::now();
     // 1) the final opaque depths of the culling-free schedule
     HIP_TRY(enqueue_depth_pass(c, true));
     // 2) every bucket's depth pyramid and every (surface, bucket) test
@@ -850,13 +868,16 @@ static rh_status cull_schedule(rh_ctx* c, const rh::Forest& forest) {
         HIP_TRY(hipMemcpyAsync(pairMax.data(), c->dCullPairMax.p, npairs * sizeof(float), hipMemcpyDeviceToHost, c->st));
         HIP_TRY(hipStreamSynchronize(c->st));
     }
+    c->stats.ms_cull_depth = ms_since(td);
     // 3) the culling schedule, its probes, and again while a probe overturns a decision
     rh::CullInput in;
     in.pairMax = pairMax.data();
     std::string err;
     for (int pass = 0;; ++pass) {
         rh::Schedule S;
+        const auto ts = std::chrono::steady_clock::now();
         if (!rh::simulate(s, c->prims.data(), n, forest, &in, &pairs, S, err)) return fail(RH_ERR_INVALID, err);
+        c->stats.ms_simulate += ms_since(ts);
         c->sched = std::move(S);
         rh_status r = upload_schedule(c);
         if (r != RH_OK) return r;
@@ -864,34 +885,77 @@ static rh_status cull_schedule(rh_ctx* c, const rh::Forest& forest) {
         const std::vector<rh::CullProbe>& probes = c->sched.probes;
         if (probes.empty()) break;
         if (pass > 64) return fail(RH_ERR_STATE, "occlusion culling: probes kept overturning decisions");
-        // the schedule's task lists, then each probe's bucket up to its moment
+        const auto tp = std::chrono::steady_clock::now();
+        // the schedule's task lists, then each probe's bucket up to its moment.
+        // Probes go in rounds, round r holding each bucket's r-th probe: a
+        // round's buckets are sampled in one launch, each task cut at its
+        // probe's moment, and one launch reads all the round's leaves.
         HIP_TRY(enqueue_depth_pass(c, false));
-        HIP_TRY(c->dProbeOut.ensure(probes.size() * sizeof(float)));
-        for (size_t i = 0; i < probes.size(); ++i) {
-            const rh::CullProbe& pr = probes[i];
-            const int t0 = s.taskStart[pr.bucket], t1 = s.taskStart[pr.bucket + 1];
-            if (t1 > t0) {
+        const size_t np = probes.size();
+        std::vector<int> nth(s.buckets.size(), 0), round(np);
+        int nrounds = 0;
+        for (size_t i = 0; i < np; ++i) {
+            round[i] = nth[probes[i].bucket]++;
+            nrounds = std::max(nrounds, round[i] + 1);
+        }
+        std::vector<std::vector<uint32_t>> byRound(nrounds);
+        for (size_t i = 0; i < np; ++i) byRound[round[i]].push_back((uint32_t)i);
+        std::vector<uint2> items;                 // static sampler: (task, all its blocks)
+        std::vector<uint32_t> taskOf, cut;        // per item: task, list cut
+        std::vector<int4> pb, pc;                 // per probe in round order: (bucket, kind, result), cells
+        std::vector<int> itemStart(nrounds + 1, 0), probeStart(nrounds + 1, 0);
+        for (int r = 0; r < nrounds; ++r) {
+            for (uint32_t i : byRound[r]) {
+                const rh::CullProbe& pr = probes[i];
+                const int t0 = s.taskStart[pr.bucket], t1 = s.taskStart[pr.bucket + 1];
+                for (int t = t0; t < t1; ++t) {
+                    const auto& T = s.tasks[t];
+                    const int nblk = ((T.x1 - T.x0 + s.blockW - 1) / s.blockW) * ((T.y1 - T.y0 + s.blockH - 1) / s.blockH);
+                    items.push_back(make_uint2((uint32_t)t, (uint32_t)nblk << 16));
+                    taskOf.push_back((uint32_t)t);
+                    cut.push_back(pr.mpCut);
+                }
+                pb.push_back(make_int4(pr.bucket, t1 > t0 ? pr.kind : 0, (int)i, 0));
+                pc.push_back(make_int4(pr.cx0, pr.cx1, pr.cy0, pr.cy1));
+            }
+            itemStart[r + 1] = (int)items.size();
+            probeStart[r + 1] = (int)pb.size();
+        }
+        HIP_TRY(c->dProbeOut.ensure(np * sizeof(float)));
+        HIP_TRY(upload(c->dProbeItems, items, c->st));
+        HIP_TRY(upload(c->dProbeTasks, taskOf, c->st));
+        HIP_TRY(upload(c->dProbeCut, cut, c->st));
+        HIP_TRY(upload(c->dProbeB, pb, c->st));
+        HIP_TRY(upload(c->dProbeC, pc, c->st));
+        for (int r = 0; r < nrounds; ++r) {
+            const int ni = itemStart[r + 1] - itemStart[r];
+            if (ni > 0) {
                 SampleLaunch a = depth_launch(c);
-                a.itemBase = t0;
-                a.nStatic = t1 - t0;
-                a.rowBase = t0;
-                a.idCut = pr.mpCut;
+                a.items = c->dProbeItems.as<uint2>();
+                a.itemBase = itemStart[r];
+                a.nStatic = ni;
+                a.rowBase = 0;
+                a.itemCut = c->dProbeCut.as<uint32_t>();
+                a.taskList = c->dProbeTasks.as<uint32_t>();
                 if (c->sched.motion) {
-                    HIP_TRY(rh_launch_sample_mb(c->st, &a, t1 - t0, c->dMprecMB.as<float4>()));
+                    HIP_TRY(rh_launch_sample_mb(c->st, &a, ni, c->dMprecMB.as<float4>()));
                 } else if (s.dof) {
                     const DofTables dt = c->dof();
-                    HIP_TRY(rh_launch_sample_dof(c->st, &a, t1 - t0, &dt));
+                    HIP_TRY(rh_launch_sample_dof(c->st, &a, ni, &dt));
                 } else {
-                    HIP_TRY(rh_launch_sample(c->st, 0, t1 - t0, &a));
+                    HIP_TRY(rh_launch_sample(c->st, 0, ni, &a));
                 }
             }
-            HIP_TRY(rh_launch_cull_probe_max(c->st, c->dFrame.as<FrameDev>(), c->dCullTrees.as<CullTreeDev>(),
-                                             pr.bucket, t1 > t0 ? pr.kind : 0, pr.cx0, pr.cx1, pr.cy0, pr.cy1,
-                                             c->dSamples.as<float4>(), c->dProbeOut.as<float>() + i));
+            HIP_TRY(rh_launch_cull_probes_max(c->st, c->dFrame.as<FrameDev>(), c->dCullTrees.as<CullTreeDev>(),
+                                              c->dProbeB.as<int4>() + probeStart[r],
+                                              c->dProbeC.as<int4>() + probeStart[r],
+                                              probeStart[r + 1] - probeStart[r], c->dSamples.as<float4>(),
+                                              c->dProbeOut.as<float>()));
         }
         std::vector<float> res(probes.size());
         HIP_TRY(hipMemcpyAsync(res.data(), c->dProbeOut.p, res.size() * sizeof(float), hipMemcpyDeviceToHost, c->st));
         HIP_TRY(hipStreamSynchronize(c->st));
+        c->stats.ms_cull_probes += ms_since(tp);
         c->stats.cull_probes += probes.size();
         // Decisions up to the first overturned one stand (the schedule before
         // it is the reference's); that one is fixed and the rest recomputed.
@@ -905,7 +969,8 @@ static rh_status cull_schedule(rh_ctx* c, const rh::Forest& forest) {
         if (!overturned) break;
     }
     // the scratch of the tests goes back to the pool
-    for (DBuf* b : {&c->dCullPyr, &c->dCullBounds, &c->dCullRanges, &c->dCullStart, &c->dCullPairMax})
+    for (DBuf* b : {&c->dCullPyr, &c->dCullBounds, &c->dCullRanges, &c->dCullStart, &c->dCullPairMax,
+                    &c->dProbeItems, &c->dProbeTasks, &c->dProbeCut, &c->dProbeB, &c->dProbeC})
         b->release();
     return RH_OK;
 }
@@ -1285,7 +1350,9 @@ uint64_t rh_internal_device_bytes(const rh_ctx* c) {
                          &c->dBucketRefs, &c->dHitData, &c->dHitNext, &c->dHitCtl, &c->dCombine, &c->dFrontDepth,
                          &c->dItemsStatic, &c->dItems, &c->dItemCount, &c->dQueue, &c->dRows, &c->dMprecMB,
                          &c->dCullTrees, &c->dCullPyr, &c->dCullBounds, &c->dCullRanges, &c->dCullStart,
-                         &c->dCullPairMax, &c->dProbeOut};
+                         &c->dCullPairMax, &c->dProbeOut, &c->dProbeItems, &c->dProbeTasks, &c->dProbeCut,
+                         &c->dProbeB, &c->dProbeC, &c->dShuffled, &c->dLens, &c->dPixDof, &c->dDofCells,
+                         &c->dLongWork};
     uint64_t total = 0;
     for (const DBuf* b : all) total += b->cap;
     return total;
@@ -1303,7 +1370,7 @@ static uint64_t frame_bytes(const rh::Shape& s) {
 
 // Device bytes of one frame of a scheduled scene.
 static uint64_t footprint_bytes(const rh::Shape& s, const rh::Schedule& S, const rh_prim* prims, int32_t n) {
-    uint64_t bytes = frame_bytes(s) + S.grids.size() * sizeof(GridDev) + S.params.size() * 4 +
+    uint64_t bytes = frame_bytes(s) + S.grids.size() * sizeof(GridDev) + S.paramFloats * 4 +
                      (uint64_t)n * sizeof(PrimDev) + S.candGrid.size() * (4 + 4 + 17 * 4) +
                      S.micropolygons * (RH_MPREC_FLOATS * 4 + 16) + S.capStart.back() * 4 +
                      (S.motion ? S.micropolygons * RH_MPREC_MB_FLOATS * 4 : 0);
@@ -1329,10 +1396,12 @@ bool rh_internal_schedule(const rh_options& opt, const rh_prim* prims, int32_t n
     rh::Shape s;
     if (!rh::make_shape(opt, s, err)) return false;
     if (!rh::build_forest(s, prims, n, forest, err, threads)) return false;
-    if (!rh::simulate(s, prims, n, *forest, nullptr, nullptr, S, err)) return false;
-    bytes = footprint_bytes(s, S, prims, n);
     bool cullable = false;
     for (int32_t i = 0; i < n && !cullable; ++i) cullable = !prims[i].no_cull_hidden;
+    if (!(cullable ? rh::depth_schedule(s, prims, n, *forest, S, err)
+                   : rh::simulate(s, prims, n, *forest, nullptr, nullptr, S, err)))
+        return false;
+    bytes = footprint_bytes(s, S, prims, n);
     if (cullable) {
         // the tests' scratch: depth pyramids, surface bounds and ranges, pair results
         rh::CullPairs pairs;

@@ -836,7 +836,8 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
           const float* __restrict__ pos2d, const float* __restrict__ pos1d, const uchar4* __restrict__ pixPattern,
           float4* __restrict__ samples, float4* __restrict__ hitData, uint32_t* __restrict__ hitNext,
           uint32_t* __restrict__ hitCounter, uint4* __restrict__ combineList, float* __restrict__ frontDepth,
-          uint32_t* __restrict__ rowCounts, int rowBase, uint32_t idCut, int depthOnly) {
+          uint32_t* __restrict__ rowCounts, int rowBase, uint32_t idCut, const uint32_t* __restrict__ itemCut,
+          int depthOnly) {
     __shared__ float4 recs[CH * 9];
     __shared__ uint32_t wsum[RH_S_WAVES][5];
     __shared__ uint2 meta[CH];
@@ -887,7 +888,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
     const int tw = T.x1 - T.x0;
     const int nbx = (tw + bw - 1) / bw;
     const uint32_t* L = lists + capStart[task];
-    const uint32_t n = list_cut(L, listCount[task], idCut);
+    const uint32_t n = list_cut(L, listCount[task], itemCut ? itemCut[it] : idCut);
 
     for (int p0 = 0; p0 < blkN; p0 += RH_S_WAVES * SLOTS) {
         bool has[SLOTS], live[SLOTS], lodOk[SLOTS];
@@ -1365,7 +1366,8 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                                                         uint32_t* __restrict__ hitNext, uint32_t* __restrict__ hitCounter,
                                                         uint4* __restrict__ combineList, float* __restrict__ frontDepth,
                                                         uint32_t* __restrict__ rowCounts, int rowBase, int taskBase,
-                                                        uint32_t idCut, int depthOnly) {
+                                                        uint32_t idCut, const uint32_t* __restrict__ itemCut,
+                                                        const uint32_t* __restrict__ taskList, int depthOnly) {
     __shared__ float4 recs[RH_MB_RECS * 8];
     __shared__ float kbs[RH_MB_RECS][12];          // key bounds, kb[k][c] at [k * 6 + c]
     __shared__ float dts[RH_MB_RECS];
@@ -1389,13 +1391,14 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
     __shared__ uint32_t wtot[NW];
     __shared__ int nfitShared;
     __shared__ uint32_t wsum[NW][5];
-    const int task = taskBase + (int)blockIdx.x;
+    const int slot = taskBase + (int)blockIdx.x;
+    const int task = (taskList && slot < ntasks) ? (int)taskList[slot] : slot;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned long long lanesBelow = (1ull << lane) - 1ull;
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
     uint32_t wbBase = 0;
     int wbLeft = 0;
-    if (task < ntasks) {
+    if (slot < ntasks) {
     const int ns = fr->ns, nx = fr->nx, ny = fr->ny, bw = fr->blockW, bh = fr->blockH;
     const int BW = bw * nx, BH = bh * ny;
     const int rx = lane % BW, ry = lane / BW;
@@ -1406,7 +1409,7 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
     const BucketDev B = buckets[T.bucket];
     const int nbx = (T.x1 - T.x0 + bw - 1) / bw, nby = (T.y1 - T.y0 + bh - 1) / bh;
     const uint32_t* L = lists + capStart[task];
-    const uint32_t n = list_cut(L, listCount[task], idCut);
+    const uint32_t n = list_cut(L, listCount[task], itemCut ? itemCut[slot] : idCut);
     const float opentime = fr->shutterOpen, closetime = fr->shutterClose;
     // isClose (math.h:174-182)
     const float dsh = fabsf(closetime - opentime);
@@ -2032,7 +2035,8 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
               const float2* __restrict__ lensTab, const uint8_t* __restrict__ pixDof, const float4* __restrict__ cells,
               float4* __restrict__ samples, float4* __restrict__ hitData, uint32_t* __restrict__ hitNext,
               uint32_t* __restrict__ hitCounter, uint4* __restrict__ combineList, float* __restrict__ frontDepth,
-              uint32_t* __restrict__ rowCounts, int rowBase, uint32_t idCut, int depthOnly) {
+              uint32_t* __restrict__ rowCounts, int rowBase, uint32_t idCut, const uint32_t* __restrict__ itemCut,
+              const uint32_t* __restrict__ taskList, int depthOnly) {
     __shared__ float4 recs[RH_DOF_CH * 9];
     __shared__ uint8_t matched[RH_DOF_WAVES][RH_DOF_CH];
     __shared__ unsigned long long matM[RH_DOF_WAVES][RH_DOF_CH];
@@ -2040,12 +2044,13 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
     __shared__ uint32_t wsum[RH_DOF_WAVES][5];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ns = fr->ns;
-    const TaskDev T = tasks[itemBase + blockIdx.x];
+    const int task = taskList ? (int)taskList[itemBase + blockIdx.x] : itemBase + (int)blockIdx.x;
+    const TaskDev T = tasks[task];
     const BucketDev B = buckets[T.bucket];
     const int tilesX = (T.x1 - T.x0 + 7) >> 3, tilesY = (T.y1 - T.y0 + 7) >> 3;
     const int nblk = tilesX * tilesY * ns;
-    const uint32_t* L = lists + capStart[itemBase + blockIdx.x];
-    const uint32_t n = list_cut(L, listCount[itemBase + blockIdx.x], idCut);
+    const uint32_t* L = lists + capStart[task];
+    const uint32_t n = list_cut(L, listCount[task], itemCut ? itemCut[itemBase + blockIdx.x] : idCut);
     const unsigned long long rowRep = 0x0101010101010101ull;   // bit 8r of every row r
     const int lx = lane & 7, ly = lane >> 3;
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
@@ -2835,18 +2840,22 @@ __global__ void __launch_bounds__(256) rh_cull_pairs(const CullTreeDev* __restri
 
 // One workgroup per probe: the largest depth over a probe's cells, from a
 // frame's sample records rendered up to the probe's moment.
-__global__ void __launch_bounds__(256) rh_cull_probe_max(const FrameDev* __restrict__ fr,
-                                                         const CullTreeDev* __restrict__ trees, int bucket, int kind,
-                                                         int cx0, int cx1, int cy0, int cy1,
-                                                         const float4* __restrict__ samples,
-                                                         float* __restrict__ out) {
+// Several probes at once: probe i reads the leaves of bucket pb[i].x, kind
+// pb[i].y, cells pc[i] (cx0, cx1, cy0, cy1), into out[pb[i].z].
+__global__ void __launch_bounds__(256) rh_cull_probes_max(const FrameDev* __restrict__ fr,
+                                                          const CullTreeDev* __restrict__ trees,
+                                                          const int4* __restrict__ pb, const int4* __restrict__ pc,
+                                                          const float4* __restrict__ samples,
+                                                          float* __restrict__ out) {
     __shared__ float part[4];
-    const CullTreeDev t = trees[bucket];
+    const int4 b = pb[blockIdx.x], c = pc[blockIdx.x];
+    const int kind = b.y;
+    const CullTreeDev t = trees[b.x];
     float m = -FLT_MAX;
-    const int w = cx1 - cx0 + 1, h = cy1 - cy0 + 1;
+    const int w = c.y - c.x + 1, h = c.w - c.z + 1;
     if (kind != 0 && w > 0 && h > 0)
         for (int i = threadIdx.x; i < w * h; i += blockDim.x)
-            m = fmaxf(m, cell_depth(fr, t, samples, cx0 + i % w, cy0 + i / w));
+            m = fmaxf(m, cell_depth(fr, t, samples, c.x + i % w, c.z + i / w));
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
     __syncthreads();
@@ -2854,8 +2863,16 @@ __global__ void __launch_bounds__(256) rh_cull_probe_max(const FrameDev* __restr
         float r = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
         if (kind == 2) r = fmaxf(r, 0.0f);
         if (kind == 0 || r == -FLT_MAX) r = kind == 2 ? 0.0f : -1.0f;
-        *out = r;
+        out[b.z] = r;
     }
+}
+
+extern "C" hipError_t rh_launch_cull_probes_max(hipStream_t st, const FrameDev* fr, const CullTreeDev* trees,
+                                                const int4* pb, const int4* pc, int n, const float4* samples,
+                                                float* out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rh_cull_probes_max, dim3(n), dim3(256), 0, st, fr, trees, pb, pc, samples, out);
+    return hipGetLastError();
 }
 
 extern "C" hipError_t rh_launch_cull_pyramid(hipStream_t st, const FrameDev* fr, const CullTreeDev* trees, int nbuckets,
@@ -2873,14 +2890,6 @@ extern "C" hipError_t rh_launch_cull_pairs(hipStream_t st, const CullTreeDev* tr
     if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
     hipLaunchKernelGGL(rh_cull_pairs, dim3((unsigned)blocks), dim3(256), 0, st, trees, bx0, by0, nbr, bounds, ranges,
                        start, nnodes, npairs, pyr, pairMax);
-    return hipGetLastError();
-}
-
-extern "C" hipError_t rh_launch_cull_probe_max(hipStream_t st, const FrameDev* fr, const CullTreeDev* trees, int bucket,
-                                               int kind, int cx0, int cx1, int cy0, int cy1, const float4* samples,
-                                               float* out) {
-    hipLaunchKernelGGL(rh_cull_probe_max, dim3(1), dim3(256), 0, st, fr, trees, bucket, kind, cx0, cx1, cy0, cy1,
-                       samples, out);
     return hipGetLastError();
 }
 
@@ -2953,7 +2962,7 @@ static void launch_sample_t(hipStream_t st, int grid, const SampleLaunch& a) {
                        a.itemBase, a.nStatic, a.nDevice, a.persistent, a.queueHead, a.capStart, a.listCount, a.lists,
                        reinterpret_cast<const float4*>(a.mprec), a.grids, a.prims, a.pos2d, a.pos1d, a.pixPattern,
                        a.samples, a.hitData, a.hitNext, a.hitCounter, a.combineList, a.frontDepth, a.rowCounts,
-                       a.rowBase, a.idCut, a.depthOnly);
+                       a.rowBase, a.idCut, a.itemCut, a.depthOnly);
 }
 
 // One sampling launch of `grid` workgroups with kernel variant `variant`
@@ -2979,7 +2988,8 @@ extern "C" hipError_t rh_launch_sample_mb(hipStream_t st, const SampleLaunch* a,
     hipLaunchKernelGGL((rh_sample_mb<RH_MB_WAVES>), dim3(ntasks), dim3(RH_MB_WAVES * 64), 0, st, a->fr, a->buckets, a->tasks,
                        a->itemBase + ntasks, a->capStart, a->listCount, a->lists, mprecMB, a->grids, a->prims,
                        a->pos2d, a->pos1d, a->pixPattern, a->samples, a->hitData, a->hitNext, a->hitCounter,
-                       a->combineList, a->frontDepth, a->rowCounts, a->rowBase, a->itemBase, a->idCut, a->depthOnly);
+                       a->combineList, a->frontDepth, a->rowCounts, a->rowBase, a->itemBase, a->idCut, a->itemCut,
+                       a->taskList, a->depthOnly);
     return hipGetLastError();
 }
 
@@ -2991,7 +3001,7 @@ extern "C" hipError_t rh_launch_sample_dof(hipStream_t st, const SampleLaunch* a
                        a->itemBase, a->capStart, a->listCount, a->lists, reinterpret_cast<const float4*>(a->mprec),
                        a->grids, a->prims, a->pos2d, a->pos1d, a->pixPattern, t->shuffled, t->lens, t->pixDof,
                        t->cells, a->samples, a->hitData, a->hitNext, a->hitCounter, a->combineList, a->frontDepth,
-                       a->rowCounts, a->rowBase, a->idCut, a->depthOnly);
+                       a->rowCounts, a->rowBase, a->idCut, a->itemCut, a->taskList, a->depthOnly);
     return hipGetLastError();
 }
 

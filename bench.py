@@ -196,7 +196,9 @@ def e2e_frames(h, prims, steps):
         times.append((time.perf_counter() - t0) * 1e3)
         st = h.stats()
         parts = {"schedule_ms": st["ms_schedule"], "cull_ms": st["ms_cull"], "upload_ms": st["ms_upload"],
-                 "device_ms": st["ms_device"], "cull_passes": st["cull_passes"], "cull_probes": st["cull_probes"]}
+                 "device_ms": st["ms_device"], "cull_passes": st["cull_passes"], "cull_probes": st["cull_probes"],
+                 "forest_ms": st["ms_forest"], "simulate_ms": st["ms_simulate"],
+                 "cull_depth_ms": st["ms_cull_depth"], "cull_probes_ms": st["ms_cull_probes"]}
     times.sort()
     med = times[len(times) // 2]
     return {"ms_per_frame": med, "frames_per_s": 1e3 / med, "steps": steps, "last": parts}

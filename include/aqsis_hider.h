@@ -152,6 +152,12 @@ typedef struct rh_stats {
     uint64_t cull_passes;      /* schedules computed for the frame              */
     double ms_cull;            /* host + device: the culling schedule beyond the
                                   first split/dice schedule (a part of ms_schedule) */
+    /* parts of ms_schedule */
+    double ms_forest;          /* host: split trees of the surfaces              */
+    double ms_simulate;        /* host: bucket loops and schedule records, all passes */
+    double ms_cull_depth;      /* host + device: final depths, depth pyramids and
+                                  the (surface, bucket) tests                   */
+    double ms_cull_probes;     /* host + device: rendered probes                 */
 } rh_stats;
 
 /* Work and device memory one frame of a scene needs, from the host

@@ -131,3 +131,88 @@ def test_cull_schedule_matches_oracle(oracle, harness, make):
     assert probes <= max(2, culled // 20)
     assert culled == st["occlusion_culled"] > 0
     assert (grids, mps, surfaces) == (st["grids"], st["micropolygons"], st["surfaces"])
+
+
+DEPTH_HARNESS = r'''
+#include "front.h"
+#include <algorithm>
+#include <cstdio>
+#include <tuple>
+#include <vector>
+// argv: options file, prims file, n.  Prints "same" when depth_schedule dices
+// exactly the culling-free schedule's grids, each in the same bucket, with
+// the same candidate lists as sets and the same list capacities.
+int main(int argc, char** argv) {
+    rh_options o;
+    FILE* f = std::fopen(argv[1], "rb");
+    if (std::fread(&o, sizeof o, 1, f) != 1) return 2;
+    std::fclose(f);
+    const int n = std::atoi(argv[3]);
+    std::vector<rh_prim> p(n);
+    f = std::fopen(argv[2], "rb");
+    if (n && std::fread(p.data(), sizeof(rh_prim), n, f) != (size_t)n) return 2;
+    std::fclose(f);
+    rh::Shape s;
+    std::string err;
+    if (!rh::make_shape(o, s, err)) return 3;
+    rh::ForestPtr fo;
+    if (!rh::build_forest(s, p.data(), n, fo, err, 3)) return 4;
+    rh::Schedule A, B;
+    if (!rh::simulate(s, p.data(), n, *fo, nullptr, nullptr, A, err)) return 5;
+    if (!rh::depth_schedule(s, p.data(), n, *fo, B, err)) return 6;
+    using K = std::tuple<int, uint32_t, int, int, int, int>;
+    auto keys = [](const rh::Schedule& S) {
+        std::vector<K> v;
+        for (const auto& g : S.grids) v.emplace_back(g.diceBucket, g.paramOff, g.prim, g.type, g.cu, g.cv);
+        std::sort(v.begin(), v.end());
+        return v;
+    };
+    auto cands = [](const rh::Schedule& S) {
+        std::vector<std::pair<uint32_t, uint32_t>> v;   // (bucket, paramOff)
+        for (size_t k = 0; k + 1 < S.candStart.size(); ++k)
+            for (uint32_t e = S.candStart[k]; e < S.candStart[k + 1]; ++e) v.emplace_back(k, S.grids[S.candGrid[e]].paramOff);
+        std::sort(v.begin(), v.end());
+        return v;
+    };
+    const bool same = keys(A) == keys(B) && cands(A) == cands(B) && A.capStart == B.capStart &&
+                      A.micropolygons == B.micropolygons && A.vertices == B.vertices && A.motion == B.motion;
+    std::printf("%s %zu\n", same ? "same" : "differ", A.grids.size());
+    return 0;
+}
+'''
+
+
+@pytest.fixture(scope="module")
+def depth_harness(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not installed")
+    d = tmp_path_factory.mktemp("depthsched")
+    src = d / "h.cpp"
+    src.write_text(DEPTH_HARNESS)
+    exe = d / "h"
+    subprocess.run([gxx, "-O2", "-std=c++17", "-ffp-contract=off", "-I", CSRC, str(src),
+                    os.path.join(CSRC, "front.cpp"), "-o", str(exe), "-lpthread"], check=True)
+    return str(exe), d
+
+
+@pytest.mark.parametrize("make", [
+    lambda: scenes.spheres_scene(seed=1),
+    lambda: scenes.patches_scene(seed=5, n=400, res=(320, 180)),
+    lambda: scenes.eye_plane_scene(seed=3),
+    lambda: scenes.motion_scene(seed=2, n=80, res=(128, 128)),
+    lambda: _with_dof(scenes.patches_scene(seed=53, n=260, res=(160, 120), samples=(3, 3)), (2.8, 1.2, 16.0)),
+    lambda: scenes.clustered_scene(seed=1, res=(256, 256), median=300.0, cap=20000),
+], ids=["spheres", "patches", "eye_plane", "motion", "patches_dof", "clustered"])
+def test_depth_schedule_dices_the_culling_free_grids(depth_harness, make):
+    """The opaque-depth pass of the culling schedule dices every leaf in the
+    first bucket of its range without running the bucket loop: the same
+    grids in the same buckets as the culling-free loop, so the same lists."""
+    exe, d = depth_harness
+    opts, prims = make()
+    arr = prim_array(prims)
+    (d / "o.bin").write_bytes(bytes(opts))
+    (d / "p.bin").write_bytes(bytes(arr))
+    out = subprocess.run([exe, str(d / "o.bin"), str(d / "p.bin"), str(len(prims))],
+                         capture_output=True, text=True, check=True).stdout.split()
+    assert out[0] == "same" and int(out[1]) > 0
