@@ -69,6 +69,7 @@ class RhPrim(C.Structure):
         ("P1", (C.c_float * 3) * 16),
         ("no_cull_hidden", C.c_int32),
         ("focus_factor", C.c_float),
+        ("shading_interpolation", C.c_int32),
     ]
 
 

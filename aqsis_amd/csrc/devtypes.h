@@ -100,7 +100,8 @@ struct TaskDev {
 struct PrimDev {
     float Ci[3];
     float Oi[3];
-    float pad[2];
+    uint32_t constantShading;             // ShadingInterpolation "constant"
+    float pad;
 };
 
 // Arguments of one sampling launch (see rh_sample in kernels.hip).

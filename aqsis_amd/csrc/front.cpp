@@ -953,6 +953,10 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
             return false;
         }
         if (!(p.shading_rate > 0)) { err = "shading rate must be positive"; return false; }
+        if (p.shading_interpolation != RH_SHADING_SMOOTH && p.shading_interpolation != RH_SHADING_CONSTANT) {
+            err = "shading_interpolation must be RH_SHADING_SMOOTH or RH_SHADING_CONSTANT";
+            return false;
+        }
         if (p.type == RH_PRIM_SPHERE) {
             if (!(p.sphere[0] != 0)) { err = "sphere radius must be non-zero"; return false; }
             if (p.sphere[3] != 360.0f || p.sphere[1] != -p.sphere[0] || p.sphere[2] != p.sphere[0]) {

@@ -605,7 +605,8 @@ static rh_status upload_schedule(rh_ctx* c) {
             p[i].Oi[k] = c->prims[i].Os[k];
             p[i].Ci[k] = c->prims[i].Os[k] * c->prims[i].Cs[k];
         }
-        p[i].pad[0] = p[i].pad[1] = 0;
+        p[i].constantShading = c->prims[i].shading_interpolation == RH_SHADING_CONSTANT ? 1u : 0u;
+        p[i].pad = 0;
     }
     std::vector<unsigned long long> cap(S.capStart.begin(), S.capStart.end());
     HIP_TRY(upload(c->dGrids, g, c->st));

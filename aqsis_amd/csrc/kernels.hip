@@ -785,11 +785,17 @@ __device__ __forceinline__ void expand_record(const float4* __restrict__ src, ui
     out[8] = make_float4(r4.x, __uint_as_float((fl >> 9) & 3u), r4.z, __uint_as_float(id));
 }
 
-// Colour and opacity of a hit: InterpolateOutputs with smooth shading of
-// four equal vertex values (micropolygon.cpp:1443-1462).  A = (Ci.rgb, depth),
+// Colour and opacity of a hit: InterpolateOutputs (micropolygon.cpp:
+// 1443-1462) of four equal vertex values, smooth (bilinear in the hit's uv,
+// the default) or constant (the first vertex's).  A = (Ci.rgb, depth),
 // B = (Oi.rgb, micropolygon id bits).
 __device__ __forceinline__ void shade_hit(const PrimDev& P, float u, float v, float D, uint32_t id, float4& A,
                                           float4& B) {
+    if (P.constantShading) {
+        A = make_float4(P.Ci[0], P.Ci[1], P.Ci[2], D);
+        B = make_float4(P.Oi[0], P.Oi[1], P.Oi[2], __uint_as_float(id));
+        return;
+    }
     const float w0 = (1 - u) * (1 - v), w1 = u * (1 - v), w2 = (1 - u) * v, w3 = u * v;
     A.x = ((w0 * P.Ci[0] + w1 * P.Ci[0]) + w2 * P.Ci[0]) + w3 * P.Ci[0];
     A.y = ((w0 * P.Ci[1] + w1 * P.Ci[1]) + w2 * P.Ci[1]) + w3 * P.Ci[1];

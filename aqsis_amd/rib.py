@@ -8,7 +8,8 @@ token rules: strings in double quotes, numbers, [ ] arrays, # comments):
   Format, PixelSamples, PixelFilter, ShadingRate, Exposure, Quantize "rgba",
   Projection "perspective" "fov", Clipping, CropWindow, Shutter,
   DepthOfField (no arguments: the pinhole camera), GeometricApproximation
-  "focusfactor",
+  "focusfactor", ShadingInterpolation "constant" / "smooth", Attribute "cull"
+  "hidden",
   Option "limits" "bucketsize" / "eyesplits" / "zthreshold",
   Hider "hidden" "jitter", Display, FrameBegin/End, WorldBegin/End,
   AttributeBegin/End, TransformBegin/End, Identity, Transform,
@@ -187,6 +188,13 @@ def _set_keys(prim, times, P1=None, xform1=None):
         prim.xform[1][:] = [float(v) for v in xform1.reshape(-1)]
 
 
+def _apply_attrs(p, attr):
+    # the surface attributes the hider reads besides colour, opacity and rate
+    p.focus_factor = attr["focus"]
+    p.shading_interpolation = attr["interp"]
+    p.no_cull_hidden = 0 if attr["cull_hidden"] else 1
+
+
 def parse(text, displays=None):
     """RIB text -> (rh_options, [rh_prim]) for the hider.  The Display
     requests are appended to `displays` when a list is given, as dicts with
@@ -200,7 +208,8 @@ def parse(text, displays=None):
     camera, camkeys = np.eye(4), None
     ktimes = None
     in_world = False
-    attr = {"Cs": (1.0, 1.0, 1.0), "Os": (1.0, 1.0, 1.0), "rate": 1.0, "surface": "constant", "focus": 1.0}
+    attr = {"Cs": (1.0, 1.0, 1.0), "Os": (1.0, 1.0, 1.0), "rate": 1.0, "surface": "constant", "focus": 1.0,
+            "interp": 0, "cull_hidden": 1}
     astack, tstack = [], []
     motion = None          # the open MotionBegin block: times, line, requests
 
@@ -257,6 +266,16 @@ def parse(text, displays=None):
                 opts.dof[:] = [FLT_MAX, FLT_MAX, FLT_MAX] if not flat else [float(v) for v in flat[:3]]
                 if flat and len(flat) != 3:
                     raise RibError(f"line {line}: DepthOfField needs fstop, focal length and focal distance")
+            elif req == "ShadingInterpolation":
+                # RiShadingInterpolation (ri.cpp:1406-1416); smooth by default
+                if args[0] not in ("constant", "smooth"):
+                    raise RibError(f"line {line}: ShadingInterpolation must be \"constant\" or \"smooth\"")
+                attr["interp"] = 1 if args[0] == "constant" else 0
+            elif req == "Attribute":
+                p = _params(args[1:])
+                if args[0] != "cull" or set(p) - {"hidden"}:
+                    raise RibError(f"line {line}: only Attribute \"cull\" \"hidden\" is supported")
+                attr["cull_hidden"] = int(p["hidden"][0])
             elif req == "GeometricApproximation":
                 if args[0] != "focusfactor":
                     raise RibError(f"line {line}: only GeometricApproximation \"focusfactor\" is supported")
@@ -317,7 +336,7 @@ def parse(text, displays=None):
                         raise RibError(f"line {block['line']}: the motion keys are patches of different types")
                     make = scenes.bicubic if k0 == "bicubic" else scenes.bilinear
                     p = make(P0, attr["Cs"], attr["Os"], attr["rate"])
-                    p.focus_factor = attr["focus"]
+                    _apply_attrs(p, attr)
                     _set_keys(p, block["times"], P1=P1)
                     prims.append(p)
                 else:
@@ -357,7 +376,7 @@ def parse(text, displays=None):
                 mk = key_matrices()
                 m = ctm @ camera if mk is None else mk[0]
                 p = scenes.sphere(flat[0], (0.0, 0.0, 0.0), attr["Cs"], attr["Os"], attr["rate"])
-                p.focus_factor = attr["focus"]
+                _apply_attrs(p, attr)
                 p.sphere[:] = [float(v) for v in flat[:4]]
                 p.xform[0][:] = [float(v) for v in m.reshape(-1)]
                 p.xform[1][:] = [float(v) for v in m.reshape(-1)]
@@ -371,7 +390,7 @@ def parse(text, displays=None):
                 kind, P = _patch_points(line, args, ctm @ camera if mk is None else mk[0])
                 make = scenes.bicubic if kind == "bicubic" else scenes.bilinear
                 p = make(P, attr["Cs"], attr["Os"], attr["rate"])
-                p.focus_factor = attr["focus"]
+                _apply_attrs(p, attr)
                 if mk is not None:
                     _set_keys(p, ktimes, P1=_patch_points(line, args, mk[1])[1])
                 prims.append(p)

@@ -407,6 +407,8 @@ def to_rib(opts, prims):
         out.append(f'Opacity [{p.Os[0]} {p.Os[1]} {p.Os[2]}]')
         out.append(f'ShadingRate {p.shading_rate}')
         out.append(f'GeometricApproximation "focusfactor" {p.focus_factor}')
+        out.append('ShadingInterpolation "constant"' if p.shading_interpolation else 'ShadingInterpolation "smooth"')
+        out.append(f'Attribute "cull" "hidden" [{0 if p.no_cull_hidden else 1}]')
         moving = p.nkeys == 2
         if moving:
             motion = f'MotionBegin [{p.key_time[0]} {p.key_time[1]}]'

@@ -122,7 +122,15 @@ typedef struct rh_prim {
     float P1[16][3];
     int32_t no_cull_hidden;
     float focus_factor;
+    int32_t shading_interpolation; /* RH_SHADING_SMOOTH (0, the default) or RH_SHADING_CONSTANT */
 } rh_prim;
+
+/* rh_prim.shading_interpolation: RiShadingInterpolation (ri.cpp:1406-1416),
+ * the attribute "System" "ShadingInterpolation" (default smooth,
+ * attributes.cpp:118).  Smooth interpolates the colour and opacity of a hit
+ * bilinearly from the micropolygon's four vertices; constant takes its
+ * first vertex's (micropolygon.cpp:1443-1462). */
+enum { RH_SHADING_SMOOTH = 0, RH_SHADING_CONSTANT = 1 };
 
 /* Counters of one frame.  spl_* are the reference's own hider counters
  * (libs/core/stats.h:380-382, incremented at bucketprocessor.cpp:1172,

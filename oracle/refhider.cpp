@@ -294,6 +294,7 @@ struct Grid {
     int cu = 0, cv = 0;
     std::vector<V3> P;  // camera space after dicing; raster x,y + camera z after bust
     float Ci[3], Oi[3];
+    bool constantShading = false;   // ShadingInterpolation "constant"
     uint32_t mpBase = 0;
     int refs = 0;
 };
@@ -1129,6 +1130,7 @@ struct Hider {
         st.grids++;
         // constant shader: Oi = Os; Ci = Os * Cs (shaders/surface/constant.sl)
         for (int k = 0; k < 3; ++k) { g->Oi[k] = p.Os[k]; g->Ci[k] = p.Os[k] * p.Cs[k]; }
+        g->constantShading = p.shading_interpolation == RH_SHADING_CONSTANT;
         // Transparency culling (micropolygon.cpp:497-520): Oi == black culls every MP.
         bool allBlack = (g->Oi[0] == 0.0f && g->Oi[1] == 0.0f && g->Oi[2] == 0.0f);
         int cu = g->cu, cv = g->cv;
@@ -1179,6 +1181,7 @@ struct Hider {
         surf_dice(key1_of(*s), g1);
         st.grids++;
         for (int k = 0; k < 3; ++k) { g->Oi[k] = p.Os[k]; g->Ci[k] = p.Os[k] * p.Cs[k]; }
+        g->constantShading = p.shading_interpolation == RH_SHADING_CONSTANT;
         const int cu = g->cu, cv = g->cv;
         for (Grid* gk : {g, &g1})
             for (int i = (int)gk->P.size() - 1; i >= 0; --i) {
@@ -1622,15 +1625,24 @@ struct Hider {
             hit = &sd.hits.back();
             hit->flags = 0;
         }
-        // InterpolateOutputs, smooth (micropolygon.cpp:1443-1462)
+        // InterpolateOutputs (micropolygon.cpp:1443-1462): smooth, the
+        // default, bilinear over the four (here equal) vertex values;
+        // constant, the first vertex's
         const Grid& g = *mp->g;
-        float w0 = (1 - uv.x) * (1 - uv.y);
-        float w1 = uv.x * (1 - uv.y);
-        float w2 = (1 - uv.x) * uv.y;
-        float w3 = uv.x * uv.y;
-        for (int k = 0; k < 3; ++k) {
-            hit->d[k] = ((w0 * g.Ci[k] + w1 * g.Ci[k]) + w2 * g.Ci[k]) + w3 * g.Ci[k];
-            hit->d[3 + k] = ((w0 * g.Oi[k] + w1 * g.Oi[k]) + w2 * g.Oi[k]) + w3 * g.Oi[k];
+        if (g.constantShading) {
+            for (int k = 0; k < 3; ++k) {
+                hit->d[k] = g.Ci[k];
+                hit->d[3 + k] = g.Oi[k];
+            }
+        } else {
+            float w0 = (1 - uv.x) * (1 - uv.y);
+            float w1 = uv.x * (1 - uv.y);
+            float w2 = (1 - uv.x) * uv.y;
+            float w3 = uv.x * uv.y;
+            for (int k = 0; k < 3; ++k) {
+                hit->d[k] = ((w0 * g.Ci[k] + w1 * g.Ci[k]) + w2 * g.Ci[k]) + w3 * g.Ci[k];
+                hit->d[3 + k] = ((w0 * g.Oi[k] + w1 * g.Oi[k]) + w2 * g.Oi[k]) + w3 * g.Oi[k];
+            }
         }
         hit->d[6] = D;
         hit->id = mp->id;

@@ -291,6 +291,33 @@ def deep_stack_100():
     return opts, prims
 
 
+def constant_interpolation_mixed():
+    # ShadingInterpolation "constant" on every third surface, smooth (the
+    # default) on the others; transparent and opaque, static and moving,
+    # with and without depth of field: colours taken as they are on the
+    # constant surfaces, bilinearly interpolated from equal corners on the
+    # smooth ones.
+    opts, prims = scenes.patches_scene(seed=83, n=240, res=(160, 96), samples=(3, 3),
+                                       opacity=(0.3, 1.0))
+    for p in prims[::3]:
+        p.shading_interpolation = 1
+    return opts, prims
+
+
+def constant_interpolation_motion():
+    opts, prims = motion_mixed_opaque()
+    for p in prims[1::2]:
+        p.shading_interpolation = 1
+    return opts, prims
+
+
+def constant_interpolation_dof():
+    opts, prims = spheres_dof_transparent()
+    for p in prims[::2]:
+        p.shading_interpolation = 1
+    return opts, prims
+
+
 CASES = {
     "spheres_config": spheres_config,
     "patches_small": patches_small,
@@ -324,4 +351,7 @@ CASES = {
     "decal_dof": decal_dof,
     "patches_dof_1x1_crop": patches_dof_1x1_crop,
     "deep_stack_100": deep_stack_100,
+    "constant_interpolation_mixed": constant_interpolation_mixed,
+    "constant_interpolation_motion": constant_interpolation_motion,
+    "constant_interpolation_dof": constant_interpolation_dof,
 }

@@ -26,7 +26,9 @@ def _static_keys_cleared(p):
 
 @pytest.mark.parametrize("case", ["spheres_config", "patches_crop_odd", "patches_transparent_8x8_zthreshold",
                                   "spheres_exposure_mitchell", "patches_gridsampler", "motion_transparent_8x8",
-                                  "motion_mixed_opaque", "motion_spheres", "decal_dof", "spheres_dof_transparent"])
+                                  "motion_mixed_opaque", "motion_spheres", "decal_dof", "spheres_dof_transparent",
+                                  "patches_cull_flag_mixed", "constant_interpolation_mixed",
+                                  "constant_interpolation_motion"])
 def test_round_trip(case):
     opts, prims = CASES[case]()
     text = scenes.to_rib(opts, prims)
