@@ -28,6 +28,19 @@ RH_PRIM_SPHERE = 1
 RH_PRIM_BICUBIC = 2
 RH_PRIM_BILINEAR = 3
 
+# rh_options.depth_filter: Hider "hidden" "depthfilter" (optioncache.cpp:96-110)
+RH_DEPTH_MIN = 0
+RH_DEPTH_MIDPOINT = 1
+RH_DEPTH_MAX = 2
+RH_DEPTH_AVERAGE = 3
+DEPTH_FILTERS = {"min": RH_DEPTH_MIN, "midpoint": RH_DEPTH_MIDPOINT, "max": RH_DEPTH_MAX,
+                 "average": RH_DEPTH_AVERAGE}
+
+# rh_prim.matte: RiMatte's value as CacheGridInfo reads it (micropolygon.cpp:49-55)
+RH_MATTE_NONE = 0
+RH_MATTE = 1
+RH_MATTE_ALPHA = 2
+
 FLT_EPSILON = 1.1920928955078125e-07
 FLT_MAX = 3.4028234663852886e38
 
@@ -52,6 +65,8 @@ class RhOptions(C.Structure):
         ("num_displays", C.c_int32),
         ("zthreshold", C.c_float * 3),
         ("dof", C.c_float * 3),
+        ("depth_filter", C.c_int32),
+        ("display_z", C.c_int32),
     ]
 
 
@@ -70,6 +85,7 @@ class RhPrim(C.Structure):
         ("no_cull_hidden", C.c_int32),
         ("focus_factor", C.c_float),
         ("shading_interpolation", C.c_int32),
+        ("matte", C.c_int32),
     ]
 
 

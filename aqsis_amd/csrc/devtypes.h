@@ -9,6 +9,13 @@
 #define RH_PRIM_BILINEAR 3
 #endif
 
+// rh_prim.matte and rh_options.depth_filter values (include/aqsis_hider.h)
+#define RH_DEV_MATTE 1
+#define RH_DEV_MATTE_ALPHA 2
+#define RH_DEV_DEPTH_MIDPOINT 1
+#define RH_DEV_DEPTH_MAX 2
+#define RH_DEV_DEPTH_AVERAGE 3
+
 // GridDev::type flag of a deforming surface's grid (two motion keys): its
 // parameters hold key 0's then key 1's, then the key times.
 #define RH_GRID_MOTION 0x100
@@ -28,12 +35,14 @@
 //           component-wise, color.h:432-435 (10)
 //  [18]     grid index (int bits)  [19] unused
 #define RH_MPREC_FLOATS 20
+#define RH_MPREC_GRID 18
 // Motion-frame micropolygon record, in floats:
 //  [0..11]  key 0: x, y, z of the four vertices in grid order
 //  [12..23] key 1 (a static micropolygon repeats key 0)
 //  [24]     flags as above, plus RH_MB_MOVING  [25] grid index (int bits)
 //  [26..27] key times  [28..31] static bound min x, max x, min y, max y
 #define RH_MPREC_MB_FLOATS 32
+#define RH_MPREC_MB_GRID 25
 #define RH_MB_MOVING (1u << 11)
 // Device counters: 0..4 the hider's SPL counters (sample tests, bound hits,
 // stored hits, point-in-polygon hits, second Newton steps), 5 tasks whose
@@ -75,6 +84,12 @@ struct FrameDev {
     int dofOn;
     float dofMult, dofOneOverF;   // m_DofMultiplier, m_OneOverFocalDistance
     float dofScaleX, dofScaleY;   // m_DepthOfFieldScale
+    // StoreSample's modes (bucketprocessor.cpp:1076-1081, 1499-1525):
+    // cullableHits = 0 with a z display and the max or average depth filter
+    // (every hit listed, no occlusion tests); midpointStore with a z display
+    // and the midpoint filter (the occluding depth is the second nearest)
+    int cullableHits, midpointStore;
+    int depthFilter;              // RH_DEPTH_* for the combine
 };
 
 struct GridDev {
@@ -101,7 +116,7 @@ struct PrimDev {
     float Ci[3];
     float Oi[3];
     uint32_t constantShading;             // ShadingInterpolation "constant"
-    float pad;
+    uint32_t matte;                       // RH_MATTE_*
 };
 
 // Arguments of one sampling launch (see rh_sample in kernels.hip).
@@ -139,6 +154,9 @@ struct SampleLaunch {
     // of field) the task of each such workgroup (null: consecutive tasks).
     const uint32_t* itemCut;
     const uint32_t* taskList;
+    // the midpoint depth filter's store (FrameDev::midpointStore): the
+    // static sampler's midpoint form, whatever the variant
+    int midpoint;
 };
 
 // Lens tables of a frame with depth of field: per pattern the

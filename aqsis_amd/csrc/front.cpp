@@ -30,6 +30,7 @@ bool make_shape(const rh_options& o, Shape& s, std::string& err) {
     if (!(o.fov > 0 && o.fov < 180)) { err = "fov must be in (0, 180)"; return false; }
     if (o.num_displays < 0 || o.num_displays > 16) { err = "num_displays out of range"; return false; }
     if (!(o.pixel_aspect > 0)) { err = "pixel aspect must be positive"; return false; }
+    if (o.depth_filter < RH_DEPTH_MIN || o.depth_filter > RH_DEPTH_AVERAGE) { err = "unknown depth filter"; return false; }
     if (o.dof[0] < FLT_MAX && !(o.dof[0] > 0 && o.dof[1] > 0 && o.dof[2] > 0)) {
         err = "DepthOfField fstop, focal length and focal distance must be positive";
         return false;
@@ -953,6 +954,10 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
             return false;
         }
         if (!(p.shading_rate > 0)) { err = "shading rate must be positive"; return false; }
+        if (p.matte != RH_MATTE_NONE && p.matte != RH_MATTE && p.matte != RH_MATTE_ALPHA) {
+            err = "matte must be RH_MATTE_NONE, RH_MATTE or RH_MATTE_ALPHA";
+            return false;
+        }
         if (p.shading_interpolation != RH_SHADING_SMOOTH && p.shading_interpolation != RH_SHADING_CONSTANT) {
             err = "shading_interpolation must be RH_SHADING_SMOOTH or RH_SHADING_CONSTANT";
             return false;

@@ -192,6 +192,19 @@ constexpr float kKeepMargin = 1.0f / 65536.0f;
 //  bilinear: 4 corners xyz (12 floats)
 enum { SPH_RADIUS = 0, SPH_MAT = 1, SPH_IDENT = 17, SPH_TABLES = 18 };
 
+// StoreSample's isCullable without CSG (bucketprocessor.cpp:1076-1081):
+// false with a z display and the max or average depth filter.
+inline bool cullable_hits(const rh_options& o) {
+    return !(o.display_z && (o.depth_filter == RH_DEPTH_MAX || o.depth_filter == RH_DEPTH_AVERAGE));
+}
+// Whether a surface's hits go to the per-sample lists (not the occluding
+// hit): when hits are not cullable, or the surface is not opaque and not
+// matte alpha (bucketprocessor.cpp:1490-1491).
+inline bool hits_listed(const rh_options& o, const rh_prim& q) {
+    if (!cullable_hits(o)) return true;
+    return !(q.Os[0] >= 1.0f && q.Os[1] >= 1.0f && q.Os[2] >= 1.0f) && q.matte != RH_MATTE_ALPHA;
+}
+
 // build_forest + simulate without occlusion culling.
 bool make_schedule(const Shape& s, const rh_prim* prims, int n, Schedule& out, std::string& err,
                    int threads);

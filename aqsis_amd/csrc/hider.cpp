@@ -34,10 +34,11 @@ extern "C" hipError_t rh_launch_plan_items(hipStream_t, const FrameDev*, const T
                                            uint32_t, int, uint2*, uint32_t*, uint32_t*);
 extern "C" hipError_t rh_launch_frame_begin(hipStream_t, unsigned long long*, uint32_t*);
 extern "C" hipError_t rh_launch_combine_long(hipStream_t, const FrameDev*, size_t, const uint4*, uint32_t*,
-                                             const float4*, const uint32_t*, const PrimDev*, float4*, float*, float*,
-                                             uint32_t*, int*);
+                                             const float4*, const uint32_t*, const PrimDev*, const float*, int, int,
+                                             const GridDev*, float4*, float*, float*, uint32_t*, int*);
 extern "C" hipError_t rh_launch_combine(hipStream_t, const FrameDev*, size_t, const uint4*, uint32_t*,
-                                        const float4*, const uint32_t*, const PrimDev*, float4*, float*);
+                                        const float4*, const uint32_t*, const PrimDev*, const float*, int, int,
+                                        const GridDev*, float4*, float*);
 extern "C" hipError_t rh_launch_reduce_counters(hipStream_t, const uint32_t*, int, const uint32_t*, int,
                                                 unsigned long long*);
 extern "C" size_t rh_filter_tile_lds(int, int, int);
@@ -53,6 +54,9 @@ extern "C" hipError_t rh_launch_cull_probes_max(hipStream_t, const FrameDev*, co
                                                 const int4*, int, const float4*, float*);
 
 static_assert(rh::GRID_MOTION == RH_GRID_MOTION, "motion grid flag differs between host and device");
+static_assert(RH_MATTE == RH_DEV_MATTE && RH_MATTE_ALPHA == RH_DEV_MATTE_ALPHA, "matte values differ");
+static_assert(RH_DEPTH_MIDPOINT == RH_DEV_DEPTH_MIDPOINT && RH_DEPTH_MAX == RH_DEV_DEPTH_MAX &&
+                  RH_DEPTH_AVERAGE == RH_DEV_DEPTH_AVERAGE, "depth filter values differ");
 
 namespace {
 
@@ -225,6 +229,8 @@ void rh_options_default(rh_options* o) {
     o->num_displays = 1;
     o->zthreshold[0] = o->zthreshold[1] = o->zthreshold[2] = 1.0f;
     o->dof[0] = o->dof[1] = o->dof[2] = FLT_MAX;   // pinhole camera (options.cpp:297)
+    o->depth_filter = RH_DEPTH_MIN;                 // optioncache.cpp:44
+    o->display_z = 0;
 }
 
 const char* rh_last_error(void) { return g_err.c_str(); }
@@ -377,6 +383,9 @@ rh_status rh_create(const rh_options* opt, rh_ctx** out) {
     f.dofOn = s.dof ? 1 : 0;
     f.dofMult = s.dofMult; f.dofOneOverF = s.dofOneOverF;
     f.dofScaleX = s.dofScaleX; f.dofScaleY = s.dofScaleY;
+    f.cullableHits = rh::cullable_hits(*opt) ? 1 : 0;
+    f.midpointStore = (opt->display_z && opt->depth_filter == RH_DEPTH_MIDPOINT) ? 1 : 0;
+    f.depthFilter = opt->depth_filter;
     c->frame = f;
 
     std::vector<BucketDev> bk;
@@ -507,10 +516,8 @@ rh_status rh_clear_surfaces(rh_ctx* c) {
 // exact size.
 static uint64_t arena_guess(const rh::Shape& s, const rh::Schedule& S, const rh_prim* prims) {
     uint64_t tmp = 0;
-    for (const auto& g : S.grids) {
-        const rh_prim& q = prims[g.prim];
-        if (!(q.Os[0] >= 1.0f && q.Os[1] >= 1.0f && q.Os[2] >= 1.0f)) tmp += (uint64_t)g.cu * g.cv;
-    }
+    for (const auto& g : S.grids)
+        if (rh::hits_listed(s.o, prims[g.prim])) tmp += (uint64_t)g.cu * g.cv;
     if (tmp == 0) return 0;
     return std::min<uint64_t>(std::max<uint64_t>(1u << 20, tmp * (uint64_t)s.ns), 0x7fffffffull);
 }
@@ -531,8 +538,10 @@ static rh_status set_hit_capacity(rh_ctx* c, uint64_t n) {
 static rh_status upload_schedule(rh_ctx* c);
 static rh_status cull_schedule(rh_ctx* c, const rh::Forest& f);
 
-// Some surface may be occlusion culled (Attribute "cull" "hidden", default 1).
-static bool any_cullable(const std::vector<rh_prim>& prims) {
+// Some surface may be occlusion culled (Attribute "cull" "hidden", default 1;
+// RenderSurface tests none when hits are not cullable, bucketprocessor.cpp:945-947).
+static bool any_cullable(const rh_options& o, const std::vector<rh_prim>& prims) {
+    if (!rh::cullable_hits(o)) return false;
     for (const auto& q : prims)
         if (!q.no_cull_hidden) return true;
     return false;
@@ -556,7 +565,7 @@ static rh_status install_schedule(rh_ctx* c, rh::Schedule&& S, const rh::Forest&
     HIP_TRY(c->dParams.ensure(nparams * sizeof(float)));
     if (nparams) HIP_TRY(hipMemcpyAsync(c->dParams.p, params, nparams * sizeof(float), hipMemcpyHostToDevice, c->st));
     rh_status r = upload_schedule(c);
-    if (r != RH_OK || c->prims.empty() || !any_cullable(c->prims)) return r;
+    if (r != RH_OK || c->prims.empty() || !any_cullable(c->shape.o, c->prims)) return r;
     const auto tc = std::chrono::steady_clock::now();
     r = cull_schedule(c, f);
     c->stats.ms_cull = ms_since(tc);
@@ -579,7 +588,7 @@ static rh_status schedule(rh_ctx* c) {
     const auto ts = std::chrono::steady_clock::now();
     // A scene that may be occlusion culled first renders the culling-free
     // grids for their opaque depths, in any order within a bucket.
-    if (!(any_cullable(c->prims)
+    if (!(any_cullable(c->shape.o, c->prims)
               ? rh::depth_schedule(c->shape, c->prims.data(), (int)c->prims.size(), *f, S, err)
               : rh::simulate(c->shape, c->prims.data(), (int)c->prims.size(), *f, nullptr, nullptr, S, err)))
         return fail(RH_ERR_INVALID, err);
@@ -606,7 +615,7 @@ static rh_status upload_schedule(rh_ctx* c) {
             p[i].Ci[k] = c->prims[i].Os[k] * c->prims[i].Cs[k];
         }
         p[i].constantShading = c->prims[i].shading_interpolation == RH_SHADING_CONSTANT ? 1u : 0u;
-        p[i].pad = 0;
+        p[i].matte = (uint32_t)c->prims[i].matte;
     }
     std::vector<unsigned long long> cap(S.capStart.begin(), S.capStart.end());
     HIP_TRY(upload(c->dGrids, g, c->st));
@@ -622,11 +631,11 @@ static rh_status upload_schedule(rh_ctx* c) {
     if (S.motion) HIP_TRY(c->dMprecMB.ensure(S.micropolygons * RH_MPREC_MB_FLOATS * sizeof(float)));
     // Transparent hits need list storage; start at two per sample and grow
     // when a frame asks for more.
+    const rh::Shape& s = c->shape;
     bool transparent = false;
     for (const auto& q : c->prims)
-        if (!(q.Os[0] >= 1.0f && q.Os[1] >= 1.0f && q.Os[2] >= 1.0f)) transparent = true;
+        if (rh::hits_listed(s.o, q)) transparent = true;
     c->transparent = transparent;
-    const rh::Shape& s = c->shape;
     const uint64_t nsamp = (uint64_t)s.extW * s.extH * s.ns;
     const uint64_t want = transparent ? arena_guess(s, S, c->prims.data()) : 64;
     HIP_TRY(c->dCombine.ensure((transparent ? nsamp : 1) * sizeof(uint4)));
@@ -664,6 +673,12 @@ static hipError_t enqueue_filter(rh_ctx* c) {
                             c->dSamples.as<float4>(), c->tiledFilter ? c->dBoxMask.as<uint16_t>() : nullptr,
                             c->dPix.as<uchar4>(), c->dPos2d.as<float>(), c->dFilter.as<float>(),
                             c->dDither.as<float>(), c->dChannels.as<float>(), c->dRgba.as<uchar4>());
+}
+
+// The micropolygon records the combine finds an occluding hit's grid in:
+// the motion records in a frame with moving micropolygons.
+static const float* occl_records(rh_ctx* c) {
+    return c->sched.motion ? reinterpret_cast<const float*>(c->dMprecMB.p) : c->dMprec.as<float>();
 }
 
 // Enqueue one device frame on the context's stream.  With `timed`, HIP
@@ -705,6 +720,7 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed, const FrameCtl& k, bool g
     a.rowCounts = k.rows; a.rowBase = 0;
     a.idCut = 0xffffffffu; a.depthOnly = 0;
     a.itemCut = nullptr; a.taskList = nullptr;
+    a.midpoint = c->frame.midpointStore;
     const int variant = variant_of(tu);
     int rows = 0;
     if (S.motion) {
@@ -743,7 +759,9 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed, const FrameCtl& k, bool g
     if (c->transparent)
         ENQ(rh_launch_combine(c->st, f, (size_t)s.extW * s.extH * s.ns, c->dCombine.as<uint4>(),
                               k.hitCtl, c->dHitData.as<float4>(), c->dHitNext.as<uint32_t>(),
-                              c->dPrims.as<PrimDev>(), c->dSamples.as<float4>(), c->dFrontDepth.as<float>()));
+                              c->dPrims.as<PrimDev>(), occl_records(c), S.motion ? RH_MPREC_MB_FLOATS : RH_MPREC_FLOATS,
+                              S.motion ? RH_MPREC_MB_GRID : RH_MPREC_GRID, c->dGrids.as<GridDev>(), c->dSamples.as<float4>(),
+                              c->dFrontDepth.as<float>()));
     if (timed) ENQ(hipEventRecord(c->ev[3], c->st));
     ENQ(rh_launch_reduce_counters(c->st, k.rows, rows, c->dBucketRefs.as<uint32_t>(), nb, k.counters));
     ENQ(enqueue_filter(c));
@@ -764,10 +782,13 @@ static rh_status combine_long_lists(rh_ctx* c, const FrameCtl& k, const uint32_t
     float* workD = reinterpret_cast<float*>(w);
     uint32_t* workX = reinterpret_cast<uint32_t*>(w + entries * sizeof(float));
     int* workStk = reinterpret_cast<int*>(w + entries * (sizeof(float) + sizeof(uint32_t)));
+    const bool motion = c->sched.motion;
     HIP_TRY(rh_launch_combine_long(c->st, c->dFrame.as<FrameDev>(), (size_t)s.extW * s.extH * s.ns,
                                    c->dCombine.as<uint4>(), k.hitCtl, c->dHitData.as<float4>(),
-                                   c->dHitNext.as<uint32_t>(), c->dPrims.as<PrimDev>(), c->dSamples.as<float4>(),
-                                   c->dFrontDepth.as<float>(), workD, workX, workStk));
+                                   c->dHitNext.as<uint32_t>(), c->dPrims.as<PrimDev>(), occl_records(c),
+                                   motion ? RH_MPREC_MB_FLOATS : RH_MPREC_FLOATS, motion ? RH_MPREC_MB_GRID : RH_MPREC_GRID,
+                                   c->dGrids.as<GridDev>(), c->dSamples.as<float4>(), c->dFrontDepth.as<float>(),
+                                   workD, workX, workStk));
     HIP_TRY(enqueue_filter(c));
     HIP_TRY(hipStreamSynchronize(c->st));
     return RH_OK;
@@ -789,6 +810,8 @@ static SampleLaunch depth_launch(rh_ctx* c) {
     a.combineList = c->dCombine.as<uint4>(); a.frontDepth = nullptr;
     a.rowCounts = c->dRows.as<uint32_t>(); a.rowBase = 0;
     a.idCut = 0xffffffffu; a.depthOnly = 1;
+    // with the midpoint store the occluding depth is the second nearest
+    a.midpoint = c->frame.midpointStore;
     return a;
 }
 
@@ -1377,7 +1400,7 @@ static uint64_t footprint_bytes(const rh::Shape& s, const rh::Schedule& S, const
                      (S.motion ? S.micropolygons * RH_MPREC_MB_FLOATS * 4 : 0);
     bool transparent = false;
     for (int32_t i = 0; i < n; ++i)
-        if (!(prims[i].Os[0] >= 1.0f && prims[i].Os[1] >= 1.0f && prims[i].Os[2] >= 1.0f)) transparent = true;
+        if (rh::hits_listed(s.o, prims[i])) transparent = true;
     if (transparent) {
         // hit arena (24 B an entry), combine list and front depths
         const uint64_t nsamp = (uint64_t)s.extW * s.extH * s.ns;
@@ -1398,7 +1421,7 @@ bool rh_internal_schedule(const rh_options& opt, const rh_prim* prims, int32_t n
     if (!rh::make_shape(opt, s, err)) return false;
     if (!rh::build_forest(s, prims, n, forest, err, threads)) return false;
     bool cullable = false;
-    for (int32_t i = 0; i < n && !cullable; ++i) cullable = !prims[i].no_cull_hidden;
+    for (int32_t i = 0; i < n && !cullable && rh::cullable_hits(opt); ++i) cullable = !prims[i].no_cull_hidden;
     if (!(cullable ? rh::depth_schedule(s, prims, n, *forest, S, err)
                    : rh::simulate(s, prims, n, *forest, nullptr, nullptr, S, err)))
         return false;

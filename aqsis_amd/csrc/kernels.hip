@@ -339,9 +339,11 @@ __global__ void __launch_bounds__(256) rh_dice_bust(const FrameDev* __restrict__
         float bminz = fmin_ref(fmin_ref(fmin_ref(pA.z, pB.z), pC.z), pD.z);
         float bmaxx = fmax_ref(fmax_ref(fmax_ref(pA.x, pB.x), pC.x), pD.x);
         float bmaxy = fmax_ref(fmax_ref(fmax_ref(pA.y, pB.y), pC.y), pD.y);
-        // isOpaque = Oi >= (1,1,1) component-wise (micropolygon.cpp:1482, color.h:432-435)
+        // isOpaque = Oi >= (1,1,1) component-wise (micropolygon.cpp:1482,
+        // color.h:432-435); a matte alpha surface's hits are stored as
+        // occluding hits as an opaque one's (bucketprocessor.cpp:1490-1491)
         const PrimDev& Pr = prims[G.prim];
-        const bool opaque = Pr.Oi[0] >= 1.0f && Pr.Oi[1] >= 1.0f && Pr.Oi[2] >= 1.0f;
+        const bool opaque = (Pr.Oi[0] >= 1.0f && Pr.Oi[1] >= 1.0f && Pr.Oi[2] >= 1.0f) || Pr.matte == RH_DEV_MATTE_ALPHA;
         float4* out = reinterpret_cast<float4*>(mprec + (size_t)(G.mpBase + m) * RH_MPREC_FLOATS);
         const uint32_t flags = (uint32_t)(code & 0xff) | (degen ? 1u << 8 : 0u) | (linear ? 1u << 9 : 0u) |
                                (opaque ? 1u << 10 : 0u);
@@ -807,6 +809,25 @@ __device__ __forceinline__ void shade_hit(const PrimDev& P, float u, float v, fl
     B.w = __uint_as_float(id);
 }
 
+// A sample's occluding hit as a sampler writes it (A, B from shade_hit):
+// without a hit list, Combine's no-list branch (imagepixel.cpp:301-323): a
+// matte surface's hit is transparent black and the midpoint filter averages
+// its depth with the occluding depth; an opaque-depth pass writes the
+// occluding depth (the occlusion tree's).
+__device__ __forceinline__ void finish_occluding(const FrameDev* __restrict__ fr, const PrimDev& P, bool listed,
+                                                 int depthOnly, float occlZ, float4& A, float4& B) {
+    if (depthOnly) {
+        A.w = occlZ;
+        return;
+    }
+    if (listed) return;
+    if (P.matte == RH_DEV_MATTE) {
+        A.x = A.y = A.z = 0.0f;
+        B.x = B.y = B.z = 0.0f;
+    }
+    if (fr->depthFilter == RH_DEV_DEPTH_MIDPOINT) A.w = (float)(0.5 * (double)(A.w + occlZ));
+}
+
 // The part of a task's list (ascending micropolygon ids, the busting order)
 // below idCut: the micropolygons rendered in a bucket before a surface is
 // taken from its heap (0xffffffff: the whole list).
@@ -832,7 +853,7 @@ __device__ __forceinline__ uint32_t list_cut(const uint32_t* __restrict__ L, uin
 // workgroups pulling items from a queue (persistent != 0; the count is in
 // *nDevice).  Each workgroup writes one row of counters at rowBase +
 // blockIdx.x.
-template <int CH, int SLOTS>
+template <int CH, int SLOTS, bool MID = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
 rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets, const TaskDev* __restrict__ tasks,
           const uint2* __restrict__ items, int itemBase, int nStatic, const uint32_t* __restrict__ nDevice,
@@ -866,6 +887,9 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
     uint32_t wbBase = 0;                  // the wave's current arena block
     int wbLeft = 0;
     const int itemEnd = persistent ? (int)nDevice[0] : itemBase + nStatic;
+    // isCullable (bucketprocessor.cpp:1076-1081): occlusion tests and the
+    // occluding hit apply; else every hit is listed
+    const bool cullH = fr->cullableHits != 0;
 
     for (int round = 0;; ++round) {
     int it;
@@ -901,12 +925,14 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
         int gx[SLOTS], gy[SLOTS], bgx[SLOTS], bgy[SLOTS];
         float sx[SLOTS], sy[SLOTS];
         float occlZ[SLOTS], wu[SLOTS], wv[SLOTS];
+        float sD[SLOTS];                  // MID: the occluding hit's depth
         uint32_t win[SLOTS], head[SLOTS];
         int nh[SLOTS];
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
             head[k] = 0xffffffffu;
             nh[k] = 0;
+            sD[k] = FLT_MAX;
             const int blk = blk0 + p0 + wave + RH_S_WAVES * k;
             has[k] = p0 + wave + RH_S_WAVES * k < blkN;
             const int bx = T.x0 + (blk % nbx) * bw, by = T.y0 + (blk / nbx) * bh;
@@ -984,7 +1010,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                         mt = meta[j];
                         ok = (int)(mt.x & 0xffffu) < BX1 && (int)(mt.x >> 16) > BX0 && (int)(mt.y & 0xffffu) < BY1 &&
                              (int)(mt.y >> 16) > BY0;
-                        if (ok && recs[j * 9 + 8].x > zmax) {
+                        if (ok && cullH && recs[j * 9 + 8].x > zmax) {
                             hidden = true;
                             ok = false;
                         }
@@ -1063,7 +1089,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                             const float4* R = &recs[j * 9];
                             const float4 r8 = R[8];
                             // occlusion skip against the current opaque hit (bucketprocessor.cpp:1179)
-                            if (!(r8.x > occlZ[k])) {
+                            if (!(cullH && r8.x > occlZ[k])) {
                                 ++nBound;
                                 // fContains (micropolygon.cpp:1295-1342): edges 0,1 reject on <= 0, edges 2,3 on < 0
                                 const float4 YM = R[0], XM = R[1], X = R[2], Y = R[3];
@@ -1105,21 +1131,34 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                                                 w3 = v * u;
                                     D = ((w0 * Z.x + w1 * Z.y) + w2 * Z.z) + w3 * Z.w;
                                     // StoreSample (bucketprocessor.cpp:1471-1569): reject when occlZ <= D
-                                    hit = !(occlZ[k] <= D);
+                                    hit = !cullH || !(occlZ[k] <= D);
                                     hid = __float_as_uint(r8.w);
                                     hgrid = __float_as_uint(r8.z);
                                 }
                             }
                         }
                         nHit += hit ? 1u : 0u;
-                        // opaque: the occluding hit
-                        const bool opq = hit && (hflags & 2u) != 0;
-                        occlZ[k] = opq ? D : occlZ[k];
-                        win[k] = opq ? hid : win[k];
-                        wu[k] = opq ? u : wu[k];
-                        wv[k] = opq ? v : wv[k];
-                        // transparent: a new entry of the sample's hit list
-                        const bool app = hit && (hflags & 2u) == 0 && !depthOnly;
+                        // opaque (or matte alpha) with occlusion: the occluding hit
+                        const bool opq = hit && (hflags & 2u) != 0 && cullH;
+                        if (MID) {
+                            // midpoint: the nearest opaque hit is kept and the
+                            // second nearest occludes (bucketprocessor.cpp:1502-1525)
+                            const float prevZ = win[k] != 0xffffffffu ? sD[k] : FLT_MAX;
+                            const bool keep = !(prevZ < D);
+                            const bool store = opq && keep;
+                            occlZ[k] = opq ? (keep ? prevZ : D) : occlZ[k];
+                            win[k] = store ? hid : win[k];
+                            wu[k] = store ? u : wu[k];
+                            wv[k] = store ? v : wv[k];
+                            sD[k] = store ? D : sD[k];
+                        } else {
+                            occlZ[k] = opq ? D : occlZ[k];
+                            win[k] = opq ? hid : win[k];
+                            wu[k] = opq ? u : wu[k];
+                            wv[k] = opq ? v : wv[k];
+                        }
+                        // otherwise a new entry of the sample's hit list
+                        const bool app = hit && !opq && !depthOnly;
                         // Append the wave's transparent hits (converged): entries come
                         // from 64-entry blocks of the frame's arena, one atomic per block.
                         const unsigned long long am = __ballot(app);
@@ -1159,11 +1198,14 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
             if (win[k] != 0xffffffffu) {
                 const float4 r8 = mprec4[(size_t)win[k] * (RH_MPREC_FLOATS / 4) + 4];
                 const PrimDev& P = prims[grids[__float_as_int(r8.z)].prim];
-                shade_hit(P, wu[k], wv[k], occlZ[k], win[k], a, b);
+                shade_hit(P, wu[k], wv[k], MID ? sD[k] : occlZ[k], win[k], a, b);
+                finish_occluding(fr, P, nh[k] > 0, depthOnly, occlZ[k], a, b);
             }
             samples[2 * sidx] = a;
             samples[2 * sidx + 1] = b;
-            if (frontDepth) frontDepth[sidx] = occlZ[k];
+            // the combine's occluding depth for a listed sample, else the
+            // sample's final depth
+            if (frontDepth) frontDepth[sidx] = nh[k] > 0 ? occlZ[k] : a.w;
             // Samples with transparent hits are combined by rh_combine.
             const bool multi = nh[k] > 0;
             const unsigned long long cm = __ballot(multi);
@@ -1399,6 +1441,7 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
     __shared__ uint32_t wsum[NW][5];
     const int slot = taskBase + (int)blockIdx.x;
     const int task = (taskList && slot < ntasks) ? (int)taskList[slot] : slot;
+    const bool cullH = fr->cullableHits != 0, midS = fr->midpointStore != 0;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned long long lanesBelow = (1ull << lane) - 1ull;
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
@@ -1456,6 +1499,7 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
             lodOk = !(0.0f > lod || lod >= 1.0f);
         }
         float occlZ = FLT_MAX, wu = 0.f, wv = 0.f;
+        float sD = FLT_MAX;               // midpoint: the occluding hit's depth
         uint32_t win = 0xffffffffu, head = 0xffffffffu;
         int nh = 0;
         const unsigned long long liveM = __ballot(live);
@@ -1837,7 +1881,7 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                                 opaque = (__float_as_uint(r6.x) >> 10) & 1u;
                                 hid = ids[r];
                                 hgrid = __float_as_int(r6.y);
-                                if (!(mz > occlZ)) {
+                                if (!(cullH && mz > occlZ)) {
                                     ++nBound;
                                     if (rc & 0x200u) {
                                         ++nIn;
@@ -1845,18 +1889,24 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                                         D = ringD[wave][slot];
                                         u = ringU[wave][slot];
                                         v = ringV[wave][slot];
-                                        hit = !(occlZ <= D);
+                                        hit = !cullH || !(occlZ <= D);
                                     }
                                 }
                             }
-                            // StoreSample (bucketprocessor.cpp:1471-1569)
+                            // StoreSample (bucketprocessor.cpp:1471-1569); the
+                            // midpoint case keeps the nearest opaque hit and
+                            // occludes with the second nearest (:1502-1525)
                             nHit += hit ? 1u : 0u;
-                            const bool opq = hit && opaque;
-                            occlZ = opq ? D : occlZ;
-                            win = opq ? hid : win;
-                            wu = opq ? u : wu;
-                            wv = opq ? v : wv;
-                            const bool app = hit && !opaque && !depthOnly;
+                            const bool opq = hit && opaque && cullH;
+                            const float prevZ = win != 0xffffffffu ? sD : FLT_MAX;
+                            const bool keep = !midS || !(prevZ < D);
+                            const bool store = opq && keep;
+                            occlZ = opq ? (midS && keep ? prevZ : D) : occlZ;
+                            win = store ? hid : win;
+                            wu = store ? u : wu;
+                            wv = store ? v : wv;
+                            sD = store ? D : sD;
+                            const bool app = hit && !opq && !depthOnly;
                             const unsigned long long am = __ballot(app);
                             if (am) {
                                 const int na = __popcll(am);
@@ -1895,11 +1945,13 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
             float4 a = make_float4(0.f, 0.f, 0.f, FLT_MAX), b = make_float4(0.f, 0.f, 0.f, __uint_as_float(win));
             if (win != 0xffffffffu) {
                 const float4 r6w = mprecMB[(size_t)win * 8 + 6];
-                shade_hit(prims[grids[__float_as_int(r6w.y)].prim], wu, wv, occlZ, win, a, b);
+                const PrimDev& P = prims[grids[__float_as_int(r6w.y)].prim];
+                shade_hit(P, wu, wv, sD, win, a, b);
+                finish_occluding(fr, P, nh > 0, depthOnly, occlZ, a, b);
             }
             samples[2 * sidx] = a;
             samples[2 * sidx + 1] = b;
-            if (frontDepth) frontDepth[sidx] = occlZ;
+            if (frontDepth) frontDepth[sidx] = nh > 0 ? occlZ : a.w;
         }
         const bool multi = live && nh > 0;
         const unsigned long long cm = __ballot(multi);
@@ -2053,6 +2105,7 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
     const int task = taskList ? (int)taskList[itemBase + blockIdx.x] : itemBase + (int)blockIdx.x;
     const TaskDev T = tasks[task];
     const BucketDev B = buckets[T.bucket];
+    const bool cullH = fr->cullableHits != 0, midS = fr->midpointStore != 0;
     const int tilesX = (T.x1 - T.x0 + 7) >> 3, tilesY = (T.y1 - T.y0 + 7) >> 3;
     const int nblk = tilesX * tilesY * ns;
     const uint32_t* L = lists + capStart[task];
@@ -2091,6 +2144,7 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
         const unsigned long long liveM = __ballot(live), inM = __ballot(live && lodOk);
         const int BX0 = tx0 - B.sx0, BY0 = ty0 - B.sy0;
         float occlZ = FLT_MAX, wu = 0.f, wv = 0.f;
+        float sD = FLT_MAX;               // midpoint: the occluding hit's depth
         uint32_t win = 0xffffffffu, head = 0xffffffffu;
         int nh = 0;
         for (uint32_t c0 = 0; c0 < n; c0 += RH_DOF_CH) {
@@ -2153,7 +2207,7 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
                         sb = make_float4(bminx, bmaxx, bminy, bmaxy);
                     }
                     nTest += (uint32_t)__popcll(m);
-                    ok = m != 0ull && !(recs[j * 9 + 4].x > zmax);
+                    ok = m != 0ull && !(cullH && recs[j * 9 + 4].x > zmax);
                 }
                 const unsigned long long msk = __ballot(ok);
                 if (ok) {
@@ -2190,7 +2244,7 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
                         const float4* R = &recs[matched[wave][i0 + i] * 9];
                         const float4 r4 = R[4];
                         // occlusion skip against the current opaque hit (bucketprocessor.cpp:1396-1397)
-                        if (!(r4.x > occlZ)) {
+                        if (!(cullH && r4.x > occlZ)) {
                             ++nBound;
                             // dofSampleInBound (micropolygon.cpp:1531-1550)
                             const float4 cm = R[7], bd = R[3];
@@ -2212,7 +2266,7 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
                                     ++nIn;
                                     if (!lin) ++nIn2;
                                     // StoreSample (bucketprocessor.cpp:1471-1569): reject when occlZ <= D
-                                    hit = !(occlZ <= D);
+                                    hit = !cullH || !(occlZ <= D);
                                     hid = __float_as_uint(r4.w);
                                     hgrid = __float_as_uint(r4.z);
                                 }
@@ -2220,12 +2274,18 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
                         }
                     }
                     nHit += hit ? 1u : 0u;
-                    const bool opq = hit && (hflags & (1u << 10)) != 0;
-                    occlZ = opq ? D : occlZ;
-                    win = opq ? hid : win;
-                    wu = opq ? u : wu;
-                    wv = opq ? v : wv;
-                    const bool app = hit && (hflags & (1u << 10)) == 0 && !depthOnly;
+                    // the occluding hit; midpoint keeps the nearest opaque hit
+                    // and occludes with the second nearest (:1502-1525)
+                    const bool opq = hit && (hflags & (1u << 10)) != 0 && cullH;
+                    const float prevZ = win != 0xffffffffu ? sD : FLT_MAX;
+                    const bool keep = !midS || !(prevZ < D);
+                    const bool store = opq && keep;
+                    occlZ = opq ? (midS && keep ? prevZ : D) : occlZ;
+                    win = store ? hid : win;
+                    wu = store ? u : wu;
+                    wv = store ? v : wv;
+                    sD = store ? D : sD;
+                    const bool app = hit && !opq && !depthOnly;
                     const unsigned long long am = __ballot(app);
                     if (am) {
                         const int na = __popcll(am);
@@ -2255,11 +2315,13 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
         float4 A = make_float4(0.f, 0.f, 0.f, FLT_MAX), Bv = make_float4(0.f, 0.f, 0.f, __uint_as_float(win));
         if (win != 0xffffffffu) {
             const float4 r4 = mprec4[(size_t)win * (RH_MPREC_FLOATS / 4) + 4];
-            shade_hit(prims[grids[__float_as_int(r4.z)].prim], wu, wv, occlZ, win, A, Bv);
+            const PrimDev& P = prims[grids[__float_as_int(r4.z)].prim];
+            shade_hit(P, wu, wv, sD, win, A, Bv);
+            finish_occluding(fr, P, nh > 0, depthOnly, occlZ, A, Bv);
         }
         samples[2 * sidx] = A;
         samples[2 * sidx + 1] = Bv;
-        if (frontDepth) frontDepth[sidx] = occlZ;
+        if (frontDepth) frontDepth[sidx] = nh > 0 ? occlZ : A.w;
         const bool multi = nh > 0;
         const unsigned long long cmk = __ballot(multi);
         if (cmk) {
@@ -2359,13 +2421,27 @@ __global__ void rh_frame_begin(unsigned long long* __restrict__ counters, uint32
 // ---------------------------------------------------------------------------
 // Combine: samples with transparent hits
 // ---------------------------------------------------------------------------
-// CqImagePixel::Combine (imagepixel.cpp:144-332, no CSG, no matte) for each
-// sample the sampling kernel listed: its occluding hit (already in the
-// sample buffer, if any) joins its transparent hits, the list is sorted by
-// depth as std::sort does, then composited back to front; the depth is that
-// of the nearest hit whose opacity reaches zthreshold, else the occluding
-// depth; the front-most hit gives the id, and its depth goes to frontDepth.
-// One thread per listed sample, one wave per workgroup.
+// CqImagePixel::Combine (imagepixel.cpp:144-332, no CSG) for each sample
+// the sampling kernel listed: its occluding hit (already in the sample
+// buffer, if any) joins its transparent hits, the list is sorted by depth as
+// std::sort does, then composited back to front (a matte hit cuts out what
+// lies behind it); the depth is chosen by the depth filter: min, the nearest
+// hit whose opacity reaches zthreshold, else the occluding depth; midpoint,
+// the mean of the two nearest such depths (the second nearest starting as
+// the occluding depth); max, the farthest such depth; average, the mean
+// depth of the hits with any opacity channel at zthreshold.  The front-most
+// hit gives the id, and its depth goes to frontDepth.  One thread per listed
+// sample, one wave per workgroup.
+//
+// Where the occluding hit's surface is found: its micropolygon record, of
+// occ.recFloats floats with the grid index (int bits) at occ.gridField, and
+// the grid's surface.
+struct OcclRec {
+    const float* mprec;
+    int recFloats, gridField;
+    const GridDev* grids;
+};
+
 // CqImagePixel::Combine of one listed sample (see rh_combine); d, x: room
 // for its hits and its occluding hit; stk: the sort's work stack.  Inlined
 // into each kernel, so that rh_combine keeps its lists in LDS with LDS
@@ -2373,45 +2449,93 @@ __global__ void rh_frame_begin(unsigned long long* __restrict__ counters, uint32
 __device__ __forceinline__ void combine_sample(const FrameDev* __restrict__ fr, const uint4 e, float* d, uint32_t* x, int* stk,
                                bool longList, const float4* __restrict__ hitData,
                                const uint32_t* __restrict__ hitNext, const PrimDev* __restrict__ prims,
-                               float4* __restrict__ samples, float* __restrict__ frontDepth) {
+                               const OcclRec occ, float4* __restrict__ samples, float* __restrict__ frontDepth) {
     const size_t sidx = (size_t)e.x | ((size_t)e.y << 32);
     uint32_t head = e.z;
     int n = (int)e.w;
     const float4 oA = samples[2 * sidx], oB = samples[2 * sidx + 1];
-    const float occlZ = oA.w;
+    // the sampler's occluding depth (SqSampleData::occlZ): the occluding
+    // hit's depth, or with the midpoint store the second nearest opaque one
+    const float occlZ = frontDepth[sidx];
     // the list is linked newest first; rebuild insertion order
     for (int q = n - 1; q >= 0; --q) {
         x[q] = head;
         d[q] = hitData[head].z;
         head = hitNext[2 * (size_t)head];
     }
-    if (__float_as_uint(oB.w) != 0xffffffffu) {
+    const uint32_t occId = __float_as_uint(oB.w);
+    bool occMatte = false;
+    if (occId != 0xffffffffu) {
         x[n] = 0xffffffffu;
         d[n] = oA.w;
         ++n;
+        const int g = __float_as_int(occ.mprec[(size_t)occId * occ.recFloats + occ.gridField]);
+        occMatte = prims[occ.grids[g].prim].matte == RH_DEV_MATTE;
     }
     if (longList) rhsort::sort_long(d, x, n, stk);
     else rhsort::sort_with(d, x, n, stk);
     float col[3] = {0.0f, 0.0f, 0.0f}, opa[3] = {0.0f, 0.0f, 0.0f};
-    float opaqueDepth = occlZ;
+    float opaqueDepth0 = occlZ, opaqueDepth1 = FLT_MAX, maxOpaqueDepth = FLT_MAX;
     const float zt0 = fr->zThreshold[0], zt1 = fr->zThreshold[1], zt2 = fr->zThreshold[2];
     for (int q = n - 1; q >= 0; --q) {
         float4 hA = oA, hB = oB;
+        bool matte = occMatte;
         if (x[q] != 0xffffffffu) {
             // InterpolateOutputs of the hit's surface at its (u, v)
             const float4 h = hitData[x[q]];
-            shade_hit(prims[hitNext[2 * (size_t)x[q] + 1]], h.x, h.y, h.z, __float_as_uint(h.w), hA, hB);
+            const PrimDev& P = prims[hitNext[2 * (size_t)x[q] + 1]];
+            shade_hit(P, h.x, h.y, h.z, __float_as_uint(h.w), hA, hB);
+            matte = P.matte == RH_DEV_MATTE;
         }
         const float Ci[3] = {hA.x, hA.y, hA.z}, Oi[3] = {hB.x, hB.y, hB.z};
-        for (int c = 0; c < 3; ++c) {
-            const float oc = Oi[c] < 0.0f ? 0.0f : (Oi[c] > 1.0f ? 1.0f : Oi[c]);
-            col[c] = col[c] * (1.0f - oc) + Ci[c];
-            opa[c] = (1.0f - opa[c]) * Oi[c] + opa[c];
+        if (matte) {
+            // lerp(t, x0, 0) = (1 - t) x0 + t 0 (math.h:88-91), the colour
+            // weighted by the opacity and the opacity by the colour
+            for (int c = 0; c < 3; ++c) {
+                col[c] = (1.0f - Oi[c]) * col[c] + Oi[c] * 0.0f;
+                opa[c] = (1.0f - Ci[c]) * opa[c] + Ci[c] * 0.0f;
+            }
+        } else {
+            for (int c = 0; c < 3; ++c) {
+                const float oc = Oi[c] < 0.0f ? 0.0f : (Oi[c] > 1.0f ? 1.0f : Oi[c]);
+                col[c] = col[c] * (1.0f - oc) + Ci[c];
+                opa[c] = (1.0f - opa[c]) * Oi[c] + opa[c];
+            }
         }
-        if (Oi[0] >= zt0 && Oi[1] >= zt1 && Oi[2] >= zt2) opaqueDepth = hA.w;
+        if (Oi[0] >= zt0 && Oi[1] >= zt1 && Oi[2] >= zt2) {
+            opaqueDepth1 = opaqueDepth0;
+            opaqueDepth0 = hA.w;
+            if (!(maxOpaqueDepth < FLT_MAX)) maxOpaqueDepth = hA.w;
+        }
     }
-    const uint32_t front = x[0] == 0xffffffffu ? __float_as_uint(oB.w) : __float_as_uint(hitData[x[0]].w);
-    samples[2 * sidx] = make_float4(col[0], col[1], col[2], opaqueDepth);
+    float depth = opaqueDepth0;
+    const int df = fr->depthFilter;
+    if (df == RH_DEV_DEPTH_MIDPOINT) {
+        depth = n > 1 ? (opaqueDepth0 + opaqueDepth1) * 0.5f : FLT_MAX;
+    } else if (df == RH_DEV_DEPTH_MAX) {
+        depth = maxOpaqueDepth;
+    } else if (df == RH_DEV_DEPTH_AVERAGE) {
+        // front to back over the sorted list (0 / 0 without such a hit)
+        float tot = 0.0f;
+        int cnt = 0;
+        for (int q = 0; q < n; ++q) {
+            float4 hB = oB;
+            float D = oA.w;
+            if (x[q] != 0xffffffffu) {
+                const float4 h = hitData[x[q]];
+                float4 hA;
+                shade_hit(prims[hitNext[2 * (size_t)x[q] + 1]], h.x, h.y, h.z, __float_as_uint(h.w), hA, hB);
+                D = h.z;
+            }
+            if (hB.x >= zt0 || hB.y >= zt1 || hB.z >= zt2) {
+                tot += D;
+                ++cnt;
+            }
+        }
+        depth = tot / (float)cnt;
+    }
+    const uint32_t front = x[0] == 0xffffffffu ? occId : __float_as_uint(hitData[x[0]].w);
+    samples[2 * sidx] = make_float4(col[0], col[1], col[2], depth);
     frontDepth[sidx] = d[0];
     samples[2 * sidx + 1] = make_float4(opa[0], opa[1], opa[2], __uint_as_float(front));
 }
@@ -2422,8 +2546,8 @@ rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
                                                   uint32_t* __restrict__ hitCounter,
                                                   const float4* __restrict__ hitData,
                                                   const uint32_t* __restrict__ hitNext,
-                                                  const PrimDev* __restrict__ prims, float4* __restrict__ samples,
-                                                  float* __restrict__ frontDepth) {
+                                                  const PrimDev* __restrict__ prims, const OcclRec occ,
+                                                  float4* __restrict__ samples, float* __restrict__ frontDepth) {
     __shared__ float cmbD[RH_COMBINE_WG * (RH_MAX_HITS + 1)];
     __shared__ uint32_t cmbX[RH_COMBINE_WG * (RH_MAX_HITS + 1)];
     __shared__ int cmbS[RH_COMBINE_WG * 25];
@@ -2440,7 +2564,7 @@ rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
     // the hit list and the sort's work stack in LDS (no scratch memory),
     // each lane's array an odd number of words from the next
     combine_sample(fr, e, cmbD + threadIdx.x * (RH_MAX_HITS + 1), cmbX + threadIdx.x * (RH_MAX_HITS + 1),
-                   cmbS + threadIdx.x * 25, false, hitData, hitNext, prims, samples, frontDepth);
+                   cmbS + threadIdx.x * 25, false, hitData, hitNext, prims, occ, samples, frontDepth);
 }
 
 // The samples rh_combine left (more than RH_MAX_HITS - 1 transparent hits,
@@ -2452,16 +2576,17 @@ rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
 __global__ void __launch_bounds__(64)
 rh_combine_long(const FrameDev* __restrict__ fr, const uint4* __restrict__ list, uint32_t* __restrict__ hitCounter,
                 const float4* __restrict__ hitData, const uint32_t* __restrict__ hitNext,
-                const PrimDev* __restrict__ prims, float4* __restrict__ samples, float* __restrict__ frontDepth,
-                float* __restrict__ workD, uint32_t* __restrict__ workX, int* __restrict__ workStk) {
+                const PrimDev* __restrict__ prims, const OcclRec occ, float4* __restrict__ samples,
+                float* __restrict__ frontDepth, float* __restrict__ workD, uint32_t* __restrict__ workX,
+                int* __restrict__ workStk) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= hitCounter[2]) return;
     const uint4 e = list[t];
     if (e.w < RH_MAX_HITS) return;
     const uint32_t off = atomicAdd(&hitCounter[4], e.w + 1u);
     const uint32_t k = atomicAdd(&hitCounter[5], 1u);
-    combine_sample(fr, e, workD + off, workX + off, workStk + (size_t)k * 192, true, hitData, hitNext, prims, samples,
-                   frontDepth);
+    combine_sample(fr, e, workD + off, workX + off, workStk + (size_t)k * 192, true, hitData, hitNext, prims, occ,
+                   samples, frontDepth);
 }
 
 
@@ -2962,9 +3087,9 @@ extern "C" hipError_t rh_launch_bin(hipStream_t st, const FrameDev* fr, const Bu
     return hipGetLastError();
 }
 
-template <int CH, int SLOTS>
+template <int CH, int SLOTS, bool MID = false>
 static void launch_sample_t(hipStream_t st, int grid, const SampleLaunch& a) {
-    hipLaunchKernelGGL((rh_sample<CH, SLOTS>), dim3(grid), dim3(256), 0, st, a.fr, a.buckets, a.tasks, a.items,
+    hipLaunchKernelGGL((rh_sample<CH, SLOTS, MID>), dim3(grid), dim3(256), 0, st, a.fr, a.buckets, a.tasks, a.items,
                        a.itemBase, a.nStatic, a.nDevice, a.persistent, a.queueHead, a.capStart, a.listCount, a.lists,
                        reinterpret_cast<const float4*>(a.mprec), a.grids, a.prims, a.pos2d, a.pos1d, a.pixPattern,
                        a.samples, a.hitData, a.hitNext, a.hitCounter, a.combineList, a.frontDepth, a.rowCounts,
@@ -2972,9 +3097,16 @@ static void launch_sample_t(hipStream_t st, int grid, const SampleLaunch& a) {
 }
 
 // One sampling launch of `grid` workgroups with kernel variant `variant`
-// (RH_VARIANT_CH / RH_VARIANT_SLOTS).
+// (RH_VARIANT_CH / RH_VARIANT_SLOTS).  The midpoint store takes one form
+// (96-record chunks, 2 blocks per wave) whatever the variant: a wave loops
+// over an item's blocks, so any form covers any item.
 extern "C" hipError_t rh_launch_sample(hipStream_t st, int variant, int grid, const SampleLaunch* a) {
     if (grid <= 0) return hipSuccess;
+    if (variant < 0 || variant >= RH_SAMPLE_VARIANTS) return hipErrorInvalidValue;
+    if (a->midpoint) {
+        launch_sample_t<96, 2, true>(st, grid, *a);
+        return hipGetLastError();
+    }
     switch (variant) {
     case 0: launch_sample_t<216, 4>(st, grid, *a); break;
     case 1: launch_sample_t<216, 2>(st, grid, *a); break;
@@ -3034,12 +3166,13 @@ extern "C" hipError_t rh_launch_plan_items(hipStream_t st, const FrameDev* fr, c
 // the kernel reads the actual count from hitCounter[2].
 extern "C" hipError_t rh_launch_combine(hipStream_t st, const FrameDev* fr, size_t maxEntries, const uint4* list,
                                         uint32_t* hitCounter, const float4* hitData, const uint32_t* hitNext,
-                                        const PrimDev* prims,
-                                        float4* samples, float* frontDepth) {
+                                        const PrimDev* prims, const float* mprec, int recFloats, int gridField,
+                                        const GridDev* grids, float4* samples, float* frontDepth) {
     if (maxEntries == 0) return hipSuccess;
+    const OcclRec occ = {mprec, recFloats, gridField, grids};
     hipLaunchKernelGGL(rh_combine, dim3((unsigned)((maxEntries + RH_COMBINE_WG - 1) / RH_COMBINE_WG)),
                        dim3(RH_COMBINE_WG), 0, st, fr, list,
-                       hitCounter, hitData, hitNext, prims, samples, frontDepth);
+                       hitCounter, hitData, hitNext, prims, occ, samples, frontDepth);
     return hipGetLastError();
 }
 
@@ -3047,11 +3180,13 @@ extern "C" hipError_t rh_launch_combine(hipStream_t st, const FrameDev* fr, size
 // kernel reads the count from hitCounter[2]).
 extern "C" hipError_t rh_launch_combine_long(hipStream_t st, const FrameDev* fr, size_t maxEntries, const uint4* list,
                                              uint32_t* hitCounter, const float4* hitData, const uint32_t* hitNext,
-                                             const PrimDev* prims, float4* samples, float* frontDepth, float* workD,
+                                             const PrimDev* prims, const float* mprec, int recFloats, int gridField,
+                                             const GridDev* grids, float4* samples, float* frontDepth, float* workD,
                                              uint32_t* workX, int* workStk) {
     if (maxEntries == 0) return hipSuccess;
+    const OcclRec occ = {mprec, recFloats, gridField, grids};
     hipLaunchKernelGGL(rh_combine_long, dim3((unsigned)((maxEntries + 63) / 64)), dim3(64), 0, st, fr, list,
-                       hitCounter, hitData, hitNext, prims, samples, frontDepth, workD, workX, workStk);
+                       hitCounter, hitData, hitNext, prims, occ, samples, frontDepth, workD, workX, workStk);
     return hipGetLastError();
 }
 

@@ -9,9 +9,10 @@ token rules: strings in double quotes, numbers, [ ] arrays, # comments):
   Projection "perspective" "fov", Clipping, CropWindow, Shutter,
   DepthOfField (no arguments: the pinhole camera), GeometricApproximation
   "focusfactor", ShadingInterpolation "constant" / "smooth", Attribute "cull"
-  "hidden",
+  "hidden", Matte,
   Option "limits" "bucketsize" / "eyesplits" / "zthreshold",
-  Hider "hidden" "jitter", Display, FrameBegin/End, WorldBegin/End,
+  Hider "hidden" "jitter" / "depthfilter", Display, FrameBegin/End,
+  WorldBegin/End,
   AttributeBegin/End, TransformBegin/End, Identity, Transform,
   ConcatTransform, Translate, Rotate, Scale, Color, Opacity,
   Surface "constant", Sphere, Patch "bicubic" / "bilinear" "P",
@@ -32,16 +33,19 @@ Display requests are counted into num_displays, since each display request
 draws one dither value per pixel from the shared random stream
 (ddmanager.cpp:1050).  As RiDisplay does (libs/core/api/ri.cpp:995-1004), a
 request whose name starts with '+' adds a display and any other replaces the
-requests made so far; the name is kept without its '+'.  render_file writes
+requests made so far; the name is kept without its '+'.  A mode that reads
+"z" after an optional "rgb" and "a" (ri.cpp:947-964) puts z in the display
+mode (display_z): added to it by a '+' request, replacing it otherwise.  render_file writes
 the first display as a TIFF file when its type is "file" or "tiff"
 (tools/displays/file/display.cpp).
 """
 import math
 import re
+import warnings
 
 import numpy as np
 
-from .abi import FILTERS, FLT_MAX, default_options
+from .abi import DEPTH_FILTERS, FILTERS, FLT_MAX, RH_DEPTH_MIN, default_options
 from . import scenes
 
 
@@ -193,6 +197,16 @@ def _apply_attrs(p, attr):
     p.focus_factor = attr["focus"]
     p.shading_interpolation = attr["interp"]
     p.no_cull_hidden = 0 if attr["cull_hidden"] else 1
+    p.matte = attr["matte"]
+
+
+def _mode_has_z(mode):
+    """DMode_Z of a display mode string: "rgb", then "a", then "z", each
+    optional and matched as a prefix in that order (ri.cpp:947-964)."""
+    i = 3 if mode.startswith("rgb") else 0
+    if mode.startswith("a", i):
+        i += 1
+    return mode.startswith("z", i)
 
 
 def parse(text, displays=None):
@@ -209,7 +223,7 @@ def parse(text, displays=None):
     ktimes = None
     in_world = False
     attr = {"Cs": (1.0, 1.0, 1.0), "Os": (1.0, 1.0, 1.0), "rate": 1.0, "surface": "constant", "focus": 1.0,
-            "interp": 0, "cull_hidden": 1}
+            "interp": 0, "cull_hidden": 1, "matte": 0}
     astack, tstack = [], []
     motion = None          # the open MotionBegin block: times, line, requests
 
@@ -276,6 +290,11 @@ def parse(text, displays=None):
                 if args[0] != "cull" or set(p) - {"hidden"}:
                     raise RibError(f"line {line}: only Attribute \"cull\" \"hidden\" is supported")
                 attr["cull_hidden"] = int(p["hidden"][0])
+            elif req == "Matte":
+                # RiMatte's RtBoolean as CacheGridInfo reads it: 0 none, 2
+                # matte alpha, any other value matte (micropolygon.cpp:49-55)
+                v = int(flat[0])
+                attr["matte"] = 0 if v == 0 else (2 if v == 2 else 1)
             elif req == "GeometricApproximation":
                 if args[0] != "focusfactor":
                     raise RibError(f"line {line}: only GeometricApproximation \"focusfactor\" is supported")
@@ -293,14 +312,23 @@ def parse(text, displays=None):
                 p = _params(args[1:])
                 if "jitter" in p:
                     opts.jitter = int(p["jitter"][0])
+                if "depthfilter" in p:
+                    # an unknown name leaves "min", with a warning (optioncache.cpp:96-110)
+                    name = p["depthfilter"][0]
+                    if name not in DEPTH_FILTERS:
+                        warnings.warn(f"line {line}: invalid depthfilter {name!r}, depthfilter set to \"min\"")
+                    opts.depth_filter = DEPTH_FILTERS.get(name, RH_DEPTH_MIN)
             elif req == "Display":
                 if len(args) < 3 or not all(isinstance(a, str) for a in args[:3]):
                     raise RibError(f"line {line}: Display needs a name, a type and a mode")
                 name = args[0]
+                z = _mode_has_z(args[2])
                 if name.startswith("+"):
                     name = name[1:]
+                    opts.display_z = 1 if (opts.display_z or z) else 0
                 else:
                     requests = []          # ClearDisplayRequests
+                    opts.display_z = 1 if z else 0
                 requests.append({"name": name, "type": args[1], "mode": args[2],
                                  "params": _params(args[3:]), "line": line})
             elif req in ("FrameBegin", "FrameEnd", "Declare", "Sides"):

@@ -382,6 +382,9 @@ def serving_requests(seed=1, count=2000, sides=(256, 512, 1024, 2048), spp=(2, 4
         yield opts, prims, t
 
 
+_DEPTH_NAMES = ["min", "midpoint", "max", "average"]
+
+
 def to_rib(opts, prims):
     """RIB text of a scene in camera space (spheres as a translation), which
     aqsis_amd.rib.parse reads back to the same options and records; two-key
@@ -396,11 +399,17 @@ def to_rib(opts, prims):
            f'Clipping {opts.clip_near} {opts.clip_far}',
            f'Option "limits" "bucketsize" [{opts.bucket_x} {opts.bucket_y}] "eyesplits" [{opts.eyesplits}]',
            f'Option "limits" "zthreshold" [{opts.zthreshold[0]} {opts.zthreshold[1]} {opts.zthreshold[2]}]',
-           f'Hider "hidden" "jitter" [{opts.jitter}]',
+           f'Hider "hidden" "jitter" [{opts.jitter}] "depthfilter" "{_DEPTH_NAMES[opts.depth_filter]}"',
            f'Shutter {opts.shutter_open} {opts.shutter_close}',
            (f'DepthOfField {opts.dof[0]} {opts.dof[1]} {opts.dof[2]}' if opts.dof[0] < FLT_MAX else 'DepthOfField'),
            f'Projection "perspective" "fov" [{opts.fov}]',
            'WorldBegin', 'Surface "constant"']
+    # display requests, unless the default single one without z: one per
+    # num_displays, z in the first one's mode with display_z
+    if opts.display_z or opts.num_displays != 1:
+        displays = [f'Display "frame" "null" "{"rgbaz" if opts.display_z else "rgba"}"']
+        displays += [f'Display "+frame{i}" "null" "rgba"' for i in range(1, opts.num_displays)]
+        out[0:0] = displays
     for p in prims:
         out.append('AttributeBegin')
         out.append(f'Color [{p.Cs[0]} {p.Cs[1]} {p.Cs[2]}]')
@@ -409,6 +418,7 @@ def to_rib(opts, prims):
         out.append(f'GeometricApproximation "focusfactor" {p.focus_factor}')
         out.append('ShadingInterpolation "constant"' if p.shading_interpolation else 'ShadingInterpolation "smooth"')
         out.append(f'Attribute "cull" "hidden" [{0 if p.no_cull_hidden else 1}]')
+        out.append(f'Matte {p.matte}')
         moving = p.nkeys == 2
         if moving:
             motion = f'MotionBegin [{p.key_time[0]} {p.key_time[1]}]'

@@ -84,7 +84,17 @@ typedef struct rh_options {
     float dof[3];                /* DepthOfField fstop focallength focaldistance
                                     (FLT_MAX x3: pinhole; depth of field is on
                                     when fstop < FLT_MAX, renderer.h:368-377) */
+    int32_t depth_filter;        /* Hider "hidden" "depthfilter": RH_DEPTH_MIN (0) */
+    int32_t display_z;           /* a display request's mode has z (DMode_Z) (0) */
 } rh_options;
+
+/* rh_options.depth_filter (optioncache.cpp:96-105).  The filter sets each
+ * sample's depth in CqImagePixel::Combine (imagepixel.cpp:262-305).  With a z
+ * display (display_z) it also changes how hits are stored
+ * (bucketprocessor.cpp:1076-1081, 1499-1525): max and average keep every hit
+ * (nothing is occlusion culled, at the sample or the surface), midpoint keeps
+ * the nearest opaque hit and occludes with the second nearest. */
+enum { RH_DEPTH_MIN = 0, RH_DEPTH_MIDPOINT = 1, RH_DEPTH_MAX = 2, RH_DEPTH_AVERAGE = 3 };
 
 /*
  * One surface, already in camera space (the state in which
@@ -123,7 +133,15 @@ typedef struct rh_prim {
     int32_t no_cull_hidden;
     float focus_factor;
     int32_t shading_interpolation; /* RH_SHADING_SMOOTH (0, the default) or RH_SHADING_CONSTANT */
+    int32_t matte;               /* RiMatte: 0 (the default), 1 matte, 2 matte alpha */
 } rh_prim;
+
+/* rh_prim.matte: the attribute "System" "Matte" (CacheGridInfo,
+ * micropolygon.cpp:49-55).  A matte surface (1) hides what lies behind it
+ * and shows nothing itself: its hits composite to transparent black
+ * (imagepixel.cpp:202-213, 301-313); a matte alpha surface (2) is stored as
+ * an occluding hit whatever its opacity (bucketprocessor.cpp:1490-1491). */
+enum { RH_MATTE_NONE = 0, RH_MATTE = 1, RH_MATTE_ALPHA = 2 };
 
 /* rh_prim.shading_interpolation: RiShadingInterpolation (ri.cpp:1406-1416),
  * the attribute "System" "ShadingInterpolation" (default smooth,

@@ -107,6 +107,11 @@ struct Frame {
     std::vector<float> filterValues;  // InitialiseFilterValues
     Dof dof;                     // lens (CqRenderer's depth-of-field state)
     std::vector<Bound> dofBounds;  // CalculateDofBounds: one per lens cell
+    // isCullable without CSG (bucketprocessor.cpp:1076-1081): false with a z
+    // display and the max or average depth filter
+    bool cullableHits = true;
+    // StoreSample's midpoint case (bucketprocessor.cpp:1502-1525)
+    bool midpointStore = false;
 };
 
 static void setup_frame(const rh_options& o, Frame& f) {
@@ -122,6 +127,8 @@ static void setup_frame(const rh_options& o, Frame& f) {
     f.xfwo2 = std::ceil(o.filter_xwidth) * 0.5f;
     f.yfwo2 = std::ceil(o.filter_ywidth) * 0.5f;
     f.nx = o.xsamples; f.ny = o.ysamples; f.nsamp = f.nx * f.ny;
+    f.cullableHits = !(o.display_z && (o.depth_filter == RH_DEPTH_MAX || o.depth_filter == RH_DEPTH_AVERAGE));
+    f.midpointStore = o.display_z && o.depth_filter == RH_DEPTH_MIDPOINT;
     // SetImage (imagebuffer.cpp:174-228)
     f.cXBuckets = (xr - 1) / o.bucket_x + 1;
     f.cYBuckets = (yr - 1) / o.bucket_y + 1;
@@ -295,6 +302,7 @@ struct Grid {
     std::vector<V3> P;  // camera space after dicing; raster x,y + camera z after bust
     float Ci[3], Oi[3];
     bool constantShading = false;   // ShadingInterpolation "constant"
+    uint32_t matteFlag = 0;         // CacheGridInfo (micropolygon.cpp:49-55)
     uint32_t mpBase = 0;
     int refs = 0;
 };
@@ -810,7 +818,7 @@ static void surf_dice(const Surf& s, Grid& g) {
 // ---------------------------------------------------------------------------
 // Samples and pixels: SqSampleData / SqImageSample / CqImagePixel
 // (imagepixel.h:76-327).
-enum { Flag_Valid = 0x0004 };
+enum { Flag_Matte = 0x0001, Flag_MatteAlpha = 0x0002, Flag_Valid = 0x0004 };
 struct Hit {
     float d[7];     // Ci.rgb, Oi.rgb, depth (Sample_Red..Sample_Depth)
     uint32_t id;    // micropolygon id
@@ -1131,6 +1139,7 @@ struct Hider {
         // constant shader: Oi = Os; Ci = Os * Cs (shaders/surface/constant.sl)
         for (int k = 0; k < 3; ++k) { g->Oi[k] = p.Os[k]; g->Ci[k] = p.Os[k] * p.Cs[k]; }
         g->constantShading = p.shading_interpolation == RH_SHADING_CONSTANT;
+        g->matteFlag = p.matte == 0 ? 0u : (p.matte == 2 ? (uint32_t)Flag_MatteAlpha : (uint32_t)Flag_Matte);
         // Transparency culling (micropolygon.cpp:497-520): Oi == black culls every MP.
         bool allBlack = (g->Oi[0] == 0.0f && g->Oi[1] == 0.0f && g->Oi[2] == 0.0f);
         int cu = g->cu, cv = g->cv;
@@ -1182,6 +1191,7 @@ struct Hider {
         st.grids++;
         for (int k = 0; k < 3; ++k) { g->Oi[k] = p.Os[k]; g->Ci[k] = p.Os[k] * p.Cs[k]; }
         g->constantShading = p.shading_interpolation == RH_SHADING_CONSTANT;
+        g->matteFlag = p.matte == 0 ? 0u : (p.matte == 2 ? (uint32_t)Flag_MatteAlpha : (uint32_t)Flag_Matte);
         const int cu = g->cu, cv = g->cv;
         for (Grid* gk : {g, &g1})
             for (int i = (int)gk->P.size() - 1; i >= 0; --i) {
@@ -1370,7 +1380,7 @@ struct Hider {
                         st.spl_count++;
                         // CqBound::Contains2D (bound.h:144-151)
                         if (vecP.x < Bd.mn.x || vecP.x > Bd.mx.x || vecP.y < Bd.mn.y || vecP.y > Bd.mx.y) continue;
-                        if (Bd.mn.z > sd.occlZ) continue;
+                        if (F.cullableHits && Bd.mn.z > sd.occlZ) continue;
                         // LevelOfDetailBounds default [0,1] (attributes.cpp:125)
                         if (0.0f > sd.lod || sd.lod >= 1.0f) continue;
                         st.spl_bound_hits++;
@@ -1438,7 +1448,7 @@ struct Hider {
                     st.spl_count++;
                     // the shifted bound's Contains2D (bound.h:144-151)
                     if (vecP.x < bminx || vecP.x > bmaxx || vecP.y < bminy || vecP.y > bmaxy) continue;
-                    if (Bd.mn.z > sd.occlZ) continue;
+                    if (F.cullableHits && Bd.mn.z > sd.occlZ) continue;
                     if (0.0f > sd.lod || sd.lod >= 1.0f) continue;
                     st.spl_bound_hits++;
                     // CqMicroPolygon::Sample with DoF (micropolygon.cpp:1561-1593):
@@ -1592,7 +1602,7 @@ struct Hider {
                         st.spl_count++;
                         if (time < time0 || time > time1) continue;
                         if (vecP.x < Bd.mn.x || vecP.x > Bd.mx.x || vecP.y < Bd.mn.y || vecP.y > Bd.mx.y) continue;
-                        if (Bd.mn.z > sd.occlZ) continue;
+                        if (F.cullableHits && Bd.mn.z > sd.occlZ) continue;
                         if (0.0f > sd.lod || sd.lod >= 1.0f) continue;
                         st.spl_bound_hits++;
                         float D;
@@ -1608,17 +1618,32 @@ struct Hider {
         }
     }
 
-    // StoreSample (bucketprocessor.cpp:1471-1569) with min depth filter.
+    // StoreSample (bucketprocessor.cpp:1471-1569).
     void store_sample(MP* mp, int px, int py, int index, float D, V2 uv, bool isOpaque) {
         Sample& sd = sample_at(px, py, index);
-        if (sd.occlZ <= D) return;
+        const bool isCullable = F.cullableHits;
+        if (isCullable && sd.occlZ <= D) return;
         st.spl_hits++;
         pixelHasValid[(size_t)(py - F.extY0) * F.extW + (px - F.extX0)] = 1;
+        const uint32_t matteFlag = mp->g->matteFlag;
         Hit* hit;
-        if (isOpaque) {
+        if ((isOpaque || (matteFlag & Flag_MatteAlpha)) && isCullable) {
             hit = &sd.occl;
-            sd.occlZ = D;
-            if (cull) tree.set_depth(D, sd.occlIndex);
+            if (F.midpointStore) {
+                // the occluding depth becomes the second nearest opaque depth
+                float hitPrevZ = FLT_MAX;
+                if (hit->flags & Flag_Valid) hitPrevZ = hit->d[6];
+                if (hitPrevZ < D) {
+                    sd.occlZ = D;
+                    if (cull) tree.set_depth(D, sd.occlIndex);
+                    return;
+                }
+                sd.occlZ = hitPrevZ;
+                if (cull) tree.set_depth(hitPrevZ, sd.occlIndex);
+            } else {
+                sd.occlZ = D;
+                if (cull) tree.set_depth(D, sd.occlIndex);
+            }
             hit->flags = Flag_Valid;
         } else {
             sd.hits.push_back(Hit());
@@ -1646,6 +1671,7 @@ struct Hider {
         }
         hit->d[6] = D;
         hit->id = mp->id;
+        hit->flags |= matteFlag;
     }
 
     // --- Bucket processing -------------------------------------------------
@@ -1732,7 +1758,8 @@ struct Hider {
     // RenderSurface (bucketprocessor.cpp:942-1059); the cull test applies to
     // surfaces whose Attribute "cull" "hidden" is 1, the default (:950).
     void render_surface(Bucket& b, Surf* s) {
-        if (cull && s->cachedBound && !prims[s->prim].no_cull_hidden && tree.can_cull(s->rasterBound)) {
+        if (cull && F.cullableHits && s->cachedBound && !prims[s->prim].no_cull_hidden &&
+            tree.can_cull(s->rasterBound)) {
             st.occlusion_culled++;
             repost_surface(b, s);
             return;
@@ -1752,30 +1779,70 @@ struct Hider {
         }
     }
 
-    // CqImagePixel::Combine, min depth filter (imagepixel.cpp:144-332)
+    // CqImagePixel::Combine without CSG (imagepixel.cpp:144-332)
     void combine_sample(Sample& sd) {
-        if (sd.hits.empty()) return;
+        const int depthfilter = F.o.depth_filter;
+        const float* zt = F.o.zthreshold;
+        if (sd.hits.empty()) {
+            if (sd.occl.flags & Flag_Valid) {
+                if (sd.occl.flags & Flag_Matte) {
+                    // opaque matte: transparent black
+                    for (int k = 0; k < 6; ++k) sd.occl.d[k] = 0.0f;
+                }
+                if (depthfilter == RH_DEPTH_MIDPOINT)
+                    sd.occl.d[6] = (float)(0.5 * (double)(sd.occl.d[6] + sd.occlZ));
+            }
+            return;
+        }
         if (sd.occl.flags & Flag_Valid) sd.hits.push_back(sd.occl);
         std::sort(sd.hits.begin(), sd.hits.end(),
                   [](const Hit& a, const Hit& b) { return a.d[6] < b.d[6]; });
         float col[3] = {0, 0, 0}, opa[3] = {0, 0, 0};
         float opaqueDepths[2] = {sd.occlZ, FLT_MAX};
-        const float* zt = F.o.zthreshold;
+        float maxOpaqueDepth = FLT_MAX;
         for (auto it = sd.hits.rbegin(); it != sd.hits.rend(); ++it) {
             const float* d = it->d;
-            for (int k = 0; k < 3; ++k) {
-                col[k] = col[k] * (1.0f - tclamp(d[3 + k], 0.0f, 1.0f)) + d[k];
-                opa[k] = (1.0f - opa[k]) * d[3 + k] + opa[k];
+            if (it->flags & Flag_Matte) {
+                // lerp(t, x0, 0) = (1 - t) * x0 + t * 0 (math.h:88-91); the
+                // opacity takes the hit's colour as its weight
+                for (int k = 0; k < 3; ++k) {
+                    const float c = col[k], o = opa[k];
+                    col[k] = (1 - d[3 + k]) * c + d[3 + k] * 0.0f;
+                    opa[k] = (1 - d[k]) * o + d[k] * 0.0f;
+                }
+            } else {
+                for (int k = 0; k < 3; ++k) {
+                    col[k] = col[k] * (1.0f - tclamp(d[3 + k], 0.0f, 1.0f)) + d[k];
+                    opa[k] = (1.0f - opa[k]) * d[3 + k] + opa[k];
+                }
             }
             if (d[3] >= zt[0] && d[4] >= zt[1] && d[5] >= zt[2]) {
                 opaqueDepths[1] = opaqueDepths[0];
                 opaqueDepths[0] = d[6];
+                if (!(maxOpaqueDepth < FLT_MAX)) maxOpaqueDepth = d[6];
             }
         }
         Hit front = sd.hits.front();
         for (int k = 0; k < 3; ++k) { front.d[k] = col[k]; front.d[3 + k] = opa[k]; }
-        front.d[6] = opaqueDepths[0];
         front.flags |= Flag_Valid;
+        float& occlDepth = front.d[6];
+        if (depthfilter == RH_DEPTH_MIDPOINT) {
+            occlDepth = sd.hits.size() > 1 ? (opaqueDepths[0] + opaqueDepths[1]) * 0.5f : FLT_MAX;
+        } else if (depthfilter == RH_DEPTH_MAX) {
+            occlDepth = maxOpaqueDepth;
+        } else if (depthfilter == RH_DEPTH_AVERAGE) {
+            float totDepth = 0.0f;
+            int totCount = 0;
+            for (const Hit& h : sd.hits)
+                if (h.d[3] >= zt[0] || h.d[4] >= zt[1] || h.d[5] >= zt[2]) {
+                    totDepth += h.d[6];
+                    totCount++;
+                }
+            totDepth /= totCount;
+            occlDepth = totDepth;
+        } else {
+            occlDepth = opaqueDepths[0];
+        }
         sd.occl = front;
     }
 

@@ -318,6 +318,127 @@ def constant_interpolation_dof():
     return opts, prims
 
 
+def _z_display(opts, filt):
+    from aqsis_amd.abi import DEPTH_FILTERS
+    opts.display_z = 1
+    opts.depth_filter = DEPTH_FILTERS[filt]
+    return opts
+
+
+def depth_midpoint_z():
+    # A z display with the midpoint depth filter: the occluding hit is the
+    # nearest opaque one and the occluding depth the second nearest
+    # (StoreSample's special case); samples without a list average the two,
+    # listed ones take the mean of the two nearest opaque depths.  Mixed
+    # opacity, occlusion culling against the second nearest depth.
+    opts, prims = scenes.patches_scene(seed=89, n=260, res=(160, 96), samples=(3, 3), opacity=(0.3, 1.0),
+                                       filt=("box", 1.0, 1.0))
+    for p in prims[1::2]:
+        p.Os[:] = [1.0, 1.0, 1.0]
+    return _z_display(opts, "midpoint"), prims
+
+
+def depth_max_z():
+    # A z display with the max depth filter: no hit is occlusion cullable,
+    # every hit (hidden ones too) is listed and composited, the depth is the
+    # farthest one with opacity at zthreshold.
+    opts, prims = scenes.patches_scene(seed=97, n=160, res=(128, 80), samples=(2, 2), opacity=(0.4, 1.0),
+                                       filt=("gaussian", 2.0, 2.0))
+    for p in prims[::2]:
+        p.Os[:] = [1.0, 1.0, 1.0]
+    return _z_display(opts, "max"), prims
+
+
+def depth_average_z():
+    # The average depth filter (every hit listed): the mean depth of the hits
+    # with ANY opacity channel at zthreshold (0.6), while the composite's
+    # opaque depths need all three.  Every surface has its red opacity above
+    # the threshold, so no sample averages over nothing.
+    opts, prims = scenes.patches_scene(seed=101, n=160, res=(128, 80), samples=(2, 2), filt=("box", 1.0, 1.0))
+    rng = np.random.default_rng(101)
+    for p in prims:
+        p.Os[:] = [float(np.float32(rng.uniform(0.65, 1.0))), float(np.float32(rng.uniform(0.2, 1.0))),
+                   float(np.float32(rng.uniform(0.2, 1.0)))]
+    opts.zthreshold[:] = [0.6, 0.6, 0.6]
+    return _z_display(opts, "average"), prims
+
+
+def depth_midpoint_no_z():
+    # The midpoint filter without a z display: hits stored as usual, the
+    # combine's midpoint of the two nearest opaque depths for listed samples.
+    opts, prims = scenes.patches_scene(seed=103, n=200, res=(144, 96), samples=(2, 2), opacity=(0.3, 1.0))
+    for p in prims[::3]:
+        p.Os[:] = [1.0, 1.0, 1.0]
+    opts.depth_filter = 1
+    return opts, prims
+
+
+def matte_mixed():
+    # RiMatte: matte surfaces (1) cut out what lies behind them, opaque ones
+    # to transparent black, transparent ones through the combine's lerps;
+    # matte alpha surfaces (2) are stored as occluding hits whatever their
+    # opacity.  Opaque, transparent, matte and matte alpha patches mixed.
+    opts, prims = scenes.patches_scene(seed=107, n=260, res=(160, 96), samples=(3, 3), opacity=(0.3, 1.0))
+    for i, p in enumerate(prims):
+        if i % 2 == 0:
+            p.Os[:] = [1.0, 1.0, 1.0]
+        if i % 5 == 1:
+            p.matte = 1
+        elif i % 5 == 3:
+            p.matte = 2
+    return opts, prims
+
+
+def matte_max_z_motion():
+    # Matte surfaces with every hit listed (a z display with the max
+    # filter), in a frame with moving surfaces.
+    opts, prims = motion_mixed_opaque()
+    rng = np.random.default_rng(109)
+    for i, p in enumerate(prims):
+        if i % 4 == 1:
+            p.matte = 1
+        elif i % 4 == 2:
+            o = float(np.float32(rng.uniform(0.3, 0.8)))
+            p.Os[:] = [o, o, o]
+        elif i % 4 == 3:
+            p.matte = 2
+    return _z_display(opts, "max"), prims
+
+
+def matte_midpoint_z_motion():
+    # The midpoint store and matte surfaces in the motion sampler.
+    opts, prims = motion_transparent_8x8()
+    for i, p in enumerate(prims):
+        if i % 3 == 0:
+            p.Os[:] = [1.0, 1.0, 1.0]
+        if i % 4 == 1:
+            p.matte = 1
+        elif i % 4 == 3:
+            p.matte = 2
+    return _z_display(opts, "midpoint"), prims
+
+
+def matte_midpoint_z_dof():
+    # The midpoint store and matte surfaces under the lens.
+    opts, prims = spheres_dof_transparent()
+    for i, p in enumerate(prims):
+        if i % 3 == 0:
+            p.matte = 1
+        elif i % 3 == 1:
+            p.matte = 2
+    return _z_display(opts, "midpoint"), prims
+
+
+def depth_average_z_dof():
+    # Every hit listed under the lens (the average filter).
+    opts, prims = patches_dof()
+    rng = np.random.default_rng(113)
+    for p in prims[::2]:
+        p.Os[:] = [float(np.float32(rng.uniform(0.65, 1.0))), 0.3, 0.5]
+    opts.zthreshold[:] = [0.6, 0.6, 0.6]
+    return _z_display(opts, "average"), prims
+
+
 CASES = {
     "spheres_config": spheres_config,
     "patches_small": patches_small,
@@ -354,4 +475,13 @@ CASES = {
     "constant_interpolation_mixed": constant_interpolation_mixed,
     "constant_interpolation_motion": constant_interpolation_motion,
     "constant_interpolation_dof": constant_interpolation_dof,
+    "depth_midpoint_z": depth_midpoint_z,
+    "depth_max_z": depth_max_z,
+    "depth_average_z": depth_average_z,
+    "depth_midpoint_no_z": depth_midpoint_no_z,
+    "matte_mixed": matte_mixed,
+    "matte_max_z_motion": matte_max_z_motion,
+    "matte_midpoint_z_motion": matte_midpoint_z_motion,
+    "matte_midpoint_z_dof": matte_midpoint_z_dof,
+    "depth_average_z_dof": depth_average_z_dof,
 }

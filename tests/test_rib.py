@@ -28,7 +28,8 @@ def _static_keys_cleared(p):
                                   "spheres_exposure_mitchell", "patches_gridsampler", "motion_transparent_8x8",
                                   "motion_mixed_opaque", "motion_spheres", "decal_dof", "spheres_dof_transparent",
                                   "patches_cull_flag_mixed", "constant_interpolation_mixed",
-                                  "constant_interpolation_motion"])
+                                  "constant_interpolation_motion", "depth_midpoint_z", "depth_average_z",
+                                  "depth_midpoint_no_z", "matte_mixed", "matte_max_z_motion"])
 def test_round_trip(case):
     opts, prims = CASES[case]()
     text = scenes.to_rib(opts, prims)
