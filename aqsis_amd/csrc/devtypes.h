@@ -15,6 +15,12 @@
 #define RH_DEV_DEPTH_MIDPOINT 1
 #define RH_DEV_DEPTH_MAX 2
 #define RH_DEV_DEPTH_AVERAGE 3
+// StoreSample's modes (bucketprocessor.cpp:1076-1081, 1490-1535): the usual
+// occluding hit; the midpoint store (a z display with the midpoint filter);
+// every hit listed (a z display with the max or average filter)
+#define RH_STORE_OCCLUDE 0
+#define RH_STORE_MIDPOINT 1
+#define RH_STORE_LIST_ALL 2
 
 // GridDev::type flag of a deforming surface's grid (two motion keys): its
 // parameters hold key 0's then key 1's, then the key times.
@@ -154,9 +160,9 @@ struct SampleLaunch {
     // of field) the task of each such workgroup (null: consecutive tasks).
     const uint32_t* itemCut;
     const uint32_t* taskList;
-    // the midpoint depth filter's store (FrameDev::midpointStore): the
-    // static sampler's midpoint form, whatever the variant
-    int midpoint;
+    // StoreSample's mode (RH_STORE_*, from FrameDev::cullableHits and
+    // midpointStore): the static sampler's form for it
+    int store;
 };
 
 // Lens tables of a frame with depth of field: per pattern the

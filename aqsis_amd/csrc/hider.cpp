@@ -675,6 +675,12 @@ static hipError_t enqueue_filter(rh_ctx* c) {
                             c->dDither.as<float>(), c->dChannels.as<float>(), c->dRgba.as<uchar4>());
 }
 
+// StoreSample's mode of a frame (RH_STORE_*).
+static int store_mode(const FrameDev& f) {
+    if (!f.cullableHits) return RH_STORE_LIST_ALL;
+    return f.midpointStore ? RH_STORE_MIDPOINT : RH_STORE_OCCLUDE;
+}
+
 // The micropolygon records the combine finds an occluding hit's grid in:
 // the motion records in a frame with moving micropolygons.
 static const float* occl_records(rh_ctx* c) {
@@ -720,7 +726,7 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed, const FrameCtl& k, bool g
     a.rowCounts = k.rows; a.rowBase = 0;
     a.idCut = 0xffffffffu; a.depthOnly = 0;
     a.itemCut = nullptr; a.taskList = nullptr;
-    a.midpoint = c->frame.midpointStore;
+    a.store = store_mode(c->frame);
     const int variant = variant_of(tu);
     int rows = 0;
     if (S.motion) {
@@ -811,7 +817,7 @@ static SampleLaunch depth_launch(rh_ctx* c) {
     a.rowCounts = c->dRows.as<uint32_t>(); a.rowBase = 0;
     a.idCut = 0xffffffffu; a.depthOnly = 1;
     // with the midpoint store the occluding depth is the second nearest
-    a.midpoint = c->frame.midpointStore;
+    a.store = store_mode(c->frame);
     return a;
 }
 

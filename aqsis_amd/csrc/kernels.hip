@@ -852,8 +852,9 @@ __device__ __forceinline__ uint32_t list_cut(const uint32_t* __restrict__ L, uin
 // bucket (itemBase/nStatic select the bucket's items), and persistent
 // workgroups pulling items from a queue (persistent != 0; the count is in
 // *nDevice).  Each workgroup writes one row of counters at rowBase +
-// blockIdx.x.
-template <int CH, int SLOTS, bool MID = false>
+// blockIdx.x.  STORE selects StoreSample's mode (RH_STORE_*): the usual
+// occluding hit, the midpoint store, or every hit listed (not cullable).
+template <int CH, int SLOTS, int STORE = RH_STORE_OCCLUDE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
 rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets, const TaskDev* __restrict__ tasks,
           const uint2* __restrict__ items, int itemBase, int nStatic, const uint32_t* __restrict__ nDevice,
@@ -889,7 +890,8 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
     const int itemEnd = persistent ? (int)nDevice[0] : itemBase + nStatic;
     // isCullable (bucketprocessor.cpp:1076-1081): occlusion tests and the
     // occluding hit apply; else every hit is listed
-    const bool cullH = fr->cullableHits != 0;
+    constexpr bool cullH = STORE != RH_STORE_LIST_ALL;
+    constexpr bool MID = STORE == RH_STORE_MIDPOINT;
 
     for (int round = 0;; ++round) {
     int it;
@@ -3087,9 +3089,9 @@ extern "C" hipError_t rh_launch_bin(hipStream_t st, const FrameDev* fr, const Bu
     return hipGetLastError();
 }
 
-template <int CH, int SLOTS, bool MID = false>
+template <int CH, int SLOTS, int STORE = RH_STORE_OCCLUDE>
 static void launch_sample_t(hipStream_t st, int grid, const SampleLaunch& a) {
-    hipLaunchKernelGGL((rh_sample<CH, SLOTS, MID>), dim3(grid), dim3(256), 0, st, a.fr, a.buckets, a.tasks, a.items,
+    hipLaunchKernelGGL((rh_sample<CH, SLOTS, STORE>), dim3(grid), dim3(256), 0, st, a.fr, a.buckets, a.tasks, a.items,
                        a.itemBase, a.nStatic, a.nDevice, a.persistent, a.queueHead, a.capStart, a.listCount, a.lists,
                        reinterpret_cast<const float4*>(a.mprec), a.grids, a.prims, a.pos2d, a.pos1d, a.pixPattern,
                        a.samples, a.hitData, a.hitNext, a.hitCounter, a.combineList, a.frontDepth, a.rowCounts,
@@ -3097,14 +3099,18 @@ static void launch_sample_t(hipStream_t st, int grid, const SampleLaunch& a) {
 }
 
 // One sampling launch of `grid` workgroups with kernel variant `variant`
-// (RH_VARIANT_CH / RH_VARIANT_SLOTS).  The midpoint store takes one form
-// (96-record chunks, 2 blocks per wave) whatever the variant: a wave loops
-// over an item's blocks, so any form covers any item.
+// (RH_VARIANT_CH / RH_VARIANT_SLOTS).  The midpoint and list-all stores
+// take one form each (96-record chunks, 2 blocks per wave) whatever the
+// variant: a wave loops over an item's blocks, so any form covers any item.
 extern "C" hipError_t rh_launch_sample(hipStream_t st, int variant, int grid, const SampleLaunch* a) {
     if (grid <= 0) return hipSuccess;
     if (variant < 0 || variant >= RH_SAMPLE_VARIANTS) return hipErrorInvalidValue;
-    if (a->midpoint) {
-        launch_sample_t<96, 2, true>(st, grid, *a);
+    if (a->store == RH_STORE_MIDPOINT) {
+        launch_sample_t<96, 2, RH_STORE_MIDPOINT>(st, grid, *a);
+        return hipGetLastError();
+    }
+    if (a->store == RH_STORE_LIST_ALL) {
+        launch_sample_t<96, 2, RH_STORE_LIST_ALL>(st, grid, *a);
         return hipGetLastError();
     }
     switch (variant) {
