@@ -158,6 +158,7 @@ struct rh_ctx {
     DBuf dHitData, dHitNext, dHitCtl, dCombine, dFrontDepth;
     DBuf dLongWork;                       // the long-list combine's work arrays (grown on demand)
     bool transparent = false;             // some posted surface has Os < 1
+    bool anyMatte = false;                // some posted surface is a matte (RH_MATTE)
     // work items: one per task (grid and per-bucket forms), cost-split (batched)
     DBuf dItemsStatic, dItems, dItemCount, dQueue, dRows;
     // occlusion culling: bucket trees, depth pyramids, (surface, bucket) tests, probe results
@@ -632,10 +633,13 @@ static rh_status upload_schedule(rh_ctx* c) {
     // Transparent hits need list storage; start at two per sample and grow
     // when a frame asks for more.
     const rh::Shape& s = c->shape;
-    bool transparent = false;
-    for (const auto& q : c->prims)
+    bool transparent = false, anyMatte = false;
+    for (const auto& q : c->prims) {
         if (rh::hits_listed(s.o, q)) transparent = true;
+        if (q.matte == RH_MATTE) anyMatte = true;
+    }
     c->transparent = transparent;
+    c->anyMatte = anyMatte;
     const uint64_t nsamp = (uint64_t)s.extW * s.extH * s.ns;
     const uint64_t want = transparent ? arena_guess(s, S, c->prims.data()) : 64;
     HIP_TRY(c->dCombine.ensure((transparent ? nsamp : 1) * sizeof(uint4)));
@@ -682,8 +686,10 @@ static int store_mode(const FrameDev& f) {
 }
 
 // The micropolygon records the combine finds an occluding hit's grid in:
-// the motion records in a frame with moving micropolygons.
+// the motion records in a frame with moving micropolygons; null when no
+// surface is a matte (the combine then looks up no surface for it).
 static const float* occl_records(rh_ctx* c) {
+    if (!c->anyMatte) return nullptr;
     return c->sched.motion ? reinterpret_cast<const float*>(c->dMprecMB.p) : c->dMprec.as<float>();
 }
 

@@ -873,7 +873,8 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
     __shared__ unsigned long long batM[RH_S_WAVES][64];
     __shared__ float4 batB[RH_S_WAVES][64];
     __shared__ int sItem;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the wave index is uniform: block and slot geometry stay in scalar registers
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int ns = fr->ns, nx = fr->nx, ny = fr->ny, bw = fr->blockW, bh = fr->blockH;
     // lane -> (column, row) of the block's sub-sample grid -> pixel, sub-sample
     const int BW = bw * nx, BH = bh * ny;
@@ -883,8 +884,9 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
     const int pix = ly * bw + lx, si = sn * nx + sm;
     unsigned long long rowRep = 0ull;
     for (int r = 0; r < BH; ++r) rowRep |= 1ull << (r * BW);
-    uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
-    uint32_t nTestW = 0;                  // wave-uniform part of nTest
+    uint32_t nTest = 0;
+    // wave-uniform counts (ballots): nTest's uniform part and the others
+    uint32_t nTestW = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
     uint32_t wbBase = 0;                  // the wave's current arena block
     int wbLeft = 0;
     const int itemEnd = persistent ? (int)nDevice[0] : itemBase + nStatic;
@@ -924,7 +926,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
 
     for (int p0 = 0; p0 < blkN; p0 += RH_S_WAVES * SLOTS) {
         bool has[SLOTS], live[SLOTS], lodOk[SLOTS];
-        int gx[SLOTS], gy[SLOTS], bgx[SLOTS], bgy[SLOTS];
+        int bxs[SLOTS], bys[SLOTS], bgx[SLOTS], bgy[SLOTS];
         float sx[SLOTS], sy[SLOTS];
         float occlZ[SLOTS], wu[SLOTS], wv[SLOTS];
         float sD[SLOTS];                  // MID: the occluding hit's depth
@@ -940,8 +942,8 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
             const int bx = T.x0 + (blk % nbx) * bw, by = T.y0 + (blk / nbx) * bh;
             const int px = bx + lx, py = by + ly;
             live[k] = has[k] && laneOk && px < T.x1 && py < T.y1;
-            gx[k] = (px - B.sx0) * nx + sm;
-            gy[k] = (py - B.sy0) * ny + sn;
+            bxs[k] = bx;
+            bys[k] = by;
             bgx[k] = (bx - B.sx0) * nx;
             bgy[k] = (by - B.sy0) * ny;
             occlZ[k] = FLT_MAX;
@@ -1081,7 +1083,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                     while (__ballot(pb != 0ull)) {
                         // One hit test per lane; the outcome is applied below in
                         // one place (opaque store or transparent append).
-                        bool hit = false;
+                        bool hit = false, cBound = false, cIn = false, cIn2 = false;
                         float D = 0.f, u = 0.5f, v = 0.5f;
                         uint32_t hflags = 0u, hid = 0u, hgrid = 0u;
                         if (pb != 0ull) {
@@ -1092,7 +1094,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                             const float4 r8 = R[8];
                             // occlusion skip against the current opaque hit (bucketprocessor.cpp:1179)
                             if (!(cullH && r8.x > occlZ[k])) {
-                                ++nBound;
+                                cBound = true;
                                 // fContains (micropolygon.cpp:1295-1342): edges 0,1 reject on <= 0, edges 2,3 on < 0
                                 const float4 YM = R[0], XM = R[1], X = R[2], Y = R[3];
                                 const float px_ = sx[k], py_ = sy[k];
@@ -1101,7 +1103,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                                 const float t2 = ((py_ - Y.z) * YM.z) - ((px_ - X.z) * XM.z);
                                 const float t3 = ((py_ - Y.w) * YM.w) - ((px_ - X.w) * XM.w);
                                 if (!(t0 <= 0 || t1 <= 0 || t2 < 0 || t3 < 0)) {
-                                    ++nIn;
+                                    cIn = true;
                                     // CqInvBilinear::operator() (bilinear.h:277-305)
                                     const float4 r5 = R[5], r6 = R[6], Z = R[4];
                                     const float Ax = r5.x, Ay = r5.y, Ex = r5.z, Ey = r5.w, Fx = r6.x, Fy = r6.y,
@@ -1118,7 +1120,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                                     }
                                     hflags = __float_as_uint(r8.y);
                                     if ((hflags & 1u) == 0) {
-                                        ++nIn2;
+                                        cIn2 = true;
                                         const float M1x = Ex + Gx * v, M1y = Ey + Gy * v;
                                         const float M2x = Fx + Gx * u, M2y = Fy + Gy * u;
                                         const float bx = (((Ax + Ex * u) + Fx * v) + (Gx * u) * v) - px_;
@@ -1139,7 +1141,10 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
                                 }
                             }
                         }
-                        nHit += hit ? 1u : 0u;
+                        nBound += (uint32_t)__popcll(__ballot(cBound));
+                        nIn += (uint32_t)__popcll(__ballot(cIn));
+                        nIn2 += (uint32_t)__popcll(__ballot(cIn2));
+                        nHit += (uint32_t)__popcll(__ballot(hit));
                         // opaque (or matte alpha) with occlusion: the occluding hit
                         const bool opq = hit && (hflags & 2u) != 0 && cullH;
                         if (MID) {
@@ -1194,7 +1199,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
             if (!live[k]) continue;
-            const int px = B.sx0 + gx[k] / nx, py = B.sy0 + gy[k] / ny;
+            const int px = bxs[k] + lx, py = bys[k] + ly;
             const size_t sidx = ((size_t)(py - fr->extY0) * fr->extW + (px - fr->extX0)) * ns + si;
             float4 a = make_float4(0.f, 0.f, 0.f, FLT_MAX), b = make_float4(0.f, 0.f, 0.f, __uint_as_float(win[k]));
             if (win[k] != 0xffffffffu) {
@@ -1227,13 +1232,7 @@ rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets
     // The workgroup's hider counters: wave sums, then one row (summed by
     // rh_reduce_counters; same-address atomics from every wave would
     // serialise in L2).
-    for (int off = 32; off > 0; off >>= 1) {
-        nTest += __shfl_xor(nTest, off);
-        nBound += __shfl_xor(nBound, off);
-        nHit += __shfl_xor(nHit, off);
-        nIn += __shfl_xor(nIn, off);
-        nIn2 += __shfl_xor(nIn2, off);
-    }
+    for (int off = 32; off > 0; off >>= 1) nTest += __shfl_xor(nTest, off);
     if (lane == 0) {
         wsum[wave][0] = nTest; wsum[wave][1] = nBound; wsum[wave][2] = nHit; wsum[wave][3] = nIn;
         wsum[wave][4] = nIn2;
@@ -1400,7 +1399,7 @@ __device__ __forceinline__ int select_bit(unsigned long long m, int k) {
     return pos + ((k >= (int)(w & 1u)) ? 1 : 0);
 }
 
-template <int NW>
+template <int NW, int STORE>
 __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restrict__ fr,
                                                         const BucketDev* __restrict__ buckets,
                                                         const TaskDev* __restrict__ tasks, int ntasks,
@@ -1443,8 +1442,9 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
     __shared__ uint32_t wsum[NW][5];
     const int slot = taskBase + (int)blockIdx.x;
     const int task = (taskList && slot < ntasks) ? (int)taskList[slot] : slot;
-    const bool cullH = fr->cullableHits != 0, midS = fr->midpointStore != 0;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // StoreSample's mode (RH_STORE_*)
+    constexpr bool cullH = STORE != RH_STORE_LIST_ALL, midS = STORE == RH_STORE_MIDPOINT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const unsigned long long lanesBelow = (1ull << lane) - 1ull;
     uint32_t nTest = 0, nBound = 0, nHit = 0, nIn = 0, nIn2 = 0;
     uint32_t wbBase = 0;
@@ -1900,14 +1900,19 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
                             // occludes with the second nearest (:1502-1525)
                             nHit += hit ? 1u : 0u;
                             const bool opq = hit && opaque && cullH;
-                            const float prevZ = win != 0xffffffffu ? sD : FLT_MAX;
-                            const bool keep = !midS || !(prevZ < D);
-                            const bool store = opq && keep;
-                            occlZ = opq ? (midS && keep ? prevZ : D) : occlZ;
+                            bool store = opq;
+                            if (midS) {
+                                const float prevZ = win != 0xffffffffu ? sD : FLT_MAX;
+                                const bool keep = !(prevZ < D);
+                                store = opq && keep;
+                                occlZ = opq ? (keep ? prevZ : D) : occlZ;
+                                sD = store ? D : sD;
+                            } else {
+                                occlZ = opq ? D : occlZ;
+                            }
                             win = store ? hid : win;
                             wu = store ? u : wu;
                             wv = store ? v : wv;
-                            sD = store ? D : sD;
                             const bool app = hit && !opq && !depthOnly;
                             const unsigned long long am = __ballot(app);
                             if (am) {
@@ -1948,7 +1953,7 @@ __global__ void __launch_bounds__(NW * 64) rh_sample_mb(const FrameDev* __restri
             if (win != 0xffffffffu) {
                 const float4 r6w = mprecMB[(size_t)win * 8 + 6];
                 const PrimDev& P = prims[grids[__float_as_int(r6w.y)].prim];
-                shade_hit(P, wu, wv, sD, win, a, b);
+                shade_hit(P, wu, wv, midS ? sD : occlZ, win, a, b);
                 finish_occluding(fr, P, nh > 0, depthOnly, occlZ, a, b);
             }
             samples[2 * sidx] = a;
@@ -2102,7 +2107,7 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
     __shared__ unsigned long long matM[RH_DOF_WAVES][RH_DOF_CH];
     __shared__ float4 matB[RH_DOF_WAVES][RH_DOF_CH];
     __shared__ uint32_t wsum[RH_DOF_WAVES][5];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int ns = fr->ns;
     const int task = taskList ? (int)taskList[itemBase + blockIdx.x] : itemBase + (int)blockIdx.x;
     const TaskDev T = tasks[task];
@@ -2437,7 +2442,8 @@ __global__ void rh_frame_begin(unsigned long long* __restrict__ counters, uint32
 //
 // Where the occluding hit's surface is found: its micropolygon record, of
 // occ.recFloats floats with the grid index (int bits) at occ.gridField, and
-// the grid's surface.
+// the grid's surface.  occ.mprec is null when no surface is a matte: then
+// no hit's matte flag is looked at.
 struct OcclRec {
     const float* mprec;
     int recFloats, gridField;
@@ -2457,8 +2463,9 @@ __device__ __forceinline__ void combine_sample(const FrameDev* __restrict__ fr, 
     int n = (int)e.w;
     const float4 oA = samples[2 * sidx], oB = samples[2 * sidx + 1];
     // the sampler's occluding depth (SqSampleData::occlZ): the occluding
-    // hit's depth, or with the midpoint store the second nearest opaque one
-    const float occlZ = frontDepth[sidx];
+    // hit's depth (or FLT_MAX without one), or with the midpoint store the
+    // second nearest opaque depth, which the sampler leaves in frontDepth
+    const float occlZ = fr->midpointStore ? frontDepth[sidx] : oA.w;
     // the list is linked newest first; rebuild insertion order
     for (int q = n - 1; q >= 0; --q) {
         x[q] = head;
@@ -2471,8 +2478,10 @@ __device__ __forceinline__ void combine_sample(const FrameDev* __restrict__ fr, 
         x[n] = 0xffffffffu;
         d[n] = oA.w;
         ++n;
-        const int g = __float_as_int(occ.mprec[(size_t)occId * occ.recFloats + occ.gridField]);
-        occMatte = prims[occ.grids[g].prim].matte == RH_DEV_MATTE;
+        if (occ.mprec) {
+            const int g = __float_as_int(occ.mprec[(size_t)occId * occ.recFloats + occ.gridField]);
+            occMatte = prims[occ.grids[g].prim].matte == RH_DEV_MATTE;
+        }
     }
     if (longList) rhsort::sort_long(d, x, n, stk);
     else rhsort::sort_with(d, x, n, stk);
@@ -2487,7 +2496,7 @@ __device__ __forceinline__ void combine_sample(const FrameDev* __restrict__ fr, 
             const float4 h = hitData[x[q]];
             const PrimDev& P = prims[hitNext[2 * (size_t)x[q] + 1]];
             shade_hit(P, h.x, h.y, h.z, __float_as_uint(h.w), hA, hB);
-            matte = P.matte == RH_DEV_MATTE;
+            matte = occ.mprec && P.matte == RH_DEV_MATTE;
         }
         const float Ci[3] = {hA.x, hA.y, hA.z}, Oi[3] = {hB.x, hB.y, hB.z};
         if (matte) {
@@ -3129,11 +3138,19 @@ extern "C" hipError_t rh_launch_sample(hipStream_t st, int variant, int grid, co
 extern "C" hipError_t rh_launch_sample_mb(hipStream_t st, const SampleLaunch* a, int ntasks,
                                           const float4* mprecMB) {
     if (ntasks <= 0) return hipSuccess;
-    hipLaunchKernelGGL((rh_sample_mb<RH_MB_WAVES>), dim3(ntasks), dim3(RH_MB_WAVES * 64), 0, st, a->fr, a->buckets, a->tasks,
-                       a->itemBase + ntasks, a->capStart, a->listCount, a->lists, mprecMB, a->grids, a->prims,
-                       a->pos2d, a->pos1d, a->pixPattern, a->samples, a->hitData, a->hitNext, a->hitCounter,
-                       a->combineList, a->frontDepth, a->rowCounts, a->rowBase, a->itemBase, a->idCut, a->itemCut,
-                       a->taskList, a->depthOnly);
+#define RH_LAUNCH_MB(STORE)                                                                                          \
+    hipLaunchKernelGGL((rh_sample_mb<RH_MB_WAVES, STORE>), dim3(ntasks), dim3(RH_MB_WAVES * 64), 0, st, a->fr,        \
+                       a->buckets, a->tasks, a->itemBase + ntasks, a->capStart, a->listCount, a->lists, mprecMB,       \
+                       a->grids, a->prims, a->pos2d, a->pos1d, a->pixPattern, a->samples, a->hitData, a->hitNext,     \
+                       a->hitCounter, a->combineList, a->frontDepth, a->rowCounts, a->rowBase, a->itemBase, a->idCut, \
+                       a->itemCut, a->taskList, a->depthOnly)
+    switch (a->store) {
+    case RH_STORE_OCCLUDE: RH_LAUNCH_MB(RH_STORE_OCCLUDE); break;
+    case RH_STORE_MIDPOINT: RH_LAUNCH_MB(RH_STORE_MIDPOINT); break;
+    case RH_STORE_LIST_ALL: RH_LAUNCH_MB(RH_STORE_LIST_ALL); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef RH_LAUNCH_MB
     return hipGetLastError();
 }
 
