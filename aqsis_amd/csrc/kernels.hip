@@ -844,6 +844,11 @@ __device__ __forceinline__ uint32_t list_cut(const uint32_t* __restrict__ L, uin
 
 #define RH_S_WAVES 4
 #define RH_S_GROUP 16
+// Waves per SIMD of the 96-record, 2-blocks-per-wave variant: its 95
+// registers and 21 KB of LDS leave room for 5 (the others: 4; 6 would spill).
+#ifndef RH_WPE_96_2
+#define RH_WPE_96_2 5
+#endif
 
 // Work items: (task, first block | block count << 16).  CH records per LDS
 // chunk, SLOTS blocks of 64 samples per wave per pass.  Three launch forms:
@@ -854,8 +859,8 @@ __device__ __forceinline__ uint32_t list_cut(const uint32_t* __restrict__ L, uin
 // *nDevice).  Each workgroup writes one row of counters at rowBase +
 // blockIdx.x.  STORE selects StoreSample's mode (RH_STORE_*): the usual
 // occluding hit, the midpoint store, or every hit listed (not cullable).
-template <int CH, int SLOTS, int STORE = RH_STORE_OCCLUDE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+template <int CH, int SLOTS, int STORE = RH_STORE_OCCLUDE, int WPE = 4>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 rh_sample(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets, const TaskDev* __restrict__ tasks,
           const uint2* __restrict__ items, int itemBase, int nStatic, const uint32_t* __restrict__ nDevice,
           int persistent, uint32_t* __restrict__ queueHead, const unsigned long long* __restrict__ capStart,
@@ -3098,9 +3103,9 @@ extern "C" hipError_t rh_launch_bin(hipStream_t st, const FrameDev* fr, const Bu
     return hipGetLastError();
 }
 
-template <int CH, int SLOTS, int STORE = RH_STORE_OCCLUDE>
+template <int CH, int SLOTS, int STORE = RH_STORE_OCCLUDE, int WPE = 4>
 static void launch_sample_t(hipStream_t st, int grid, const SampleLaunch& a) {
-    hipLaunchKernelGGL((rh_sample<CH, SLOTS, STORE>), dim3(grid), dim3(256), 0, st, a.fr, a.buckets, a.tasks, a.items,
+    hipLaunchKernelGGL((rh_sample<CH, SLOTS, STORE, WPE>), dim3(grid), dim3(256), 0, st, a.fr, a.buckets, a.tasks, a.items,
                        a.itemBase, a.nStatic, a.nDevice, a.persistent, a.queueHead, a.capStart, a.listCount, a.lists,
                        reinterpret_cast<const float4*>(a.mprec), a.grids, a.prims, a.pos2d, a.pos1d, a.pixPattern,
                        a.samples, a.hitData, a.hitNext, a.hitCounter, a.combineList, a.frontDepth, a.rowCounts,
@@ -3126,7 +3131,7 @@ extern "C" hipError_t rh_launch_sample(hipStream_t st, int variant, int grid, co
     case 0: launch_sample_t<216, 4>(st, grid, *a); break;
     case 1: launch_sample_t<216, 2>(st, grid, *a); break;
     case 2: launch_sample_t<96, 4>(st, grid, *a); break;
-    case 3: launch_sample_t<96, 2>(st, grid, *a); break;
+    case 3: launch_sample_t<96, 2, RH_STORE_OCCLUDE, RH_WPE_96_2>(st, grid, *a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -3172,7 +3177,9 @@ extern "C" hipError_t rh_sample_occupancy(int variant, int* blocksPerCU) {
     case 0: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<216, 4>, 256, 0);
     case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<216, 2>, 256, 0);
     case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<96, 4>, 256, 0);
-    case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<96, 2>, 256, 0);
+    case 3:
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, rh_sample<96, 2, RH_STORE_OCCLUDE, RH_WPE_96_2>,
+                                                            256, 0);
     default: return hipErrorInvalidValue;
     }
 }
