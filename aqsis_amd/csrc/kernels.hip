@@ -2732,9 +2732,10 @@ __global__ void __launch_bounds__(256) rh_box_masks(const FrameDev* __restrict__
 }
 
 // Tiled filter.  One workgroup per RH_FT x RH_FT tile of display pixels,
-// two waves: wave 0 sums Ci.rgb and z, wave 1 sums Oi.rgb, each lane one
-// pixel, each sum in the reference's order (filter rows, filter columns,
-// samples), so the split changes no rounding.  The tile's halo of samples is
+// four waves over the channels: wave 0 sums Ci.r and Ci.g (with the weight
+// total and the count), wave 1 Ci.b and z, wave 2 Oi.r and Oi.g, wave 3
+// Oi.b; each lane one pixel, each sum in the reference's order (filter rows,
+// filter columns, samples), so the split changes no rounding.  The tile's halo of samples is
 // staged in LDS once, with RH_FU coalesced loads in flight per lane, as 8
 // dwords per sample: Ci.rgb, z, Oi.rgb and a word holding the sample's
 // box-test mask (bits 0..15) and whether it holds a surface (bit 31).  Halo
@@ -2742,6 +2743,7 @@ __global__ void __launch_bounds__(256) rh_box_masks(const FrameDev* __restrict__
 // different LDS banks.
 #define RH_FT 8
 #define RH_FU 16
+#define RH_FTHREADS 256
 #ifndef RH_FPAD
 #define RH_FPAD 1
 #endif
@@ -2806,14 +2808,14 @@ __device__ __forceinline__ void filter_taps(const float* __restrict__ rec0, cons
             }
         }
 }
-__global__ void __launch_bounds__(128) rh_filter_tiled(const FrameDev* __restrict__ fr,
+__global__ void __launch_bounds__(RH_FTHREADS) rh_filter_tiled(const FrameDev* __restrict__ fr,
                                                        const float4* __restrict__ samples,
                                                        const uint16_t* __restrict__ boxMask,
                                                        const float* __restrict__ filterTab,
                                                        const float* __restrict__ dither, float* __restrict__ channels,
                                                        uchar4* __restrict__ rgba8, int tilesX, int ntiles) {
     // LDS: halo records (RH_FT+2*xm)*(RH_FT+2*ym)*(8*ns+RH_FPAD) floats; the
-    // tail exchange reuses the first 192 floats.
+    // tail exchange reuses the first 320 floats.
     extern __shared__ float lds[];
     // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
     // contiguous run of tiles so neighbouring tiles share halos in its L2.
@@ -2831,16 +2833,16 @@ __global__ void __launch_bounds__(128) rh_filter_tiled(const FrameDev* __restric
     // 1. Stage the halo's sample records (2 float4 per sample) and masks.
     {
         const int per_px = 2 * ns, total = NH * per_px;
-        const int dhp = 128 / per_px, dpart = 128 % per_px;
+        const int dhp = RH_FTHREADS / per_px, dpart = RH_FTHREADS % per_px;
         int hp = tid / per_px, part = tid % per_px;
         int hy = hp / HX, hx = hp % HX;
-        for (int base = 0; base < total; base += 128 * RH_FU) {
+        for (int base = 0; base < total; base += RH_FTHREADS * RH_FU) {
             float4 v[RH_FU];
             uint32_t mk[RH_FU];
             int dst[RH_FU];
 #pragma unroll
             for (int u = 0; u < RH_FU; ++u) {
-                const int i = base + u * 128 + tid;
+                const int i = base + u * RH_FTHREADS + tid;
                 v[u] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0xffffffffu));
                 mk[u] = 0;
                 dst[u] = -1;
@@ -2873,28 +2875,36 @@ __global__ void __launch_bounds__(128) rh_filter_tiled(const FrameDev* __restric
         }
     }
     __syncthreads();
-    // 2. Weighted sums, one lane per pixel per half of the channels.
-    const int lane = tid & 63, half = tid >> 6;
+    // 2. Weighted sums, one lane per pixel per wave's channels.
+    const int lane = tid & 63, part = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lx = lane & (RH_FT - 1), ly = lane / RH_FT;
     const int x = tx0 + lx, y = ty0 + ly;
     const bool valid = x < fr->cropX1 && y < fr->cropY1;
     float gTot = 0.0f;
     int count = 0;
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float acc[2] = {0.0f, 0.0f};
     if (valid) {
-        if (half) filter_taps<3, false>(lds + 4, filterTab, ns, xm, ym, HX, S, lx, ly, 3, acc, gTot, count);
-        else filter_taps<4, true>(lds, filterTab, ns, xm, ym, HX, S, lx, ly, 7, acc, gTot, count);
+        // record words: Ci.rgb, z, Oi.rgb, mask
+        if (part == 0) filter_taps<2, true>(lds, filterTab, ns, xm, ym, HX, S, lx, ly, 7, acc, gTot, count);
+        else if (part == 1) filter_taps<2, false>(lds + 2, filterTab, ns, xm, ym, HX, S, lx, ly, 5, acc, gTot, count);
+        else if (part == 2) filter_taps<2, false>(lds + 4, filterTab, ns, xm, ym, HX, S, lx, ly, 3, acc, gTot, count);
+        else filter_taps<1, false>(lds + 6, filterTab, ns, xm, ym, HX, S, lx, ly, 1, acc, gTot, count);
     }
     __syncthreads();                  // the halo records are no longer read
-    if (half) {
-        lds[lane * 3 + 0] = acc[0];
-        lds[lane * 3 + 1] = acc[1];
-        lds[lane * 3 + 2] = acc[2];
+    // waves 1..3 hand their sums to wave 0: Ci.b, z, Oi.r, Oi.g, Oi.b
+    if (part == 1) {
+        lds[lane * 5 + 0] = acc[0];
+        lds[lane * 5 + 1] = acc[1];
+    } else if (part == 2) {
+        lds[lane * 5 + 2] = acc[0];
+        lds[lane * 5 + 3] = acc[1];
+    } else if (part == 3) {
+        lds[lane * 5 + 4] = acc[0];
     }
     __syncthreads();
-    if (!half && valid) {
-        const float full[7] = {acc[0], acc[1], acc[2], lds[lane * 3 + 0], lds[lane * 3 + 1], lds[lane * 3 + 2],
-                               acc[3]};
+    if (part == 0 && valid) {
+        const float full[7] = {acc[0], acc[1], lds[lane * 5 + 0], lds[lane * 5 + 2], lds[lane * 5 + 3],
+                               lds[lane * 5 + 4], lds[lane * 5 + 1]};
         const size_t t = (size_t)(y - fr->cropY0) * (fr->cropX1 - fr->cropX0) + (x - fr->cropX0);
         finish_pixel(fr, full, gTot, count, ns, t, dither, channels, rgba8);
     }
@@ -3261,7 +3271,7 @@ extern "C" hipError_t rh_launch_filter(hipStream_t st, const FrameDev* fr, int w
         const int tilesX = (width + RH_FT - 1) / RH_FT, tilesY = (height + RH_FT - 1) / RH_FT;
         const int ntiles = tilesX * tilesY;
         const int per = (ntiles + 7) / 8;
-        hipLaunchKernelGGL(rh_filter_tiled, dim3(8 * per), dim3(128), lds, st, fr, samples, boxMask, filterTab,
+        hipLaunchKernelGGL(rh_filter_tiled, dim3(8 * per), dim3(RH_FTHREADS), lds, st, fr, samples, boxMask, filterTab,
                            dither, channels, rgba8, tilesX, ntiles);
     } else {
         hipLaunchKernelGGL(rh_filter, dim3((npix + 255) / 256), dim3(256), 0, st, fr, samples, pixPattern, pos2d,
