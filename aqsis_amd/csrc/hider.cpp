@@ -35,10 +35,10 @@ extern "C" hipError_t rh_launch_plan_items(hipStream_t, const FrameDev*, const T
 extern "C" hipError_t rh_launch_frame_begin(hipStream_t, unsigned long long*, uint32_t*);
 extern "C" hipError_t rh_launch_combine_long(hipStream_t, const FrameDev*, size_t, const uint4*, uint32_t*,
                                              const float4*, const uint32_t*, const PrimDev*, const float*, int, int,
-                                             const GridDev*, float4*, float*, float*, uint32_t*, int*);
+                                             const GridDev*, int, float4*, float*, float*, uint32_t*, int*);
 extern "C" hipError_t rh_launch_combine(hipStream_t, const FrameDev*, size_t, const uint4*, uint32_t*,
                                         const float4*, const uint32_t*, const PrimDev*, const float*, int, int,
-                                        const GridDev*, float4*, float*);
+                                        const GridDev*, int, float4*, float*);
 extern "C" hipError_t rh_launch_reduce_counters(hipStream_t, const uint32_t*, int, const uint32_t*, int,
                                                 unsigned long long*);
 extern "C" size_t rh_filter_tile_lds(int, int, int);
@@ -679,6 +679,12 @@ static hipError_t enqueue_filter(rh_ctx* c) {
                             c->dDither.as<float>(), c->dChannels.as<float>(), c->dRgba.as<uchar4>());
 }
 
+// Whether the combine takes its full form: a depth filter other than min, or
+// some matte surface.
+static int combine_full(const rh_ctx* c) {
+    return (c->frame.depthFilter != RH_DEPTH_MIN || c->anyMatte) ? 1 : 0;
+}
+
 // StoreSample's mode of a frame (RH_STORE_*).
 static int store_mode(const FrameDev& f) {
     if (!f.cullableHits) return RH_STORE_LIST_ALL;
@@ -772,7 +778,8 @@ static hipError_t enqueue_frame(rh_ctx* c, bool timed, const FrameCtl& k, bool g
         ENQ(rh_launch_combine(c->st, f, (size_t)s.extW * s.extH * s.ns, c->dCombine.as<uint4>(),
                               k.hitCtl, c->dHitData.as<float4>(), c->dHitNext.as<uint32_t>(),
                               c->dPrims.as<PrimDev>(), occl_records(c), S.motion ? RH_MPREC_MB_FLOATS : RH_MPREC_FLOATS,
-                              S.motion ? RH_MPREC_MB_GRID : RH_MPREC_GRID, c->dGrids.as<GridDev>(), c->dSamples.as<float4>(),
+                              S.motion ? RH_MPREC_MB_GRID : RH_MPREC_GRID, c->dGrids.as<GridDev>(), combine_full(c),
+                              c->dSamples.as<float4>(),
                               c->dFrontDepth.as<float>()));
     if (timed) ENQ(hipEventRecord(c->ev[3], c->st));
     ENQ(rh_launch_reduce_counters(c->st, k.rows, rows, c->dBucketRefs.as<uint32_t>(), nb, k.counters));
@@ -799,8 +806,8 @@ static rh_status combine_long_lists(rh_ctx* c, const FrameCtl& k, const uint32_t
                                    c->dCombine.as<uint4>(), k.hitCtl, c->dHitData.as<float4>(),
                                    c->dHitNext.as<uint32_t>(), c->dPrims.as<PrimDev>(), occl_records(c),
                                    motion ? RH_MPREC_MB_FLOATS : RH_MPREC_FLOATS, motion ? RH_MPREC_MB_GRID : RH_MPREC_GRID,
-                                   c->dGrids.as<GridDev>(), c->dSamples.as<float4>(), c->dFrontDepth.as<float>(),
-                                   workD, workX, workStk));
+                                   c->dGrids.as<GridDev>(), combine_full(c), c->dSamples.as<float4>(),
+                                   c->dFrontDepth.as<float>(), workD, workX, workStk));
     HIP_TRY(enqueue_filter(c));
     HIP_TRY(hipStreamSynchronize(c->st));
     return RH_OK;

@@ -2458,7 +2458,9 @@ struct OcclRec {
 // CqImagePixel::Combine of one listed sample (see rh_combine); d, x: room
 // for its hits and its occluding hit; stk: the sort's work stack.  Inlined
 // into each kernel, so that rh_combine keeps its lists in LDS with LDS
-// instructions.
+// instructions.  FULL: a depth filter other than min or some matte surface
+// (without, the steps that only they take are compiled out).
+template <bool FULL>
 __device__ __forceinline__ void combine_sample(const FrameDev* __restrict__ fr, const uint4 e, float* d, uint32_t* x, int* stk,
                                bool longList, const float4* __restrict__ hitData,
                                const uint32_t* __restrict__ hitNext, const PrimDev* __restrict__ prims,
@@ -2470,7 +2472,7 @@ __device__ __forceinline__ void combine_sample(const FrameDev* __restrict__ fr, 
     // the sampler's occluding depth (SqSampleData::occlZ): the occluding
     // hit's depth (or FLT_MAX without one), or with the midpoint store the
     // second nearest opaque depth, which the sampler leaves in frontDepth
-    const float occlZ = fr->midpointStore ? frontDepth[sidx] : oA.w;
+    const float occlZ = FULL && fr->midpointStore ? frontDepth[sidx] : oA.w;
     // the list is linked newest first; rebuild insertion order
     for (int q = n - 1; q >= 0; --q) {
         x[q] = head;
@@ -2483,7 +2485,7 @@ __device__ __forceinline__ void combine_sample(const FrameDev* __restrict__ fr, 
         x[n] = 0xffffffffu;
         d[n] = oA.w;
         ++n;
-        if (occ.mprec) {
+        if (FULL && occ.mprec) {
             const int g = __float_as_int(occ.mprec[(size_t)occId * occ.recFloats + occ.gridField]);
             occMatte = prims[occ.grids[g].prim].matte == RH_DEV_MATTE;
         }
@@ -2501,10 +2503,10 @@ __device__ __forceinline__ void combine_sample(const FrameDev* __restrict__ fr, 
             const float4 h = hitData[x[q]];
             const PrimDev& P = prims[hitNext[2 * (size_t)x[q] + 1]];
             shade_hit(P, h.x, h.y, h.z, __float_as_uint(h.w), hA, hB);
-            matte = occ.mprec && P.matte == RH_DEV_MATTE;
+            matte = FULL && occ.mprec && P.matte == RH_DEV_MATTE;
         }
         const float Ci[3] = {hA.x, hA.y, hA.z}, Oi[3] = {hB.x, hB.y, hB.z};
-        if (matte) {
+        if (FULL && matte) {
             // lerp(t, x0, 0) = (1 - t) x0 + t 0 (math.h:88-91), the colour
             // weighted by the opacity and the opacity by the colour
             for (int c = 0; c < 3; ++c) {
@@ -2525,7 +2527,7 @@ __device__ __forceinline__ void combine_sample(const FrameDev* __restrict__ fr, 
         }
     }
     float depth = opaqueDepth0;
-    const int df = fr->depthFilter;
+    const int df = FULL ? fr->depthFilter : 0;
     if (df == RH_DEV_DEPTH_MIDPOINT) {
         depth = n > 1 ? (opaqueDepth0 + opaqueDepth1) * 0.5f : FLT_MAX;
     } else if (df == RH_DEV_DEPTH_MAX) {
@@ -2557,6 +2559,7 @@ __device__ __forceinline__ void combine_sample(const FrameDev* __restrict__ fr, 
 }
 
 #define RH_COMBINE_WG 64
+template <bool FULL>
 __global__ void __launch_bounds__(RH_COMBINE_WG)
 rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
                                                   uint32_t* __restrict__ hitCounter,
@@ -2579,7 +2582,7 @@ rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
     }
     // the hit list and the sort's work stack in LDS (no scratch memory),
     // each lane's array an odd number of words from the next
-    combine_sample(fr, e, cmbD + threadIdx.x * (RH_MAX_HITS + 1), cmbX + threadIdx.x * (RH_MAX_HITS + 1),
+    combine_sample<FULL>(fr, e, cmbD + threadIdx.x * (RH_MAX_HITS + 1), cmbX + threadIdx.x * (RH_MAX_HITS + 1),
                    cmbS + threadIdx.x * 25, false, hitData, hitNext, prims, occ, samples, frontDepth);
 }
 
@@ -2589,6 +2592,7 @@ rh_combine(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
 // any length), one thread per listed sample.  workD and workX hold
 // hitCounter[3] entries, workStk 192 ints per deferred sample
 // (hitCounter[1] of them); hitCounter[4] and [5] hand them out.
+template <bool FULL>
 __global__ void __launch_bounds__(64)
 rh_combine_long(const FrameDev* __restrict__ fr, const uint4* __restrict__ list, uint32_t* __restrict__ hitCounter,
                 const float4* __restrict__ hitData, const uint32_t* __restrict__ hitNext,
@@ -2601,7 +2605,7 @@ rh_combine_long(const FrameDev* __restrict__ fr, const uint4* __restrict__ list,
     if (e.w < RH_MAX_HITS) return;
     const uint32_t off = atomicAdd(&hitCounter[4], e.w + 1u);
     const uint32_t k = atomicAdd(&hitCounter[5], 1u);
-    combine_sample(fr, e, workD + off, workX + off, workStk + (size_t)k * 192, true, hitData, hitNext, prims, occ,
+    combine_sample<FULL>(fr, e, workD + off, workX + off, workStk + (size_t)k * 192, true, hitData, hitNext, prims, occ,
                    samples, frontDepth);
 }
 
@@ -3149,7 +3153,9 @@ extern "C" hipError_t rh_launch_sample(hipStream_t st, int variant, int grid, co
 
 // The motion form of the sampler: one workgroup of RH_MB_WAVES waves per
 // task, tasks [a->itemBase, a->itemBase + ntasks).
+#ifndef RH_MB_WAVES
 #define RH_MB_WAVES 16
+#endif
 extern "C" hipError_t rh_launch_sample_mb(hipStream_t st, const SampleLaunch* a, int ntasks,
                                           const float4* mprecMB) {
     if (ntasks <= 0) return hipSuccess;
@@ -3207,12 +3213,16 @@ extern "C" hipError_t rh_launch_plan_items(hipStream_t st, const FrameDev* fr, c
 extern "C" hipError_t rh_launch_combine(hipStream_t st, const FrameDev* fr, size_t maxEntries, const uint4* list,
                                         uint32_t* hitCounter, const float4* hitData, const uint32_t* hitNext,
                                         const PrimDev* prims, const float* mprec, int recFloats, int gridField,
-                                        const GridDev* grids, float4* samples, float* frontDepth) {
+                                        const GridDev* grids, int full, float4* samples, float* frontDepth) {
     if (maxEntries == 0) return hipSuccess;
     const OcclRec occ = {mprec, recFloats, gridField, grids};
-    hipLaunchKernelGGL(rh_combine, dim3((unsigned)((maxEntries + RH_COMBINE_WG - 1) / RH_COMBINE_WG)),
-                       dim3(RH_COMBINE_WG), 0, st, fr, list,
-                       hitCounter, hitData, hitNext, prims, occ, samples, frontDepth);
+    const dim3 grid((unsigned)((maxEntries + RH_COMBINE_WG - 1) / RH_COMBINE_WG));
+    if (full)
+        hipLaunchKernelGGL(rh_combine<true>, grid, dim3(RH_COMBINE_WG), 0, st, fr, list, hitCounter, hitData, hitNext,
+                           prims, occ, samples, frontDepth);
+    else
+        hipLaunchKernelGGL(rh_combine<false>, grid, dim3(RH_COMBINE_WG), 0, st, fr, list, hitCounter, hitData, hitNext,
+                           prims, occ, samples, frontDepth);
     return hipGetLastError();
 }
 
@@ -3221,12 +3231,17 @@ extern "C" hipError_t rh_launch_combine(hipStream_t st, const FrameDev* fr, size
 extern "C" hipError_t rh_launch_combine_long(hipStream_t st, const FrameDev* fr, size_t maxEntries, const uint4* list,
                                              uint32_t* hitCounter, const float4* hitData, const uint32_t* hitNext,
                                              const PrimDev* prims, const float* mprec, int recFloats, int gridField,
-                                             const GridDev* grids, float4* samples, float* frontDepth, float* workD,
-                                             uint32_t* workX, int* workStk) {
+                                             const GridDev* grids, int full, float4* samples, float* frontDepth,
+                                             float* workD, uint32_t* workX, int* workStk) {
     if (maxEntries == 0) return hipSuccess;
     const OcclRec occ = {mprec, recFloats, gridField, grids};
-    hipLaunchKernelGGL(rh_combine_long, dim3((unsigned)((maxEntries + 63) / 64)), dim3(64), 0, st, fr, list,
-                       hitCounter, hitData, hitNext, prims, occ, samples, frontDepth, workD, workX, workStk);
+    const dim3 grid((unsigned)((maxEntries + 63) / 64));
+    if (full)
+        hipLaunchKernelGGL(rh_combine_long<true>, grid, dim3(64), 0, st, fr, list, hitCounter, hitData, hitNext, prims,
+                           occ, samples, frontDepth, workD, workX, workStk);
+    else
+        hipLaunchKernelGGL(rh_combine_long<false>, grid, dim3(64), 0, st, fr, list, hitCounter, hitData, hitNext, prims,
+                           occ, samples, frontDepth, workD, workX, workStk);
     return hipGetLastError();
 }
 
