@@ -5,10 +5,111 @@
 #include <algorithm>
 #include <functional>
 #include <atomic>
+#include <condition_variable>
+#include <mutex>
+#include <pthread.h>
 #include <random>
 #include <thread>
 
 namespace rh {
+
+// ---------------------------------------------------------------------------
+// Host worker pool
+// ---------------------------------------------------------------------------
+// A process-wide pool of worker threads, started on first use and kept until
+// the process ends (never destroyed: its threads wait on a condition).
+// Starting and joining threads per parallel loop cost tens of microseconds a
+// thread, which made up most of a small scene's schedule.  One loop runs on
+// the pool at a time (callers on other threads wait for it); the caller
+// takes part in its own loop.
+namespace {
+struct WorkPool {
+    std::mutex runMutex;                  // one loop at a time
+    std::mutex m;
+    std::condition_variable wake, finished;
+    std::vector<std::thread> workers;
+    const std::function<void(int)>* fn = nullptr;
+    std::atomic<int> next{0};
+    int count = 0;
+    int wanted = 0;                       // workers taking part in the current loop
+    int done = 0;                         // of those, the ones finished
+    uint64_t gen = 0;
+
+    void loop(int id) {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(m);
+        for (;;) {
+            wake.wait(lk, [&] { return gen != seen; });
+            seen = gen;
+            if (id >= wanted) continue;
+            const std::function<void(int)>* f = fn;
+            const int cnt = count;
+            lk.unlock();
+            for (;;) {
+                const int i = next.fetch_add(1);
+                if (i >= cnt) break;
+                (*f)(i);
+            }
+            lk.lock();
+            if (++done == wanted) finished.notify_all();
+        }
+    }
+
+    void run(int n, int threads, const std::function<void(int)>& f) {
+        std::lock_guard<std::mutex> one(runMutex);
+        const int helpers = std::min(threads, n) - 1;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            while ((int)workers.size() < helpers) {
+                const int id = (int)workers.size();
+                workers.emplace_back([this, id] { loop(id); });
+                workers.back().detach();
+            }
+            fn = &f;
+            count = n;
+            next = 0;
+            wanted = helpers;
+            done = 0;
+            ++gen;
+        }
+        wake.notify_all();
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= n) break;
+            f(i);
+        }
+        std::unique_lock<std::mutex> lk(m);
+        finished.wait(lk, [&] { return done == wanted; });
+        fn = nullptr;
+    }
+};
+
+WorkPool* g_pool = nullptr;
+
+// A forked child has none of the parent's workers: it starts a pool of its own.
+void pool_after_fork() { g_pool = new WorkPool(); }
+
+WorkPool& work_pool() {
+    static const bool registered = [] {
+        g_pool = new WorkPool();              // kept until the process ends
+        pthread_atfork(nullptr, nullptr, pool_after_fork);
+        return true;
+    }();
+    (void)registered;
+    return *g_pool;
+}
+}  // namespace
+
+// Run fn(i) for i in [0, n) on up to `threads` threads (the caller and the
+// pool's workers), in no particular order.
+static void run_parallel(int n, int threads, const std::function<void(int)>& fn) {
+    if (n <= 0) return;
+    if (std::min(threads, n) <= 1) {
+        for (int i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    work_pool().run(n, threads, fn);
+}
 
 // ---------------------------------------------------------------------------
 // Frame shape
@@ -1000,12 +1101,8 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
     std::vector<uint32_t> leafCount(n, 0);
     f.trees.assign(n, Tree());
     f.threads = std::max(1, threads);
-    int nt = std::max(1, std::min(threads, n));
-    std::atomic<int> next(0);
-    auto work = [&]() {
-        for (;;) {
-            int i = next.fetch_add(1);
-            if (i >= n) break;
+    auto work = [&](int i) {
+        {
             build_tree(c, i, f.trees[i]);
             std::vector<Node>& t = f.trees[i].n;
             // the least minimum depth of each node's subtree (children follow
@@ -1028,12 +1125,7 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
             leafCount[i] = nl;
         }
     };
-    if (nt == 1) work();
-    else {
-        std::vector<std::thread> pool;
-        for (int k = 0; k < nt; ++k) pool.emplace_back(work);
-        for (auto& th : pool) th.join();
-    }
+    run_parallel(n, threads, work);
     f.nodeBase.assign(n + 1, 0);
     for (int i = 0; i < n; ++i) f.nodeBase[i + 1] = f.nodeBase[i] + f.trees[i].n.size();
     if (f.nodeBase[n] >= 0xffffffffull) {
@@ -1060,11 +1152,8 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
     f.params.reset(new float[std::max<uint64_t>(f.nparams, 1)]);
     // the bucket loop's packed nodes
     f.sim.assign(f.nodeBase[n], SimNode());
-    next = 0;
-    auto pack = [&]() {
-        for (;;) {
-            int i = next.fetch_add(1);
-            if (i >= n) break;
+    auto pack = [&](int i) {
+        {
             const std::vector<Node>& t = f.trees[i].n;
             uint64_t po = paramBase[i], pq = pairBase[i];
             for (size_t j = 0; j < t.size(); ++j) {
@@ -1095,12 +1184,7 @@ bool build_forest(const Shape& s, const rh_prim* prims, int n, ForestPtr& out, s
             }
         }
     };
-    if (nt == 1) pack();
-    else {
-        std::vector<std::thread> pool;
-        for (int k = 0; k < nt; ++k) pool.emplace_back(pack);
-        for (auto& th : pool) th.join();
-    }
+    run_parallel(n, threads, pack);
     // the caller's shape and surfaces are passed again to simulate
     c.s = nullptr;
     c.prims = nullptr;
@@ -1176,22 +1260,7 @@ uint64_t cull_trees(const Shape& s, std::vector<CullTreeDev>& out) {
 
 // Run fn(i) for i in [0, n) on up to `threads` threads.
 template <typename Fn> static void parallel_for(int n, int threads, Fn fn) {
-    const int nt = std::max(1, std::min(threads, n));
-    if (nt <= 1) {
-        for (int i = 0; i < n; ++i) fn(i);
-        return;
-    }
-    std::atomic<int> next(0);
-    std::vector<std::thread> pool;
-    for (int k = 0; k < nt; ++k)
-        pool.emplace_back([&]() {
-            for (;;) {
-                const int i = next.fetch_add(1);
-                if (i >= n) break;
-                fn(i);
-            }
-        });
-    for (auto& th : pool) th.join();
+    run_parallel(n, threads, std::function<void(int)>(fn));
 }
 
 static void finish_schedule(const Shape& s, const Forest& f, Schedule& out);
