@@ -18,6 +18,10 @@ Other workloads: "clustered" (2048x2048, micropolygons per bucket log-normal),
 0-1), "spheres" (256x256, 64 spheres) and "serving" (a Poisson request
 stream).
 
+The default run (the HD workload) adds short runs of the motion, clustered
+and serving configurations in "other_workloads" (--no-workloads or
+--no-targets skips them).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hd|clustered|motion|spheres|serving]
 """
 import argparse
@@ -343,6 +347,42 @@ def serving(args):
     }
 
 
+def other_workloads(args):
+    """The other configurations, short, beside the default line: the motion
+    and clustered workloads (first frame with the host schedule, then device
+    frames on resident inputs in the batched form) and a 200-request serving
+    stream at the default rate."""
+    from aqsis_amd.hider import Hider
+    res = {}
+    for name in ("motion", "clustered"):
+        opts, prims, desc = workload(name)
+        with Hider(opts) as h:
+            h.post_surfaces(prims)
+            t0 = time.perf_counter()
+            h.render()
+            first = (time.perf_counter() - t0) * 1e3
+            h.set_tuning(mode="batched")
+            for _ in range(2):
+                h.render_device()
+            n = 5
+            t0 = time.perf_counter()
+            for _ in range(n):
+                h.render_device()
+            wall = time.perf_counter() - t0
+            st = h.stats()
+        res[name] = {"config": desc["workload"], "ms_per_frame": wall * 1e3 / n, "frames_per_s": n / wall,
+                     "sample_tests_per_s": st["spl_count"] * n / wall, "spl_count": st["spl_count"],
+                     "first_frame_ms_with_schedule": first, "mode": "batched (216-record chunks, 4 blocks a wave)"}
+    sargs = argparse.Namespace(**vars(args))
+    sargs.requests = 200
+    sv = serving(sargs)
+    res["serving"] = {"config": sv["config"]["workload"], "requests": 200, "rate_hz": sargs.rate,
+                      "frames_per_s": sv["frames_per_s"], "latency_ms_p50": sv["latency_ms_p50"],
+                      "latency_ms_p99": sv["latency_ms_p99"], "sample_tests_per_s": sv["value"],
+                      "failed": sv["failed"]}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,7 +400,10 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="serving: jobs in flight")
     ap.add_argument("--budget-gb", type=int, default=160, help="serving: device memory budget")
     ap.add_argument("--e2e-steps", type=int, default=5, help="new-scene frames timed end to end")
-    ap.add_argument("--no-targets", action="store_true", help="skip the CPU-runnable configuration's target line")
+    ap.add_argument("--no-targets", action="store_true",
+                    help="skip the CPU-runnable configuration's target line (and the other configurations' runs)")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the other configurations' short runs beside the default workload")
     ap.add_argument("--copy-kernel", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -513,6 +556,8 @@ def main():
         if "speedup_vs_plain_form_frames" in out:
             t["batched_vs_plain_frames"] = out["speedup_vs_plain_form_frames"]
         out["targets"] = t
+    if args.workload == "hd" and not args.no_workloads and not args.no_targets:
+        out["other_workloads"] = other_workloads(args)
     print(json.dumps(out))
 
 
