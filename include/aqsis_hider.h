@@ -376,7 +376,10 @@ rh_status rh_tiff_write_typed(const char* path, const void* pixels, int32_t widt
  * (x, y) at index (y*ext_width + x)*samples_per_pixel + s:
  *   mp_id[]  id of the front-most micropolygon hit (0xffffffff: none);
  *            ids count busted micropolygons in dicing order;
- *   depth[]  camera-space depth of that hit (FLT_MAX: none);
+ *   depth[]  camera-space depth of that hit (FLT_MAX: none); for a sample
+ *            without transparent hits under the midpoint depth filter,
+ *            the combine's depth (the mean of that hit's depth and the
+ *            occluding depth, imagepixel.cpp:319-326);
  *   rgbaz[]  7 floats per sample after CqImagePixel::Combine
  *            (imagepixel.cpp:144-332): Ci.rgb, Oi.rgb, z.
  * Any pointer may be NULL. */

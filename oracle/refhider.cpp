@@ -2073,8 +2073,10 @@ int oracle_render(const rh_options* opt, const rh_prim* prims, int32_t nprims, i
         bool valid = (s.occl.flags & Flag_Valid) != 0;
         if (mp_id) mp_id[i] = valid ? s.occl.id : 0xffffffffu;
         if (depth) {
-            // depth of the front-most hit: the occluding hit's own depth for
-            // opaque-only samples, the first sorted hit otherwise.
+            // depth of the front-most hit: the occluding hit's depth after
+            // the combine for opaque-only samples (its own, or under the
+            // midpoint filter the mean with the occluding depth), the first
+            // sorted hit otherwise.
             float z = FLT_MAX;
             if (valid) z = s.hits.empty() ? s.occl.d[6] : s.hits.front().d[6];
             depth[i] = z;
