@@ -12,7 +12,10 @@
 namespace rh {
 
 int display_type(const rh_options& o) {
-    const float one = o.quantize_one, mn = o.quantize_min, mx = o.quantize_max;
+    return select_data_format(o.quantize_one, o.quantize_min, o.quantize_max);
+}
+
+int select_data_format(float one, float mn, float mx) {
     if (one == 0) return RH_DSPY_FLOAT32;
     if (mn >= 0) {
         if (mx <= 255) return RH_DSPY_UNSIGNED8;
@@ -43,14 +46,14 @@ int32_t to_i32(double v) {
 }
 }  // namespace
 
-void format_pixel(const rh_options& o, const float* ch, double s, int type, unsigned char* dst) {
-    const float vals[4] = {ch[0], ch[1], ch[2], ch[6]};
+// One value: quantised with q = (one, min, max, dither) and draw s when one
+// != 0, stored as `type`; returns the next byte.
+static unsigned char* put_value(const float* q, double value, double s, int type, unsigned char* dst) {
     const float zero = 0.0f;
-    for (int k = 0; k < 4; ++k) {
-        double value = vals[k];
-        if (o.quantize_one != 0) {
-            value = (double)round_l(zero + value * (double)(o.quantize_one - zero) + ((double)o.quantize_dither * s));
-            value = clampv<double>(value, o.quantize_min, o.quantize_max);
+    {
+        if (q[0] != 0) {
+            value = (double)round_l(zero + value * (double)(q[0] - zero) + ((double)q[3] * s));
+            value = clampv<double>(value, q[1], q[2]);
         }
         switch (type) {
         case RH_DSPY_FLOAT32: {
@@ -87,6 +90,27 @@ void format_pixel(const rh_options& o, const float* ch, double s, int type, unsi
         }
         }
     }
+    return dst;
+}
+
+void format_pixel(const rh_options& o, const float* ch, double s, int type, unsigned char* dst) {
+    const float q[4] = {o.quantize_one, o.quantize_min, o.quantize_max, o.quantize_dither};
+    const float vals[4] = {ch[0], ch[1], ch[2], ch[6]};
+    for (int k = 0; k < 4; ++k) dst = put_value(q, vals[k], s, type, dst);
+}
+
+int mode_channels(int mode) {
+    if (mode <= 0 || mode > (RH_MODE_RGB | RH_MODE_A | RH_MODE_Z)) return 0;
+    return ((mode & RH_MODE_A) ? 1 : 0) + ((mode & RH_MODE_RGB) ? 3 : 0) + ((mode & RH_MODE_Z) ? 1 : 0);
+}
+
+int format_request(const float* q, int mode, const float* ch, double s, int type, unsigned char* dst) {
+    unsigned char* p = dst;
+    if (mode & RH_MODE_A) p = put_value(q, ch[6], s, type, p);
+    if (mode & RH_MODE_RGB)
+        for (int k = 0; k < 3; ++k) p = put_value(q, ch[k], s, type, p);
+    if (mode & RH_MODE_Z) p = put_value(q, ch[7], s, type, p);
+    return (int)(p - dst);
 }
 
 }  // namespace rh

@@ -11,6 +11,8 @@ namespace rh {
 // quantisation (one == 0), else the smallest unsigned (min >= 0) or signed
 // integer type whose range holds [min, max].
 int display_type(const rh_options& o);
+// selectDataFormat of explicit Quantize settings (one, min, max).
+int select_data_format(float one, float mn, float mx);
 
 // Bytes of one value of a RH_DSPY_* type; 0 for a type that is not one.
 int display_type_bytes(int type);
@@ -20,5 +22,15 @@ int display_type_bytes(int type);
 // channels), quantised as lround(value * one + dither * s) clamped to
 // [min, max] when one != 0, and stored as `type` at dst (4 values).
 void format_pixel(const rh_options& o, const float* ch, double s, int type, unsigned char* dst);
+
+// A display request's values of one pixel: mode bits RH_MODE_* in the order
+// the request builds its formats (ddmanager.cpp:456-483): a (channel 6),
+// then r, g, b (0..2), then z (the filtered depth, channel 7); quantised
+// with q = (one, min, max, dither) as format_pixel does.  Returns the bytes
+// written.
+int format_request(const float* q, int mode, const float* ch, double s, int type, unsigned char* dst);
+
+// Values per pixel of a request's mode (0: not a mode).
+int mode_channels(int mode);
 
 }  // namespace rh

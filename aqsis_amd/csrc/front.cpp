@@ -483,6 +483,8 @@ void make_plan(const Shape& s, Plan& p) {
     p.pixPattern.assign((size_t)s.extW * s.extH * 4, 0);
     p.pixDof.assign((size_t)s.extW * s.extH, 0);
     p.dither.assign((size_t)s.width() * s.height(), 0.0f);
+    const size_t npix = (size_t)s.width() * s.height();
+    p.ditherAll.assign(npix * (size_t)std::max(1, s.o.num_displays), 0.0f);
     for (const auto& b : s.buckets) {
         if (s.o.jitter) {
             for (int y = b.sy0; y < b.sy1; ++y)
@@ -503,8 +505,11 @@ void make_plan(const Shape& s, Plan& p) {
             for (int y = b.y; y < b.y + b.h; ++y)
                 for (int x = b.x; x < b.x + b.w; ++x) {
                     float v = st.f();
-                    if (d == 0 && x >= s.cropX0 && x < s.cropX1 && y >= s.cropY0 && y < s.cropY1)
-                        p.dither[(size_t)(y - s.cropY0) * s.width() + (x - s.cropX0)] = v;
+                    if (x >= s.cropX0 && x < s.cropX1 && y >= s.cropY0 && y < s.cropY1) {
+                        const size_t i = (size_t)(y - s.cropY0) * s.width() + (x - s.cropX0);
+                        p.ditherAll[(size_t)d * npix + i] = v;
+                        if (d == 0) p.dither[i] = v;
+                    }
                 }
     }
 }

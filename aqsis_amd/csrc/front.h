@@ -74,6 +74,8 @@ struct Plan {
     std::vector<float> pos1d;             // cache * ns
     std::vector<uint8_t> pixPattern;      // ext pixels * 4: position, time, lod, shuffle
     std::vector<float> dither;            // display pixels: display 0's RandomFloat()
+    // every display request's draws: request d's at [d * display pixels, ...)
+    std::vector<float> ditherAll;
     // depth of field
     std::vector<uint8_t> shuffled;        // cache * ns: getShuffledIndices tables
     std::vector<float> lens;              // cache * ns * 2: projectToCircle(2 * pos2d - 1)

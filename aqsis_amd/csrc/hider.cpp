@@ -273,7 +273,7 @@ PlanCache& plan_cache() {
 constexpr size_t kPlanCacheBytes = size_t(2) << 30;
 
 size_t plan_bytes(const rh::Plan& p) {
-    return (p.pos2d.size() + p.pos1d.size() + p.dither.size() + p.lens.size()) * sizeof(float) +
+    return (p.pos2d.size() + p.pos1d.size() + p.dither.size() + p.ditherAll.size() + p.lens.size()) * sizeof(float) +
            p.pixPattern.size() + p.pixDof.size() + p.shuffled.size();
 }
 
@@ -1285,6 +1285,32 @@ static rh_status display_values(rh_ctx* c, int32_t type, std::vector<unsigned ch
     HIP_TRY(hipMemcpy(ch.data(), c->dChannels.p, ch.size() * sizeof(float), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < npix; ++i)
         rh::format_pixel(s.o, &ch[i * 9], (double)c->plan.dither[i], type, &out[i * 4 * vb]);
+    return RH_OK;
+}
+
+rh_status rh_select_format(float one, float min, float max, int32_t* type) {
+    if (!type) return fail(RH_ERR_INVALID, "null type");
+    *type = rh::select_data_format(one, min, max);
+    return RH_OK;
+}
+
+rh_status rh_read_display_request(rh_ctx* c, int32_t mode, const float* quantize, int32_t index, int32_t type,
+                                  void* out) {
+    if (!c || !out || !quantize) return fail(RH_ERR_INVALID, "bad arguments");
+    if (!c->rendered) return fail(RH_ERR_STATE, "no rendered frame");
+    const rh::Shape& s = c->shape;
+    const int nv = rh::mode_channels(mode);
+    if (nv == 0) return fail(RH_ERR_INVALID, "mode must combine RH_MODE_RGB, RH_MODE_A and RH_MODE_Z");
+    if (index < 0 || index >= std::max(1, s.o.num_displays))
+        return fail(RH_ERR_INVALID, "display index outside the frame's display requests");
+    const int vb = rh::display_type_bytes(type);
+    if (vb == 0) return fail(RH_ERR_INVALID, "unknown display value type " + std::to_string(type));
+    const size_t npix = (size_t)s.width() * s.height();
+    std::vector<float> ch(npix * 9);
+    HIP_TRY(hipMemcpy(ch.data(), c->dChannels.p, ch.size() * sizeof(float), hipMemcpyDeviceToHost));
+    const float* d = &c->plan.ditherAll[(size_t)index * npix];
+    unsigned char* dst = static_cast<unsigned char*>(out);
+    for (size_t i = 0; i < npix; ++i) dst += rh::format_request(quantize, mode, &ch[i * 9], (double)d[i], type, dst);
     return RH_OK;
 }
 

@@ -40,9 +40,26 @@ class RhTuning(C.Structure):
 
 MODES = {"grid": 0, "plain": 1, "batched": 2}
 
+# display request modes (include/aqsis_hider.h RH_MODE_*)
+MODE_RGB, MODE_A, MODE_Z = 1, 2, 4
+
+
+def mode_bits(mode):
+    """RH_MODE_* bits of a RiDisplay mode string: "rgb", then "a", then "z",
+    each optional and matched as a prefix in that order (ri.cpp:947-964)."""
+    bits, i = 0, 0
+    if mode.startswith("rgb"):
+        bits, i = MODE_RGB, 3
+    if mode.startswith("a", i):
+        bits, i = bits | MODE_A, i + 1
+    if mode.startswith("z", i):
+        bits |= MODE_Z
+    return bits
+
 EXPORTS = ["rh_options_default", "rh_create", "rh_post_surfaces", "rh_clear_surfaces", "rh_render",
            "rh_render_device", "rh_render_device_noalloc", "rh_render_stages", "rh_frame_size", "rh_read_channels", "rh_read_rgba8", "rh_display", "rh_tiff_write",
            "rh_display_format", "rh_read_display", "rh_format_display", "rh_display_as", "rh_tiff_write_typed",
+           "rh_select_format", "rh_read_display_request",
            "rh_read_samples",
            "rh_get_stats", "rh_destroy", "rh_last_error", "rh_estimate", "rh_sample_positions",
            "rh_set_tuning", "rh_get_tuning", "rh_autotune"]
@@ -85,6 +102,8 @@ def lib():
         L.rh_read_display.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
         L.rh_format_display.argtypes = [P(RhOptions), C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]
         L.rh_display_as.argtypes = [C.c_void_p, DSPY_DATA_FN, C.c_void_p, C.c_int32, C.c_int32]
+        L.rh_select_format.argtypes = [C.c_float, C.c_float, C.c_float, P(C.c_int32)]
+        L.rh_read_display_request.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
         L.rh_tiff_write_typed.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_int32, C.c_void_p, C.c_void_p, C.c_char_p]
         L.rh_get_stats.argtypes = [C.c_void_p, P(RhStats)]
@@ -175,6 +194,27 @@ class Hider:
         t = self.display_type if type is None else (DSPY[type] if isinstance(type, str) else int(type))
         out = np.zeros((self.height, self.width, 4), DSPY_DTYPE[t])
         _check(lib().rh_read_display(self._ctx, t, _ptr(out)))
+        return out
+
+    def display_request_values(self, mode, quantize=(0.0, 0.0, 0.0, 0.0), index=0, type=None):
+        """A display request's values (height, width, k) of mode `mode`
+        (RH_MODE_* bits or a mode string such as "z" or "rgbaz"), channels in
+        the request's format order a, r, g, b, z, quantised with `quantize`
+        (one, min, max, dither: the request's settings, Quantize "z" for a z
+        mode; the default is Quantize "z"'s default, floats) and dither draws
+        of display request `index` (rh_read_display_request); `type`
+        defaults to the one selectDataFormat picks for the settings."""
+        m = mode_bits(mode) if isinstance(mode, str) else int(mode)
+        q = np.ascontiguousarray(quantize, np.float32)
+        if type is None:
+            t = C.c_int32()
+            _check(lib().rh_select_format(float(q[0]), float(q[1]), float(q[2]), C.byref(t)))
+            t = t.value
+        else:
+            t = DSPY[type] if isinstance(type, str) else int(type)
+        k = (1 if m & MODE_A else 0) + (3 if m & MODE_RGB else 0) + (1 if m & MODE_Z else 0)
+        out = np.zeros((self.height, self.width, max(k, 1)), DSPY_DTYPE.get(t, np.uint8))
+        _check(lib().rh_read_display_request(self._ctx, m, _ptr(q), index, t, _ptr(out)))
         return out
 
     def display(self, data, scanline_order=False, type=None):

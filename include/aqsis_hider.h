@@ -317,6 +317,29 @@ rh_status rh_read_display(rh_ctx* ctx, int32_t type, void* out);
 rh_status rh_format_display(const rh_options* opt, const float* channels, const float* dither, int64_t npix,
                             int32_t type, void* out);
 
+/* A display request's mode (RiDisplay's mode string, ri.cpp:947-964):
+ * "rgb" RH_MODE_RGB, "a" RH_MODE_A, "z" RH_MODE_Z, combined as "rgbaz". */
+enum { RH_MODE_RGB = 1, RH_MODE_A = 2, RH_MODE_Z = 4 };
+
+/* selectDataFormat (ddmanager.cpp:249-283) of a display request's own
+ * Quantize settings: *type as rh_display_format's. */
+rh_status rh_select_format(float one, float min, float max, int32_t* type);
+
+/* The rendered frame as display request `index` (its place among the
+ * num_displays requests, whose dither draws it takes) of mode `mode`
+ * receives it (CqDisplayRequest::FormatBucketForDisplay,
+ * ddmanager.cpp:1022-1118): per pixel, the request's channels in the order
+ * it builds its formats (ddmanager.cpp:456-483): a, then r, g, b, then z
+ * (the filtered depth), quantised with quantize = (one, min, max, dither)
+ * as rh_read_display does, stored as `type`; width x height pixels row by
+ * row.  The settings a request takes (ddmanager.cpp:429-455): its RiDisplay
+ * "quantize"/"dither" parameters, else Quantize "z" when its mode has z,
+ * else Quantize "rgba" (rh_options' quantize_*).  No counterpart of its own
+ * in the reference beyond the data path it restates; for depth images
+ * (mode "z") and the z channel of "rgbaz" requests. */
+rh_status rh_read_display_request(rh_ctx* ctx, int32_t mode, const float* quantize, int32_t index, int32_t type,
+                                  void* out);
+
 /* A display driver's image-data entry point, with DspyImageData's argument
  * meaning (include/aqsis/ri/ndspy.h:158): the rectangle [xmin, xmaxplus1) x
  * [ymin, ymaxplus1), the bytes of one pixel, and the pixels row by row.  A

@@ -79,13 +79,15 @@ def render(opts, prims, occlusion_cull=False, samples=True):
             "stats": st.as_dict(), "ext": (ew, eh, spp)}
 
 
-def sample_positions(opts):
+def sample_positions(opts, display=0):
+    """The sample positions (eh, ew, spp, 2) and the (h, w) dither draws of
+    display request `display`."""
     w, h, ew, eh, spp = frame_size(opts)
     L = lib()
-    L.oracle_sample_positions.argtypes = [C.POINTER(RhOptions), C.c_void_p, C.c_void_p]
+    L.oracle_sample_positions_of.argtypes = [C.POINTER(RhOptions), C.c_int32, C.c_void_p, C.c_void_p]
     xy = np.zeros((eh, ew, spp, 2), np.float32)
     d = np.zeros((h, w), np.float32)
-    L.oracle_sample_positions(C.byref(opts), _ptr(xy), _ptr(d))
+    L.oracle_sample_positions_of(C.byref(opts), display, _ptr(xy), _ptr(d))
     return xy, d
 
 
@@ -149,6 +151,26 @@ def display_values(opts, channels, dither, type=None):
     assert d.shape == (h, w)
     out = np.zeros((h, w, 4), DISPLAY_DTYPE[t])
     L.oracle_display_values(C.byref(opts), _ptr(ch), _ptr(d), h * w, t, _ptr(out))
+    return out
+
+
+MODE_CHANNELS = {1: 3, 2: 1, 4: 1}
+
+
+def display_request_values(quantize, mode, channels, dither, type):
+    """A display request's (h, w, k) values of PkDspy* `type`: mode bits (1
+    rgb, 2 a, 4 z) in the request's format order a, r, g, b, z, quantised
+    with (one, min, max, dither)."""
+    L = lib()
+    L.oracle_display_request_values.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
+                                                C.c_int32, C.c_void_p]
+    q = np.ascontiguousarray(quantize, np.float32)
+    ch = np.ascontiguousarray(channels, np.float32)
+    d = np.ascontiguousarray(dither, np.float32)
+    h, w = ch.shape[:2]
+    k = sum(n for bit, n in MODE_CHANNELS.items() if mode & bit)
+    out = np.zeros((h, w, k), DISPLAY_DTYPE[type])
+    L.oracle_display_request_values(_ptr(q), mode, _ptr(ch), _ptr(d), h * w, type, _ptr(out))
     return out
 
 

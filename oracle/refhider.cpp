@@ -941,6 +941,7 @@ struct Hider {
     std::vector<float> channels;  // display region, 9 per pixel
     std::vector<uint8_t> rgba8;   // display region
     std::vector<float> ditherRec; // display 0's dither draw per display pixel
+    int ditherIndex = 0;          // the display whose draws ditherRec records
     uint32_t mpCounter = 0;
     rh_stats st;
     // current bucket state
@@ -1931,9 +1932,9 @@ struct Hider {
             for (int y = b.yPos; y < b.yPos + b.ySize; ++y)
                 for (int x = b.xPos; x < b.xPos + b.xSize; ++x) {
                     double s = rng.rfloat();
-                    if (d != 0) continue;
                     if (x < F.cropXMin || x >= F.cropXMax || y < F.cropYMin || y >= F.cropYMax) continue;
-                    ditherRec[(size_t)(y - F.cropYMin) * W + (x - F.cropXMin)] = (float)s;
+                    if (d == ditherIndex) ditherRec[(size_t)(y - F.cropYMin) * W + (x - F.cropXMin)] = (float)s;
+                    if (d != 0) continue;
                     const float* ch = &channels[((size_t)(y - F.cropYMin) * W + (x - F.cropXMin)) * 9];
                     const float vals[4] = {ch[0], ch[1], ch[2], ch[6]};
                     uint8_t* out = &rgba8[((size_t)(y - F.cropYMin) * W + (x - F.cropXMin)) * 4];
@@ -2117,8 +2118,17 @@ int oracle_dice(const rh_prim* p, int32_t u, int32_t v, float* out) {
     return 0;
 }
 
+int oracle_sample_positions_of(const rh_options* opt, int32_t index, float* xy, float* dither);
 int oracle_sample_positions(const rh_options* opt, float* xy, float* dither) {
+    return oracle_sample_positions_of(opt, 0, xy, dither);
+}
+
+// The same with the dither draws of display request `index` (one
+// RandomFloat per display pixel per request, the requests in order within
+// each bucket: ddmanager.cpp:146-171, 1050).
+int oracle_sample_positions_of(const rh_options* opt, int32_t index, float* xy, float* dither) {
     std::unique_ptr<Hider> H(new Hider());
+    H->ditherIndex = index;
     memset(&H->st, 0, sizeof(H->st));
     setup_frame(*opt, H->F);
     Frame& F = H->F;
@@ -2311,24 +2321,16 @@ int32_t oracle_display_type(const rh_options* o) {
     return 3;
 }
 
-// FormatBucketForDisplay's per-value path (ddmanager.cpp:1045-1118) over
-// npix pixels: r, g, b = channels 0..2 and a = channel 6 of each pixel's
-// 9, quantised with the pixel's dither draw s and stored as `type`, with
-// the reference's own static_casts (x86-64 g++ code, as its build).
-void oracle_display_values(const rh_options* o, const float* channels, const float* dither, int64_t npix,
-                           int32_t type, void* out) {
-    unsigned char* pdata = static_cast<unsigned char*>(out);
-    const float zero = 0.0f;
-    for (int64_t i = 0; i < npix; ++i) {
-        const double s = dither[i];
-        const float* ch = channels + i * 9;
-        const float vals[4] = {ch[0], ch[1], ch[2], ch[6]};
-        for (int k = 0; k < 4; ++k) {
-            double value = vals[k];
-            if (o->quantize_one != 0) {
+// FormatBucketForDisplay's per-value path (ddmanager.cpp:1045-1118) for one
+// value: quantised with (one, min, max, dither) q and draw s when one != 0,
+// stored as `type` with the reference's own static_casts (x86-64 g++ code,
+// as its build); returns the next byte.
+static unsigned char* oracle_put_value(const float* q, double value, double s, int32_t type, unsigned char* pdata) {
+            const float zero = 0.0f;
+            if (q[0] != 0) {
                 // Aqsis::lround (floor(x - 0.5) + 1), not the C library's
-                value = (double)orc::lround(zero + value * (double)(o->quantize_one - zero) + ((double)o->quantize_dither * s));
-                value = tclamp<double>(value, o->quantize_min, o->quantize_max);
+                value = (double)orc::lround(zero + value * (double)(q[0] - zero) + ((double)q[3] * s));
+                value = tclamp<double>(value, q[1], q[2]);
             }
             switch (type) {
             case 1: { float v = (float)value; memcpy(pdata, &v, 4); pdata += 4; break; }
@@ -2348,7 +2350,34 @@ void oracle_display_values(const rh_options* o, const float* channels, const flo
             case 6: { uint8_t v = static_cast<uint8_t>(value); *pdata++ = v; break; }
             default: { int8_t v = static_cast<int8_t>(value); memcpy(pdata, &v, 1); pdata += 1; break; }
             }
-        }
+            return pdata;
+}
+
+// FormatBucketForDisplay over npix pixels: r, g, b = channels 0..2 and a =
+// channel 6 of each pixel's 9, with the options' Quantize "rgba" settings.
+void oracle_display_values(const rh_options* o, const float* channels, const float* dither, int64_t npix,
+                           int32_t type, void* out) {
+    unsigned char* pdata = static_cast<unsigned char*>(out);
+    const float q[4] = {o->quantize_one, o->quantize_min, o->quantize_max, o->quantize_dither};
+    for (int64_t i = 0; i < npix; ++i) {
+        const float* ch = channels + i * 9;
+        const float vals[4] = {ch[0], ch[1], ch[2], ch[6]};
+        for (int k = 0; k < 4; ++k) pdata = oracle_put_value(q, vals[k], dither[i], type, pdata);
+    }
+}
+
+// A display request of mode bits (1 rgb, 2 a, 4 z) with Quantize settings
+// q: its channels in the order it builds its formats (ddmanager.cpp:456-483):
+// a (channel 6), then r, g, b (0..2), then z (the filtered depth, channel 7).
+void oracle_display_request_values(const float* q, int32_t mode, const float* channels, const float* dither,
+                                   int64_t npix, int32_t type, void* out) {
+    unsigned char* pdata = static_cast<unsigned char*>(out);
+    for (int64_t i = 0; i < npix; ++i) {
+        const float* ch = channels + i * 9;
+        if (mode & 2) pdata = oracle_put_value(q, ch[6], dither[i], type, pdata);
+        if (mode & 1)
+            for (int k = 0; k < 3; ++k) pdata = oracle_put_value(q, ch[k], dither[i], type, pdata);
+        if (mode & 4) pdata = oracle_put_value(q, ch[7], dither[i], type, pdata);
     }
 }
 

@@ -227,3 +227,57 @@ def test_typed_tiff(tmp_path, compression, name, img):
 def test_typed_tiff_refuses_signed_types(tmp_path):
     with pytest.raises(HiderError):
         write_tiff(tmp_path / "x.tif", np.zeros((2, 2, 4), np.int16))
+
+
+# --- display requests of any mode (rh_read_display_request) ----------------
+
+def test_request_values_reorder_the_rgba_path(oracle):
+    """A request of mode "rgba" builds its formats a, r, g, b
+    (ddmanager.cpp:456-472): the same values as the rgba path, alpha first;
+    its z channel is the filtered depth, unquantised as a float without
+    Quantize "z" (its default, 0 0 0 0), else through the same formula."""
+    rng = np.random.default_rng(5)
+    ch = rng.uniform(-0.2, 1.4, (6, 7, 9)).astype(np.float32)
+    ch[..., 7] = rng.uniform(1.0, 300.0, (6, 7)).astype(np.float32)
+    d = rng.uniform(0.0, 1.0, (6, 7)).astype(np.float32)
+    for q, t in (((255.0, 0.0, 255.0, 0.5), 6), ((65535.0, 0.0, 65535.0, 1.0), 4), ((0.0, 0.0, 0.0, 0.0), 1)):
+        o = _opts(*q)
+        rgba = oracle.display_values(o, ch, d, t)
+        req = oracle.display_request_values(q, 3, ch, d, t)
+        assert np.array_equal(req[..., 0], rgba[..., 3])
+        assert np.array_equal(req[..., 1:4], rgba[..., 0:3])
+        # z through the rgba path: the depth put in the red channel
+        chz = ch.copy()
+        chz[..., 0] = ch[..., 7]
+        z = oracle.display_request_values(q, 4, ch, d, t)
+        assert z.shape[-1] == 1 and np.array_equal(z[..., 0], oracle.display_values(o, chz, d, t)[..., 0])
+        full = oracle.display_request_values(q, 7, ch, d, t)
+        assert np.array_equal(full[..., :4], req) and np.array_equal(full[..., 4], z[..., 0])
+    zf = oracle.display_request_values((0.0, 0.0, 0.0, 0.0), 4, ch, d, 1)
+    assert np.array_equal(zf[..., 0], ch[..., 7])
+
+
+def test_each_display_request_takes_its_own_draws(oracle):
+    """Within a bucket the requests draw in turn, one RandomFloat per pixel
+    each (ddmanager.cpp:146-171, 1050): without jitter (no pattern draws
+    between buckets), request 1 of a one-bucket frame draws what request 0
+    of a two-bucket frame draws for its second bucket."""
+    one = default_options(xres=16, yres=16, jitter=0, num_displays=2)
+    two = default_options(xres=32, yres=16, jitter=0, num_displays=1)
+    _, d0 = oracle.sample_positions(one, display=0)
+    _, d1 = oracle.sample_positions(one, display=1)
+    _, e = oracle.sample_positions(two, display=0)
+    assert np.array_equal(d0, e[:, :16])
+    assert np.array_equal(d1, e[:, 16:])
+    assert not np.array_equal(d0, d1)
+
+
+def test_select_format_matches_the_options_form():
+    import ctypes as C
+    from aqsis_amd.hider import lib
+    for q, name in QUANTIZE.values():
+        o = _opts(*q)
+        a, nb, b = C.c_int32(), C.c_int32(), C.c_int32()
+        assert lib().rh_display_format(C.byref(o), C.byref(a), C.byref(nb)) == 0
+        assert lib().rh_select_format(q[0], q[1], q[2], C.byref(b)) == 0
+        assert a.value == b.value == DSPY[name]

@@ -145,3 +145,42 @@ def test_rib_file_display_of_a_16_bit_frame(tmp_path):
         img = h.display_values()
     tags, strips = _tiff(written[0])
     assert img.dtype == np.uint16 and strips == img.tobytes()
+
+
+REQUESTS = {                       # mode, Quantize settings of the request
+    "z_float": ("z", (0.0, 0.0, 0.0, 0.0)),
+    "z_u16_dither": ("z", (1000.0, 0.0, 65535.0, 1.0)),
+    "rgbaz_float": ("rgbaz", (0.0, 0.0, 0.0, 0.0)),
+    "rgba_s16": ("rgba", (32767.0, -32768.0, 32767.0, 1.0)),
+    "a_u8": ("a", (255.0, 0.0, 255.0, 0.5)),
+    "rgb_u8": ("rgb", (255.0, 0.0, 255.0, 0.5)),
+}
+
+
+@pytest.mark.parametrize("index", [0, 1])
+@pytest.mark.parametrize("req", list(REQUESTS))
+def test_display_request_values_match_the_oracle(oracle, req, index):
+    """rh_read_display_request: the frame's filtered channels as a display
+    request of any mode receives them, with the request's own Quantize
+    settings and the dither draws of its place among the requests (two
+    requests here), against the oracle's restatement on the oracle's
+    channels; a cropped frame."""
+    from aqsis_amd.hider import Hider, mode_bits
+    mode, q = REQUESTS[req]
+    opts, prims = _scene(QUANTIZE["u8"], CROPS["crop"], seed=65)
+    opts.num_displays = 2
+    print(f"display request {req} index {index}", flush=True)
+    ref = oracle.render(opts, prims, occlusion_cull=True, samples=False)
+    _, dither = oracle.sample_positions(opts, display=index)
+    with Hider(opts) as h:
+        h.post_surfaces(prims)
+        h.render()
+        got = h.display_request_values(mode, q, index=index)
+        assert np.array_equal(h.channels().view(np.uint32), ref["channels"].view(np.uint32))
+    from aqsis_amd.hider import lib
+    import ctypes as C
+    t = C.c_int32()
+    lib().rh_select_format(q[0], q[1], q[2], C.byref(t))
+    want = oracle.display_request_values(q, mode_bits(mode), ref["channels"], dither, t.value)
+    assert got.dtype == want.dtype and got.shape == want.shape
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
