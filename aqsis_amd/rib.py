@@ -5,7 +5,7 @@ rh_prim records.
 Requests (libs/core/api/ri.cpp; parsing follows libs/riutil/ribparser.cpp's
 token rules: strings in double quotes, numbers, [ ] arrays, # comments):
 
-  Format, PixelSamples, PixelFilter, ShadingRate, Exposure, Quantize "rgba",
+  Format, PixelSamples, PixelFilter, ShadingRate, Exposure, Quantize "rgba" / "z",
   Projection "perspective" "fov", Clipping, CropWindow, Shutter,
   DepthOfField (no arguments: the pinhole camera), GeometricApproximation
   "focusfactor", ShadingInterpolation "constant" / "smooth", Attribute "cull"
@@ -35,7 +35,9 @@ draws one dither value per pixel from the shared random stream
 request whose name starts with '+' adds a display and any other replaces the
 requests made so far; the name is kept without its '+'.  A mode that reads
 "z" after an optional "rgb" and "a" (ri.cpp:947-964) puts z in the display
-mode (display_z): added to it by a '+' request, replacing it otherwise.  render_file writes
+mode (display_z): added to it by a '+' request, replacing it otherwise.  Each
+request dict also carries its mode bits and its resolved Quantize settings
+(rh_read_display_request's arguments).  render_file writes
 the first display as a TIFF file when its type is "file" or "tiff"
 (tools/displays/file/display.cpp).
 """
@@ -226,6 +228,7 @@ def parse(text, displays=None):
             "interp": 0, "cull_hidden": 1, "matte": 0}
     astack, tstack = [], []
     motion = None          # the open MotionBegin block: times, line, requests
+    zquant = [0.0, 0.0, 0.0, 0.0]   # Quantize "z" (options.cpp:307)
 
     def key_matrices():
         """Object->camera matrices per key, or None when nothing moves."""
@@ -260,9 +263,14 @@ def parse(text, displays=None):
             elif req == "Exposure":
                 opts.exposure_gain, opts.exposure_gamma = flat[0], flat[1]
             elif req == "Quantize":
-                if args[0] != "rgba":
-                    raise RibError(f"line {line}: only Quantize \"rgba\" is supported")
-                opts.quantize_one, opts.quantize_min, opts.quantize_max, opts.quantize_dither = flat[:4]
+                # RiQuantize (ri.cpp:902-928): "rgba" the colour settings,
+                # "z" the depth settings display requests with z take
+                if args[0] == "rgba":
+                    opts.quantize_one, opts.quantize_min, opts.quantize_max, opts.quantize_dither = flat[:4]
+                elif args[0] == "z":
+                    zquant = [float(v) for v in flat[:4]]
+                else:
+                    raise RibError(f"line {line}: only Quantize \"rgba\" and \"z\" are supported")
             elif req == "Projection":
                 if args[0] != "perspective":
                     raise RibError(f"line {line}: only the perspective projection is supported")
@@ -430,6 +438,23 @@ def parse(text, displays=None):
         raise RibError(f"line {motion['line']}: MotionBegin without MotionEnd")
     if requests:
         opts.num_displays = len(requests)
+    # each request's Quantize settings (ddmanager.cpp:429-455, 848-870): its
+    # own "quantize" [zero one min max] and "dither", else Quantize "z" when
+    # its mode has z, else Quantize "rgba"
+    from .hider import MODE_Z, mode_bits
+    rgba = [opts.quantize_one, opts.quantize_min, opts.quantize_max, opts.quantize_dither]
+    for r in requests:
+        bits = mode_bits(r["mode"])
+        q = list(zquant if bits & MODE_Z else rgba)
+        pq, pd = r["params"].get("quantize"), r["params"].get("dither")
+        if pq is not None:
+            if len(pq) != 4 or pq[0] != 0:
+                raise RibError(f"line {r['line']}: Display \"quantize\" must be [0 one min max]")
+            q[0:3] = [float(v) for v in pq[1:4]]
+        if pd is not None:
+            q[3] = float(pd[0])
+        r["mode_bits"] = bits
+        r["quantize"] = tuple(q)
     if displays is not None:
         displays.extend(requests)
     return opts, prims

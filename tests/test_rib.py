@@ -176,3 +176,27 @@ def test_file_displays_refused_before_rendering(tmp_path, first, second, msg):
     path.write_text("\n".join(lines) + "\nFormat 16 16 1\nWorldBegin\nWorldEnd\n")
     with pytest.raises(rib.RibError, match=msg):
         rib.render_file(str(path))
+
+
+def test_display_request_quantize_settings():
+    """Each request's Quantize settings (ddmanager.cpp:429-455, 848-870):
+    its own "quantize" [0 one min max] / "dither", else Quantize "z" for a
+    mode with z, else Quantize "rgba"."""
+    text = """Quantize "rgba" 255 0 255 0.5
+    Quantize "z" 1000 0 65535 1
+    Display "a.tif" "file" "rgba"
+    Display "+d.tif" "file" "z"
+    Display "+e.tif" "file" "rgbaz" "quantize" [0 65535 0 65535] "dither" [0.25]
+    WorldBegin
+    Sphere 1 -1 1 360
+    WorldEnd
+    """
+    displays = []
+    opts, _ = rib.parse(text, displays)
+    assert [d["mode_bits"] for d in displays] == [3, 4, 7]
+    assert displays[0]["quantize"] == (255.0, 0.0, 255.0, 0.5)
+    assert displays[1]["quantize"] == (1000.0, 0.0, 65535.0, 1.0)
+    assert displays[2]["quantize"] == (65535.0, 0.0, 65535.0, 0.25)
+    assert opts.num_displays == 3 and opts.display_z == 1
+    with pytest.raises(rib.RibError, match="quantize"):
+        rib.parse('Display "a" "file" "rgba" "quantize" [1 255 0 255]\nWorldBegin\nWorldEnd\n')
