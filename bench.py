@@ -209,7 +209,7 @@ def e2e_frames(h, prims, steps):
 
 
 def config1_target(args):
-    """The north-star target on the CPU-runnable configuration (256x256, 64
+    """The 20x target on the CPU-runnable configuration (256x256, 64
     spheres, 4x4, gaussian 2x2): the device hider against the oracle on all
     host cores, end to end (new scene per frame) and on resident inputs, with
     the frame's parity checked in the same run."""
@@ -418,7 +418,7 @@ def main():
         print(json.dumps({"copy_bytes": n * 4}))
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; the hider's multi-GPU sharding "
+        print("bench.py: measures one GPU only; the hider's multi-GPU sharding "
               "(screen-space bucket rows per GPU) is a design note, not code.", file=sys.stderr)
         raise SystemExit(2)
 
