@@ -2095,7 +2095,9 @@ __device__ __forceinline__ bool dof_quad_hit(const float* qx, const float* qy, c
 // Contains2D and the LOD range are order-free; then each lane, in list
 // order: occlusion skip, SPL_bound_hits, dofSampleInBound (micropolygon.cpp:
 // 1531-1550) against the micropolygon's bound, the moved-quad hit test and
-// StoreSample.  One workgroup of four waves per task.
+// StoreSample.  One workgroup of four waves per task.  STORE: StoreSample's
+// mode (RH_STORE_*).
+template <int STORE>
 __global__ void __launch_bounds__(RH_DOF_WAVES * 64)
 rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buckets, const TaskDev* __restrict__ tasks,
               int itemBase, const unsigned long long* __restrict__ capStart, const uint32_t* __restrict__ listCount,
@@ -2117,7 +2119,7 @@ rh_sample_dof(const FrameDev* __restrict__ fr, const BucketDev* __restrict__ buc
     const int task = taskList ? (int)taskList[itemBase + blockIdx.x] : itemBase + (int)blockIdx.x;
     const TaskDev T = tasks[task];
     const BucketDev B = buckets[T.bucket];
-    const bool cullH = fr->cullableHits != 0, midS = fr->midpointStore != 0;
+    constexpr bool cullH = STORE != RH_STORE_LIST_ALL, midS = STORE == RH_STORE_MIDPOINT;
     const int tilesX = (T.x1 - T.x0 + 7) >> 3, tilesY = (T.y1 - T.y0 + 7) >> 3;
     const int nblk = tilesX * tilesY * ns;
     const uint32_t* L = lists + capStart[task];
@@ -3179,11 +3181,20 @@ extern "C" hipError_t rh_launch_sample_mb(hipStream_t st, const SampleLaunch* a,
 // [a->itemBase, a->itemBase + ntasks).
 extern "C" hipError_t rh_launch_sample_dof(hipStream_t st, const SampleLaunch* a, int ntasks, const DofTables* t) {
     if (ntasks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(rh_sample_dof, dim3(ntasks), dim3(RH_DOF_WAVES * 64), 0, st, a->fr, a->buckets, a->tasks,
-                       a->itemBase, a->capStart, a->listCount, a->lists, reinterpret_cast<const float4*>(a->mprec),
-                       a->grids, a->prims, a->pos2d, a->pos1d, a->pixPattern, t->shuffled, t->lens, t->pixDof,
-                       t->cells, a->samples, a->hitData, a->hitNext, a->hitCounter, a->combineList, a->frontDepth,
-                       a->rowCounts, a->rowBase, a->idCut, a->itemCut, a->taskList, a->depthOnly);
+#define RH_LAUNCH_DOF(STORE)                                                                                       \
+    hipLaunchKernelGGL((rh_sample_dof<STORE>), dim3(ntasks), dim3(RH_DOF_WAVES * 64), 0, st, a->fr, a->buckets,     \
+                       a->tasks, a->itemBase, a->capStart, a->listCount, a->lists,                                   \
+                       reinterpret_cast<const float4*>(a->mprec), a->grids, a->prims, a->pos2d, a->pos1d,           \
+                       a->pixPattern, t->shuffled, t->lens, t->pixDof, t->cells, a->samples, a->hitData, a->hitNext, \
+                       a->hitCounter, a->combineList, a->frontDepth, a->rowCounts, a->rowBase, a->idCut, a->itemCut, \
+                       a->taskList, a->depthOnly)
+    switch (a->store) {
+    case RH_STORE_OCCLUDE: RH_LAUNCH_DOF(RH_STORE_OCCLUDE); break;
+    case RH_STORE_MIDPOINT: RH_LAUNCH_DOF(RH_STORE_MIDPOINT); break;
+    case RH_STORE_LIST_ALL: RH_LAUNCH_DOF(RH_STORE_LIST_ALL); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef RH_LAUNCH_DOF
     return hipGetLastError();
 }
 
